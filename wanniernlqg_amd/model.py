@@ -1,0 +1,232 @@
+"""Tight-binding model container, Wannier ``_tb.dat`` reader/writer and seeded synthetic models.
+
+The container mirrors ``TightBindingModel`` (reference ``src/Core/Models.jl:11-75``):
+row lattice vectors in Angstrom, integer R vectors, R degeneracies, H(R) in eV and
+the position operator A(R) in Angstrom.  Arrays are stored R-major so that one
+R block is contiguous:
+
+* ``hamiltonian_r[r, i, j]``   == reference ``hamiltonian_r[i, j, r]``
+* ``position_r[r, a, i, j]``   == reference ``position_r[i, j, a, r]``
+* ``r_vectors[r, :]``          == reference ``r_vectors[:, r]``
+
+The file format follows ``read_wannier_tb`` (reference ``src/IO/WannierTB.jl:53-111``).
+"""
+
+from __future__ import annotations
+
+import itertools
+from dataclasses import dataclass, field
+from typing import Optional
+
+import numpy as np
+
+
+@dataclass
+class TightBindingModel:
+    lattice: np.ndarray                 # (3, 3) rows a1, a2, a3 in Angstrom
+    r_vectors: np.ndarray               # (N_R, 3) int
+    r_degeneracies: np.ndarray          # (N_R,) int
+    hamiltonian_r: np.ndarray           # (N_R, n, n) complex128, eV
+    position_r: Optional[np.ndarray] = None  # (N_R, 3, n, n) complex128, Angstrom
+    name: str = "model"
+    meta: dict = field(default_factory=dict)
+
+    def __post_init__(self):
+        self.lattice = np.ascontiguousarray(self.lattice, dtype=np.float64).reshape(3, 3)
+        self.r_vectors = np.ascontiguousarray(self.r_vectors, dtype=np.int32).reshape(-1, 3)
+        self.r_degeneracies = np.ascontiguousarray(self.r_degeneracies, dtype=np.int32).reshape(-1)
+        self.hamiltonian_r = np.ascontiguousarray(self.hamiltonian_r, dtype=np.complex128)
+        nr, n, n2 = self.hamiltonian_r.shape
+        if n != n2:
+            raise ValueError("hamiltonian_r must be (N_R, n, n)")
+        if self.r_vectors.shape[0] != nr or self.r_degeneracies.shape[0] != nr:
+            raise ValueError("R vectors / degeneracies do not match hamiltonian_r")
+        if np.any(self.r_degeneracies <= 0):
+            raise ValueError("R degeneracies must be positive")
+        if self.position_r is not None:
+            self.position_r = np.ascontiguousarray(self.position_r, dtype=np.complex128)
+            if self.position_r.shape != (nr, 3, n, n):
+                raise ValueError("position_r must be (N_R, 3, n, n)")
+
+    @property
+    def num_orbitals(self) -> int:
+        return int(self.hamiltonian_r.shape[1])
+
+    @property
+    def num_r_vectors(self) -> int:
+        return int(self.hamiltonian_r.shape[0])
+
+    @property
+    def cell_volume(self) -> float:
+        """Signed det(lattice), the Integral measure (reference ``NativeModelRuntime.jl:600``)."""
+        return float(np.linalg.det(self.lattice))
+
+    def position_or_zero(self) -> np.ndarray:
+        if self.position_r is not None:
+            return self.position_r
+        n, nr = self.num_orbitals, self.num_r_vectors
+        return np.zeros((nr, 3, n, n), dtype=np.complex128)
+
+
+def read_wannier_tb(path: str, name: Optional[str] = None) -> TightBindingModel:
+    """Parse a Wannier90 ``_tb.dat`` (reference ``IO/WannierTB.jl:53-111``).
+
+    Header line, three lattice rows, orbital count, R count, degeneracies 15 per
+    line, then per R a blank line, the R vector and n*n ``row col re im`` records
+    in column-major order; then the position blocks with three complex components.
+    """
+    with open(path, "r") as fh:
+        lines = fh.read().split("\n")
+    pos = 1
+    lattice = np.array([[float(x) for x in lines[pos + i].split()] for i in range(3)])
+    pos += 3
+    n = int(lines[pos].strip()); pos += 1
+    nr = int(lines[pos].strip()); pos += 1
+    deg = []
+    for _ in range((nr + 14) // 15):
+        deg.extend(int(x) for x in lines[pos].split()); pos += 1
+    if len(deg) != nr:
+        raise ValueError("degeneracy count mismatch")
+    rvec = np.zeros((nr, 3), dtype=np.int32)
+    hr = np.zeros((nr, n, n), dtype=np.complex128)
+    for r in range(nr):
+        pos += 1  # blank line
+        rvec[r] = [int(x) for x in lines[pos].split()]; pos += 1
+        for col in range(n):
+            for row in range(n):
+                tok = lines[pos].split(); pos += 1
+                hr[r, row, col] = complex(float(tok[2]), float(tok[3]))
+    ar = np.zeros((nr, 3, n, n), dtype=np.complex128)
+    for r in range(nr):
+        pos += 2
+        for col in range(n):
+            for row in range(n):
+                tok = lines[pos].split(); pos += 1
+                for a in range(3):
+                    ar[r, a, row, col] = complex(float(tok[2 + 2 * a]), float(tok[3 + 2 * a]))
+    return TightBindingModel(lattice, rvec, np.array(deg), hr, ar, name=name or path)
+
+
+def write_wannier_tb(model: TightBindingModel, path: str, header: str = "wanniernlqg_amd synthetic model") -> str:
+    """Write ``model`` in the format :func:`read_wannier_tb` (and the reference reader) parses."""
+    n, nr = model.num_orbitals, model.num_r_vectors
+    ar = model.position_or_zero()
+    out = [header]
+    for i in range(3):
+        out.append(" ".join("%.17e" % x for x in model.lattice[i]))
+    out.append(str(n))
+    out.append(str(nr))
+    deg = [str(int(d)) for d in model.r_degeneracies]
+    for i in range(0, nr, 15):
+        out.append(" ".join(deg[i:i + 15]))
+    for r in range(nr):
+        out.append("")
+        out.append(" ".join(str(int(x)) for x in model.r_vectors[r]))
+        for col in range(n):
+            for row in range(n):
+                v = model.hamiltonian_r[r, row, col]
+                out.append("%d %d %.17e %.17e" % (row + 1, col + 1, v.real, v.imag))
+    for r in range(nr):
+        out.append("")
+        out.append(" ".join(str(int(x)) for x in model.r_vectors[r]))
+        for col in range(n):
+            for row in range(n):
+                v = ar[r, :, row, col]
+                out.append("%d %d %.17e %.17e %.17e %.17e %.17e %.17e" % (
+                    row + 1, col + 1, v[0].real, v[0].imag, v[1].real, v[1].imag, v[2].real, v[2].imag))
+    with open(path, "w") as fh:
+        fh.write("\n".join(out) + "\n")
+    return path
+
+
+def _cube_vectors(r_max: int) -> np.ndarray:
+    rng = range(-r_max, r_max + 1)
+    return np.array(list(itertools.product(rng, rng, rng)), dtype=np.int32)
+
+
+def _lex_positive(r) -> bool:
+    for x in r:
+        if x != 0:
+            return x > 0
+    return False
+
+
+def synthetic_model(num_orbitals: int, r_max: int, seed: int, *, lattice_constant: float = 4.0,
+                    decay: float = 1.5, hopping_scale: float = 1.0, onsite_range: float = 2.0,
+                    position_scale: Optional[float] = None, name: Optional[str] = None) -> TightBindingModel:
+    """Seeded orthogonal TB model on the cube |R_i| <= r_max with all degeneracies one.
+
+    Choices (fixed here, so every run of a given seed builds the same model):
+
+    * lattice = lattice_constant * I (rows), R ordered with R1 slowest, R3 fastest;
+    * |R| is measured in lattice units;
+    * for each lexicographically positive R: H_ij(R) = hopping_scale*exp(-|R|/decay)*(x+iy)/sqrt(2),
+      x, y ~ N(0,1) drawn from ``numpy.random.default_rng(seed)`` in R order, row-major
+      (i, j); then H(-R) = H(R)^dagger;
+    * H(0): off-diagonal i<j drawn the same way with amplitude hopping_scale, mirrored
+      Hermitian; diagonal uniform in [-onsite_range, onsite_range];
+    * position operator (if ``position_scale``): drawn after H from the same generator,
+      A_a(R) for a = x, y, z with amplitude position_scale*exp(-|R|/decay),
+      A_a(-R) = A_a(R)^dagger, A_a(0) Hermitian with real N(0,1)*position_scale diagonal.
+    """
+    n = int(num_orbitals)
+    rng = np.random.default_rng(seed)
+    rvec = _cube_vectors(r_max)
+    nr = rvec.shape[0]
+    index = {tuple(r): i for i, r in enumerate(rvec.tolist())}
+    zero = index[(0, 0, 0)]
+    norms = np.sqrt(np.sum(rvec.astype(np.float64) ** 2, axis=1))
+    hr = np.zeros((nr, n, n), dtype=np.complex128)
+
+    def gauss(shape):
+        x = rng.standard_normal(shape)
+        y = rng.standard_normal(shape)
+        return (x + 1j * y) / np.sqrt(2.0)
+
+    for r in range(nr):
+        if not _lex_positive(rvec[r]):
+            continue
+        block = hopping_scale * np.exp(-norms[r] / decay) * gauss((n, n))
+        hr[r] = block
+        hr[index[tuple((-rvec[r]).tolist())]] = block.conj().T
+    off = hopping_scale * gauss((n, n))
+    h0 = np.triu(off, 1)
+    h0 = h0 + h0.conj().T
+    h0[np.diag_indices(n)] = rng.uniform(-onsite_range, onsite_range, n)
+    hr[zero] = h0
+
+    ar = None
+    if position_scale is not None:
+        ar = np.zeros((nr, 3, n, n), dtype=np.complex128)
+        for r in range(nr):
+            if not _lex_positive(rvec[r]):
+                continue
+            block = position_scale * np.exp(-norms[r] / decay) * gauss((3, n, n))
+            ar[r] = block
+            ar[index[tuple((-rvec[r]).tolist())]] = np.conj(np.swapaxes(block, 1, 2))
+        for a in range(3):
+            off = position_scale * gauss((n, n))
+            a0 = np.triu(off, 1)
+            a0 = a0 + a0.conj().T
+            a0[np.diag_indices(n)] = position_scale * rng.standard_normal(n)
+            ar[zero, a] = a0
+    lattice = lattice_constant * np.eye(3)
+    return TightBindingModel(lattice, rvec, np.ones(nr, dtype=np.int32), hr, ar,
+                             name=name or f"synthetic_n{n}_r{r_max}_s{seed}",
+                             meta=dict(num_orbitals=n, r_max=r_max, seed=seed))
+
+
+def config_model(index: int, seed: Optional[int] = None) -> TightBindingModel:
+    """Models of the benchmark configurations.
+
+    0: n=8,  |R_i|<=3 (343 R), A = 0            (plain FP64 shift current, 32^3 mesh)
+    1: n=32, |R_i|<=5 (1331 R), A amplitude 0.1 (shift + injection, 128^3 mesh)
+    4: n=96, |R_i|<=6 (2197 R), A amplitude 0.1 (SHG, 256^3 mesh)
+    """
+    if index == 0:
+        return synthetic_model(8, 3, 0 if seed is None else seed, name="cfg0_n8_r3")
+    if index == 1:
+        return synthetic_model(32, 5, 1 if seed is None else seed, position_scale=0.1, name="cfg1_n32_r5")
+    if index == 4:
+        return synthetic_model(96, 6, 4 if seed is None else seed, position_scale=0.1, name="cfg4_n96_r6")
+    raise ValueError(f"no model for configuration {index}")
