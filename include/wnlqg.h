@@ -1,0 +1,174 @@
+/*
+ * wnlqg.h -- C ABI of the MI355X k-mesh response engine (libwnlqg_hip.so).
+ *
+ * The engine replaces the reference's dense-mesh Integral k-loop for one model and one
+ * task bundle: everything between the per-worker workspace preparation
+ * (reference src/Runtime/Execution/IntegralDriver.jl:475-572) and finish!
+ * (IntegralDriver.jl:909).  Per-lane partial tensors come back already scaled exactly as
+ * consume_point! leaves them in state.worker_data[lane] (IntegralDriver.jl:820-874), so the
+ * reference's _reduce_integral_states! (src/Runtime/Reduction/DeterministicReduction.jl:2-37)
+ * and its writers run unchanged.
+ *
+ * Conventions
+ *   - Complex values are interleaved (re, im) doubles.
+ *   - Arrays described as "column-major [d1, d2, ...]" use Julia's layout (d1 fastest), so a
+ *     Julia Array{ComplexF64} can be passed by pointer without copying.
+ *   - All entry points return 0 on success or a negative WNLQG_E_* code; the message is
+ *     available from wnlqg_last_error().  Nothing is thrown across the ABI.
+ *   - One engine owns one HIP stream on the device that was current at wnlqg_create; calls
+ *     on one engine must not overlap.  Calls block until their results are in host memory.
+ */
+#ifndef WNLQG_H
+#define WNLQG_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define WNLQG_ABI_VERSION 1
+
+/* Status codes. */
+#define WNLQG_OK 0
+#define WNLQG_E_INVALID (-1)     /* invalid argument or inconsistent sizes */
+#define WNLQG_E_NONFINITE (-2)   /* non-finite model input or non-finite result */
+#define WNLQG_E_GAP (-3)         /* spectral isolation violated (OrthogonalBandDerivatives.jl:74-78) */
+#define WNLQG_E_HIP (-4)         /* HIP runtime error */
+#define WNLQG_E_OOM (-5)         /* device allocation failed */
+#define WNLQG_E_UNSUPPORTED (-6) /* configuration outside what this build implements */
+
+/* Response quantities (bitmask). Output states are ordered by increasing bit. */
+#define WNLQG_SHIFT_CURRENT 1     /* Conventional SC, ConventionalShiftCurrent.jl:527-884 */
+#define WNLQG_INJECTION_CURRENT 2 /* Conventional IC, ConventionalInjectionCurrent.jl:23-140 */
+
+/* Broadening (TransitionScreening.jl:1-29). */
+#define WNLQG_GAUSSIAN 0
+#define WNLQG_LORENTZIAN 1
+
+/* Arithmetic forms. */
+#define WNLQG_PRECISION_FP64 0  /* FP64 everywhere (plain GPU path) */
+#define WNLQG_PRECISION_MIXED 1 /* split-BF16 MFMA interpolation, FP64 denominators/k-sum */
+
+/*
+ * Tight-binding model (TightBindingModel, reference src/Core/Models.jl:11-75; the fields of
+ * read_wannier_tb, src/IO/WannierTB.jl:53-111).
+ */
+typedef struct wnlqg_model {
+    int32_t num_orbitals;            /* n */
+    int32_t num_r_vectors;           /* N_R */
+    const double *lattice;           /* [9] Julia lattice[3,3] column-major; rows are a1,a2,a3 (Angstrom) */
+    const int32_t *r_vectors;        /* [3*N_R] Julia r_vectors[3,N_R] column-major */
+    const int32_t *r_degeneracies;   /* [N_R] */
+    const double *hamiltonian_r;     /* complex, column-major [n,n,N_R] (eV) */
+    const double *position_r;        /* complex, column-major [n,n,3,N_R] (Angstrom), or NULL for A = 0 */
+} wnlqg_model;
+
+/*
+ * Resolved task parameters (subset of EffectiveTaskConfig, reference src/Runtime/Config.jl:4-170).
+ * Wannier-center convention: Convention_II (the default, Config.jl:154) only.
+ */
+typedef struct wnlqg_task {
+    int32_t quantities;               /* WNLQG_SHIFT_CURRENT | WNLQG_INJECTION_CURRENT */
+    int32_t num_photon_energies;      /* n_w >= 1 */
+    const double *photon_energies;    /* [n_w] eV, ascending */
+    int32_t broadening_type;          /* WNLQG_GAUSSIAN or WNLQG_LORENTZIAN */
+    double broadening;                /* eV (> 0) */
+    double fermi_energy;              /* eV */
+    double temperature;               /* K (>= 0) */
+    double denominator_regularization;/* eta in eV (>= 0); 0 = bare 1/dE with isolation check */
+    double transition_window_factor;  /* Lorentzian screening window, in broadenings */
+    int32_t band_window_size;         /* -1 = all bands (only -1 or a window covering all bands) */
+    int32_t precision;                /* WNLQG_PRECISION_* */
+    int32_t batch_size;               /* k points per device batch; 0 = automatic */
+    int32_t reserved;
+} wnlqg_task;
+
+/* Counters of the last run (device event timers, milliseconds). */
+typedef struct wnlqg_stats {
+    double total_ms;        /* whole k-loop: phase, interpolation, eigen, screening, response, k-sum */
+    double interp_ms;       /* interpolation GEMM kernel */
+    double eig_ms;          /* Jacobi eigensolver kernel */
+    double rotate_ms;       /* U^+ X U kernel */
+    double pairs_ms;        /* screening + generalized derivative + response kernels */
+    double accum_ms;        /* frequency contraction + deterministic k-sum */
+    int64_t kpoints;        /* k points evaluated */
+    int64_t active_pairs;   /* band pairs contributing */
+    int32_t batches;        /* device batches */
+    int32_t interp_launches;/* launches of the interpolation kernel */
+    double interp_flops;    /* algorithmic interpolation flops: 8 * N_R * n^2 * C per k point */
+} wnlqg_stats;
+
+typedef struct wnlqg_engine wnlqg_engine;
+
+/* ABI version (WNLQG_ABI_VERSION). */
+int32_t wnlqg_version(void);
+
+/* Copy the message of the last failing call on this thread into buf (NUL-terminated).
+ * Returns the full message length. */
+int32_t wnlqg_last_error(char *buf, size_t len);
+
+/* Upload the model, build the R-space channels on the device and allocate the k-batch
+ * working set (replaces prepare_real_space!, HamiltonianMatrixElements.jl:180-199, and the
+ * workspace setup of IntegralDriver.jl:212-446).  *out is NULL on failure. */
+int32_t wnlqg_create(const wnlqg_model *model, const wnlqg_task *task, wnlqg_engine **out);
+
+/* Release all device memory of the engine. NULL is accepted. */
+int32_t wnlqg_destroy(wnlqg_engine *engine);
+
+/* Number of output states (set bits of task.quantities) and interpolation channels. */
+int32_t wnlqg_num_states(const wnlqg_engine *engine);
+int32_t wnlqg_num_channels(const wnlqg_engine *engine);
+
+/*
+ * Dense Gamma-centred mesh k_i = i/N_i, axis 1 fastest (integral_kpoint!,
+ * src/Core/CalculationUtils.jl:280-292), integrated with the reference's deterministic lanes
+ * (src/Runtime/Setup/BundleAssembly.jl:35-47): lane L covers k indices
+ * floor(L*N_k/n_lanes) .. floor((L+1)*N_k/n_lanes)-1 (0-based), 1 <= n_lanes <= N_k
+ * (the reference uses n_lanes = min(N_k, 64)).
+ *
+ * lane_out: complex [n_lanes][n_states][n_w*27]; each state block is Julia column-major
+ * [n_w,3,3,3] and equals sum_{k in lane} (1/N_k) * value_k with the state's prefactor
+ * (ConventionalCurrentIntegralExecution.jl:243-245 for SC, :296-298 for IC).
+ * total_out (optional, may be NULL): complex [n_states][n_w*27], the lanes summed in lane order.
+ * stats may be NULL.
+ */
+int32_t wnlqg_run_mesh(wnlqg_engine *engine, const int32_t mesh[3], int32_t n_lanes,
+                       double *lane_out, double *total_out, wnlqg_stats *stats);
+
+/*
+ * Per-point values (point_value!, IntegralDriver.jl:596, with num_kpoints = 1) at m arbitrary
+ * fractional k points.  kpoints: column-major [3,m].  out: complex [m][n_states][n_w*27].
+ */
+int32_t wnlqg_eval_points(wnlqg_engine *engine, const double *kpoints, int32_t m, double *out);
+
+/*
+ * Wannier-gauge interpolated channels at m points (the Fourier sums of
+ * HamiltonianMatrixElements.jl:42-58,427-462 and PositionMatrixElements.jl:239-409).
+ * out: complex [m][C][n][n] with C = wnlqg_num_channels(); channel order:
+ *   0 H, 1-3 dH_a (a = x,y,z), 4-9 ddH_ab for ab = xx,yy,zz,yz,xz,xy,
+ *   10-12 A_a, 13-21 dA_ab (= Fourier of i R_b A_a, index 13 + 3a + b)  (the A channels
+ *   exist only when the model has a position operator and a quantity needs them).
+ * Element [i][j] is row i, column j.
+ */
+int32_t wnlqg_fourier_points(wnlqg_engine *engine, const double *kpoints, int32_t m, double *out);
+
+/*
+ * Eigensystem of the upper triangle of H(k) (DenseSpectra.hermitian_spectrum,
+ * src/Core/ESKernel/DenseSpectra.jl:14-16) with the canonical column phase
+ * (canonicalize_eigenvectors!, HamiltonianMatrixElements.jl:6-25).
+ * energies: [m][n] ascending.  vectors: complex [m][n][n], element [i][j] = component i of
+ * eigenvector j.
+ */
+int32_t wnlqg_spectrum_points(wnlqg_engine *engine, const double *kpoints, int32_t m,
+                              double *energies, double *vectors);
+
+/* Block until all work of the engine's stream is complete. */
+int32_t wnlqg_synchronize(wnlqg_engine *engine);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* WNLQG_H */

@@ -1,0 +1,69 @@
+"""CPU-side checks of the C-ABI library: it loads, exports exactly what include/wnlqg.h declares,
+and rejects invalid input before touching the GPU (no compute calls here)."""
+import ctypes
+import os
+import re
+import subprocess
+
+import numpy as np
+import pytest
+
+from wanniernlqg_amd import engine as E
+from wanniernlqg_amd.model import synthetic_model
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+HEADER = os.path.join(ROOT, "include", "wnlqg.h")
+
+
+def header_functions():
+    text = open(HEADER).read()
+    return sorted(set(re.findall(r"^\s*int32_t\s+(wnlqg_\w+)\s*\(", text, flags=re.M)))
+
+
+def test_library_loads_and_version():
+    lib = E.load_library()
+    assert lib.wnlqg_version() == 1
+
+
+def test_exports_match_header():
+    declared = header_functions()
+    assert declared == sorted(E.EXPORTED)
+    out = subprocess.run(["nm", "-D", "--defined-only", E.LIB_PATH], capture_output=True, text=True).stdout
+    exported = sorted(set(re.findall(r"\bT (wnlqg_\w+)", out)))
+    assert exported == declared
+
+
+def _task(**kw):
+    base = dict(photon_energies=np.linspace(0, 4, 20), broadening=0.02)
+    base.update(kw)
+    return E.EngineTask(**base)
+
+
+def test_rejects_nonfinite_model():
+    m = synthetic_model(4, 1, 0)
+    m.hamiltonian_r[3, 1, 2] = np.nan
+    with pytest.raises(E.WnlqgError) as ei:
+        E.Engine(m, _task())
+    assert ei.value.code == -2
+
+
+def test_rejects_unsorted_frequencies():
+    with pytest.raises(E.WnlqgError) as ei:
+        E.Engine(synthetic_model(4, 1, 0), _task(photon_energies=np.array([1.0, 0.5])))
+    assert ei.value.code == -6
+
+
+def test_rejects_unknown_smearing():
+    with pytest.raises(E.WnlqgError) as ei:
+        E.Engine(synthetic_model(4, 1, 0), _task(broadening_type="Voigt"))
+    assert ei.value.code == -1
+
+
+def test_rejects_partial_band_window_and_large_n():
+    with pytest.raises(E.WnlqgError) as ei:
+        E.Engine(synthetic_model(8, 1, 0), _task(band_window_size=2))
+    assert ei.value.code == -6
+    big = synthetic_model(65, 0, 0)
+    with pytest.raises(E.WnlqgError) as ei:
+        E.Engine(big, _task())
+    assert ei.value.code == -6

@@ -1,0 +1,184 @@
+"""GPU parity of the HIP engine against the FP64 oracle (oracle/kloop.py), through the C ABI.
+
+Bar: FP64 path within 1e-10 relative (L2 over the omega spectrum, per component) of the oracle;
+integer bookkeeping (lanes) exact.  Each test prints the case it runs before the GPU call."""
+import os
+import sys
+
+import numpy as np
+import pytest
+
+from oracle import kloop
+from wanniernlqg_amd import engine as E
+from wanniernlqg_amd.model import config_model, read_wannier_tb, synthetic_model
+
+pytestmark = pytest.mark.gpu
+HERE = os.path.dirname(os.path.abspath(__file__))
+FIXTURE = os.path.join(HERE, "golden", "synthetic_runtime", "synthetic_tb.dat")
+
+
+def say(msg):
+    print(msg, flush=True)
+    sys.stdout.flush()
+
+
+def rel(a, b):
+    return np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-300)
+
+
+def mk_task(**kw):
+    base = dict(photon_energies=np.linspace(0.0, 4.0, 200), broadening=0.02, shift_current=True,
+                injection_current=True)
+    base.update(kw)
+    return base
+
+
+def engine(model, **kw):
+    return E.Engine(model, E.EngineTask(**mk_task(**kw)))
+
+
+def oracle_task(**kw):
+    t = mk_task(**kw)
+    return kloop.Task(photon_energies=t["photon_energies"], broadening=t["broadening"],
+                      broadening_type=t.get("broadening_type", "Gaussian"), fermi_energy=t.get("fermi_energy", 0.0),
+                      temperature=t.get("temperature", 0.0),
+                      denominator_regularization=t.get("denominator_regularization", 1e-3),
+                      transition_window_factor=t.get("transition_window_factor", 5.0),
+                      shift_current=t["shift_current"], injection_current=t["injection_current"])
+
+
+MODELS = {
+    "fixture_n4": lambda: read_wannier_tb(FIXTURE),
+    "cfg0_n8": lambda: config_model(0),
+    "n12_pos": lambda: synthetic_model(12, 2, 7, position_scale=0.1),
+    "n32_pos": lambda: synthetic_model(32, 2, 11, position_scale=0.1),
+}
+
+
+def channels_oracle(model, kp):
+    w = kloop.interpolate(model, kp)
+    n = model.num_orbitals
+    sym = [(0, 0), (1, 1), (2, 2), (1, 2), (0, 2), (0, 1)]
+    chans = [w["H"][:, None]] + [w["dH"][:, a][:, None] for a in range(3)]
+    chans += [w["ddH"][:, a, b][:, None] for a, b in sym]
+    if model.position_r is not None:
+        chans += [w["A"][:, a][:, None] for a in range(3)]
+        chans += [w["dA"][:, a, b][:, None] for a in range(3) for b in range(3)]
+    return np.concatenate(chans, axis=1)
+
+
+@pytest.mark.parametrize("name", list(MODELS))
+def test_fourier_channels(name):
+    model = MODELS[name]()
+    rng = np.random.default_rng(5)
+    kp = rng.uniform(-1, 1, (37, 3))
+    say(f"[fourier] model={name} n={model.num_orbitals} nr={model.num_r_vectors} points=37")
+    with engine(model) as eng:
+        got = eng.fourier_points(kp)
+    exp = channels_oracle(model, kp)
+    assert got.shape == exp.shape
+    scale = np.max(np.abs(exp), axis=(2, 3), keepdims=True) + 1e-300
+    err = np.max(np.abs(got - exp) / scale)
+    say(f"  max rel err {err:.3e}")
+    assert err < 1e-12
+
+
+@pytest.mark.parametrize("name", list(MODELS))
+def test_spectrum(name):
+    model = MODELS[name]()
+    rng = np.random.default_rng(6)
+    kp = rng.uniform(0, 1, (41, 3))
+    say(f"[spectrum] model={name} points=41")
+    with engine(model) as eng:
+        e, u = eng.spectrum_points(kp)
+    hk = kloop.interpolate(model, kp)["H"]
+    e0, u0 = kloop.hermitian_spectrum(hk)
+    scale = np.max(np.abs(e0))
+    assert np.max(np.abs(e - e0)) < 1e-12 * scale
+    # residual of the upper-triangle Hermitian matrix and orthonormality
+    hu = np.triu(hk) + np.conj(np.swapaxes(np.triu(hk, 1), 1, 2))
+    hu[:, np.arange(model.num_orbitals), np.arange(model.num_orbitals)] = np.real(np.diagonal(hk, axis1=1, axis2=2))
+    res = hu @ u - u * e[:, None, :]
+    assert np.max(np.abs(res)) < 1e-11 * scale
+    eye = np.conj(np.swapaxes(u, 1, 2)) @ u
+    assert np.max(np.abs(eye - np.eye(model.num_orbitals))) < 1e-12
+    # canonical phase: first largest-|.| entry of each column is real and >= 0
+    piv = np.argmax(np.abs(u0), axis=1)
+    pv = np.take_along_axis(u, piv[:, None, :], axis=1)[:, 0, :]
+    assert np.max(np.abs(pv.imag)) < 1e-12 and np.min(pv.real) >= 0
+    gaps = np.min(np.diff(e0, axis=1), axis=1)
+    iso = gaps > 1e-6
+    assert np.max(np.abs(u[iso] - u0[iso])) < 1e-8
+
+
+@pytest.mark.parametrize("name", list(MODELS))
+@pytest.mark.parametrize("smearing", ["Gaussian", "Lorentzian"])
+def test_point_values(name, smearing):
+    model = MODELS[name]()
+    rng = np.random.default_rng(7)
+    kp = rng.uniform(0, 1, (23, 3))
+    kw = dict(broadening_type=smearing, photon_energies=np.linspace(0, 6, 150), broadening=0.05,
+              fermi_energy=0.1 if name != "fixture_n4" else 0.0)
+    say(f"[points] model={name} smearing={smearing}")
+    with engine(model, **kw) as eng:
+        got = eng.eval_points(kp)
+    exp = kloop.point_values(model, oracle_task(**kw), kp)
+    for key in ("sc", "ic"):
+        r = rel(got[key], exp[key])
+        say(f"  {key} rel {r:.3e} norm {np.linalg.norm(exp[key]):.3e}")
+        assert r < 1e-10
+
+
+def test_point_values_finite_temperature_and_eta():
+    model = MODELS["n12_pos"]()
+    kp = np.random.default_rng(8).uniform(0, 1, (19, 3))
+    kw = dict(temperature=300.0, denominator_regularization=0.02, photon_energies=np.linspace(0.2, 5, 90))
+    say("[points] finite T, eta = 0.02")
+    with engine(model, **kw) as eng:
+        got = eng.eval_points(kp)
+    exp = kloop.point_values(model, oracle_task(**kw), kp)
+    for key in ("sc", "ic"):
+        assert rel(got[key], exp[key]) < 1e-10
+
+
+@pytest.mark.parametrize("name,mesh", [("cfg0_n8", (8, 8, 8)), ("fixture_n4", (5, 3, 2)), ("n32_pos", (4, 3, 3)),
+                                       ("n12_pos", (1, 1, 1))])
+def test_mesh_integral(name, mesh):
+    model = MODELS[name]()
+    kw = dict(photon_energies=np.linspace(0, 5, 120), broadening=0.04)
+    say(f"[mesh] model={name} mesh={mesh}")
+    with engine(model, **kw) as eng:
+        lanes, total, stats = eng.run_mesh(mesh)
+        lanes2, total2, _ = eng.run_mesh(mesh)
+    ol, ot = kloop.integrate_mesh(model, oracle_task(**kw), mesh, batch=64)
+    for key in ("sc", "ic"):
+        assert lanes[key].shape == ol[key].shape
+        r = kloop.spectrum_relative_error(total[key], ot[key])
+        say(f"  {key} worst component rel {r.max():.3e}")
+        assert r.max() < 1e-10
+        assert rel(lanes[key], ol[key]) < 1e-10
+        # determinism: repeated runs are bitwise identical
+        assert np.array_equal(lanes[key], lanes2[key])
+        # total = lanes summed in lane order
+        acc = np.zeros_like(total[key])
+        for l in range(lanes[key].shape[0]):
+            acc = acc + lanes[key][l]
+        assert np.array_equal(acc, total[key])
+    assert stats["kpoints"] == int(np.prod(mesh))
+
+
+def test_mesh_lane_counts_and_batches():
+    """Results do not depend on the lane count beyond rounding; small batches exercise the
+    multi-batch chunk plan (lanes straddling batches)."""
+    model = MODELS["cfg0_n8"]()
+    kw = dict(photon_energies=np.linspace(0, 5, 60), broadening=0.04, batch_size=128)
+    say("[mesh] cfg0 6x5x7 batch 128, lanes 64 vs 7")
+    with engine(model, **kw) as eng:
+        l64, t64, st = eng.run_mesh((6, 5, 7))
+        l7, t7, _ = eng.run_mesh((6, 5, 7), n_lanes=7)
+    assert st["batches"] == 2
+    ol, ot = kloop.integrate_mesh(model, oracle_task(**kw), (6, 5, 7), batch=50)
+    for key in ("sc", "ic"):
+        assert rel(t64[key], ot[key]) < 1e-10
+        assert rel(t7[key], ot[key]) < 1e-10
+        assert rel(l64[key], ol[key]) < 1e-10

@@ -1,0 +1,61 @@
+// Shared device helpers of the MI355X k-mesh response engine (gfx950, wave64).
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+typedef double f64x4 __attribute__((ext_vector_type(4)));
+typedef float f32x4 __attribute__((ext_vector_type(4)));
+typedef float f32x16 __attribute__((ext_vector_type(16)));
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+
+// Interpolation channel map.  Channel c of the per-k block [C][n][n]:
+//   0 H, 1-3 dH_a, 4-9 ddH for (xx,yy,zz,yz,xz,xy), 10-12 A_a, 13-21 dA_ab (13 + 3a + b).
+#define WN_CH_H 0
+#define WN_CH_DH 1
+#define WN_CH_DDH 4
+#define WN_CH_A 10
+#define WN_CH_DA 13
+#define WN_CH_BASE 10
+#define WN_CH_FULL 22
+
+// GEMM tile of the interpolation kernels: 128 k points x 128 real output columns,
+// K blocks of 8 R vectors (16 real rows: 8 cos parts then 8 sin parts).
+#define WN_GM 128
+#define WN_GN 128
+#define WN_RB 8
+
+// k points of one accumulation chunk and frequencies of one accumulation tile.
+#define WN_WT 64
+
+__host__ __device__ inline int wn_sym(int a, int b) {
+    // (0,0)->0 (1,1)->1 (2,2)->2 (1,2)->3 (0,2)->4 (0,1)->5
+    if (a == b) return a;
+    int s = a + b;  // 3 -> (1,2), 2 -> (0,2), 1 -> (0,1)
+    return s == 3 ? 3 : (s == 2 ? 4 : 5);
+}
+
+__device__ __forceinline__ double2 cmul(double2 a, double2 b) {
+    return make_double2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
+}
+__device__ __forceinline__ double2 cadd(double2 a, double2 b) { return make_double2(a.x + b.x, a.y + b.y); }
+__device__ __forceinline__ double2 csub(double2 a, double2 b) { return make_double2(a.x - b.x, a.y - b.y); }
+__device__ __forceinline__ double2 cscale(double2 a, double s) { return make_double2(a.x * s, a.y * s); }
+__device__ __forceinline__ double2 cconj(double2 a) { return make_double2(a.x, -a.y); }
+// i * a
+__device__ __forceinline__ double2 cmuli(double2 a) { return make_double2(-a.y, a.x); }
+// a*b + c
+__device__ __forceinline__ double2 cfma(double2 a, double2 b, double2 c) {
+    return make_double2(c.x + a.x * b.x - a.y * b.y, c.y + a.x * b.y + a.y * b.x);
+}
+
+// d/(d^2+eta^2) with the scaled branch for over/underflow
+// (reference PositionVertexRegularization.jl:17-29); eta == 0 selects the bare 1/d (0 at d == 0).
+__device__ __forceinline__ double wn_reg_inverse(double d, double eta) {
+    if (eta == 0.0) return d == 0.0 ? 0.0 : 1.0 / d;
+    double den = d * d + eta * eta;
+    if (isfinite(den) && den != 0.0) return d / den;
+    double s = fmax(fabs(d), eta);
+    double sd = d / s, sw = eta / s;
+    return (sd / (sd * sd + sw * sw)) / s;
+}

@@ -1,0 +1,558 @@
+// Host side of libwnlqg_hip.so: the C ABI declared in include/wnlqg.h.
+//
+// One engine = one model + one resolved task on one device and one HIP stream.  wnlqg_create
+// uploads the model and builds the R-space channels on the device once (cold, as
+// prepare_real_space!, HamiltonianMatrixElements.jl:180-199); wnlqg_run_mesh streams the mesh in
+// k batches through phase -> interpolation -> eigensolve -> rotation -> pairs -> contraction ->
+// lane sums, all on the device, and returns only the lane tensors.
+
+#include "../../include/wnlqg.h"
+#include "wnlqg_common.h"
+#include "wnlqg_kernels.h"
+
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <array>
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+namespace {
+
+thread_local std::string g_last_error;
+
+int fail(int code, const char *fmt, ...) {
+    char buf[1024];
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(buf, sizeof(buf), fmt, ap);
+    va_end(ap);
+    g_last_error = buf;
+    return code;
+}
+
+#define WN_HIP(expr)                                                                                      \
+    do {                                                                                                  \
+        hipError_t _e = (expr);                                                                           \
+        if (_e != hipSuccess)                                                                             \
+            return fail(_e == hipErrorOutOfMemory ? WNLQG_E_OOM : WNLQG_E_HIP, "%s failed: %s (%s:%d)", #expr, \
+                        hipGetErrorString(_e), __FILE__, __LINE__);                                       \
+    } while (0)
+
+// SC / IC prefactors with kpoint_weight = num_kpoints = 1
+// (ConventionalCurrentIntegralExecution.jl:243-245, :296-298; Core/Constants.jl:1-4).
+const double kCharge = 1.602176634e-19;
+const double kHbarJs = 1.054571817e-34;
+const double kHbarEvs = 6.582119569e-16;
+const double kPi = 3.14159265358979323846;
+
+double sc_prefactor(double vol) {
+    return kHbarEvs * kPi * (-kCharge) * (-kCharge) * (-kCharge) / (2.0 * kHbarJs * kHbarJs * 1 * vol);
+}
+double ic_prefactor(double vol) { return -kPi * kCharge * kCharge * kCharge / (kHbarJs * kHbarJs * 1 * vol); }
+
+template <typename T>
+struct DevBuf {
+    T *p = nullptr;
+    size_t n = 0;
+    hipError_t alloc(size_t count) {
+        release();
+        n = count;
+        if (count == 0) return hipSuccess;
+        return hipMalloc((void **)&p, count * sizeof(T));
+    }
+    void release() {
+        if (p) (void)hipFree(p);
+        p = nullptr;
+        n = 0;
+    }
+};
+
+}  // namespace
+
+struct wnlqg_engine {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    WnShape sh{};
+    WnScreen sc{};
+    int precision = WNLQG_PRECISION_FP64;
+    double cell_volume = 0.0;
+    double pref[2] = {0.0, 0.0};
+    int rmin[3] = {0, 0, 0};
+    int batch = 0;      // k rows per batch (multiple of 128)
+    int kpc = 16;       // k points per accumulation chunk
+
+    DevBuf<double> hr, ar, rc, bf, kp, x, e, omega;
+    DevBuf<int> deg, sweeps, count, flags;
+    DevBuf<int4> ridx, pmeta, chunks, lanes;
+    DevBuf<double2> tab, u, pmd, q, partial, lanebuf, lanecomp;
+    int partial_cap = 0;  // chunks the partial buffer holds
+    std::vector<double> omega_host;
+    hipEvent_t ev[8] = {};
+};
+
+namespace {
+
+int check_task(const wnlqg_task *t, int n) {
+    if (!t) return fail(WNLQG_E_INVALID, "task is NULL");
+    if ((t->quantities & ~(WNLQG_SHIFT_CURRENT | WNLQG_INJECTION_CURRENT)) || t->quantities == 0)
+        return fail(WNLQG_E_UNSUPPORTED, "quantities bitmask %d: only shift (1) and injection (2) current", t->quantities);
+    if (t->num_photon_energies < 1 || !t->photon_energies)
+        return fail(WNLQG_E_INVALID, "photon energies missing");
+    for (int i = 0; i < t->num_photon_energies; ++i) {
+        if (!std::isfinite(t->photon_energies[i])) return fail(WNLQG_E_NONFINITE, "non-finite photon energy");
+        if (i > 0 && t->photon_energies[i] < t->photon_energies[i - 1])
+            return fail(WNLQG_E_UNSUPPORTED, "photon energies must be ascending");
+    }
+    if (t->broadening_type != WNLQG_GAUSSIAN && t->broadening_type != WNLQG_LORENTZIAN)
+        return fail(WNLQG_E_INVALID, "Unknown smearing type: %d", t->broadening_type);
+    if (!(t->broadening > 0.0) || !std::isfinite(t->broadening))
+        return fail(WNLQG_E_INVALID, "broadening must be finite and positive");
+    if (!(t->temperature >= 0.0) || !std::isfinite(t->temperature) || !std::isfinite(t->fermi_energy))
+        return fail(WNLQG_E_INVALID, "invalid temperature or Fermi energy");
+    if (!(t->denominator_regularization >= 0.0) || !std::isfinite(t->denominator_regularization))
+        return fail(WNLQG_E_INVALID, "denominator_regularization must be finite and >= 0");
+    if (t->band_window_size != -1 && 2 * t->band_window_size <= n)
+        return fail(WNLQG_E_UNSUPPORTED, "band_window_size %d selects a partial window; only all bands are implemented",
+                    t->band_window_size);
+    if (t->precision != WNLQG_PRECISION_FP64 && t->precision != WNLQG_PRECISION_MIXED)
+        return fail(WNLQG_E_INVALID, "unknown precision %d", t->precision);
+    if (t->batch_size < 0) return fail(WNLQG_E_INVALID, "batch_size must be >= 0");
+    return WNLQG_OK;
+}
+
+int ensure_rows(wnlqg_engine *g, int rows) {
+    // working set for `rows` k points (multiple of 128)
+    const size_t n2 = (size_t)g->sh.n * g->sh.n;
+    if (g->x.n >= (size_t)rows * g->sh.ldo) return WNLQG_OK;
+    WN_HIP(g->x.alloc((size_t)rows * g->sh.ldo));
+    WN_HIP(g->kp.alloc((size_t)rows * 3));
+    WN_HIP(g->tab.alloc((size_t)rows * 3 * g->sh.S));
+    WN_HIP(g->e.alloc((size_t)rows * g->sh.n));
+    WN_HIP(g->u.alloc((size_t)rows * n2));
+    WN_HIP(g->sweeps.alloc((size_t)rows));
+    WN_HIP(g->count.alloc((size_t)rows));
+    WN_HIP(g->pmeta.alloc((size_t)rows * n2));
+    WN_HIP(g->pmd.alloc((size_t)rows * n2));
+    WN_HIP(g->q.alloc((size_t)rows * n2 * 27 * g->sc.nq));
+    return WNLQG_OK;
+}
+
+int ensure_partial(wnlqg_engine *g, int nchunks) {
+    if (g->partial_cap >= nchunks) return WNLQG_OK;
+    const size_t nelem = (size_t)27 * g->sc.nq * g->sc.nw;
+    WN_HIP(g->partial.alloc((size_t)nchunks * nelem));
+    WN_HIP(g->chunks.alloc((size_t)nchunks));
+    g->partial_cap = nchunks;
+    return WNLQG_OK;
+}
+
+// Interpolation + eigensystem for `count` k points already in g->kp (rows padded to 128).
+int run_front(wnlqg_engine *g, int count, int rows, bool eig) {
+    WN_HIP(wn_launch_phase_tables(g->kp.p, g->rmin, g->tab.p, rows, g->sh.S, g->stream));
+    WN_HIP(wn_launch_interp_f64(g->bf.p, g->tab.p, g->ridx.p, g->x.p, rows, g->sh, g->stream));
+    if (eig) WN_HIP(wn_launch_eig_f64(g->x.p, g->e.p, g->u.p, g->sweeps.p, count, g->sh, g->stream));
+    return WNLQG_OK;
+}
+
+int upload_points(wnlqg_engine *g, const double *kpoints, int m, int rows) {
+    std::vector<double> k((size_t)rows * 3, 0.0);
+    for (int i = 0; i < m; ++i)
+        for (int a = 0; a < 3; ++a) {
+            double v = kpoints[3 * i + a];
+            if (!std::isfinite(v)) return fail(WNLQG_E_NONFINITE, "non-finite k point %d", i);
+            k[3 * i + a] = v;
+        }
+    // the previous batch may still read kp: drain the stream, then copy synchronously
+    WN_HIP(hipStreamSynchronize(g->stream));
+    WN_HIP(hipMemcpy(g->kp.p, k.data(), k.size() * sizeof(double), hipMemcpyHostToDevice));
+    return WNLQG_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int32_t wnlqg_version(void) { return WNLQG_ABI_VERSION; }
+
+int32_t wnlqg_last_error(char *buf, size_t len) {
+    if (buf && len) {
+        size_t c = std::min(len - 1, g_last_error.size());
+        memcpy(buf, g_last_error.data(), c);
+        buf[c] = 0;
+    }
+    return (int32_t)g_last_error.size();
+}
+
+int32_t wnlqg_num_states(const wnlqg_engine *g) { return g ? g->sc.nq : 0; }
+int32_t wnlqg_num_channels(const wnlqg_engine *g) { return g ? g->sh.C : 0; }
+
+int32_t wnlqg_destroy(wnlqg_engine *g) {
+    if (!g) return WNLQG_OK;
+    (void)hipSetDevice(g->device);
+    if (g->stream) (void)hipStreamSynchronize(g->stream);
+    for (auto &e : g->ev)
+        if (e) (void)hipEventDestroy(e);
+    g->hr.release(); g->ar.release(); g->rc.release(); g->bf.release(); g->kp.release(); g->x.release();
+    g->e.release(); g->omega.release(); g->deg.release(); g->sweeps.release(); g->count.release();
+    g->flags.release(); g->ridx.release(); g->pmeta.release(); g->chunks.release(); g->lanes.release();
+    g->tab.release(); g->u.release(); g->pmd.release(); g->q.release(); g->partial.release();
+    g->lanebuf.release(); g->lanecomp.release();
+    if (g->stream) (void)hipStreamDestroy(g->stream);
+    delete g;
+    return WNLQG_OK;
+}
+
+int32_t wnlqg_create(const wnlqg_model *model, const wnlqg_task *task, wnlqg_engine **out) {
+    if (!out) return fail(WNLQG_E_INVALID, "out is NULL");
+    *out = nullptr;
+    if (!model || !model->lattice || !model->r_vectors || !model->r_degeneracies || !model->hamiltonian_r)
+        return fail(WNLQG_E_INVALID, "model pointers missing");
+    const int n = model->num_orbitals, nr = model->num_r_vectors;
+    if (n < 1 || nr < 1) return fail(WNLQG_E_INVALID, "num_orbitals and num_r_vectors must be positive");
+    if (n > 64) return fail(WNLQG_E_UNSUPPORTED, "num_orbitals %d > 64 is not supported by this build", n);
+    int rc = check_task(task, n);
+    if (rc) return rc;
+    const size_t n2 = (size_t)n * n;
+    for (size_t i = 0; i < 2 * n2 * nr; ++i)
+        if (!std::isfinite(model->hamiltonian_r[i])) return fail(WNLQG_E_NONFINITE, "non-finite H(R) entry %zu", i);
+    if (model->position_r)
+        for (size_t i = 0; i < 2 * n2 * 3 * nr; ++i)
+            if (!std::isfinite(model->position_r[i])) return fail(WNLQG_E_NONFINITE, "non-finite A(R) entry %zu", i);
+    for (int i = 0; i < 9; ++i)
+        if (!std::isfinite(model->lattice[i])) return fail(WNLQG_E_NONFINITE, "non-finite lattice");
+    for (int r = 0; r < nr; ++r)
+        if (model->r_degeneracies[r] <= 0) return fail(WNLQG_E_INVALID, "R degeneracy %d must be positive", r);
+
+    wnlqg_engine *g = new wnlqg_engine();
+    auto bail = [&](int code) {
+        wnlqg_destroy(g);
+        return code;
+    };
+    if (hipGetDevice(&g->device) != hipSuccess) return bail(fail(WNLQG_E_HIP, "no HIP device"));
+    if (hipStreamCreateWithFlags(&g->stream, hipStreamNonBlocking) != hipSuccess)
+        return bail(fail(WNLQG_E_HIP, "stream creation failed"));
+    for (auto &e : g->ev)
+        if (hipEventCreate(&e) != hipSuccess) return bail(fail(WNLQG_E_HIP, "event creation failed"));
+
+    // lattice: Julia column-major lattice[i, a] at i + 3a; rows are lattice vectors
+    const double *L = model->lattice;
+    g->cell_volume = L[0] * (L[4] * L[8] - L[7] * L[5]) - L[3] * (L[1] * L[8] - L[7] * L[2]) +
+                     L[6] * (L[1] * L[5] - L[4] * L[2]);
+    g->precision = task->precision;
+    g->sc.nw = task->num_photon_energies;
+    g->sc.gaussian = task->broadening_type == WNLQG_GAUSSIAN;
+    g->sc.width = task->broadening;
+    g->sc.mu = task->fermi_energy;
+    g->sc.temperature = task->temperature;
+    g->sc.eta = task->denominator_regularization;
+    g->sc.window = task->transition_window_factor * task->broadening;
+    g->sc.sc = (task->quantities & WNLQG_SHIFT_CURRENT) != 0;
+    g->sc.ic = (task->quantities & WNLQG_INJECTION_CURRENT) != 0;
+    g->sc.nq = g->sc.sc + g->sc.ic;
+    int qi = 0;
+    if (g->sc.sc) g->pref[qi++] = sc_prefactor(g->cell_volume);
+    if (g->sc.ic) g->pref[qi++] = ic_prefactor(g->cell_volume);
+    g->omega_host.assign(task->photon_energies, task->photon_energies + g->sc.nw);
+
+    // shape
+    g->sh.n = n;
+    g->sh.nr = nr;
+    g->sh.has_pos = model->position_r != nullptr;
+    g->sh.C = g->sh.has_pos ? WN_CH_FULL : WN_CH_BASE;
+    g->sh.Kb = (nr + WN_RB - 1) / WN_RB;
+    g->sh.nNt = (int)((2 * (size_t)g->sh.C * n2 + WN_GN - 1) / WN_GN);
+    g->sh.ldo = g->sh.nNt * WN_GN;
+    int rmax[3];
+    for (int a = 0; a < 3; ++a) {
+        g->rmin[a] = model->r_vectors[a];
+        rmax[a] = model->r_vectors[a];
+    }
+    for (int r = 0; r < nr; ++r)
+        for (int a = 0; a < 3; ++a) {
+            g->rmin[a] = std::min(g->rmin[a], model->r_vectors[3 * r + a]);
+            rmax[a] = std::max(rmax[a], model->r_vectors[3 * r + a]);
+        }
+    g->sh.S = 1;
+    for (int a = 0; a < 3; ++a) g->sh.S = std::max(g->sh.S, rmax[a] - g->rmin[a] + 1);
+
+    // host-side R tables: Cartesian R (as _r_vector_cartesian, HamiltonianMatrixElements.jl:74-85)
+    std::vector<double> rcart((size_t)3 * nr);
+    std::vector<int4> rid((size_t)g->sh.Kb * WN_RB, make_int4(0, 0, 0, 0));
+    for (int r = 0; r < nr; ++r) {
+        const int *R = model->r_vectors + 3 * r;
+        for (int a = 0; a < 3; ++a)
+            rcart[3 * r + a] = R[0] * L[0 + 3 * a] + R[1] * L[1 + 3 * a] + R[2] * L[2 + 3 * a];
+        rid[r] = make_int4(R[0] - g->rmin[0], R[1] - g->rmin[1], R[2] - g->rmin[2], 0);
+    }
+    if (g->hr.alloc(2 * n2 * nr) || g->rc.alloc(3 * (size_t)nr) || g->deg.alloc(nr) ||
+        g->ridx.alloc(rid.size()) || g->omega.alloc(g->sc.nw) || g->flags.alloc(4) ||
+        g->bf.alloc((size_t)g->sh.Kb * g->sh.nNt * 2048) ||
+        (g->sh.has_pos && g->ar.alloc(2 * n2 * 3 * nr)))
+        return bail(fail(WNLQG_E_OOM, "device allocation for the model failed"));
+    auto up = [&](void *dst, const void *src, size_t bytes) {
+        return hipMemcpy(dst, src, bytes, hipMemcpyHostToDevice) == hipSuccess;
+    };
+    if (!up(g->hr.p, model->hamiltonian_r, 2 * n2 * nr * sizeof(double)) ||
+        !up(g->rc.p, rcart.data(), rcart.size() * sizeof(double)) ||
+        !up(g->deg.p, model->r_degeneracies, nr * sizeof(int)) ||
+        !up(g->ridx.p, rid.data(), rid.size() * sizeof(int4)) ||
+        !up(g->omega.p, g->omega_host.data(), g->sc.nw * sizeof(double)) ||
+        (g->sh.has_pos && !up(g->ar.p, model->position_r, 2 * n2 * 3 * nr * sizeof(double))))
+        return bail(fail(WNLQG_E_HIP, "model upload failed"));
+    if (wn_launch_build_coeff(g->hr.p, g->ar.p, g->rc.p, g->deg.p, g->bf.p, g->sh, g->stream) != hipSuccess)
+        return bail(fail(WNLQG_E_HIP, "coefficient build launch failed"));
+    if (hipStreamSynchronize(g->stream) != hipSuccess) return bail(fail(WNLQG_E_HIP, "coefficient build failed"));
+
+    // batch size: interpolation rows + pair buffers within ~24 GB unless the caller fixed it
+    const size_t per_k = (size_t)g->sh.ldo * 8 + n2 * (sizeof(int4) + sizeof(double2) + 27 * g->sc.nq * 16) +
+                         n2 * 16 + (size_t)3 * g->sh.S * 16;
+    long long b = task->batch_size > 0 ? task->batch_size : (long long)(24.0e9 / (double)per_k);
+    b = std::max(128LL, std::min(b, 65536LL));
+    g->batch = (int)((b + WN_GM - 1) / WN_GM * WN_GM);
+    rc = ensure_rows(g, g->batch);
+    if (rc) return bail(rc);
+    *out = g;
+    return WNLQG_OK;
+}
+
+int32_t wnlqg_synchronize(wnlqg_engine *g) {
+    if (!g) return fail(WNLQG_E_INVALID, "engine is NULL");
+    WN_HIP(hipStreamSynchronize(g->stream));
+    return WNLQG_OK;
+}
+
+int32_t wnlqg_fourier_points(wnlqg_engine *g, const double *kpoints, int32_t m, double *out) {
+    if (!g || !kpoints || !out || m < 0) return fail(WNLQG_E_INVALID, "invalid arguments");
+    WN_HIP(hipSetDevice(g->device));
+    const size_t blk = (size_t)g->sh.C * g->sh.n * g->sh.n * 2;
+    for (int p0 = 0; p0 < m; p0 += g->batch) {
+        int cnt = std::min(g->batch, m - p0);
+        int rows = (cnt + WN_GM - 1) / WN_GM * WN_GM;
+        int rc = upload_points(g, kpoints + 3 * (size_t)p0, cnt, rows);
+        if (rc) return rc;
+        rc = run_front(g, cnt, rows, false);
+        if (rc) return rc;
+        WN_HIP(hipMemcpy2DAsync(out + (size_t)p0 * blk, blk * sizeof(double), g->x.p, g->sh.ldo * sizeof(double),
+                                blk * sizeof(double), cnt, hipMemcpyDeviceToHost, g->stream));
+        WN_HIP(hipStreamSynchronize(g->stream));
+    }
+    return WNLQG_OK;
+}
+
+int32_t wnlqg_spectrum_points(wnlqg_engine *g, const double *kpoints, int32_t m, double *energies, double *vectors) {
+    if (!g || !kpoints || !energies || !vectors || m < 0) return fail(WNLQG_E_INVALID, "invalid arguments");
+    WN_HIP(hipSetDevice(g->device));
+    const size_t n = g->sh.n;
+    for (int p0 = 0; p0 < m; p0 += g->batch) {
+        int cnt = std::min(g->batch, m - p0);
+        int rows = (cnt + WN_GM - 1) / WN_GM * WN_GM;
+        int rc = upload_points(g, kpoints + 3 * (size_t)p0, cnt, rows);
+        if (rc) return rc;
+        rc = run_front(g, cnt, rows, true);
+        if (rc) return rc;
+        WN_HIP(hipMemcpyAsync(energies + (size_t)p0 * n, g->e.p, cnt * n * sizeof(double), hipMemcpyDeviceToHost,
+                              g->stream));
+        WN_HIP(hipMemcpyAsync(vectors + (size_t)p0 * n * n * 2, g->u.p, cnt * n * n * sizeof(double2),
+                              hipMemcpyDeviceToHost, g->stream));
+        WN_HIP(hipStreamSynchronize(g->stream));
+    }
+    return WNLQG_OK;
+}
+
+namespace {
+
+int check_flags(wnlqg_engine *g) {
+    int f = 0;
+    WN_HIP(hipMemcpy(&f, g->flags.p, sizeof(int), hipMemcpyDeviceToHost));
+    if (f & 1)
+        return fail(WNLQG_E_GAP,
+                    "GEOMETRY_SINGLE_BAND_SPECTRUM_NOT_ISOLATED: denominator_regularization = 0 needs gaps > 1e-10 eV");
+    return WNLQG_OK;
+}
+
+}  // namespace
+
+int32_t wnlqg_eval_points(wnlqg_engine *g, const double *kpoints, int32_t m, double *out) {
+    if (!g || !kpoints || !out || m < 0) return fail(WNLQG_E_INVALID, "invalid arguments");
+    WN_HIP(hipSetDevice(g->device));
+    const int nelem = 27 * g->sc.nq * g->sc.nw;
+    WN_HIP(hipMemsetAsync(g->flags.p, 0, sizeof(int), g->stream));
+    std::vector<double> part;
+    for (int p0 = 0; p0 < m; p0 += g->batch) {
+        int cnt = std::min(g->batch, m - p0);
+        int rows = (cnt + WN_GM - 1) / WN_GM * WN_GM;
+        int rc = upload_points(g, kpoints + 3 * (size_t)p0, cnt, rows);
+        if (rc) return rc;
+        rc = run_front(g, cnt, rows, true);
+        if (rc) return rc;
+        WN_HIP(wn_launch_rotate_f64(g->x.p, g->u.p, cnt, g->sh, g->stream));
+        WN_HIP(wn_launch_pairs_f64(g->x.p, g->e.p, g->omega.p, g->count.p, g->pmeta.p, g->pmd.p, g->q.p, g->flags.p,
+                                   cnt, g->sh, g->sc, g->stream));
+        rc = ensure_partial(g, cnt);
+        if (rc) return rc;
+        std::vector<int4> ch(cnt);
+        for (int i = 0; i < cnt; ++i) ch[i] = make_int4(i, 1, i, 0);
+        WN_HIP(hipMemcpy(g->chunks.p, ch.data(), cnt * sizeof(int4), hipMemcpyHostToDevice));
+        WN_HIP(wn_launch_accum(g->chunks.p, cnt, g->count.p, g->pmeta.p, g->pmd.p, g->q.p, g->omega.p, g->partial.p,
+                               g->sh, g->sc, g->stream));
+        part.resize((size_t)cnt * nelem * 2);
+        WN_HIP(hipMemcpyAsync(part.data(), g->partial.p, part.size() * sizeof(double), hipMemcpyDeviceToHost,
+                              g->stream));
+        WN_HIP(hipStreamSynchronize(g->stream));
+        for (int i = 0; i < cnt; ++i)
+            for (int qv = 0; qv < g->sc.nq; ++qv)
+                for (int e = 0; e < 27 * g->sc.nw; ++e) {
+                    size_t src = ((size_t)i * nelem + (size_t)qv * 27 * g->sc.nw + e) * 2;
+                    size_t dst = (((size_t)(p0 + i) * g->sc.nq + qv) * 27 * g->sc.nw + e) * 2;
+                    out[dst] = g->pref[qv] * part[src];
+                    out[dst + 1] = g->pref[qv] * part[src + 1];
+                }
+    }
+    int rc = check_flags(g);
+    if (rc) return rc;
+    for (size_t i = 0; i < (size_t)m * nelem * 2; ++i)
+        if (!std::isfinite(out[i])) return fail(WNLQG_E_NONFINITE, "non-finite response value");
+    return WNLQG_OK;
+}
+
+int32_t wnlqg_run_mesh(wnlqg_engine *g, const int32_t mesh[3], int32_t n_lanes, double *lane_out, double *total_out,
+                       wnlqg_stats *stats) {
+    if (!g || !mesh || !lane_out) return fail(WNLQG_E_INVALID, "invalid arguments");
+    if (mesh[0] < 1 || mesh[1] < 1 || mesh[2] < 1) return fail(WNLQG_E_INVALID, "mesh sizes must be positive");
+    const long long nk = (long long)mesh[0] * mesh[1] * mesh[2];
+    if (n_lanes < 1 || n_lanes > nk) return fail(WNLQG_E_INVALID, "n_lanes must be in [1, N_k]");
+    WN_HIP(hipSetDevice(g->device));
+    const int nelem = 27 * g->sc.nq * g->sc.nw;
+
+    // chunk plan: lanes are contiguous k ranges (BundleAssembly.jl:41-47); each batch splits the
+    // part of every lane it covers into chunks of at most kpc points, in k order.
+    std::vector<long long> lstart(n_lanes + 1);
+    for (int l = 0; l <= n_lanes; ++l) lstart[l] = (long long)((__int128)l * nk / n_lanes);
+    const long long nbatch = (nk + g->batch - 1) / g->batch;
+    std::vector<int4> all_chunks, all_lanes;
+    std::vector<long long> bchunk(nbatch + 1), blane(nbatch + 1);
+    int max_chunks = 0;
+    for (long long bi = 0; bi < nbatch; ++bi) {
+        const long long k0 = bi * g->batch, k1 = std::min(nk, k0 + g->batch);
+        bchunk[bi] = all_chunks.size();
+        blane[bi] = all_lanes.size();
+        int l = (int)(std::upper_bound(lstart.begin(), lstart.end(), k0) - lstart.begin()) - 1;
+        for (; l < n_lanes && lstart[l] < k1; ++l) {
+            long long a = std::max(k0, lstart[l]), b = std::min(k1, lstart[l + 1]);
+            if (a >= b) continue;
+            int cb = (int)(all_chunks.size() - bchunk[bi]);
+            for (long long c = a; c < b; c += g->kpc)
+                all_chunks.push_back(make_int4((int)(c - k0), (int)std::min<long long>(g->kpc, b - c), l, 0));
+            int ce = (int)(all_chunks.size() - bchunk[bi]);
+            all_lanes.push_back(make_int4(l, cb, ce, 0));
+        }
+        max_chunks = std::max(max_chunks, (int)(all_chunks.size() - bchunk[bi]));
+    }
+    bchunk[nbatch] = all_chunks.size();
+    blane[nbatch] = all_lanes.size();
+    int rc = ensure_partial(g, max_chunks);
+    if (rc) return rc;
+    DevBuf<int4> dchunks, dlanes;
+    WN_HIP(dchunks.alloc(all_chunks.size()));
+    WN_HIP(dlanes.alloc(all_lanes.size()));
+    WN_HIP(hipMemcpy(dchunks.p, all_chunks.data(), all_chunks.size() * sizeof(int4), hipMemcpyHostToDevice));
+    WN_HIP(hipMemcpy(dlanes.p, all_lanes.data(), all_lanes.size() * sizeof(int4), hipMemcpyHostToDevice));
+    if (g->lanebuf.n < (size_t)n_lanes * nelem) {
+        WN_HIP(g->lanebuf.alloc((size_t)n_lanes * nelem));
+        WN_HIP(g->lanecomp.alloc((size_t)n_lanes * nelem));
+    }
+    WN_HIP(hipMemsetAsync(g->lanebuf.p, 0, (size_t)n_lanes * nelem * sizeof(double2), g->stream));
+    WN_HIP(hipMemsetAsync(g->lanecomp.p, 0, (size_t)n_lanes * nelem * sizeof(double2), g->stream));
+    WN_HIP(hipMemsetAsync(g->flags.p, 0, sizeof(int), g->stream));
+    double2 *comp = g->precision == WNLQG_PRECISION_MIXED ? g->lanecomp.p : nullptr;
+
+    std::vector<hipEvent_t> evs;
+    auto mark = [&]() -> hipEvent_t {
+        hipEvent_t e;
+        if (hipEventCreate(&e) != hipSuccess) return nullptr;
+        hipEventRecord(e, g->stream);
+        evs.push_back(e);
+        return e;
+    };
+    std::vector<std::array<hipEvent_t, 7>> marks(nbatch);
+    for (long long bi = 0; bi < nbatch; ++bi) {
+        const long long k0 = bi * g->batch;
+        const int cnt = (int)std::min<long long>(g->batch, nk - k0);
+        const int rows = (cnt + WN_GM - 1) / WN_GM * WN_GM;
+        auto &mk = marks[bi];
+        mk[0] = mark();
+        WN_HIP(wn_launch_mesh_kpoints(g->kp.p, mesh, k0, cnt, rows, g->stream));
+        WN_HIP(wn_launch_phase_tables(g->kp.p, g->rmin, g->tab.p, rows, g->sh.S, g->stream));
+        mk[1] = mark();
+        WN_HIP(wn_launch_interp_f64(g->bf.p, g->tab.p, g->ridx.p, g->x.p, rows, g->sh, g->stream));
+        mk[2] = mark();
+        WN_HIP(wn_launch_eig_f64(g->x.p, g->e.p, g->u.p, g->sweeps.p, cnt, g->sh, g->stream));
+        mk[3] = mark();
+        WN_HIP(wn_launch_rotate_f64(g->x.p, g->u.p, cnt, g->sh, g->stream));
+        mk[4] = mark();
+        WN_HIP(wn_launch_pairs_f64(g->x.p, g->e.p, g->omega.p, g->count.p, g->pmeta.p, g->pmd.p, g->q.p, g->flags.p,
+                                   cnt, g->sh, g->sc, g->stream));
+        mk[5] = mark();
+        const int nch = (int)(bchunk[bi + 1] - bchunk[bi]);
+        const int nln = (int)(blane[bi + 1] - blane[bi]);
+        WN_HIP(wn_launch_accum(dchunks.p + bchunk[bi], nch, g->count.p, g->pmeta.p, g->pmd.p, g->q.p, g->omega.p,
+                               g->partial.p, g->sh, g->sc, g->stream));
+        WN_HIP(wn_launch_lane_reduce(dlanes.p + blane[bi], nln, g->partial.p, g->lanebuf.p, comp, nelem, g->stream));
+        mk[6] = mark();
+    }
+    std::vector<double> lanes((size_t)n_lanes * nelem * 2);
+    WN_HIP(hipMemcpyAsync(lanes.data(), g->lanebuf.p, lanes.size() * sizeof(double), hipMemcpyDeviceToHost,
+                          g->stream));
+    WN_HIP(hipStreamSynchronize(g->stream));
+    dchunks.release();
+    dlanes.release();
+    wnlqg_stats st{};
+    float ms = 0.0f;
+    for (long long bi = 0; bi < nbatch; ++bi) {
+        auto &mk = marks[bi];
+        if (!mk[0] || !mk[6]) continue;
+        hipEventElapsedTime(&ms, mk[1], mk[2]); st.interp_ms += ms;
+        hipEventElapsedTime(&ms, mk[2], mk[3]); st.eig_ms += ms;
+        hipEventElapsedTime(&ms, mk[3], mk[4]); st.rotate_ms += ms;
+        hipEventElapsedTime(&ms, mk[4], mk[5]); st.pairs_ms += ms;
+        hipEventElapsedTime(&ms, mk[5], mk[6]); st.accum_ms += ms;
+    }
+    if (nbatch > 0 && marks[0][0] && marks[nbatch - 1][6]) {
+        hipEventElapsedTime(&ms, marks[0][0], marks[nbatch - 1][6]);
+        st.total_ms = ms;
+    }
+    for (auto e : evs) hipEventDestroy(e);
+    rc = check_flags(g);
+    if (rc) return rc;
+    st.kpoints = nk;
+    st.batches = (int)nbatch;
+    st.interp_launches = (int)nbatch;
+    st.interp_flops = 8.0 * g->sh.nr * (double)g->sh.n * g->sh.n * g->sh.C * (double)nk;
+    if (stats) *stats = st;
+
+    // scale: each state * its prefactor, then 1/N_k (consume_point!, IntegralDriver.jl:820-874)
+    const double scale = 1.0 / (double)nk;
+    const int per_state = 27 * g->sc.nw;
+    for (int l = 0; l < n_lanes; ++l)
+        for (int qv = 0; qv < g->sc.nq; ++qv)
+            for (int e = 0; e < per_state; ++e) {
+                size_t i = ((size_t)l * nelem + (size_t)qv * per_state + e) * 2;
+                lane_out[i] = (g->pref[qv] * lanes[i]) * scale;
+                lane_out[i + 1] = (g->pref[qv] * lanes[i + 1]) * scale;
+            }
+    for (size_t i = 0; i < lanes.size(); ++i)
+        if (!std::isfinite(lane_out[i])) return fail(WNLQG_E_NONFINITE, "non-finite response value");
+    if (total_out) {
+        for (size_t i = 0; i < (size_t)nelem * 2; ++i) total_out[i] = 0.0;
+        for (int l = 0; l < n_lanes; ++l)
+            for (size_t i = 0; i < (size_t)nelem * 2; ++i) total_out[i] += lane_out[(size_t)l * nelem * 2 + i];
+    }
+    return WNLQG_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,61 @@
+// Launch wrappers of the engine's HIP kernels (host-callable, C++ linkage, internal to the library).
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+struct WnShape {
+    int n;        // orbitals
+    int nr;       // R vectors
+    int C;        // interpolation channels
+    int has_pos;  // position channels present
+    int Kb;       // blocks of 8 R vectors
+    int nNt;      // 128-column tiles of the interpolation output
+    int ldo;      // per-k row stride (doubles) of the interpolation output = nNt * 128
+    int S;        // span of one phase table axis
+};
+
+struct WnScreen {
+    int nw;             // photon energies
+    int gaussian;       // 1 Gaussian, 0 Lorentzian
+    double width;       // broadening
+    double mu;          // Fermi energy
+    double temperature; // K
+    double eta;         // denominator regularization
+    double window;      // transition_window_factor * width (Lorentzian screening)
+    int nq;             // states (1 or 2)
+    int sc;             // shift current requested
+    int ic;             // injection current requested
+};
+
+// Build the interpolation B operand in fragment order from H(R), A(R) (Julia layouts).
+hipError_t wn_launch_build_coeff(const double *hr, const double *ar, const double *rc, const int *deg,
+                                 double *bf, const WnShape &sh, hipStream_t st);
+// k points of a mesh batch: global indices k0 .. k0+count-1, rows up to rows_pad (extra rows k = 0).
+hipError_t wn_launch_mesh_kpoints(double *kp, const int *mesh, long long k0, int count, int rows_pad,
+                                  hipStream_t st);
+// Phase tables T[k][axis][j] = exp(2 pi i k_axis (rmin_axis + j)).
+hipError_t wn_launch_phase_tables(const double *kp, const int *rmin, double2 *tab, int rows, int S,
+                                  hipStream_t st);
+// FP64 MFMA interpolation GEMM.
+hipError_t wn_launch_interp_f64(const double *bf, const double2 *tab, const int4 *ridx, double *out, int rows_pad,
+                                const WnShape &sh, hipStream_t st);
+// Split-BF16 MFMA interpolation GEMM (FP32 accumulate, FP64 output).
+hipError_t wn_launch_interp_bf16(const uint16_t *bfh, const uint16_t *bfl, const double2 *tab, const int4 *ridx,
+                                 double *out, int rows_pad, const WnShape &sh, hipStream_t st);
+// Jacobi eigensolver of the upper triangle of channel 0.
+hipError_t wn_launch_eig_f64(const double *x, double *e, double2 *u, int *sweeps, int count, const WnShape &sh,
+                             hipStream_t st);
+// U^+ X U for channels 1 .. C-1, in place.
+hipError_t wn_launch_rotate_f64(double *x, const double2 *u, int count, const WnShape &sh, hipStream_t st);
+// Screening, pair lists, generalized derivatives and SC/IC kernels.
+hipError_t wn_launch_pairs_f64(const double *x, const double *e, const double *omega, int *count, int4 *pmeta,
+                               double2 *pmd, double2 *q, int *flags, int nk, const WnShape &sh, const WnScreen &sc,
+                               hipStream_t st);
+// Frequency contraction of the pairs of each chunk.
+hipError_t wn_launch_accum(const int4 *chunks, int nchunks, const int *count, const int4 *pmeta, const double2 *pmd,
+                           const double2 *q, const double *omega, double2 *partial, const WnShape &sh,
+                           const WnScreen &sc, hipStream_t st);
+// Fixed-order chunk -> lane sums.
+hipError_t wn_launch_lane_reduce(const int4 *lanes, int nlanes, const double2 *partial, double2 *lanebuf,
+                                 double2 *lanecomp, int nelem, hipStream_t st);

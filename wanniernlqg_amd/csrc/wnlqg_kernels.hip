@@ -1,0 +1,904 @@
+// HIP kernels of the MI355X k-mesh response engine (gfx950 / CDNA4, wave64).
+//
+// Stage map (reference functions in brackets):
+//   build_coeff   R-space channels i R_a H, -R_a R_b H, A_a, i R_b A_a / deg(R) in MFMA fragment order
+//                 [prepare_hamiltonian_real_space!, HamiltonianMatrixElements.jl:92-177;
+//                  prepare_position_real_space!, PositionMatrixElements.jl:8-84]
+//   phase_tables  exp(2 pi i k_a R_a) per axis   [_prepare_fourier_factors!, HamiltonianMatrixElements.jl:28-39]
+//   interp_f64    X(k) = sum_R f_R(k) X(R) as a k x R GEMM on FP64 MFMA
+//                 [_fourier_hamiltonian!, :42-58; derivative sums :427-462; PositionMatrixElements.jl:239-409]
+//   eig_f64       Hermitian Jacobi, upper triangle, canonical column phase
+//                 [DenseSpectra.hermitian_spectrum, DenseSpectra.jl:14-16; canonicalize_eigenvectors!, :6-25]
+//   rotate_f64    U^+ X U                                [transform_to_hamiltonian_gauge!, :61-71]
+//   pairs_f64     occupations, delta screen, active pairs, D, Berry connection, generalized derivative,
+//                 SC/IC kernels   [TransitionScreening.jl:32-162; PositionMatrixElements.jl:444-489;
+//                 ConventionalShiftCurrent.jl:191-229,527-767; ConventionalInjectionCurrent.jl:23-97]
+//   accum         sum over pairs of df * delta(w) * Q per chunk of k points
+//                 [accumulate_conventional_response!, ConventionalShiftCurrent.jl:774-884]
+//   lane_reduce   chunk partials -> deterministic lanes [consume_point!, IntegralDriver.jl:820-874]
+
+#include "wnlqg_common.h"
+#include "wnlqg_kernels.h"
+
+#include <math.h>
+
+// ------------------------------------------------------------------------------------------------
+// R-space channels in MFMA fragment order.
+//
+// Real GEMM: out[k][2J+p] = sum_kk A[k][kk] B[kk][2J+p] with, per block of 8 R vectors, 16 K rows
+// ordered (cos part of r0..r7, sin part of r0..r7).  B entries for complex X_J(R)/deg(R):
+//   cos row: (Re X, Im X) into (re, im) output columns; sin row: (-Im X, Re X).
+// Fragment order: bf[((kb*nNt + nt)*4 + s)*8 + j][lane] holds B[kb*16 + 4s + lane/16][nt*128 + 16j + lane%16],
+// which is the per-lane B operand of v_mfma_f64_16x16x4f64 for k-substep s and 16-column subtile j.
+// ------------------------------------------------------------------------------------------------
+__global__ void k_build_coeff(const double *__restrict__ hr, const double *__restrict__ ar,
+                              const double *__restrict__ rc, const int *__restrict__ deg, double *__restrict__ bf,
+                              WnShape sh) {
+    const long long total = (long long)sh.Kb * sh.nNt * 2048;
+    const int n = sh.n, n2 = n * n;
+    for (long long e = (long long)blockIdx.x * blockDim.x + threadIdx.x; e < total;
+         e += (long long)gridDim.x * blockDim.x) {
+        const int lane = (int)(e & 63);
+        const int j = (int)((e >> 6) & 7);
+        const int s = (int)((e >> 9) & 3);
+        const long long rest = e >> 11;
+        const int nt = (int)(rest % sh.nNt);
+        const int kb = (int)(rest / sh.nNt);
+        const int kk = 4 * s + (lane >> 4);
+        const int part = kk >> 3;
+        const int r = kb * WN_RB + (kk & 7);
+        const int col = nt * WN_GN + 16 * j + (lane & 15);
+        const int J = col >> 1, p2 = col & 1;
+        double v = 0.0;
+        if (r < sh.nr && J < sh.C * n2) {
+            const int c = J / n2, ij = J - c * n2;
+            const int i = ij / n, jj = ij - i * n;
+            // Julia column-major element (i, jj, r) and (i, jj, a, r)
+            const double *h = hr + 2 * ((size_t)i + (size_t)n * jj + (size_t)n2 * r);
+            double xr = 0.0, xi = 0.0;
+            const double *R = rc + 3 * r;
+            if (c == WN_CH_H) {
+                xr = h[0];
+                xi = h[1];
+            } else if (c < WN_CH_DDH) {
+                const double d = R[c - WN_CH_DH];
+                // (i d) * h
+                xr = -d * h[1];
+                xi = d * h[0];
+            } else if (c < WN_CH_A) {
+                const int t = c - WN_CH_DDH;
+                const int a = t < 3 ? t : (t == 3 ? 1 : 0);
+                const int b = t < 3 ? t : (t == 5 ? 1 : 2);
+                const double f = -R[a] * R[b];
+                xr = f * h[0];
+                xi = f * h[1];
+            } else {
+                const int a = c < WN_CH_DA ? c - WN_CH_A : (c - WN_CH_DA) / 3;
+                const double *p = ar + 2 * ((size_t)i + (size_t)n * jj + (size_t)n2 * a + (size_t)3 * n2 * r);
+                if (c < WN_CH_DA) {
+                    xr = p[0];
+                    xi = p[1];
+                } else {
+                    const int b = (c - WN_CH_DA) % 3;
+                    const double d = R[b];
+                    xr = -d * p[1];
+                    xi = d * p[0];
+                }
+            }
+            const double inv_deg = 1.0 / (double)deg[r];
+            xr *= inv_deg;
+            xi *= inv_deg;
+            v = part == 0 ? (p2 == 0 ? xr : xi) : (p2 == 0 ? -xi : xr);
+        }
+        bf[e] = v;
+    }
+}
+
+hipError_t wn_launch_build_coeff(const double *hr, const double *ar, const double *rc, const int *deg, double *bf,
+                                 const WnShape &sh, hipStream_t st) {
+    long long total = (long long)sh.Kb * sh.nNt * 2048;
+    long long blocks = (total + 255) / 256;
+    if (blocks > 65536) blocks = 65536;
+    hipLaunchKernelGGL(k_build_coeff, dim3((unsigned)blocks), dim3(256), 0, st, hr, ar, rc, deg, bf, sh);
+    return hipGetLastError();
+}
+
+// ------------------------------------------------------------------------------------------------
+// Mesh k points and phase tables.
+// ------------------------------------------------------------------------------------------------
+__global__ void k_mesh_kpoints(double *__restrict__ kp, int n1, int n2, int n3, long long k0, int count, int rows) {
+    int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= rows) return;
+    double k1 = 0.0, k2 = 0.0, k3 = 0.0;
+    if (t < count) {
+        long long g = k0 + t;
+        long long i1 = g % n1;
+        long long i2 = (g / n1) % n2;
+        long long i3 = g / ((long long)n1 * n2);
+        k1 = (double)i1 / (double)n1;
+        k2 = (double)i2 / (double)n2;
+        k3 = (double)i3 / (double)n3;
+    }
+    kp[3 * t + 0] = k1;
+    kp[3 * t + 1] = k2;
+    kp[3 * t + 2] = k3;
+}
+
+hipError_t wn_launch_mesh_kpoints(double *kp, const int *mesh, long long k0, int count, int rows_pad,
+                                  hipStream_t st) {
+    hipLaunchKernelGGL(k_mesh_kpoints, dim3((rows_pad + 255) / 256), dim3(256), 0, st, kp, mesh[0], mesh[1], mesh[2],
+                       k0, count, rows_pad);
+    return hipGetLastError();
+}
+
+__global__ void k_phase_tables(const double *__restrict__ kp, int rmin0, int rmin1, int rmin2, double2 *__restrict__ tab,
+                               int rows, int S) {
+    int t = blockIdx.x * blockDim.x + threadIdx.x;
+    int total = rows * 3 * S;
+    if (t >= total) return;
+    int k = t / (3 * S);
+    int rem = t - k * 3 * S;
+    int a = rem / S;
+    int j = rem - a * S;
+    int rmin = a == 0 ? rmin0 : (a == 1 ? rmin1 : rmin2);
+    double x = 2.0 * kp[3 * k + a] * (double)(rmin + j);
+    double s, c;
+    sincospi(x, &s, &c);
+    tab[t] = make_double2(c, s);
+}
+
+hipError_t wn_launch_phase_tables(const double *kp, const int *rmin, double2 *tab, int rows, int S, hipStream_t st) {
+    int total = rows * 3 * S;
+    hipLaunchKernelGGL(k_phase_tables, dim3((total + 255) / 256), dim3(256), 0, st, kp, rmin[0], rmin[1], rmin[2], tab,
+                       rows, S);
+    return hipGetLastError();
+}
+
+// ------------------------------------------------------------------------------------------------
+// FP64 MFMA interpolation GEMM.
+//
+// Workgroup: 256 threads = 4 waves in 2x2, tile 128 k x 128 real columns; each wave owns a 64x64
+// block = 4x4 subtiles of v_mfma_f64_16x16x4f64.  The A operand (Fourier factors) is generated in
+// registers from the per-axis phase tables: lane l needs rows 16i + l%16 and R vectors
+// kb*8 + l/16 and kb*8 + 4 + l/16 of each K block, cos parts for substeps 0,1 and sin parts for
+// substeps 2,3.  B fragments stream from HBM/L2 in fragment order, one K block ahead.
+// XCD-aware tile order: workgroup g runs on XCD g%8; each XCD sweeps all k tiles of its own
+// column panels so that a 2.7 MB B panel stays in that XCD's L2.
+// ------------------------------------------------------------------------------------------------
+__global__ __launch_bounds__(256, 2) void k_interp_f64(const double *__restrict__ bf, const double2 *__restrict__ tab,
+                                                      const int4 *__restrict__ ridx, double *__restrict__ out,
+                                                      int nMt, int nNt, int Kb, int S, int ldo) {
+    const int g = blockIdx.x;
+    const int xcd = g & 7;
+    const int jj = g >> 3;
+    const int nl = jj / nMt;
+    const int mt = jj - nl * nMt;
+    const int nt = nl * 8 + xcd;
+    if (nt >= nNt) return;
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int wm = wave >> 1, wn = wave & 1;
+    const int lrow = lane & 15, lq = lane >> 4;
+
+    const double2 *tk[4];
+#pragma unroll
+    for (int i = 0; i < 4; ++i) tk[i] = tab + (size_t)(mt * WN_GM + wm * 64 + 16 * i + lrow) * 3 * S;
+
+    f64x4 acc[4][4];
+#pragma unroll
+    for (int i = 0; i < 4; ++i)
+#pragma unroll
+        for (int j = 0; j < 4; ++j) acc[i][j] = (f64x4){0.0, 0.0, 0.0, 0.0};
+
+    const size_t kb_stride = (size_t)nNt * 2048;
+    const double *bp = bf + (size_t)nt * 2048 + (size_t)wn * 256 + lane;
+    double bcur[4][4], bnext[4][4];
+#pragma unroll
+    for (int s = 0; s < 4; ++s)
+#pragma unroll
+        for (int j = 0; j < 4; ++j) bcur[s][j] = bp[s * 512 + j * 64];
+
+    for (int kb = 0; kb < Kb; ++kb) {
+        if (kb + 1 < Kb) {
+            const double *bq = bp + (size_t)(kb + 1) * kb_stride;
+#pragma unroll
+            for (int s = 0; s < 4; ++s)
+#pragma unroll
+                for (int j = 0; j < 4; ++j) bnext[s][j] = bq[s * 512 + j * 64];
+        }
+        const int4 id0 = ridx[kb * WN_RB + lq];
+        const int4 id1 = ridx[kb * WN_RB + 4 + lq];
+        double a[4][4];
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+            const double2 *t = tk[i];
+            double2 f0 = cmul(cmul(t[id0.x], t[S + id0.y]), t[2 * S + id0.z]);
+            double2 f1 = cmul(cmul(t[id1.x], t[S + id1.y]), t[2 * S + id1.z]);
+            a[0][i] = f0.x;
+            a[1][i] = f1.x;
+            a[2][i] = f0.y;
+            a[3][i] = f1.y;
+        }
+#pragma unroll
+        for (int s = 0; s < 4; ++s)
+#pragma unroll
+            for (int i = 0; i < 4; ++i)
+#pragma unroll
+                for (int j = 0; j < 4; ++j)
+                    acc[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[s][i], bcur[s][j], acc[i][j], 0, 0, 0);
+        if (kb + 1 < Kb) {
+#pragma unroll
+            for (int s = 0; s < 4; ++s)
+#pragma unroll
+                for (int j = 0; j < 4; ++j) bcur[s][j] = bnext[s][j];
+        }
+    }
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            const size_t row = (size_t)mt * WN_GM + wm * 64 + 16 * i + 4 * lq + q;
+            double *o = out + row * ldo + (size_t)nt * WN_GN + wn * 64 + lrow;
+#pragma unroll
+            for (int j = 0; j < 4; ++j) o[16 * j] = acc[i][j][q];
+        }
+    }
+}
+
+hipError_t wn_launch_interp_f64(const double *bf, const double2 *tab, const int4 *ridx, double *out, int rows_pad,
+                                const WnShape &sh, hipStream_t st) {
+    const int nMt = rows_pad / WN_GM;
+    const int panels = (sh.nNt + 7) / 8;
+    const long long grid = 8LL * panels * nMt;
+    hipLaunchKernelGGL(k_interp_f64, dim3((unsigned)grid), dim3(256), 0, st, bf, tab, ridx, out, nMt, sh.nNt, sh.Kb,
+                       sh.S, sh.ldo);
+    return hipGetLastError();
+}
+
+// ------------------------------------------------------------------------------------------------
+// Jacobi eigensolver: one wave64 workgroup per k point, H and V in LDS (n <= 64).
+//
+// The upper triangle of channel 0 defines the Hermitian matrix (LAPACK uplo = 'U' semantics;
+// the imaginary part of the diagonal is ignored).  Parallel cyclic Jacobi with a round-robin
+// tournament: each round rotates n/2 disjoint (p,q) planes, first on columns (H J, V J), then on
+// rows (J^+ H).  For h_pq = b e^{i phi}: J = diag(1, e^{-i phi}) [[c, s], [-s, c]] with
+// t = sign(tau)/(|tau| + sqrt(1 + tau^2)), tau = (h_qq - h_pp)/(2b).  Sweeps stop when
+// ||offdiag(H)||_F <= tol ||H||_F.  Eigenvalues are sorted ascending and every eigenvector
+// column is rotated so that its first largest-magnitude entry is real and non-negative.
+// ------------------------------------------------------------------------------------------------
+__global__ __launch_bounds__(64) void k_eig_f64(const double *__restrict__ x, double *__restrict__ eout,
+                                               double2 *__restrict__ uout, int *__restrict__ sweeps_out, int n,
+                                               int ldo, double tol) {
+    extern __shared__ double2 sm[];
+    const int k = blockIdx.x;
+    const int t = threadIdx.x;
+    const int ld = n + 1;
+    const int np = n + (n & 1);
+    const int half = np / 2;
+    double2 *H = sm;
+    double2 *V = H + n * ld;
+    double2 *rotA = V + n * ld;   // (c, s)
+    double2 *rotB = rotA + half;  // (cos phi, sin phi)
+    int2 *pq = (int2 *)(rotB + half);
+    double *ev = (double *)(pq + half);
+    int *rank = (int *)(ev + n);
+
+    const double2 *xk = (const double2 *)(x + (size_t)k * ldo);
+    for (int e = t; e < n * n; e += 64) {
+        int i = e / n, j = e - i * n;
+        if (i <= j) {
+            double2 h = xk[i * n + j];
+            if (i == j) {
+                H[i * ld + i] = make_double2(h.x, 0.0);
+            } else {
+                H[i * ld + j] = h;
+                H[j * ld + i] = cconj(h);
+            }
+        }
+        V[i * ld + j] = make_double2(i == j ? 1.0 : 0.0, 0.0);
+    }
+    __syncthreads();
+
+    int sweep = 0;
+    for (; sweep < 40; ++sweep) {
+        double off = 0.0, dia = 0.0;
+        for (int e = t; e < n * n; e += 64) {
+            int i = e / n, j = e - i * n;
+            double2 h = H[i * ld + j];
+            double m2 = h.x * h.x + h.y * h.y;
+            if (i == j)
+                dia += m2;
+            else
+                off += m2;
+        }
+        for (int o = 32; o > 0; o >>= 1) {
+            off += __shfl_xor(off, o);
+            dia += __shfl_xor(dia, o);
+        }
+        if (off <= tol * tol * (dia + off)) break;
+        for (int r = 0; r < np - 1; ++r) {
+            if (t < half) {
+                int p = (t == 0) ? 0 : ((t - 1 + r) % (np - 1)) + 1;
+                int q = ((np - 2 - t + r) % (np - 1)) + 1;
+                if (p > q) {
+                    int tmp = p;
+                    p = q;
+                    q = tmp;
+                }
+                double c = 1.0, s = 0.0, cph = 1.0, sph = 0.0;
+                if (q < n) {
+                    double a = H[p * ld + p].x, d = H[q * ld + q].x;
+                    double2 h = H[p * ld + q];
+                    double b = hypot(h.x, h.y);
+                    if (b > 0.0) {
+                        double tau = (d - a) / (2.0 * b);
+                        double tt = (tau >= 0.0 ? 1.0 : -1.0) / (fabs(tau) + sqrt(1.0 + tau * tau));
+                        c = 1.0 / sqrt(1.0 + tt * tt);
+                        s = tt * c;
+                        cph = h.x / b;
+                        sph = h.y / b;
+                    }
+                } else {
+                    q = -1;
+                }
+                rotA[t] = make_double2(c, s);
+                rotB[t] = make_double2(cph, sph);
+                pq[t] = make_int2(p, q);
+            }
+            __syncthreads();
+            // columns: H <- H J, V <- V J
+            for (int e = t; e < n * half; e += 64) {
+                int pr = e / n, i = e - pr * n;
+                int2 ij = pq[pr];
+                if (ij.y < 0) continue;
+                const double2 ra = rotA[pr], rb = rotB[pr];
+                const double4 R = make_double4(ra.x, ra.y, rb.x, rb.y);
+                if (R.y == 0.0) continue;
+                {
+                    double2 xp = H[i * ld + ij.x], yq = H[i * ld + ij.y];
+                    // e^{-i phi} y
+                    double2 ey = make_double2(R.z * yq.x + R.w * yq.y, R.z * yq.y - R.w * yq.x);
+                    H[i * ld + ij.x] = make_double2(R.x * xp.x - R.y * ey.x, R.x * xp.y - R.y * ey.y);
+                    H[i * ld + ij.y] = make_double2(R.y * xp.x + R.x * ey.x, R.y * xp.y + R.x * ey.y);
+                }
+                {
+                    double2 xp = V[i * ld + ij.x], yq = V[i * ld + ij.y];
+                    double2 ey = make_double2(R.z * yq.x + R.w * yq.y, R.z * yq.y - R.w * yq.x);
+                    V[i * ld + ij.x] = make_double2(R.x * xp.x - R.y * ey.x, R.x * xp.y - R.y * ey.y);
+                    V[i * ld + ij.y] = make_double2(R.y * xp.x + R.x * ey.x, R.y * xp.y + R.x * ey.y);
+                }
+            }
+            __syncthreads();
+            // rows: H <- J^+ H
+            for (int e = t; e < n * half; e += 64) {
+                int pr = e / n, j = e - pr * n;
+                int2 ij = pq[pr];
+                if (ij.y < 0) continue;
+                const double2 ra = rotA[pr], rb = rotB[pr];
+                const double4 R = make_double4(ra.x, ra.y, rb.x, rb.y);
+                if (R.y == 0.0) continue;
+                double2 xp = H[ij.x * ld + j], yq = H[ij.y * ld + j];
+                // e^{i phi} y
+                double2 ey = make_double2(R.z * yq.x - R.w * yq.y, R.z * yq.y + R.w * yq.x);
+                H[ij.x * ld + j] = make_double2(R.x * xp.x - R.y * ey.x, R.x * xp.y - R.y * ey.y);
+                H[ij.y * ld + j] = make_double2(R.y * xp.x + R.x * ey.x, R.y * xp.y + R.x * ey.y);
+            }
+            __syncthreads();
+        }
+    }
+    // sort ascending (stable by index) and canonicalize columns
+    if (t < n) ev[t] = H[t * ld + t].x;
+    __syncthreads();
+    if (t < n) {
+        double et = ev[t];
+        int rk = 0;
+        for (int j = 0; j < n; ++j) {
+            double ej = ev[j];
+            rk += (ej < et) || (ej == et && j < t);
+        }
+        rank[t] = rk;
+    }
+    __syncthreads();
+    // sorted eigenvectors into H storage
+    for (int e = t; e < n * n; e += 64) {
+        int i = e / n, j = e - i * n;
+        H[i * ld + rank[j]] = V[i * ld + j];
+    }
+    if (t < n) eout[(size_t)k * n + rank[t]] = ev[t];
+    __syncthreads();
+    if (t < n) {
+        int piv = 0;
+        double pm = hypot(H[t].x, H[t].y);
+        for (int i = 1; i < n; ++i) {
+            double2 v = H[i * ld + t];
+            double m = hypot(v.x, v.y);
+            if (m > pm) {
+                pm = m;
+                piv = i;
+            }
+        }
+        if (pm > 0.0) {
+            double2 v = H[piv * ld + t];
+            double2 ph = make_double2(v.x / pm, -v.y / pm);
+            for (int i = 0; i < n; ++i) H[i * ld + t] = cmul(H[i * ld + t], ph);
+        }
+    }
+    __syncthreads();
+    double2 *uk = uout + (size_t)k * n * n;
+    for (int e = t; e < n * n; e += 64) {
+        int i = e / n, j = e - i * n;
+        uk[e] = H[i * ld + j];
+    }
+    if (t == 0 && sweeps_out) sweeps_out[k] = sweep;
+}
+
+hipError_t wn_launch_eig_f64(const double *x, double *e, double2 *u, int *sweeps, int count, const WnShape &sh,
+                             hipStream_t st) {
+    const int n = sh.n;
+    const int half = (n + 1) / 2;
+    size_t lds = (size_t)2 * n * (n + 1) * sizeof(double2) + half * (2 * sizeof(double2) + sizeof(int2)) +
+                 n * (sizeof(double) + sizeof(int));
+    hipLaunchKernelGGL(k_eig_f64, dim3(count), dim3(64), lds, st, x, e, u, sweeps, n, sh.ldo, 1.0e-15);
+    return hipGetLastError();
+}
+
+// ------------------------------------------------------------------------------------------------
+// Hamiltonian-gauge rotation X_H = U^+ X U for channels 1..C-1 (in place), one 256-thread
+// workgroup per k point; U and X staged in LDS, U^+ X kept in registers between the halves.
+// ------------------------------------------------------------------------------------------------
+#define WN_ROT_MAXO 16
+__global__ __launch_bounds__(256) void k_rotate_f64(double *__restrict__ x, const double2 *__restrict__ u, int n,
+                                                   int C, int ldo) {
+    extern __shared__ double2 sm[];
+    const int k = blockIdx.x;
+    const int t = threadIdx.x;
+    const int ld = n + 1;
+    const int n2 = n * n;
+    double2 *Us = sm;
+    double2 *Xs = Us + n * ld;
+    const double2 *uk = u + (size_t)k * n2;
+    for (int e = t; e < n2; e += 256) {
+        int i = e / n, j = e - i * n;
+        Us[i * ld + j] = uk[e];
+    }
+    double2 *xk = (double2 *)(x + (size_t)k * ldo);
+    for (int c = 1; c < C; ++c) {
+        double2 *xc = xk + (size_t)c * n2;
+        __syncthreads();
+        for (int e = t; e < n2; e += 256) {
+            int i = e / n, j = e - i * n;
+            Xs[i * ld + j] = xc[e];
+        }
+        __syncthreads();
+        double2 tv[WN_ROT_MAXO];
+#pragma unroll
+        for (int o = 0; o < WN_ROT_MAXO; ++o) {
+            const int e = t + 256 * o;
+            if (e < n2) {
+                int i = e / n, j = e - i * n;
+                double2 acc = make_double2(0.0, 0.0);
+                for (int l = 0; l < n; ++l) acc = cfma(cconj(Us[l * ld + i]), Xs[l * ld + j], acc);
+                tv[o] = acc;
+            }
+        }
+        __syncthreads();
+#pragma unroll
+        for (int o = 0; o < WN_ROT_MAXO; ++o) {
+            const int e = t + 256 * o;
+            if (e < n2) {
+                int i = e / n, j = e - i * n;
+                Xs[i * ld + j] = tv[o];
+            }
+        }
+        __syncthreads();
+        for (int e = t; e < n2; e += 256) {
+            int i = e / n, j = e - i * n;
+            double2 acc = make_double2(0.0, 0.0);
+            for (int l = 0; l < n; ++l) acc = cfma(Xs[i * ld + l], Us[l * ld + j], acc);
+            xc[e] = acc;
+        }
+    }
+}
+
+hipError_t wn_launch_rotate_f64(double *x, const double2 *u, int count, const WnShape &sh, hipStream_t st) {
+    const int n = sh.n;
+    size_t lds = (size_t)2 * n * (n + 1) * sizeof(double2);
+    hipLaunchKernelGGL(k_rotate_f64, dim3(count), dim3(256), lds, st, x, u, n, sh.C, sh.ldo);
+    return hipGetLastError();
+}
+
+// ------------------------------------------------------------------------------------------------
+// Screening, active pairs and SC/IC kernels per k point (256 threads).
+//
+// Occupations (CalculationUtils.jl:21-45), df_nm = f_n - f_m, transition energy dE = E_m - E_n.
+// Gaussian: pair active iff |df| > 1e-10 and some w_j has |dE - w_j| < sqrt(200) width; the exact
+// inclusive window [wlo, whi) of such j is stored.  Lorentzian: pairs with |df| > 1e-10, window
+// all j; the k point is active only if some |dE - w_j| < window (TransitionScreening.jl:32-94).
+// Pairs are stored n-major.  For each active (n, m) one thread forms the generalized derivatives
+// r^b_{nm;a} and r^b_{mn;a} for all 9 (a, b) and writes
+//   SC: Q[a+3b+9c] = -i (G_nm[c][a] B_mn[b] - G_mn[b][a] B_nm[c]),   B = A_H + i D
+//   IC: Q[27*sc + a+3b+9c] = (V_mm^a - V_nn^a) B_nm[c] B_mn[b].
+// ------------------------------------------------------------------------------------------------
+struct PairView {
+    const double2 *V[3];
+    const double2 *W[6];
+    const double2 *A[3];
+    const double2 *dA[9];
+    int has_pos;
+};
+
+__device__ __forceinline__ double2 ld2(const double2 *p, int idx) { return p[idx]; }
+
+__device__ void wn_gderiv(const PairView &pv, const double *E, double eta, int n, int nb, int m, double2 G[3][3]) {
+    // r^b_{nm;a} for all (a = x, b = y); ConventionalShiftCurrent.jl:191-229
+    double2 sAD[3][3], sHD[3][3];
+#pragma unroll
+    for (int x = 0; x < 3; ++x)
+#pragma unroll
+        for (int y = 0; y < 3; ++y) {
+            sAD[x][y] = make_double2(0.0, 0.0);
+            sHD[x][y] = make_double2(0.0, 0.0);
+        }
+    const double En = E[n], Em = E[m];
+    for (int l = 0; l < nb; ++l) {
+        if (l == n || l == m) continue;
+        const double El = E[l];
+        const double inl = wn_reg_inverse(En - El, eta);
+        const double ilm = wn_reg_inverse(El - Em, eta);
+        double2 Vnl[3], Vlm[3], Anl[3], Alm[3], Dnl[3], Dlm[3];
+#pragma unroll
+        for (int x = 0; x < 3; ++x) {
+            Vnl[x] = pv.V[x][n * nb + l];
+            Vlm[x] = pv.V[x][l * nb + m];
+            Dnl[x] = cscale(Vnl[x], -inl);
+            Dlm[x] = cscale(Vlm[x], -ilm);
+            if (pv.has_pos) {
+                Anl[x] = pv.A[x][n * nb + l];
+                Alm[x] = pv.A[x][l * nb + m];
+            } else {
+                Anl[x] = make_double2(0.0, 0.0);
+                Alm[x] = make_double2(0.0, 0.0);
+            }
+        }
+#pragma unroll
+        for (int x = 0; x < 3; ++x)
+#pragma unroll
+            for (int y = 0; y < 3; ++y) {
+                sAD[x][y] = csub(cfma(Anl[x], Dlm[y], sAD[x][y]), cmul(Dnl[y], Alm[x]));
+                sHD[x][y] = csub(cfma(Vnl[x], Dlm[y], sHD[x][y]), cmul(Dnl[y], Vlm[x]));
+            }
+    }
+    const double inm = wn_reg_inverse(En - Em, eta);
+    double2 Dnm[3], Ann[3], Amm[3], Anm[3], Vnn[3], Vmm[3];
+#pragma unroll
+    for (int x = 0; x < 3; ++x) {
+        double2 v = pv.V[x][n * nb + m];
+        Dnm[x] = cscale(v, -inm);
+        Vnn[x] = pv.V[x][n * nb + n];
+        Vmm[x] = pv.V[x][m * nb + m];
+        if (pv.has_pos) {
+            Ann[x] = pv.A[x][n * nb + n];
+            Amm[x] = pv.A[x][m * nb + m];
+            Anm[x] = pv.A[x][n * nb + m];
+        } else {
+            Ann[x] = Amm[x] = Anm[x] = make_double2(0.0, 0.0);
+        }
+    }
+#pragma unroll
+    for (int x = 0; x < 3; ++x)
+#pragma unroll
+        for (int y = 0; y < 3; ++y) {
+            double2 dAy = csub(Ann[y], Amm[y]), dAx = csub(Ann[x], Amm[x]);
+            double2 dVy = csub(Vnn[y], Vmm[y]), dVx = csub(Vnn[x], Vmm[x]);
+            double2 ad_bit = cadd(cmul(dAy, Dnm[x]), cmul(dAx, Dnm[y]));
+            double2 aa_bit = cmul(dAy, Anm[x]);
+            double2 dv_bit = cadd(cmul(dVy, Dnm[x]), cmul(dVx, Dnm[y]));
+            double2 da = pv.has_pos ? pv.dA[3 * x + y][n * nb + m] : make_double2(0.0, 0.0);
+            double2 w = pv.W[wn_sym(x, y)][n * nb + m];
+            double2 g = cadd(da, ad_bit);
+            g = csub(g, cmuli(aa_bit));
+            g = cadd(g, sAD[x][y]);
+            double2 h = cadd(cadd(w, sHD[x][y]), dv_bit);
+            g = cadd(g, cscale(cmuli(h), -inm));
+            G[x][y] = g;
+        }
+}
+
+__global__ __launch_bounds__(256) void k_pairs_f64(const double *__restrict__ x, const double *__restrict__ eg,
+                                                  const double *__restrict__ omega, int *__restrict__ count,
+                                                  int4 *__restrict__ pmeta, double2 *__restrict__ pmd,
+                                                  double2 *__restrict__ qout, int *__restrict__ flags, WnShape sh,
+                                                  WnScreen sc) {
+    __shared__ double E[64];
+    __shared__ double F[64];
+    __shared__ int scan[256];
+    __shared__ int kactive;
+    const int k = blockIdx.x;
+    const int t = threadIdx.x;
+    const int nb = sh.n;
+    const int n2 = nb * nb;
+    if (t < nb) E[t] = eg[(size_t)k * nb + t];
+    if (t == 0) kactive = 0;
+    __syncthreads();
+    if (t < nb) {
+        const double kt = 8.617333262e-5 * sc.temperature;
+        double f;
+        if (kt == 0.0) {
+            f = E[t] <= sc.mu ? 1.0 : 0.0;
+        } else {
+            double arg = fmin(fmax((E[t] - sc.mu) / kt, -700.0), 700.0);
+            f = 1.0 / (exp(arg) + 1.0);
+        }
+        F[t] = f;
+    }
+    __syncthreads();
+    const double cutoff = sc.width * sqrt(200.0);
+    // candidate pairs in n-major order, contiguous chunk per thread
+    const int cpt = (n2 + 255) / 256;
+    const int c0 = t * cpt;
+    int mycount = 0;
+    int myflag = 0;
+    for (int c = c0; c < c0 + cpt && c < n2; ++c) {
+        int n = c / nb, m = c - n * nb;
+        double df = F[n] - F[m];
+        if (!(fabs(df) > 1.0e-10)) continue;
+        double de = E[m] - E[n];
+        if (sc.gaussian) {
+            // first j with w_j > de - cutoff
+            int lo = 0, hi = sc.nw;
+            while (lo < hi) {
+                int mid = (lo + hi) >> 1;
+                if (omega[mid] > de - cutoff) hi = mid;
+                else lo = mid + 1;
+            }
+            int j0 = lo > 0 ? lo - 1 : 0;
+            int first = -1, last = -1;
+            for (int j = j0; j < sc.nw; ++j) {
+                double xx = de - omega[j];
+                if (fabs(xx) < cutoff) {
+                    if (first < 0) first = j;
+                    last = j;
+                } else if (omega[j] > de) {
+                    break;
+                }
+            }
+            if (first >= 0) {
+                ++mycount;
+                myflag = 1;
+            }
+        } else {
+            ++mycount;
+            if (sc.window > 0.0) {
+                for (int j = 0; j < sc.nw; ++j)
+                    if (fabs(de - omega[j]) < sc.window) {
+                        myflag = 1;
+                        break;
+                    }
+            } else {
+                myflag = 1;
+            }
+        }
+    }
+    scan[t] = mycount;
+    if (myflag) atomicOr(&kactive, 1);
+    __syncthreads();
+    for (int o = 1; o < 256; o <<= 1) {
+        int v = t >= o ? scan[t - o] : 0;
+        __syncthreads();
+        scan[t] += v;
+        __syncthreads();
+    }
+    const int total = scan[255];
+    const int active = kactive;
+    int slot = scan[t] - mycount;
+    const size_t pbase = (size_t)k * n2;
+    if (active) {
+        for (int c = c0; c < c0 + cpt && c < n2; ++c) {
+            int n = c / nb, m = c - n * nb;
+            double df = F[n] - F[m];
+            if (!(fabs(df) > 1.0e-10)) continue;
+            double de = E[m] - E[n];
+            int first = 0, last = sc.nw - 1;
+            if (sc.gaussian) {
+                int lo = 0, hi = sc.nw;
+                while (lo < hi) {
+                    int mid = (lo + hi) >> 1;
+                    if (omega[mid] > de - cutoff) hi = mid;
+                    else lo = mid + 1;
+                }
+                int j0 = lo > 0 ? lo - 1 : 0;
+                first = -1;
+                last = -1;
+                for (int j = j0; j < sc.nw; ++j) {
+                    double xx = de - omega[j];
+                    if (fabs(xx) < cutoff) {
+                        if (first < 0) first = j;
+                        last = j;
+                    } else if (omega[j] > de) {
+                        break;
+                    }
+                }
+                if (first < 0) continue;
+            }
+            pmeta[pbase + slot] = make_int4(n, m, first, last + 1);
+            pmd[pbase + slot] = make_double2(de, df);
+            ++slot;
+        }
+    }
+    if (t == 0) count[k] = active ? total : 0;
+    __syncthreads();
+    if (!active) return;
+
+    // eta == 0: bare inverse requires isolated levels (OrthogonalBandDerivatives.jl:66-82)
+    if (sc.eta == 0.0 && t < nb) {
+        for (int j = t + 1; j < nb; ++j)
+            if (!(fabs(E[j] - E[t]) > 1.0e-10)) atomicOr(flags, 1);
+    }
+
+    const double2 *xk = (const double2 *)(x + (size_t)k * sh.ldo);
+    PairView pv;
+    pv.has_pos = sh.has_pos;
+#pragma unroll
+    for (int a = 0; a < 3; ++a) pv.V[a] = xk + (size_t)(WN_CH_DH + a) * n2;
+#pragma unroll
+    for (int a = 0; a < 6; ++a) pv.W[a] = xk + (size_t)(WN_CH_DDH + a) * n2;
+#pragma unroll
+    for (int a = 0; a < 3; ++a) pv.A[a] = sh.has_pos ? xk + (size_t)(WN_CH_A + a) * n2 : nullptr;
+#pragma unroll
+    for (int a = 0; a < 9; ++a) pv.dA[a] = sh.has_pos ? xk + (size_t)(WN_CH_DA + a) * n2 : nullptr;
+    const int ncomp = 27 * sc.nq;
+    for (int p = t; p < total; p += 256) {
+        const int4 mt = pmeta[pbase + p];
+        const int n = mt.x, m = mt.y;
+        double2 *q = qout + (pbase + p) * ncomp;
+        const double inm = wn_reg_inverse(E[n] - E[m], sc.eta);
+        const double imn = wn_reg_inverse(E[m] - E[n], sc.eta);
+        double2 Bnm[3], Bmn[3];
+#pragma unroll
+        for (int a = 0; a < 3; ++a) {
+            double2 anm = sh.has_pos ? pv.A[a][n * nb + m] : make_double2(0.0, 0.0);
+            double2 amn = sh.has_pos ? pv.A[a][m * nb + n] : make_double2(0.0, 0.0);
+            double2 dnm = cscale(pv.V[a][n * nb + m], -inm);
+            double2 dmn = cscale(pv.V[a][m * nb + n], -imn);
+            Bnm[a] = cadd(anm, cmuli(dnm));
+            Bmn[a] = cadd(amn, cmuli(dmn));
+        }
+        int off = 0;
+        if (sc.sc) {
+            double2 Gnm[3][3], Gmn[3][3];
+            wn_gderiv(pv, E, sc.eta, n, nb, m, Gnm);
+            wn_gderiv(pv, E, sc.eta, m, nb, n, Gmn);
+            for (int c = 0; c < 3; ++c)
+                for (int b = 0; b < 3; ++b)
+                    for (int a = 0; a < 3; ++a) {
+                        double2 z = csub(cmul(Gnm[c][a], Bmn[b]), cmul(Gmn[b][a], Bnm[c]));
+                        q[a + 3 * b + 9 * c] = make_double2(z.y, -z.x);  // -i z
+                    }
+            off = 27;
+        }
+        if (sc.ic) {
+            for (int c = 0; c < 3; ++c)
+                for (int b = 0; b < 3; ++b)
+                    for (int a = 0; a < 3; ++a) {
+                        double2 vd = csub(pv.V[a][m * nb + m], pv.V[a][n * nb + n]);
+                        q[off + a + 3 * b + 9 * c] = cmul(cmul(vd, Bnm[c]), Bmn[b]);
+                    }
+        }
+    }
+}
+
+hipError_t wn_launch_pairs_f64(const double *x, const double *e, const double *omega, int *count, int4 *pmeta,
+                               double2 *pmd, double2 *q, int *flags, int nk, const WnShape &sh, const WnScreen &sc,
+                               hipStream_t st) {
+    hipLaunchKernelGGL(k_pairs_f64, dim3(nk), dim3(256), 0, st, x, e, omega, count, pmeta, pmd, q, flags, sh, sc);
+    return hipGetLastError();
+}
+
+// ------------------------------------------------------------------------------------------------
+// Frequency contraction per chunk of k points.  Workgroup (chunk, 64-frequency tile): wave w
+// owns components [14w, 14w+14), lane = frequency.  For every pair whose window meets the
+// tile, weight = df * delta(dE - w) (exact Gaussian cutoff test per frequency) and
+// acc[c] += weight * Q[c]; the per-chunk sums go to partial[chunk][component][frequency].
+// ------------------------------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void k_accum(const int4 *__restrict__ chunks, const int *__restrict__ count,
+                                              const int4 *__restrict__ pmeta, const double2 *__restrict__ pmd,
+                                              const double2 *__restrict__ q, const double *__restrict__ omega,
+                                              double2 *__restrict__ partial, int pcap, int ncomp, int nw, int nwt,
+                                              int gaussian, double width) {
+    const int chunk = blockIdx.x / nwt;
+    const int wt = blockIdx.x - chunk * nwt;
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int w = wt * WN_WT + lane;
+    const double om = w < nw ? omega[w] : 0.0;
+    const int cbase = wave * 14;
+    const double cutoff = width * sqrt(200.0);
+    const double gscale = 1.0 / (sqrt(M_PI) * width);
+    const double lscale = width / M_PI;
+    const int tlo = wt * WN_WT, thi = tlo + WN_WT;
+    double2 acc[14];
+#pragma unroll
+    for (int c = 0; c < 14; ++c) acc[c] = make_double2(0.0, 0.0);
+    const int4 ch = chunks[chunk];
+    for (int k = ch.x; k < ch.x + ch.y; ++k) {
+        const int cnt = count[k];
+        const size_t base = (size_t)k * pcap;
+        for (int s = 0; s < cnt; ++s) {
+            const int4 mt = pmeta[base + s];
+            if (mt.w <= tlo || mt.z >= thi) continue;
+            const double2 md = pmd[base + s];
+            const double xx = md.x - om;
+            double dl;
+            if (gaussian) {
+                dl = fabs(xx) < cutoff ? gscale * exp(-(xx / width) * (xx / width)) : 0.0;
+            } else {
+                dl = lscale / (xx * xx + width * width);
+            }
+            const double wgt = w < nw ? md.y * dl : 0.0;
+            const double2 *qq = q + (base + s) * ncomp + cbase;
+#pragma unroll
+            for (int c = 0; c < 14; ++c) {
+                if (cbase + c < ncomp) {
+                    double2 v = qq[c];
+                    acc[c].x += wgt * v.x;
+                    acc[c].y += wgt * v.y;
+                }
+            }
+        }
+    }
+    if (w < nw) {
+        double2 *o = partial + (size_t)chunk * ncomp * nw;
+#pragma unroll
+        for (int c = 0; c < 14; ++c)
+            if (cbase + c < ncomp) o[(size_t)(cbase + c) * nw + w] = acc[c];
+    }
+}
+
+hipError_t wn_launch_accum(const int4 *chunks, int nchunks, const int *count, const int4 *pmeta, const double2 *pmd,
+                           const double2 *q, const double *omega, double2 *partial, const WnShape &sh,
+                           const WnScreen &sc, hipStream_t st) {
+    const int nwt = (sc.nw + WN_WT - 1) / WN_WT;
+    const int ncomp = 27 * sc.nq;
+    hipLaunchKernelGGL(k_accum, dim3(nchunks * nwt), dim3(256), 0, st, chunks, count, pmeta, pmd, q, omega, partial,
+                       sh.n * sh.n, ncomp, sc.nw, nwt, sc.gaussian, sc.width);
+    return hipGetLastError();
+}
+
+// ------------------------------------------------------------------------------------------------
+// Chunk partials -> lanes in fixed chunk order (Kahan-compensated when lanecomp != nullptr).
+// lanes[i] = (lane, chunk_begin, chunk_end, -).
+// ------------------------------------------------------------------------------------------------
+__global__ void k_lane_reduce(const int4 *__restrict__ lanes, const double2 *__restrict__ partial,
+                              double2 *__restrict__ lanebuf, double2 *__restrict__ lanecomp, int nelem) {
+    const int e = blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= nelem) return;
+    const int4 li = lanes[blockIdx.y];
+    const size_t o = (size_t)li.x * nelem + e;
+    double2 acc = lanebuf[o];
+    if (lanecomp) {
+        double2 comp = lanecomp[o];
+        for (int c = li.y; c < li.z; ++c) {
+            double2 v = partial[(size_t)c * nelem + e];
+            double yx = v.x - comp.x, yy = v.y - comp.y;
+            double tx = acc.x + yx, ty = acc.y + yy;
+            comp.x = (tx - acc.x) - yx;
+            comp.y = (ty - acc.y) - yy;
+            acc.x = tx;
+            acc.y = ty;
+        }
+        lanecomp[o] = comp;
+    } else {
+        for (int c = li.y; c < li.z; ++c) {
+            double2 v = partial[(size_t)c * nelem + e];
+            acc.x += v.x;
+            acc.y += v.y;
+        }
+    }
+    lanebuf[o] = acc;
+}
+
+hipError_t wn_launch_lane_reduce(const int4 *lanes, int nlanes, const double2 *partial, double2 *lanebuf,
+                                 double2 *lanecomp, int nelem, hipStream_t st) {
+    if (nlanes == 0) return hipSuccess;
+    hipLaunchKernelGGL(k_lane_reduce, dim3((nelem + 255) / 256, nlanes), dim3(256), 0, st, lanes, partial, lanebuf,
+                       lanecomp, nelem);
+    return hipGetLastError();
+}

@@ -1,0 +1,229 @@
+"""ctypes binding of ``libwnlqg_hip.so`` (include/wnlqg.h).
+
+The binding is the host-side mirror a Julia ``ccall`` shim would be: it marshals a
+``TightBindingModel`` into the reference's column-major layouts and a resolved task into
+``wnlqg_task``.  Loading fails loudly if the library is missing; there is no CPU fallback.
+"""
+
+from __future__ import annotations
+
+import ctypes as C
+import os
+from dataclasses import dataclass
+from typing import Optional, Sequence
+
+import numpy as np
+
+from .model import TightBindingModel
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libwnlqg_hip.so")
+
+WNLQG_SHIFT_CURRENT = 1
+WNLQG_INJECTION_CURRENT = 2
+WNLQG_GAUSSIAN = 0
+WNLQG_LORENTZIAN = 1
+WNLQG_PRECISION_FP64 = 0
+WNLQG_PRECISION_MIXED = 1
+
+ERRORS = {-1: "WNLQG_E_INVALID", -2: "WNLQG_E_NONFINITE", -3: "WNLQG_E_GAP", -4: "WNLQG_E_HIP",
+          -5: "WNLQG_E_OOM", -6: "WNLQG_E_UNSUPPORTED"}
+
+EXPORTED = ["wnlqg_version", "wnlqg_last_error", "wnlqg_create", "wnlqg_destroy", "wnlqg_num_states",
+            "wnlqg_num_channels", "wnlqg_run_mesh", "wnlqg_eval_points", "wnlqg_fourier_points",
+            "wnlqg_spectrum_points", "wnlqg_synchronize"]
+
+
+class WnlqgError(RuntimeError):
+    def __init__(self, code: int, message: str):
+        super().__init__(f"{ERRORS.get(code, code)}: {message}")
+        self.code = code
+
+
+class _Model(C.Structure):
+    _fields_ = [("num_orbitals", C.c_int32), ("num_r_vectors", C.c_int32),
+                ("lattice", C.POINTER(C.c_double)), ("r_vectors", C.POINTER(C.c_int32)),
+                ("r_degeneracies", C.POINTER(C.c_int32)), ("hamiltonian_r", C.POINTER(C.c_double)),
+                ("position_r", C.POINTER(C.c_double))]
+
+
+class _Task(C.Structure):
+    _fields_ = [("quantities", C.c_int32), ("num_photon_energies", C.c_int32),
+                ("photon_energies", C.POINTER(C.c_double)), ("broadening_type", C.c_int32),
+                ("broadening", C.c_double), ("fermi_energy", C.c_double), ("temperature", C.c_double),
+                ("denominator_regularization", C.c_double), ("transition_window_factor", C.c_double),
+                ("band_window_size", C.c_int32), ("precision", C.c_int32), ("batch_size", C.c_int32),
+                ("reserved", C.c_int32)]
+
+
+class Stats(C.Structure):
+    _fields_ = [("total_ms", C.c_double), ("interp_ms", C.c_double), ("eig_ms", C.c_double),
+                ("rotate_ms", C.c_double), ("pairs_ms", C.c_double), ("accum_ms", C.c_double),
+                ("kpoints", C.c_int64), ("active_pairs", C.c_int64), ("batches", C.c_int32),
+                ("interp_launches", C.c_int32), ("interp_flops", C.c_double)]
+
+    def as_dict(self) -> dict:
+        return {name: getattr(self, name) for name, _ in self._fields_}
+
+
+_lib = None
+
+
+def load_library(path: str = LIB_PATH) -> C.CDLL:
+    """Load the HIP library; raises if it has not been built."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(path):
+        raise RuntimeError(f"{path} is missing: build it with `python -m wanniernlqg_amd.build` "
+                           "(the engine has no CPU fallback)")
+    lib = C.CDLL(path)
+    dp = C.POINTER(C.c_double)
+    lib.wnlqg_version.restype = C.c_int32
+    lib.wnlqg_last_error.argtypes = [C.c_char_p, C.c_size_t]
+    lib.wnlqg_last_error.restype = C.c_int32
+    lib.wnlqg_create.argtypes = [C.POINTER(_Model), C.POINTER(_Task), C.POINTER(C.c_void_p)]
+    lib.wnlqg_destroy.argtypes = [C.c_void_p]
+    lib.wnlqg_num_states.argtypes = [C.c_void_p]
+    lib.wnlqg_num_channels.argtypes = [C.c_void_p]
+    lib.wnlqg_run_mesh.argtypes = [C.c_void_p, C.POINTER(C.c_int32), C.c_int32, dp, dp, C.POINTER(Stats)]
+    lib.wnlqg_eval_points.argtypes = [C.c_void_p, dp, C.c_int32, dp]
+    lib.wnlqg_fourier_points.argtypes = [C.c_void_p, dp, C.c_int32, dp]
+    lib.wnlqg_spectrum_points.argtypes = [C.c_void_p, dp, C.c_int32, dp, dp]
+    lib.wnlqg_synchronize.argtypes = [C.c_void_p]
+    for name in EXPORTED:
+        getattr(lib, name)
+        if name not in ("wnlqg_last_error",):
+            getattr(lib, name).restype = C.c_int32
+    _lib = lib
+    return lib
+
+
+def _check(lib, rc: int):
+    if rc != 0:
+        buf = C.create_string_buffer(2048)
+        lib.wnlqg_last_error(buf, 2048)
+        raise WnlqgError(rc, buf.value.decode(errors="replace"))
+
+
+def _ptr(a: np.ndarray, ctype=C.c_double):
+    return a.ctypes.data_as(C.POINTER(ctype))
+
+
+@dataclass
+class EngineTask:
+    """Resolved task parameters of the Integral path (subset of ``EffectiveTaskConfig``,
+    reference ``src/Runtime/Config.jl:4-170``; defaults from ``Config.jl:126-155``)."""
+    photon_energies: np.ndarray
+    shift_current: bool = True
+    injection_current: bool = False
+    broadening: float = 0.040
+    broadening_type: str = "Gaussian"
+    fermi_energy: float = 0.0
+    temperature: float = 0.0
+    denominator_regularization: float = 0.001
+    transition_window_factor: float = 5.0
+    band_window_size: int = -1
+    precision: str = "fp64"
+    batch_size: int = 0
+
+
+class Engine:
+    """One model + one resolved task on the current HIP device."""
+
+    def __init__(self, model: TightBindingModel, task: EngineTask):
+        self.lib = load_library()
+        self.model = model
+        self.task = task
+        n, nr = model.num_orbitals, model.num_r_vectors
+        # Julia column-major layouts
+        self._lat = np.ascontiguousarray(model.lattice.T).ravel()                     # lattice[i,a] at i+3a
+        self._rv = np.ascontiguousarray(model.r_vectors, dtype=np.int32).ravel()     # [3,N_R]
+        self._deg = np.ascontiguousarray(model.r_degeneracies, dtype=np.int32)
+        self._hr = np.ascontiguousarray(np.transpose(model.hamiltonian_r, (0, 2, 1))).view(np.float64).ravel()
+        self._ar = None
+        if model.position_r is not None:
+            self._ar = np.ascontiguousarray(np.transpose(model.position_r, (0, 1, 3, 2))).view(np.float64).ravel()
+        self._om = np.ascontiguousarray(task.photon_energies, dtype=np.float64)
+        m = _Model(n, nr, _ptr(self._lat), _ptr(self._rv, C.c_int32), _ptr(self._deg, C.c_int32), _ptr(self._hr),
+                   _ptr(self._ar) if self._ar is not None else C.POINTER(C.c_double)())
+        q = (WNLQG_SHIFT_CURRENT if task.shift_current else 0) | (WNLQG_INJECTION_CURRENT if task.injection_current else 0)
+        bt = {"Gaussian": WNLQG_GAUSSIAN, "Lorentzian": WNLQG_LORENTZIAN}.get(task.broadening_type, 99)
+        prec = {"fp64": WNLQG_PRECISION_FP64, "mixed": WNLQG_PRECISION_MIXED}[task.precision]
+        t = _Task(q, self._om.shape[0], _ptr(self._om), bt, task.broadening, task.fermi_energy, task.temperature,
+                  task.denominator_regularization, task.transition_window_factor, task.band_window_size, prec,
+                  task.batch_size, 0)
+        h = C.c_void_p()
+        _check(self.lib, self.lib.wnlqg_create(C.byref(m), C.byref(t), C.byref(h)))
+        self.handle = h
+        self.num_states = self.lib.wnlqg_num_states(h)
+        self.num_channels = self.lib.wnlqg_num_channels(h)
+        self.states = [s for s, on in (("sc", task.shift_current), ("ic", task.injection_current)) if on]
+
+    def close(self):
+        if getattr(self, "handle", None):
+            self.lib.wnlqg_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    # -- entry points ---------------------------------------------------------------------
+    def run_mesh(self, mesh: Sequence[int], n_lanes: Optional[int] = None):
+        """Integrate the Gamma-centred mesh.  Returns (lanes, total, stats) with
+        lanes[state] of shape [n_lanes, n_w, 3, 3, 3] and total[state] [n_w, 3, 3, 3] ([a, b, c] last)."""
+        mesh_a = np.ascontiguousarray(mesh, dtype=np.int32)
+        nk = int(np.prod(mesh_a.astype(np.int64)))
+        nl = min(nk, 64) if n_lanes is None else int(n_lanes)
+        nw = self._om.shape[0]
+        lane_out = np.zeros(nl * self.num_states * 27 * nw * 2)
+        total_out = np.zeros(self.num_states * 27 * nw * 2)
+        st = Stats()
+        _check(self.lib, self.lib.wnlqg_run_mesh(self.handle, _ptr(mesh_a, C.c_int32), nl, _ptr(lane_out),
+                                                 _ptr(total_out), C.byref(st)))
+        lanes = _unpack(lane_out, (nl,), self.num_states, nw)
+        total = _unpack(total_out, (), self.num_states, nw)
+        return ({s: lanes[..., i, :, :, :, :] for i, s in enumerate(self.states)},
+                {s: total[i] for i, s in enumerate(self.states)}, st.as_dict())
+
+    def eval_points(self, kpoints: np.ndarray):
+        k = np.ascontiguousarray(kpoints, dtype=np.float64).reshape(-1, 3)
+        nw = self._om.shape[0]
+        out = np.zeros(k.shape[0] * self.num_states * 27 * nw * 2)
+        _check(self.lib, self.lib.wnlqg_eval_points(self.handle, _ptr(k), k.shape[0], _ptr(out)))
+        vals = _unpack(out, (k.shape[0],), self.num_states, nw)
+        return {s: vals[:, i] for i, s in enumerate(self.states)}
+
+    def fourier_points(self, kpoints: np.ndarray) -> np.ndarray:
+        k = np.ascontiguousarray(kpoints, dtype=np.float64).reshape(-1, 3)
+        n = self.model.num_orbitals
+        out = np.zeros(k.shape[0] * self.num_channels * n * n * 2)
+        _check(self.lib, self.lib.wnlqg_fourier_points(self.handle, _ptr(k), k.shape[0], _ptr(out)))
+        return out.view(np.complex128).reshape(k.shape[0], self.num_channels, n, n)
+
+    def spectrum_points(self, kpoints: np.ndarray):
+        k = np.ascontiguousarray(kpoints, dtype=np.float64).reshape(-1, 3)
+        n = self.model.num_orbitals
+        e = np.zeros(k.shape[0] * n)
+        v = np.zeros(k.shape[0] * n * n * 2)
+        _check(self.lib, self.lib.wnlqg_spectrum_points(self.handle, _ptr(k), k.shape[0], _ptr(e), _ptr(v)))
+        return e.reshape(k.shape[0], n), v.view(np.complex128).reshape(k.shape[0], n, n)
+
+    def synchronize(self):
+        _check(self.lib, self.lib.wnlqg_synchronize(self.handle))
+
+
+def _unpack(flat: np.ndarray, lead: tuple, nq: int, nw: int) -> np.ndarray:
+    """Complex [..., state, (n_w,3,3,3 column-major)] -> [..., state, n_w, a, b, c]."""
+    z = flat.view(np.complex128).reshape(lead + (nq, 3, 3, 3, nw))   # c, b, a, w (C order of Julia [w,a,b,c])
+    return np.transpose(z, tuple(range(len(lead))) + (len(lead), len(lead) + 4, len(lead) + 3, len(lead) + 2,
+                                                       len(lead) + 1))
