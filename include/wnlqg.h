@@ -49,7 +49,8 @@ extern "C" {
 
 /* Arithmetic forms. */
 #define WNLQG_PRECISION_FP64 0  /* FP64 everywhere (plain GPU path) */
-#define WNLQG_PRECISION_MIXED 1 /* split-BF16 MFMA interpolation, FP64 denominators/k-sum */
+#define WNLQG_PRECISION_MIXED 1 /* split-BF16 (3 products) MFMA interpolation, FP64 denominators, Kahan k-sum */
+#define WNLQG_PRECISION_MIXED_HIGH 2 /* split-BF16 with 6 products (~FP32 interpolation accuracy) */
 
 /*
  * Tight-binding model (TightBindingModel, reference src/Core/Models.jl:11-75; the fields of

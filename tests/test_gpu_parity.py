@@ -182,3 +182,33 @@ def test_mesh_lane_counts_and_batches():
         assert rel(t64[key], ot[key]) < 1e-10
         assert rel(t7[key], ot[key]) < 1e-10
         assert rel(l64[key], ol[key]) < 1e-10
+
+
+@pytest.mark.parametrize("prec,tol", [("mixed", 2e-4), ("mixed6", 2e-6)])
+def test_mixed_fourier_channels(prec, tol):
+    """Split-BF16 interpolation: per-channel error relative to the channel's largest entry."""
+    model = MODELS["n32_pos"]()
+    kp = np.random.default_rng(5).uniform(-1, 1, (64, 3))
+    say(f"[fourier-{prec}] n32")
+    with engine(model, precision=prec) as eng:
+        got = eng.fourier_points(kp)
+    exp = channels_oracle(model, kp)
+    scale = np.max(np.abs(exp), axis=(2, 3), keepdims=True) + 1e-300
+    err = np.max(np.abs(got - exp) / scale)
+    say(f"  max rel err {err:.3e}")
+    assert err < tol
+
+
+@pytest.mark.parametrize("prec,tol", [("mixed", 1e-3), ("mixed6", 1e-5)])
+def test_mixed_mesh_vs_fp64(prec, tol):
+    model = MODELS["n12_pos"]()
+    kw = dict(photon_energies=np.linspace(0, 5, 100), broadening=0.05)
+    say(f"[mesh-{prec}] n12 8x8x8")
+    with engine(model, **kw) as e64:
+        _, t64, _ = e64.run_mesh((8, 8, 8))
+    with engine(model, precision=prec, **kw) as emx:
+        _, tmx, _ = emx.run_mesh((8, 8, 8))
+    for key in ("sc", "ic"):
+        r = kloop.spectrum_relative_error(tmx[key], t64[key]).max()
+        say(f"  {key} worst component rel {r:.3e}")
+        assert r < tol

@@ -4,6 +4,10 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+// MFMA lane layouts used here (measured on gfx950):
+//   v_mfma_f64_16x16x4f64   A: lane l -> A[l%16][l/16];  B: B[l/16][l%16];  D reg q -> D[l/16 + 4q][l%16]
+//   v_mfma_f32_32x32x16_bf16 A: A[l%32][8(l/32)+j]; B: B[8(l/32)+j][l%32];
+//                            D reg q -> D[8(q/4) + 4(l/32) + q%4][l%32]
 typedef double f64x4 __attribute__((ext_vector_type(4)));
 typedef float f32x4 __attribute__((ext_vector_type(4)));
 typedef float f32x16 __attribute__((ext_vector_type(16)));

@@ -90,6 +90,9 @@ struct wnlqg_engine {
     DevBuf<int> deg, sweeps, count, flags;
     DevBuf<int4> ridx, pmeta, chunks, lanes;
     DevBuf<double2> tab, u, pmd, q, partial, lanebuf, lanecomp;
+    DevBuf<float2> tab32;
+    DevBuf<__bf16> bh, bm, bl;  // split-BF16 coefficient parts (mixed forms)
+    int nterms = 0;             // 0 = FP64 interpolation, 3 or 6 = split-BF16 products
     int partial_cap = 0;  // chunks the partial buffer holds
     std::vector<double> omega_host;
     hipEvent_t ev[8] = {};
@@ -119,7 +122,8 @@ int check_task(const wnlqg_task *t, int n) {
     if (t->band_window_size != -1 && 2 * t->band_window_size <= n)
         return fail(WNLQG_E_UNSUPPORTED, "band_window_size %d selects a partial window; only all bands are implemented",
                     t->band_window_size);
-    if (t->precision != WNLQG_PRECISION_FP64 && t->precision != WNLQG_PRECISION_MIXED)
+    if (t->precision != WNLQG_PRECISION_FP64 && t->precision != WNLQG_PRECISION_MIXED &&
+        t->precision != WNLQG_PRECISION_MIXED_HIGH)
         return fail(WNLQG_E_INVALID, "unknown precision %d", t->precision);
     if (t->batch_size < 0) return fail(WNLQG_E_INVALID, "batch_size must be >= 0");
     return WNLQG_OK;
@@ -132,6 +136,7 @@ int ensure_rows(wnlqg_engine *g, int rows) {
     WN_HIP(g->x.alloc((size_t)rows * g->sh.ldo));
     WN_HIP(g->kp.alloc((size_t)rows * 3));
     WN_HIP(g->tab.alloc((size_t)rows * 3 * g->sh.S));
+    if (g->nterms) WN_HIP(g->tab32.alloc((size_t)rows * 3 * g->sh.S));
     WN_HIP(g->e.alloc((size_t)rows * g->sh.n));
     WN_HIP(g->u.alloc((size_t)rows * n2));
     WN_HIP(g->sweeps.alloc((size_t)rows));
@@ -151,10 +156,22 @@ int ensure_partial(wnlqg_engine *g, int nchunks) {
     return WNLQG_OK;
 }
 
+hipError_t launch_phases(wnlqg_engine *g, int rows) {
+    return wn_launch_phase_tables(g->kp.p, g->rmin, g->tab.p, g->nterms ? g->tab32.p : nullptr, rows, g->sh.S,
+                                  g->stream);
+}
+
+hipError_t launch_interp(wnlqg_engine *g, int rows) {
+    if (g->nterms)
+        return wn_launch_interp_bf16(g->bh.p, g->bm.p, g->bl.p, g->tab32.p, g->ridx.p, g->x.p, rows, g->nterms, g->sh,
+                                     g->stream);
+    return wn_launch_interp_f64(g->bf.p, g->tab.p, g->ridx.p, g->x.p, rows, g->sh, g->stream);
+}
+
 // Interpolation + eigensystem for `count` k points already in g->kp (rows padded to 128).
 int run_front(wnlqg_engine *g, int count, int rows, bool eig) {
-    WN_HIP(wn_launch_phase_tables(g->kp.p, g->rmin, g->tab.p, rows, g->sh.S, g->stream));
-    WN_HIP(wn_launch_interp_f64(g->bf.p, g->tab.p, g->ridx.p, g->x.p, rows, g->sh, g->stream));
+    WN_HIP(launch_phases(g, rows));
+    WN_HIP(launch_interp(g, rows));
     if (eig) WN_HIP(wn_launch_eig_f64(g->x.p, g->e.p, g->u.p, g->sweeps.p, count, g->sh, g->stream));
     return WNLQG_OK;
 }
@@ -201,7 +218,8 @@ int32_t wnlqg_destroy(wnlqg_engine *g) {
     g->e.release(); g->omega.release(); g->deg.release(); g->sweeps.release(); g->count.release();
     g->flags.release(); g->ridx.release(); g->pmeta.release(); g->chunks.release(); g->lanes.release();
     g->tab.release(); g->u.release(); g->pmd.release(); g->q.release(); g->partial.release();
-    g->lanebuf.release(); g->lanecomp.release();
+    g->lanebuf.release(); g->lanecomp.release(); g->tab32.release(); g->bh.release(); g->bm.release();
+    g->bl.release();
     if (g->stream) (void)hipStreamDestroy(g->stream);
     delete g;
     return WNLQG_OK;
@@ -265,7 +283,9 @@ int32_t wnlqg_create(const wnlqg_model *model, const wnlqg_task *task, wnlqg_eng
     g->sh.has_pos = model->position_r != nullptr;
     g->sh.C = g->sh.has_pos ? WN_CH_FULL : WN_CH_BASE;
     g->sh.Kb = (nr + WN_RB - 1) / WN_RB;
-    g->sh.nNt = (int)((2 * (size_t)g->sh.C * n2 + WN_GN - 1) / WN_GN);
+    // columns padded to 256 so that both interpolation kernels tile the same output rows
+    g->sh.nNt = (int)((2 * (size_t)g->sh.C * n2 + 255) / 256) * 2;
+    g->nterms = task->precision == WNLQG_PRECISION_MIXED ? 3 : (task->precision == WNLQG_PRECISION_MIXED_HIGH ? 6 : 0);
     g->sh.ldo = g->sh.nNt * WN_GN;
     int rmax[3];
     for (int a = 0; a < 3; ++a) {
@@ -291,7 +311,9 @@ int32_t wnlqg_create(const wnlqg_model *model, const wnlqg_task *task, wnlqg_eng
     }
     if (g->hr.alloc(2 * n2 * nr) || g->rc.alloc(3 * (size_t)nr) || g->deg.alloc(nr) ||
         g->ridx.alloc(rid.size()) || g->omega.alloc(g->sc.nw) || g->flags.alloc(4) ||
-        g->bf.alloc((size_t)g->sh.Kb * g->sh.nNt * 2048) ||
+        (!g->nterms && g->bf.alloc((size_t)g->sh.Kb * g->sh.nNt * 2048)) ||
+        (g->nterms && (g->bh.alloc((size_t)g->sh.Kb * 16 * g->sh.ldo) || g->bm.alloc((size_t)g->sh.Kb * 16 * g->sh.ldo))) ||
+        (g->nterms == 6 && g->bl.alloc((size_t)g->sh.Kb * 16 * g->sh.ldo)) ||
         (g->sh.has_pos && g->ar.alloc(2 * n2 * 3 * nr)))
         return bail(fail(WNLQG_E_OOM, "device allocation for the model failed"));
     auto up = [&](void *dst, const void *src, size_t bytes) {
@@ -304,8 +326,10 @@ int32_t wnlqg_create(const wnlqg_model *model, const wnlqg_task *task, wnlqg_eng
         !up(g->omega.p, g->omega_host.data(), g->sc.nw * sizeof(double)) ||
         (g->sh.has_pos && !up(g->ar.p, model->position_r, 2 * n2 * 3 * nr * sizeof(double))))
         return bail(fail(WNLQG_E_HIP, "model upload failed"));
-    if (wn_launch_build_coeff(g->hr.p, g->ar.p, g->rc.p, g->deg.p, g->bf.p, g->sh, g->stream) != hipSuccess)
-        return bail(fail(WNLQG_E_HIP, "coefficient build launch failed"));
+    hipError_t be = g->nterms ? wn_launch_build_coeff_bf16(g->hr.p, g->ar.p, g->rc.p, g->deg.p, g->bh.p, g->bm.p,
+                                                           g->bl.p, g->sh, g->stream)
+                              : wn_launch_build_coeff(g->hr.p, g->ar.p, g->rc.p, g->deg.p, g->bf.p, g->sh, g->stream);
+    if (be != hipSuccess) return bail(fail(WNLQG_E_HIP, "coefficient build launch failed"));
     if (hipStreamSynchronize(g->stream) != hipSuccess) return bail(fail(WNLQG_E_HIP, "coefficient build failed"));
 
     // batch size: interpolation rows + pair buffers within ~24 GB unless the caller fixed it
@@ -469,7 +493,7 @@ int32_t wnlqg_run_mesh(wnlqg_engine *g, const int32_t mesh[3], int32_t n_lanes, 
     WN_HIP(hipMemsetAsync(g->lanebuf.p, 0, (size_t)n_lanes * nelem * sizeof(double2), g->stream));
     WN_HIP(hipMemsetAsync(g->lanecomp.p, 0, (size_t)n_lanes * nelem * sizeof(double2), g->stream));
     WN_HIP(hipMemsetAsync(g->flags.p, 0, sizeof(int), g->stream));
-    double2 *comp = g->precision == WNLQG_PRECISION_MIXED ? g->lanecomp.p : nullptr;
+    double2 *comp = g->nterms ? g->lanecomp.p : nullptr;
 
     std::vector<hipEvent_t> evs;
     auto mark = [&]() -> hipEvent_t {
@@ -487,9 +511,9 @@ int32_t wnlqg_run_mesh(wnlqg_engine *g, const int32_t mesh[3], int32_t n_lanes, 
         auto &mk = marks[bi];
         mk[0] = mark();
         WN_HIP(wn_launch_mesh_kpoints(g->kp.p, mesh, k0, cnt, rows, g->stream));
-        WN_HIP(wn_launch_phase_tables(g->kp.p, g->rmin, g->tab.p, rows, g->sh.S, g->stream));
+        WN_HIP(launch_phases(g, rows));
         mk[1] = mark();
-        WN_HIP(wn_launch_interp_f64(g->bf.p, g->tab.p, g->ridx.p, g->x.p, rows, g->sh, g->stream));
+        WN_HIP(launch_interp(g, rows));
         mk[2] = mark();
         WN_HIP(wn_launch_eig_f64(g->x.p, g->e.p, g->u.p, g->sweeps.p, cnt, g->sh, g->stream));
         mk[3] = mark();

@@ -34,15 +34,19 @@ hipError_t wn_launch_build_coeff(const double *hr, const double *ar, const doubl
 // k points of a mesh batch: global indices k0 .. k0+count-1, rows up to rows_pad (extra rows k = 0).
 hipError_t wn_launch_mesh_kpoints(double *kp, const int *mesh, long long k0, int count, int rows_pad,
                                   hipStream_t st);
-// Phase tables T[k][axis][j] = exp(2 pi i k_axis (rmin_axis + j)).
-hipError_t wn_launch_phase_tables(const double *kp, const int *rmin, double2 *tab, int rows, int S,
+// Phase tables T[k][axis][j] = exp(2 pi i k_axis (rmin_axis + j)) (FP64, and FP32 if tab32 != nullptr).
+hipError_t wn_launch_phase_tables(const double *kp, const int *rmin, double2 *tab, float2 *tab32, int rows, int S,
                                   hipStream_t st);
+// Split-BF16 coefficient operand (hi, mid and optional lo parts) for the mixed interpolation.
+hipError_t wn_launch_build_coeff_bf16(const double *hr, const double *ar, const double *rc, const int *deg,
+                                      __bf16 *bh, __bf16 *bm, __bf16 *bl, const WnShape &sh, hipStream_t st);
 // FP64 MFMA interpolation GEMM.
 hipError_t wn_launch_interp_f64(const double *bf, const double2 *tab, const int4 *ridx, double *out, int rows_pad,
                                 const WnShape &sh, hipStream_t st);
-// Split-BF16 MFMA interpolation GEMM (FP32 accumulate, FP64 output).
-hipError_t wn_launch_interp_bf16(const uint16_t *bfh, const uint16_t *bfl, const double2 *tab, const int4 *ridx,
-                                 double *out, int rows_pad, const WnShape &sh, hipStream_t st);
+// Split-BF16 MFMA interpolation GEMM (FP32 accumulate, FP64 output); nterms = 3 or 6.
+hipError_t wn_launch_interp_bf16(const __bf16 *bh, const __bf16 *bm, const __bf16 *bl, const float2 *tab,
+                                 const int4 *ridx, double *out, int rows_pad, int nterms, const WnShape &sh,
+                                 hipStream_t st);
 // Jacobi eigensolver of the upper triangle of channel 0.
 hipError_t wn_launch_eig_f64(const double *x, double *e, double2 *u, int *sweeps, int count, const WnShape &sh,
                              hipStream_t st);

@@ -31,11 +31,57 @@
 // Fragment order: bf[((kb*nNt + nt)*4 + s)*8 + j][lane] holds B[kb*16 + 4s + lane/16][nt*128 + 16j + lane%16],
 // which is the per-lane B operand of v_mfma_f64_16x16x4f64 for k-substep s and 16-column subtile j.
 // ------------------------------------------------------------------------------------------------
+// B entry of R vector r, K part (0: multiplies cos, 1: multiplies sin) and real output column col.
+__device__ double wn_coeff_entry(const double *__restrict__ hr, const double *__restrict__ ar,
+                                 const double *__restrict__ rc, const int *__restrict__ deg, const WnShape &sh, int r,
+                                 int part, int col) {
+    const int n = sh.n, n2 = n * n;
+    const int J = col >> 1, p2 = col & 1;
+    if (r >= sh.nr || J >= sh.C * n2) return 0.0;
+    const int c = J / n2, ij = J - c * n2;
+    const int i = ij / n, jj = ij - i * n;
+    // Julia column-major element (i, jj, r) and (i, jj, a, r)
+    const double *h = hr + 2 * ((size_t)i + (size_t)n * jj + (size_t)n2 * r);
+    double xr = 0.0, xi = 0.0;
+    const double *R = rc + 3 * r;
+    if (c == WN_CH_H) {
+        xr = h[0];
+        xi = h[1];
+    } else if (c < WN_CH_DDH) {
+        const double d = R[c - WN_CH_DH];
+        // (i d) * h
+        xr = -d * h[1];
+        xi = d * h[0];
+    } else if (c < WN_CH_A) {
+        const int t = c - WN_CH_DDH;
+        const int a = t < 3 ? t : (t == 3 ? 1 : 0);
+        const int b = t < 3 ? t : (t == 5 ? 1 : 2);
+        const double f = -R[a] * R[b];
+        xr = f * h[0];
+        xi = f * h[1];
+    } else {
+        const int a = c < WN_CH_DA ? c - WN_CH_A : (c - WN_CH_DA) / 3;
+        const double *p = ar + 2 * ((size_t)i + (size_t)n * jj + (size_t)n2 * a + (size_t)3 * n2 * r);
+        if (c < WN_CH_DA) {
+            xr = p[0];
+            xi = p[1];
+        } else {
+            const int b = (c - WN_CH_DA) % 3;
+            const double d = R[b];
+            xr = -d * p[1];
+            xi = d * p[0];
+        }
+    }
+    const double inv_deg = 1.0 / (double)deg[r];
+    xr *= inv_deg;
+    xi *= inv_deg;
+    return part == 0 ? (p2 == 0 ? xr : xi) : (p2 == 0 ? -xi : xr);
+}
+
 __global__ void k_build_coeff(const double *__restrict__ hr, const double *__restrict__ ar,
                               const double *__restrict__ rc, const int *__restrict__ deg, double *__restrict__ bf,
                               WnShape sh) {
     const long long total = (long long)sh.Kb * sh.nNt * 2048;
-    const int n = sh.n, n2 = n * n;
     for (long long e = (long long)blockIdx.x * blockDim.x + threadIdx.x; e < total;
          e += (long long)gridDim.x * blockDim.x) {
         const int lane = (int)(e & 63);
@@ -48,50 +94,50 @@ __global__ void k_build_coeff(const double *__restrict__ hr, const double *__res
         const int part = kk >> 3;
         const int r = kb * WN_RB + (kk & 7);
         const int col = nt * WN_GN + 16 * j + (lane & 15);
-        const int J = col >> 1, p2 = col & 1;
-        double v = 0.0;
-        if (r < sh.nr && J < sh.C * n2) {
-            const int c = J / n2, ij = J - c * n2;
-            const int i = ij / n, jj = ij - i * n;
-            // Julia column-major element (i, jj, r) and (i, jj, a, r)
-            const double *h = hr + 2 * ((size_t)i + (size_t)n * jj + (size_t)n2 * r);
-            double xr = 0.0, xi = 0.0;
-            const double *R = rc + 3 * r;
-            if (c == WN_CH_H) {
-                xr = h[0];
-                xi = h[1];
-            } else if (c < WN_CH_DDH) {
-                const double d = R[c - WN_CH_DH];
-                // (i d) * h
-                xr = -d * h[1];
-                xi = d * h[0];
-            } else if (c < WN_CH_A) {
-                const int t = c - WN_CH_DDH;
-                const int a = t < 3 ? t : (t == 3 ? 1 : 0);
-                const int b = t < 3 ? t : (t == 5 ? 1 : 2);
-                const double f = -R[a] * R[b];
-                xr = f * h[0];
-                xi = f * h[1];
-            } else {
-                const int a = c < WN_CH_DA ? c - WN_CH_A : (c - WN_CH_DA) / 3;
-                const double *p = ar + 2 * ((size_t)i + (size_t)n * jj + (size_t)n2 * a + (size_t)3 * n2 * r);
-                if (c < WN_CH_DA) {
-                    xr = p[0];
-                    xi = p[1];
-                } else {
-                    const int b = (c - WN_CH_DA) % 3;
-                    const double d = R[b];
-                    xr = -d * p[1];
-                    xi = d * p[0];
-                }
-            }
-            const double inv_deg = 1.0 / (double)deg[r];
-            xr *= inv_deg;
-            xi *= inv_deg;
-            v = part == 0 ? (p2 == 0 ? xr : xi) : (p2 == 0 ? -xi : xr);
-        }
-        bf[e] = v;
+        bf[e] = wn_coeff_entry(hr, ar, rc, deg, sh, r, part, col);
     }
+}
+
+// Split-BF16 operand of v_mfma_f32_32x32x16_bf16 in fragment order: element
+// e = ((kb*nT32 + ct)*64 + lane)*8 + j holds B[kk = 8*(lane/32) + j][32*ct + lane%32] of K block kb, where
+// within a block kk = 2*rr + part (R vector kb*8 + rr, cos/sin part).  x = hi + mid + lo with each part the
+// BF16 rounding of the remainder.
+__global__ void k_build_coeff_bf16(const double *__restrict__ hr, const double *__restrict__ ar,
+                                   const double *__restrict__ rc, const int *__restrict__ deg,
+                                   __bf16 *__restrict__ bh, __bf16 *__restrict__ bm, __bf16 *__restrict__ bl,
+                                   WnShape sh) {
+    const int nT32 = sh.ldo / 32;
+    const long long total = (long long)sh.Kb * nT32 * 512;
+    for (long long e = (long long)blockIdx.x * blockDim.x + threadIdx.x; e < total;
+         e += (long long)gridDim.x * blockDim.x) {
+        const int j = (int)(e & 7);
+        const int lane = (int)((e >> 3) & 63);
+        const long long rest = e >> 9;
+        const int ct = (int)(rest % nT32);
+        const int kb = (int)(rest / nT32);
+        const int kk = 8 * (lane >> 5) + j;
+        const int r = kb * WN_RB + (kk >> 1);
+        const int part = kk & 1;
+        const int col = 32 * ct + (lane & 31);
+        const double x = wn_coeff_entry(hr, ar, rc, deg, sh, r, part, col);
+        const float xf = (float)x;
+        const __bf16 h = (__bf16)xf;
+        const double r1 = x - (double)(float)h;
+        const __bf16 m = (__bf16)(float)r1;
+        const double r2 = r1 - (double)(float)m;
+        bh[e] = h;
+        bm[e] = m;
+        if (bl) bl[e] = (__bf16)(float)r2;
+    }
+}
+
+hipError_t wn_launch_build_coeff_bf16(const double *hr, const double *ar, const double *rc, const int *deg,
+                                      __bf16 *bh, __bf16 *bm, __bf16 *bl, const WnShape &sh, hipStream_t st) {
+    long long total = (long long)sh.Kb * (sh.ldo / 32) * 512;
+    long long blocks = (total + 255) / 256;
+    if (blocks > 65536) blocks = 65536;
+    hipLaunchKernelGGL(k_build_coeff_bf16, dim3((unsigned)blocks), dim3(256), 0, st, hr, ar, rc, deg, bh, bm, bl, sh);
+    return hipGetLastError();
 }
 
 hipError_t wn_launch_build_coeff(const double *hr, const double *ar, const double *rc, const int *deg, double *bf,
@@ -132,7 +178,7 @@ hipError_t wn_launch_mesh_kpoints(double *kp, const int *mesh, long long k0, int
 }
 
 __global__ void k_phase_tables(const double *__restrict__ kp, int rmin0, int rmin1, int rmin2, double2 *__restrict__ tab,
-                               int rows, int S) {
+                               float2 *__restrict__ tab32, int rows, int S) {
     int t = blockIdx.x * blockDim.x + threadIdx.x;
     int total = rows * 3 * S;
     if (t >= total) return;
@@ -145,12 +191,14 @@ __global__ void k_phase_tables(const double *__restrict__ kp, int rmin0, int rmi
     double s, c;
     sincospi(x, &s, &c);
     tab[t] = make_double2(c, s);
+    if (tab32) tab32[t] = make_float2((float)c, (float)s);
 }
 
-hipError_t wn_launch_phase_tables(const double *kp, const int *rmin, double2 *tab, int rows, int S, hipStream_t st) {
+hipError_t wn_launch_phase_tables(const double *kp, const int *rmin, double2 *tab, float2 *tab32, int rows, int S,
+                                  hipStream_t st) {
     int total = rows * 3 * S;
     hipLaunchKernelGGL(k_phase_tables, dim3((total + 255) / 256), dim3(256), 0, st, kp, rmin[0], rmin[1], rmin[2], tab,
-                       rows, S);
+                       tab32, rows, S);
     return hipGetLastError();
 }
 
@@ -236,7 +284,7 @@ __global__ __launch_bounds__(256, 2) void k_interp_f64(const double *__restrict_
     for (int i = 0; i < 4; ++i) {
 #pragma unroll
         for (int q = 0; q < 4; ++q) {
-            const size_t row = (size_t)mt * WN_GM + wm * 64 + 16 * i + 4 * lq + q;
+            const size_t row = (size_t)mt * WN_GM + wm * 64 + 16 * i + lq + 4 * q;
             double *o = out + row * ldo + (size_t)nt * WN_GN + wn * 64 + lrow;
 #pragma unroll
             for (int j = 0; j < 4; ++j) o[16 * j] = acc[i][j][q];
@@ -251,6 +299,127 @@ hipError_t wn_launch_interp_f64(const double *bf, const double2 *tab, const int4
     const long long grid = 8LL * panels * nMt;
     hipLaunchKernelGGL(k_interp_f64, dim3((unsigned)grid), dim3(256), 0, st, bf, tab, ridx, out, nMt, sh.nNt, sh.Kb,
                        sh.S, sh.ldo);
+    return hipGetLastError();
+}
+
+// ------------------------------------------------------------------------------------------------
+// Split-BF16 MFMA interpolation GEMM (mixed precision form).
+//
+// x ~ x_hi + x_mid (+ x_lo) in BF16 for both the Fourier factors and the coefficients; the product is
+// accumulated in FP32 by v_mfma_f32_32x32x16_bf16 from the terms hi*hi + hi*mid + mid*hi
+// (NT = 3, relative error ~2^-16) or additionally hi*lo + lo*hi + mid*mid (NT = 6, ~2^-24).
+// Workgroup 256 threads = 2x2 waves, tile 128 k x 256 real columns; a wave owns 64x128 =
+// 2x4 subtiles of 32x32.  Lane l of row subtile i holds the Fourier factors of row 32i + l%32 for
+// R vectors kb*8 + 4*(l/32) + {0..3} as (cos, sin) pairs, generated in FP32 from the phase tables.
+// XCD-aware tile order as in the FP64 kernel.
+// ------------------------------------------------------------------------------------------------
+__device__ __forceinline__ void wn_split_phase(float x, __bf16 &h, __bf16 &m, __bf16 &l) {
+    h = (__bf16)x;
+    const float r1 = x - (float)h;
+    m = (__bf16)r1;
+    l = (__bf16)(r1 - (float)m);
+}
+
+template <int NT>
+__global__ __launch_bounds__(256, 2) void k_interp_bf16(const bf16x8 *__restrict__ bh, const bf16x8 *__restrict__ bm,
+                                                       const bf16x8 *__restrict__ bl, const float2 *__restrict__ tab,
+                                                       const int4 *__restrict__ ridx, double *__restrict__ out, int nMt,
+                                                       int nNt2, int Kb, int S, int ldo) {
+    const int g = blockIdx.x;
+    const int xcd = g & 7;
+    const int jj = g >> 3;
+    const int nl = jj / nMt;
+    const int mt = jj - nl * nMt;
+    const int nt = nl * 8 + xcd;
+    if (nt >= nNt2) return;
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int wm = wave >> 1, wn = wave & 1;
+    const int l32 = lane & 31, lh = lane >> 5;
+    const int nT32 = ldo / 32;
+
+    const float2 *tk[2];
+#pragma unroll
+    for (int i = 0; i < 2; ++i) tk[i] = tab + (size_t)(mt * WN_GM + wm * 64 + 32 * i + l32) * 3 * S;
+
+    f32x16 acc[2][4];
+#pragma unroll
+    for (int i = 0; i < 2; ++i)
+#pragma unroll
+        for (int j = 0; j < 4; ++j)
+#pragma unroll
+            for (int q = 0; q < 16; ++q) acc[i][j][q] = 0.0f;
+
+    const int ct0 = nt * 8 + wn * 4;
+    for (int kb = 0; kb < Kb; ++kb) {
+        bf16x8 b_h[4], b_m[4], b_l[4];
+        const size_t boff = ((size_t)kb * nT32 + ct0) * 64 + lane;
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            b_h[j] = bh[boff + 64 * j];
+            b_m[j] = bm[boff + 64 * j];
+            if (NT == 6) b_l[j] = bl[boff + 64 * j];
+        }
+        bf16x8 a_h[2], a_m[2], a_l[2];
+#pragma unroll
+        for (int rr = 0; rr < 4; ++rr) {
+            const int4 id = ridx[kb * WN_RB + 4 * lh + rr];
+#pragma unroll
+            for (int i = 0; i < 2; ++i) {
+                const float2 *t = tk[i];
+                const float2 t1 = t[id.x], t2 = t[S + id.y], t3 = t[2 * S + id.z];
+                const float pr = t1.x * t2.x - t1.y * t2.y, pi = t1.x * t2.y + t1.y * t2.x;
+                const float fr = pr * t3.x - pi * t3.y, fi = pr * t3.y + pi * t3.x;
+                __bf16 h0, m0, l0, h1, m1, l1;
+                wn_split_phase(fr, h0, m0, l0);
+                wn_split_phase(fi, h1, m1, l1);
+                a_h[i][2 * rr] = h0;
+                a_h[i][2 * rr + 1] = h1;
+                a_m[i][2 * rr] = m0;
+                a_m[i][2 * rr + 1] = m1;
+                if (NT == 6) {
+                    a_l[i][2 * rr] = l0;
+                    a_l[i][2 * rr + 1] = l1;
+                }
+            }
+        }
+#pragma unroll
+        for (int i = 0; i < 2; ++i)
+#pragma unroll
+            for (int j = 0; j < 4; ++j) {
+                acc[i][j] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a_m[i], b_h[j], acc[i][j], 0, 0, 0);
+                acc[i][j] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a_h[i], b_m[j], acc[i][j], 0, 0, 0);
+                if (NT == 6) {
+                    acc[i][j] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a_m[i], b_m[j], acc[i][j], 0, 0, 0);
+                    acc[i][j] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a_l[i], b_h[j], acc[i][j], 0, 0, 0);
+                    acc[i][j] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a_h[i], b_l[j], acc[i][j], 0, 0, 0);
+                }
+                acc[i][j] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a_h[i], b_h[j], acc[i][j], 0, 0, 0);
+            }
+    }
+#pragma unroll
+    for (int i = 0; i < 2; ++i)
+#pragma unroll
+        for (int q = 0; q < 16; ++q) {
+            const size_t row = (size_t)mt * WN_GM + wm * 64 + 32 * i + (q >> 2) * 8 + lh * 4 + (q & 3);
+            double *o = out + row * ldo + (size_t)nt * 256 + wn * 128 + l32;
+#pragma unroll
+            for (int j = 0; j < 4; ++j) o[32 * j] = (double)acc[i][j][q];
+        }
+}
+
+hipError_t wn_launch_interp_bf16(const __bf16 *bh, const __bf16 *bm, const __bf16 *bl, const float2 *tab,
+                                 const int4 *ridx, double *out, int rows_pad, int nterms, const WnShape &sh,
+                                 hipStream_t st) {
+    const int nMt = rows_pad / WN_GM;
+    const int nNt2 = sh.ldo / 256;
+    const int panels = (nNt2 + 7) / 8;
+    const long long grid = 8LL * panels * nMt;
+    if (nterms == 6)
+        hipLaunchKernelGGL(k_interp_bf16<6>, dim3((unsigned)grid), dim3(256), 0, st, (const bf16x8 *)bh,
+                           (const bf16x8 *)bm, (const bf16x8 *)bl, tab, ridx, out, nMt, nNt2, sh.Kb, sh.S, sh.ldo);
+    else
+        hipLaunchKernelGGL(k_interp_bf16<3>, dim3((unsigned)grid), dim3(256), 0, st, (const bf16x8 *)bh,
+                           (const bf16x8 *)bm, (const bf16x8 *)bl, tab, ridx, out, nMt, nNt2, sh.Kb, sh.S, sh.ldo);
     return hipGetLastError();
 }
 
@@ -499,10 +668,110 @@ __global__ __launch_bounds__(256) void k_rotate_f64(double *__restrict__ x, cons
     }
 }
 
+// MFMA form of the rotation: complex products as four real products on v_mfma_f64_16x16x4f64,
+//   T = U^+ X:  Tr = Ur^T Xr + Ui^T Xi,  Ti = Ur^T Xi - Ui^T Xr
+//   Y = T U:    Yr = Tr Ur - Ti Ui,      Yi = Tr Ui + Ti Ur
+// with U, X (then T) in LDS as planar real arrays padded to np = 16*ceil(n/16) (zeros beyond n).
+// The (np/16)^2 output tiles are dealt to the 4 waves.
+__global__ __launch_bounds__(256) void k_rotate_mfma(double *__restrict__ x, const double2 *__restrict__ u, int n,
+                                                    int C, int ldo) {
+    extern __shared__ double smd[];
+    const int k = blockIdx.x;
+    const int t = threadIdx.x;
+    const int lane = t & 63, wave = t >> 6;
+    const int np = (n + 15) & ~15;
+    const int ld = np + 1;
+    const int nt = np >> 4;
+    double *Ur = smd;
+    double *Ui = Ur + np * ld;
+    double *Xr = Ui + np * ld;
+    double *Xi = Xr + np * ld;
+    const double2 *uk = u + (size_t)k * n * n;
+    for (int e = t; e < np * np; e += 256) {
+        int i = e / np, j = e - i * np;
+        double2 v = (i < n && j < n) ? uk[i * n + j] : make_double2(0.0, 0.0);
+        Ur[i * ld + j] = v.x;
+        Ui[i * ld + j] = v.y;
+    }
+    double2 *xk = (double2 *)(x + (size_t)k * ldo);
+    const int lr = lane & 15, lq = lane >> 4;
+    for (int c = 1; c < C; ++c) {
+        double2 *xc = xk + (size_t)c * n * n;
+        __syncthreads();
+        for (int e = t; e < np * np; e += 256) {
+            int i = e / np, j = e - i * np;
+            double2 v = (i < n && j < n) ? xc[i * n + j] : make_double2(0.0, 0.0);
+            Xr[i * ld + j] = v.x;
+            Xi[i * ld + j] = v.y;
+        }
+        __syncthreads();
+        // T = U^+ X, tiles kept in registers (at most 4 per wave for np <= 64)
+        f64x4 tr[4], ti[4];
+#pragma unroll
+        for (int s4 = 0; s4 < 4; ++s4) {
+            const int tile = wave + 4 * s4;
+            f64x4 ar = {0.0, 0.0, 0.0, 0.0}, ai = {0.0, 0.0, 0.0, 0.0};
+            if (tile < nt * nt) {
+                const int r0 = (tile / nt) * 16, c0 = (tile % nt) * 16;
+                for (int k0 = 0; k0 < np; k0 += 4) {
+                    const int kr = k0 + lq;
+                    const double ur = Ur[kr * ld + r0 + lr], ui = Ui[kr * ld + r0 + lr];
+                    const double xr = Xr[kr * ld + c0 + lr], xi = Xi[kr * ld + c0 + lr];
+                    ar = __builtin_amdgcn_mfma_f64_16x16x4f64(ur, xr, ar, 0, 0, 0);
+                    ar = __builtin_amdgcn_mfma_f64_16x16x4f64(ui, xi, ar, 0, 0, 0);
+                    ai = __builtin_amdgcn_mfma_f64_16x16x4f64(ur, xi, ai, 0, 0, 0);
+                    ai = __builtin_amdgcn_mfma_f64_16x16x4f64(-ui, xr, ai, 0, 0, 0);
+                }
+            }
+            tr[s4] = ar;
+            ti[s4] = ai;
+        }
+        __syncthreads();
+#pragma unroll
+        for (int s4 = 0; s4 < 4; ++s4) {
+            const int tile = wave + 4 * s4;
+            if (tile < nt * nt) {
+                const int r0 = (tile / nt) * 16, c0 = (tile % nt) * 16;
+#pragma unroll
+                for (int q = 0; q < 4; ++q) {
+                    Xr[(r0 + lq + 4 * q) * ld + c0 + lr] = tr[s4][q];
+                    Xi[(r0 + lq + 4 * q) * ld + c0 + lr] = ti[s4][q];
+                }
+            }
+        }
+        __syncthreads();
+        // Y = T U, written to global
+        for (int tile = wave; tile < nt * nt; tile += 4) {
+            const int r0 = (tile / nt) * 16, c0 = (tile % nt) * 16;
+            f64x4 yr = {0.0, 0.0, 0.0, 0.0}, yi = {0.0, 0.0, 0.0, 0.0};
+            for (int k0 = 0; k0 < np; k0 += 4) {
+                const int kc = k0 + lq;
+                const double tr_ = Xr[(r0 + lr) * ld + kc], ti_ = Xi[(r0 + lr) * ld + kc];
+                const double ur = Ur[kc * ld + c0 + lr], ui = Ui[kc * ld + c0 + lr];
+                yr = __builtin_amdgcn_mfma_f64_16x16x4f64(tr_, ur, yr, 0, 0, 0);
+                yr = __builtin_amdgcn_mfma_f64_16x16x4f64(-ti_, ui, yr, 0, 0, 0);
+                yi = __builtin_amdgcn_mfma_f64_16x16x4f64(tr_, ui, yi, 0, 0, 0);
+                yi = __builtin_amdgcn_mfma_f64_16x16x4f64(ti_, ur, yi, 0, 0, 0);
+            }
+#pragma unroll
+            for (int q = 0; q < 4; ++q) {
+                const int row = r0 + lq + 4 * q, col = c0 + lr;
+                if (row < n && col < n) xc[row * n + col] = make_double2(yr[q], yi[q]);
+            }
+        }
+    }
+}
+
 hipError_t wn_launch_rotate_f64(double *x, const double2 *u, int count, const WnShape &sh, hipStream_t st) {
     const int n = sh.n;
-    size_t lds = (size_t)2 * n * (n + 1) * sizeof(double2);
-    hipLaunchKernelGGL(k_rotate_f64, dim3(count), dim3(256), lds, st, x, u, n, sh.C, sh.ldo);
+    if (n >= 16) {
+        const int np = (n + 15) & ~15;
+        size_t lds = (size_t)4 * np * (np + 1) * sizeof(double);
+        hipLaunchKernelGGL(k_rotate_mfma, dim3(count), dim3(256), lds, st, x, u, n, sh.C, sh.ldo);
+    } else {
+        size_t lds = (size_t)2 * n * (n + 1) * sizeof(double2);
+        hipLaunchKernelGGL(k_rotate_f64, dim3(count), dim3(256), lds, st, x, u, n, sh.C, sh.ldo);
+    }
     return hipGetLastError();
 }
 

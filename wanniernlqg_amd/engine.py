@@ -25,6 +25,7 @@ WNLQG_GAUSSIAN = 0
 WNLQG_LORENTZIAN = 1
 WNLQG_PRECISION_FP64 = 0
 WNLQG_PRECISION_MIXED = 1
+WNLQG_PRECISION_MIXED_HIGH = 2
 
 ERRORS = {-1: "WNLQG_E_INVALID", -2: "WNLQG_E_NONFINITE", -3: "WNLQG_E_GAP", -4: "WNLQG_E_HIP",
           -5: "WNLQG_E_OOM", -6: "WNLQG_E_UNSUPPORTED"}
@@ -149,7 +150,8 @@ class Engine:
                    _ptr(self._ar) if self._ar is not None else C.POINTER(C.c_double)())
         q = (WNLQG_SHIFT_CURRENT if task.shift_current else 0) | (WNLQG_INJECTION_CURRENT if task.injection_current else 0)
         bt = {"Gaussian": WNLQG_GAUSSIAN, "Lorentzian": WNLQG_LORENTZIAN}.get(task.broadening_type, 99)
-        prec = {"fp64": WNLQG_PRECISION_FP64, "mixed": WNLQG_PRECISION_MIXED}[task.precision]
+        prec = {"fp64": WNLQG_PRECISION_FP64, "mixed": WNLQG_PRECISION_MIXED,
+                "mixed6": WNLQG_PRECISION_MIXED_HIGH}[task.precision]
         t = _Task(q, self._om.shape[0], _ptr(self._om), bt, task.broadening, task.fermi_energy, task.temperature,
                   task.denominator_regularization, task.transition_window_factor, task.band_window_size, prec,
                   task.batch_size, 0)
