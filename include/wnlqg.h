@@ -99,6 +99,8 @@ typedef struct wnlqg_stats {
     int32_t batches;        /* device batches */
     int32_t interp_launches;/* launches of the interpolation kernel */
     double interp_flops;    /* algorithmic interpolation flops: 8 * N_R * n^2 * C per k point */
+    int32_t max_sweeps;     /* largest Jacobi sweep count over the k points */
+    int32_t reserved;
 } wnlqg_stats;
 
 typedef struct wnlqg_engine wnlqg_engine;

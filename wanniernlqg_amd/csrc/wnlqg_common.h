@@ -12,6 +12,7 @@ typedef double f64x4 __attribute__((ext_vector_type(4)));
 typedef float f32x4 __attribute__((ext_vector_type(4)));
 typedef float f32x16 __attribute__((ext_vector_type(16)));
 typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef _Float16 f16x8 __attribute__((ext_vector_type(8)));
 
 // Interpolation channel map.  Channel c of the per-k block [C][n][n]:
 //   0 H, 1-3 dH_a, 4-9 ddH for (xx,yy,zz,yz,xz,xy), 10-12 A_a, 13-21 dA_ab (13 + 3a + b).

@@ -17,6 +17,7 @@
 #include <cmath>
 #include <cstdarg>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <string>
 #include <vector>
@@ -91,8 +92,14 @@ struct wnlqg_engine {
     DevBuf<int4> ridx, pmeta, chunks, lanes;
     DevBuf<double2> tab, u, pmd, q, partial, lanebuf, lanecomp;
     DevBuf<float2> tab32;
-    DevBuf<__bf16> bh, bm, bl;  // split-BF16 coefficient parts (mixed forms)
-    int nterms = 0;             // 0 = FP64 interpolation, 3 or 6 = split-BF16 products
+    DevBuf<__bf16> bh, bm, bl;  // split-BF16 coefficient parts (alternative mixed kernel)
+    DevBuf<_Float16> fh, fm;    // split-FP16 coefficient parts (mixed forms)
+    DevBuf<double> cup, cdown;  // per-column power-of-two scales of the FP16 operand
+    DevBuf<float2> t12, t3;     // per-k (R1,R2)-plane and R3 phase tables (FP32)
+    DevBuf<int2> ridx2;         // per R vector: (j1*S + j2, j3)
+    int nterms = 0;             // 0 = FP64 interpolation, else split products per MFMA output
+    int kind = 0;               // 0 FP64, 1 split FP16, 2 split BF16
+    int h_tiles = 0;            // 128-column FP64 tiles holding channel H (mixed forms)
     int partial_cap = 0;  // chunks the partial buffer holds
     std::vector<double> omega_host;
     hipEvent_t ev[8] = {};
@@ -136,7 +143,11 @@ int ensure_rows(wnlqg_engine *g, int rows) {
     WN_HIP(g->x.alloc((size_t)rows * g->sh.ldo));
     WN_HIP(g->kp.alloc((size_t)rows * 3));
     WN_HIP(g->tab.alloc((size_t)rows * 3 * g->sh.S));
-    if (g->nterms) WN_HIP(g->tab32.alloc((size_t)rows * 3 * g->sh.S));
+    if (g->kind == 2) WN_HIP(g->tab32.alloc((size_t)rows * 3 * g->sh.S));
+    if (g->kind == 1) {
+        WN_HIP(g->t12.alloc((size_t)rows * g->sh.S * g->sh.S));
+        WN_HIP(g->t3.alloc((size_t)rows * g->sh.S));
+    }
     WN_HIP(g->e.alloc((size_t)rows * g->sh.n));
     WN_HIP(g->u.alloc((size_t)rows * n2));
     WN_HIP(g->sweeps.alloc((size_t)rows));
@@ -157,14 +168,27 @@ int ensure_partial(wnlqg_engine *g, int nchunks) {
 }
 
 hipError_t launch_phases(wnlqg_engine *g, int rows) {
-    return wn_launch_phase_tables(g->kp.p, g->rmin, g->tab.p, g->nterms ? g->tab32.p : nullptr, rows, g->sh.S,
-                                  g->stream);
+    hipError_t e = wn_launch_phase_tables(g->kp.p, g->rmin, g->tab.p, g->kind == 2 ? g->tab32.p : nullptr, rows,
+                                          g->sh.S, g->stream);
+    if (e == hipSuccess && g->kind == 1) e = wn_launch_phase_planes(g->tab.p, g->t12.p, g->t3.p, rows, g->sh.S, g->stream);
+    return e;
 }
 
+// Mixed forms: every channel after H in split BF16, then H (the eigenproblem input, columns
+// [0, 2n^2)) in FP64, which also overwrites the BF16 values of a shared boundary tile.
 hipError_t launch_interp(wnlqg_engine *g, int rows) {
-    if (g->nterms)
-        return wn_launch_interp_bf16(g->bh.p, g->bm.p, g->bl.p, g->tab32.p, g->ridx.p, g->x.p, rows, g->nterms, g->sh,
-                                     g->stream);
+    if (g->nterms) {
+        const int hcols = 2 * g->sh.n * g->sh.n;
+        hipError_t e = g->kind == 1
+                           ? wn_launch_interp_f16(g->fh.p, g->fm.p, g->cdown.p, g->t12.p, g->t3.p, g->ridx2.p, g->x.p,
+                                                  rows, g->nterms, hcols, g->sh, g->stream)
+                           : wn_launch_interp_bf16(g->bh.p, g->bm.p, g->bl.p, g->tab32.p, g->ridx.p, g->x.p, rows,
+                                                   g->nterms, hcols, g->sh, g->stream);
+        if (e != hipSuccess) return e;
+        WnShape hs = g->sh;
+        hs.nNt = g->h_tiles;
+        return wn_launch_interp_f64(g->bf.p, g->tab.p, g->ridx.p, g->x.p, rows, hs, g->stream);
+    }
     return wn_launch_interp_f64(g->bf.p, g->tab.p, g->ridx.p, g->x.p, rows, g->sh, g->stream);
 }
 
@@ -219,7 +243,8 @@ int32_t wnlqg_destroy(wnlqg_engine *g) {
     g->flags.release(); g->ridx.release(); g->pmeta.release(); g->chunks.release(); g->lanes.release();
     g->tab.release(); g->u.release(); g->pmd.release(); g->q.release(); g->partial.release();
     g->lanebuf.release(); g->lanecomp.release(); g->tab32.release(); g->bh.release(); g->bm.release();
-    g->bl.release();
+    g->bl.release(); g->fh.release(); g->fm.release(); g->cup.release(); g->cdown.release(); g->t12.release();
+    g->t3.release(); g->ridx2.release();
     if (g->stream) (void)hipStreamDestroy(g->stream);
     delete g;
     return WNLQG_OK;
@@ -285,7 +310,20 @@ int32_t wnlqg_create(const wnlqg_model *model, const wnlqg_task *task, wnlqg_eng
     g->sh.Kb = (nr + WN_RB - 1) / WN_RB;
     // columns padded to 256 so that both interpolation kernels tile the same output rows
     g->sh.nNt = (int)((2 * (size_t)g->sh.C * n2 + 255) / 256) * 2;
-    g->nterms = task->precision == WNLQG_PRECISION_MIXED ? 3 : (task->precision == WNLQG_PRECISION_MIXED_HIGH ? 6 : 0);
+    // mixed forms: split FP16 with 3 (MIXED) or 4 (MIXED_HIGH) products; WNLQG_MIXED_KERNEL=bf16 selects
+    // the split-BF16 kernel (WNLQG_BF16_TERMS = 3..6 products) for comparison runs
+    g->nterms = task->precision == WNLQG_PRECISION_MIXED ? 3 : (task->precision == WNLQG_PRECISION_MIXED_HIGH ? 4 : 0);
+    g->kind = g->nterms ? 1 : 0;
+    if (g->nterms) {
+        const char *kenv = getenv("WNLQG_MIXED_KERNEL");
+        if (kenv && strcmp(kenv, "bf16") == 0) {
+            g->kind = 2;
+            g->nterms = 6;
+            const char *env = getenv("WNLQG_BF16_TERMS");
+            if (env && atoi(env) >= 3 && atoi(env) <= 6) g->nterms = atoi(env);
+        }
+    }
+    g->h_tiles = (int)((2 * n2 + WN_GN - 1) / WN_GN);
     g->sh.ldo = g->sh.nNt * WN_GN;
     int rmax[3];
     for (int a = 0; a < 3; ++a) {
@@ -303,17 +341,21 @@ int32_t wnlqg_create(const wnlqg_model *model, const wnlqg_task *task, wnlqg_eng
     // host-side R tables: Cartesian R (as _r_vector_cartesian, HamiltonianMatrixElements.jl:74-85)
     std::vector<double> rcart((size_t)3 * nr);
     std::vector<int4> rid((size_t)g->sh.Kb * WN_RB, make_int4(0, 0, 0, 0));
+    std::vector<int2> rid2((size_t)g->sh.Kb * WN_RB, make_int2(0, 0));
     for (int r = 0; r < nr; ++r) {
         const int *R = model->r_vectors + 3 * r;
         for (int a = 0; a < 3; ++a)
             rcart[3 * r + a] = R[0] * L[0 + 3 * a] + R[1] * L[1 + 3 * a] + R[2] * L[2 + 3 * a];
         rid[r] = make_int4(R[0] - g->rmin[0], R[1] - g->rmin[1], R[2] - g->rmin[2], 0);
+        rid2[r] = make_int2((R[0] - g->rmin[0]) * g->sh.S + (R[1] - g->rmin[1]), R[2] - g->rmin[2]);
     }
     if (g->hr.alloc(2 * n2 * nr) || g->rc.alloc(3 * (size_t)nr) || g->deg.alloc(nr) ||
         g->ridx.alloc(rid.size()) || g->omega.alloc(g->sc.nw) || g->flags.alloc(4) ||
-        (!g->nterms && g->bf.alloc((size_t)g->sh.Kb * g->sh.nNt * 2048)) ||
-        (g->nterms && (g->bh.alloc((size_t)g->sh.Kb * 16 * g->sh.ldo) || g->bm.alloc((size_t)g->sh.Kb * 16 * g->sh.ldo))) ||
-        (g->nterms == 6 && g->bl.alloc((size_t)g->sh.Kb * 16 * g->sh.ldo)) ||
+        g->bf.alloc((size_t)g->sh.Kb * (g->nterms ? g->h_tiles : g->sh.nNt) * 2048) ||
+        (g->kind == 2 && (g->bh.alloc((size_t)g->sh.Kb * 16 * g->sh.ldo) || g->bm.alloc((size_t)g->sh.Kb * 16 * g->sh.ldo))) ||
+        (g->kind == 2 && g->nterms >= 4 && g->bl.alloc((size_t)g->sh.Kb * 16 * g->sh.ldo)) ||
+        (g->kind == 1 && (g->fh.alloc((size_t)g->sh.Kb * 16 * g->sh.ldo) || g->fm.alloc((size_t)g->sh.Kb * 16 * g->sh.ldo) ||
+                          g->cup.alloc(g->sh.ldo) || g->cdown.alloc(g->sh.ldo) || g->ridx2.alloc(rid.size()))) ||
         (g->sh.has_pos && g->ar.alloc(2 * n2 * 3 * nr)))
         return bail(fail(WNLQG_E_OOM, "device allocation for the model failed"));
     auto up = [&](void *dst, const void *src, size_t bytes) {
@@ -323,12 +365,22 @@ int32_t wnlqg_create(const wnlqg_model *model, const wnlqg_task *task, wnlqg_eng
         !up(g->rc.p, rcart.data(), rcart.size() * sizeof(double)) ||
         !up(g->deg.p, model->r_degeneracies, nr * sizeof(int)) ||
         !up(g->ridx.p, rid.data(), rid.size() * sizeof(int4)) ||
+        (g->kind == 1 && !up(g->ridx2.p, rid2.data(), rid2.size() * sizeof(int2))) ||
         !up(g->omega.p, g->omega_host.data(), g->sc.nw * sizeof(double)) ||
         (g->sh.has_pos && !up(g->ar.p, model->position_r, 2 * n2 * 3 * nr * sizeof(double))))
         return bail(fail(WNLQG_E_HIP, "model upload failed"));
-    hipError_t be = g->nterms ? wn_launch_build_coeff_bf16(g->hr.p, g->ar.p, g->rc.p, g->deg.p, g->bh.p, g->bm.p,
-                                                           g->bl.p, g->sh, g->stream)
-                              : wn_launch_build_coeff(g->hr.p, g->ar.p, g->rc.p, g->deg.p, g->bf.p, g->sh, g->stream);
+    hipError_t be = hipSuccess;
+    if (g->nterms) {
+        be = g->kind == 1 ? wn_launch_build_coeff_f16(g->hr.p, g->ar.p, g->rc.p, g->deg.p, g->cup.p, g->cdown.p,
+                                                      g->fh.p, g->fm.p, g->sh, g->stream)
+                          : wn_launch_build_coeff_bf16(g->hr.p, g->ar.p, g->rc.p, g->deg.p, g->bh.p, g->bm.p, g->bl.p,
+                                                       g->sh, g->stream);
+        WnShape hs = g->sh;
+        hs.nNt = g->h_tiles;
+        if (be == hipSuccess) be = wn_launch_build_coeff(g->hr.p, g->ar.p, g->rc.p, g->deg.p, g->bf.p, hs, g->stream);
+    } else {
+        be = wn_launch_build_coeff(g->hr.p, g->ar.p, g->rc.p, g->deg.p, g->bf.p, g->sh, g->stream);
+    }
     if (be != hipSuccess) return bail(fail(WNLQG_E_HIP, "coefficient build launch failed"));
     if (hipStreamSynchronize(g->stream) != hipSuccess) return bail(fail(WNLQG_E_HIP, "coefficient build failed"));
 
@@ -492,7 +544,7 @@ int32_t wnlqg_run_mesh(wnlqg_engine *g, const int32_t mesh[3], int32_t n_lanes, 
     }
     WN_HIP(hipMemsetAsync(g->lanebuf.p, 0, (size_t)n_lanes * nelem * sizeof(double2), g->stream));
     WN_HIP(hipMemsetAsync(g->lanecomp.p, 0, (size_t)n_lanes * nelem * sizeof(double2), g->stream));
-    WN_HIP(hipMemsetAsync(g->flags.p, 0, sizeof(int), g->stream));
+    WN_HIP(hipMemsetAsync(g->flags.p, 0, 2 * sizeof(int), g->stream));
     double2 *comp = g->nterms ? g->lanecomp.p : nullptr;
 
     std::vector<hipEvent_t> evs;
@@ -515,7 +567,7 @@ int32_t wnlqg_run_mesh(wnlqg_engine *g, const int32_t mesh[3], int32_t n_lanes, 
         mk[1] = mark();
         WN_HIP(launch_interp(g, rows));
         mk[2] = mark();
-        WN_HIP(wn_launch_eig_f64(g->x.p, g->e.p, g->u.p, g->sweeps.p, cnt, g->sh, g->stream));
+        WN_HIP(wn_launch_eig_f64(g->x.p, g->e.p, g->u.p, g->flags.p + 1, cnt, g->sh, g->stream));
         mk[3] = mark();
         WN_HIP(wn_launch_rotate_f64(g->x.p, g->u.p, cnt, g->sh, g->stream));
         mk[4] = mark();
@@ -553,6 +605,9 @@ int32_t wnlqg_run_mesh(wnlqg_engine *g, const int32_t mesh[3], int32_t n_lanes, 
     for (auto e : evs) hipEventDestroy(e);
     rc = check_flags(g);
     if (rc) return rc;
+    int fl[2] = {0, 0};
+    WN_HIP(hipMemcpy(fl, g->flags.p, 2 * sizeof(int), hipMemcpyDeviceToHost));
+    st.max_sweeps = fl[1];
     st.kpoints = nk;
     st.batches = (int)nbatch;
     st.interp_launches = (int)nbatch;

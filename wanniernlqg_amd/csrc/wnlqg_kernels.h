@@ -40,13 +40,22 @@ hipError_t wn_launch_phase_tables(const double *kp, const int *rmin, double2 *ta
 // Split-BF16 coefficient operand (hi, mid and optional lo parts) for the mixed interpolation.
 hipError_t wn_launch_build_coeff_bf16(const double *hr, const double *ar, const double *rc, const int *deg,
                                       __bf16 *bh, __bf16 *bm, __bf16 *bl, const WnShape &sh, hipStream_t st);
-// FP64 MFMA interpolation GEMM.
+// FP64 MFMA interpolation GEMM over the sh.nNt 128-column tiles stored in bf.
 hipError_t wn_launch_interp_f64(const double *bf, const double2 *tab, const int4 *ridx, double *out, int rows_pad,
                                 const WnShape &sh, hipStream_t st);
-// Split-BF16 MFMA interpolation GEMM (FP32 accumulate, FP64 output); nterms = 3 or 6.
+// Split-BF16 MFMA interpolation GEMM (FP32 accumulate, FP64 output); nterms = 3 or 6.  Computes the
+// 256-column tiles from the one containing real column col0 to the end of the row.
 hipError_t wn_launch_interp_bf16(const __bf16 *bh, const __bf16 *bm, const __bf16 *bl, const float2 *tab,
-                                 const int4 *ridx, double *out, int rows_pad, int nterms, const WnShape &sh,
+                                 const int4 *ridx, double *out, int rows_pad, int nterms, int col0, const WnShape &sh,
                                  hipStream_t st);
+// Split-FP16 mixed interpolation: per-column scales, FP16 coefficient parts, (R1,R2)-plane phase tables and GEMM.
+hipError_t wn_launch_build_coeff_f16(const double *hr, const double *ar, const double *rc, const int *deg,
+                                     double *up, double *down, _Float16 *bh, _Float16 *bm, const WnShape &sh,
+                                     hipStream_t st);
+hipError_t wn_launch_phase_planes(const double2 *tab, float2 *t12, float2 *t3, int rows, int S, hipStream_t st);
+hipError_t wn_launch_interp_f16(const _Float16 *bh, const _Float16 *bm, const double *cscale, const float2 *t12,
+                                const float2 *t3, const int2 *ridx2, double *out, int rows_pad, int nterms, int col0,
+                                const WnShape &sh, hipStream_t st);
 // Jacobi eigensolver of the upper triangle of channel 0.
 hipError_t wn_launch_eig_f64(const double *x, double *e, double2 *u, int *sweeps, int count, const WnShape &sh,
                              hipStream_t st);

@@ -303,16 +303,21 @@ hipError_t wn_launch_interp_f64(const double *bf, const double2 *tab, const int4
 }
 
 // ------------------------------------------------------------------------------------------------
-// Split-BF16 MFMA interpolation GEMM (mixed precision form).
+// Split-BF16 MFMA interpolation GEMM (mixed precision forms).
 //
-// x ~ x_hi + x_mid (+ x_lo) in BF16 for both the Fourier factors and the coefficients; the product is
-// accumulated in FP32 by v_mfma_f32_32x32x16_bf16 from the terms hi*hi + hi*mid + mid*hi
-// (NT = 3, relative error ~2^-16) or additionally hi*lo + lo*hi + mid*mid (NT = 6, ~2^-24).
-// Workgroup 256 threads = 2x2 waves, tile 128 k x 256 real columns; a wave owns 64x128 =
-// 2x4 subtiles of 32x32.  Lane l of row subtile i holds the Fourier factors of row 32i + l%32 for
-// R vectors kb*8 + 4*(l/32) + {0..3} as (cos, sin) pairs, generated in FP32 from the phase tables.
-// XCD-aware tile order as in the FP64 kernel.
+// Operands are split into BF16 parts, x ~ x_h + x_m (+ x_l), each part the BF16 rounding of the
+// remainder, and products are accumulated by v_mfma_f32_32x32x16_bf16 from the terms
+//   NT=3: hh + hm + mh   NT=4: + h*B_l   NT=5: + mm   NT=6: + A_l*h
+// (A = Fourier factors, B = coefficients).  FP32 accumulators are flushed into FP64 accumulators
+// every WN_FLUSH K blocks, so the FP32 rounding of a long K sum does not limit the result.
+// Workgroup 256 threads = 2x2 waves, tile 128 k x 128 real columns, wave tile 64x64 = 2x2 subtiles
+// of 32x32.  Per K block (8 R vectors) the workgroup generates the A fragments of its 128 rows once
+// (thread t: row subtile t/64, lane t%64 -> row 32*(t/64) + lane%32, R vectors 4*(lane/32)+{0..3}
+// as (cos, sin) pairs, in FP32 from the phase tables) into double-buffered LDS; B fragments stream
+// from HBM/L2 in fragment order one K block ahead.  XCD-aware tile order as in the FP64 kernel.
 // ------------------------------------------------------------------------------------------------
+#define WN_FLUSH 4
+
 __device__ __forceinline__ void wn_split_phase(float x, __bf16 &h, __bf16 &m, __bf16 &l) {
     h = (__bf16)x;
     const float r1 = x - (float)h;
@@ -321,105 +326,373 @@ __device__ __forceinline__ void wn_split_phase(float x, __bf16 &h, __bf16 &m, __
 }
 
 template <int NT>
-__global__ __launch_bounds__(256, 2) void k_interp_bf16(const bf16x8 *__restrict__ bh, const bf16x8 *__restrict__ bm,
-                                                       const bf16x8 *__restrict__ bl, const float2 *__restrict__ tab,
-                                                       const int4 *__restrict__ ridx, double *__restrict__ out, int nMt,
-                                                       int nNt2, int Kb, int S, int ldo) {
+__global__ __launch_bounds__(256) void k_interp_bf16(const bf16x8 *__restrict__ bh, const bf16x8 *__restrict__ bm,
+                                                    const bf16x8 *__restrict__ bl, const float2 *__restrict__ tab,
+                                                    const int4 *__restrict__ ridx, double *__restrict__ out, int nMt,
+                                                    int nt0, int nNt1, int Kb, int S, int ldo) {
+    constexpr int PA = NT >= 6 ? 3 : 2;  // parts of A in use
+    constexpr int PB = NT >= 4 ? 3 : 2;  // parts of B in use
+    __shared__ bf16x8 afrag[2][PA][4][64];
     const int g = blockIdx.x;
     const int xcd = g & 7;
     const int jj = g >> 3;
     const int nl = jj / nMt;
     const int mt = jj - nl * nMt;
-    const int nt = nl * 8 + xcd;
-    if (nt >= nNt2) return;
-    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int nt = nt0 + nl * 8 + xcd;
+    if (nt >= nNt1) return;
+    const int t = threadIdx.x;
+    const int lane = t & 63, wave = t >> 6;
     const int wm = wave >> 1, wn = wave & 1;
     const int l32 = lane & 31, lh = lane >> 5;
     const int nT32 = ldo / 32;
 
-    const float2 *tk[2];
-#pragma unroll
-    for (int i = 0; i < 2; ++i) tk[i] = tab + (size_t)(mt * WN_GM + wm * 64 + 32 * i + l32) * 3 * S;
-
-    f32x16 acc[2][4];
-#pragma unroll
-    for (int i = 0; i < 2; ++i)
-#pragma unroll
-        for (int j = 0; j < 4; ++j)
-#pragma unroll
-            for (int q = 0; q < 16; ++q) acc[i][j][q] = 0.0f;
-
-    const int ct0 = nt * 8 + wn * 4;
-    for (int kb = 0; kb < Kb; ++kb) {
-        bf16x8 b_h[4], b_m[4], b_l[4];
-        const size_t boff = ((size_t)kb * nT32 + ct0) * 64 + lane;
-#pragma unroll
-        for (int j = 0; j < 4; ++j) {
-            b_h[j] = bh[boff + 64 * j];
-            b_m[j] = bm[boff + 64 * j];
-            if (NT == 6) b_l[j] = bl[boff + 64 * j];
-        }
-        bf16x8 a_h[2], a_m[2], a_l[2];
+    // phase generation role of this thread
+    const int grs = t >> 6;
+    const float2 *tk = tab + (size_t)(mt * WN_GM + 32 * grs + l32) * 3 * S;
+    auto gen = [&](int kb, int buf) {
+        bf16x8 ph[PA];
 #pragma unroll
         for (int rr = 0; rr < 4; ++rr) {
             const int4 id = ridx[kb * WN_RB + 4 * lh + rr];
-#pragma unroll
-            for (int i = 0; i < 2; ++i) {
-                const float2 *t = tk[i];
-                const float2 t1 = t[id.x], t2 = t[S + id.y], t3 = t[2 * S + id.z];
-                const float pr = t1.x * t2.x - t1.y * t2.y, pi = t1.x * t2.y + t1.y * t2.x;
-                const float fr = pr * t3.x - pi * t3.y, fi = pr * t3.y + pi * t3.x;
-                __bf16 h0, m0, l0, h1, m1, l1;
-                wn_split_phase(fr, h0, m0, l0);
-                wn_split_phase(fi, h1, m1, l1);
-                a_h[i][2 * rr] = h0;
-                a_h[i][2 * rr + 1] = h1;
-                a_m[i][2 * rr] = m0;
-                a_m[i][2 * rr + 1] = m1;
-                if (NT == 6) {
-                    a_l[i][2 * rr] = l0;
-                    a_l[i][2 * rr + 1] = l1;
-                }
+            const float2 t1 = tk[id.x], t2 = tk[S + id.y], t3 = tk[2 * S + id.z];
+            const float pr = t1.x * t2.x - t1.y * t2.y, pi = t1.x * t2.y + t1.y * t2.x;
+            const float fr = pr * t3.x - pi * t3.y, fi = pr * t3.y + pi * t3.x;
+            __bf16 h0, m0, l0, h1, m1, l1;
+            wn_split_phase(fr, h0, m0, l0);
+            wn_split_phase(fi, h1, m1, l1);
+            ph[0][2 * rr] = h0;
+            ph[0][2 * rr + 1] = h1;
+            ph[1][2 * rr] = m0;
+            ph[1][2 * rr + 1] = m1;
+            if (PA == 3) {
+                ph[PA - 1][2 * rr] = l0;
+                ph[PA - 1][2 * rr + 1] = l1;
             }
         }
 #pragma unroll
+        for (int p = 0; p < PA; ++p) afrag[buf][p][grs][lane] = ph[p];
+    };
+
+    f32x16 acc[2][2];
+    double acc64[2][2][16];
+#pragma unroll
+    for (int i = 0; i < 2; ++i)
+#pragma unroll
+        for (int j = 0; j < 2; ++j)
+#pragma unroll
+            for (int q = 0; q < 16; ++q) {
+                acc[i][j][q] = 0.0f;
+                acc64[i][j][q] = 0.0;
+            }
+
+    const int ct0 = nt * 4 + wn * 2;
+    bf16x8 bc[PB][2], bn[PB][2];
+    auto loadb = [&](int kb, bf16x8 (&dst)[PB][2]) {
+        const size_t boff = ((size_t)kb * nT32 + ct0) * 64 + lane;
+#pragma unroll
+        for (int j = 0; j < 2; ++j) {
+            dst[0][j] = bh[boff + 64 * j];
+            dst[1][j] = bm[boff + 64 * j];
+            if (PB == 3) dst[PB - 1][j] = bl[boff + 64 * j];
+        }
+    };
+    loadb(0, bc);
+    gen(0, 0);
+    __syncthreads();
+    for (int kb = 0; kb < Kb; ++kb) {
+        const int buf = kb & 1;
+        if (kb + 1 < Kb) {
+            loadb(kb + 1, bn);
+            gen(kb + 1, buf ^ 1);
+        }
+        bf16x8 a[PA][2];
+#pragma unroll
+        for (int p = 0; p < PA; ++p)
+#pragma unroll
+            for (int i = 0; i < 2; ++i) a[p][i] = afrag[buf][p][2 * wm + i][lane];
+        // small terms first
+#pragma unroll
         for (int i = 0; i < 2; ++i)
 #pragma unroll
-            for (int j = 0; j < 4; ++j) {
-                acc[i][j] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a_m[i], b_h[j], acc[i][j], 0, 0, 0);
-                acc[i][j] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a_h[i], b_m[j], acc[i][j], 0, 0, 0);
-                if (NT == 6) {
-                    acc[i][j] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a_m[i], b_m[j], acc[i][j], 0, 0, 0);
-                    acc[i][j] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a_l[i], b_h[j], acc[i][j], 0, 0, 0);
-                    acc[i][j] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a_h[i], b_l[j], acc[i][j], 0, 0, 0);
-                }
-                acc[i][j] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a_h[i], b_h[j], acc[i][j], 0, 0, 0);
+            for (int j = 0; j < 2; ++j) {
+                if (NT >= 6) acc[i][j] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a[PA - 1][i], bc[0][j], acc[i][j], 0, 0, 0);
+                if (NT >= 5) acc[i][j] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a[1][i], bc[1][j], acc[i][j], 0, 0, 0);
+                if (NT >= 4) acc[i][j] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a[0][i], bc[PB - 1][j], acc[i][j], 0, 0, 0);
             }
+#pragma unroll
+        for (int i = 0; i < 2; ++i)
+#pragma unroll
+            for (int j = 0; j < 2; ++j) {
+                acc[i][j] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a[1][i], bc[0][j], acc[i][j], 0, 0, 0);
+                acc[i][j] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a[0][i], bc[1][j], acc[i][j], 0, 0, 0);
+            }
+#pragma unroll
+        for (int i = 0; i < 2; ++i)
+#pragma unroll
+            for (int j = 0; j < 2; ++j)
+                acc[i][j] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a[0][i], bc[0][j], acc[i][j], 0, 0, 0);
+        if ((kb % WN_FLUSH) == WN_FLUSH - 1 || kb + 1 == Kb) {
+#pragma unroll
+            for (int i = 0; i < 2; ++i)
+#pragma unroll
+                for (int j = 0; j < 2; ++j)
+#pragma unroll
+                    for (int q = 0; q < 16; ++q) {
+                        acc64[i][j][q] += (double)acc[i][j][q];
+                        acc[i][j][q] = 0.0f;
+                    }
+        }
+        if (kb + 1 < Kb) {
+#pragma unroll
+            for (int p = 0; p < PB; ++p)
+#pragma unroll
+                for (int j = 0; j < 2; ++j) bc[p][j] = bn[p][j];
+        }
+        __syncthreads();
     }
 #pragma unroll
     for (int i = 0; i < 2; ++i)
 #pragma unroll
         for (int q = 0; q < 16; ++q) {
             const size_t row = (size_t)mt * WN_GM + wm * 64 + 32 * i + (q >> 2) * 8 + lh * 4 + (q & 3);
-            double *o = out + row * ldo + (size_t)nt * 256 + wn * 128 + l32;
+            double *o = out + row * ldo + (size_t)nt * 128 + wn * 64 + l32;
 #pragma unroll
-            for (int j = 0; j < 4; ++j) o[32 * j] = (double)acc[i][j][q];
+            for (int j = 0; j < 2; ++j) o[32 * j] = acc64[i][j][q];
         }
 }
 
 hipError_t wn_launch_interp_bf16(const __bf16 *bh, const __bf16 *bm, const __bf16 *bl, const float2 *tab,
-                                 const int4 *ridx, double *out, int rows_pad, int nterms, const WnShape &sh,
+                                 const int4 *ridx, double *out, int rows_pad, int nterms, int col0, const WnShape &sh,
                                  hipStream_t st) {
     const int nMt = rows_pad / WN_GM;
-    const int nNt2 = sh.ldo / 256;
-    const int panels = (nNt2 + 7) / 8;
+    const int nNt1 = sh.ldo / 128;
+    const int nt0 = col0 / 128;
+    const int panels = (nNt1 - nt0 + 7) / 8;
+    if (panels <= 0) return hipSuccess;
     const long long grid = 8LL * panels * nMt;
-    if (nterms == 6)
-        hipLaunchKernelGGL(k_interp_bf16<6>, dim3((unsigned)grid), dim3(256), 0, st, (const bf16x8 *)bh,
-                           (const bf16x8 *)bm, (const bf16x8 *)bl, tab, ridx, out, nMt, nNt2, sh.Kb, sh.S, sh.ldo);
+#define WN_BF16_CASE(T)                                                                                        \
+    if (nterms == T) {                                                                                         \
+        hipLaunchKernelGGL(k_interp_bf16<T>, dim3((unsigned)grid), dim3(256), 0, st, (const bf16x8 *)bh,        \
+                           (const bf16x8 *)bm, (const bf16x8 *)bl, tab, ridx, out, nMt, nt0, nNt1, sh.Kb, sh.S,  \
+                           sh.ldo);                                                                            \
+        return hipGetLastError();                                                                              \
+    }
+    WN_BF16_CASE(3)
+    WN_BF16_CASE(4)
+    WN_BF16_CASE(5)
+    WN_BF16_CASE(6)
+#undef WN_BF16_CASE
+    return hipErrorInvalidValue;
+}
+
+// ------------------------------------------------------------------------------------------------
+// Split-FP16 MFMA interpolation GEMM (the mixed precision form).
+//
+// Both operands are split into two FP16 parts, x ~ x_h + x_m with x_h = fp16(x), x_m = fp16(x - x_h),
+// i.e. ~22 significant bits.  Coefficient columns are pre-scaled by an exact power of two so that
+// their largest entry lies in [2^14, 2^15) (FP16 normal range for everything that matters); the
+// scale is removed when the result is written.  v_mfma_f32_32x32x16_f16 accumulates
+//   NT=3: hh + hm + mh       NT=4: + mm
+// in FP32 within a K block; FP32 accumulators are flushed into FP64 every WN_FLUSH K blocks.
+// Workgroup 256 threads = 4 waves stacked in k, tile 128 k x 64 real columns, wave tile 32 x 64
+// (1x2 subtiles): each wave generates the Fourier factors of its own 32 rows (no sharing, no LDS,
+// no barriers) from the per-k (R1,R2)-plane table T12 and the R3 table T3, and all four waves read
+// the same B fragments (L1 reuse).  XCD-aware tile order as in the FP64 kernel.
+// ------------------------------------------------------------------------------------------------
+template <int NT>
+__global__ __launch_bounds__(256, 2) void k_interp_f16(const f16x8 *__restrict__ bhp, const f16x8 *__restrict__ bmp,
+                                                      const double *__restrict__ cscale,
+                                                      const float2 *__restrict__ t12, const float2 *__restrict__ t3,
+                                                      const int2 *__restrict__ ridx2, double *__restrict__ out,
+                                                      int nMt, int nt0, int nNt64, int Kb, int S, int ldo) {
+    const int g = blockIdx.x;
+    const int xcd = g & 7;
+    const int jj = g >> 3;
+    const int nl = jj / nMt;
+    const int mt = jj - nl * nMt;
+    const int nt = nt0 + nl * 8 + xcd;
+    if (nt >= nNt64) return;
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int l32 = lane & 31, lh = lane >> 5;
+    const int nT32 = ldo / 32;
+    const int row = mt * WN_GM + wave * 32 + l32;
+    const float2 *p12 = t12 + (size_t)row * S * S;
+    const float2 *p3 = t3 + (size_t)row * S;
+
+    f32x16 acc[2];
+    double acc64[2][16];
+#pragma unroll
+    for (int j = 0; j < 2; ++j)
+#pragma unroll
+        for (int q = 0; q < 16; ++q) {
+            acc[j][q] = 0.0f;
+            acc64[j][q] = 0.0;
+        }
+    const int ct0 = nt * 2;
+    f16x8 bh[2], bm[2], nh[2], nm[2];
+    {
+        const size_t boff = (size_t)ct0 * 64 + lane;
+#pragma unroll
+        for (int j = 0; j < 2; ++j) {
+            bh[j] = bhp[boff + 64 * j];
+            bm[j] = bmp[boff + 64 * j];
+        }
+    }
+    for (int kb = 0; kb < Kb; ++kb) {
+        if (kb + 1 < Kb) {
+            const size_t boff = ((size_t)(kb + 1) * nT32 + ct0) * 64 + lane;
+#pragma unroll
+            for (int j = 0; j < 2; ++j) {
+                nh[j] = bhp[boff + 64 * j];
+                nm[j] = bmp[boff + 64 * j];
+            }
+        }
+        f16x8 ah, am;
+#pragma unroll
+        for (int rr = 0; rr < 4; ++rr) {
+            const int2 id = ridx2[kb * WN_RB + 4 * lh + rr];
+            const float2 a = p12[id.x], b = p3[id.y];
+            const float fr = a.x * b.x - a.y * b.y, fi = a.x * b.y + a.y * b.x;
+            const _Float16 hr = (_Float16)fr, hi = (_Float16)fi;
+            ah[2 * rr] = hr;
+            ah[2 * rr + 1] = hi;
+            am[2 * rr] = (_Float16)(fr - (float)hr);
+            am[2 * rr + 1] = (_Float16)(fi - (float)hi);
+        }
+#pragma unroll
+        for (int j = 0; j < 2; ++j) {
+            if (NT >= 4) acc[j] = __builtin_amdgcn_mfma_f32_32x32x16_f16(am, bm[j], acc[j], 0, 0, 0);
+            acc[j] = __builtin_amdgcn_mfma_f32_32x32x16_f16(am, bh[j], acc[j], 0, 0, 0);
+        }
+#pragma unroll
+        for (int j = 0; j < 2; ++j) acc[j] = __builtin_amdgcn_mfma_f32_32x32x16_f16(ah, bm[j], acc[j], 0, 0, 0);
+#pragma unroll
+        for (int j = 0; j < 2; ++j) acc[j] = __builtin_amdgcn_mfma_f32_32x32x16_f16(ah, bh[j], acc[j], 0, 0, 0);
+        if ((kb % WN_FLUSH) == WN_FLUSH - 1 || kb + 1 == Kb) {
+#pragma unroll
+            for (int j = 0; j < 2; ++j)
+#pragma unroll
+                for (int q = 0; q < 16; ++q) {
+                    acc64[j][q] += (double)acc[j][q];
+                    acc[j][q] = 0.0f;
+                }
+        }
+        if (kb + 1 < Kb) {
+#pragma unroll
+            for (int j = 0; j < 2; ++j) {
+                bh[j] = nh[j];
+                bm[j] = nm[j];
+            }
+        }
+    }
+#pragma unroll
+    for (int j = 0; j < 2; ++j) {
+        const int col = nt * 64 + 32 * j + l32;
+        const double sc = cscale[col];
+#pragma unroll
+        for (int q = 0; q < 16; ++q) {
+            const size_t r = (size_t)mt * WN_GM + wave * 32 + (q >> 2) * 8 + lh * 4 + (q & 3);
+            out[r * ldo + col] = acc64[j][q] * sc;
+        }
+    }
+}
+
+hipError_t wn_launch_interp_f16(const _Float16 *bh, const _Float16 *bm, const double *cscale, const float2 *t12,
+                                const float2 *t3, const int2 *ridx2, double *out, int rows_pad, int nterms, int col0,
+                                const WnShape &sh, hipStream_t st) {
+    const int nMt = rows_pad / WN_GM;
+    const int nNt64 = sh.ldo / 64;
+    const int nt0 = col0 / 64;
+    const int panels = (nNt64 - nt0 + 7) / 8;
+    if (panels <= 0) return hipSuccess;
+    const long long grid = 8LL * panels * nMt;
+    if (nterms >= 4)
+        hipLaunchKernelGGL(k_interp_f16<4>, dim3((unsigned)grid), dim3(256), 0, st, (const f16x8 *)bh,
+                           (const f16x8 *)bm, cscale, t12, t3, ridx2, out, nMt, nt0, nNt64, sh.Kb, sh.S, sh.ldo);
     else
-        hipLaunchKernelGGL(k_interp_bf16<3>, dim3((unsigned)grid), dim3(256), 0, st, (const bf16x8 *)bh,
-                           (const bf16x8 *)bm, (const bf16x8 *)bl, tab, ridx, out, nMt, nNt2, sh.Kb, sh.S, sh.ldo);
+        hipLaunchKernelGGL(k_interp_f16<3>, dim3((unsigned)grid), dim3(256), 0, st, (const f16x8 *)bh,
+                           (const f16x8 *)bm, cscale, t12, t3, ridx2, out, nMt, nt0, nNt64, sh.Kb, sh.S, sh.ldo);
+    return hipGetLastError();
+}
+
+// Per-column power-of-two scale of the coefficient operand: largest |B| of column col into [2^14, 2^15).
+__global__ void k_coeff_colscale(const double *__restrict__ hr, const double *__restrict__ ar,
+                                 const double *__restrict__ rc, const int *__restrict__ deg, double *__restrict__ up,
+                                 double *__restrict__ down, WnShape sh) {
+    const int col = blockIdx.x * blockDim.x + threadIdx.x;
+    if (col >= sh.ldo) return;
+    double m = 0.0;
+    for (int r = 0; r < sh.nr; ++r)
+        for (int part = 0; part < 2; ++part) m = fmax(m, fabs(wn_coeff_entry(hr, ar, rc, deg, sh, r, part, col)));
+    int e = 0;
+    if (m > 0.0) {
+        frexp(m, &e);      // m = f * 2^e, f in [0.5, 1)
+        e = 15 - e;        // m * 2^(15-e) in [2^14, 2^15)
+    }
+    up[col] = ldexp(1.0, e);
+    down[col] = ldexp(1.0, -e);
+}
+
+// Split-FP16 operand of v_mfma_f32_32x32x16_f16 in the fragment order of k_build_coeff_bf16.
+__global__ void k_build_coeff_f16(const double *__restrict__ hr, const double *__restrict__ ar,
+                                  const double *__restrict__ rc, const int *__restrict__ deg,
+                                  const double *__restrict__ up, _Float16 *__restrict__ bh, _Float16 *__restrict__ bm,
+                                  WnShape sh) {
+    const int nT32 = sh.ldo / 32;
+    const long long total = (long long)sh.Kb * nT32 * 512;
+    for (long long e = (long long)blockIdx.x * blockDim.x + threadIdx.x; e < total;
+         e += (long long)gridDim.x * blockDim.x) {
+        const int j = (int)(e & 7);
+        const int lane = (int)((e >> 3) & 63);
+        const long long rest = e >> 9;
+        const int ct = (int)(rest % nT32);
+        const int kb = (int)(rest / nT32);
+        const int kk = 8 * (lane >> 5) + j;
+        const int r = kb * WN_RB + (kk >> 1);
+        const int part = kk & 1;
+        const int col = 32 * ct + (lane & 31);
+        const double x = wn_coeff_entry(hr, ar, rc, deg, sh, r, part, col) * up[col];
+        const _Float16 h = (_Float16)(float)x;
+        bh[e] = h;
+        bm[e] = (_Float16)(float)(x - (double)(float)h);
+    }
+}
+
+hipError_t wn_launch_build_coeff_f16(const double *hr, const double *ar, const double *rc, const int *deg,
+                                     double *up, double *down, _Float16 *bh, _Float16 *bm, const WnShape &sh,
+                                     hipStream_t st) {
+    hipLaunchKernelGGL(k_coeff_colscale, dim3((sh.ldo + 255) / 256), dim3(256), 0, st, hr, ar, rc, deg, up, down, sh);
+    hipError_t err = hipGetLastError();
+    if (err != hipSuccess) return err;
+    long long total = (long long)sh.Kb * (sh.ldo / 32) * 512;
+    long long blocks = (total + 255) / 256;
+    if (blocks > 65536) blocks = 65536;
+    hipLaunchKernelGGL(k_build_coeff_f16, dim3((unsigned)blocks), dim3(256), 0, st, hr, ar, rc, deg, up, bh, bm, sh);
+    return hipGetLastError();
+}
+
+// Per-k tables for the FP16 path: T12[k][j1*S + j2] = T[k][0][j1] T[k][1][j2] (FP64 product rounded to FP32)
+// and T3[k][j3] = T[k][2][j3].
+__global__ void k_phase_planes(const double2 *__restrict__ tab, float2 *__restrict__ t12, float2 *__restrict__ t3,
+                               int rows, int S) {
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    const int per = S * S + S;
+    if (t >= rows * per) return;
+    const int k = t / per, e = t - k * per;
+    const double2 *tk = tab + (size_t)k * 3 * S;
+    if (e < S * S) {
+        const int j1 = e / S, j2 = e - j1 * S;
+        const double2 v = cmul(tk[j1], tk[S + j2]);
+        t12[(size_t)k * S * S + e] = make_float2((float)v.x, (float)v.y);
+    } else {
+        const double2 v = tk[2 * S + (e - S * S)];
+        t3[(size_t)k * S + (e - S * S)] = make_float2((float)v.x, (float)v.y);
+    }
+}
+
+hipError_t wn_launch_phase_planes(const double2 *tab, float2 *t12, float2 *t3, int rows, int S, hipStream_t st) {
+    const int total = rows * (S * S + S);
+    hipLaunchKernelGGL(k_phase_planes, dim3((total + 255) / 256), dim3(256), 0, st, tab, t12, t3, rows, S);
     return hipGetLastError();
 }
 
@@ -434,46 +707,49 @@ hipError_t wn_launch_interp_bf16(const __bf16 *bh, const __bf16 *bm, const __bf1
 // ||offdiag(H)||_F <= tol ||H||_F.  Eigenvalues are sorted ascending and every eigenvector
 // column is rotated so that its first largest-magnitude entry is real and non-negative.
 // ------------------------------------------------------------------------------------------------
+template <int N>
 __global__ __launch_bounds__(64) void k_eig_f64(const double *__restrict__ x, double *__restrict__ eout,
                                                double2 *__restrict__ uout, int *__restrict__ sweeps_out, int n,
                                                int ldo, double tol) {
-    extern __shared__ double2 sm[];
+    // N: compile-time bound (even, >= n) so that every index computation divides by constants.
+    constexpr int LD = N + 1;
+    constexpr int HALF = N / 2;
+    constexpr int ITEMS = N * HALF;
+    __shared__ double2 H[N * LD];
+    __shared__ double2 V[N * LD];
+    __shared__ double2 rotA[HALF];   // (c, s)
+    __shared__ double2 rotB[HALF];   // (cos phi, sin phi)
+    __shared__ int2 pq[HALF];
+    __shared__ double ev[N];
+    __shared__ int rank[N];
     const int k = blockIdx.x;
     const int t = threadIdx.x;
-    const int ld = n + 1;
-    const int np = n + (n & 1);
-    const int half = np / 2;
-    double2 *H = sm;
-    double2 *V = H + n * ld;
-    double2 *rotA = V + n * ld;   // (c, s)
-    double2 *rotB = rotA + half;  // (cos phi, sin phi)
-    int2 *pq = (int2 *)(rotB + half);
-    double *ev = (double *)(pq + half);
-    int *rank = (int *)(ev + n);
 
     const double2 *xk = (const double2 *)(x + (size_t)k * ldo);
-    for (int e = t; e < n * n; e += 64) {
-        int i = e / n, j = e - i * n;
-        if (i <= j) {
-            double2 h = xk[i * n + j];
-            if (i == j) {
-                H[i * ld + i] = make_double2(h.x, 0.0);
+    for (int e = t; e < N * N; e += 64) {
+        const int i = e / N, j = e - (e / N) * N;
+        double2 hv = make_double2(0.0, 0.0);
+        if (i < n && j < n) {
+            if (i <= j) {
+                hv = xk[i * n + j];
+                if (i == j) hv.y = 0.0;
             } else {
-                H[i * ld + j] = h;
-                H[j * ld + i] = cconj(h);
+                hv = cconj(xk[j * n + i]);
             }
         }
-        V[i * ld + j] = make_double2(i == j ? 1.0 : 0.0, 0.0);
+        H[i * LD + j] = hv;
+        V[i * LD + j] = make_double2(i == j ? 1.0 : 0.0, 0.0);
     }
     __syncthreads();
 
     int sweep = 0;
+    double prev_off = 0.0;
     for (; sweep < 40; ++sweep) {
         double off = 0.0, dia = 0.0;
-        for (int e = t; e < n * n; e += 64) {
-            int i = e / n, j = e - i * n;
-            double2 h = H[i * ld + j];
-            double m2 = h.x * h.x + h.y * h.y;
+        for (int e = t; e < N * N; e += 64) {
+            const int i = e / N, j = e - (e / N) * N;
+            const double2 hv = H[i * LD + j];
+            const double m2 = hv.x * hv.x + hv.y * hv.y;
             if (i == j)
                 dia += m2;
             else
@@ -483,28 +759,33 @@ __global__ __launch_bounds__(64) void k_eig_f64(const double *__restrict__ x, do
             off += __shfl_xor(off, o);
             dia += __shfl_xor(dia, o);
         }
+        // converged, or at the rounding floor (a sweep no longer reduces the off-diagonal mass
+        // quadratically once it is below ~1e-20 of the total)
         if (off <= tol * tol * (dia + off)) break;
-        for (int r = 0; r < np - 1; ++r) {
-            if (t < half) {
-                int p = (t == 0) ? 0 : ((t - 1 + r) % (np - 1)) + 1;
-                int q = ((np - 2 - t + r) % (np - 1)) + 1;
+        if (sweep > 0 && off > 0.25 * prev_off && off <= 1e-20 * (dia + off)) break;
+        prev_off = off;
+        for (int r = 0; r < N - 1; ++r) {
+            if (t < HALF) {
+                int p = (t == 0) ? 0 : ((t - 1 + r) % (N - 1)) + 1;
+                int q = ((N - 2 - t + r) % (N - 1)) + 1;
                 if (p > q) {
-                    int tmp = p;
+                    const int tmp = p;
                     p = q;
                     q = tmp;
                 }
                 double c = 1.0, s = 0.0, cph = 1.0, sph = 0.0;
                 if (q < n) {
-                    double a = H[p * ld + p].x, d = H[q * ld + q].x;
-                    double2 h = H[p * ld + q];
-                    double b = hypot(h.x, h.y);
+                    const double a = H[p * LD + p].x, d = H[q * LD + q].x;
+                    const double2 h = H[p * LD + q];
+                    const double b = hypot(h.x, h.y);
                     if (b > 0.0) {
-                        double tau = (d - a) / (2.0 * b);
-                        double tt = (tau >= 0.0 ? 1.0 : -1.0) / (fabs(tau) + sqrt(1.0 + tau * tau));
+                        const double tau = (d - a) / (2.0 * b);
+                        const double tt = (tau >= 0.0 ? 1.0 : -1.0) / (fabs(tau) + sqrt(1.0 + tau * tau));
                         c = 1.0 / sqrt(1.0 + tt * tt);
                         s = tt * c;
-                        cph = h.x / b;
-                        sph = h.y / b;
+                        const double ib = 1.0 / b;
+                        cph = h.x * ib;
+                        sph = h.y * ib;
                     }
                 } else {
                     q = -1;
@@ -515,62 +796,59 @@ __global__ __launch_bounds__(64) void k_eig_f64(const double *__restrict__ x, do
             }
             __syncthreads();
             // columns: H <- H J, V <- V J
-            for (int e = t; e < n * half; e += 64) {
-                int pr = e / n, i = e - pr * n;
-                int2 ij = pq[pr];
-                if (ij.y < 0) continue;
-                const double2 ra = rotA[pr], rb = rotB[pr];
-                const double4 R = make_double4(ra.x, ra.y, rb.x, rb.y);
-                if (R.y == 0.0) continue;
+#pragma unroll 2
+            for (int e = t; e < ITEMS; e += 64) {
+                const int pr = e / N, i = e - (e / N) * N;
+                const int2 ij = pq[pr];
+                const double2 ra = rotA[pr];
+                if (ij.y < 0 || ra.y == 0.0 || i >= n) continue;
+                const double2 rb = rotB[pr];
                 {
-                    double2 xp = H[i * ld + ij.x], yq = H[i * ld + ij.y];
-                    // e^{-i phi} y
-                    double2 ey = make_double2(R.z * yq.x + R.w * yq.y, R.z * yq.y - R.w * yq.x);
-                    H[i * ld + ij.x] = make_double2(R.x * xp.x - R.y * ey.x, R.x * xp.y - R.y * ey.y);
-                    H[i * ld + ij.y] = make_double2(R.y * xp.x + R.x * ey.x, R.y * xp.y + R.x * ey.y);
+                    const double2 xp = H[i * LD + ij.x], yq = H[i * LD + ij.y];
+                    const double2 ey = make_double2(rb.x * yq.x + rb.y * yq.y, rb.x * yq.y - rb.y * yq.x);
+                    H[i * LD + ij.x] = make_double2(ra.x * xp.x - ra.y * ey.x, ra.x * xp.y - ra.y * ey.y);
+                    H[i * LD + ij.y] = make_double2(ra.y * xp.x + ra.x * ey.x, ra.y * xp.y + ra.x * ey.y);
                 }
                 {
-                    double2 xp = V[i * ld + ij.x], yq = V[i * ld + ij.y];
-                    double2 ey = make_double2(R.z * yq.x + R.w * yq.y, R.z * yq.y - R.w * yq.x);
-                    V[i * ld + ij.x] = make_double2(R.x * xp.x - R.y * ey.x, R.x * xp.y - R.y * ey.y);
-                    V[i * ld + ij.y] = make_double2(R.y * xp.x + R.x * ey.x, R.y * xp.y + R.x * ey.y);
+                    const double2 xp = V[i * LD + ij.x], yq = V[i * LD + ij.y];
+                    const double2 ey = make_double2(rb.x * yq.x + rb.y * yq.y, rb.x * yq.y - rb.y * yq.x);
+                    V[i * LD + ij.x] = make_double2(ra.x * xp.x - ra.y * ey.x, ra.x * xp.y - ra.y * ey.y);
+                    V[i * LD + ij.y] = make_double2(ra.y * xp.x + ra.x * ey.x, ra.y * xp.y + ra.x * ey.y);
                 }
             }
             __syncthreads();
             // rows: H <- J^+ H
-            for (int e = t; e < n * half; e += 64) {
-                int pr = e / n, j = e - pr * n;
-                int2 ij = pq[pr];
-                if (ij.y < 0) continue;
-                const double2 ra = rotA[pr], rb = rotB[pr];
-                const double4 R = make_double4(ra.x, ra.y, rb.x, rb.y);
-                if (R.y == 0.0) continue;
-                double2 xp = H[ij.x * ld + j], yq = H[ij.y * ld + j];
-                // e^{i phi} y
-                double2 ey = make_double2(R.z * yq.x - R.w * yq.y, R.z * yq.y + R.w * yq.x);
-                H[ij.x * ld + j] = make_double2(R.x * xp.x - R.y * ey.x, R.x * xp.y - R.y * ey.y);
-                H[ij.y * ld + j] = make_double2(R.y * xp.x + R.x * ey.x, R.y * xp.y + R.x * ey.y);
+#pragma unroll 2
+            for (int e = t; e < ITEMS; e += 64) {
+                const int pr = e / N, j = e - (e / N) * N;
+                const int2 ij = pq[pr];
+                const double2 ra = rotA[pr];
+                if (ij.y < 0 || ra.y == 0.0 || j >= n) continue;
+                const double2 rb = rotB[pr];
+                const double2 xp = H[ij.x * LD + j], yq = H[ij.y * LD + j];
+                const double2 ey = make_double2(rb.x * yq.x - rb.y * yq.y, rb.x * yq.y + rb.y * yq.x);
+                H[ij.x * LD + j] = make_double2(ra.x * xp.x - ra.y * ey.x, ra.x * xp.y - ra.y * ey.y);
+                H[ij.y * LD + j] = make_double2(ra.y * xp.x + ra.x * ey.x, ra.y * xp.y + ra.x * ey.y);
             }
             __syncthreads();
         }
     }
     // sort ascending (stable by index) and canonicalize columns
-    if (t < n) ev[t] = H[t * ld + t].x;
+    if (t < n) ev[t] = H[t * LD + t].x;
     __syncthreads();
     if (t < n) {
-        double et = ev[t];
+        const double et = ev[t];
         int rk = 0;
         for (int j = 0; j < n; ++j) {
-            double ej = ev[j];
+            const double ej = ev[j];
             rk += (ej < et) || (ej == et && j < t);
         }
         rank[t] = rk;
     }
     __syncthreads();
-    // sorted eigenvectors into H storage
-    for (int e = t; e < n * n; e += 64) {
-        int i = e / n, j = e - i * n;
-        H[i * ld + rank[j]] = V[i * ld + j];
+    for (int e = t; e < N * N; e += 64) {
+        const int i = e / N, j = e - (e / N) * N;
+        if (i < n && j < n) H[i * LD + rank[j]] = V[i * LD + j];
     }
     if (t < n) eout[(size_t)k * n + rank[t]] = ev[t];
     __syncthreads();
@@ -578,36 +856,45 @@ __global__ __launch_bounds__(64) void k_eig_f64(const double *__restrict__ x, do
         int piv = 0;
         double pm = hypot(H[t].x, H[t].y);
         for (int i = 1; i < n; ++i) {
-            double2 v = H[i * ld + t];
-            double m = hypot(v.x, v.y);
+            const double2 v = H[i * LD + t];
+            const double m = hypot(v.x, v.y);
             if (m > pm) {
                 pm = m;
                 piv = i;
             }
         }
         if (pm > 0.0) {
-            double2 v = H[piv * ld + t];
-            double2 ph = make_double2(v.x / pm, -v.y / pm);
-            for (int i = 0; i < n; ++i) H[i * ld + t] = cmul(H[i * ld + t], ph);
+            const double2 v = H[piv * LD + t];
+            const double2 ph = make_double2(v.x / pm, -v.y / pm);
+            for (int i = 0; i < n; ++i) H[i * LD + t] = cmul(H[i * LD + t], ph);
         }
     }
     __syncthreads();
     double2 *uk = uout + (size_t)k * n * n;
     for (int e = t; e < n * n; e += 64) {
-        int i = e / n, j = e - i * n;
-        uk[e] = H[i * ld + j];
+        const int i = e / n, j = e - i * n;
+        uk[e] = H[i * LD + j];
     }
-    if (t == 0 && sweeps_out) sweeps_out[k] = sweep;
+    if (t == 0 && sweeps_out) atomicMax(sweeps_out, sweep);
 }
 
 hipError_t wn_launch_eig_f64(const double *x, double *e, double2 *u, int *sweeps, int count, const WnShape &sh,
                              hipStream_t st) {
     const int n = sh.n;
-    const int half = (n + 1) / 2;
-    size_t lds = (size_t)2 * n * (n + 1) * sizeof(double2) + half * (2 * sizeof(double2) + sizeof(int2)) +
-                 n * (sizeof(double) + sizeof(int));
-    hipLaunchKernelGGL(k_eig_f64, dim3(count), dim3(64), lds, st, x, e, u, sweeps, n, sh.ldo, 1.0e-15);
-    return hipGetLastError();
+    const double tol = 1.0e-15;
+#define WN_EIG_CASE(NN)                                                                                      \
+    if (n <= NN) {                                                                                           \
+        hipLaunchKernelGGL(k_eig_f64<NN>, dim3(count), dim3(64), 0, st, x, e, u, sweeps, n, sh.ldo, tol);    \
+        return hipGetLastError();                                                                            \
+    }
+    WN_EIG_CASE(8)
+    WN_EIG_CASE(16)
+    WN_EIG_CASE(24)
+    WN_EIG_CASE(32)
+    WN_EIG_CASE(48)
+    WN_EIG_CASE(64)
+#undef WN_EIG_CASE
+    return hipErrorInvalidValue;
 }
 
 // ------------------------------------------------------------------------------------------------

@@ -61,7 +61,8 @@ class Stats(C.Structure):
     _fields_ = [("total_ms", C.c_double), ("interp_ms", C.c_double), ("eig_ms", C.c_double),
                 ("rotate_ms", C.c_double), ("pairs_ms", C.c_double), ("accum_ms", C.c_double),
                 ("kpoints", C.c_int64), ("active_pairs", C.c_int64), ("batches", C.c_int32),
-                ("interp_launches", C.c_int32), ("interp_flops", C.c_double)]
+                ("interp_launches", C.c_int32), ("interp_flops", C.c_double), ("max_sweeps", C.c_int32),
+                ("reserved", C.c_int32)]
 
     def as_dict(self) -> dict:
         return {name: getattr(self, name) for name, _ in self._fields_}
