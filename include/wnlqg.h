@@ -52,6 +52,11 @@ extern "C" {
 #define WNLQG_PRECISION_MIXED 1 /* split-BF16 (3 products) MFMA interpolation, FP64 denominators, Kahan k-sum */
 #define WNLQG_PRECISION_MIXED_HIGH 2 /* split-BF16 with 6 products (~FP32 interpolation accuracy) */
 
+/* Interpolation backends for wnlqg_run_mesh (the role of fourier_backend, FourierExecutionPlan.jl:394-480). */
+#define WNLQG_FOURIER_AUTO 0     /* separable plane contraction on meshes */
+#define WNLQG_FOURIER_DIRECT 1   /* k x R contraction on MFMA (the only form for arbitrary k points) */
+#define WNLQG_FOURIER_PLANES 2   /* separable: R3 sum per k3 plane, R2 sum per k2 line, R1 sum per k point */
+
 /*
  * Tight-binding model (TightBindingModel, reference src/Core/Models.jl:11-75; the fields of
  * read_wannier_tb, src/IO/WannierTB.jl:53-111).
@@ -83,7 +88,7 @@ typedef struct wnlqg_task {
     int32_t band_window_size;         /* -1 = all bands (only -1 or a window covering all bands) */
     int32_t precision;                /* WNLQG_PRECISION_* */
     int32_t batch_size;               /* k points per device batch; 0 = automatic */
-    int32_t reserved;
+    int32_t fourier_backend;          /* WNLQG_FOURIER_* (meshes only; points always use the direct contraction) */
 } wnlqg_task;
 
 /* Counters of the last run (device event timers, milliseconds). */
@@ -100,7 +105,8 @@ typedef struct wnlqg_stats {
     int32_t interp_launches;/* launches of the interpolation kernel */
     double interp_flops;    /* algorithmic interpolation flops: 8 * N_R * n^2 * C per k point */
     int32_t max_sweeps;     /* largest Jacobi sweep count over the k points */
-    int32_t reserved;
+    int32_t separable;      /* 1 if the mesh used the separable plane contraction */
+    double interp_bytes;    /* bytes of interpolated channels written (separable contraction) */
 } wnlqg_stats;
 
 typedef struct wnlqg_engine wnlqg_engine;

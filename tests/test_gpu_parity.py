@@ -144,6 +144,7 @@ def test_point_values_finite_temperature_and_eta():
 @pytest.mark.parametrize("name,mesh", [("cfg0_n8", (8, 8, 8)), ("fixture_n4", (5, 3, 2)), ("n32_pos", (4, 3, 3)),
                                        ("n12_pos", (1, 1, 1))])
 def test_mesh_integral(name, mesh):
+    """Default backend (separable plane contraction on meshes) against the oracle."""
     model = MODELS[name]()
     kw = dict(photon_energies=np.linspace(0, 5, 120), broadening=0.04)
     say(f"[mesh] model={name} mesh={mesh}")
@@ -167,17 +168,21 @@ def test_mesh_integral(name, mesh):
     assert stats["kpoints"] == int(np.prod(mesh))
 
 
-def test_mesh_lane_counts_and_batches():
+@pytest.mark.parametrize("backend,batch,mesh,nbatch", [("direct", 128, (6, 5, 7), 2), ("planes", 128, (6, 5, 7), 7),
+                                                      ("planes", 128, (40, 6, 3), 6)])
+def test_mesh_lane_counts_and_batches(backend, batch, mesh, nbatch):
     """Results do not depend on the lane count beyond rounding; small batches exercise the
-    multi-batch chunk plan (lanes straddling batches)."""
+    multi-batch chunk plan (lanes straddling batches, k2 sub-ranges of a plane)."""
     model = MODELS["cfg0_n8"]()
-    kw = dict(photon_energies=np.linspace(0, 5, 60), broadening=0.04, batch_size=128)
-    say("[mesh] cfg0 6x5x7 batch 128, lanes 64 vs 7")
+    kw = dict(photon_energies=np.linspace(0, 5, 60), broadening=0.04, batch_size=batch, fourier_backend=backend)
+    say(f"[mesh] cfg0 {mesh} backend {backend} batch {batch}, lanes 64 vs 7")
     with engine(model, **kw) as eng:
-        l64, t64, st = eng.run_mesh((6, 5, 7))
-        l7, t7, _ = eng.run_mesh((6, 5, 7), n_lanes=7)
-    assert st["batches"] == 2
-    ol, ot = kloop.integrate_mesh(model, oracle_task(**kw), (6, 5, 7), batch=50)
+        l64, t64, st = eng.run_mesh(mesh)
+        l7, t7, _ = eng.run_mesh(mesh, n_lanes=7)
+    assert st["batches"] == nbatch
+    assert st["separable"] == (backend == "planes")
+    okw = {k: v for k, v in kw.items() if k not in ("batch_size", "fourier_backend")}
+    ol, ot = kloop.integrate_mesh(model, oracle_task(**okw), mesh, batch=50)
     for key in ("sc", "ic"):
         assert rel(t64[key], ot[key]) < 1e-10
         assert rel(t7[key], ot[key]) < 1e-10
@@ -212,3 +217,18 @@ def test_mixed_mesh_vs_fp64(prec, tol):
         r = kloop.spectrum_relative_error(tmx[key], t64[key]).max()
         say(f"  {key} worst component rel {r:.3e}")
         assert r < tol
+
+
+@pytest.mark.parametrize("name", ["fixture_n4", "n32_pos"])
+def test_planes_equal_direct(name):
+    """The separable plane contraction reorders the same FP64 sum: channels agree to rounding."""
+    model = MODELS[name]()
+    kw = dict(photon_energies=np.linspace(0, 5, 80), broadening=0.04)
+    say(f"[planes-vs-direct] {name}")
+    with engine(model, fourier_backend="direct", **kw) as ed:
+        _, td, sd = ed.run_mesh((6, 4, 5))
+    with engine(model, fourier_backend="planes", **kw) as ep:
+        _, tp, sp = ep.run_mesh((6, 4, 5))
+    assert sd["separable"] == 0 and sp["separable"] == 1
+    for key in ("sc", "ic"):
+        assert rel(tp[key], td[key]) < 1e-11

@@ -99,6 +99,11 @@ struct wnlqg_engine {
     DevBuf<int2> ridx2;         // per R vector: (j1*S + j2, j3)
     int nterms = 0;             // 0 = FP64 interpolation, else split products per MFMA output
     int kind = 0;               // 0 FP64, 1 split FP16, 2 split BF16
+    int backend = WNLQG_FOURIER_AUTO;
+    bool separable = false;     // meshes use the separable plane contraction
+    int span[3] = {1, 1, 1};    // R box extent per axis
+    DevBuf<double> box;         // [S1*S2*S3][ldo] channel rows of the dense R box (FP64)
+    DevBuf<double2> y3, y23, e1, e2, e3;
     int h_tiles = 0;            // 128-column FP64 tiles holding channel H (mixed forms)
     int partial_cap = 0;  // chunks the partial buffer holds
     std::vector<double> omega_host;
@@ -133,6 +138,8 @@ int check_task(const wnlqg_task *t, int n) {
         t->precision != WNLQG_PRECISION_MIXED_HIGH)
         return fail(WNLQG_E_INVALID, "unknown precision %d", t->precision);
     if (t->batch_size < 0) return fail(WNLQG_E_INVALID, "batch_size must be >= 0");
+    if (t->fourier_backend < WNLQG_FOURIER_AUTO || t->fourier_backend > WNLQG_FOURIER_PLANES)
+        return fail(WNLQG_E_INVALID, "unknown fourier_backend %d", t->fourier_backend);
     return WNLQG_OK;
 }
 
@@ -244,7 +251,8 @@ int32_t wnlqg_destroy(wnlqg_engine *g) {
     g->tab.release(); g->u.release(); g->pmd.release(); g->q.release(); g->partial.release();
     g->lanebuf.release(); g->lanecomp.release(); g->tab32.release(); g->bh.release(); g->bm.release();
     g->bl.release(); g->fh.release(); g->fm.release(); g->cup.release(); g->cdown.release(); g->t12.release();
-    g->t3.release(); g->ridx2.release();
+    g->t3.release(); g->ridx2.release(); g->box.release(); g->y3.release(); g->y23.release(); g->e1.release();
+    g->e2.release(); g->e3.release();
     if (g->stream) (void)hipStreamDestroy(g->stream);
     delete g;
     return WNLQG_OK;
@@ -336,7 +344,11 @@ int32_t wnlqg_create(const wnlqg_model *model, const wnlqg_task *task, wnlqg_eng
             rmax[a] = std::max(rmax[a], model->r_vectors[3 * r + a]);
         }
     g->sh.S = 1;
-    for (int a = 0; a < 3; ++a) g->sh.S = std::max(g->sh.S, rmax[a] - g->rmin[a] + 1);
+    for (int a = 0; a < 3; ++a) {
+        g->span[a] = rmax[a] - g->rmin[a] + 1;
+        g->sh.S = std::max(g->sh.S, g->span[a]);
+    }
+    g->backend = task->fourier_backend;
 
     // host-side R tables: Cartesian R (as _r_vector_cartesian, HamiltonianMatrixElements.jl:74-85)
     std::vector<double> rcart((size_t)3 * nr);
@@ -383,6 +395,35 @@ int32_t wnlqg_create(const wnlqg_model *model, const wnlqg_task *task, wnlqg_eng
     }
     if (be != hipSuccess) return bail(fail(WNLQG_E_HIP, "coefficient build launch failed"));
     if (hipStreamSynchronize(g->stream) != hipSuccess) return bail(fail(WNLQG_E_HIP, "coefficient build failed"));
+
+    // dense R box for the separable mesh contraction (only if every box cell holds at most one R vector
+    // and the spans fit the contraction kernel)
+    if (g->backend != WNLQG_FOURIER_DIRECT) {
+        const long long nbox = (long long)g->span[0] * g->span[1] * g->span[2];
+        std::vector<int> boxmap((size_t)nbox, -1);
+        bool ok = g->span[0] <= 16 && g->span[1] <= 16 && g->span[2] <= 16 && nbox <= 8 * (long long)nr + 4096;
+        for (int r = 0; ok && r < nr; ++r) {
+            const int *R = model->r_vectors + 3 * r;
+            const long long b = ((long long)(R[0] - g->rmin[0]) * g->span[1] + (R[1] - g->rmin[1])) * g->span[2] +
+                                (R[2] - g->rmin[2]);
+            if (boxmap[b] >= 0) ok = false;
+            boxmap[b] = r;
+        }
+        if (!ok && g->backend == WNLQG_FOURIER_PLANES)
+            return bail(fail(WNLQG_E_UNSUPPORTED, "separable contraction needs distinct R vectors in a box of span <= 16"));
+        if (ok) {
+            DevBuf<int> dmap;
+            if (dmap.alloc(boxmap.size()) != hipSuccess || g->box.alloc((size_t)nbox * g->sh.ldo) != hipSuccess ||
+                g->y3.alloc((size_t)g->span[0] * g->span[1] * (g->sh.ldo / 2)) != hipSuccess)
+                return bail(fail(WNLQG_E_OOM, "device allocation for the R box failed"));
+            if (!up(dmap.p, boxmap.data(), boxmap.size() * sizeof(int)) ||
+                wn_launch_build_box(g->hr.p, g->ar.p, g->rc.p, g->deg.p, dmap.p, g->box.p, nbox, g->sh, g->stream) !=
+                    hipSuccess ||
+                hipStreamSynchronize(g->stream) != hipSuccess)
+                return bail(fail(WNLQG_E_HIP, "R box build failed"));
+            g->separable = true;
+        }
+    }
 
     // batch size: interpolation rows + pair buffers within ~24 GB unless the caller fixed it
     const size_t per_k = (size_t)g->sh.ldo * 8 + n2 * (sizeof(int4) + sizeof(double2) + 27 * g->sc.nq * 16) +
@@ -509,12 +550,38 @@ int32_t wnlqg_run_mesh(wnlqg_engine *g, const int32_t mesh[3], int32_t n_lanes, 
     // part of every lane it covers into chunks of at most kpc points, in k order.
     std::vector<long long> lstart(n_lanes + 1);
     for (int l = 0; l <= n_lanes; ++l) lstart[l] = (long long)((__int128)l * nk / n_lanes);
-    const long long nbatch = (nk + g->batch - 1) / g->batch;
+    // batches: contiguous k ranges.  Separable: k2 ranges of one k3 plane (rows k2*N1 + k1).
+    struct Batch {
+        long long k0;
+        int cnt, k3, k2b, k2c;
+    };
+    std::vector<Batch> batches;
+    const bool sep = g->separable && mesh[0] <= g->batch;
+    if (sep) {
+        const int k2step = std::max(1, std::min(mesh[1], g->batch / mesh[0]));
+        for (int k3 = 0; k3 < mesh[2]; ++k3)
+            for (int k2b = 0; k2b < mesh[1]; k2b += k2step) {
+                const int k2c = std::min(k2step, mesh[1] - k2b);
+                batches.push_back({((long long)k3 * mesh[1] + k2b) * mesh[0], k2c * mesh[0], k3, k2b, k2c});
+            }
+        const size_t m2 = g->sh.ldo / 2;
+        if (g->y23.n < (size_t)g->span[0] * k2step * m2) WN_HIP(g->y23.alloc((size_t)g->span[0] * k2step * m2));
+        WN_HIP(g->e1.alloc((size_t)mesh[0] * g->span[0]));
+        WN_HIP(g->e2.alloc((size_t)mesh[1] * g->span[1]));
+        WN_HIP(g->e3.alloc((size_t)mesh[2] * g->span[2]));
+        WN_HIP(wn_launch_axis_phases(g->e1.p, mesh[0], g->span[0], g->rmin[0], g->stream));
+        WN_HIP(wn_launch_axis_phases(g->e2.p, mesh[1], g->span[1], g->rmin[1], g->stream));
+        WN_HIP(wn_launch_axis_phases(g->e3.p, mesh[2], g->span[2], g->rmin[2], g->stream));
+    } else {
+        for (long long k0 = 0; k0 < nk; k0 += g->batch)
+            batches.push_back({k0, (int)std::min<long long>(g->batch, nk - k0), 0, 0, 0});
+    }
+    const long long nbatch = (long long)batches.size();
     std::vector<int4> all_chunks, all_lanes;
     std::vector<long long> bchunk(nbatch + 1), blane(nbatch + 1);
     int max_chunks = 0;
     for (long long bi = 0; bi < nbatch; ++bi) {
-        const long long k0 = bi * g->batch, k1 = std::min(nk, k0 + g->batch);
+        const long long k0 = batches[bi].k0, k1 = k0 + batches[bi].cnt;
         bchunk[bi] = all_chunks.size();
         blane[bi] = all_lanes.size();
         int l = (int)(std::upper_bound(lstart.begin(), lstart.end(), k0) - lstart.begin()) - 1;
@@ -556,16 +623,33 @@ int32_t wnlqg_run_mesh(wnlqg_engine *g, const int32_t mesh[3], int32_t n_lanes, 
         return e;
     };
     std::vector<std::array<hipEvent_t, 7>> marks(nbatch);
+    const long long m2 = g->sh.ldo / 2;
+    double interp_bytes = 0.0;
     for (long long bi = 0; bi < nbatch; ++bi) {
-        const long long k0 = bi * g->batch;
-        const int cnt = (int)std::min<long long>(g->batch, nk - k0);
+        const Batch &bt = batches[bi];
+        const long long k0 = bt.k0;
+        const int cnt = bt.cnt;
         const int rows = (cnt + WN_GM - 1) / WN_GM * WN_GM;
         auto &mk = marks[bi];
         mk[0] = mark();
-        WN_HIP(wn_launch_mesh_kpoints(g->kp.p, mesh, k0, cnt, rows, g->stream));
-        WN_HIP(launch_phases(g, rows));
+        if (!sep) {
+            WN_HIP(wn_launch_mesh_kpoints(g->kp.p, mesh, k0, cnt, rows, g->stream));
+            WN_HIP(launch_phases(g, rows));
+        }
         mk[1] = mark();
-        WN_HIP(launch_interp(g, rows));
+        if (sep) {
+            const double2 *boxc = (const double2 *)g->box.p;
+            if (bt.k2b == 0)
+                WN_HIP(wn_launch_contract(boxc, g->y3.p, g->e3.p, 1, bt.k3, g->span[2], (int)m2,
+                                          g->span[0] * g->span[1], g->span[2] * m2, m2, m2, 0, g->stream));
+            WN_HIP(wn_launch_contract(g->y3.p, g->y23.p, g->e2.p, bt.k2c, bt.k2b, g->span[1], (int)m2, g->span[0],
+                                      g->span[1] * m2, m2, (long long)bt.k2c * m2, m2, g->stream));
+            WN_HIP(wn_launch_contract(g->y23.p, (double2 *)g->x.p, g->e1.p, mesh[0], 0, g->span[0], (int)m2, bt.k2c, m2,
+                                      (long long)bt.k2c * m2, (long long)mesh[0] * m2, m2, g->stream));
+            interp_bytes += (double)cnt * g->sh.ldo * 8.0;
+        } else {
+            WN_HIP(launch_interp(g, rows));
+        }
         mk[2] = mark();
         WN_HIP(wn_launch_eig_f64(g->x.p, g->e.p, g->u.p, g->flags.p + 1, cnt, g->sh, g->stream));
         mk[3] = mark();
@@ -612,6 +696,8 @@ int32_t wnlqg_run_mesh(wnlqg_engine *g, const int32_t mesh[3], int32_t n_lanes, 
     st.batches = (int)nbatch;
     st.interp_launches = (int)nbatch;
     st.interp_flops = 8.0 * g->sh.nr * (double)g->sh.n * g->sh.n * g->sh.C * (double)nk;
+    st.interp_bytes = interp_bytes;
+    st.separable = sep ? 1 : 0;
     if (stats) *stats = st;
 
     // scale: each state * its prefactor, then 1/N_k (consume_point!, IntegralDriver.jl:820-874)

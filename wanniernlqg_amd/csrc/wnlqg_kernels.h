@@ -56,6 +56,12 @@ hipError_t wn_launch_phase_planes(const double2 *tab, float2 *t12, float2 *t3, i
 hipError_t wn_launch_interp_f16(const _Float16 *bh, const _Float16 *bm, const double *cscale, const float2 *t12,
                                 const float2 *t3, const int2 *ridx2, double *out, int rows_pad, int nterms, int col0,
                                 const WnShape &sh, hipStream_t st);
+// Separable mesh interpolation: dense R box of channel rows, per-axis phases, small complex contraction.
+hipError_t wn_launch_build_box(const double *hr, const double *ar, const double *rc, const int *deg, const int *boxmap,
+                               double *box, long long nbox, const WnShape &sh, hipStream_t st);
+hipError_t wn_launch_axis_phases(double2 *E, int N, int span, int rmin, hipStream_t st);
+hipError_t wn_launch_contract(const double2 *in, double2 *out, const double2 *E, int P, int p0, int Q, int M, int nb,
+                              long long ib, long long iq, long long ob, long long op, hipStream_t st);
 // Jacobi eigensolver of the upper triangle of channel 0.
 hipError_t wn_launch_eig_f64(const double *x, double *e, double2 *u, int *sweeps, int count, const WnShape &sh,
                              hipStream_t st);

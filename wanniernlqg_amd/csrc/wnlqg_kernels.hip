@@ -697,6 +697,91 @@ hipError_t wn_launch_phase_planes(const double2 *tab, float2 *t12, float2 *t3, i
 }
 
 // ------------------------------------------------------------------------------------------------
+// Separable interpolation on a uniform mesh.
+//
+// On the Gamma-centred mesh the Fourier factor factorizes, exp(2 pi i R.k) = e1(k1,R1) e2(k2,R2) e3(k3,R3),
+// so X(k) = sum_R1 e1 sum_R2 e2 sum_R3 e3 X(R) is evaluated as three small contractions over a dense
+// box of R vectors (absent R are zero), all in FP64:
+//   A (per k3 plane):  Y3[R1R2][m]       = sum_R3 e3(k3,R3) X[R1R2R3][m]
+//   B (per k2 range):  Y23[R1][k2][m]    = sum_R2 e2(k2,R2) Y3[R1R2][m]
+//   C (per k point):   X[k2][k1][m]      = sum_R1 e1(k1,R1) Y23[R1][k2][m]
+// m runs over the C*n^2 complex channel entries of a k row (the same row layout as the direct GEMM
+// output).  Stage C writes every interpolated matrix once and is bound by HBM writes.
+// ------------------------------------------------------------------------------------------------
+__global__ void k_build_box(const double *__restrict__ hr, const double *__restrict__ ar, const double *__restrict__ rc,
+                            const int *__restrict__ deg, const int *__restrict__ boxmap, double *__restrict__ box,
+                            long long nbox, WnShape sh) {
+    const long long total = nbox * sh.ldo;
+    for (long long e = (long long)blockIdx.x * blockDim.x + threadIdx.x; e < total;
+         e += (long long)gridDim.x * blockDim.x) {
+        const long long rb = e / sh.ldo;
+        const int col = (int)(e - rb * sh.ldo);
+        const int r = boxmap[rb];
+        box[e] = r < 0 ? 0.0 : wn_coeff_entry(hr, ar, rc, deg, sh, r, 0, col);
+    }
+}
+
+hipError_t wn_launch_build_box(const double *hr, const double *ar, const double *rc, const int *deg, const int *boxmap,
+                               double *box, long long nbox, const WnShape &sh, hipStream_t st) {
+    long long total = nbox * sh.ldo;
+    long long blocks = (total + 255) / 256;
+    if (blocks > 65536) blocks = 65536;
+    hipLaunchKernelGGL(k_build_box, dim3((unsigned)blocks), dim3(256), 0, st, hr, ar, rc, deg, boxmap, box, nbox, sh);
+    return hipGetLastError();
+}
+
+// E[i][j] = exp(2 pi i (i/N) (rmin + j)) for one mesh axis (i/N as in integral_kpoint!).
+__global__ void k_axis_phases(double2 *__restrict__ E, int N, int span, int rmin) {
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= N * span) return;
+    const int i = t / span, j = t - i * span;
+    const double k = (double)i / (double)N;
+    double s, c;
+    sincospi(2.0 * k * (double)(rmin + j), &s, &c);
+    E[t] = make_double2(c, s);
+}
+
+hipError_t wn_launch_axis_phases(double2 *E, int N, int span, int rmin, hipStream_t st) {
+    hipLaunchKernelGGL(k_axis_phases, dim3((N * span + 255) / 256), dim3(256), 0, st, E, N, span, rmin);
+    return hipGetLastError();
+}
+
+// out[b*ob + p*op + m] = sum_{q<Q} E[(p0 + p)*Q + q] * in[b*ib + q*iq + m]   (complex, m < M, p < P)
+#define WN_QMAX 16
+__global__ __launch_bounds__(256) void k_contract(const double2 *__restrict__ in, double2 *__restrict__ out,
+                                                 const double2 *__restrict__ E, int P, int p0, int Q, int M,
+                                                 long long ib, long long iq, long long ob, long long op) {
+    extern __shared__ double2 es[];
+    for (int e = threadIdx.x; e < P * Q; e += blockDim.x) es[e] = E[(size_t)p0 * Q + e];
+    __syncthreads();
+    const int m = blockIdx.x * blockDim.x + threadIdx.x;
+    if (m >= M) return;
+    const long long b = blockIdx.y;
+    const double2 *src = in + b * ib + m;
+    double2 v[WN_QMAX];
+#pragma unroll
+    for (int q = 0; q < WN_QMAX; ++q) v[q] = q < Q ? src[q * iq] : make_double2(0.0, 0.0);
+    double2 *dst = out + b * ob + m;
+    for (int p = 0; p < P; ++p) {
+        const double2 *ep = es + p * Q;
+        double2 acc = make_double2(0.0, 0.0);
+#pragma unroll
+        for (int q = 0; q < WN_QMAX; ++q)
+            if (q < Q) acc = cfma(ep[q], v[q], acc);
+        dst[p * op] = acc;
+    }
+}
+
+hipError_t wn_launch_contract(const double2 *in, double2 *out, const double2 *E, int P, int p0, int Q, int M, int nb,
+                              long long ib, long long iq, long long ob, long long op, hipStream_t st) {
+    if (Q > WN_QMAX) return hipErrorInvalidValue;
+    dim3 grid((M + 255) / 256, nb);
+    hipLaunchKernelGGL(k_contract, grid, dim3(256), (size_t)P * Q * sizeof(double2), st, in, out, E, P, p0, Q, M, ib,
+                       iq, ob, op);
+    return hipGetLastError();
+}
+
+// ------------------------------------------------------------------------------------------------
 // Jacobi eigensolver: one wave64 workgroup per k point, H and V in LDS (n <= 64).
 //
 // The upper triangle of channel 0 defines the Hermitian matrix (LAPACK uplo = 'U' semantics;

@@ -54,7 +54,7 @@ class _Task(C.Structure):
                 ("broadening", C.c_double), ("fermi_energy", C.c_double), ("temperature", C.c_double),
                 ("denominator_regularization", C.c_double), ("transition_window_factor", C.c_double),
                 ("band_window_size", C.c_int32), ("precision", C.c_int32), ("batch_size", C.c_int32),
-                ("reserved", C.c_int32)]
+                ("fourier_backend", C.c_int32)]
 
 
 class Stats(C.Structure):
@@ -62,7 +62,7 @@ class Stats(C.Structure):
                 ("rotate_ms", C.c_double), ("pairs_ms", C.c_double), ("accum_ms", C.c_double),
                 ("kpoints", C.c_int64), ("active_pairs", C.c_int64), ("batches", C.c_int32),
                 ("interp_launches", C.c_int32), ("interp_flops", C.c_double), ("max_sweeps", C.c_int32),
-                ("reserved", C.c_int32)]
+                ("separable", C.c_int32), ("interp_bytes", C.c_double)]
 
     def as_dict(self) -> dict:
         return {name: getattr(self, name) for name, _ in self._fields_}
@@ -128,6 +128,7 @@ class EngineTask:
     band_window_size: int = -1
     precision: str = "fp64"
     batch_size: int = 0
+    fourier_backend: str = "auto"      # meshes: "auto"/"planes" (separable) or "direct"
 
 
 class Engine:
@@ -155,7 +156,7 @@ class Engine:
                 "mixed6": WNLQG_PRECISION_MIXED_HIGH}[task.precision]
         t = _Task(q, self._om.shape[0], _ptr(self._om), bt, task.broadening, task.fermi_energy, task.temperature,
                   task.denominator_regularization, task.transition_window_factor, task.band_window_size, prec,
-                  task.batch_size, 0)
+                  task.batch_size, {"auto": 0, "direct": 1, "planes": 2}[task.fourier_backend])
         h = C.c_void_p()
         _check(self.lib, self.lib.wnlqg_create(C.byref(m), C.byref(t), C.byref(h)))
         self.handle = h
