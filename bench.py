@@ -5,10 +5,11 @@ n_w = 32 orbitals, R cube |R_i| <= 5 (1331 R vectors), position operator amplitu
 a = 4 Angstrom cubic cell; conventional shift current + injection current, all 27 abc components,
 400 photon energies in [0, 6] eV, Gaussian width 0.01 eV, mu = 0, T = 0, on the 128^3 Gamma-centred
 mesh (2,097,152 k points).  One step = one full-mesh integral (all stages on the device, model and
-R-space channels already resident).  The FP64 GPU path runs beside the headline path for the
-relative spectrum error; the CPU baseline times the NumPy FP64 oracle on a bounded k sample.
+R-space channels already resident).  The headline runs the default engine configuration (FP64, separable
+mesh interpolation); the plain FP64 k x R contraction and the split-FP16 mixed contraction run beside it
+for speed and relative spectrum error; the CPU baseline times the NumPy FP64 oracle on a bounded k sample.
 
-  python bench.py [--gpus 1] [--steps K] [--warmup W] [--precision mixed|fp64] [--mesh N]
+  python bench.py [--gpus 1] [--steps K] [--warmup W] [--precision mixed|fp64] [--backend auto|direct|planes]
 """
 
 from __future__ import annotations
@@ -50,14 +51,32 @@ def cpu_baseline(model, omega, width, k_sample: int):
     return k_sample / dt, threads, dt
 
 
+def spectrum_relative_error(x, ref, floor=1.0e-12):
+    """Per component ||x - ref||_2 / ||ref||_2 over the omega axis (absolute for components whose norm is below
+    floor times the largest component norm)."""
+    x = np.asarray(x).reshape(x.shape[0], -1)
+    ref = np.asarray(ref).reshape(ref.shape[0], -1)
+    num = np.linalg.norm(x - ref, axis=0)
+    den = np.linalg.norm(ref, axis=0)
+    big = den >= floor * max(den.max(), 1e-300)
+    return np.where(big, num / np.where(big, den, 1.0), num)
+
+
+def jacobi_flops_per_k(n: int, sweeps: float) -> float:
+    """Algorithmic flops of cyclic Jacobi on an n x n Hermitian matrix: per rotation the two H columns,
+    the two V columns and the two H rows are combined (n complex 2x2 updates each, 24 flops per pair)."""
+    return sweeps * (n * (n - 1) / 2) * 3 * n * 24.0
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=2)
+    ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--precision", default="fp64", choices=["mixed", "fp64"])
+    ap.add_argument("--backend", default="auto", choices=["auto", "direct", "planes"])
     ap.add_argument("--mesh", type=int, default=128)
-    ap.add_argument("--no-compare", action="store_true", help="skip the FP64 comparison run")
+    ap.add_argument("--no-compare", action="store_true", help="skip the plain-FP64 and mixed direct comparison runs")
     ap.add_argument("--cpu-sample", type=int, default=1024)
     ap.add_argument("--no-cpu", action="store_true")
     args = ap.parse_args()
@@ -74,14 +93,15 @@ def main():
     width = 0.01
     mesh = (args.mesh,) * 3
     nk = int(np.prod(mesh))
+    n = model.num_orbitals
 
-    def make(prec):
+    def make(prec, backend):
         return E.Engine(model, E.EngineTask(photon_energies=omega, broadening=width, shift_current=True,
-                                            injection_current=True, precision=prec))
+                                            injection_current=True, precision=prec, fourier_backend=backend))
 
-    log(f"[bench] model n=32 N_R={model.num_r_vectors} mesh={mesh} precision={args.precision}")
+    log(f"[bench] model n={n} N_R={model.num_r_vectors} mesh={mesh} precision={args.precision} backend={args.backend}")
     t0 = time.perf_counter()
-    eng = make(args.precision)
+    eng = make(args.precision, args.backend)
     log(f"[bench] engine created in {time.perf_counter() - t0:.2f} s (model upload + R-space channels)")
     for w in range(args.warmup):
         _, tot, st = eng.run_mesh(mesh)
@@ -93,38 +113,61 @@ def main():
         _, tot, st = eng.run_mesh(mesh)
         stats.append(st)
         log(f"[bench] step {s}: device {st['total_ms']:.1f} ms (interp {st['interp_ms']:.1f}, eig {st['eig_ms']:.1f}, "
-            f"rot {st['rotate_ms']:.1f}, pairs {st['pairs_ms']:.1f}, accum {st['accum_ms']:.1f})")
+            f"rot {st['rotate_ms']:.1f}, pairs {st['pairs_ms']:.1f}, accum {st['accum_ms']:.1f}; "
+            f"separable={st['separable']}, max sweeps {st['max_sweeps']})")
     eng.synchronize()
     wall = time.perf_counter() - t0
     ms_per_step = 1e3 * wall / args.steps
     value = nk * args.steps / wall
     head_total = tot
-
-    # roofline of the dominant kernel (interpolation GEMM), from its device event timings
-    st = stats[-1]
-    launches = max(1, st["interp_launches"])
-    interp_avg_s = 1e-3 * st["interp_ms"] / launches
-    flops_per_launch = st["interp_flops"] / launches
-    if args.precision == "fp64":
-        roof = {"bound": "mfma", "achieved": flops_per_launch / interp_avg_s / 1e12, "peak": FP64_PEAK_TFLOPS,
-                "unit": "TFLOP/s"}
-    else:
-        roof = {"bound": "mfma", "achieved": 3.0 * flops_per_launch / interp_avg_s / 1e12, "peak": BF16_PEAK_TFLOPS,
-                "unit": "TFLOP/s"}
-    roof["frac"] = roof["achieved"] / roof["peak"]
-    roof["traffic"] = None
     eng.close()
 
-    compare = None
-    if args.precision != "fp64" and not args.no_compare:
-        ref = make("fp64")
-        _, rtot, rst = ref.run_mesh(mesh)
-        ref.close()
-        from oracle.kloop import spectrum_relative_error  # noqa: E402  (checker only)
-        errs = {k: float(spectrum_relative_error(head_total[k], rtot[k]).max()) for k in rtot}
-        compare = {"fp64_kpoints_per_s": nk / (1e-3 * rst["total_ms"]), "speedup_vs_fp64":
-                   (nk / value) and (1e-3 * rst["total_ms"]) / (wall / args.steps), "max_rel_err": errs}
-        log(f"[bench] fp64 compare: {compare}")
+    # roofline of the dominant kernel, from its device event timings inside run_mesh
+    st = stats[-1]
+    stages = {"interp": st["interp_ms"], "eig": st["eig_ms"], "rotate": st["rotate_ms"], "pairs": st["pairs_ms"],
+              "accum": st["accum_ms"]}
+    dominant = max(stages, key=stages.get)
+    launches = max(1, st["batches"])
+    if dominant == "eig":
+        avg_s = 1e-3 * st["eig_ms"] / launches
+        flops = jacobi_flops_per_k(n, st["max_sweeps"]) * nk / launches
+        roof = {"kernel": "k_eig_f64 (Jacobi, FP64 VALU; FP64 vector and matrix peaks are equal)", "bound": "mfma",
+                "achieved": flops / avg_s / 1e12, "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s"}
+    elif dominant == "interp" and st["separable"]:
+        avg_s = 1e-3 * st["interp_ms"] / launches
+        roof = {"kernel": "k_contract (separable interpolation, writes every interpolated matrix)", "bound": "hbm",
+                "achieved": st["interp_bytes"] / launches / avg_s / 1e9, "peak": HBM_PEAK_GBS, "unit": "GB/s"}
+    elif dominant == "interp":
+        avg_s = 1e-3 * st["interp_ms"] / launches
+        mult = 1.0 if args.precision == "fp64" else 3.0
+        roof = {"kernel": "k_interp (k x R contraction)", "bound": "mfma",
+                "achieved": mult * st["interp_flops"] / launches / avg_s / 1e12,
+                "peak": FP64_PEAK_TFLOPS if args.precision == "fp64" else BF16_PEAK_TFLOPS, "unit": "TFLOP/s"}
+    elif dominant == "rotate":
+        avg_s = 1e-3 * st["rotate_ms"] / launches
+        flops = 16.0 * n ** 3 * 21 * nk / launches
+        roof = {"kernel": "k_rotate_mfma (U^+ X U on FP64 MFMA)", "bound": "mfma", "achieved": flops / avg_s / 1e12,
+                "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s"}
+    else:
+        avg_s = 1e-3 * stages[dominant] / launches
+        roof = {"kernel": dominant, "bound": "mfma", "achieved": None, "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s"}
+    if roof["achieved"] is not None:
+        roof["frac"] = roof["achieved"] / roof["peak"]
+    roof["traffic"] = None
+    roof["avg_launch_ms"] = 1e3 * avg_s
+    roof["launches"] = launches
+
+    compare = {}
+    if not args.no_compare:
+        for label, prec, backend in (("plain_fp64_direct", "fp64", "direct"), ("mixed_f16_direct", "mixed", "direct")):
+            ref = make(prec, backend)
+            _, rtot, rst = ref.run_mesh(mesh)
+            ref.close()
+            errs = {k: float(spectrum_relative_error(rtot[k], head_total[k]).max()) for k in rtot}
+            compare[label] = {"kpoints_per_s": nk / (1e-3 * rst["total_ms"]), "device_ms": rst["total_ms"],
+                              "interp_ms": rst["interp_ms"], "max_rel_spectrum_err_vs_headline": errs}
+            log(f"[bench] {label}: {compare[label]}")
+        compare["headline_speedup_vs_plain_fp64"] = compare["plain_fp64_direct"]["device_ms"] / st["total_ms"]
 
     cpu = None
     if not args.no_cpu:
@@ -145,17 +188,19 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f64" if args.precision == "fp64" else "bf16x3-mfma+f64",
+        "dtype": "f64" if args.precision == "fp64" else "f16x2-split-mfma+f64",
         "data": "synthetic (seeded TB model, no external data)",
-        "config": {"workload": f"cfg1 shift+injection current n_w=32 N_R=1331 {args.mesh}^3 mesh, 400 omega, "
-                               f"27 components, precision={args.precision}",
-                   "kpoints": nk, "n_w": 32, "num_r_vectors": model.num_r_vectors, "n_omega": 400},
+        "config": {"workload": f"shift+injection current, n_w=32, N_R=1331, {args.mesh}^3 Gamma-centred mesh, "
+                               f"400 omega in [0,6] eV, Gaussian 0.01 eV, 27 components",
+                   "kpoints": nk, "n_w": 32, "num_r_vectors": model.num_r_vectors, "n_omega": 400,
+                   "precision": args.precision, "fourier_backend": args.backend,
+                   "separable": bool(st["separable"])},
         "roofline": roof,
         "cpu_baseline": cpu,
         "stages_ms": {k: st[k] for k in ("interp_ms", "eig_ms", "rotate_ms", "pairs_ms", "accum_ms", "total_ms")},
     }
     if compare:
-        out["fp64_compare"] = compare
+        out["compare"] = compare
     print(json.dumps(out), flush=True)
 
 
