@@ -21,6 +21,7 @@
 #include "wnlqg_kernels.h"
 
 #include <math.h>
+#include <stdlib.h>
 
 // ------------------------------------------------------------------------------------------------
 // R-space channels in MFMA fragment order.
@@ -748,6 +749,7 @@ hipError_t wn_launch_axis_phases(double2 *E, int N, int span, int rmin, hipStrea
 
 // out[b*ob + p*op + m] = sum_{q<Q} E[(p0 + p)*Q + q] * in[b*ib + q*iq + m]   (complex, m < M, p < P)
 #define WN_QMAX 16
+#define WN_PCH 64
 __global__ __launch_bounds__(256) void k_contract(const double2 *__restrict__ in, double2 *__restrict__ out,
                                                  const double2 *__restrict__ E, int P, int p0, int Q, int M,
                                                  long long ib, long long iq, long long ob, long long op) {
@@ -792,37 +794,43 @@ hipError_t wn_launch_contract(const double2 *in, double2 *out, const double2 *E,
 // ||offdiag(H)||_F <= tol ||H||_F.  Eigenvalues are sorted ascending and every eigenvector
 // column is rotated so that its first largest-magnitude entry is real and non-negative.
 // ------------------------------------------------------------------------------------------------
-template <int N>
-__global__ __launch_bounds__(64) void k_eig_f64(const double *__restrict__ x, double *__restrict__ eout,
-                                               double2 *__restrict__ uout, int *__restrict__ sweeps_out, int n,
-                                               int ldo, double tol) {
-    // N: compile-time bound (even, >= n) so that every index computation divides by constants.
+// Block form of the parallel Jacobi round.  Only the upper triangle (and diagonal) of H is stored.
+// A round with disjoint planes t = (p_t, q_t) maps every 2x2 block B = H[{p1,q1}][{p2,q2}] to
+// J1^+ B J2, which reads and writes each stored element once; the diagonal blocks become
+// diag(a - t b, d + t b) exactly.  V <- V J runs in the same phase.  Element (i, j) with i > j is
+// read as conj(H[j][i]).
+template <int N, int TPB>
+__global__ __launch_bounds__(TPB) void k_eig_f64(const double *__restrict__ x, double *__restrict__ eout,
+                                                double2 *__restrict__ uout, int *__restrict__ sweeps_out, int n,
+                                                int ldo, double tol) {
     constexpr int LD = N + 1;
     constexpr int HALF = N / 2;
-    constexpr int ITEMS = N * HALF;
-    __shared__ double2 H[N * LD];
+    constexpr int NBLK = HALF * (HALF + 1) / 2;   // blocks t1 <= t2
+    constexpr int VITEMS = N * HALF;
+    constexpr int NW = TPB / 64;
+    constexpr int NP = N * (N + 1) / 2;
+    __shared__ double2 H[NP];       // packed upper triangle, row-major
     __shared__ double2 V[N * LD];
-    __shared__ double2 rotA[HALF];   // (c, s)
-    __shared__ double2 rotB[HALF];   // (cos phi, sin phi)
+    __shared__ double4 rot[HALF];   // (c, s, cos phi, sin phi)
     __shared__ int2 pq[HALF];
+    __shared__ double red[2 * NW];
     __shared__ double ev[N];
     __shared__ int rank[N];
     const int k = blockIdx.x;
     const int t = threadIdx.x;
 
+    auto P = [](int i, int j) { return i * (2 * N - i + 1) / 2 + (j - i); };
     const double2 *xk = (const double2 *)(x + (size_t)k * ldo);
-    for (int e = t; e < N * N; e += 64) {
+    for (int e = t; e < N * N; e += TPB) {
         const int i = e / N, j = e - (e / N) * N;
-        double2 hv = make_double2(0.0, 0.0);
-        if (i < n && j < n) {
-            if (i <= j) {
+        if (i <= j) {
+            double2 hv = make_double2(0.0, 0.0);
+            if (j < n) {
                 hv = xk[i * n + j];
                 if (i == j) hv.y = 0.0;
-            } else {
-                hv = cconj(xk[j * n + i]);
             }
+            H[P(i, j)] = hv;
         }
-        H[i * LD + j] = hv;
         V[i * LD + j] = make_double2(i == j ? 1.0 : 0.0, 0.0);
     }
     __syncthreads();
@@ -831,18 +839,33 @@ __global__ __launch_bounds__(64) void k_eig_f64(const double *__restrict__ x, do
     double prev_off = 0.0;
     for (; sweep < 40; ++sweep) {
         double off = 0.0, dia = 0.0;
-        for (int e = t; e < N * N; e += 64) {
+        for (int e = t; e < N * N; e += TPB) {
             const int i = e / N, j = e - (e / N) * N;
-            const double2 hv = H[i * LD + j];
+            if (i > j) continue;
+            const double2 hv = H[P(i, j)];
             const double m2 = hv.x * hv.x + hv.y * hv.y;
             if (i == j)
                 dia += m2;
             else
-                off += m2;
+                off += 2.0 * m2;
         }
         for (int o = 32; o > 0; o >>= 1) {
             off += __shfl_xor(off, o);
             dia += __shfl_xor(dia, o);
+        }
+        if (NW > 1) {
+            if ((t & 63) == 0) {
+                red[2 * (t >> 6)] = off;
+                red[2 * (t >> 6) + 1] = dia;
+            }
+            __syncthreads();
+            off = 0.0;
+            dia = 0.0;
+            for (int w = 0; w < NW; ++w) {
+                off += red[2 * w];
+                dia += red[2 * w + 1];
+            }
+            __syncthreads();
         }
         // converged, or at the rounding floor (a sweep no longer reduces the off-diagonal mass
         // quadratically once it is below ~1e-20 of the total)
@@ -860,66 +883,99 @@ __global__ __launch_bounds__(64) void k_eig_f64(const double *__restrict__ x, do
                 }
                 double c = 1.0, s = 0.0, cph = 1.0, sph = 0.0;
                 if (q < n) {
-                    const double a = H[p * LD + p].x, d = H[q * LD + q].x;
-                    const double2 h = H[p * LD + q];
-                    const double b = hypot(h.x, h.y);
-                    if (b > 0.0) {
-                        const double tau = (d - a) / (2.0 * b);
-                        const double tt = (tau >= 0.0 ? 1.0 : -1.0) / (fabs(tau) + sqrt(1.0 + tau * tau));
+                    const double a = H[P(p, p)].x, d = H[P(q, q)].x;
+                    const double2 h = H[P(p, q)];
+                    const double b2 = h.x * h.x + h.y * h.y;
+                    if (b2 > 0.0) {
+                        // t = tan(theta) = sign(d-a) 2b / (|d-a| + sqrt((d-a)^2 + 4b^2)), the smaller root of
+                        // t^2 + 2 tau t - 1 = 0 with tau = (d-a)/(2b)
+                        const double b = sqrt(b2);
+                        const double diff = d - a;
+                        const double root = sqrt(diff * diff + 4.0 * b2);
+                        const double tt = (diff >= 0.0 ? 2.0 * b : -2.0 * b) / (fabs(diff) + root);
                         c = 1.0 / sqrt(1.0 + tt * tt);
                         s = tt * c;
                         const double ib = 1.0 / b;
                         cph = h.x * ib;
                         sph = h.y * ib;
+                        H[P(p, p)] = make_double2(a - tt * b, 0.0);
+                        H[P(q, q)] = make_double2(d + tt * b, 0.0);
+                        H[P(p, q)] = make_double2(0.0, 0.0);
                     }
-                } else {
-                    q = -1;
                 }
-                rotA[t] = make_double2(c, s);
-                rotB[t] = make_double2(cph, sph);
+                rot[t] = make_double4(c, s, cph, sph);
                 pq[t] = make_int2(p, q);
             }
             __syncthreads();
-            // columns: H <- H J, V <- V J
-#pragma unroll 2
-            for (int e = t; e < ITEMS; e += 64) {
-                const int pr = e / N, i = e - (e / N) * N;
-                const int2 ij = pq[pr];
-                const double2 ra = rotA[pr];
-                if (ij.y < 0 || ra.y == 0.0 || i >= n) continue;
-                const double2 rb = rotB[pr];
-                {
-                    const double2 xp = H[i * LD + ij.x], yq = H[i * LD + ij.y];
-                    const double2 ey = make_double2(rb.x * yq.x + rb.y * yq.y, rb.x * yq.y - rb.y * yq.x);
-                    H[i * LD + ij.x] = make_double2(ra.x * xp.x - ra.y * ey.x, ra.x * xp.y - ra.y * ey.y);
-                    H[i * LD + ij.y] = make_double2(ra.y * xp.x + ra.x * ey.x, ra.y * xp.y + ra.x * ey.y);
+            // off-diagonal blocks t1 < t2 of H
+            for (int bidx = t; bidx < NBLK; bidx += TPB) {
+                // decode bidx -> (t1, t2) with t1 <= t2 (row-major over the upper triangle of blocks)
+                int t1 = 0, rem = bidx;
+                while (rem >= HALF - t1) {
+                    rem -= HALF - t1;
+                    ++t1;
                 }
-                {
-                    const double2 xp = V[i * LD + ij.x], yq = V[i * LD + ij.y];
-                    const double2 ey = make_double2(rb.x * yq.x + rb.y * yq.y, rb.x * yq.y - rb.y * yq.x);
-                    V[i * LD + ij.x] = make_double2(ra.x * xp.x - ra.y * ey.x, ra.x * xp.y - ra.y * ey.y);
-                    V[i * LD + ij.y] = make_double2(ra.y * xp.x + ra.x * ey.x, ra.y * xp.y + ra.x * ey.y);
+                const int t2 = t1 + rem;
+                if (t2 == t1) continue;
+                const double4 R1 = rot[t1], R2 = rot[t2];
+                if (R1.y == 0.0 && R2.y == 0.0) continue;
+                const int2 a1 = pq[t1], a2 = pq[t2];
+                const int rr[2] = {a1.x, a1.y}, cc[2] = {a2.x, a2.y};
+                double2 B[2][2];
+#pragma unroll
+                for (int u = 0; u < 2; ++u)
+#pragma unroll
+                    for (int v = 0; v < 2; ++v) {
+                        const int i = rr[u], j = cc[v];
+                        B[u][v] = i < j ? H[P(i, j)] : cconj(H[P(j, i)]);
+                    }
+                // B J2: col0 = c2 B0 - s2 e^{-i phi2} B1, col1 = s2 B0 + c2 e^{-i phi2} B1
+                const double2 em2 = make_double2(R2.z, -R2.w);
+#pragma unroll
+                for (int u = 0; u < 2; ++u) {
+                    const double2 y = cmul(em2, B[u][1]);
+                    const double2 x0 = B[u][0];
+                    B[u][0] = make_double2(R2.x * x0.x - R2.y * y.x, R2.x * x0.y - R2.y * y.y);
+                    B[u][1] = make_double2(R2.y * x0.x + R2.x * y.x, R2.y * x0.y + R2.x * y.y);
                 }
+                // J1^+ M: row0 = c1 M0 - s1 e^{i phi1} M1, row1 = s1 M0 + c1 e^{i phi1} M1
+                const double2 ep1 = make_double2(R1.z, R1.w);
+#pragma unroll
+                for (int v = 0; v < 2; ++v) {
+                    const double2 y = cmul(ep1, B[1][v]);
+                    const double2 x0 = B[0][v];
+                    B[0][v] = make_double2(R1.x * x0.x - R1.y * y.x, R1.x * x0.y - R1.y * y.y);
+                    B[1][v] = make_double2(R1.y * x0.x + R1.x * y.x, R1.y * x0.y + R1.x * y.y);
+                }
+#pragma unroll
+                for (int u = 0; u < 2; ++u)
+#pragma unroll
+                    for (int v = 0; v < 2; ++v) {
+                        const int i = rr[u], j = cc[v];
+                        if (i < j)
+                            H[P(i, j)] = B[u][v];
+                        else
+                            H[P(j, i)] = cconj(B[u][v]);
+                    }
             }
-            __syncthreads();
-            // rows: H <- J^+ H
-#pragma unroll 2
-            for (int e = t; e < ITEMS; e += 64) {
-                const int pr = e / N, j = e - (e / N) * N;
+            // V <- V J
+            for (int e = t; e < VITEMS; e += TPB) {
+                const int pr = e / N, i = e - (e / N) * N;
+                const double4 R = rot[pr];
+                if (R.y == 0.0 || i >= n) continue;
                 const int2 ij = pq[pr];
-                const double2 ra = rotA[pr];
-                if (ij.y < 0 || ra.y == 0.0 || j >= n) continue;
-                const double2 rb = rotB[pr];
-                const double2 xp = H[ij.x * LD + j], yq = H[ij.y * LD + j];
-                const double2 ey = make_double2(rb.x * yq.x - rb.y * yq.y, rb.x * yq.y + rb.y * yq.x);
-                H[ij.x * LD + j] = make_double2(ra.x * xp.x - ra.y * ey.x, ra.x * xp.y - ra.y * ey.y);
-                H[ij.y * LD + j] = make_double2(ra.y * xp.x + ra.x * ey.x, ra.y * xp.y + ra.x * ey.y);
+                const double2 xp = V[i * LD + ij.x], yq = V[i * LD + ij.y];
+                const double2 ey = make_double2(R.z * yq.x + R.w * yq.y, R.z * yq.y - R.w * yq.x);
+                V[i * LD + ij.x] = make_double2(R.x * xp.x - R.y * ey.x, R.x * xp.y - R.y * ey.y);
+                V[i * LD + ij.y] = make_double2(R.y * xp.x + R.x * ey.x, R.y * xp.y + R.x * ey.y);
             }
             __syncthreads();
         }
     }
-    // sort ascending (stable by index) and canonicalize columns
-    if (t < n) ev[t] = H[t * LD + t].x;
+    // sort ascending (stable by index); canonical phase per eigenvector column (first largest |v_i|
+    // made real and non-negative), written in sorted order
+    __shared__ double2 phase[N];
+    if (t < n) ev[t] = H[P(t, t)].x;
     __syncthreads();
     if (t < n) {
         const double et = ev[t];
@@ -929,36 +985,29 @@ __global__ __launch_bounds__(64) void k_eig_f64(const double *__restrict__ x, do
             rk += (ej < et) || (ej == et && j < t);
         }
         rank[t] = rk;
-    }
-    __syncthreads();
-    for (int e = t; e < N * N; e += 64) {
-        const int i = e / N, j = e - (e / N) * N;
-        if (i < n && j < n) H[i * LD + rank[j]] = V[i * LD + j];
-    }
-    if (t < n) eout[(size_t)k * n + rank[t]] = ev[t];
-    __syncthreads();
-    if (t < n) {
+        eout[(size_t)k * n + rk] = et;
         int piv = 0;
-        double pm = hypot(H[t].x, H[t].y);
+        double pm = hypot(V[t].x, V[t].y);
         for (int i = 1; i < n; ++i) {
-            const double2 v = H[i * LD + t];
+            const double2 v = V[i * LD + t];
             const double m = hypot(v.x, v.y);
             if (m > pm) {
                 pm = m;
                 piv = i;
             }
         }
+        double2 ph = make_double2(1.0, 0.0);
         if (pm > 0.0) {
-            const double2 v = H[piv * LD + t];
-            const double2 ph = make_double2(v.x / pm, -v.y / pm);
-            for (int i = 0; i < n; ++i) H[i * LD + t] = cmul(H[i * LD + t], ph);
+            const double2 v = V[piv * LD + t];
+            ph = make_double2(v.x / pm, -v.y / pm);
         }
+        phase[t] = ph;
     }
     __syncthreads();
     double2 *uk = uout + (size_t)k * n * n;
-    for (int e = t; e < n * n; e += 64) {
+    for (int e = t; e < n * n; e += TPB) {
         const int i = e / n, j = e - i * n;
-        uk[e] = H[i * LD + j];
+        uk[i * n + rank[j]] = cmul(V[i * LD + j], phase[j]);
     }
     if (t == 0 && sweeps_out) atomicMax(sweeps_out, sweep);
 }
@@ -969,12 +1018,31 @@ hipError_t wn_launch_eig_f64(const double *x, double *e, double2 *u, int *sweeps
     const double tol = 1.0e-15;
 #define WN_EIG_CASE(NN)                                                                                      \
     if (n <= NN) {                                                                                           \
-        hipLaunchKernelGGL(k_eig_f64<NN>, dim3(count), dim3(64), 0, st, x, e, u, sweeps, n, sh.ldo, tol);    \
+        constexpr int TPB = NN >= 32 ? 256 : (NN >= 24 ? 128 : 64);                                          \
+        hipLaunchKernelGGL((k_eig_f64<NN, TPB>), dim3(count), dim3(TPB), 0, st, x, e, u, sweeps, n, sh.ldo, tol); \
         return hipGetLastError();                                                                            \
     }
     WN_EIG_CASE(8)
     WN_EIG_CASE(16)
     WN_EIG_CASE(24)
+    if (n <= 32) {
+        // threads per k point for the 32-band case (tuning knob, default 128)
+        static const int tpb32 = [] {
+            const char *v = getenv("WNLQG_EIG_TPB");
+            return v ? atoi(v) : 256;
+        }();
+        if (tpb32 == 64)
+            hipLaunchKernelGGL((k_eig_f64<32, 64>), dim3(count), dim3(64), 0, st, x, e, u, sweeps, n, sh.ldo, tol);
+        else if (tpb32 == 256)
+            hipLaunchKernelGGL((k_eig_f64<32, 256>), dim3(count), dim3(256), 0, st, x, e, u, sweeps, n, sh.ldo, tol);
+        else if (tpb32 == 512)
+            hipLaunchKernelGGL((k_eig_f64<32, 512>), dim3(count), dim3(512), 0, st, x, e, u, sweeps, n, sh.ldo, tol);
+        else if (tpb32 == 1024)
+            hipLaunchKernelGGL((k_eig_f64<32, 1024>), dim3(count), dim3(1024), 0, st, x, e, u, sweeps, n, sh.ldo, tol);
+        else
+            hipLaunchKernelGGL((k_eig_f64<32, 128>), dim3(count), dim3(128), 0, st, x, e, u, sweeps, n, sh.ldo, tol);
+        return hipGetLastError();
+    }
     WN_EIG_CASE(32)
     WN_EIG_CASE(48)
     WN_EIG_CASE(64)
@@ -1169,7 +1237,7 @@ struct PairView {
 
 __device__ __forceinline__ double2 ld2(const double2 *p, int idx) { return p[idx]; }
 
-__device__ void wn_gderiv(const PairView &pv, const double *E, double eta, int n, int nb, int m, double2 G[3][3]) {
+__device__ void wn_gderiv(const PairView &pv, const double *inv, int n, int nb, int m, double2 G[3][3]) {
     // r^b_{nm;a} for all (a = x, b = y); ConventionalShiftCurrent.jl:191-229
     double2 sAD[3][3], sHD[3][3];
 #pragma unroll
@@ -1179,12 +1247,10 @@ __device__ void wn_gderiv(const PairView &pv, const double *E, double eta, int n
             sAD[x][y] = make_double2(0.0, 0.0);
             sHD[x][y] = make_double2(0.0, 0.0);
         }
-    const double En = E[n], Em = E[m];
     for (int l = 0; l < nb; ++l) {
         if (l == n || l == m) continue;
-        const double El = E[l];
-        const double inl = wn_reg_inverse(En - El, eta);
-        const double ilm = wn_reg_inverse(El - Em, eta);
+        const double inl = inv[n * nb + l];
+        const double ilm = inv[l * nb + m];
         double2 Vnl[3], Vlm[3], Anl[3], Alm[3], Dnl[3], Dlm[3];
 #pragma unroll
         for (int x = 0; x < 3; ++x) {
@@ -1208,7 +1274,7 @@ __device__ void wn_gderiv(const PairView &pv, const double *E, double eta, int n
                 sHD[x][y] = csub(cfma(Vnl[x], Dlm[y], sHD[x][y]), cmul(Dnl[y], Vlm[x]));
             }
     }
-    const double inm = wn_reg_inverse(En - Em, eta);
+    const double inm = inv[n * nb + m];
     double2 Dnm[3], Ann[3], Amm[3], Anm[3], Vnn[3], Vmm[3];
 #pragma unroll
     for (int x = 0; x < 3; ++x) {
@@ -1244,6 +1310,55 @@ __device__ void wn_gderiv(const PairView &pv, const double *E, double eta, int n
         }
 }
 
+// r^y_{nm;x} for y = 0..2 at fixed x (one row of wn_gderiv); ConventionalShiftCurrent.jl:191-229.
+__device__ void wn_gderiv_row(const PairView &pv, const double *inv, int n, int nb, int m, int x, double2 g[3]) {
+    double2 sAD[3], sHD[3];
+#pragma unroll
+    for (int y = 0; y < 3; ++y) {
+        sAD[y] = make_double2(0.0, 0.0);
+        sHD[y] = make_double2(0.0, 0.0);
+    }
+    const double2 *Vx = pv.V[x];
+    const double2 *Ax = pv.has_pos ? pv.A[x] : nullptr;
+    for (int l = 0; l < nb; ++l) {
+        if (l == n || l == m) continue;
+        const double inl = inv[n * nb + l];
+        const double ilm = inv[l * nb + m];
+        const double2 vxnl = Vx[n * nb + l], vxlm = Vx[l * nb + m];
+        const double2 axnl = Ax ? Ax[n * nb + l] : make_double2(0.0, 0.0);
+        const double2 axlm = Ax ? Ax[l * nb + m] : make_double2(0.0, 0.0);
+#pragma unroll
+        for (int y = 0; y < 3; ++y) {
+            const double2 dnl = cscale(pv.V[y][n * nb + l], -inl);
+            const double2 dlm = cscale(pv.V[y][l * nb + m], -ilm);
+            sAD[y] = csub(cfma(axnl, dlm, sAD[y]), cmul(dnl, axlm));
+            sHD[y] = csub(cfma(vxnl, dlm, sHD[y]), cmul(dnl, vxlm));
+        }
+    }
+    const double inm = inv[n * nb + m];
+    double2 Dnm[3], dA[3], dV[3];
+#pragma unroll
+    for (int y = 0; y < 3; ++y) {
+        Dnm[y] = cscale(pv.V[y][n * nb + m], -inm);
+        dV[y] = csub(pv.V[y][n * nb + n], pv.V[y][m * nb + m]);
+        dA[y] = pv.has_pos ? csub(pv.A[y][n * nb + n], pv.A[y][m * nb + m]) : make_double2(0.0, 0.0);
+    }
+    const double2 Anm = pv.has_pos ? pv.A[x][n * nb + m] : make_double2(0.0, 0.0);
+#pragma unroll
+    for (int y = 0; y < 3; ++y) {
+        const double2 ad_bit = cadd(cmul(dA[y], Dnm[x]), cmul(dA[x], Dnm[y]));
+        const double2 aa_bit = cmul(dA[y], Anm);
+        const double2 dv_bit = cadd(cmul(dV[y], Dnm[x]), cmul(dV[x], Dnm[y]));
+        const double2 da = pv.has_pos ? pv.dA[3 * x + y][n * nb + m] : make_double2(0.0, 0.0);
+        const double2 w = pv.W[wn_sym(x, y)][n * nb + m];
+        double2 gg = cadd(da, ad_bit);
+        gg = csub(gg, cmuli(aa_bit));
+        gg = cadd(gg, sAD[y]);
+        const double2 h = cadd(cadd(w, sHD[y]), dv_bit);
+        g[y] = cadd(gg, cscale(cmuli(h), -inm));
+    }
+}
+
 __global__ __launch_bounds__(256) void k_pairs_f64(const double *__restrict__ x, const double *__restrict__ eg,
                                                   const double *__restrict__ omega, int *__restrict__ count,
                                                   int4 *__restrict__ pmeta, double2 *__restrict__ pmd,
@@ -1253,6 +1368,7 @@ __global__ __launch_bounds__(256) void k_pairs_f64(const double *__restrict__ x,
     __shared__ double F[64];
     __shared__ int scan[256];
     __shared__ int kactive;
+    extern __shared__ double inv[];   // [n][n] regularized inverse energy differences
     const int k = blockIdx.x;
     const int t = threadIdx.x;
     const int nb = sh.n;
@@ -1369,6 +1485,11 @@ __global__ __launch_bounds__(256) void k_pairs_f64(const double *__restrict__ x,
     __syncthreads();
     if (!active) return;
 
+    for (int e = t; e < n2; e += 256) {
+        const int i = e / nb, j = e - i * nb;
+        inv[e] = wn_reg_inverse(E[i] - E[j], sc.eta);
+    }
+    __syncthreads();
     // eta == 0: bare inverse requires isolated levels (OrthogonalBandDerivatives.jl:66-82)
     if (sc.eta == 0.0 && t < nb) {
         for (int j = t + 1; j < nb; ++j)
@@ -1387,50 +1508,63 @@ __global__ __launch_bounds__(256) void k_pairs_f64(const double *__restrict__ x,
 #pragma unroll
     for (int a = 0; a < 9; ++a) pv.dA[a] = sh.has_pos ? xk + (size_t)(WN_CH_DA + a) * n2 : nullptr;
     const int ncomp = 27 * sc.nq;
-    for (int p = t; p < total; p += 256) {
-        const int4 mt = pmeta[pbase + p];
-        const int n = mt.x, m = mt.y;
-        double2 *q = qout + (pbase + p) * ncomp;
-        const double inm = wn_reg_inverse(E[n] - E[m], sc.eta);
-        const double imn = wn_reg_inverse(E[m] - E[n], sc.eta);
-        double2 Bnm[3], Bmn[3];
+    // Chunks of WN_PCH pairs.  Phase A: one item per (pair, direction d, component x) forms
+    // r^y_{n'm';x} for y = 0..2 with (n', m') = (n, m) or (m, n); one item per pair also stages the
+    // Berry connections B_nm, B_mn and the velocity differences.  Phase B: one item per (pair, output).
+    __shared__ double2 Gs[WN_PCH][2][3][3];
+    __shared__ double2 Bs[WN_PCH][2][3];
+    __shared__ double2 Vd[WN_PCH][3];
+    for (int c0 = 0; c0 < total; c0 += WN_PCH) {
+        const int cn = min(WN_PCH, total - c0);
+        for (int item = t; item < cn * 6; item += 256) {
+            const int i = item % cn, dx = item / cn;
+            const int d = dx / 3, xcomp = dx - 3 * d;
+            const int4 mt = pmeta[pbase + c0 + i];
+            const int n = d ? mt.y : mt.x, m = d ? mt.x : mt.y;
+            double2 g[3];
+            wn_gderiv_row(pv, inv, n, nb, m, xcomp, g);
 #pragma unroll
-        for (int a = 0; a < 3; ++a) {
-            double2 anm = sh.has_pos ? pv.A[a][n * nb + m] : make_double2(0.0, 0.0);
-            double2 amn = sh.has_pos ? pv.A[a][m * nb + n] : make_double2(0.0, 0.0);
-            double2 dnm = cscale(pv.V[a][n * nb + m], -inm);
-            double2 dmn = cscale(pv.V[a][m * nb + n], -imn);
-            Bnm[a] = cadd(anm, cmuli(dnm));
-            Bmn[a] = cadd(amn, cmuli(dmn));
+            for (int y = 0; y < 3; ++y) Gs[i][d][xcomp][y] = g[y];
+            if (dx == 0) {
+                const double inm = inv[n * nb + m];
+                const double imn = inv[m * nb + n];
+#pragma unroll
+                for (int a = 0; a < 3; ++a) {
+                    const double2 anm = sh.has_pos ? pv.A[a][n * nb + m] : make_double2(0.0, 0.0);
+                    const double2 amn = sh.has_pos ? pv.A[a][m * nb + n] : make_double2(0.0, 0.0);
+                    const double2 dnm = cscale(pv.V[a][n * nb + m], -inm);
+                    const double2 dmn = cscale(pv.V[a][m * nb + n], -imn);
+                    Bs[i][0][a] = cadd(anm, cmuli(dnm));
+                    Bs[i][1][a] = cadd(amn, cmuli(dmn));
+                    Vd[i][a] = csub(pv.V[a][m * nb + m], pv.V[a][n * nb + n]);
+                }
+            }
         }
-        int off = 0;
-        if (sc.sc) {
-            double2 Gnm[3][3], Gmn[3][3];
-            wn_gderiv(pv, E, sc.eta, n, nb, m, Gnm);
-            wn_gderiv(pv, E, sc.eta, m, nb, n, Gmn);
-            for (int c = 0; c < 3; ++c)
-                for (int b = 0; b < 3; ++b)
-                    for (int a = 0; a < 3; ++a) {
-                        double2 z = csub(cmul(Gnm[c][a], Bmn[b]), cmul(Gmn[b][a], Bnm[c]));
-                        q[a + 3 * b + 9 * c] = make_double2(z.y, -z.x);  // -i z
-                    }
-            off = 27;
+        __syncthreads();
+        for (int item = t; item < cn * ncomp; item += 256) {
+            const int i = item / ncomp, comp = item - i * ncomp;
+            double2 *q = qout + (pbase + c0 + i) * ncomp;
+            const int sc_on = sc.sc;
+            if (sc_on && comp < 27) {
+                const int a = comp % 3, b = (comp / 3) % 3, c = comp / 9;
+                // -i (G_nm[c][a] B_mn[b] - G_mn[b][a] B_nm[c])
+                const double2 z = csub(cmul(Gs[i][0][c][a], Bs[i][1][b]), cmul(Gs[i][1][b][a], Bs[i][0][c]));
+                q[comp] = make_double2(z.y, -z.x);
+            } else {
+                const int cc = comp - (sc_on ? 27 : 0);
+                const int a = cc % 3, b = (cc / 3) % 3, c = cc / 9;
+                q[comp] = cmul(cmul(Vd[i][a], Bs[i][0][c]), Bs[i][1][b]);
+            }
         }
-        if (sc.ic) {
-            for (int c = 0; c < 3; ++c)
-                for (int b = 0; b < 3; ++b)
-                    for (int a = 0; a < 3; ++a) {
-                        double2 vd = csub(pv.V[a][m * nb + m], pv.V[a][n * nb + n]);
-                        q[off + a + 3 * b + 9 * c] = cmul(cmul(vd, Bnm[c]), Bmn[b]);
-                    }
-        }
+        __syncthreads();
     }
 }
 
 hipError_t wn_launch_pairs_f64(const double *x, const double *e, const double *omega, int *count, int4 *pmeta,
                                double2 *pmd, double2 *q, int *flags, int nk, const WnShape &sh, const WnScreen &sc,
                                hipStream_t st) {
-    hipLaunchKernelGGL(k_pairs_f64, dim3(nk), dim3(256), 0, st, x, e, omega, count, pmeta, pmd, q, flags, sh, sc);
+    hipLaunchKernelGGL(k_pairs_f64, dim3(nk), dim3(256), (size_t)sh.n * sh.n * sizeof(double), st, x, e, omega, count,
+                       pmeta, pmd, q, flags, sh, sc);
     return hipGetLastError();
 }
 
