@@ -28,7 +28,7 @@
 extern "C" {
 #endif
 
-#define WNLQG_ABI_VERSION 1
+#define WNLQG_ABI_VERSION 2
 
 /* Status codes. */
 #define WNLQG_OK 0
@@ -39,9 +39,13 @@ extern "C" {
 #define WNLQG_E_OOM (-5)         /* device allocation failed */
 #define WNLQG_E_UNSUPPORTED (-6) /* configuration outside what this build implements */
 
-/* Response quantities (bitmask). Output states are ordered by increasing bit. */
+/* Response quantities (bitmask). Output states are ordered by increasing bit.  One engine serves one
+ * reference driver family: SC and/or IC (IntegralDriver.jl), or BCD alone (BerryMultipoleDriver.jl),
+ * or intrinsic AHC alone (SpectralResponseDriver.jl). */
 #define WNLQG_SHIFT_CURRENT 1     /* Conventional SC, ConventionalShiftCurrent.jl:527-884 */
 #define WNLQG_INJECTION_CURRENT 2 /* Conventional IC, ConventionalInjectionCurrent.jl:23-140 */
+#define WNLQG_BERRY_CURVATURE_DIPOLE 4 /* BCD, Fermi-surface form: BerryMultipoleMoments.jl:7-45 */
+#define WNLQG_INTRINSIC_HALL 8    /* intrinsic AHC, Matrix method: IntrinsicHallOccupations.jl:8-44 */
 
 /* Broadening (TransitionScreening.jl:1-29). */
 #define WNLQG_GAUSSIAN 0
@@ -49,8 +53,9 @@ extern "C" {
 
 /* Arithmetic forms. */
 #define WNLQG_PRECISION_FP64 0  /* FP64 everywhere (plain GPU path) */
-#define WNLQG_PRECISION_MIXED 1 /* split-BF16 (3 products) MFMA interpolation, FP64 denominators, Kahan k-sum */
-#define WNLQG_PRECISION_MIXED_HIGH 2 /* split-BF16 with 6 products (~FP32 interpolation accuracy) */
+#define WNLQG_PRECISION_MIXED 1 /* split-FP16 (3 products) MFMA interpolation with FP64 partial sums, FP64
+                                   eigensolve, denominators and vertices, Kahan k-sum (SC / IC only) */
+#define WNLQG_PRECISION_MIXED_HIGH 2 /* split-FP16 with 4 products */
 
 /* Interpolation backends for wnlqg_run_mesh (the role of fourier_backend, FourierExecutionPlan.jl:394-480). */
 #define WNLQG_FOURIER_AUTO 0     /* separable plane contraction on meshes */
@@ -76,9 +81,10 @@ typedef struct wnlqg_model {
  * Wannier-center convention: Convention_II (the default, Config.jl:154) only.
  */
 typedef struct wnlqg_task {
-    int32_t quantities;               /* WNLQG_SHIFT_CURRENT | WNLQG_INJECTION_CURRENT */
-    int32_t num_photon_energies;      /* n_w >= 1 */
-    const double *photon_energies;    /* [n_w] eV, ascending */
+    int32_t quantities;               /* WNLQG_SHIFT_CURRENT | WNLQG_INJECTION_CURRENT, or one of
+                                         WNLQG_BERRY_CURVATURE_DIPOLE, WNLQG_INTRINSIC_HALL */
+    int32_t num_photon_energies;      /* n_w >= 1 (SC / IC; ignored otherwise) */
+    const double *photon_energies;    /* [n_w] eV, ascending (SC / IC) */
     int32_t broadening_type;          /* WNLQG_GAUSSIAN or WNLQG_LORENTZIAN */
     double broadening;                /* eV (> 0) */
     double fermi_energy;              /* eV */
@@ -89,6 +95,11 @@ typedef struct wnlqg_task {
     int32_t precision;                /* WNLQG_PRECISION_* */
     int32_t batch_size;               /* k points per device batch; 0 = automatic */
     int32_t fourier_backend;          /* WNLQG_FOURIER_* (meshes only; points always use the direct contraction) */
+    /* BCD / AHC (BerryMultipoleDCParameters, BerryMultipoleDC.jl:6-55; TaskConfiguration.jl:568-578) */
+    int32_t num_fermi_energies;       /* n_mu, 1 .. 64 */
+    const double *fermi_energies;     /* [n_mu] eV */
+    double fermi_surface_broadening;  /* eta_fs_ev > 0: Gaussian -f'(E) at T = 0 (BCD); must be 0 when T > 0 */
+    double degeneracy_threshold;      /* eV: BCD spectral isolation threshold (reference default 0.002) */
 } wnlqg_task;
 
 /* Counters of the last run (device event timers, milliseconds). */
@@ -126,7 +137,7 @@ int32_t wnlqg_create(const wnlqg_model *model, const wnlqg_task *task, wnlqg_eng
 /* Release all device memory of the engine. NULL is accepted. */
 int32_t wnlqg_destroy(wnlqg_engine *engine);
 
-/* Number of output states (set bits of task.quantities) and interpolation channels. */
+/* Number of output states (set bits of task.quantities) and interpolation channels stored per k point. */
 int32_t wnlqg_num_states(const wnlqg_engine *engine);
 int32_t wnlqg_num_channels(const wnlqg_engine *engine);
 
@@ -141,6 +152,15 @@ int32_t wnlqg_num_channels(const wnlqg_engine *engine);
  * [n_w,3,3,3] and equals sum_{k in lane} (1/N_k) * value_k with the state's prefactor
  * (ConventionalCurrentIntegralExecution.jl:243-245 for SC, :296-298 for IC).
  * total_out (optional, may be NULL): complex [n_states][n_w*27], the lanes summed in lane order.
+ *
+ * BCD engines: each lane block is [n_mu,3,3,3] column-major (complex, imaginary parts 0) holding the
+ * lane's sum of raw per-point moments D[mu,a,b,c] = sum_n -f'(E_n) Omega^n_ab dE_n/dk_c (what
+ * BerryMultipoleDriver.jl:279-285 leaves in buffers[lane]); total_out = lanes summed / N_k / |det lattice|,
+ * the integrated moment that berry_dipole_dc_conductivity (BerryMultipoleDC.jl:95-113) converts.  A level
+ * pair closer than degeneracy_threshold at any k returns WNLQG_E_GAP.
+ * AHC engines: each lane block is [n_mu,3,3] column-major, the lane's sum of per-point
+ * (-e^2/hbar 1e-20) Omega_ab(mu) (SpectralResponseDriver.jl:388-400); total_out = lanes summed / N_k /
+ * (|det lattice| 1e-30), sigma_ab in S/m (SpectralResponseDriver.jl:541-552).
  * stats may be NULL.
  */
 int32_t wnlqg_run_mesh(wnlqg_engine *engine, const int32_t mesh[3], int32_t n_lanes,
@@ -148,7 +168,8 @@ int32_t wnlqg_run_mesh(wnlqg_engine *engine, const int32_t mesh[3], int32_t n_la
 
 /*
  * Per-point values (point_value!, IntegralDriver.jl:596, with num_kpoints = 1) at m arbitrary
- * fractional k points.  kpoints: column-major [3,m].  out: complex [m][n_states][n_w*27].
+ * fractional k points.  kpoints: column-major [3,m].  out: complex [m][n_states][n_w*27]; for BCD
+ * [m][n_mu*27] raw moments, for AHC [m][n_mu*9] scaled curvature (the per-point lane contributions).
  */
 int32_t wnlqg_eval_points(wnlqg_engine *engine, const double *kpoints, int32_t m, double *out);
 
@@ -159,6 +180,7 @@ int32_t wnlqg_eval_points(wnlqg_engine *engine, const double *kpoints, int32_t m
  *   0 H, 1-3 dH_a (a = x,y,z), 4-9 ddH_ab for ab = xx,yy,zz,yz,xz,xy,
  *   10-12 A_a, 13-21 dA_ab (= Fourier of i R_b A_a, index 13 + 3a + b)  (the A channels
  *   exist only when the model has a position operator and a quantity needs them).
+ *   BCD / AHC engines store no ddH: 0 H, 1-3 dH_a, 4-6 A_a, 7-15 dA_ab (index 7 + 3a + b).
  * Element [i][j] is row i, column j.
  */
 int32_t wnlqg_fourier_points(wnlqg_engine *engine, const double *kpoints, int32_t m, double *out);

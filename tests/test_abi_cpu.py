@@ -22,7 +22,7 @@ def header_functions():
 
 def test_library_loads_and_version():
     lib = E.load_library()
-    assert lib.wnlqg_version() == 1
+    assert lib.wnlqg_version() == 2
 
 
 def test_exports_match_header():
@@ -66,4 +66,33 @@ def test_rejects_partial_band_window_and_large_n():
     big = synthetic_model(65, 0, 0)
     with pytest.raises(E.WnlqgError) as ei:
         E.Engine(big, _task())
+    assert ei.value.code == -6
+
+
+def _geom(**kw):
+    base = dict(berry_curvature_dipole=True, fermi_energies=[0.0], temperature=300.0,
+                denominator_regularization=0.0)
+    base.update(kw)
+    return E.EngineTask(**base)
+
+
+@pytest.mark.parametrize("kw,code", [
+    (dict(temperature=0.0), -1),                                    # T = 0 Fermi surface needs eta_fs
+    (dict(temperature=300.0, fermi_surface_broadening=0.02), -1),   # eta_fs only at T = 0
+    (dict(fermi_energies=np.zeros(65)), -1),                        # at most 64 Fermi energies
+    (dict(fermi_energies=[np.nan]), -2),
+    (dict(degeneracy_threshold=-1.0), -1),
+    (dict(precision="mixed"), -6),
+    (dict(berry_curvature_dipole=False, intrinsic_hall=True, temperature=10.0), -1),  # AHC needs T = 0
+])
+def test_rejects_invalid_geometry_tasks(kw, code):
+    with pytest.raises(E.WnlqgError) as ei:
+        E.Engine(synthetic_model(4, 1, 0), _geom(**kw))
+    assert ei.value.code == code
+
+
+def test_rejects_mixed_driver_families():
+    # BCD and AHC belong to different reference drivers: refused before any device work
+    with pytest.raises(E.WnlqgError) as ei:
+        E.Engine(synthetic_model(4, 1, 0), _geom(intrinsic_hall=True))
     assert ei.value.code == -6

@@ -1,0 +1,139 @@
+"""GPU parity of the BCD and intrinsic AHC stages against the oracle (oracle/geometry.py), through
+the C ABI.  Bar: FP64, 1e-10 relative (Frobenius norm over all components and Fermi energies).
+Each test prints the case it runs before the GPU call."""
+import math
+import sys
+
+import numpy as np
+import pytest
+
+from oracle import geometry as G
+from wanniernlqg_amd import engine as E
+from wanniernlqg_amd.model import TightBindingModel, synthetic_model
+
+pytestmark = pytest.mark.gpu
+
+
+def say(msg):
+    print(msg, flush=True)
+    sys.stdout.flush()
+
+
+def rel(a, b):
+    return np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-300)
+
+
+def qwz_layers(mass, a=2.5, c=3.0):
+    sx = np.array([[0, 1], [1, 0]], dtype=complex)
+    sy = np.array([[0, -1j], [1j, 0]], dtype=complex)
+    sz = np.array([[1, 0], [0, -1]], dtype=complex)
+    rv = [(0, 0, 0), (1, 0, 0), (-1, 0, 0), (0, 1, 0), (0, -1, 0)]
+    hr = np.zeros((5, 2, 2), dtype=complex)
+    hr[0] = mass * sz
+    hr[1] = sx / 2j + sz / 2
+    hr[2] = -sx / 2j + sz / 2
+    hr[3] = sy / 2j + sz / 2
+    hr[4] = -sy / 2j + sz / 2
+    return TightBindingModel(np.diag([a, a, c]), np.array(rv), np.ones(5, dtype=np.int32), hr, None)
+
+
+BCD_CASES = {
+    "n6_thermal": (lambda: synthetic_model(6, 1, 8), dict(fermi_energies=[-0.5, 0.0, 0.7], temperature=600.0)),
+    "n8_pos_gauss": (lambda: synthetic_model(8, 1, 5, position_scale=0.1),
+                     dict(fermi_energies=[-0.3, 0.4], temperature=0.0, fermi_surface_broadening=0.2)),
+    "n32_pos_eta": (lambda: synthetic_model(32, 1, 11, position_scale=0.1),
+                    dict(fermi_energies=[-1.0, 0.0, 1.0], temperature=300.0, denominator_regularization=1e-3)),
+}
+
+
+def bcd_engine(model, kw, **extra):
+    t = dict(berry_curvature_dipole=True, denominator_regularization=0.0, degeneracy_threshold=1e-9)
+    t.update(kw)
+    t.update(extra)
+    return E.Engine(model, E.EngineTask(**t))
+
+
+def bcd_task(kw):
+    return G.GeometryTask(fermi_energies=kw["fermi_energies"], temperature=kw.get("temperature", 0.0),
+                          fs_broadening=kw.get("fermi_surface_broadening") or None,
+                          denominator_regularization=kw.get("denominator_regularization", 0.0),
+                          degeneracy_threshold=1e-9)
+
+
+@pytest.mark.parametrize("case", sorted(BCD_CASES))
+def test_bcd_points(case):
+    make, kw = BCD_CASES[case]
+    m = make()
+    k = np.random.default_rng(4).uniform(-0.5, 1.0, (9, 3))
+    say(f"[bcd points] {case}")
+    with bcd_engine(m, kw) as eng:
+        got = eng.eval_points(k)["bcd"]
+    ref = G.bcd_point_moments(m, bcd_task(kw), k)
+    assert got.shape == ref.shape
+    assert rel(got, ref) < 1e-10, rel(got, ref)
+
+
+@pytest.mark.parametrize("case,backend", [("n6_thermal", "planes"), ("n8_pos_gauss", "direct"),
+                                          ("n8_pos_gauss", "planes"), ("n32_pos_eta", "auto")])
+def test_bcd_mesh(case, backend):
+    make, kw = BCD_CASES[case]
+    m = make()
+    mesh = (5, 4, 6)
+    say(f"[bcd mesh] {case} {backend} {mesh}")
+    with bcd_engine(m, kw, fourier_backend=backend) as eng:
+        lanes, total, st = eng.run_mesh(mesh)
+    rl, rt = G.integrate_geometry_mesh(m, bcd_task(kw), mesh, "bcd")
+    assert lanes["bcd"].shape == rl.shape
+    assert rel(lanes["bcd"], rl) < 1e-10
+    assert rel(total["bcd"], rt) < 1e-10
+
+
+AHC_CASES = {
+    "n6": (lambda: synthetic_model(6, 1, 3), [-0.5, 0.0, 0.5]),
+    "n12_pos": (lambda: synthetic_model(12, 1, 9, position_scale=0.1), [-1.0, -0.2, 0.3, 1.5]),
+    "n32_pos": (lambda: synthetic_model(32, 1, 2, position_scale=0.1), [0.0]),
+}
+
+
+@pytest.mark.parametrize("case", sorted(AHC_CASES))
+def test_ahc_points_and_mesh(case):
+    make, mus = AHC_CASES[case]
+    m = make()
+    k = np.random.default_rng(6).uniform(0, 1, (7, 3))
+    say(f"[ahc] {case}")
+    task = G.GeometryTask(fermi_energies=mus)
+    with E.Engine(m, E.EngineTask(intrinsic_hall=True, fermi_energies=mus, denominator_regularization=0.0)) as eng:
+        got = eng.eval_points(k)["ahc"]
+        lanes, total, _ = eng.run_mesh((4, 5, 3))
+    assert rel(got, G.ahc_point_values(m, task, k)) < 1e-10
+    rl, rt = G.integrate_geometry_mesh(m, task, (4, 5, 3), "ahc")
+    assert rel(lanes["ahc"], rl) < 1e-10
+    assert rel(total["ahc"], rt) < 1e-10
+
+
+@pytest.mark.parametrize("mass", [1.0, -1.2, 2.6])
+def test_ahc_chern_layers_quantized(mass):
+    """Stacked Qi-Wu-Zhang layers: sigma_xy = -(e^2/h) C / c (C = +-1 for 0 < |m| < 2, else 0)."""
+    c = 3.0
+    m = qwz_layers(mass, c=c)
+    chern = 0 if abs(mass) > 2 else (-1 if mass > 0 else 1)
+    say(f"[ahc chern] mass {mass}")
+    with E.Engine(m, E.EngineTask(intrinsic_hall=True, fermi_energies=[0.0], denominator_regularization=0.0)) as eng:
+        _, total, _ = eng.run_mesh((48, 48, 1))
+    quantum = G.RESPONSE_CHARGE_C ** 2 / (2 * math.pi * G.RESPONSE_HBAR_JS) / (c * 1e-10)
+    sxy = total["ahc"][0, 0, 1]
+    assert abs(sxy - (-quantum * chern)) < 1e-6 * quantum, (sxy / quantum, chern)
+
+
+def test_bcd_gap_error():
+    base = synthetic_model(2, 1, 4)
+    hr = np.zeros((base.num_r_vectors, 4, 4), dtype=complex)
+    hr[:, :2, :2] = base.hamiltonian_r
+    hr[:, 2:, 2:] = base.hamiltonian_r
+    dup = TightBindingModel(base.lattice, base.r_vectors, base.r_degeneracies, hr, None)
+    say("[bcd gap] degenerate copies")
+    with E.Engine(dup, E.EngineTask(berry_curvature_dipole=True, fermi_energies=[0.0], temperature=300.0,
+                                    denominator_regularization=0.0)) as eng:
+        with pytest.raises(E.WnlqgError) as ei:
+            eng.run_mesh((3, 3, 3))
+    assert ei.value.code == -3

@@ -76,6 +76,8 @@ struct DevBuf {
 }  // namespace
 
 struct wnlqg_engine {
+    int family = 0;     // 0 SC / IC, 1 BCD, 2 intrinsic AHC
+    WnGeom gp{};        // BCD / AHC parameters
     int device = 0;
     hipStream_t stream = nullptr;
     WnShape sh{};
@@ -112,10 +114,51 @@ struct wnlqg_engine {
 
 namespace {
 
+// driver family of a quantities bitmask: 0 SC / IC, 1 BCD, 2 AHC, -1 invalid
+int task_family(int q) {
+    if (q == WNLQG_BERRY_CURVATURE_DIPOLE) return 1;
+    if (q == WNLQG_INTRINSIC_HALL) return 2;
+    if (q != 0 && (q & ~(WNLQG_SHIFT_CURRENT | WNLQG_INJECTION_CURRENT)) == 0) return 0;
+    return -1;
+}
+
+// BCD / AHC task checks (BerryMultipoleDC.jl:16-55, IntrinsicHallOccupations.jl:16-21).
+int check_geometry_task(const wnlqg_task *t, int family) {
+    if (t->num_fermi_energies < 1 || t->num_fermi_energies > WN_GEOM_MAXMU || !t->fermi_energies)
+        return fail(WNLQG_E_INVALID, "1 to %d Fermi energies are required", WN_GEOM_MAXMU);
+    for (int i = 0; i < t->num_fermi_energies; ++i)
+        if (!std::isfinite(t->fermi_energies[i])) return fail(WNLQG_E_NONFINITE, "non-finite Fermi energy");
+    if (!(t->temperature >= 0.0) || !std::isfinite(t->temperature))
+        return fail(WNLQG_E_INVALID, "temperature_k must be finite and nonnegative");
+    if (!(t->denominator_regularization >= 0.0) || !std::isfinite(t->denominator_regularization))
+        return fail(WNLQG_E_INVALID, "denominator_regularization must be finite and >= 0");
+    if (t->precision != WNLQG_PRECISION_FP64)
+        return fail(WNLQG_E_UNSUPPORTED, "BCD / AHC run in FP64 only");
+    if (t->batch_size < 0) return fail(WNLQG_E_INVALID, "batch_size must be >= 0");
+    if (t->fourier_backend < WNLQG_FOURIER_AUTO || t->fourier_backend > WNLQG_FOURIER_PLANES)
+        return fail(WNLQG_E_INVALID, "unknown fourier_backend %d", t->fourier_backend);
+    if (family == 1) {
+        if (!(t->degeneracy_threshold >= 0.0) || !std::isfinite(t->degeneracy_threshold))
+            return fail(WNLQG_E_INVALID, "invalid spectral isolation threshold");
+        const double w = t->fermi_surface_broadening;
+        if (t->temperature == 0.0 && !(w > 0.0 && std::isfinite(w)))
+            return fail(WNLQG_E_INVALID, "zero-temperature Fermi surface requires explicit positive eta_fs_ev");
+        if (t->temperature > 0.0 && w != 0.0)
+            return fail(WNLQG_E_INVALID, "eta_fs_ev is applicable only to zero-temperature Fermi surface");
+    } else if (t->temperature != 0.0) {
+        return fail(WNLQG_E_INVALID, "INTRINSIC_HALL_REQUIRES_ZERO_TEMPERATURE");
+    }
+    return WNLQG_OK;
+}
+
 int check_task(const wnlqg_task *t, int n) {
     if (!t) return fail(WNLQG_E_INVALID, "task is NULL");
-    if ((t->quantities & ~(WNLQG_SHIFT_CURRENT | WNLQG_INJECTION_CURRENT)) || t->quantities == 0)
-        return fail(WNLQG_E_UNSUPPORTED, "quantities bitmask %d: only shift (1) and injection (2) current", t->quantities);
+    const int family = task_family(t->quantities);
+    if (family < 0)
+        return fail(WNLQG_E_UNSUPPORTED,
+                    "quantities bitmask %d: shift (1) and/or injection (2) current, or BCD (4) alone, or AHC (8) alone",
+                    t->quantities);
+    if (family > 0) return check_geometry_task(t, family);
     if (t->num_photon_energies < 1 || !t->photon_energies)
         return fail(WNLQG_E_INVALID, "photon energies missing");
     for (int i = 0; i < t->num_photon_energies; ++i) {
@@ -143,6 +186,13 @@ int check_task(const wnlqg_task *t, int n) {
     return WNLQG_OK;
 }
 
+// complex values per k point (or chunk, or lane) of all output states
+int output_elems(const wnlqg_engine *g) {
+    if (g->family == 1) return 27 * g->gp.nmu;
+    if (g->family == 2) return 9 * g->gp.nmu;
+    return 27 * g->sc.nq * g->sc.nw;
+}
+
 int ensure_rows(wnlqg_engine *g, int rows) {
     // working set for `rows` k points (multiple of 128)
     const size_t n2 = (size_t)g->sh.n * g->sh.n;
@@ -158,16 +208,18 @@ int ensure_rows(wnlqg_engine *g, int rows) {
     WN_HIP(g->e.alloc((size_t)rows * g->sh.n));
     WN_HIP(g->u.alloc((size_t)rows * n2));
     WN_HIP(g->sweeps.alloc((size_t)rows));
-    WN_HIP(g->count.alloc((size_t)rows));
-    WN_HIP(g->pmeta.alloc((size_t)rows * n2));
-    WN_HIP(g->pmd.alloc((size_t)rows * n2));
-    WN_HIP(g->q.alloc((size_t)rows * n2 * 27 * g->sc.nq));
+    if (g->family == 0) {
+        WN_HIP(g->count.alloc((size_t)rows));
+        WN_HIP(g->pmeta.alloc((size_t)rows * n2));
+        WN_HIP(g->pmd.alloc((size_t)rows * n2));
+        WN_HIP(g->q.alloc((size_t)rows * n2 * 27 * g->sc.nq));
+    }
     return WNLQG_OK;
 }
 
 int ensure_partial(wnlqg_engine *g, int nchunks) {
     if (g->partial_cap >= nchunks) return WNLQG_OK;
-    const size_t nelem = (size_t)27 * g->sc.nq * g->sc.nw;
+    const size_t nelem = (size_t)output_elems(g);
     WN_HIP(g->partial.alloc((size_t)nchunks * nelem));
     WN_HIP(g->chunks.alloc((size_t)nchunks));
     g->partial_cap = nchunks;
@@ -236,7 +288,7 @@ int32_t wnlqg_last_error(char *buf, size_t len) {
     return (int32_t)g_last_error.size();
 }
 
-int32_t wnlqg_num_states(const wnlqg_engine *g) { return g ? g->sc.nq : 0; }
+int32_t wnlqg_num_states(const wnlqg_engine *g) { return g ? (g->family ? 1 : g->sc.nq) : 0; }
 int32_t wnlqg_num_channels(const wnlqg_engine *g) { return g ? g->sh.C : 0; }
 
 int32_t wnlqg_destroy(wnlqg_engine *g) {
@@ -295,7 +347,16 @@ int32_t wnlqg_create(const wnlqg_model *model, const wnlqg_task *task, wnlqg_eng
     g->cell_volume = L[0] * (L[4] * L[8] - L[7] * L[5]) - L[3] * (L[1] * L[8] - L[7] * L[2]) +
                      L[6] * (L[1] * L[5] - L[4] * L[2]);
     g->precision = task->precision;
-    g->sc.nw = task->num_photon_energies;
+    g->family = task_family(task->quantities);
+    if (g->family > 0) {
+        g->gp.kind = g->family;
+        g->gp.nmu = task->num_fermi_energies;
+        g->gp.temperature = task->temperature;
+        g->gp.eta_fs = task->fermi_surface_broadening;
+        g->gp.eta = task->denominator_regularization;
+        g->gp.gap_tol = task->degeneracy_threshold;
+    }
+    g->sc.nw = g->family ? task->num_fermi_energies : task->num_photon_energies;
     g->sc.gaussian = task->broadening_type == WNLQG_GAUSSIAN;
     g->sc.width = task->broadening;
     g->sc.mu = task->fermi_energy;
@@ -304,17 +365,36 @@ int32_t wnlqg_create(const wnlqg_model *model, const wnlqg_task *task, wnlqg_eng
     g->sc.window = task->transition_window_factor * task->broadening;
     g->sc.sc = (task->quantities & WNLQG_SHIFT_CURRENT) != 0;
     g->sc.ic = (task->quantities & WNLQG_INJECTION_CURRENT) != 0;
-    g->sc.nq = g->sc.sc + g->sc.ic;
+    g->sc.nq = g->family ? 1 : g->sc.sc + g->sc.ic;
     int qi = 0;
     if (g->sc.sc) g->pref[qi++] = sc_prefactor(g->cell_volume);
     if (g->sc.ic) g->pref[qi++] = ic_prefactor(g->cell_volume);
-    g->omega_host.assign(task->photon_energies, task->photon_energies + g->sc.nw);
+    // per-point AHC scale -e^2/hbar * 1e-20 (SpectralResponseDriver.jl:388-400); BCD moments are raw
+    if (g->family == 1) g->pref[0] = 1.0;
+    if (g->family == 2) g->pref[0] = -kCharge * kCharge / kHbarJs * 1e-20;
+    // the frequency axis of SC / IC, or the Fermi-energy axis of BCD / AHC
+    const double *axis = g->family ? task->fermi_energies : task->photon_energies;
+    g->omega_host.assign(axis, axis + g->sc.nw);
 
     // shape
     g->sh.n = n;
     g->sh.nr = nr;
     g->sh.has_pos = model->position_r != nullptr;
-    g->sh.C = g->sh.has_pos ? WN_CH_FULL : WN_CH_BASE;
+    // stored channels: SC / IC need H, dH, ddH (+ A, dA); BCD / AHC need H, dH (+ A, dA)
+    {
+        int c = 0;
+        for (int l = 0; l < WN_CH_FULL; ++l) {
+            const bool pos = l >= WN_CH_A;
+            const bool second = l >= WN_CH_DDH && l < WN_CH_A;
+            if (pos && !g->sh.has_pos) continue;
+            if (second && g->family != 0) continue;
+            g->sh.lch[c++] = (signed char)l;
+        }
+        g->sh.C = c;
+        g->gp.c_dh = 1;
+        g->gp.c_a = g->sh.has_pos ? (g->family ? 4 : WN_CH_A) : -1;
+        g->gp.c_da = g->sh.has_pos ? (g->family ? 7 : WN_CH_DA) : -1;
+    }
     g->sh.Kb = (nr + WN_RB - 1) / WN_RB;
     // columns padded to 256 so that both interpolation kernels tile the same output rows
     g->sh.nNt = (int)((2 * (size_t)g->sh.C * n2 + 255) / 256) * 2;
@@ -426,7 +506,8 @@ int32_t wnlqg_create(const wnlqg_model *model, const wnlqg_task *task, wnlqg_eng
     }
 
     // batch size: interpolation rows + pair buffers within ~24 GB unless the caller fixed it
-    const size_t per_k = (size_t)g->sh.ldo * 8 + n2 * (sizeof(int4) + sizeof(double2) + 27 * g->sc.nq * 16) +
+    const size_t per_k = (size_t)g->sh.ldo * 8 +
+                         (g->family == 0 ? n2 * (sizeof(int4) + sizeof(double2) + 27 * g->sc.nq * 16) : 0) +
                          n2 * 16 + (size_t)3 * g->sh.S * 16;
     long long b = task->batch_size > 0 ? task->batch_size : (long long)(24.0e9 / (double)per_k);
     b = std::max(128LL, std::min(b, 65536LL));
@@ -484,12 +565,37 @@ int32_t wnlqg_spectrum_points(wnlqg_engine *g, const double *kpoints, int32_t m,
 namespace {
 
 int check_flags(wnlqg_engine *g) {
-    int f = 0;
-    WN_HIP(hipMemcpy(&f, g->flags.p, sizeof(int), hipMemcpyDeviceToHost));
-    if (f & 1)
+    int f[4] = {0, 0, 0, 0};
+    WN_HIP(hipMemcpy(f, g->flags.p, sizeof(f), hipMemcpyDeviceToHost));
+    if (f[0] & 1)
         return fail(WNLQG_E_GAP,
                     "GEOMETRY_SINGLE_BAND_SPECTRUM_NOT_ISOLATED: denominator_regularization = 0 needs gaps > 1e-10 eV");
+    if (f[2] & 1)
+        return fail(WNLQG_E_GAP,
+                    "GEOMETRY_SINGLE_BAND_SPECTRUM_NOT_ISOLATED: two levels within threshold_ev=%g at k index %d",
+                    g->gp.gap_tol, f[3]);
+    if (f[2] & 2) return fail(WNLQG_E_NONFINITE, "nonfinite BCD/BCQ or intrinsic Hall integrand");
     return WNLQG_OK;
+}
+
+// clear the status words: [0] pair-gap flag, [1] max sweeps, [2] geometry flags, [3] first failing k
+hipError_t clear_flags(wnlqg_engine *g) {
+    hipError_t e = hipMemsetAsync(g->flags.p, 0, 3 * sizeof(int), g->stream);
+    if (e == hipSuccess) e = hipMemsetAsync(g->flags.p + 3, 0x7f, sizeof(int), g->stream);
+    return e;
+}
+
+// per-chunk stage after the rotation: SC / IC pairs + frequency contraction, or BCD / AHC
+hipError_t launch_response(wnlqg_engine *g, const int4 *chunks, int nch, int cnt, long long kbase) {
+    if (g->family == 0) {
+        hipError_t e = wn_launch_pairs_f64(g->x.p, g->e.p, g->omega.p, g->count.p, g->pmeta.p, g->pmd.p, g->q.p,
+                                           g->flags.p, cnt, g->sh, g->sc, g->stream);
+        if (e != hipSuccess) return e;
+        return wn_launch_accum(chunks, nch, g->count.p, g->pmeta.p, g->pmd.p, g->q.p, g->omega.p, g->partial.p, g->sh,
+                               g->sc, g->stream);
+    }
+    return wn_launch_geom_f64(g->x.p, g->e.p, g->omega.p, chunks, nch, g->partial.p, g->flags.p, kbase, g->sh, g->gp,
+                              g->stream);
 }
 
 }  // namespace
@@ -497,8 +603,9 @@ int check_flags(wnlqg_engine *g) {
 int32_t wnlqg_eval_points(wnlqg_engine *g, const double *kpoints, int32_t m, double *out) {
     if (!g || !kpoints || !out || m < 0) return fail(WNLQG_E_INVALID, "invalid arguments");
     WN_HIP(hipSetDevice(g->device));
-    const int nelem = 27 * g->sc.nq * g->sc.nw;
-    WN_HIP(hipMemsetAsync(g->flags.p, 0, sizeof(int), g->stream));
+    const int nelem = output_elems(g);
+    const int per_state = nelem / g->sc.nq;
+    WN_HIP(clear_flags(g));
     std::vector<double> part;
     for (int p0 = 0; p0 < m; p0 += g->batch) {
         int cnt = std::min(g->batch, m - p0);
@@ -508,24 +615,22 @@ int32_t wnlqg_eval_points(wnlqg_engine *g, const double *kpoints, int32_t m, dou
         rc = run_front(g, cnt, rows, true);
         if (rc) return rc;
         WN_HIP(wn_launch_rotate_f64(g->x.p, g->u.p, cnt, g->sh, g->stream));
-        WN_HIP(wn_launch_pairs_f64(g->x.p, g->e.p, g->omega.p, g->count.p, g->pmeta.p, g->pmd.p, g->q.p, g->flags.p,
-                                   cnt, g->sh, g->sc, g->stream));
         rc = ensure_partial(g, cnt);
         if (rc) return rc;
         std::vector<int4> ch(cnt);
         for (int i = 0; i < cnt; ++i) ch[i] = make_int4(i, 1, i, 0);
+        WN_HIP(hipStreamSynchronize(g->stream));
         WN_HIP(hipMemcpy(g->chunks.p, ch.data(), cnt * sizeof(int4), hipMemcpyHostToDevice));
-        WN_HIP(wn_launch_accum(g->chunks.p, cnt, g->count.p, g->pmeta.p, g->pmd.p, g->q.p, g->omega.p, g->partial.p,
-                               g->sh, g->sc, g->stream));
+        WN_HIP(launch_response(g, g->chunks.p, cnt, cnt, p0));
         part.resize((size_t)cnt * nelem * 2);
         WN_HIP(hipMemcpyAsync(part.data(), g->partial.p, part.size() * sizeof(double), hipMemcpyDeviceToHost,
                               g->stream));
         WN_HIP(hipStreamSynchronize(g->stream));
         for (int i = 0; i < cnt; ++i)
             for (int qv = 0; qv < g->sc.nq; ++qv)
-                for (int e = 0; e < 27 * g->sc.nw; ++e) {
-                    size_t src = ((size_t)i * nelem + (size_t)qv * 27 * g->sc.nw + e) * 2;
-                    size_t dst = (((size_t)(p0 + i) * g->sc.nq + qv) * 27 * g->sc.nw + e) * 2;
+                for (int e = 0; e < per_state; ++e) {
+                    size_t src = ((size_t)i * nelem + (size_t)qv * per_state + e) * 2;
+                    size_t dst = (((size_t)(p0 + i) * g->sc.nq + qv) * per_state + e) * 2;
                     out[dst] = g->pref[qv] * part[src];
                     out[dst + 1] = g->pref[qv] * part[src + 1];
                 }
@@ -544,7 +649,7 @@ int32_t wnlqg_run_mesh(wnlqg_engine *g, const int32_t mesh[3], int32_t n_lanes, 
     const long long nk = (long long)mesh[0] * mesh[1] * mesh[2];
     if (n_lanes < 1 || n_lanes > nk) return fail(WNLQG_E_INVALID, "n_lanes must be in [1, N_k]");
     WN_HIP(hipSetDevice(g->device));
-    const int nelem = 27 * g->sc.nq * g->sc.nw;
+    const int nelem = output_elems(g);
 
     // chunk plan: lanes are contiguous k ranges (BundleAssembly.jl:41-47); each batch splits the
     // part of every lane it covers into chunks of at most kpc points, in k order.
@@ -611,7 +716,7 @@ int32_t wnlqg_run_mesh(wnlqg_engine *g, const int32_t mesh[3], int32_t n_lanes, 
     }
     WN_HIP(hipMemsetAsync(g->lanebuf.p, 0, (size_t)n_lanes * nelem * sizeof(double2), g->stream));
     WN_HIP(hipMemsetAsync(g->lanecomp.p, 0, (size_t)n_lanes * nelem * sizeof(double2), g->stream));
-    WN_HIP(hipMemsetAsync(g->flags.p, 0, 2 * sizeof(int), g->stream));
+    WN_HIP(clear_flags(g));
     double2 *comp = g->nterms ? g->lanecomp.p : nullptr;
 
     std::vector<hipEvent_t> evs;
@@ -655,13 +760,19 @@ int32_t wnlqg_run_mesh(wnlqg_engine *g, const int32_t mesh[3], int32_t n_lanes, 
         mk[3] = mark();
         WN_HIP(wn_launch_rotate_f64(g->x.p, g->u.p, cnt, g->sh, g->stream));
         mk[4] = mark();
-        WN_HIP(wn_launch_pairs_f64(g->x.p, g->e.p, g->omega.p, g->count.p, g->pmeta.p, g->pmd.p, g->q.p, g->flags.p,
-                                   cnt, g->sh, g->sc, g->stream));
-        mk[5] = mark();
         const int nch = (int)(bchunk[bi + 1] - bchunk[bi]);
         const int nln = (int)(blane[bi + 1] - blane[bi]);
-        WN_HIP(wn_launch_accum(dchunks.p + bchunk[bi], nch, g->count.p, g->pmeta.p, g->pmd.p, g->q.p, g->omega.p,
-                               g->partial.p, g->sh, g->sc, g->stream));
+        if (g->family == 0) {
+            WN_HIP(wn_launch_pairs_f64(g->x.p, g->e.p, g->omega.p, g->count.p, g->pmeta.p, g->pmd.p, g->q.p,
+                                       g->flags.p, cnt, g->sh, g->sc, g->stream));
+            mk[5] = mark();
+            WN_HIP(wn_launch_accum(dchunks.p + bchunk[bi], nch, g->count.p, g->pmeta.p, g->pmd.p, g->q.p, g->omega.p,
+                                   g->partial.p, g->sh, g->sc, g->stream));
+        } else {
+            WN_HIP(wn_launch_geom_f64(g->x.p, g->e.p, g->omega.p, dchunks.p + bchunk[bi], nch, g->partial.p,
+                                      g->flags.p, k0, g->sh, g->gp, g->stream));
+            mk[5] = mark();
+        }
         WN_HIP(wn_launch_lane_reduce(dlanes.p + blane[bi], nln, g->partial.p, g->lanebuf.p, comp, nelem, g->stream));
         mk[6] = mark();
     }
@@ -700,9 +811,11 @@ int32_t wnlqg_run_mesh(wnlqg_engine *g, const int32_t mesh[3], int32_t n_lanes, 
     st.separable = sep ? 1 : 0;
     if (stats) *stats = st;
 
-    // scale: each state * its prefactor, then 1/N_k (consume_point!, IntegralDriver.jl:820-874)
-    const double scale = 1.0 / (double)nk;
-    const int per_state = 27 * g->sc.nw;
+    // SC / IC: each state * its prefactor, then 1/N_k (consume_point!, IntegralDriver.jl:820-874).
+    // BCD / AHC: lanes hold raw sums of the per-point values (BerryMultipoleDriver.jl:279-285); the total is
+    // divided by N_k and multiplied by the measure (BerryMultipoleDriver.jl:290-303, SpectralResponseDriver.jl:541-552).
+    const double scale = g->family ? 1.0 : 1.0 / (double)nk;
+    const int per_state = nelem / g->sc.nq;
     for (int l = 0; l < n_lanes; ++l)
         for (int qv = 0; qv < g->sc.nq; ++qv)
             for (int e = 0; e < per_state; ++e) {
@@ -716,6 +829,10 @@ int32_t wnlqg_run_mesh(wnlqg_engine *g, const int32_t mesh[3], int32_t n_lanes, 
         for (size_t i = 0; i < (size_t)nelem * 2; ++i) total_out[i] = 0.0;
         for (int l = 0; l < n_lanes; ++l)
             for (size_t i = 0; i < (size_t)nelem * 2; ++i) total_out[i] += lane_out[(size_t)l * nelem * 2 + i];
+        if (g->family) {
+            const double vol = std::fabs(g->cell_volume) * (g->family == 2 ? 1.0e-30 : 1.0);
+            for (size_t i = 0; i < (size_t)nelem * 2; ++i) total_out[i] = total_out[i] / (double)nk / vol;
+        }
     }
     return WNLQG_OK;
 }

@@ -4,16 +4,34 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include "wnlqg_common.h"
+
 struct WnShape {
     int n;        // orbitals
     int nr;       // R vectors
-    int C;        // interpolation channels
+    int C;        // interpolation channels stored per k point
     int has_pos;  // position channels present
     int Kb;       // blocks of 8 R vectors
     int nNt;      // 128-column tiles of the interpolation output
     int ldo;      // per-k row stride (doubles) of the interpolation output = nNt * 128
     int S;        // span of one phase table axis
+    // stored channel s holds logical channel lch[s] of the full map (WN_CH_H .. WN_CH_DA + 8)
+    signed char lch[WN_CH_FULL];
 };
+
+// Berry-curvature dipole / intrinsic anomalous Hall per-k stage.
+struct WnGeom {
+    int kind;            // 1 BCD (Fermi-surface moments), 2 AHC (Matrix method, T = 0)
+    int nmu;             // Fermi energies (<= WN_GEOM_MAXMU)
+    double temperature;  // K
+    double eta_fs;       // Gaussian width of -f'(E) at T = 0 (BCD)
+    double eta;          // denominator regularization of D (0: bare 1/dE)
+    double gap_tol;      // BCD spectral isolation threshold (eV)
+    int c_dh;            // stored slot of dH_x (dH_y, dH_z follow)
+    int c_a;             // stored slot of A_x, or -1 without a position operator
+    int c_da;            // stored slot of dA_xx (d_b A_a at c_da + 3a + b)
+};
+#define WN_GEOM_MAXMU 64
 
 struct WnScreen {
     int nw;             // photon energies
@@ -75,6 +93,12 @@ hipError_t wn_launch_pairs_f64(const double *x, const double *e, const double *o
 hipError_t wn_launch_accum(const int4 *chunks, int nchunks, const int *count, const int4 *pmeta, const double2 *pmd,
                            const double2 *q, const double *omega, double2 *partial, const WnShape &sh,
                            const WnScreen &sc, hipStream_t st);
+// BCD moments or AHC curvature per chunk of k points: partial[chunk][comp][mu] (comp = a + 3b (+ 9c)).
+// flags[2] |= 1 and flags[3] = min global k index on a spectral isolation failure; flags[2] |= 2 on a
+// non-finite value.
+hipError_t wn_launch_geom_f64(const double *x, const double *e, const double *mu, const int4 *chunks, int nchunks,
+                              double2 *partial, int *flags, long long kbase, const WnShape &sh, const WnGeom &gp,
+                              hipStream_t st);
 // Fixed-order chunk -> lane sums.
 hipError_t wn_launch_lane_reduce(const int4 *lanes, int nlanes, const double2 *partial, double2 *lanebuf,
                                  double2 *lanecomp, int nelem, hipStream_t st);

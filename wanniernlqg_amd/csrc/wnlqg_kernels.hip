@@ -39,7 +39,8 @@ __device__ double wn_coeff_entry(const double *__restrict__ hr, const double *__
     const int n = sh.n, n2 = n * n;
     const int J = col >> 1, p2 = col & 1;
     if (r >= sh.nr || J >= sh.C * n2) return 0.0;
-    const int c = J / n2, ij = J - c * n2;
+    const int slot = J / n2, ij = J - slot * n2;
+    const int c = sh.lch[slot];
     const int i = ij / n, jj = ij - i * n;
     // Julia column-major element (i, jj, r) and (i, jj, a, r)
     const double *h = hr + 2 * ((size_t)i + (size_t)n * jj + (size_t)n2 * r);

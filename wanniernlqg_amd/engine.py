@@ -21,6 +21,8 @@ LIB_PATH = os.path.join(HERE, "libwnlqg_hip.so")
 
 WNLQG_SHIFT_CURRENT = 1
 WNLQG_INJECTION_CURRENT = 2
+WNLQG_BERRY_CURVATURE_DIPOLE = 4
+WNLQG_INTRINSIC_HALL = 8
 WNLQG_GAUSSIAN = 0
 WNLQG_LORENTZIAN = 1
 WNLQG_PRECISION_FP64 = 0
@@ -54,7 +56,9 @@ class _Task(C.Structure):
                 ("broadening", C.c_double), ("fermi_energy", C.c_double), ("temperature", C.c_double),
                 ("denominator_regularization", C.c_double), ("transition_window_factor", C.c_double),
                 ("band_window_size", C.c_int32), ("precision", C.c_int32), ("batch_size", C.c_int32),
-                ("fourier_backend", C.c_int32)]
+                ("fourier_backend", C.c_int32), ("num_fermi_energies", C.c_int32),
+                ("fermi_energies", C.POINTER(C.c_double)), ("fermi_surface_broadening", C.c_double),
+                ("degeneracy_threshold", C.c_double)]
 
 
 class Stats(C.Structure):
@@ -114,11 +118,19 @@ def _ptr(a: np.ndarray, ctype=C.c_double):
 
 @dataclass
 class EngineTask:
-    """Resolved task parameters of the Integral path (subset of ``EffectiveTaskConfig``,
-    reference ``src/Runtime/Config.jl:4-170``; defaults from ``Config.jl:126-155``)."""
-    photon_energies: np.ndarray
+    """Resolved task parameters (subset of ``EffectiveTaskConfig``, reference ``src/Runtime/Config.jl:4-170``;
+    defaults from ``Config.jl:126-155``).  One engine serves either SC and/or IC (the Integral driver), or
+    BCD alone (``BerryMultipoleDriver.jl``; ``fermi_energies``, ``temperature``, ``fermi_surface_broadening``,
+    ``degeneracy_threshold``, ``denominator_regularization`` = 0 by default for BCD,
+    ``TaskConfiguration.jl:568-578``), or intrinsic AHC alone (``SpectralResponseDriver.jl``, T = 0)."""
+    photon_energies: Optional[np.ndarray] = None
     shift_current: bool = True
     injection_current: bool = False
+    berry_curvature_dipole: bool = False
+    intrinsic_hall: bool = False
+    fermi_energies: Optional[np.ndarray] = None
+    fermi_surface_broadening: float = 0.0
+    degeneracy_threshold: float = 0.002
     broadening: float = 0.040
     broadening_type: str = "Gaussian"
     fermi_energy: float = 0.0
@@ -147,22 +159,39 @@ class Engine:
         self._ar = None
         if model.position_r is not None:
             self._ar = np.ascontiguousarray(np.transpose(model.position_r, (0, 1, 3, 2))).view(np.float64).ravel()
-        self._om = np.ascontiguousarray(task.photon_energies, dtype=np.float64)
+        geometry = task.berry_curvature_dipole or task.intrinsic_hall
+        self._om = np.ascontiguousarray(np.zeros(1) if task.photon_energies is None else task.photon_energies,
+                                        dtype=np.float64)
+        self._mu = np.ascontiguousarray(np.atleast_1d(np.zeros(1) if task.fermi_energies is None
+                                                      else task.fermi_energies), dtype=np.float64)
         m = _Model(n, nr, _ptr(self._lat), _ptr(self._rv, C.c_int32), _ptr(self._deg, C.c_int32), _ptr(self._hr),
                    _ptr(self._ar) if self._ar is not None else C.POINTER(C.c_double)())
-        q = (WNLQG_SHIFT_CURRENT if task.shift_current else 0) | (WNLQG_INJECTION_CURRENT if task.injection_current else 0)
+        if geometry:
+            q = (WNLQG_BERRY_CURVATURE_DIPOLE if task.berry_curvature_dipole else 0) | \
+                (WNLQG_INTRINSIC_HALL if task.intrinsic_hall else 0)
+        else:
+            q = (WNLQG_SHIFT_CURRENT if task.shift_current else 0) | \
+                (WNLQG_INJECTION_CURRENT if task.injection_current else 0)
         bt = {"Gaussian": WNLQG_GAUSSIAN, "Lorentzian": WNLQG_LORENTZIAN}.get(task.broadening_type, 99)
         prec = {"fp64": WNLQG_PRECISION_FP64, "mixed": WNLQG_PRECISION_MIXED,
                 "mixed6": WNLQG_PRECISION_MIXED_HIGH}[task.precision]
         t = _Task(q, self._om.shape[0], _ptr(self._om), bt, task.broadening, task.fermi_energy, task.temperature,
                   task.denominator_regularization, task.transition_window_factor, task.band_window_size, prec,
-                  task.batch_size, {"auto": 0, "direct": 1, "planes": 2}[task.fourier_backend])
+                  task.batch_size, {"auto": 0, "direct": 1, "planes": 2}[task.fourier_backend],
+                  self._mu.shape[0] if geometry else 0, _ptr(self._mu), task.fermi_surface_broadening,
+                  task.degeneracy_threshold)
         h = C.c_void_p()
         _check(self.lib, self.lib.wnlqg_create(C.byref(m), C.byref(t), C.byref(h)))
         self.handle = h
         self.num_states = self.lib.wnlqg_num_states(h)
         self.num_channels = self.lib.wnlqg_num_channels(h)
-        self.states = [s for s, on in (("sc", task.shift_current), ("ic", task.injection_current)) if on]
+        if geometry:
+            self.states = ["bcd"] if task.berry_curvature_dipole else ["ahc"]
+        else:
+            self.states = [s for s, on in (("sc", task.shift_current), ("ic", task.injection_current)) if on]
+        self.geometry = geometry
+        self._axis = self._mu.shape[0] if geometry else self._om.shape[0]
+        self._ncomp = 9 if task.intrinsic_hall and geometry else 27
 
     def close(self):
         if getattr(self, "handle", None):
@@ -188,22 +217,38 @@ class Engine:
         mesh_a = np.ascontiguousarray(mesh, dtype=np.int32)
         nk = int(np.prod(mesh_a.astype(np.int64)))
         nl = min(nk, 64) if n_lanes is None else int(n_lanes)
-        nw = self._om.shape[0]
-        lane_out = np.zeros(nl * self.num_states * 27 * nw * 2)
-        total_out = np.zeros(self.num_states * 27 * nw * 2)
+        nw = self._axis
+        lane_out = np.zeros(nl * self.num_states * self._ncomp * nw * 2)
+        total_out = np.zeros(self.num_states * self._ncomp * nw * 2)
         st = Stats()
         _check(self.lib, self.lib.wnlqg_run_mesh(self.handle, _ptr(mesh_a, C.c_int32), nl, _ptr(lane_out),
                                                  _ptr(total_out), C.byref(st)))
+        if self.geometry:
+            return ({self.states[0]: self._unpack_geometry(lane_out, (nl,))},
+                    {self.states[0]: self._unpack_geometry(total_out, ())}, st.as_dict())
         lanes = _unpack(lane_out, (nl,), self.num_states, nw)
         total = _unpack(total_out, (), self.num_states, nw)
         return ({s: lanes[..., i, :, :, :, :] for i, s in enumerate(self.states)},
                 {s: total[i] for i, s in enumerate(self.states)}, st.as_dict())
 
+    def _unpack_geometry(self, flat: np.ndarray, lead: tuple) -> np.ndarray:
+        """Complex column-major [n_mu,3,3,3] (BCD) or [n_mu,3,3] (AHC) -> real [..., n_mu, a, b(, c)]."""
+        z = flat.view(np.complex128)
+        if self._ncomp == 27:
+            z = z.reshape(lead + (3, 3, 3, self._axis))
+            perm = (len(lead) + 3, len(lead) + 2, len(lead) + 1, len(lead))
+        else:
+            z = z.reshape(lead + (3, 3, self._axis))
+            perm = (len(lead) + 2, len(lead) + 1, len(lead))
+        return np.transpose(z, tuple(range(len(lead))) + perm).real.copy()
+
     def eval_points(self, kpoints: np.ndarray):
         k = np.ascontiguousarray(kpoints, dtype=np.float64).reshape(-1, 3)
-        nw = self._om.shape[0]
-        out = np.zeros(k.shape[0] * self.num_states * 27 * nw * 2)
+        nw = self._axis
+        out = np.zeros(k.shape[0] * self.num_states * self._ncomp * nw * 2)
         _check(self.lib, self.lib.wnlqg_eval_points(self.handle, _ptr(k), k.shape[0], _ptr(out)))
+        if self.geometry:
+            return {self.states[0]: self._unpack_geometry(out, (k.shape[0],))}
         vals = _unpack(out, (k.shape[0],), self.num_states, nw)
         return {s: vals[:, i] for i, s in enumerate(self.states)}
 
