@@ -46,6 +46,7 @@ extern "C" {
 #define WNLQG_INJECTION_CURRENT 2 /* Conventional IC, ConventionalInjectionCurrent.jl:23-140 */
 #define WNLQG_BERRY_CURVATURE_DIPOLE 4 /* BCD, Fermi-surface form: BerryMultipoleMoments.jl:7-45 */
 #define WNLQG_INTRINSIC_HALL 8    /* intrinsic AHC, Matrix method: IntrinsicHallOccupations.jl:8-44 */
+#define WNLQG_SECOND_HARMONIC 16  /* SHG chi / sigma(-2w; w, w): SecondHarmonicKernels.jl:115-294 */
 
 /* Broadening (TransitionScreening.jl:1-29). */
 #define WNLQG_GAUSSIAN 0
@@ -99,7 +100,15 @@ typedef struct wnlqg_task {
     int32_t num_fermi_energies;       /* n_mu, 1 .. 64 */
     const double *fermi_energies;     /* [n_mu] eV */
     double fermi_surface_broadening;  /* eta_fs_ev > 0: Gaussian -f'(E) at T = 0 (BCD); must be 0 when T > 0 */
-    double degeneracy_threshold;      /* eV: BCD spectral isolation threshold (reference default 0.002) */
+    double degeneracy_threshold;      /* eV: BCD spectral isolation threshold, SHG one-band isolation
+                                         (reference default 0.002) */
+    /* SHG (Config.jl:107-109; SecondHarmonicDriver.jl:146-168).  SHG uses photon_energies, broadening,
+     * broadening_type, fermi_energy, temperature and degeneracy_threshold as well. */
+    int32_t num_shg_components;       /* 0 = all 18 in the reference order (a; bc = xx,yy,zz,yz,xz,xy) */
+    const int32_t *shg_components;    /* [3 * num_shg_components] 0-based (a, b, c) */
+    double shg_low_frequency_broadening;    /* eV (> 0; reference default 0.025) */
+    double shg_intermediate_regularization; /* eV (> 0) */
+    int32_t shg_eta_correction;       /* 1: eta-corrected optical position derivative (reference default) */
 } wnlqg_task;
 
 /* Counters of the last run (device event timers, milliseconds). */
@@ -161,6 +170,10 @@ int32_t wnlqg_num_channels(const wnlqg_engine *engine);
  * AHC engines: each lane block is [n_mu,3,3] column-major, the lane's sum of per-point
  * (-e^2/hbar 1e-20) Omega_ab(mu) (SpectralResponseDriver.jl:388-400); total_out = lanes summed / N_k /
  * (|det lattice| 1e-30), sigma_ab in S/m (SpectralResponseDriver.jl:541-552).
+ * SHG engines: each lane block is [n_w, n_comp, 7, 2] column-major, the lane's sum of raw per-point
+ * densities (the buffers[lane] of SecondHarmonicDriver.jl:178-185; term 1..7, quantity chi, sigma);
+ * total_out = lanes summed, quantity q times factors[q] / N_k with factors = (e/(eps0 V) 1e12,
+ * e^2/(hbar V)), V = |det lattice| (SecondHarmonicDriver.jl:135-139, 236-242).
  * stats may be NULL.
  */
 int32_t wnlqg_run_mesh(wnlqg_engine *engine, const int32_t mesh[3], int32_t n_lanes,
@@ -169,7 +182,8 @@ int32_t wnlqg_run_mesh(wnlqg_engine *engine, const int32_t mesh[3], int32_t n_la
 /*
  * Per-point values (point_value!, IntegralDriver.jl:596, with num_kpoints = 1) at m arbitrary
  * fractional k points.  kpoints: column-major [3,m].  out: complex [m][n_states][n_w*27]; for BCD
- * [m][n_mu*27] raw moments, for AHC [m][n_mu*9] scaled curvature (the per-point lane contributions).
+ * [m][n_mu*27] raw moments, for AHC [m][n_mu*9] scaled curvature, for SHG [m][n_w*n_comp*14] raw
+ * densities (the per-point lane contributions).
  */
 int32_t wnlqg_eval_points(wnlqg_engine *engine, const double *kpoints, int32_t m, double *out);
 

@@ -194,3 +194,57 @@ def _one_band(out, E, A, V, W, freqs, comps, bare, mu, temperature, lowb, gaussi
             out[:, ci, 5, 1] += 0.5j * bcd * kernel
             out[:, ci, 6, 0] -= 0.25j * semi * kernel * low ** 2
             out[:, ci, 6, 1] -= 0.5 * semi * kernel * low
+
+
+SHG_ELECTRIC_CONSTANT = 8.8541878128e-12
+SHG_CHARGE_C = 1.602176634e-19
+SHG_HBAR_JS = 1.054571817e-34
+
+
+def shg_point_values(model, kpts, frequencies, components=None, *, fermi_energy=0.0, temperature=0.0,
+                     broadening=0.02, low_frequency_broadening=0.025, gaussian=True,
+                     intermediate_regularization=1e-3, eta_correction=True, degeneracy_threshold=0.002):
+    """Raw per-point SHG densities [K, n_w, n_comp, 7, 2] (``point_value!`` of
+    ``Runtime/Execution/SecondHarmonicDriver.jl:146-168``)."""
+    from . import kloop
+    data = kloop.compute_kpoint_data(model, np.asarray(kpts, dtype=np.float64), 1e-3)
+    K = data.energies.shape[0]
+    comps = SECOND_HARMONIC_COMPONENTS if components is None else components
+    out = np.zeros((K, len(frequencies), len(comps), 7, 2), dtype=np.complex128)
+    for k in range(K):
+        out[k] = second_harmonic_response(
+            data.energies[k], data.internal_connection[k], data.hamiltonian_derivatives[k],
+            data.internal_connection_derivatives[k], data.hamiltonian_second_derivatives[k], frequencies, comps,
+            fermi_energy=fermi_energy, temperature=temperature, broadening=broadening,
+            low_frequency_broadening=low_frequency_broadening, gaussian=gaussian,
+            intermediate_regularization=intermediate_regularization, eta_correction=eta_correction,
+            degeneracy_threshold=degeneracy_threshold)
+    return out
+
+
+def shg_factors(model):
+    """(chi, sigma) unit factors e/(eps0 V) 1e12 and e^2/(hbar V), V = |det lattice|
+    (``SecondHarmonicDriver.jl:135-139``)."""
+    vol = abs(np.linalg.det(model.lattice))
+    return (SHG_CHARGE_C / (SHG_ELECTRIC_CONSTANT * vol) * 1e12, SHG_CHARGE_C ** 2 / (SHG_HBAR_JS * vol))
+
+
+def integrate_shg_mesh(model, mesh, frequencies, components=None, **kw):
+    """Dense-mesh SHG: lanes[l] = sum of raw point values over lane l (``consume_point!``,
+    ``SecondHarmonicDriver.jl:178-185``); total = lanes summed in order, quantity q scaled by
+    factors[q] / N_k (``finish!``, ``SecondHarmonicDriver.jl:221-242``)."""
+    from . import kloop
+    nk = int(np.prod(mesh))
+    nl = kloop.lane_count(nk)
+    comps = SECOND_HARMONIC_COMPONENTS if components is None else components
+    lanes = np.zeros((nl, len(frequencies), len(comps), 7, 2), dtype=np.complex128)
+    kp = kloop.integral_kpoints(mesh)
+    vals = shg_point_values(model, kp, frequencies, comps, **kw)
+    for lane in range(nl):
+        a, b = kloop.lane_range(nk, nl, lane)
+        lanes[lane] = vals[a:b].sum(axis=0)
+    total = lanes.sum(axis=0)
+    f = shg_factors(model)
+    for q in range(2):
+        total[..., q] *= f[q] / nk
+    return lanes, total

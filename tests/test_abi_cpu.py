@@ -96,3 +96,18 @@ def test_rejects_mixed_driver_families():
     with pytest.raises(E.WnlqgError) as ei:
         E.Engine(synthetic_model(4, 1, 0), _geom(intrinsic_hall=True))
     assert ei.value.code == -6
+
+
+@pytest.mark.parametrize("kw,code", [
+    (dict(shg_components=[(0, 1, 3)]), -1),
+    (dict(shg_low_frequency_broadening=0.0), -1),
+    (dict(shg_intermediate_regularization=-1.0), -1),
+    (dict(precision="mixed"), -6),
+    (dict(berry_curvature_dipole=True, fermi_energies=[0.0]), -6),   # SHG with BCD: two drivers
+])
+def test_rejects_invalid_shg_tasks(kw, code):
+    base = dict(photon_energies=np.linspace(0, 3, 10), second_harmonic=True, broadening=0.02)
+    base.update(kw)
+    with pytest.raises(E.WnlqgError) as ei:
+        E.Engine(synthetic_model(4, 1, 0), E.EngineTask(**base))
+    assert ei.value.code == code

@@ -76,8 +76,12 @@ struct DevBuf {
 }  // namespace
 
 struct wnlqg_engine {
-    int family = 0;     // 0 SC / IC, 1 BCD, 2 intrinsic AHC
+    int family = 0;     // 0 SC / IC, 1 BCD, 2 intrinsic AHC, 3 SHG
     WnGeom gp{};        // BCD / AHC parameters
+    WnShg sp{};         // SHG parameters
+    DevBuf<double2> der, coef, one;  // SHG optical position derivatives, pair coefficients, one-band sums
+    DevBuf<int2> pairs;               // SHG pair lists
+    DevBuf<double> pdiff;             // SHG pair energy differences
     int device = 0;
     hipStream_t stream = nullptr;
     WnShape sh{};
@@ -118,6 +122,7 @@ namespace {
 int task_family(int q) {
     if (q == WNLQG_BERRY_CURVATURE_DIPOLE) return 1;
     if (q == WNLQG_INTRINSIC_HALL) return 2;
+    if (q == WNLQG_SECOND_HARMONIC) return 3;
     if (q != 0 && (q & ~(WNLQG_SHIFT_CURRENT | WNLQG_INJECTION_CURRENT)) == 0) return 0;
     return -1;
 }
@@ -156,9 +161,23 @@ int check_task(const wnlqg_task *t, int n) {
     const int family = task_family(t->quantities);
     if (family < 0)
         return fail(WNLQG_E_UNSUPPORTED,
-                    "quantities bitmask %d: shift (1) and/or injection (2) current, or BCD (4) alone, or AHC (8) alone",
+                    "quantities bitmask %d: shift (1) and/or injection (2) current, or BCD (4), AHC (8) or SHG (16) alone",
                     t->quantities);
-    if (family > 0) return check_geometry_task(t, family);
+    if (family == 1 || family == 2) return check_geometry_task(t, family);
+    if (family == 3) {
+        if (t->num_shg_components < 0 || t->num_shg_components > 18 || (t->num_shg_components > 0 && !t->shg_components))
+            return fail(WNLQG_E_INVALID, "0 to 18 SHG components");
+        for (int i = 0; i < 3 * t->num_shg_components; ++i)
+            if (t->shg_components[i] < 0 || t->shg_components[i] > 2)
+                return fail(WNLQG_E_INVALID, "SHG component indices must be 0, 1 or 2");
+        if (!(t->shg_low_frequency_broadening > 0.0) || !std::isfinite(t->shg_low_frequency_broadening))
+            return fail(WNLQG_E_INVALID, "shg_low_frequency_broadening must be finite and positive");
+        if (!(t->shg_intermediate_regularization > 0.0) || !std::isfinite(t->shg_intermediate_regularization))
+            return fail(WNLQG_E_INVALID, "shg_intermediate_regularization must be finite and positive");
+        if (!(t->degeneracy_threshold >= 0.0) || !std::isfinite(t->degeneracy_threshold))
+            return fail(WNLQG_E_INVALID, "invalid degeneracy threshold");
+        if (t->precision != WNLQG_PRECISION_FP64) return fail(WNLQG_E_UNSUPPORTED, "SHG runs in FP64 only");
+    }
     if (t->num_photon_energies < 1 || !t->photon_energies)
         return fail(WNLQG_E_INVALID, "photon energies missing");
     for (int i = 0; i < t->num_photon_energies; ++i) {
@@ -190,8 +209,12 @@ int check_task(const wnlqg_task *t, int n) {
 int output_elems(const wnlqg_engine *g) {
     if (g->family == 1) return 27 * g->gp.nmu;
     if (g->family == 2) return 9 * g->gp.nmu;
+    if (g->family == 3) return 19 * g->sp.ncomp * g->sp.nw + g->sp.ncomp;
     return 27 * g->sc.nq * g->sc.nw;
 }
+
+// complex values per k point (or lane) handed to the caller
+int result_elems(const wnlqg_engine *g) { return g->family == 3 ? 14 * g->sp.ncomp * g->sp.nw : output_elems(g); }
 
 int ensure_rows(wnlqg_engine *g, int rows) {
     // working set for `rows` k points (multiple of 128)
@@ -208,6 +231,14 @@ int ensure_rows(wnlqg_engine *g, int rows) {
     WN_HIP(g->e.alloc((size_t)rows * g->sh.n));
     WN_HIP(g->u.alloc((size_t)rows * n2));
     WN_HIP(g->sweeps.alloc((size_t)rows));
+    if (g->family == 3) {
+        WN_HIP(g->count.alloc((size_t)rows));
+        WN_HIP(g->der.alloc((size_t)rows * 9 * n2));
+        WN_HIP(g->pairs.alloc((size_t)rows * n2));
+        WN_HIP(g->pdiff.alloc((size_t)rows * n2));
+        WN_HIP(g->coef.alloc((size_t)rows * n2 * 19 * g->sp.ncomp));
+        WN_HIP(g->one.alloc((size_t)rows * g->sp.ncomp));
+    }
     if (g->family == 0) {
         WN_HIP(g->count.alloc((size_t)rows));
         WN_HIP(g->pmeta.alloc((size_t)rows * n2));
@@ -289,6 +320,65 @@ int32_t wnlqg_last_error(char *buf, size_t len) {
 }
 
 int32_t wnlqg_num_states(const wnlqg_engine *g) { return g ? (g->family ? 1 : g->sc.nq) : 0; }
+
+namespace {
+
+// Frequency-only factors of the SHG lane sums (SecondHarmonicKernels.jl:86-98, 200-236, 280-292):
+// acc = [19 nc][nw] column sums then [nc] (one-band BCD, semiclassical) sums -> [nw, nc, 7, 2] column-major.
+void shg_finalize(const wnlqg_engine *g, const double2 *acc, double *out) {
+    const WnShg &sp = g->sp;
+    const int nc = sp.ncomp, nw = sp.nw;
+    const double2 *one = acc + (size_t)19 * nc * nw;
+    for (size_t i = 0; i < (size_t)nw * nc * 14 * 2; ++i) out[i] = 0.0;
+    auto at = [&](int w, int ci, int t, int q) { return out + 2 * ((size_t)w + (size_t)nw * (ci + (size_t)nc * (t + 7 * q))); };
+    auto add = [&](int w, int ci, int t, int q, double re, double im) {
+        double *o = at(w, ci, t, q);
+        o[0] += re;
+        o[1] += im;
+    };
+    for (int w = 0; w < nw; ++w) {
+        const double om = g->omega_host[w];
+        const double lb = sp.lowb;
+        const double low = om / (om * om + lb * lb);
+        const double y = om / lb;
+        const double dl = sp.gaussian ? std::exp(-y * y) / (std::sqrt(kPi) * lb) : 1.0 / (kPi * lb * (1.0 + y * y));
+        // kernel = low - i delta
+        const double kr = low, ki = -dl;
+        for (int ci = 0; ci < nc; ++ci) {
+            auto col = [&](int c) { return acc[(size_t)c * nw + w]; };
+            const int d1 = ci * 8, d2 = 8 * nc + ci * 6, dn = 14 * nc + ci * 2, lw = 16 * nc + ci * 3;
+            double2 v;
+            v = col(d1 + 0); add(w, ci, 0, 0, v.x, v.y);
+            v = col(d1 + 1); add(w, ci, 1, 0, v.x, v.y);
+            v = col(d1 + 2); add(w, ci, 3, 0, v.x, v.y);
+            v = col(d1 + 3); add(w, ci, 4, 0, v.x * low * low, v.y * low * low);
+            v = col(d1 + 4); add(w, ci, 0, 1, v.x, v.y);
+            v = col(d1 + 5); add(w, ci, 1, 1, v.x, v.y);
+            v = col(d1 + 6); add(w, ci, 3, 1, v.x, v.y);
+            v = col(d1 + 7); add(w, ci, 4, 1, v.x * low, v.y * low);
+            v = col(d2 + 0); add(w, ci, 0, 0, v.x, v.y);
+            v = col(d2 + 1); add(w, ci, 1, 0, v.x, v.y);
+            v = col(d2 + 2); add(w, ci, 2, 0, v.x, v.y);
+            v = col(d2 + 3); add(w, ci, 0, 1, v.x, v.y);
+            v = col(d2 + 4); add(w, ci, 1, 1, v.x, v.y);
+            v = col(d2 + 5); add(w, ci, 2, 1, v.x, v.y);
+            v = col(dn + 0); add(w, ci, 2, 0, v.x, v.y);
+            v = col(dn + 1); add(w, ci, 2, 1, v.x, v.y);
+            v = col(lw + 0); add(w, ci, 0, 0, v.x * low, v.y * low);
+            v = col(lw + 1); add(w, ci, 1, 0, v.x * low, v.y * low);
+            v = col(lw + 2); add(w, ci, 3, 0, v.x * low, v.y * low);
+            const double bcd = one[ci].x, semi = one[ci].y;
+            // t5q0 -= bcd kernel low / 4;  t5q1 += i bcd kernel / 2
+            add(w, ci, 5, 0, -0.25 * bcd * kr * low, -0.25 * bcd * ki * low);
+            add(w, ci, 5, 1, -0.5 * bcd * ki, 0.5 * bcd * kr);
+            // t6q0 -= i semi kernel low^2 / 4;  t6q1 -= semi kernel low / 2
+            add(w, ci, 6, 0, 0.25 * semi * ki * low * low, -0.25 * semi * kr * low * low);
+            add(w, ci, 6, 1, -0.5 * semi * kr * low, -0.5 * semi * ki * low);
+        }
+    }
+}
+
+}  // namespace
 int32_t wnlqg_num_channels(const wnlqg_engine *g) { return g ? g->sh.C : 0; }
 
 int32_t wnlqg_destroy(wnlqg_engine *g) {
@@ -356,7 +446,27 @@ int32_t wnlqg_create(const wnlqg_model *model, const wnlqg_task *task, wnlqg_eng
         g->gp.eta = task->denominator_regularization;
         g->gp.gap_tol = task->degeneracy_threshold;
     }
-    g->sc.nw = g->family ? task->num_fermi_energies : task->num_photon_energies;
+    if (g->family == 3) {
+        static const int kDefault[18][3] = {{0, 0, 0}, {0, 1, 1}, {0, 2, 2}, {0, 1, 2}, {0, 0, 2}, {0, 0, 1},
+                                            {1, 0, 0}, {1, 1, 1}, {1, 2, 2}, {1, 1, 2}, {1, 0, 2}, {1, 0, 1},
+                                            {2, 0, 0}, {2, 1, 1}, {2, 2, 2}, {2, 1, 2}, {2, 0, 2}, {2, 0, 1}};
+        WnShg &sp = g->sp;
+        sp.ncomp = task->num_shg_components ? task->num_shg_components : 18;
+        for (int i = 0; i < sp.ncomp; ++i) {
+            const int *c = task->num_shg_components ? task->shg_components + 3 * i : kDefault[i];
+            sp.comp[i] = c[0] + 3 * c[1] + 9 * c[2];
+        }
+        sp.nw = task->num_photon_energies;
+        sp.gaussian = task->broadening_type == WNLQG_GAUSSIAN;
+        sp.width = task->broadening;
+        sp.lowb = task->shg_low_frequency_broadening;
+        sp.eta = task->shg_intermediate_regularization;
+        sp.eta_correction = task->shg_eta_correction != 0;
+        sp.mu = task->fermi_energy;
+        sp.temperature = task->temperature;
+        sp.tol = task->degeneracy_threshold;
+    }
+    g->sc.nw = (g->family == 1 || g->family == 2) ? task->num_fermi_energies : task->num_photon_energies;
     g->sc.gaussian = task->broadening_type == WNLQG_GAUSSIAN;
     g->sc.width = task->broadening;
     g->sc.mu = task->fermi_energy;
@@ -373,7 +483,7 @@ int32_t wnlqg_create(const wnlqg_model *model, const wnlqg_task *task, wnlqg_eng
     if (g->family == 1) g->pref[0] = 1.0;
     if (g->family == 2) g->pref[0] = -kCharge * kCharge / kHbarJs * 1e-20;
     // the frequency axis of SC / IC, or the Fermi-energy axis of BCD / AHC
-    const double *axis = g->family ? task->fermi_energies : task->photon_energies;
+    const double *axis = (g->family == 1 || g->family == 2) ? task->fermi_energies : task->photon_energies;
     g->omega_host.assign(axis, axis + g->sc.nw);
 
     // shape
@@ -387,7 +497,7 @@ int32_t wnlqg_create(const wnlqg_model *model, const wnlqg_task *task, wnlqg_eng
             const bool pos = l >= WN_CH_A;
             const bool second = l >= WN_CH_DDH && l < WN_CH_A;
             if (pos && !g->sh.has_pos) continue;
-            if (second && g->family != 0) continue;
+            if (second && (g->family == 1 || g->family == 2)) continue;
             g->sh.lch[c++] = (signed char)l;
         }
         g->sh.C = c;
@@ -508,6 +618,7 @@ int32_t wnlqg_create(const wnlqg_model *model, const wnlqg_task *task, wnlqg_eng
     // batch size: interpolation rows + pair buffers within ~24 GB unless the caller fixed it
     const size_t per_k = (size_t)g->sh.ldo * 8 +
                          (g->family == 0 ? n2 * (sizeof(int4) + sizeof(double2) + 27 * g->sc.nq * 16) : 0) +
+                         (g->family == 3 ? n2 * (9 * 16 + 8 + 8 + 19 * (size_t)g->sp.ncomp * 16) : 0) +
                          n2 * 16 + (size_t)3 * g->sh.S * 16;
     long long b = task->batch_size > 0 ? task->batch_size : (long long)(24.0e9 / (double)per_k);
     b = std::max(128LL, std::min(b, 65536LL));
@@ -621,11 +732,22 @@ int32_t wnlqg_eval_points(wnlqg_engine *g, const double *kpoints, int32_t m, dou
         for (int i = 0; i < cnt; ++i) ch[i] = make_int4(i, 1, i, 0);
         WN_HIP(hipStreamSynchronize(g->stream));
         WN_HIP(hipMemcpy(g->chunks.p, ch.data(), cnt * sizeof(int4), hipMemcpyHostToDevice));
-        WN_HIP(launch_response(g, g->chunks.p, cnt, cnt, p0));
+        if (g->family == 3) {
+            WN_HIP(wn_launch_shg(g->x.p, g->e.p, g->omega.p, g->chunks.p, cnt, cnt, g->der.p, g->count.p, g->pairs.p,
+                                 g->pdiff.p, g->coef.p, g->one.p, g->partial.p, g->sh, g->sp, g->stream));
+        } else {
+            WN_HIP(launch_response(g, g->chunks.p, cnt, cnt, p0));
+        }
         part.resize((size_t)cnt * nelem * 2);
         WN_HIP(hipMemcpyAsync(part.data(), g->partial.p, part.size() * sizeof(double), hipMemcpyDeviceToHost,
                               g->stream));
         WN_HIP(hipStreamSynchronize(g->stream));
+        if (g->family == 3) {
+            const size_t rel = (size_t)result_elems(g);
+            for (int i = 0; i < cnt; ++i)
+                shg_finalize(g, (const double2 *)(part.data() + (size_t)i * nelem * 2), out + (size_t)(p0 + i) * rel * 2);
+            continue;
+        }
         for (int i = 0; i < cnt; ++i)
             for (int qv = 0; qv < g->sc.nq; ++qv)
                 for (int e = 0; e < per_state; ++e) {
@@ -637,7 +759,7 @@ int32_t wnlqg_eval_points(wnlqg_engine *g, const double *kpoints, int32_t m, dou
     }
     int rc = check_flags(g);
     if (rc) return rc;
-    for (size_t i = 0; i < (size_t)m * nelem * 2; ++i)
+    for (size_t i = 0; i < (size_t)m * result_elems(g) * 2; ++i)
         if (!std::isfinite(out[i])) return fail(WNLQG_E_NONFINITE, "non-finite response value");
     return WNLQG_OK;
 }
@@ -768,6 +890,12 @@ int32_t wnlqg_run_mesh(wnlqg_engine *g, const int32_t mesh[3], int32_t n_lanes, 
             mk[5] = mark();
             WN_HIP(wn_launch_accum(dchunks.p + bchunk[bi], nch, g->count.p, g->pmeta.p, g->pmd.p, g->q.p, g->omega.p,
                                    g->partial.p, g->sh, g->sc, g->stream));
+        } else if (g->family == 3) {
+            WN_HIP(wn_launch_shg(g->x.p, g->e.p, g->omega.p, dchunks.p + bchunk[bi], 0, cnt, g->der.p, g->count.p,
+                                 g->pairs.p, g->pdiff.p, g->coef.p, g->one.p, g->partial.p, g->sh, g->sp, g->stream));
+            mk[5] = mark();
+            WN_HIP(wn_launch_shg(g->x.p, g->e.p, g->omega.p, dchunks.p + bchunk[bi], nch, 0, g->der.p, g->count.p,
+                                 g->pairs.p, g->pdiff.p, g->coef.p, g->one.p, g->partial.p, g->sh, g->sp, g->stream));
         } else {
             WN_HIP(wn_launch_geom_f64(g->x.p, g->e.p, g->omega.p, dchunks.p + bchunk[bi], nch, g->partial.p,
                                       g->flags.p, k0, g->sh, g->gp, g->stream));
@@ -811,6 +939,28 @@ int32_t wnlqg_run_mesh(wnlqg_engine *g, const int32_t mesh[3], int32_t n_lanes, 
     st.separable = sep ? 1 : 0;
     if (stats) *stats = st;
 
+    if (g->family == 3) {
+        // SHG: lane = raw sums (SecondHarmonicDriver.jl:178-185); total quantity q * factors[q] / N_k
+        const size_t rel = (size_t)result_elems(g);
+        for (int l = 0; l < n_lanes; ++l)
+            shg_finalize(g, (const double2 *)(lanes.data() + (size_t)l * nelem * 2), lane_out + (size_t)l * rel * 2);
+        for (size_t i = 0; i < (size_t)n_lanes * rel * 2; ++i)
+            if (!std::isfinite(lane_out[i])) return fail(WNLQG_E_NONFINITE, "Nonfinite SHG response");
+        if (total_out) {
+            const double vol = std::fabs(g->cell_volume);
+            const double factors[2] = {kCharge / (8.8541878128e-12 * vol) * 1e12, kCharge * kCharge / (kHbarJs * vol)};
+            for (size_t i = 0; i < rel * 2; ++i) total_out[i] = 0.0;
+            for (int l = 0; l < n_lanes; ++l)
+                for (size_t i = 0; i < rel * 2; ++i) total_out[i] += lane_out[(size_t)l * rel * 2 + i];
+            // column-major [nw, nc, 7, 2]: quantity q occupies complex entries [q rel/2, (q+1) rel/2)
+            for (size_t i = 0; i < rel; ++i) {
+                const double f = factors[i < rel / 2 ? 0 : 1] / (double)nk;
+                total_out[2 * i] *= f;
+                total_out[2 * i + 1] *= f;
+            }
+        }
+        return WNLQG_OK;
+    }
     // SC / IC: each state * its prefactor, then 1/N_k (consume_point!, IntegralDriver.jl:820-874).
     // BCD / AHC: lanes hold raw sums of the per-point values (BerryMultipoleDriver.jl:279-285); the total is
     // divided by N_k and multiplied by the measure (BerryMultipoleDriver.jl:290-303, SpectralResponseDriver.jl:541-552).

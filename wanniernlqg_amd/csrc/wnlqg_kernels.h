@@ -33,6 +33,21 @@ struct WnGeom {
 };
 #define WN_GEOM_MAXMU 64
 
+// Second-harmonic generation parameters (SecondHarmonicKernels.jl:115-236, Config.jl:107-109).
+struct WnShg {
+    int ncomp;             // requested components (1 .. 18)
+    int comp[18];          // a + 3b + 9c of each component
+    int nw;                // photon energies
+    int gaussian;          // 1 Gaussian, 0 Lorentzian
+    double width;          // broadening (eV)
+    double lowb;           // low-frequency broadening (eV)
+    double eta;            // intermediate regularization (eV)
+    int eta_correction;    // 1: eta-corrected optical position derivative
+    double mu;             // Fermi energy
+    double temperature;    // K
+    double tol;            // degeneracy threshold of the one-band BCD term
+};
+
 struct WnScreen {
     int nw;             // photon energies
     int gaussian;       // 1 Gaussian, 0 Lorentzian
@@ -99,6 +114,12 @@ hipError_t wn_launch_accum(const int4 *chunks, int nchunks, const int *count, co
 hipError_t wn_launch_geom_f64(const double *x, const double *e, const double *mu, const int4 *chunks, int nchunks,
                               double2 *partial, int *flags, long long kbase, const WnShape &sh, const WnGeom &gp,
                               hipStream_t st);
+// SHG: optical position derivatives der[k][9][n][n], pair lists and coefficient rows for `count_k` k
+// points, then the frequency contraction of `nchunks` chunks into partial[chunk] = [19 nc][nw] complex
+// followed by [nc] (one-band BCD, semiclassical) sums.
+hipError_t wn_launch_shg(const double *x, const double *e, const double *omega, const int4 *chunks, int nchunks,
+                         int count_k, double2 *der, int *count, int2 *pairs, double *pdiff, double2 *coef,
+                         double2 *one, double2 *partial, const WnShape &sh, const WnShg &sp, hipStream_t st);
 // Fixed-order chunk -> lane sums.
 hipError_t wn_launch_lane_reduce(const int4 *lanes, int nlanes, const double2 *partial, double2 *lanebuf,
                                  double2 *lanecomp, int nelem, hipStream_t st);

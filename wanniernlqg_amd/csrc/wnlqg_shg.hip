@@ -1,0 +1,425 @@
+// Second-harmonic generation (SHG) per-k stage (reference
+// src/Responses/SecondHarmonicGeneration/SecondHarmonicKernels.jl:71-294 and
+// src/MatrixElements/Interpolated/OpticalPositionDerivatives.jl:13-143).
+//
+// Per k point (Hamiltonian-gauge channels from the rotation kernel, ascending eigenvalues):
+//   occupations f_n (strict step at T <= 1e-7 K), active pairs n != m with |f_n - f_m| >= 1e-10;
+//   bare and regularized vertices  r0 = A + i V/(E_m - E_n) (1/d cut below 1e-7 eV),
+//   r_eta = A + i V (E_m - E_n)/((E_m - E_n)^2 + eta^2);
+//   k_shg_der   : the optical position derivative der[a][b][n][m] of every active pair;
+//   k_shg_coef  : per active pair and component (a,b,c) the seven-term coefficients of the three
+//                 resonances d1 = R(dE - w), d2 = R(dE - 2w), dn = R(-dE - w), R(x) = PV(x) + i pi delta(x),
+//                 and of the k-independent low-frequency factor; per band the one-band (Fermi-surface)
+//                 BCD and semiclassical sums at T > 0;
+//   k_shg_accum : sum over the chunk's k points and pairs of resonance x coefficient for every
+//                 frequency (lane = frequency), in a fixed order.
+// The remaining factors depend on the frequency only (low = PV(w, w_low), the one-band kernel
+// low - i delta(w, w_low)) and are applied to the lane sums on the host, which is exact because the
+// k sum is linear.
+//
+// Coefficient columns of one pair (nc requested components, column = kind block + comp * width + j):
+//   d1 block (width 8): t0q0 f inv second, t1q0 f inv third, t3q0 -f inv fourth/2, t4q0 -f fifth/4 (x low^2),
+//                       t0q1 o second,    t1q1 o third,     t3q1 -o fourth/2,     t4q1 -o fifth/4 (x low)
+//   d2 block (width 6): t0q0 2 f inv first, t1q0 -4 f inv third, t2q0 o tf, t0q1 o first, t1q1 -2 o third,
+//                       t2q1 -i o dE tf
+//   dn block (width 2): t2q0 o to/2, t2q1 i o dE to
+//   low block (width 3): t0q0 f inv (first + second), t1q0 -f inv third, t3q0 -f inv fourth/2
+// with o = f_n - f_m, f = i o / 2, inv = 1/dE, dE = E_m - E_n.
+#include "wnlqg_common.h"
+#include "wnlqg_kernels.h"
+
+#include <math.h>
+
+#define WN_SHG_TPB 256
+
+namespace {
+
+constexpr double kShgKb = 8.617333262145e-5;
+
+__device__ inline double shg_occupation(double e, double mu, double temperature) {
+    if (temperature <= 1.0e-7) return e <= mu ? 1.0 : 0.0;
+    const double x = (e - mu) / (kShgKb * temperature);
+    return x >= 0.0 ? exp(-x) / (1.0 + exp(-x)) : 1.0 / (1.0 + exp(x));
+}
+
+// 1/d with the 1e-7 eV cut (PositionVertexRegularization.jl:1-8)
+__device__ inline double shg_bare_inverse(double d) { return fabs(d) < 1.0e-7 ? 0.0 : 1.0 / d; }
+
+__device__ inline double shg_principal(double x, double w) { return x / (x * x + w * w); }
+
+__device__ inline double shg_weight(double d, double eta) { return (eta * eta) / (d * d + eta * eta); }
+
+struct ShgView {
+    const double2 *A[3];   // A_H (nullptr without a position operator)
+    const double2 *V[3];   // dH_H
+    const double2 *dA[9];  // d_b A_a at 3a + b
+    const double2 *W[6];   // ddH_H, unique (xx, yy, zz, yz, xz, xy)
+    const double *E;
+    int n;
+    double eta;
+};
+
+__device__ inline double2 sv_A(const ShgView &s, int a, int i, int j) {
+    return s.A[0] ? s.A[a][i * s.n + j] : make_double2(0.0, 0.0);
+}
+__device__ inline double2 sv_V(const ShgView &s, int a, int i, int j) { return s.V[a][i * s.n + j]; }
+// bare vertex part V_ij / (E_j - E_i) (cut) and regularized part
+__device__ inline double2 sv_bare(const ShgView &s, int a, int i, int j) {
+    return cscale(sv_V(s, a, i, j), shg_bare_inverse(s.E[j] - s.E[i]));
+}
+__device__ inline double2 sv_reg(const ShgView &s, int a, int i, int j) {
+    return cscale(sv_V(s, a, i, j), wn_reg_inverse(s.E[j] - s.E[i], s.eta));
+}
+// r0 = A + i bare, r_eta = A + i reg
+__device__ inline double2 sv_dip(const ShgView &s, int a, int i, int j) {
+    return cadd(sv_A(s, a, i, j), cmuli(sv_bare(s, a, i, j)));
+}
+__device__ inline double2 sv_rdip(const ShgView &s, int a, int i, int j) {
+    return cadd(sv_A(s, a, i, j), cmuli(sv_reg(s, a, i, j)));
+}
+
+__device__ inline ShgView make_view(const double *x, const double *e, long long k, const WnShape &sh, double eta) {
+    ShgView s;
+    const int n = sh.n, n2 = n * n;
+    const double2 *xk = (const double2 *)(x + (size_t)k * sh.ldo);
+    for (int a = 0; a < 3; ++a) {
+        s.V[a] = xk + (size_t)(WN_CH_DH + a) * n2;
+        s.A[a] = sh.has_pos ? xk + (size_t)(WN_CH_A + a) * n2 : nullptr;
+    }
+    for (int a = 0; a < 9; ++a) s.dA[a] = sh.has_pos ? xk + (size_t)(WN_CH_DA + a) * n2 : nullptr;
+    for (int a = 0; a < 6; ++a) s.W[a] = xk + (size_t)(WN_CH_DDH + a) * n2;
+    s.E = e + (size_t)k * n;
+    s.n = n;
+    s.eta = eta;
+    return s;
+}
+
+__device__ inline bool shg_active(const double *occ, int i, int j) { return i != j && fabs(occ[i] - occ[j]) >= 1.0e-10; }
+
+}  // namespace
+
+// ------------------------------------------------------------------------------------------------
+// der[k][3a+b][n][m] (OpticalPositionDerivatives.jl:13-130) for active pairs, 0 elsewhere.
+// Grid (count, 9): block (k, ab).
+// ------------------------------------------------------------------------------------------------
+__global__ __launch_bounds__(WN_SHG_TPB) void k_shg_der(const double *__restrict__ x, const double *__restrict__ eg,
+                                                       double2 *__restrict__ der, WnShape sh, WnShg sp) {
+    __shared__ double occ[64];
+    const long long k = blockIdx.x;
+    const int ab = blockIdx.y, a = ab / 3, b = ab - (ab / 3) * 3;
+    const int n = sh.n, n2 = n * n;
+    const ShgView s = make_view(x, eg, k, sh, sp.eta);
+    const int t = threadIdx.x;
+    if (t < n) occ[t] = shg_occupation(s.E[t], sp.mu, sp.temperature);
+    __syncthreads();
+    double2 *out = der + ((size_t)k * 9 + ab) * n2;
+    const bool pos = s.A[0] != nullptr;
+    for (int e = t; e < n2; e += WN_SHG_TPB) {
+        const int nn = e / n, m = e - (e / n) * n;
+        if (!shg_active(occ, nn, m)) {
+            out[e] = make_double2(0.0, 0.0);
+            continue;
+        }
+        // commutators [A_a, reg_b] and [V_a, reg_b]
+        double2 ca = make_double2(0.0, 0.0), ch = make_double2(0.0, 0.0);
+        double2 eta_num = make_double2(0.0, 0.0);
+        for (int p = 0; p < n; ++p) {
+            const double2 rpm = sv_reg(s, b, p, m), rnp = sv_reg(s, b, nn, p);
+            if (pos) ca = csub(cadd(ca, cmul(sv_A(s, a, nn, p), rpm)), cmul(rnp, sv_A(s, a, p, m)));
+            ch = csub(cadd(ch, cmul(sv_V(s, a, nn, p), rpm)), cmul(rnp, sv_V(s, a, p, m)));
+            if (sp.eta_correction && p != nn && p != m && pos) {
+                const double wpm = shg_weight(s.E[p] - s.E[m], sp.eta);
+                const double wnp = shg_weight(s.E[nn] - s.E[p], sp.eta);
+                const double2 Aanp = sv_A(s, a, nn, p), Vbpm = sv_V(s, b, p, m), Abpm = sv_A(s, b, p, m);
+                const double2 Vanp = sv_V(s, a, nn, p);
+                // V_a,np + i (E_n - E_p) A_a,np
+                const double2 van = cadd(Vanp, cmuli(cscale(Aanp, s.E[nn] - s.E[p])));
+                eta_num = cadd(eta_num, cscale(csub(cmul(Aanp, Vbpm), cmul(van, Abpm)), wpm));
+                const double2 Vbnp = sv_V(s, b, nn, p), Aapm = sv_A(s, a, p, m), Abnp = sv_A(s, b, nn, p);
+                const double2 Vapm = sv_V(s, a, p, m);
+                const double2 vap = cadd(Vapm, cmuli(cscale(Aapm, s.E[p] - s.E[m])));
+                eta_num = csub(eta_num, cscale(csub(cmul(Vbnp, Aapm), cmul(Abnp, vap)), wnp));
+            }  // without a position operator every eta-correction product carries an A factor
+        }
+        const double2 Aan = sv_A(s, a, nn, nn), Aam = sv_A(s, a, m, m);
+        const double2 Abn = sv_A(s, b, nn, nn), Abm = sv_A(s, b, m, m);
+        const double2 Van = sv_V(s, a, nn, nn), Vam = sv_V(s, a, m, m);
+        const double2 Vbn = sv_V(s, b, nn, nn), Vbm = sv_V(s, b, m, m);
+        const double2 reg_b = sv_reg(s, b, nn, m);
+        const double2 bare_a = sv_bare(s, a, nn, m), bare_b = sv_bare(s, b, nn, m);
+        const double2 dAa = csub(Aan, Aam), dAb = csub(Abn, Abm);
+        const double2 sum_a = csub(ca, cmul(dAa, reg_b));
+        const double2 sum_h = csub(ch, cmul(csub(Van, Vam), reg_b));
+        const double2 dAab = pos ? s.dA[3 * a + b][nn * n + m] : make_double2(0.0, 0.0);
+        double2 finite = cadd(dAab, sum_a);
+        finite = cadd(finite, cmul(dAa, bare_b));
+        finite = cadd(finite, cmul(dAb, bare_a));
+        finite = csub(finite, cmuli(cmul(sv_A(s, a, nn, m), dAb)));
+        const double2 Wab = s.W[wn_sym(a, b)][nn * n + m];
+        double2 inner = cadd(Wab, sum_h);
+        inner = cadd(inner, cscale(bare_a, Vbn.x - Vbm.x));
+        inner = cadd(inner, cscale(bare_b, Van.x - Vam.x));
+        double2 numerator = cadd(cmuli(inner), eta_num);
+        const double iv = shg_bare_inverse(s.E[m] - s.E[nn]);
+        out[e] = iv == 0.0 ? finite : cadd(finite, cscale(numerator, iv));
+    }
+}
+
+// ------------------------------------------------------------------------------------------------
+// Pair list, coefficient rows and one-band sums of one k point (one workgroup per k).
+// pairs[k][p] = (n, m), pdiff[k][p] = E_m - E_n, coef[k][p][col], count[k]; one[k][comp][2] = (bcd, semi).
+// ------------------------------------------------------------------------------------------------
+__global__ __launch_bounds__(WN_SHG_TPB) void k_shg_coef(const double *__restrict__ x, const double *__restrict__ eg,
+                                                        const double2 *__restrict__ der, int *__restrict__ count,
+                                                        int2 *__restrict__ pairs, double *__restrict__ pdiff,
+                                                        double2 *__restrict__ coef, double2 *__restrict__ one,
+                                                        WnShape sh, WnShg sp) {
+    __shared__ double occ[64];
+    __shared__ int rowcnt[64];
+    __shared__ int rowoff[65];
+    extern __shared__ double lds_one[];  // [n][nc][2] one-band contributions
+    const long long k = blockIdx.x;
+    const int n = sh.n, n2 = n * n, nc = sp.ncomp;
+    const int ncol = 19 * nc;
+    const ShgView s = make_view(x, eg, k, sh, sp.eta);
+    const double2 *dk = der + (size_t)k * 9 * n2;
+    const int t = threadIdx.x;
+    if (t < n) occ[t] = shg_occupation(s.E[t], sp.mu, sp.temperature);
+    __syncthreads();
+    // n-major pair order (SecondHarmonicKernels.jl:140-141)
+    if (t < n) {
+        int c = 0;
+        for (int m = 0; m < n; ++m) c += shg_active(occ, t, m);
+        rowcnt[t] = c;
+    }
+    __syncthreads();
+    if (t == 0) {
+        rowoff[0] = 0;
+        for (int i = 0; i < n; ++i) rowoff[i + 1] = rowoff[i] + rowcnt[i];
+        count[k] = rowoff[n];
+    }
+    __syncthreads();
+    const size_t pbase = (size_t)k * n2;
+    if (t < n) {
+        int p = rowoff[t];
+        for (int m = 0; m < n; ++m)
+            if (shg_active(occ, t, m)) {
+                pairs[pbase + p] = make_int2(t, m);
+                pdiff[pbase + p] = s.E[m] - s.E[t];
+                ++p;
+            }
+    }
+    __syncthreads();
+    const int np = rowoff[n];
+    auto D = [&](int aa, int bb, int i, int j) { return dk[(size_t)(3 * aa + bb) * n2 + i * n + j]; };
+    for (int item = t; item < np * nc; item += WN_SHG_TPB) {
+        const int p = item / nc, ci = item - (item / nc) * nc;
+        const int2 pr = pairs[pbase + p];
+        const int nn = pr.x, m = pr.y;
+        const int cc = sp.comp[ci];
+        const int a = cc % 3, b = (cc / 3) % 3, c = cc / 9;
+        const double diff = s.E[m] - s.E[nn];
+        const double inverse = shg_bare_inverse(diff);
+        const double o = occ[nn] - occ[m];
+        const double2 dip_a_nm = sv_dip(s, a, nn, m);
+        const double2 dip_b_mn = sv_dip(s, b, m, nn), dip_c_mn = sv_dip(s, c, m, nn);
+        const double2 dip_b_nm = sv_dip(s, b, nn, m), dip_c_nm = sv_dip(s, c, nn, m);
+        const double2 first = cmul(dip_a_nm, cadd(D(b, c, m, nn), D(c, b, m, nn)));
+        const double2 second = cadd(cmul(D(a, b, nn, m), dip_c_mn), cmul(D(a, c, nn, m), dip_b_mn));
+        const double vc = sv_V(s, c, m, m).x - sv_V(s, c, nn, nn).x;
+        const double vb = sv_V(s, b, m, m).x - sv_V(s, b, nn, nn).x;
+        const double2 third =
+            cscale(cmul(dip_a_nm, cadd(cscale(dip_b_mn, vc), cscale(dip_c_mn, vb))), inverse);
+        const double2 fourth = cadd(cmul(D(b, a, nn, m), dip_c_mn), cmul(D(c, a, nn, m), dip_b_mn));
+        const double va = sv_V(s, a, nn, nn).x - sv_V(s, a, m, m).x;
+        const double2 fifth = cscale(cadd(cmul(dip_b_nm, dip_c_mn), cmul(dip_c_nm, dip_b_mn)), va);
+        double2 tf = make_double2(0.0, 0.0), to = make_double2(0.0, 0.0);
+        for (int q = 0; q < n; ++q) {
+            if (q == nn || q == m) continue;
+            const double2 rb_mq = sv_rdip(s, b, m, q), rc_qn = sv_rdip(s, c, q, nn);
+            const double2 rc_mq = sv_rdip(s, c, m, q), rb_qn = sv_rdip(s, b, q, nn);
+            const double pv0 = shg_principal(2.0 * s.E[q] - s.E[nn] - s.E[m], sp.eta);
+            tf = cadd(tf, cscale(cmul(dip_a_nm, cadd(cmul(rb_mq, rc_qn), cmul(rc_mq, rb_qn))), pv0));
+            const double pv1 = shg_principal(2.0 * s.E[nn] - s.E[m] - s.E[q], sp.eta);
+            const double2 ra_mq = sv_rdip(s, a, m, q);
+            to = cadd(to, cscale(cmul(ra_mq, cadd(cmul(rb_qn, dip_c_nm), cmul(rc_qn, dip_b_nm))), pv1));
+            const double pv2 = shg_principal(2.0 * s.E[m] - s.E[q] - s.E[nn], sp.eta);
+            const double2 ra_qn = sv_rdip(s, a, q, nn);
+            to = cadd(to, cscale(cmul(ra_qn, cadd(cmul(dip_b_nm, rc_mq), cmul(dip_c_nm, rb_mq))), pv2));
+        }
+        // f = i o / 2;  f * inv * z = i (o inv / 2) z
+        const double h = 0.5 * o;
+        const double hi = h * inverse;
+        double2 *row = coef + ((size_t)pbase + p) * ncol;
+        double2 *d1 = row + ci * 8;
+        double2 *d2 = row + nc * 8 + ci * 6;
+        double2 *dn = row + nc * 14 + ci * 2;
+        double2 *lw = row + nc * 16 + ci * 3;
+        d1[0] = cmuli(cscale(second, hi));
+        d1[1] = cmuli(cscale(third, hi));
+        d1[2] = cmuli(cscale(fourth, -0.5 * hi));
+        d1[3] = cmuli(cscale(fifth, -0.25 * h));
+        d1[4] = cscale(second, o);
+        d1[5] = cscale(third, o);
+        d1[6] = cscale(fourth, -0.5 * o);
+        d1[7] = cscale(fifth, -0.25 * o);
+        d2[0] = cmuli(cscale(first, 2.0 * hi));
+        d2[1] = cmuli(cscale(third, -4.0 * hi));
+        d2[2] = cscale(tf, o);
+        d2[3] = cscale(first, o);
+        d2[4] = cscale(third, -2.0 * o);
+        d2[5] = cmuli(cscale(tf, -o * diff));
+        dn[0] = cscale(to, 0.5 * o);
+        dn[1] = cmuli(cscale(to, o * diff));
+        lw[0] = cmuli(cscale(cadd(first, second), hi));
+        lw[1] = cmuli(cscale(third, -hi));
+        lw[2] = cmuli(cscale(fourth, -0.5 * hi));
+    }
+    // one-band terms (SecondHarmonicKernels.jl:245-294), T > 1e-7 only
+    double2 *ok = one + (size_t)k * nc;
+    if (sp.temperature <= 1.0e-7) {
+        for (int ci = t; ci < nc; ci += WN_SHG_TPB) ok[ci] = make_double2(0.0, 0.0);
+        return;
+    }
+    const double kt = kShgKb * sp.temperature;
+    for (int item = t; item < n * nc; item += WN_SHG_TPB) {
+        const int nn = item / nc, ci = item - (item / nc) * nc;
+        double bcd = 0.0, semi = 0.0;
+        const double off = s.E[nn] - sp.mu;
+        if (fabs(off) <= 5.0 * kt) {
+            const double d1 = -exp(-0.5 * (off / kt) * (off / kt)) / (kt * sqrt(2.0 * M_PI));
+            const double d2 = -off / (kt * kt) * d1;
+            const bool isolated = (nn == 0 || s.E[nn] - s.E[nn - 1] > sp.tol) && (nn == n - 1 || s.E[nn + 1] - s.E[nn] > sp.tol);
+            const int cc = sp.comp[ci];
+            const int a = cc % 3, b = (cc / 3) % 3, c = cc / 9;
+            double hess = s.W[wn_sym(b, c)][nn * n + nn].x;
+            for (int q = 0; q < n; ++q) {
+                if (q == nn) continue;
+                const double2 pr = cmul(sv_V(s, b, nn, q), sv_V(s, c, q, nn));
+                hess += 2.0 * pr.x * shg_principal(s.E[nn] - s.E[q], sp.eta);
+            }
+            const double va = sv_V(s, a, nn, nn).x, vb = sv_V(s, b, nn, nn).x, vc = sv_V(s, c, nn, nn).x;
+            semi = d2 * va * vb * vc + d1 * va * hess;
+            if (isolated) {
+                // curv[a][y] for y = c and y = b (zero on the diagonal)
+                double cv[2] = {0.0, 0.0};
+                const int ys[2] = {c, b};
+                for (int u = 0; u < 2; ++u) {
+                    const int y = ys[u];
+                    if (y == a) continue;
+                    double2 val = s.dA[0] ? csub(s.dA[3 * y + a][nn * n + nn], s.dA[3 * a + y][nn * n + nn])
+                                          : make_double2(0.0, 0.0);
+                    for (int q = 0; q < n; ++q) {
+                        const double2 ba_qn = sv_bare(s, a, q, nn), ba_nq = sv_bare(s, a, nn, q);
+                        const double2 by_qn = sv_bare(s, y, q, nn), by_nq = sv_bare(s, y, nn, q);
+                        val = cadd(val, cmul(sv_A(s, y, nn, q), ba_qn));
+                        val = csub(val, cmul(ba_nq, sv_A(s, y, q, nn)));
+                        val = csub(val, cmul(sv_A(s, a, nn, q), by_qn));
+                        val = cadd(val, cmul(by_nq, sv_A(s, a, q, nn)));
+                        val = csub(val, cmuli(csub(cmul(ba_nq, by_qn), cmul(by_nq, ba_qn))));
+                    }
+                    cv[u] = val.x;
+                }
+                bcd = d1 * (vb * cv[0] + vc * cv[1]);
+            }
+        }
+        lds_one[(nn * nc + ci) * 2] = bcd;
+        lds_one[(nn * nc + ci) * 2 + 1] = semi;
+    }
+    __syncthreads();
+    for (int ci = t; ci < nc; ci += WN_SHG_TPB) {
+        double sb = 0.0, ss = 0.0;
+        for (int nn = 0; nn < n; ++nn) {
+            sb += lds_one[(nn * nc + ci) * 2];
+            ss += lds_one[(nn * nc + ci) * 2 + 1];
+        }
+        ok[ci] = make_double2(sb, ss);
+    }
+}
+
+// ------------------------------------------------------------------------------------------------
+// Frequency contraction of one chunk: workgroup (chunk, frequency tile of 64, column tile of 128);
+// wave w owns columns [ct*128 + 32w, +32), lane = frequency.  partial[chunk] = [ncol][nw] complex,
+// then [nc] (bcd, semi) sums over the chunk's k points (written by column tile 0, frequency tile 0).
+// ------------------------------------------------------------------------------------------------
+__device__ inline double2 shg_resonance(double xx, double width, int gaussian) {
+    const double y = xx / width;
+    const double dl = gaussian ? exp(-y * y) / (sqrt(M_PI) * width) : 1.0 / (M_PI * width * (1.0 + y * y));
+    return make_double2(shg_principal(xx, width), M_PI * dl);
+}
+
+__global__ __launch_bounds__(WN_SHG_TPB) void k_shg_accum(const int4 *__restrict__ chunks, const int *__restrict__ count,
+                                                         const double *__restrict__ pdiff,
+                                                         const double2 *__restrict__ coef,
+                                                         const double2 *__restrict__ one,
+                                                         const double *__restrict__ omega, double2 *__restrict__ partial,
+                                                         int pcap, int nwt, int nct, WnShg sp) {
+    const int nc = sp.ncomp, ncol = 19 * nc, nw = sp.nw;
+    const int per_chunk = nwt * nct;
+    const int chunk = blockIdx.x / per_chunk;
+    const int rem = blockIdx.x - chunk * per_chunk;
+    const int wt = rem / nct, ct = rem - (rem / nct) * nct;
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int w = wt * 64 + lane;
+    const double om = w < nw ? omega[w] : 0.0;
+    const int c0 = ct * 128 + wave * 32;
+    // kinds of this wave's columns: 0 d1 [0, 8nc), 1 d2 [8nc, 14nc), 2 dn [14nc, 16nc), 3 low [16nc, 19nc)
+    const int b1 = 8 * nc, b2 = 14 * nc, b3 = 16 * nc;
+    const bool need1 = c0 < b1, need2 = c0 + 32 > b1 && c0 < b2, needn = c0 + 32 > b2 && c0 < b3;
+    double2 acc[32];
+#pragma unroll
+    for (int j = 0; j < 32; ++j) acc[j] = make_double2(0.0, 0.0);
+    const int4 ch = chunks[chunk];
+    for (int k = ch.x; k < ch.x + ch.y; ++k) {
+        const int cnt = count[k];
+        const size_t base = (size_t)k * pcap;
+        for (int p = 0; p < cnt; ++p) {
+            const double diff = pdiff[base + p];
+            const double2 r1 = need1 ? shg_resonance(diff - om, sp.width, sp.gaussian) : make_double2(0.0, 0.0);
+            const double2 r2 = need2 ? shg_resonance(diff - 2.0 * om, sp.width, sp.gaussian) : make_double2(0.0, 0.0);
+            const double2 rn = needn ? shg_resonance(-diff - om, sp.width, sp.gaussian) : make_double2(0.0, 0.0);
+            const double2 *row = coef + (base + p) * ncol;
+#pragma unroll
+            for (int j = 0; j < 32; ++j) {
+                const int col = c0 + j;
+                if (col < ncol) {
+                    const double2 r = col < b1 ? r1 : (col < b2 ? r2 : (col < b3 ? rn : make_double2(1.0, 0.0)));
+                    acc[j] = cfma(r, row[col], acc[j]);
+                }
+            }
+        }
+    }
+    double2 *o = partial + (size_t)chunk * ((size_t)ncol * nw + nc);
+    if (w < nw) {
+#pragma unroll
+        for (int j = 0; j < 32; ++j)
+            if (c0 + j < ncol) o[(size_t)(c0 + j) * nw + w] = acc[j];
+    }
+    if (wt == 0 && ct == 0 && threadIdx.x < nc) {
+        double sb = 0.0, ss = 0.0;
+        for (int k = ch.x; k < ch.x + ch.y; ++k) {
+            const double2 v = one[(size_t)k * nc + threadIdx.x];
+            sb += v.x;
+            ss += v.y;
+        }
+        o[(size_t)ncol * nw + threadIdx.x] = make_double2(sb, ss);
+    }
+}
+
+hipError_t wn_launch_shg(const double *x, const double *e, const double *omega, const int4 *chunks, int nchunks,
+                         int count_k, double2 *der, int *count, int2 *pairs, double *pdiff, double2 *coef,
+                         double2 *one, double2 *partial, const WnShape &sh, const WnShg &sp, hipStream_t st) {
+    if (sh.n > 64 || sp.ncomp < 1 || sp.ncomp > 18) return hipErrorInvalidValue;
+    if (count_k > 0) {
+        hipLaunchKernelGGL(k_shg_der, dim3(count_k, 9), dim3(WN_SHG_TPB), 0, st, x, e, der, sh, sp);
+        hipLaunchKernelGGL(k_shg_coef, dim3(count_k), dim3(WN_SHG_TPB), (size_t)sh.n * sp.ncomp * 2 * sizeof(double),
+                           st, x, e, der, count, pairs, pdiff, coef, one, sh, sp);
+    }
+    if (nchunks > 0) {
+        const int nwt = (sp.nw + 63) / 64;
+        const int nct = (19 * sp.ncomp + 127) / 128;
+        hipLaunchKernelGGL(k_shg_accum, dim3(nchunks * nwt * nct), dim3(WN_SHG_TPB), 0, st, chunks, count, pdiff, coef,
+                           one, omega, partial, sh.n * sh.n, nwt, nct, sp);
+    }
+    return hipGetLastError();
+}

@@ -23,6 +23,8 @@ WNLQG_SHIFT_CURRENT = 1
 WNLQG_INJECTION_CURRENT = 2
 WNLQG_BERRY_CURVATURE_DIPOLE = 4
 WNLQG_INTRINSIC_HALL = 8
+WNLQG_SECOND_HARMONIC = 16
+SECOND_HARMONIC_COMPONENTS = [(a, b, c) for a in range(3) for (b, c) in ((0, 0), (1, 1), (2, 2), (1, 2), (0, 2), (0, 1))]
 WNLQG_GAUSSIAN = 0
 WNLQG_LORENTZIAN = 1
 WNLQG_PRECISION_FP64 = 0
@@ -58,7 +60,9 @@ class _Task(C.Structure):
                 ("band_window_size", C.c_int32), ("precision", C.c_int32), ("batch_size", C.c_int32),
                 ("fourier_backend", C.c_int32), ("num_fermi_energies", C.c_int32),
                 ("fermi_energies", C.POINTER(C.c_double)), ("fermi_surface_broadening", C.c_double),
-                ("degeneracy_threshold", C.c_double)]
+                ("degeneracy_threshold", C.c_double), ("num_shg_components", C.c_int32),
+                ("shg_components", C.POINTER(C.c_int32)), ("shg_low_frequency_broadening", C.c_double),
+                ("shg_intermediate_regularization", C.c_double), ("shg_eta_correction", C.c_int32)]
 
 
 class Stats(C.Structure):
@@ -131,6 +135,11 @@ class EngineTask:
     fermi_energies: Optional[np.ndarray] = None
     fermi_surface_broadening: float = 0.0
     degeneracy_threshold: float = 0.002
+    second_harmonic: bool = False             # SHG alone (SecondHarmonicDriver.jl)
+    shg_components: Optional[Sequence] = None  # (a, b, c) triples, 0-based; None = all 18
+    shg_low_frequency_broadening: float = 0.025
+    shg_intermediate_regularization: float = 1.0e-3
+    shg_eta_correction: bool = True
     broadening: float = 0.040
     broadening_type: str = "Gaussian"
     fermi_energy: float = 0.0
@@ -160,13 +169,18 @@ class Engine:
         if model.position_r is not None:
             self._ar = np.ascontiguousarray(np.transpose(model.position_r, (0, 1, 3, 2))).view(np.float64).ravel()
         geometry = task.berry_curvature_dipole or task.intrinsic_hall
+        shg = task.second_harmonic
+        comps = SECOND_HARMONIC_COMPONENTS if task.shg_components is None else [tuple(c) for c in task.shg_components]
+        self._comps = np.ascontiguousarray(np.array(comps, dtype=np.int32).ravel())
         self._om = np.ascontiguousarray(np.zeros(1) if task.photon_energies is None else task.photon_energies,
                                         dtype=np.float64)
         self._mu = np.ascontiguousarray(np.atleast_1d(np.zeros(1) if task.fermi_energies is None
                                                       else task.fermi_energies), dtype=np.float64)
         m = _Model(n, nr, _ptr(self._lat), _ptr(self._rv, C.c_int32), _ptr(self._deg, C.c_int32), _ptr(self._hr),
                    _ptr(self._ar) if self._ar is not None else C.POINTER(C.c_double)())
-        if geometry:
+        if shg:
+            q = WNLQG_SECOND_HARMONIC | (WNLQG_BERRY_CURVATURE_DIPOLE if geometry else 0)
+        elif geometry:
             q = (WNLQG_BERRY_CURVATURE_DIPOLE if task.berry_curvature_dipole else 0) | \
                 (WNLQG_INTRINSIC_HALL if task.intrinsic_hall else 0)
         else:
@@ -179,13 +193,19 @@ class Engine:
                   task.denominator_regularization, task.transition_window_factor, task.band_window_size, prec,
                   task.batch_size, {"auto": 0, "direct": 1, "planes": 2}[task.fourier_backend],
                   self._mu.shape[0] if geometry else 0, _ptr(self._mu), task.fermi_surface_broadening,
-                  task.degeneracy_threshold)
+                  task.degeneracy_threshold, 0 if task.shg_components is None else len(comps),
+                  _ptr(self._comps, C.c_int32), task.shg_low_frequency_broadening,
+                  task.shg_intermediate_regularization, 1 if task.shg_eta_correction else 0)
         h = C.c_void_p()
         _check(self.lib, self.lib.wnlqg_create(C.byref(m), C.byref(t), C.byref(h)))
         self.handle = h
         self.num_states = self.lib.wnlqg_num_states(h)
         self.num_channels = self.lib.wnlqg_num_channels(h)
-        if geometry:
+        self.shg = shg
+        self.shg_components = comps
+        if shg:
+            self.states = ["shg"]
+        elif geometry:
             self.states = ["bcd"] if task.berry_curvature_dipole else ["ahc"]
         else:
             self.states = [s for s, on in (("sc", task.shift_current), ("ic", task.injection_current)) if on]
@@ -218,11 +238,14 @@ class Engine:
         nk = int(np.prod(mesh_a.astype(np.int64)))
         nl = min(nk, 64) if n_lanes is None else int(n_lanes)
         nw = self._axis
-        lane_out = np.zeros(nl * self.num_states * self._ncomp * nw * 2)
-        total_out = np.zeros(self.num_states * self._ncomp * nw * 2)
+        per = self._result_elems()
+        lane_out = np.zeros(nl * per * 2)
+        total_out = np.zeros(per * 2)
         st = Stats()
         _check(self.lib, self.lib.wnlqg_run_mesh(self.handle, _ptr(mesh_a, C.c_int32), nl, _ptr(lane_out),
                                                  _ptr(total_out), C.byref(st)))
+        if self.shg:
+            return ({"shg": self._unpack_shg(lane_out, (nl,))}, {"shg": self._unpack_shg(total_out, ())}, st.as_dict())
         if self.geometry:
             return ({self.states[0]: self._unpack_geometry(lane_out, (nl,))},
                     {self.states[0]: self._unpack_geometry(total_out, ())}, st.as_dict())
@@ -230,6 +253,18 @@ class Engine:
         total = _unpack(total_out, (), self.num_states, nw)
         return ({s: lanes[..., i, :, :, :, :] for i, s in enumerate(self.states)},
                 {s: total[i] for i, s in enumerate(self.states)}, st.as_dict())
+
+    def _result_elems(self) -> int:
+        if self.shg:
+            return self._om.shape[0] * len(self.shg_components) * 14
+        return self.num_states * self._ncomp * self._axis
+
+    def _unpack_shg(self, flat: np.ndarray, lead: tuple) -> np.ndarray:
+        """Complex column-major [n_w, n_comp, 7, 2] -> [..., n_w, n_comp, 7, 2]."""
+        nc = len(self.shg_components)
+        z = flat.view(np.complex128).reshape(lead + (2, 7, nc, self._om.shape[0]))
+        L = len(lead)
+        return np.transpose(z, tuple(range(L)) + (L + 3, L + 2, L + 1, L)).copy()
 
     def _unpack_geometry(self, flat: np.ndarray, lead: tuple) -> np.ndarray:
         """Complex column-major [n_mu,3,3,3] (BCD) or [n_mu,3,3] (AHC) -> real [..., n_mu, a, b(, c)]."""
@@ -245,8 +280,10 @@ class Engine:
     def eval_points(self, kpoints: np.ndarray):
         k = np.ascontiguousarray(kpoints, dtype=np.float64).reshape(-1, 3)
         nw = self._axis
-        out = np.zeros(k.shape[0] * self.num_states * self._ncomp * nw * 2)
+        out = np.zeros(k.shape[0] * self._result_elems() * 2)
         _check(self.lib, self.lib.wnlqg_eval_points(self.handle, _ptr(k), k.shape[0], _ptr(out)))
+        if self.shg:
+            return {"shg": self._unpack_shg(out, (k.shape[0],))}
         if self.geometry:
             return {self.states[0]: self._unpack_geometry(out, (k.shape[0],))}
         vals = _unpack(out, (k.shape[0],), self.num_states, nw)
