@@ -212,6 +212,36 @@ int32_t wnlqg_spectrum_points(wnlqg_engine *engine, const double *kpoints, int32
 /* Block until all work of the engine's stream is complete. */
 int32_t wnlqg_synchronize(wnlqg_engine *engine);
 
+/*
+ * Learned refinement proposal.  No counterpart in the reference (it has no learned component); for
+ * callers that rank the cells of a k mesh for adaptive refinement from the integrand variance of
+ * cheaper plain runs.  The integrand itself is always computed exactly by the engine above.
+ *
+ * A 3 -> 64 -> 64 -> 1 multilayer perceptron with SiLU activations in FP32.  Parameter vector:
+ * W1[64][3] | b1[64] | W2[64][64] | b2[64] | W3[64] | b3 (row-major, WNLQG_MLP_PARAMS floats).
+ * Samples x are [n][3] floats, targets and outputs [n].  The loss is the mean squared error; the
+ * gradient is deterministic (fixed per-wave sample ranges summed in wave order).  Training is Adam
+ * (beta1 0.9, beta2 0.999, eps 1e-8) on minibatches of a per-epoch shuffle seeded by `seed`.
+ * Errors: the same WNLQG_E_* codes, message from wnlqg_mlp_last_error.
+ */
+#define WNLQG_MLP_INPUTS 3
+#define WNLQG_MLP_HIDDEN 64
+#define WNLQG_MLP_PARAMS 4481
+
+typedef struct wnlqg_mlp wnlqg_mlp;
+
+int32_t wnlqg_mlp_create(uint64_t seed, wnlqg_mlp **out);   /* Glorot-uniform weights, zero biases */
+int32_t wnlqg_mlp_destroy(wnlqg_mlp *mlp);
+int32_t wnlqg_mlp_last_error(char *buf, size_t len);
+int32_t wnlqg_mlp_get_params(wnlqg_mlp *mlp, float *params);
+int32_t wnlqg_mlp_set_params(wnlqg_mlp *mlp, const float *params);  /* also resets the Adam moments */
+int32_t wnlqg_mlp_predict(wnlqg_mlp *mlp, const float *x, int64_t n, float *y);
+/* gradient of the mean squared error over all n samples, and the loss */
+int32_t wnlqg_mlp_gradient(wnlqg_mlp *mlp, const float *x, const float *t, int64_t n, float *grad, float *loss);
+/* `steps` Adam steps of `batch` samples; loss_history[steps] (may be NULL) receives each batch loss */
+int32_t wnlqg_mlp_train(wnlqg_mlp *mlp, const float *x, const float *t, int64_t n, int32_t steps, int32_t batch,
+                        float lr, uint64_t seed, float *loss_history);
+
 #ifdef __cplusplus
 }
 #endif

@@ -36,7 +36,9 @@ ERRORS = {-1: "WNLQG_E_INVALID", -2: "WNLQG_E_NONFINITE", -3: "WNLQG_E_GAP", -4:
 
 EXPORTED = ["wnlqg_version", "wnlqg_last_error", "wnlqg_create", "wnlqg_destroy", "wnlqg_num_states",
             "wnlqg_num_channels", "wnlqg_run_mesh", "wnlqg_eval_points", "wnlqg_fourier_points",
-            "wnlqg_spectrum_points", "wnlqg_synchronize"]
+            "wnlqg_spectrum_points", "wnlqg_synchronize", "wnlqg_mlp_create", "wnlqg_mlp_destroy",
+            "wnlqg_mlp_last_error", "wnlqg_mlp_get_params", "wnlqg_mlp_set_params", "wnlqg_mlp_predict",
+            "wnlqg_mlp_gradient", "wnlqg_mlp_train"]
 
 
 class WnlqgError(RuntimeError):
@@ -101,10 +103,17 @@ def load_library(path: str = LIB_PATH) -> C.CDLL:
     lib.wnlqg_fourier_points.argtypes = [C.c_void_p, dp, C.c_int32, dp]
     lib.wnlqg_spectrum_points.argtypes = [C.c_void_p, dp, C.c_int32, dp, dp]
     lib.wnlqg_synchronize.argtypes = [C.c_void_p]
+    fp = C.POINTER(C.c_float)
+    lib.wnlqg_mlp_create.argtypes = [C.c_uint64, C.POINTER(C.c_void_p)]
+    lib.wnlqg_mlp_destroy.argtypes = [C.c_void_p]
+    lib.wnlqg_mlp_last_error.argtypes = [C.c_char_p, C.c_size_t]
+    lib.wnlqg_mlp_get_params.argtypes = [C.c_void_p, fp]
+    lib.wnlqg_mlp_set_params.argtypes = [C.c_void_p, fp]
+    lib.wnlqg_mlp_predict.argtypes = [C.c_void_p, fp, C.c_int64, fp]
+    lib.wnlqg_mlp_gradient.argtypes = [C.c_void_p, fp, fp, C.c_int64, fp, fp]
+    lib.wnlqg_mlp_train.argtypes = [C.c_void_p, fp, fp, C.c_int64, C.c_int32, C.c_int32, C.c_float, C.c_uint64, fp]
     for name in EXPORTED:
-        getattr(lib, name)
-        if name not in ("wnlqg_last_error",):
-            getattr(lib, name).restype = C.c_int32
+        getattr(lib, name).restype = C.c_int32
     _lib = lib
     return lib
 

@@ -216,17 +216,75 @@ def synthetic_model(num_orbitals: int, r_max: int, seed: int, *, lattice_constan
                              meta=dict(num_orbitals=n, r_max=r_max, seed=seed))
 
 
+def hotspot_model(seed: int = 3, *, num_orbitals: int = 16, r_max: int = 4, lattice_constant: float = 4.0,
+                  blocks: int = 4, velocity: float = 0.3, gap_range=(1.0e-3, 5.0e-3), coupling: float = 1.0e-3,
+                  background_scale: float = 0.25, background_offset: float = 3.0,
+                  name: Optional[str] = None) -> TightBindingModel:
+    """Model of the learned-proposal form: ``num_orbitals`` orbitals on the cube |R_i| <= r_max, of which
+    ``blocks`` pairs (orbitals 2s, 2s+1) form Weyl-like two-band blocks
+
+        H_s(k) = eps_s + velocity * sum_a sin(2 pi (k_a - k*_a,s)) sigma_a + (gap_s / 2) sigma_z
+
+    whose two bands come within gap_s (uniform in ``gap_range``) of each other at the seeded random
+    point k*_s (and at the seven points k*_s + half-integer shifts), with |eps_s| <= 2 meV.  The
+    remaining orbitals are a ``synthetic_model`` background (same seed); block <-> anything couplings
+    are the background's draws scaled by ``coupling``.  The background hoppings are scaled by
+    ``background_scale`` and its on-site energies moved by -+``background_offset`` (first half down,
+    second half up), so that no background band reaches the chemical potential 0 and the Fermi-surface
+    response comes from the hot spots.  Choices fixed here, so a seed builds one model."""
+    n = int(num_orbitals)
+    rng = np.random.default_rng(seed + 7919)
+    g = synthetic_model(n, r_max, seed, lattice_constant=lattice_constant)
+    hot = 2 * blocks
+    mask = np.full((n, n), coupling)
+    mask[hot:, hot:] = background_scale
+    for s in range(blocks):
+        mask[2 * s:2 * s + 2, 2 * s:2 * s + 2] = 0.0
+    hr = g.hamiltonian_r * mask[None]
+    index = {tuple(r): i for i, r in enumerate(g.r_vectors.tolist())}
+    nb = n - hot
+    for j in range(nb):
+        hr[index[(0, 0, 0)], hot + j, hot + j] += -background_offset if j < nb // 2 else background_offset
+    sig = [np.array([[0, 1], [1, 0]], dtype=complex), np.array([[0, -1j], [1j, 0]], dtype=complex),
+           np.array([[1, 0], [0, -1]], dtype=complex)]
+    spots = []
+    for s in range(blocks):
+        kstar = rng.uniform(0.0, 1.0, 3)
+        gap = rng.uniform(*gap_range)
+        eps = rng.uniform(-2.0e-3, 2.0e-3)
+        sl = slice(2 * s, 2 * s + 2)
+        hr[index[(0, 0, 0)], sl, sl] += eps * np.eye(2) + 0.5 * gap * sig[2]
+        for a in range(3):
+            e = [0, 0, 0]
+            e[a] = 1
+            plus = np.exp(-2j * np.pi * kstar[a]) / 2j * velocity * sig[a]
+            hr[index[tuple(e)], sl, sl] += plus
+            hr[index[tuple(-x for x in e)], sl, sl] += plus.conj().T
+        spots.append(dict(kstar=kstar, gap=gap, eps=eps))
+    return TightBindingModel(g.lattice, g.r_vectors, g.r_degeneracies, hr, None,
+                             name=name or f"hotspot_n{n}_r{r_max}_s{seed}",
+                             meta=dict(num_orbitals=n, r_max=r_max, seed=seed, spots=spots))
+
+
 def config_model(index: int, seed: Optional[int] = None) -> TightBindingModel:
     """Models of the benchmark configurations.
 
     0: n=8,  |R_i|<=3 (343 R), A = 0            (plain FP64 shift current, 32^3 mesh)
     1: n=32, |R_i|<=5 (1331 R), A amplitude 0.1 (shift + injection, 128^3 mesh)
+    2: model `seed` (0..63) of the batching form: n = 4 (seed mod 12 + 1), |R_i|<=4 (729 R), A = 0
+    3: n=16, |R_i|<=4, four Weyl-like near-degenerate pairs (hotspot_model; BCD learned proposal)
     4: n=96, |R_i|<=6 (2197 R), A amplitude 0.1 (SHG, 256^3 mesh)
     """
     if index == 0:
         return synthetic_model(8, 3, 0 if seed is None else seed, name="cfg0_n8_r3")
     if index == 1:
         return synthetic_model(32, 5, 1 if seed is None else seed, position_scale=0.1, name="cfg1_n32_r5")
+    if index == 2:
+        # batching form: model `seed` of 64, n_w = 4 (seed mod 12 + 1), |R_i| <= 4, built as config 0
+        sd = 0 if seed is None else seed
+        return synthetic_model(4 * (sd % 12 + 1), 4, sd, name=f"cfg2_model{sd}")
+    if index == 3:
+        return hotspot_model(3 if seed is None else seed, name="cfg3_n16_r4_hotspots")
     if index == 4:
         return synthetic_model(96, 6, 4 if seed is None else seed, position_scale=0.1, name="cfg4_n96_r6")
     raise ValueError(f"no model for configuration {index}")
