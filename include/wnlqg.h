@@ -213,6 +213,25 @@ int32_t wnlqg_spectrum_points(wnlqg_engine *engine, const double *kpoints, int32
 int32_t wnlqg_synchronize(wnlqg_engine *engine);
 
 /*
+ * Batching form: `count` independent (model, task, mesh) integrals of uneven n_w run by one
+ * hipGraph launch (the reference runs such bundles one task at a time through the same
+ * execute_prepared_tasks! seam, IntegralDriver.jl:1079 / BerryMultipoleDriver.jl / SpectralResponseDriver.jl).
+ * wnlqg_batch_create builds one engine per entry (tasks with batch_size 0 share a working-set
+ * budget), plans every mesh, optionally autotunes the eigensolver's threads per k point for each n_w
+ * (autotune_tiles != 0), and captures all kernel sequences into one graph whose entries run on four
+ * concurrent streams.  meshes: [3*count]; n_lanes: [count].
+ * wnlqg_batch_run replays the graph and writes, for entry i, exactly what wnlqg_run_mesh writes into
+ * lane_out[i] / total_out[i] (total_out or any total_out[i] may be NULL); elapsed_ms = device time of the graph.
+ */
+typedef struct wnlqg_batch wnlqg_batch;
+int32_t wnlqg_batch_create(int32_t count, const wnlqg_model *models, const wnlqg_task *tasks, const int32_t *meshes,
+                           const int32_t *n_lanes, int32_t autotune_tiles, wnlqg_batch **out);
+int32_t wnlqg_batch_run(wnlqg_batch *batch, double *const *lane_out, double *const *total_out, double *elapsed_ms);
+/* the eigensolver threads per k point chosen for entry `index` (0 = size-class default) */
+int32_t wnlqg_batch_tuning(const wnlqg_batch *batch, int32_t index, int32_t *eig_threads);
+int32_t wnlqg_batch_destroy(wnlqg_batch *batch);
+
+/*
  * Learned refinement proposal.  No counterpart in the reference (it has no learned component); for
  * callers that rank the cells of a k mesh for adaptive refinement from the integrand variance of
  * cheaper plain runs.  The integrand itself is always computed exactly by the engine above.

@@ -92,10 +92,15 @@ def test_rejects_invalid_geometry_tasks(kw, code):
 
 
 def test_rejects_mixed_driver_families():
-    # BCD and AHC belong to different reference drivers: refused before any device work
+    # SC with BCD: two reference drivers, refused before any device work
+    with pytest.raises(E.WnlqgError) as ei:
+        E.Engine(synthetic_model(4, 1, 0), E.EngineTask(photon_energies=np.linspace(0, 1, 5), shift_current=True,
+                                                        berry_curvature_dipole=True, fermi_energies=[0.0]))
+    assert ei.value.code == -6
+    # BCD with AHC (one engine sharing the front end): AHC still requires T = 0
     with pytest.raises(E.WnlqgError) as ei:
         E.Engine(synthetic_model(4, 1, 0), _geom(intrinsic_hall=True))
-    assert ei.value.code == -6
+    assert ei.value.code == -1
 
 
 @pytest.mark.parametrize("kw,code", [

@@ -137,3 +137,22 @@ def test_bcd_gap_error():
         with pytest.raises(E.WnlqgError) as ei:
             eng.run_mesh((3, 3, 3))
     assert ei.value.code == -3
+
+
+def test_bcd_and_ahc_in_one_engine_match_separate_engines():
+    m = synthetic_model(10, 1, 4, position_scale=0.1)
+    mus = [-0.4, 0.2]
+    common = dict(fermi_energies=mus, temperature=0.0, fermi_surface_broadening=0.1, denominator_regularization=0.0,
+                  degeneracy_threshold=1e-9)
+    k = np.random.default_rng(8).uniform(0, 1, (6, 3))
+    say("[bcd+ahc] one engine")
+    with E.Engine(m, E.EngineTask(berry_curvature_dipole=True, intrinsic_hall=True, **common)) as both, \
+            E.Engine(m, E.EngineTask(berry_curvature_dipole=True, **common)) as b, \
+            E.Engine(m, E.EngineTask(intrinsic_hall=True, **common)) as a:
+        pb = both.eval_points(k)
+        lb, tb, _ = both.run_mesh((4, 3, 5))
+        assert np.array_equal(pb["bcd"], b.eval_points(k)["bcd"])
+        assert np.array_equal(pb["ahc"], a.eval_points(k)["ahc"])
+        _, t1, _ = b.run_mesh((4, 3, 5))
+        _, t2, _ = a.run_mesh((4, 3, 5))
+    assert np.array_equal(tb["bcd"], t1["bcd"]) and np.array_equal(tb["ahc"], t2["ahc"])

@@ -73,10 +73,30 @@ struct DevBuf {
     }
 };
 
+// One batch of a mesh: k range [k0, k0 + cnt); separable: k3 plane and k2 range [k2b, k2b + k2c).
+struct MeshBatch {
+    long long k0;
+    int cnt, k3, k2b, k2c;
+};
+
+// Device tables of one mesh integral, kept between calls with the same mesh and lane count.
+struct MeshPlan {
+    bool valid = false;
+    int32_t mesh[3] = {0, 0, 0};
+    int n_lanes = 0;
+    long long nk = 0;
+    bool sep = false;
+    std::vector<MeshBatch> batches;
+    std::vector<long long> bchunk, blane;
+    DevBuf<int4> dchunks, dlanes;
+};
+
 }  // namespace
 
 struct wnlqg_engine {
-    int family = 0;     // 0 SC / IC, 1 BCD, 2 intrinsic AHC, 3 SHG
+    int family = 0;     // 0 SC / IC, 1 BCD and/or intrinsic AHC (gp.kind), 3 SHG
+    std::vector<double> elem_pref;  // per output element: the state's per-point prefactor
+    std::vector<double> elem_norm;  // per output element: divisor of the total besides N_k (0: none)
     WnGeom gp{};        // BCD / AHC parameters
     WnShg sp{};         // SHG parameters
     DevBuf<double2> der, coef, one;  // SHG optical position derivatives, pair coefficients, one-band sums
@@ -112,23 +132,27 @@ struct wnlqg_engine {
     DevBuf<double2> y3, y23, e1, e2, e3;
     int h_tiles = 0;            // 128-column FP64 tiles holding channel H (mixed forms)
     int partial_cap = 0;  // chunks the partial buffer holds
+    int eig_tpb = 0;      // Jacobi threads per k point (0: default; set by the batch autotuner)
+    MeshPlan plan;        // batches and chunk / lane tables of the last mesh
     std::vector<double> omega_host;
     hipEvent_t ev[8] = {};
 };
 
 namespace {
 
-// driver family of a quantities bitmask: 0 SC / IC, 1 BCD, 2 AHC, -1 invalid
+// driver family of a quantities bitmask: 0 SC / IC, 1 BCD and/or AHC, 3 SHG, -1 invalid
 int task_family(int q) {
-    if (q == WNLQG_BERRY_CURVATURE_DIPOLE) return 1;
-    if (q == WNLQG_INTRINSIC_HALL) return 2;
+    const int geom = WNLQG_BERRY_CURVATURE_DIPOLE | WNLQG_INTRINSIC_HALL;
+    if (q != 0 && (q & ~geom) == 0) return 1;
     if (q == WNLQG_SECOND_HARMONIC) return 3;
     if (q != 0 && (q & ~(WNLQG_SHIFT_CURRENT | WNLQG_INJECTION_CURRENT)) == 0) return 0;
     return -1;
 }
 
 // BCD / AHC task checks (BerryMultipoleDC.jl:16-55, IntrinsicHallOccupations.jl:16-21).
-int check_geometry_task(const wnlqg_task *t, int family) {
+int check_geometry_task(const wnlqg_task *t) {
+    const bool bcd = (t->quantities & WNLQG_BERRY_CURVATURE_DIPOLE) != 0;
+    const bool ahc = (t->quantities & WNLQG_INTRINSIC_HALL) != 0;
     if (t->num_fermi_energies < 1 || t->num_fermi_energies > WN_GEOM_MAXMU || !t->fermi_energies)
         return fail(WNLQG_E_INVALID, "1 to %d Fermi energies are required", WN_GEOM_MAXMU);
     for (int i = 0; i < t->num_fermi_energies; ++i)
@@ -142,7 +166,7 @@ int check_geometry_task(const wnlqg_task *t, int family) {
     if (t->batch_size < 0) return fail(WNLQG_E_INVALID, "batch_size must be >= 0");
     if (t->fourier_backend < WNLQG_FOURIER_AUTO || t->fourier_backend > WNLQG_FOURIER_PLANES)
         return fail(WNLQG_E_INVALID, "unknown fourier_backend %d", t->fourier_backend);
-    if (family == 1) {
+    if (bcd) {
         if (!(t->degeneracy_threshold >= 0.0) || !std::isfinite(t->degeneracy_threshold))
             return fail(WNLQG_E_INVALID, "invalid spectral isolation threshold");
         const double w = t->fermi_surface_broadening;
@@ -150,9 +174,8 @@ int check_geometry_task(const wnlqg_task *t, int family) {
             return fail(WNLQG_E_INVALID, "zero-temperature Fermi surface requires explicit positive eta_fs_ev");
         if (t->temperature > 0.0 && w != 0.0)
             return fail(WNLQG_E_INVALID, "eta_fs_ev is applicable only to zero-temperature Fermi surface");
-    } else if (t->temperature != 0.0) {
-        return fail(WNLQG_E_INVALID, "INTRINSIC_HALL_REQUIRES_ZERO_TEMPERATURE");
     }
+    if (ahc && t->temperature != 0.0) return fail(WNLQG_E_INVALID, "INTRINSIC_HALL_REQUIRES_ZERO_TEMPERATURE");
     return WNLQG_OK;
 }
 
@@ -161,9 +184,9 @@ int check_task(const wnlqg_task *t, int n) {
     const int family = task_family(t->quantities);
     if (family < 0)
         return fail(WNLQG_E_UNSUPPORTED,
-                    "quantities bitmask %d: shift (1) and/or injection (2) current, or BCD (4), AHC (8) or SHG (16) alone",
+                    "quantities bitmask %d: shift (1) and/or injection (2) current, or BCD (4) and/or AHC (8), or SHG (16)",
                     t->quantities);
-    if (family == 1 || family == 2) return check_geometry_task(t, family);
+    if (family == 1) return check_geometry_task(t);
     if (family == 3) {
         if (t->num_shg_components < 0 || t->num_shg_components > 18 || (t->num_shg_components > 0 && !t->shg_components))
             return fail(WNLQG_E_INVALID, "0 to 18 SHG components");
@@ -207,8 +230,7 @@ int check_task(const wnlqg_task *t, int n) {
 
 // complex values per k point (or chunk, or lane) of all output states
 int output_elems(const wnlqg_engine *g) {
-    if (g->family == 1) return 27 * g->gp.nmu;
-    if (g->family == 2) return 9 * g->gp.nmu;
+    if (g->family == 1) return ((g->gp.kind & 1) ? 27 : 0) * g->gp.nmu + ((g->gp.kind & 2) ? 9 : 0) * g->gp.nmu;
     if (g->family == 3) return 19 * g->sp.ncomp * g->sp.nw + g->sp.ncomp;
     return 27 * g->sc.nq * g->sc.nw;
 }
@@ -286,7 +308,7 @@ hipError_t launch_interp(wnlqg_engine *g, int rows) {
 int run_front(wnlqg_engine *g, int count, int rows, bool eig) {
     WN_HIP(launch_phases(g, rows));
     WN_HIP(launch_interp(g, rows));
-    if (eig) WN_HIP(wn_launch_eig_f64(g->x.p, g->e.p, g->u.p, g->sweeps.p, count, g->sh, g->stream));
+    if (eig) WN_HIP(wn_launch_eig_f64(g->x.p, g->e.p, g->u.p, g->sweeps.p, count, g->sh, g->eig_tpb, g->stream));
     return WNLQG_OK;
 }
 
@@ -319,7 +341,7 @@ int32_t wnlqg_last_error(char *buf, size_t len) {
     return (int32_t)g_last_error.size();
 }
 
-int32_t wnlqg_num_states(const wnlqg_engine *g) { return g ? (g->family ? 1 : g->sc.nq) : 0; }
+int32_t wnlqg_num_states(const wnlqg_engine *g) { return g ? g->sc.nq : 0; }
 
 namespace {
 
@@ -394,7 +416,8 @@ int32_t wnlqg_destroy(wnlqg_engine *g) {
     g->lanebuf.release(); g->lanecomp.release(); g->tab32.release(); g->bh.release(); g->bm.release();
     g->bl.release(); g->fh.release(); g->fm.release(); g->cup.release(); g->cdown.release(); g->t12.release();
     g->t3.release(); g->ridx2.release(); g->box.release(); g->y3.release(); g->y23.release(); g->e1.release();
-    g->e2.release(); g->e3.release();
+    g->e2.release(); g->e3.release(); g->plan.dchunks.release(); g->plan.dlanes.release();
+    g->der.release(); g->coef.release(); g->one.release(); g->pairs.release(); g->pdiff.release();
     if (g->stream) (void)hipStreamDestroy(g->stream);
     delete g;
     return WNLQG_OK;
@@ -438,8 +461,9 @@ int32_t wnlqg_create(const wnlqg_model *model, const wnlqg_task *task, wnlqg_eng
                      L[6] * (L[1] * L[5] - L[4] * L[2]);
     g->precision = task->precision;
     g->family = task_family(task->quantities);
-    if (g->family > 0) {
-        g->gp.kind = g->family;
+    if (g->family == 1) {
+        g->gp.kind = ((task->quantities & WNLQG_BERRY_CURVATURE_DIPOLE) ? 1 : 0) |
+                     ((task->quantities & WNLQG_INTRINSIC_HALL) ? 2 : 0);
         g->gp.nmu = task->num_fermi_energies;
         g->gp.temperature = task->temperature;
         g->gp.eta_fs = task->fermi_surface_broadening;
@@ -466,7 +490,7 @@ int32_t wnlqg_create(const wnlqg_model *model, const wnlqg_task *task, wnlqg_eng
         sp.temperature = task->temperature;
         sp.tol = task->degeneracy_threshold;
     }
-    g->sc.nw = (g->family == 1 || g->family == 2) ? task->num_fermi_energies : task->num_photon_energies;
+    g->sc.nw = g->family == 1 ? task->num_fermi_energies : task->num_photon_energies;
     g->sc.gaussian = task->broadening_type == WNLQG_GAUSSIAN;
     g->sc.width = task->broadening;
     g->sc.mu = task->fermi_energy;
@@ -475,15 +499,34 @@ int32_t wnlqg_create(const wnlqg_model *model, const wnlqg_task *task, wnlqg_eng
     g->sc.window = task->transition_window_factor * task->broadening;
     g->sc.sc = (task->quantities & WNLQG_SHIFT_CURRENT) != 0;
     g->sc.ic = (task->quantities & WNLQG_INJECTION_CURRENT) != 0;
-    g->sc.nq = g->family ? 1 : g->sc.sc + g->sc.ic;
+    g->sc.nq = g->family == 1 ? ((g->gp.kind & 1) + ((g->gp.kind >> 1) & 1)) : (g->family == 3 ? 1 : g->sc.sc + g->sc.ic);
     int qi = 0;
     if (g->sc.sc) g->pref[qi++] = sc_prefactor(g->cell_volume);
     if (g->sc.ic) g->pref[qi++] = ic_prefactor(g->cell_volume);
-    // per-point AHC scale -e^2/hbar * 1e-20 (SpectralResponseDriver.jl:388-400); BCD moments are raw
-    if (g->family == 1) g->pref[0] = 1.0;
-    if (g->family == 2) g->pref[0] = -kCharge * kCharge / kHbarJs * 1e-20;
-    // the frequency axis of SC / IC, or the Fermi-energy axis of BCD / AHC
-    const double *axis = (g->family == 1 || g->family == 2) ? task->fermi_energies : task->photon_energies;
+    // per-element prefactors and total divisors: SC / IC states [27 n_w] each (prefactor, 1/N_k on the lanes);
+    // BCD [27 n_mu] raw moments, total / (N_k |det L|); AHC [9 n_mu] with the per-point scale
+    // -e^2/hbar 1e-20 (SpectralResponseDriver.jl:388-400), total / (N_k |det L| 1e-30)
+    if (g->family == 0) {
+        for (int q = 0; q < g->sc.nq; ++q)
+            for (int e = 0; e < 27 * g->sc.nw; ++e) {
+                g->elem_pref.push_back(g->pref[q]);
+                g->elem_norm.push_back(0.0);
+            }
+    } else if (g->family == 1) {
+        const double vol = std::fabs(g->cell_volume);
+        if (g->gp.kind & 1)
+            for (int e = 0; e < 27 * g->gp.nmu; ++e) {
+                g->elem_pref.push_back(1.0);
+                g->elem_norm.push_back(vol);
+            }
+        if (g->gp.kind & 2)
+            for (int e = 0; e < 9 * g->gp.nmu; ++e) {
+                g->elem_pref.push_back(-kCharge * kCharge / kHbarJs * 1e-20);
+                g->elem_norm.push_back(vol * 1.0e-30);
+            }
+    }
+    // the frequency axis of SC / IC / SHG, or the Fermi-energy axis of BCD / AHC
+    const double *axis = g->family == 1 ? task->fermi_energies : task->photon_energies;
     g->omega_host.assign(axis, axis + g->sc.nw);
 
     // shape
@@ -497,13 +540,13 @@ int32_t wnlqg_create(const wnlqg_model *model, const wnlqg_task *task, wnlqg_eng
             const bool pos = l >= WN_CH_A;
             const bool second = l >= WN_CH_DDH && l < WN_CH_A;
             if (pos && !g->sh.has_pos) continue;
-            if (second && (g->family == 1 || g->family == 2)) continue;
+            if (second && g->family == 1) continue;
             g->sh.lch[c++] = (signed char)l;
         }
         g->sh.C = c;
         g->gp.c_dh = 1;
-        g->gp.c_a = g->sh.has_pos ? (g->family ? 4 : WN_CH_A) : -1;
-        g->gp.c_da = g->sh.has_pos ? (g->family ? 7 : WN_CH_DA) : -1;
+        g->gp.c_a = g->sh.has_pos ? (g->family == 1 ? 4 : WN_CH_A) : -1;
+        g->gp.c_da = g->sh.has_pos ? (g->family == 1 ? 7 : WN_CH_DA) : -1;
     }
     g->sh.Kb = (nr + WN_RB - 1) / WN_RB;
     // columns padded to 256 so that both interpolation kernels tile the same output rows
@@ -690,11 +733,7 @@ int check_flags(wnlqg_engine *g) {
 }
 
 // clear the status words: [0] pair-gap flag, [1] max sweeps, [2] geometry flags, [3] first failing k
-hipError_t clear_flags(wnlqg_engine *g) {
-    hipError_t e = hipMemsetAsync(g->flags.p, 0, 3 * sizeof(int), g->stream);
-    if (e == hipSuccess) e = hipMemsetAsync(g->flags.p + 3, 0x7f, sizeof(int), g->stream);
-    return e;
-}
+hipError_t clear_flags(wnlqg_engine *g) { return wn_launch_reset_flags(g->flags.p, g->stream); }
 
 // per-chunk stage after the rotation: SC / IC pairs + frequency contraction, or BCD / AHC
 hipError_t launch_response(wnlqg_engine *g, const int4 *chunks, int nch, int cnt, long long kbase) {
@@ -715,7 +754,6 @@ int32_t wnlqg_eval_points(wnlqg_engine *g, const double *kpoints, int32_t m, dou
     if (!g || !kpoints || !out || m < 0) return fail(WNLQG_E_INVALID, "invalid arguments");
     WN_HIP(hipSetDevice(g->device));
     const int nelem = output_elems(g);
-    const int per_state = nelem / g->sc.nq;
     WN_HIP(clear_flags(g));
     std::vector<double> part;
     for (int p0 = 0; p0 < m; p0 += g->batch) {
@@ -749,13 +787,11 @@ int32_t wnlqg_eval_points(wnlqg_engine *g, const double *kpoints, int32_t m, dou
             continue;
         }
         for (int i = 0; i < cnt; ++i)
-            for (int qv = 0; qv < g->sc.nq; ++qv)
-                for (int e = 0; e < per_state; ++e) {
-                    size_t src = ((size_t)i * nelem + (size_t)qv * per_state + e) * 2;
-                    size_t dst = (((size_t)(p0 + i) * g->sc.nq + qv) * per_state + e) * 2;
-                    out[dst] = g->pref[qv] * part[src];
-                    out[dst + 1] = g->pref[qv] * part[src + 1];
-                }
+            for (int e = 0; e < nelem; ++e) {
+                const size_t src = ((size_t)i * nelem + e) * 2, dst = ((size_t)(p0 + i) * nelem + e) * 2;
+                out[dst] = g->elem_pref[e] * part[src];
+                out[dst + 1] = g->elem_pref[e] * part[src + 1];
+            }
     }
     int rc = check_flags(g);
     if (rc) return rc;
@@ -764,32 +800,31 @@ int32_t wnlqg_eval_points(wnlqg_engine *g, const double *kpoints, int32_t m, dou
     return WNLQG_OK;
 }
 
-int32_t wnlqg_run_mesh(wnlqg_engine *g, const int32_t mesh[3], int32_t n_lanes, double *lane_out, double *total_out,
-                       wnlqg_stats *stats) {
-    if (!g || !mesh || !lane_out) return fail(WNLQG_E_INVALID, "invalid arguments");
-    if (mesh[0] < 1 || mesh[1] < 1 || mesh[2] < 1) return fail(WNLQG_E_INVALID, "mesh sizes must be positive");
-    const long long nk = (long long)mesh[0] * mesh[1] * mesh[2];
-    if (n_lanes < 1 || n_lanes > nk) return fail(WNLQG_E_INVALID, "n_lanes must be in [1, N_k]");
-    WN_HIP(hipSetDevice(g->device));
-    const int nelem = output_elems(g);
+}  // extern "C"
 
-    // chunk plan: lanes are contiguous k ranges (BundleAssembly.jl:41-47); each batch splits the
-    // part of every lane it covers into chunks of at most kpc points, in k order.
+namespace {
+
+// Batches, lane-aligned chunks and the device tables of a mesh (host work and allocations; not
+// captured).  Lanes are contiguous k ranges (BundleAssembly.jl:41-47); each batch splits the part of
+// every lane it covers into chunks of at most kpc points, in k order.
+int plan_mesh(wnlqg_engine *g, const int32_t mesh[3], int32_t n_lanes) {
+    MeshPlan &p = g->plan;
+    if (p.valid && p.mesh[0] == mesh[0] && p.mesh[1] == mesh[1] && p.mesh[2] == mesh[2] && p.n_lanes == n_lanes)
+        return WNLQG_OK;
+    p.valid = false;
+    const long long nk = (long long)mesh[0] * mesh[1] * mesh[2];
+    const int nelem = output_elems(g);
     std::vector<long long> lstart(n_lanes + 1);
     for (int l = 0; l <= n_lanes; ++l) lstart[l] = (long long)((__int128)l * nk / n_lanes);
     // batches: contiguous k ranges.  Separable: k2 ranges of one k3 plane (rows k2*N1 + k1).
-    struct Batch {
-        long long k0;
-        int cnt, k3, k2b, k2c;
-    };
-    std::vector<Batch> batches;
-    const bool sep = g->separable && mesh[0] <= g->batch;
-    if (sep) {
+    p.batches.clear();
+    p.sep = g->separable && mesh[0] <= g->batch;
+    if (p.sep) {
         const int k2step = std::max(1, std::min(mesh[1], g->batch / mesh[0]));
         for (int k3 = 0; k3 < mesh[2]; ++k3)
             for (int k2b = 0; k2b < mesh[1]; k2b += k2step) {
                 const int k2c = std::min(k2step, mesh[1] - k2b);
-                batches.push_back({((long long)k3 * mesh[1] + k2b) * mesh[0], k2c * mesh[0], k3, k2b, k2c});
+                p.batches.push_back({((long long)k3 * mesh[1] + k2b) * mesh[0], k2c * mesh[0], k3, k2b, k2c});
             }
         const size_t m2 = g->sh.ldo / 2;
         if (g->y23.n < (size_t)g->span[0] * k2step * m2) WN_HIP(g->y23.alloc((size_t)g->span[0] * k2step * m2));
@@ -801,144 +836,131 @@ int32_t wnlqg_run_mesh(wnlqg_engine *g, const int32_t mesh[3], int32_t n_lanes, 
         WN_HIP(wn_launch_axis_phases(g->e3.p, mesh[2], g->span[2], g->rmin[2], g->stream));
     } else {
         for (long long k0 = 0; k0 < nk; k0 += g->batch)
-            batches.push_back({k0, (int)std::min<long long>(g->batch, nk - k0), 0, 0, 0});
+            p.batches.push_back({k0, (int)std::min<long long>(g->batch, nk - k0), 0, 0, 0});
     }
-    const long long nbatch = (long long)batches.size();
+    const long long nbatch = (long long)p.batches.size();
     std::vector<int4> all_chunks, all_lanes;
-    std::vector<long long> bchunk(nbatch + 1), blane(nbatch + 1);
+    p.bchunk.assign(nbatch + 1, 0);
+    p.blane.assign(nbatch + 1, 0);
     int max_chunks = 0;
     for (long long bi = 0; bi < nbatch; ++bi) {
-        const long long k0 = batches[bi].k0, k1 = k0 + batches[bi].cnt;
-        bchunk[bi] = all_chunks.size();
-        blane[bi] = all_lanes.size();
+        const long long k0 = p.batches[bi].k0, k1 = k0 + p.batches[bi].cnt;
+        p.bchunk[bi] = all_chunks.size();
+        p.blane[bi] = all_lanes.size();
         int l = (int)(std::upper_bound(lstart.begin(), lstart.end(), k0) - lstart.begin()) - 1;
         for (; l < n_lanes && lstart[l] < k1; ++l) {
             long long a = std::max(k0, lstart[l]), b = std::min(k1, lstart[l + 1]);
             if (a >= b) continue;
-            int cb = (int)(all_chunks.size() - bchunk[bi]);
+            int cb = (int)(all_chunks.size() - p.bchunk[bi]);
             for (long long c = a; c < b; c += g->kpc)
                 all_chunks.push_back(make_int4((int)(c - k0), (int)std::min<long long>(g->kpc, b - c), l, 0));
-            int ce = (int)(all_chunks.size() - bchunk[bi]);
+            int ce = (int)(all_chunks.size() - p.bchunk[bi]);
             all_lanes.push_back(make_int4(l, cb, ce, 0));
         }
-        max_chunks = std::max(max_chunks, (int)(all_chunks.size() - bchunk[bi]));
+        max_chunks = std::max(max_chunks, (int)(all_chunks.size() - p.bchunk[bi]));
     }
-    bchunk[nbatch] = all_chunks.size();
-    blane[nbatch] = all_lanes.size();
+    p.bchunk[nbatch] = all_chunks.size();
+    p.blane[nbatch] = all_lanes.size();
     int rc = ensure_partial(g, max_chunks);
     if (rc) return rc;
-    DevBuf<int4> dchunks, dlanes;
-    WN_HIP(dchunks.alloc(all_chunks.size()));
-    WN_HIP(dlanes.alloc(all_lanes.size()));
-    WN_HIP(hipMemcpy(dchunks.p, all_chunks.data(), all_chunks.size() * sizeof(int4), hipMemcpyHostToDevice));
-    WN_HIP(hipMemcpy(dlanes.p, all_lanes.data(), all_lanes.size() * sizeof(int4), hipMemcpyHostToDevice));
+    WN_HIP(p.dchunks.alloc(all_chunks.size()));
+    WN_HIP(p.dlanes.alloc(all_lanes.size()));
+    WN_HIP(hipStreamSynchronize(g->stream));
+    WN_HIP(hipMemcpy(p.dchunks.p, all_chunks.data(), all_chunks.size() * sizeof(int4), hipMemcpyHostToDevice));
+    WN_HIP(hipMemcpy(p.dlanes.p, all_lanes.data(), all_lanes.size() * sizeof(int4), hipMemcpyHostToDevice));
     if (g->lanebuf.n < (size_t)n_lanes * nelem) {
         WN_HIP(g->lanebuf.alloc((size_t)n_lanes * nelem));
         WN_HIP(g->lanecomp.alloc((size_t)n_lanes * nelem));
     }
-    WN_HIP(hipMemsetAsync(g->lanebuf.p, 0, (size_t)n_lanes * nelem * sizeof(double2), g->stream));
-    WN_HIP(hipMemsetAsync(g->lanecomp.p, 0, (size_t)n_lanes * nelem * sizeof(double2), g->stream));
+    for (int a = 0; a < 3; ++a) p.mesh[a] = mesh[a];
+    p.n_lanes = n_lanes;
+    p.nk = nk;
+    p.valid = true;
+    return WNLQG_OK;
+}
+
+// The kernel sequence of one planned mesh integral on g->stream.  Capture-safe: no allocation, no
+// host synchronization, no memset.  marks (optional) holds 7 pre-created events per batch.
+int enqueue_mesh(wnlqg_engine *g, std::vector<std::array<hipEvent_t, 7>> *marks, double *interp_bytes) {
+    const MeshPlan &p = g->plan;
+    const int nelem = output_elems(g);
+    const long long nbatch = (long long)p.batches.size();
+    WN_HIP(wn_launch_zero((double *)g->lanebuf.p, (long long)p.n_lanes * nelem * 2, g->stream));
+    if (g->nterms) WN_HIP(wn_launch_zero((double *)g->lanecomp.p, (long long)p.n_lanes * nelem * 2, g->stream));
     WN_HIP(clear_flags(g));
     double2 *comp = g->nterms ? g->lanecomp.p : nullptr;
-
-    std::vector<hipEvent_t> evs;
-    auto mark = [&]() -> hipEvent_t {
-        hipEvent_t e;
-        if (hipEventCreate(&e) != hipSuccess) return nullptr;
-        hipEventRecord(e, g->stream);
-        evs.push_back(e);
-        return e;
-    };
-    std::vector<std::array<hipEvent_t, 7>> marks(nbatch);
     const long long m2 = g->sh.ldo / 2;
-    double interp_bytes = 0.0;
+    auto mark = [&](long long bi, int i) {
+        if (marks) hipEventRecord((*marks)[bi][i], g->stream);
+    };
     for (long long bi = 0; bi < nbatch; ++bi) {
-        const Batch &bt = batches[bi];
+        const MeshBatch &bt = p.batches[bi];
         const long long k0 = bt.k0;
         const int cnt = bt.cnt;
         const int rows = (cnt + WN_GM - 1) / WN_GM * WN_GM;
-        auto &mk = marks[bi];
-        mk[0] = mark();
-        if (!sep) {
-            WN_HIP(wn_launch_mesh_kpoints(g->kp.p, mesh, k0, cnt, rows, g->stream));
+        mark(bi, 0);
+        if (!p.sep) {
+            WN_HIP(wn_launch_mesh_kpoints(g->kp.p, p.mesh, k0, cnt, rows, g->stream));
             WN_HIP(launch_phases(g, rows));
         }
-        mk[1] = mark();
-        if (sep) {
+        mark(bi, 1);
+        if (p.sep) {
             const double2 *boxc = (const double2 *)g->box.p;
             if (bt.k2b == 0)
                 WN_HIP(wn_launch_contract(boxc, g->y3.p, g->e3.p, 1, bt.k3, g->span[2], (int)m2,
                                           g->span[0] * g->span[1], g->span[2] * m2, m2, m2, 0, g->stream));
             WN_HIP(wn_launch_contract(g->y3.p, g->y23.p, g->e2.p, bt.k2c, bt.k2b, g->span[1], (int)m2, g->span[0],
                                       g->span[1] * m2, m2, (long long)bt.k2c * m2, m2, g->stream));
-            WN_HIP(wn_launch_contract(g->y23.p, (double2 *)g->x.p, g->e1.p, mesh[0], 0, g->span[0], (int)m2, bt.k2c, m2,
-                                      (long long)bt.k2c * m2, (long long)mesh[0] * m2, m2, g->stream));
-            interp_bytes += (double)cnt * g->sh.ldo * 8.0;
+            WN_HIP(wn_launch_contract(g->y23.p, (double2 *)g->x.p, g->e1.p, p.mesh[0], 0, g->span[0], (int)m2, bt.k2c,
+                                      m2, (long long)bt.k2c * m2, (long long)p.mesh[0] * m2, m2, g->stream));
+            if (interp_bytes) *interp_bytes += (double)cnt * g->sh.ldo * 8.0;
         } else {
             WN_HIP(launch_interp(g, rows));
         }
-        mk[2] = mark();
-        WN_HIP(wn_launch_eig_f64(g->x.p, g->e.p, g->u.p, g->flags.p + 1, cnt, g->sh, g->stream));
-        mk[3] = mark();
+        mark(bi, 2);
+        WN_HIP(wn_launch_eig_f64(g->x.p, g->e.p, g->u.p, g->flags.p + 1, cnt, g->sh, g->eig_tpb, g->stream));
+        mark(bi, 3);
         WN_HIP(wn_launch_rotate_f64(g->x.p, g->u.p, cnt, g->sh, g->stream));
-        mk[4] = mark();
-        const int nch = (int)(bchunk[bi + 1] - bchunk[bi]);
-        const int nln = (int)(blane[bi + 1] - blane[bi]);
+        mark(bi, 4);
+        const int nch = (int)(p.bchunk[bi + 1] - p.bchunk[bi]);
+        const int nln = (int)(p.blane[bi + 1] - p.blane[bi]);
+        const int4 *chunks = p.dchunks.p + p.bchunk[bi];
         if (g->family == 0) {
             WN_HIP(wn_launch_pairs_f64(g->x.p, g->e.p, g->omega.p, g->count.p, g->pmeta.p, g->pmd.p, g->q.p,
                                        g->flags.p, cnt, g->sh, g->sc, g->stream));
-            mk[5] = mark();
-            WN_HIP(wn_launch_accum(dchunks.p + bchunk[bi], nch, g->count.p, g->pmeta.p, g->pmd.p, g->q.p, g->omega.p,
-                                   g->partial.p, g->sh, g->sc, g->stream));
+            mark(bi, 5);
+            WN_HIP(wn_launch_accum(chunks, nch, g->count.p, g->pmeta.p, g->pmd.p, g->q.p, g->omega.p, g->partial.p,
+                                   g->sh, g->sc, g->stream));
         } else if (g->family == 3) {
-            WN_HIP(wn_launch_shg(g->x.p, g->e.p, g->omega.p, dchunks.p + bchunk[bi], 0, cnt, g->der.p, g->count.p,
-                                 g->pairs.p, g->pdiff.p, g->coef.p, g->one.p, g->partial.p, g->sh, g->sp, g->stream));
-            mk[5] = mark();
-            WN_HIP(wn_launch_shg(g->x.p, g->e.p, g->omega.p, dchunks.p + bchunk[bi], nch, 0, g->der.p, g->count.p,
-                                 g->pairs.p, g->pdiff.p, g->coef.p, g->one.p, g->partial.p, g->sh, g->sp, g->stream));
+            WN_HIP(wn_launch_shg(g->x.p, g->e.p, g->omega.p, chunks, 0, cnt, g->der.p, g->count.p, g->pairs.p,
+                                 g->pdiff.p, g->coef.p, g->one.p, g->partial.p, g->sh, g->sp, g->stream));
+            mark(bi, 5);
+            WN_HIP(wn_launch_shg(g->x.p, g->e.p, g->omega.p, chunks, nch, 0, g->der.p, g->count.p, g->pairs.p,
+                                 g->pdiff.p, g->coef.p, g->one.p, g->partial.p, g->sh, g->sp, g->stream));
         } else {
-            WN_HIP(wn_launch_geom_f64(g->x.p, g->e.p, g->omega.p, dchunks.p + bchunk[bi], nch, g->partial.p,
-                                      g->flags.p, k0, g->sh, g->gp, g->stream));
-            mk[5] = mark();
+            WN_HIP(wn_launch_geom_f64(g->x.p, g->e.p, g->omega.p, chunks, nch, g->partial.p, g->flags.p, k0, g->sh,
+                                      g->gp, g->stream));
+            mark(bi, 5);
         }
-        WN_HIP(wn_launch_lane_reduce(dlanes.p + blane[bi], nln, g->partial.p, g->lanebuf.p, comp, nelem, g->stream));
-        mk[6] = mark();
+        WN_HIP(wn_launch_lane_reduce(p.dlanes.p + p.blane[bi], nln, g->partial.p, g->lanebuf.p, comp, nelem,
+                                     g->stream));
+        mark(bi, 6);
     }
+    return WNLQG_OK;
+}
+
+// Lane sums to the caller after the enqueued integral: flags, prefactors, totals.
+int finish_mesh(wnlqg_engine *g, double *lane_out, double *total_out) {
+    const MeshPlan &p = g->plan;
+    const int nelem = output_elems(g);
+    const int n_lanes = p.n_lanes;
+    const long long nk = p.nk;
     std::vector<double> lanes((size_t)n_lanes * nelem * 2);
     WN_HIP(hipMemcpyAsync(lanes.data(), g->lanebuf.p, lanes.size() * sizeof(double), hipMemcpyDeviceToHost,
                           g->stream));
     WN_HIP(hipStreamSynchronize(g->stream));
-    dchunks.release();
-    dlanes.release();
-    wnlqg_stats st{};
-    float ms = 0.0f;
-    for (long long bi = 0; bi < nbatch; ++bi) {
-        auto &mk = marks[bi];
-        if (!mk[0] || !mk[6]) continue;
-        hipEventElapsedTime(&ms, mk[1], mk[2]); st.interp_ms += ms;
-        hipEventElapsedTime(&ms, mk[2], mk[3]); st.eig_ms += ms;
-        hipEventElapsedTime(&ms, mk[3], mk[4]); st.rotate_ms += ms;
-        hipEventElapsedTime(&ms, mk[4], mk[5]); st.pairs_ms += ms;
-        hipEventElapsedTime(&ms, mk[5], mk[6]); st.accum_ms += ms;
-    }
-    if (nbatch > 0 && marks[0][0] && marks[nbatch - 1][6]) {
-        hipEventElapsedTime(&ms, marks[0][0], marks[nbatch - 1][6]);
-        st.total_ms = ms;
-    }
-    for (auto e : evs) hipEventDestroy(e);
-    rc = check_flags(g);
+    int rc = check_flags(g);
     if (rc) return rc;
-    int fl[2] = {0, 0};
-    WN_HIP(hipMemcpy(fl, g->flags.p, 2 * sizeof(int), hipMemcpyDeviceToHost));
-    st.max_sweeps = fl[1];
-    st.kpoints = nk;
-    st.batches = (int)nbatch;
-    st.interp_launches = (int)nbatch;
-    st.interp_flops = 8.0 * g->sh.nr * (double)g->sh.n * g->sh.n * g->sh.C * (double)nk;
-    st.interp_bytes = interp_bytes;
-    st.separable = sep ? 1 : 0;
-    if (stats) *stats = st;
-
     if (g->family == 3) {
         // SHG: lane = raw sums (SecondHarmonicDriver.jl:178-185); total quantity q * factors[q] / N_k
         const size_t rel = (size_t)result_elems(g);
@@ -963,27 +985,295 @@ int32_t wnlqg_run_mesh(wnlqg_engine *g, const int32_t mesh[3], int32_t n_lanes, 
     }
     // SC / IC: each state * its prefactor, then 1/N_k (consume_point!, IntegralDriver.jl:820-874).
     // BCD / AHC: lanes hold raw sums of the per-point values (BerryMultipoleDriver.jl:279-285); the total is
-    // divided by N_k and multiplied by the measure (BerryMultipoleDriver.jl:290-303, SpectralResponseDriver.jl:541-552).
+    // divided by N_k and by the measure (BerryMultipoleDriver.jl:290-303, SpectralResponseDriver.jl:541-552).
     const double scale = g->family ? 1.0 : 1.0 / (double)nk;
-    const int per_state = nelem / g->sc.nq;
     for (int l = 0; l < n_lanes; ++l)
-        for (int qv = 0; qv < g->sc.nq; ++qv)
-            for (int e = 0; e < per_state; ++e) {
-                size_t i = ((size_t)l * nelem + (size_t)qv * per_state + e) * 2;
-                lane_out[i] = (g->pref[qv] * lanes[i]) * scale;
-                lane_out[i + 1] = (g->pref[qv] * lanes[i + 1]) * scale;
-            }
+        for (int e = 0; e < nelem; ++e) {
+            const size_t i = ((size_t)l * nelem + e) * 2;
+            lane_out[i] = (g->elem_pref[e] * lanes[i]) * scale;
+            lane_out[i + 1] = (g->elem_pref[e] * lanes[i + 1]) * scale;
+        }
     for (size_t i = 0; i < lanes.size(); ++i)
         if (!std::isfinite(lane_out[i])) return fail(WNLQG_E_NONFINITE, "non-finite response value");
     if (total_out) {
         for (size_t i = 0; i < (size_t)nelem * 2; ++i) total_out[i] = 0.0;
         for (int l = 0; l < n_lanes; ++l)
             for (size_t i = 0; i < (size_t)nelem * 2; ++i) total_out[i] += lane_out[(size_t)l * nelem * 2 + i];
-        if (g->family) {
-            const double vol = std::fabs(g->cell_volume) * (g->family == 2 ? 1.0e-30 : 1.0);
-            for (size_t i = 0; i < (size_t)nelem * 2; ++i) total_out[i] = total_out[i] / (double)nk / vol;
+        for (int e = 0; e < nelem; ++e)
+            if (g->elem_norm[e] != 0.0) {
+                total_out[2 * e] = total_out[2 * e] / (double)nk / g->elem_norm[e];
+                total_out[2 * e + 1] = total_out[2 * e + 1] / (double)nk / g->elem_norm[e];
+            }
+    }
+    return WNLQG_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int32_t wnlqg_run_mesh(wnlqg_engine *g, const int32_t mesh[3], int32_t n_lanes, double *lane_out, double *total_out,
+                       wnlqg_stats *stats) {
+    if (!g || !mesh || !lane_out) return fail(WNLQG_E_INVALID, "invalid arguments");
+    if (mesh[0] < 1 || mesh[1] < 1 || mesh[2] < 1) return fail(WNLQG_E_INVALID, "mesh sizes must be positive");
+    const long long nk = (long long)mesh[0] * mesh[1] * mesh[2];
+    if (n_lanes < 1 || n_lanes > nk) return fail(WNLQG_E_INVALID, "n_lanes must be in [1, N_k]");
+    WN_HIP(hipSetDevice(g->device));
+    int rc = plan_mesh(g, mesh, n_lanes);
+    if (rc) return rc;
+    const long long nbatch = (long long)g->plan.batches.size();
+    std::vector<std::array<hipEvent_t, 7>> marks(nbatch);
+    bool events_ok = true;
+    for (auto &mk : marks)
+        for (auto &e : mk) {
+            e = nullptr;
+            if (events_ok && hipEventCreate(&e) != hipSuccess) events_ok = false;
+        }
+    auto release_events = [&]() {
+        for (auto &mk : marks)
+            for (auto e : mk)
+                if (e) hipEventDestroy(e);
+    };
+    double interp_bytes = 0.0;
+    rc = enqueue_mesh(g, events_ok ? &marks : nullptr, &interp_bytes);
+    if (!rc) rc = finish_mesh(g, lane_out, total_out);
+    if (rc) {
+        release_events();
+        return rc;
+    }
+    wnlqg_stats st{};
+    float ms = 0.0f;
+    if (events_ok) {
+        for (long long bi = 0; bi < nbatch; ++bi) {
+            auto &mk = marks[bi];
+            hipEventElapsedTime(&ms, mk[1], mk[2]); st.interp_ms += ms;
+            hipEventElapsedTime(&ms, mk[2], mk[3]); st.eig_ms += ms;
+            hipEventElapsedTime(&ms, mk[3], mk[4]); st.rotate_ms += ms;
+            hipEventElapsedTime(&ms, mk[4], mk[5]); st.pairs_ms += ms;
+            hipEventElapsedTime(&ms, mk[5], mk[6]); st.accum_ms += ms;
+        }
+        if (nbatch > 0) {
+            hipEventElapsedTime(&ms, marks[0][0], marks[nbatch - 1][6]);
+            st.total_ms = ms;
         }
     }
+    release_events();
+    int fl[2] = {0, 0};
+    WN_HIP(hipMemcpy(fl, g->flags.p, 2 * sizeof(int), hipMemcpyDeviceToHost));
+    st.max_sweeps = fl[1];
+    st.kpoints = nk;
+    st.batches = (int)nbatch;
+    st.interp_launches = (int)nbatch;
+    st.interp_flops = 8.0 * g->sh.nr * (double)g->sh.n * g->sh.n * g->sh.C * (double)nk;
+    st.interp_bytes = interp_bytes;
+    st.separable = g->plan.sep ? 1 : 0;
+    if (stats) *stats = st;
+    return WNLQG_OK;
+}
+
+// ------------------------------------------------------------------------------------------------
+// Batching form: many (model, task, mesh) triples of uneven n_w in one captured HIP graph.
+// ------------------------------------------------------------------------------------------------
+}  // extern "C"
+
+struct wnlqg_batch {
+    int device = 0;
+    std::vector<wnlqg_engine *> engines;
+    hipStream_t s0 = nullptr;
+    std::vector<hipStream_t> streams;
+    std::vector<hipEvent_t> events;
+    hipEvent_t t0 = nullptr, t1 = nullptr;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+};
+
+namespace {
+
+// complex channels stored per k point for a task family (H, dH, ddH, A, dA as needed)
+int channels_for(int family, bool pos) {
+    if (family == 1) return pos ? 16 : 4;
+    return pos ? WN_CH_FULL : WN_CH_BASE;
+}
+
+// Autotuner: Jacobi threads per k point for each distinct n_w, timed on the first batch of each size
+// class's first engine (interpolation of that batch, then the eigensolver with 64, 128, 256 threads).
+int autotune(wnlqg_batch *b) {
+    std::vector<int> done(65, 0);
+    hipEvent_t a, z;
+    WN_HIP(hipEventCreate(&a));
+    WN_HIP(hipEventCreate(&z));
+    int rc = WNLQG_OK;
+    for (wnlqg_engine *g : b->engines) {
+        const int n = g->sh.n;
+        if (done[n]) {
+            g->eig_tpb = done[n];
+            continue;
+        }
+        const int cnt = (int)std::min<long long>(std::min<long long>(g->batch, 8192), g->plan.nk);
+        const int rows = (cnt + WN_GM - 1) / WN_GM * WN_GM;
+        if (wn_launch_mesh_kpoints(g->kp.p, g->plan.mesh, 0, cnt, rows, g->stream) != hipSuccess) {
+            rc = fail(WNLQG_E_HIP, "autotune probe failed");
+            break;
+        }
+        rc = run_front(g, cnt, rows, false);
+        if (rc) break;
+        float best = 1e30f;
+        int best_tpb = 0;
+        for (int tpb : {64, 128, 256}) {
+            float ms = 0.0f;
+            for (int rep = 0; rep < 2; ++rep) {  // first launch warms up
+                hipEventRecord(a, g->stream);
+                wn_launch_eig_f64(g->x.p, g->e.p, g->u.p, g->sweeps.p, cnt, g->sh, tpb, g->stream);
+                hipEventRecord(z, g->stream);
+                if (hipEventSynchronize(z) != hipSuccess) {
+                    rc = fail(WNLQG_E_HIP, "autotune timing failed");
+                    break;
+                }
+                hipEventElapsedTime(&ms, a, z);
+            }
+            if (rc) break;
+            if (ms < best) {
+                best = ms;
+                best_tpb = tpb;
+            }
+        }
+        if (rc) break;
+        g->eig_tpb = best_tpb;
+        done[n] = best_tpb;
+    }
+    hipEventDestroy(a);
+    hipEventDestroy(z);
+    return rc;
+}
+
+}  // namespace
+
+extern "C" {
+
+int32_t wnlqg_batch_destroy(wnlqg_batch *b) {
+    if (!b) return WNLQG_OK;
+    (void)hipSetDevice(b->device);
+    if (b->s0) (void)hipStreamSynchronize(b->s0);
+    if (b->exec) (void)hipGraphExecDestroy(b->exec);
+    if (b->graph) (void)hipGraphDestroy(b->graph);
+    for (auto e : b->events) (void)hipEventDestroy(e);
+    if (b->t0) (void)hipEventDestroy(b->t0);
+    if (b->t1) (void)hipEventDestroy(b->t1);
+    for (auto st : b->streams) (void)hipStreamDestroy(st);
+    if (b->s0) (void)hipStreamDestroy(b->s0);
+    for (auto g : b->engines) wnlqg_destroy(g);
+    delete b;
+    return WNLQG_OK;
+}
+
+int32_t wnlqg_batch_create(int32_t count, const wnlqg_model *models, const wnlqg_task *tasks, const int32_t *meshes,
+                           const int32_t *n_lanes, int32_t autotune_tiles, wnlqg_batch **out) {
+    if (!out) return fail(WNLQG_E_INVALID, "out is NULL");
+    *out = nullptr;
+    if (count < 1 || !models || !tasks || !meshes || !n_lanes) return fail(WNLQG_E_INVALID, "invalid arguments");
+    for (int i = 0; i < count; ++i) {
+        const int32_t *m = meshes + 3 * i;
+        if (m[0] < 1 || m[1] < 1 || m[2] < 1) return fail(WNLQG_E_INVALID, "entry %d: mesh sizes must be positive", i);
+        const long long nk = (long long)m[0] * m[1] * m[2];
+        if (n_lanes[i] < 1 || n_lanes[i] > nk) return fail(WNLQG_E_INVALID, "entry %d: n_lanes must be in [1, N_k]", i);
+    }
+    wnlqg_batch *b = new wnlqg_batch();
+    auto bail = [&](int code) {
+        wnlqg_batch_destroy(b);
+        return code;
+    };
+    if (hipGetDevice(&b->device) != hipSuccess) return bail(fail(WNLQG_E_HIP, "no HIP device"));
+    // one engine per entry; working sets share a 96 GB budget unless the task fixes its batch size
+    for (int i = 0; i < count; ++i) {
+        wnlqg_task t = tasks[i];
+        if (t.batch_size == 0) {
+            const int fam = task_family(t.quantities);
+            const double n2 = (double)models[i].num_orbitals * models[i].num_orbitals;
+            const double per_k = 2.4 * channels_for(fam, models[i].position_r != nullptr) * n2 * 8.0 + 64.0 * n2 + 4096.0;
+            const long long nk = (long long)meshes[3 * i] * meshes[3 * i + 1] * meshes[3 * i + 2];
+            long long bs = (long long)(96.0e9 / count / per_k);
+            bs = std::max(128LL, std::min({bs, 65536LL, std::max(nk, 128LL)}));
+            t.batch_size = (int32_t)bs;
+        }
+        wnlqg_engine *g = nullptr;
+        int rc = wnlqg_create(models + i, &t, &g);
+        if (rc) return bail(fail(rc, "entry %d: %s", i, g_last_error.c_str()));
+        b->engines.push_back(g);
+        rc = plan_mesh(g, meshes + 3 * i, n_lanes[i]);
+        if (rc) return bail(rc);
+        if (hipStreamSynchronize(g->stream) != hipSuccess) return bail(fail(WNLQG_E_HIP, "entry %d: setup failed", i));
+    }
+    if (autotune_tiles) {
+        int rc = autotune(b);
+        if (rc) return bail(rc);
+    }
+    // capture every entry's kernel sequence into one graph: entries are dealt round-robin to four
+    // streams forked from and joined back into the capture stream, so independent models overlap
+    const int ns = std::min(count, 4);
+    if (hipStreamCreateWithFlags(&b->s0, hipStreamNonBlocking) != hipSuccess ||
+        hipEventCreate(&b->t0) != hipSuccess || hipEventCreate(&b->t1) != hipSuccess)
+        return bail(fail(WNLQG_E_HIP, "stream / event creation failed"));
+    for (int s = 0; s < ns; ++s) {
+        hipStream_t st;
+        if (hipStreamCreateWithFlags(&st, hipStreamNonBlocking) != hipSuccess)
+            return bail(fail(WNLQG_E_HIP, "stream creation failed"));
+        b->streams.push_back(st);
+    }
+    for (int e = 0; e < ns + 1; ++e) {
+        hipEvent_t ev;
+        if (hipEventCreateWithFlags(&ev, hipEventDisableTiming) != hipSuccess)
+            return bail(fail(WNLQG_E_HIP, "event creation failed"));
+        b->events.push_back(ev);
+    }
+    if (hipStreamBeginCapture(b->s0, hipStreamCaptureModeThreadLocal) != hipSuccess)
+        return bail(fail(WNLQG_E_HIP, "stream capture failed to start"));
+    int rc = WNLQG_OK;
+    hipError_t he = hipEventRecord(b->events[0], b->s0);
+    for (int s = 0; s < ns && he == hipSuccess; ++s) he = hipStreamWaitEvent(b->streams[s], b->events[0], 0);
+    for (int i = 0; i < count && he == hipSuccess && !rc; ++i) {
+        wnlqg_engine *g = b->engines[i];
+        hipStream_t own = g->stream;
+        g->stream = b->streams[i % ns];
+        rc = enqueue_mesh(g, nullptr, nullptr);
+        g->stream = own;
+    }
+    for (int s = 0; s < ns && he == hipSuccess; ++s) {
+        he = hipEventRecord(b->events[s + 1], b->streams[s]);
+        if (he == hipSuccess) he = hipStreamWaitEvent(b->s0, b->events[s + 1], 0);
+    }
+    hipError_t ce = hipStreamEndCapture(b->s0, &b->graph);
+    if (rc) return bail(rc);
+    if (he != hipSuccess || ce != hipSuccess) return bail(fail(WNLQG_E_HIP, "graph capture failed"));
+    if (hipGraphInstantiate(&b->exec, b->graph, nullptr, nullptr, 0) != hipSuccess)
+        return bail(fail(WNLQG_E_HIP, "graph instantiation failed"));
+    *out = b;
+    return WNLQG_OK;
+}
+
+int32_t wnlqg_batch_run(wnlqg_batch *b, double *const *lane_out, double *const *total_out, double *elapsed_ms) {
+    if (!b || !lane_out) return fail(WNLQG_E_INVALID, "invalid arguments");
+    for (size_t i = 0; i < b->engines.size(); ++i)
+        if (!lane_out[i]) return fail(WNLQG_E_INVALID, "lane_out[%zu] is NULL", i);
+    WN_HIP(hipSetDevice(b->device));
+    WN_HIP(hipEventRecord(b->t0, b->s0));
+    WN_HIP(hipGraphLaunch(b->exec, b->s0));
+    WN_HIP(hipEventRecord(b->t1, b->s0));
+    WN_HIP(hipStreamSynchronize(b->s0));
+    float ms = 0.0f;
+    WN_HIP(hipEventElapsedTime(&ms, b->t0, b->t1));
+    if (elapsed_ms) *elapsed_ms = ms;
+    for (size_t i = 0; i < b->engines.size(); ++i) {
+        int rc = finish_mesh(b->engines[i], lane_out[i], total_out ? total_out[i] : nullptr);
+        if (rc) return fail(rc, "entry %zu: %s", i, g_last_error.c_str());
+    }
+    return WNLQG_OK;
+}
+
+int32_t wnlqg_batch_tuning(const wnlqg_batch *b, int32_t index, int32_t *eig_threads) {
+    if (!b || index < 0 || index >= (int32_t)b->engines.size() || !eig_threads)
+        return fail(WNLQG_E_INVALID, "invalid arguments");
+    const wnlqg_engine *g = b->engines[index];
+    *eig_threads = g->eig_tpb;
     return WNLQG_OK;
 }
 

@@ -25,7 +25,7 @@
 #include <math.h>
 
 #define WN_GEOM_TPB 256
-#define WN_GEOM_OUT 7  // outputs per thread: ceil(27 * WN_GEOM_MAXMU / 256)
+#define WN_GEOM_OUT 9  // outputs per thread: ceil((27 + 9) * WN_GEOM_MAXMU / 256)
 
 namespace {
 
@@ -71,8 +71,10 @@ __global__ __launch_bounds__(WN_GEOM_TPB) void k_geom_f64(const double *__restri
     const int n = sh.n, n2 = n * n;
     const int t = threadIdx.x;
     const int nmu = gp.nmu;
-    const int ncomp = gp.kind == 1 ? 27 : 9;
-    const int nelem = ncomp * nmu;
+    const bool do_bcd = (gp.kind & 1) != 0, do_ahc = (gp.kind & 2) != 0;
+    // outputs: BCD [27][nmu] first, then AHC [9][nmu]
+    const int aoff = do_bcd ? 27 * nmu : 0;
+    const int nelem = aoff + (do_ahc ? 9 * nmu : 0);
     // LDS carve: E[64] | inv[n2] | Om[n*9] | Fd[n*9] | Vd[n*3] | W[nmu*n] | red[256*9] | res[9*nmu]
     double *E = lds;
     double *inv = E + 64;
@@ -80,7 +82,7 @@ __global__ __launch_bounds__(WN_GEOM_TPB) void k_geom_f64(const double *__restri
     double *Fd = Om + n * 9;
     double *Vd = Fd + n * 9;
     double *W = Vd + n * 3;
-    double *red = W + (gp.kind == 1 ? nmu * n : 0);
+    double *red = W + (do_bcd ? nmu * n : 0);
     double *res = red + WN_GEOM_TPB * 9;
 
     const int groups = WN_GEOM_TPB / n;
@@ -99,7 +101,7 @@ __global__ __launch_bounds__(WN_GEOM_TPB) void k_geom_f64(const double *__restri
         const double2 *AA = has_pos ? X + (size_t)gp.c_a * n2 : nullptr;
         if (t < n) E[t] = eg[(size_t)k * n + t];
         __syncthreads();
-        if (gp.kind == 1 && t + 1 < n && !(fabs(E[t + 1] - E[t]) > gp.gap_tol)) {
+        if (do_bcd && t + 1 < n && !(fabs(E[t + 1] - E[t]) > gp.gap_tol)) {
             atomicOr(flags + 2, 1);
             atomicMin(flags + 3, (int)(kbase + k));
         }
@@ -131,7 +133,7 @@ __global__ __launch_bounds__(WN_GEOM_TPB) void k_geom_f64(const double *__restri
                     bn[a] = make_double2(an[a].x + dn.y * ivn, an[a].y - dn.x * ivn);
                     bm[a] = make_double2(am[a].x + dm.y * ivm, am[a].y - dm.x * ivm);
                 }
-                if (gp.kind == 1 && m != band) {
+                if (do_bcd && m != band) {
 #pragma unroll
                     for (int a = 0; a < 3; ++a)
 #pragma unroll
@@ -156,7 +158,7 @@ __global__ __launch_bounds__(WN_GEOM_TPB) void k_geom_f64(const double *__restri
             Fd[e] = f;
         }
         __syncthreads();
-        if (gp.kind == 1) {
+        if (do_bcd) {
             group_reduce9(red, om, band, grp, groups, n);
             for (int e = t; e < n * 9; e += WN_GEOM_TPB) Om[e] = 2.0 * red[e] + Fd[e];
             if (t < n * 3) {
@@ -171,7 +173,7 @@ __global__ __launch_bounds__(WN_GEOM_TPB) void k_geom_f64(const double *__restri
 #pragma unroll
             for (int j = 0; j < WN_GEOM_OUT; ++j) {
                 const int o = t + j * WN_GEOM_TPB;
-                if (o < nelem) {
+                if (o < aoff) {
                     // output component a + 3b + 9c (column-major [mu, a, b, c]); Om is stored [band][3a + b]
                     const int comp = o / nmu, q = o - (o / nmu) * nmu;
                     const int a = comp % 3, b = (comp / 3) % 3, c = comp / 9;
@@ -181,7 +183,8 @@ __global__ __launch_bounds__(WN_GEOM_TPB) void k_geom_f64(const double *__restri
                     acc[j] += s;
                 }
             }
-        } else {
+        }
+        if (do_ahc) {
             for (int q = 0; q < nmu; ++q) {
                 int occ = 0;
                 for (int bb = 0; bb < n; ++bb) occ += E[bb] <= mu[q];
@@ -229,7 +232,7 @@ __global__ __launch_bounds__(WN_GEOM_TPB) void k_geom_f64(const double *__restri
 #pragma unroll
             for (int j = 0; j < WN_GEOM_OUT; ++j) {
                 const int o = t + j * WN_GEOM_TPB;
-                if (o < nelem) acc[j] += res[o];
+                if (o >= aoff && o < nelem) acc[j] += res[o - aoff];
             }
         }
         __syncthreads();
@@ -250,11 +253,11 @@ hipError_t wn_launch_geom_f64(const double *x, const double *e, const double *mu
                               double2 *partial, int *flags, long long kbase, const WnShape &sh, const WnGeom &gp,
                               hipStream_t st) {
     if (nchunks == 0) return hipSuccess;
-    if (sh.n > 64 || gp.nmu < 1 || gp.nmu > WN_GEOM_MAXMU || (gp.kind != 1 && gp.kind != 2))
+    if (sh.n > 64 || gp.nmu < 1 || gp.nmu > WN_GEOM_MAXMU || gp.kind < 1 || gp.kind > 3)
         return hipErrorInvalidValue;
     const int n = sh.n;
     const size_t lds = sizeof(double) * (64 + (size_t)n * n + (size_t)n * 9 * 2 + (size_t)n * 3 +
-                                         (gp.kind == 1 ? (size_t)gp.nmu * n : 0) + (size_t)WN_GEOM_TPB * 9 +
+                                         ((gp.kind & 1) ? (size_t)gp.nmu * n : 0) + (size_t)WN_GEOM_TPB * 9 +
                                          (size_t)9 * gp.nmu);
     hipLaunchKernelGGL(k_geom_f64, dim3(nchunks), dim3(WN_GEOM_TPB), lds, st, x, e, mu, chunks, partial, flags, kbase,
                        sh, gp);

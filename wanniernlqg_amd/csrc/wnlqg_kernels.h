@@ -96,8 +96,12 @@ hipError_t wn_launch_axis_phases(double2 *E, int N, int span, int rmin, hipStrea
 hipError_t wn_launch_contract(const double2 *in, double2 *out, const double2 *E, int P, int p0, int Q, int M, int nb,
                               long long ib, long long iq, long long ob, long long op, hipStream_t st);
 // Jacobi eigensolver of the upper triangle of channel 0.
+// tpb: threads per k point (64, 128, 256; 0 = default of the size class).
 hipError_t wn_launch_eig_f64(const double *x, double *e, double2 *u, int *sweeps, int count, const WnShape &sh,
-                             hipStream_t st);
+                             int tpb, hipStream_t st);
+// Clears that are safe under stream capture: p[0 .. n) = 0; flags = {0, 0, 0, 0x7f7f7f7f}.
+hipError_t wn_launch_zero(double *p, long long n, hipStream_t st);
+hipError_t wn_launch_reset_flags(int *f, hipStream_t st);
 // U^+ X U for channels 1 .. C-1, in place.
 hipError_t wn_launch_rotate_f64(double *x, const double2 *u, int count, const WnShape &sh, hipStream_t st);
 // Screening, pair lists, generalized derivatives and SC/IC kernels.

@@ -21,6 +21,8 @@
 #include "wnlqg_kernels.h"
 
 #include <math.h>
+
+#include <algorithm>
 #include <stdlib.h>
 
 // ------------------------------------------------------------------------------------------------
@@ -1014,41 +1016,54 @@ __global__ __launch_bounds__(TPB) void k_eig_f64(const double *__restrict__ x, d
 }
 
 hipError_t wn_launch_eig_f64(const double *x, double *e, double2 *u, int *sweeps, int count, const WnShape &sh,
-                             hipStream_t st) {
+                             int tpb, hipStream_t st) {
     const int n = sh.n;
     const double tol = 1.0e-15;
-#define WN_EIG_CASE(NN)                                                                                      \
-    if (n <= NN) {                                                                                           \
-        constexpr int TPB = NN >= 32 ? 256 : (NN >= 24 ? 128 : 64);                                          \
-        hipLaunchKernelGGL((k_eig_f64<NN, TPB>), dim3(count), dim3(TPB), 0, st, x, e, u, sweeps, n, sh.ldo, tol); \
-        return hipGetLastError();                                                                            \
+    // tpb: threads per k point (64, 128 or 256); 0 selects the default of the size class
+#define WN_EIG_LAUNCH(NN, T) \
+    hipLaunchKernelGGL((k_eig_f64<NN, T>), dim3(count), dim3(T), 0, st, x, e, u, sweeps, n, sh.ldo, tol)
+#define WN_EIG_CASE(NN, DEF)                   \
+    if (n <= NN) {                             \
+        const int t = tpb ? tpb : DEF;         \
+        if (t == 64)                           \
+            WN_EIG_LAUNCH(NN, 64);             \
+        else if (t == 128)                     \
+            WN_EIG_LAUNCH(NN, 128);            \
+        else                                   \
+            WN_EIG_LAUNCH(NN, 256);            \
+        return hipGetLastError();              \
     }
-    WN_EIG_CASE(8)
-    WN_EIG_CASE(16)
-    WN_EIG_CASE(24)
-    if (n <= 32) {
-        // threads per k point for the 32-band case (tuning knob, default 128)
-        static const int tpb32 = [] {
-            const char *v = getenv("WNLQG_EIG_TPB");
-            return v ? atoi(v) : 256;
-        }();
-        if (tpb32 == 64)
-            hipLaunchKernelGGL((k_eig_f64<32, 64>), dim3(count), dim3(64), 0, st, x, e, u, sweeps, n, sh.ldo, tol);
-        else if (tpb32 == 256)
-            hipLaunchKernelGGL((k_eig_f64<32, 256>), dim3(count), dim3(256), 0, st, x, e, u, sweeps, n, sh.ldo, tol);
-        else if (tpb32 == 512)
-            hipLaunchKernelGGL((k_eig_f64<32, 512>), dim3(count), dim3(512), 0, st, x, e, u, sweeps, n, sh.ldo, tol);
-        else if (tpb32 == 1024)
-            hipLaunchKernelGGL((k_eig_f64<32, 1024>), dim3(count), dim3(1024), 0, st, x, e, u, sweeps, n, sh.ldo, tol);
-        else
-            hipLaunchKernelGGL((k_eig_f64<32, 128>), dim3(count), dim3(128), 0, st, x, e, u, sweeps, n, sh.ldo, tol);
-        return hipGetLastError();
-    }
-    WN_EIG_CASE(32)
-    WN_EIG_CASE(48)
-    WN_EIG_CASE(64)
+    WN_EIG_CASE(8, 64)
+    WN_EIG_CASE(16, 64)
+    WN_EIG_CASE(24, 128)
+    WN_EIG_CASE(32, 256)
+    WN_EIG_CASE(48, 256)
+    WN_EIG_CASE(64, 256)
 #undef WN_EIG_CASE
+#undef WN_EIG_LAUNCH
     return hipErrorInvalidValue;
+}
+
+// Capture-safe clears (no memset nodes in captured graphs).
+__global__ void k_zero(double *__restrict__ p, long long n) {
+    for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x)
+        p[i] = 0.0;
+}
+
+__global__ void k_reset_flags(int *__restrict__ f) {
+    if (threadIdx.x < 4) f[threadIdx.x] = threadIdx.x == 3 ? 0x7f7f7f7f : 0;
+}
+
+hipError_t wn_launch_zero(double *p, long long n, hipStream_t st) {
+    if (n <= 0) return hipSuccess;
+    const long long blocks = std::min<long long>((n + 255) / 256, 4096);
+    hipLaunchKernelGGL(k_zero, dim3((unsigned)blocks), dim3(256), 0, st, p, n);
+    return hipGetLastError();
+}
+
+hipError_t wn_launch_reset_flags(int *f, hipStream_t st) {
+    hipLaunchKernelGGL(k_reset_flags, dim3(1), dim3(64), 0, st, f);
+    return hipGetLastError();
 }
 
 // ------------------------------------------------------------------------------------------------

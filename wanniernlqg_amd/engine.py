@@ -38,7 +38,8 @@ EXPORTED = ["wnlqg_version", "wnlqg_last_error", "wnlqg_create", "wnlqg_destroy"
             "wnlqg_num_channels", "wnlqg_run_mesh", "wnlqg_eval_points", "wnlqg_fourier_points",
             "wnlqg_spectrum_points", "wnlqg_synchronize", "wnlqg_mlp_create", "wnlqg_mlp_destroy",
             "wnlqg_mlp_last_error", "wnlqg_mlp_get_params", "wnlqg_mlp_set_params", "wnlqg_mlp_predict",
-            "wnlqg_mlp_gradient", "wnlqg_mlp_train"]
+            "wnlqg_mlp_gradient", "wnlqg_mlp_train", "wnlqg_batch_create", "wnlqg_batch_run", "wnlqg_batch_tuning",
+            "wnlqg_batch_destroy"]
 
 
 class WnlqgError(RuntimeError):
@@ -112,6 +113,12 @@ def load_library(path: str = LIB_PATH) -> C.CDLL:
     lib.wnlqg_mlp_predict.argtypes = [C.c_void_p, fp, C.c_int64, fp]
     lib.wnlqg_mlp_gradient.argtypes = [C.c_void_p, fp, fp, C.c_int64, fp, fp]
     lib.wnlqg_mlp_train.argtypes = [C.c_void_p, fp, fp, C.c_int64, C.c_int32, C.c_int32, C.c_float, C.c_uint64, fp]
+    lib.wnlqg_batch_create.argtypes = [C.c_int32, C.POINTER(_Model), C.POINTER(_Task), C.POINTER(C.c_int32),
+                                       C.POINTER(C.c_int32), C.c_int32, C.POINTER(C.c_void_p)]
+    lib.wnlqg_batch_run.argtypes = [C.c_void_p, C.POINTER(C.POINTER(C.c_double)), C.POINTER(C.POINTER(C.c_double)),
+                                    C.POINTER(C.c_double)]
+    lib.wnlqg_batch_tuning.argtypes = [C.c_void_p, C.c_int32, C.POINTER(C.c_int32)]
+    lib.wnlqg_batch_destroy.argtypes = [C.c_void_p]
     for name in EXPORTED:
         getattr(lib, name).restype = C.c_int32
     _lib = lib
@@ -137,7 +144,7 @@ class EngineTask:
     ``degeneracy_threshold``, ``denominator_regularization`` = 0 by default for BCD,
     ``TaskConfiguration.jl:568-578``), or intrinsic AHC alone (``SpectralResponseDriver.jl``, T = 0)."""
     photon_energies: Optional[np.ndarray] = None
-    shift_current: bool = True
+    shift_current: Optional[bool] = None      # None: on unless another quantity is requested
     injection_current: bool = False
     berry_curvature_dipole: bool = False
     intrinsic_hall: bool = False
@@ -165,9 +172,19 @@ class Engine:
     """One model + one resolved task on the current HIP device."""
 
     def __init__(self, model: TightBindingModel, task: EngineTask):
+        m, t = self._marshal(model, task)
+        h = C.c_void_p()
+        _check(self.lib, self.lib.wnlqg_create(C.byref(m), C.byref(t), C.byref(h)))
+        self.handle = h
+        self.num_states = self.lib.wnlqg_num_states(h)
+        self.num_channels = self.lib.wnlqg_num_channels(h)
+
+    def _marshal(self, model: TightBindingModel, task: EngineTask):
+        """Julia-layout copies of the model and the wnlqg_model / wnlqg_task structs (kept alive by self)."""
         self.lib = load_library()
         self.model = model
         self.task = task
+        self.handle = None
         n, nr = model.num_orbitals, model.num_r_vectors
         # Julia column-major layouts
         self._lat = np.ascontiguousarray(model.lattice.T).ravel()                     # lattice[i,a] at i+3a
@@ -187,14 +204,10 @@ class Engine:
                                                       else task.fermi_energies), dtype=np.float64)
         m = _Model(n, nr, _ptr(self._lat), _ptr(self._rv, C.c_int32), _ptr(self._deg, C.c_int32), _ptr(self._hr),
                    _ptr(self._ar) if self._ar is not None else C.POINTER(C.c_double)())
-        if shg:
-            q = WNLQG_SECOND_HARMONIC | (WNLQG_BERRY_CURVATURE_DIPOLE if geometry else 0)
-        elif geometry:
-            q = (WNLQG_BERRY_CURVATURE_DIPOLE if task.berry_curvature_dipole else 0) | \
-                (WNLQG_INTRINSIC_HALL if task.intrinsic_hall else 0)
-        else:
-            q = (WNLQG_SHIFT_CURRENT if task.shift_current else 0) | \
-                (WNLQG_INJECTION_CURRENT if task.injection_current else 0)
+        sc = task.shift_current if task.shift_current is not None else not (geometry or shg or task.injection_current)
+        q = ((WNLQG_SHIFT_CURRENT if sc else 0) | (WNLQG_INJECTION_CURRENT if task.injection_current else 0) |
+             (WNLQG_BERRY_CURVATURE_DIPOLE if task.berry_curvature_dipole else 0) |
+             (WNLQG_INTRINSIC_HALL if task.intrinsic_hall else 0) | (WNLQG_SECOND_HARMONIC if shg else 0))
         bt = {"Gaussian": WNLQG_GAUSSIAN, "Lorentzian": WNLQG_LORENTZIAN}.get(task.broadening_type, 99)
         prec = {"fp64": WNLQG_PRECISION_FP64, "mixed": WNLQG_PRECISION_MIXED,
                 "mixed6": WNLQG_PRECISION_MIXED_HIGH}[task.precision]
@@ -205,22 +218,35 @@ class Engine:
                   task.degeneracy_threshold, 0 if task.shg_components is None else len(comps),
                   _ptr(self._comps, C.c_int32), task.shg_low_frequency_broadening,
                   task.shg_intermediate_regularization, 1 if task.shg_eta_correction else 0)
-        h = C.c_void_p()
-        _check(self.lib, self.lib.wnlqg_create(C.byref(m), C.byref(t), C.byref(h)))
-        self.handle = h
-        self.num_states = self.lib.wnlqg_num_states(h)
-        self.num_channels = self.lib.wnlqg_num_channels(h)
         self.shg = shg
         self.shg_components = comps
         if shg:
             self.states = ["shg"]
         elif geometry:
-            self.states = ["bcd"] if task.berry_curvature_dipole else ["ahc"]
+            self.states = (["bcd"] if task.berry_curvature_dipole else []) + (["ahc"] if task.intrinsic_hall else [])
         else:
-            self.states = [s for s, on in (("sc", task.shift_current), ("ic", task.injection_current)) if on]
+            self.states = [s for s, on in (("sc", sc), ("ic", task.injection_current)) if on]
         self.geometry = geometry
         self._axis = self._mu.shape[0] if geometry else self._om.shape[0]
-        self._ncomp = 9 if task.intrinsic_hall and geometry else 27
+        # complex elements per k point (or lane) of each state
+        if geometry:
+            self._sizes = [(s, (27 if s == "bcd" else 9) * self._axis) for s in self.states]
+        else:
+            self._sizes = [(s, 27 * self._axis) for s in self.states]
+        self.num_states = len(self.states)
+        self._structs = (m, t)
+        return m, t
+
+    def _unpack_mesh(self, lane_out: np.ndarray, total_out: np.ndarray, nl: int):
+        if self.shg:
+            return {"shg": self._unpack_shg(lane_out, (nl,))}, {"shg": self._unpack_shg(total_out, ())}
+        if self.geometry:
+            return self._unpack_geometry(lane_out, (nl,)), self._unpack_geometry(total_out, ())
+        nw = self._axis
+        lanes = _unpack(lane_out, (nl,), self.num_states, nw)
+        total = _unpack(total_out, (), self.num_states, nw)
+        return ({s: lanes[..., i, :, :, :, :] for i, s in enumerate(self.states)},
+                {s: total[i] for i, s in enumerate(self.states)})
 
     def close(self):
         if getattr(self, "handle", None):
@@ -253,20 +279,13 @@ class Engine:
         st = Stats()
         _check(self.lib, self.lib.wnlqg_run_mesh(self.handle, _ptr(mesh_a, C.c_int32), nl, _ptr(lane_out),
                                                  _ptr(total_out), C.byref(st)))
-        if self.shg:
-            return ({"shg": self._unpack_shg(lane_out, (nl,))}, {"shg": self._unpack_shg(total_out, ())}, st.as_dict())
-        if self.geometry:
-            return ({self.states[0]: self._unpack_geometry(lane_out, (nl,))},
-                    {self.states[0]: self._unpack_geometry(total_out, ())}, st.as_dict())
-        lanes = _unpack(lane_out, (nl,), self.num_states, nw)
-        total = _unpack(total_out, (), self.num_states, nw)
-        return ({s: lanes[..., i, :, :, :, :] for i, s in enumerate(self.states)},
-                {s: total[i] for i, s in enumerate(self.states)}, st.as_dict())
+        lanes, total = self._unpack_mesh(lane_out, total_out, nl)
+        return lanes, total, st.as_dict()
 
     def _result_elems(self) -> int:
         if self.shg:
             return self._om.shape[0] * len(self.shg_components) * 14
-        return self.num_states * self._ncomp * self._axis
+        return sum(size for _, size in self._sizes)
 
     def _unpack_shg(self, flat: np.ndarray, lead: tuple) -> np.ndarray:
         """Complex column-major [n_w, n_comp, 7, 2] -> [..., n_w, n_comp, 7, 2]."""
@@ -275,16 +294,23 @@ class Engine:
         L = len(lead)
         return np.transpose(z, tuple(range(L)) + (L + 3, L + 2, L + 1, L)).copy()
 
-    def _unpack_geometry(self, flat: np.ndarray, lead: tuple) -> np.ndarray:
-        """Complex column-major [n_mu,3,3,3] (BCD) or [n_mu,3,3] (AHC) -> real [..., n_mu, a, b(, c)]."""
-        z = flat.view(np.complex128)
-        if self._ncomp == 27:
-            z = z.reshape(lead + (3, 3, 3, self._axis))
-            perm = (len(lead) + 3, len(lead) + 2, len(lead) + 1, len(lead))
-        else:
-            z = z.reshape(lead + (3, 3, self._axis))
-            perm = (len(lead) + 2, len(lead) + 1, len(lead))
-        return np.transpose(z, tuple(range(len(lead))) + perm).real.copy()
+    def _unpack_geometry(self, flat: np.ndarray, lead: tuple) -> dict:
+        """Per state: complex column-major [n_mu,3,3,3] (BCD) then [n_mu,3,3] (AHC) -> real [..., n_mu, a, b(, c)]."""
+        total = self._result_elems()
+        z = flat.view(np.complex128).reshape(lead + (total,))
+        out, off = {}, 0
+        L = len(lead)
+        for s, size in self._sizes:
+            blk = z[..., off:off + size]
+            off += size
+            if s == "bcd":
+                blk = blk.reshape(lead + (3, 3, 3, self._axis))
+                perm = (L + 3, L + 2, L + 1, L)
+            else:
+                blk = blk.reshape(lead + (3, 3, self._axis))
+                perm = (L + 2, L + 1, L)
+            out[s] = np.transpose(blk, tuple(range(L)) + perm).real.copy()
+        return out
 
     def eval_points(self, kpoints: np.ndarray):
         k = np.ascontiguousarray(kpoints, dtype=np.float64).reshape(-1, 3)
@@ -294,7 +320,7 @@ class Engine:
         if self.shg:
             return {"shg": self._unpack_shg(out, (k.shape[0],))}
         if self.geometry:
-            return {self.states[0]: self._unpack_geometry(out, (k.shape[0],))}
+            return self._unpack_geometry(out, (k.shape[0],))
         vals = _unpack(out, (k.shape[0],), self.num_states, nw)
         return {s: vals[:, i] for i, s in enumerate(self.states)}
 
@@ -322,3 +348,73 @@ def _unpack(flat: np.ndarray, lead: tuple, nq: int, nw: int) -> np.ndarray:
     z = flat.view(np.complex128).reshape(lead + (nq, 3, 3, 3, nw))   # c, b, a, w (C order of Julia [w,a,b,c])
     return np.transpose(z, tuple(range(len(lead))) + (len(lead), len(lead) + 4, len(lead) + 3, len(lead) + 2,
                                                        len(lead) + 1))
+
+
+class Batch:
+    """Batching form (``wnlqg_batch_*``): many (model, task, mesh) integrals of uneven n_w captured in
+    one HIP graph and replayed by one launch."""
+
+    def __init__(self, entries, autotune: bool = True):
+        """entries: sequence of (model, EngineTask, mesh[, n_lanes])."""
+        self.lib = load_library()
+        self.shells = []
+        models, tasks, meshes, lanes = [], [], [], []
+        for ent in entries:
+            model, task, mesh = ent[0], ent[1], ent[2]
+            shell = Engine.__new__(Engine)
+            m, t = shell._marshal(model, task)
+            nk = int(np.prod(np.asarray(mesh, dtype=np.int64)))
+            nl = min(nk, 64) if len(ent) < 4 or ent[3] is None else int(ent[3])
+            self.shells.append((shell, nl))
+            models.append(m)
+            tasks.append(t)
+            meshes.extend(int(x) for x in mesh)
+            lanes.append(nl)
+        count = len(self.shells)
+        self._models = (_Model * count)(*models)
+        self._tasks = (_Task * count)(*tasks)
+        self._meshes = np.ascontiguousarray(meshes, dtype=np.int32)
+        self._lanes = np.ascontiguousarray(lanes, dtype=np.int32)
+        h = C.c_void_p()
+        _check(self.lib, self.lib.wnlqg_batch_create(count, self._models, self._tasks, _ptr(self._meshes, C.c_int32),
+                                                     _ptr(self._lanes, C.c_int32), 1 if autotune else 0,
+                                                     C.byref(h)))
+        self.handle = h
+
+    def tuning(self):
+        out = []
+        for i in range(len(self.shells)):
+            v = C.c_int32()
+            _check(self.lib, self.lib.wnlqg_batch_tuning(self.handle, i, C.byref(v)))
+            out.append(v.value)
+        return out
+
+    def run(self):
+        """Replay the graph; returns ([(lanes, total) per entry], device milliseconds)."""
+        bufs = []
+        for shell, nl in self.shells:
+            per = shell._result_elems()
+            bufs.append((np.zeros(nl * per * 2), np.zeros(per * 2)))
+        lp = (C.POINTER(C.c_double) * len(bufs))(*[_ptr(b[0]) for b in bufs])
+        tp = (C.POINTER(C.c_double) * len(bufs))(*[_ptr(b[1]) for b in bufs])
+        ms = C.c_double()
+        _check(self.lib, self.lib.wnlqg_batch_run(self.handle, lp, tp, C.byref(ms)))
+        results = [shell._unpack_mesh(b[0], b[1], nl) for (shell, nl), b in zip(self.shells, bufs)]
+        return results, ms.value
+
+    def close(self):
+        if getattr(self, "handle", None):
+            self.lib.wnlqg_batch_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
