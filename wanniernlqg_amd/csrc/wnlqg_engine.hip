@@ -133,6 +133,10 @@ struct wnlqg_engine {
     int h_tiles = 0;            // 128-column FP64 tiles holding channel H (mixed forms)
     int partial_cap = 0;  // chunks the partial buffer holds
     int eig_tpb = 0;      // Jacobi threads per k point (0: default; set by the batch autotuner)
+    // mesh pipeline: batch b is interpolated and diagonalized on `stream` into buffer b % 2 while
+    // batch b - 1 is rotated and integrated on `stream2`
+    hipStream_t stream2 = nullptr;
+    hipEvent_t ev_fork = nullptr, ev_join = nullptr, ev_ready[2] = {}, ev_free[2] = {};
     MeshPlan plan;        // batches and chunk / lane tables of the last mesh
     std::vector<double> omega_host;
     hipEvent_t ev[8] = {};
@@ -241,8 +245,9 @@ int result_elems(const wnlqg_engine *g) { return g->family == 3 ? 14 * g->sp.nco
 int ensure_rows(wnlqg_engine *g, int rows) {
     // working set for `rows` k points (multiple of 128)
     const size_t n2 = (size_t)g->sh.n * g->sh.n;
-    if (g->x.n >= (size_t)rows * g->sh.ldo) return WNLQG_OK;
-    WN_HIP(g->x.alloc((size_t)rows * g->sh.ldo));
+    if (g->x.n >= (size_t)2 * rows * g->sh.ldo) return WNLQG_OK;
+    // channels, eigenvalues and eigenvectors are double-buffered for the two-stream mesh pipeline
+    WN_HIP(g->x.alloc((size_t)2 * rows * g->sh.ldo));
     WN_HIP(g->kp.alloc((size_t)rows * 3));
     WN_HIP(g->tab.alloc((size_t)rows * 3 * g->sh.S));
     if (g->kind == 2) WN_HIP(g->tab32.alloc((size_t)rows * 3 * g->sh.S));
@@ -250,8 +255,8 @@ int ensure_rows(wnlqg_engine *g, int rows) {
         WN_HIP(g->t12.alloc((size_t)rows * g->sh.S * g->sh.S));
         WN_HIP(g->t3.alloc((size_t)rows * g->sh.S));
     }
-    WN_HIP(g->e.alloc((size_t)rows * g->sh.n));
-    WN_HIP(g->u.alloc((size_t)rows * n2));
+    WN_HIP(g->e.alloc((size_t)2 * rows * g->sh.n));
+    WN_HIP(g->u.alloc((size_t)2 * rows * n2));
     WN_HIP(g->sweeps.alloc((size_t)rows));
     if (g->family == 3) {
         WN_HIP(g->count.alloc((size_t)rows));
@@ -288,26 +293,26 @@ hipError_t launch_phases(wnlqg_engine *g, int rows) {
 
 // Mixed forms: every channel after H in split BF16, then H (the eigenproblem input, columns
 // [0, 2n^2)) in FP64, which also overwrites the BF16 values of a shared boundary tile.
-hipError_t launch_interp(wnlqg_engine *g, int rows) {
+hipError_t launch_interp(wnlqg_engine *g, int rows, double *out) {
     if (g->nterms) {
         const int hcols = 2 * g->sh.n * g->sh.n;
         hipError_t e = g->kind == 1
-                           ? wn_launch_interp_f16(g->fh.p, g->fm.p, g->cdown.p, g->t12.p, g->t3.p, g->ridx2.p, g->x.p,
+                           ? wn_launch_interp_f16(g->fh.p, g->fm.p, g->cdown.p, g->t12.p, g->t3.p, g->ridx2.p, out,
                                                   rows, g->nterms, hcols, g->sh, g->stream)
-                           : wn_launch_interp_bf16(g->bh.p, g->bm.p, g->bl.p, g->tab32.p, g->ridx.p, g->x.p, rows,
+                           : wn_launch_interp_bf16(g->bh.p, g->bm.p, g->bl.p, g->tab32.p, g->ridx.p, out, rows,
                                                    g->nterms, hcols, g->sh, g->stream);
         if (e != hipSuccess) return e;
         WnShape hs = g->sh;
         hs.nNt = g->h_tiles;
-        return wn_launch_interp_f64(g->bf.p, g->tab.p, g->ridx.p, g->x.p, rows, hs, g->stream);
+        return wn_launch_interp_f64(g->bf.p, g->tab.p, g->ridx.p, out, rows, hs, g->stream);
     }
-    return wn_launch_interp_f64(g->bf.p, g->tab.p, g->ridx.p, g->x.p, rows, g->sh, g->stream);
+    return wn_launch_interp_f64(g->bf.p, g->tab.p, g->ridx.p, out, rows, g->sh, g->stream);
 }
 
 // Interpolation + eigensystem for `count` k points already in g->kp (rows padded to 128).
 int run_front(wnlqg_engine *g, int count, int rows, bool eig) {
     WN_HIP(launch_phases(g, rows));
-    WN_HIP(launch_interp(g, rows));
+    WN_HIP(launch_interp(g, rows, g->x.p));
     if (eig) WN_HIP(wn_launch_eig_f64(g->x.p, g->e.p, g->u.p, g->sweeps.p, count, g->sh, g->eig_tpb, g->stream));
     return WNLQG_OK;
 }
@@ -418,6 +423,9 @@ int32_t wnlqg_destroy(wnlqg_engine *g) {
     g->t3.release(); g->ridx2.release(); g->box.release(); g->y3.release(); g->y23.release(); g->e1.release();
     g->e2.release(); g->e3.release(); g->plan.dchunks.release(); g->plan.dlanes.release();
     g->der.release(); g->coef.release(); g->one.release(); g->pairs.release(); g->pdiff.release();
+    for (hipEvent_t e : {g->ev_fork, g->ev_join, g->ev_ready[0], g->ev_ready[1], g->ev_free[0], g->ev_free[1]})
+        if (e) (void)hipEventDestroy(e);
+    if (g->stream2) (void)hipStreamDestroy(g->stream2);
     if (g->stream) (void)hipStreamDestroy(g->stream);
     delete g;
     return WNLQG_OK;
@@ -450,8 +458,12 @@ int32_t wnlqg_create(const wnlqg_model *model, const wnlqg_task *task, wnlqg_eng
         return code;
     };
     if (hipGetDevice(&g->device) != hipSuccess) return bail(fail(WNLQG_E_HIP, "no HIP device"));
-    if (hipStreamCreateWithFlags(&g->stream, hipStreamNonBlocking) != hipSuccess)
+    if (hipStreamCreateWithFlags(&g->stream, hipStreamNonBlocking) != hipSuccess ||
+        hipStreamCreateWithFlags(&g->stream2, hipStreamNonBlocking) != hipSuccess)
         return bail(fail(WNLQG_E_HIP, "stream creation failed"));
+    for (hipEvent_t *e : {&g->ev_fork, &g->ev_join, &g->ev_ready[0], &g->ev_ready[1], &g->ev_free[0], &g->ev_free[1]})
+        if (hipEventCreateWithFlags(e, hipEventDisableTiming) != hipSuccess)
+            return bail(fail(WNLQG_E_HIP, "event creation failed"));
     for (auto &e : g->ev)
         if (hipEventCreate(&e) != hipSuccess) return bail(fail(WNLQG_E_HIP, "event creation failed"));
 
@@ -879,9 +891,10 @@ int plan_mesh(wnlqg_engine *g, const int32_t mesh[3], int32_t n_lanes) {
     return WNLQG_OK;
 }
 
-// The kernel sequence of one planned mesh integral on g->stream.  Capture-safe: no allocation, no
-// host synchronization, no memset.  marks (optional) holds 7 pre-created events per batch.
-int enqueue_mesh(wnlqg_engine *g, std::vector<std::array<hipEvent_t, 7>> *marks, double *interp_bytes) {
+// The kernel sequence of one planned mesh integral, ordered on g->stream (forked to g->stream2 and
+// joined back).  Capture-safe: no allocation, no host synchronization, no memset.  marks (optional)
+// holds 8 pre-created events per batch.
+int enqueue_mesh(wnlqg_engine *g, std::vector<std::array<hipEvent_t, 8>> *marks, double *interp_bytes) {
     const MeshPlan &p = g->plan;
     const int nelem = output_elems(g);
     const long long nbatch = (long long)p.batches.size();
@@ -890,62 +903,78 @@ int enqueue_mesh(wnlqg_engine *g, std::vector<std::array<hipEvent_t, 7>> *marks,
     WN_HIP(clear_flags(g));
     double2 *comp = g->nterms ? g->lanecomp.p : nullptr;
     const long long m2 = g->sh.ldo / 2;
-    auto mark = [&](long long bi, int i) {
-        if (marks) hipEventRecord((*marks)[bi][i], g->stream);
+    hipStream_t s0 = g->stream, s1 = g->stream2;
+    // fork the second stream from the first (this also brings it into a stream capture)
+    WN_HIP(hipEventRecord(g->ev_fork, s0));
+    WN_HIP(hipStreamWaitEvent(s1, g->ev_fork, 0));
+    auto mark = [&](long long bi, int i, hipStream_t st) {
+        if (marks) hipEventRecord((*marks)[bi][i], st);
     };
     for (long long bi = 0; bi < nbatch; ++bi) {
         const MeshBatch &bt = p.batches[bi];
         const long long k0 = bt.k0;
         const int cnt = bt.cnt;
         const int rows = (cnt + WN_GM - 1) / WN_GM * WN_GM;
-        mark(bi, 0);
+        const int buf = (int)(bi & 1);
+        double *xb = g->x.p + (size_t)buf * g->batch * g->sh.ldo;
+        double *eb = g->e.p + (size_t)buf * g->batch * g->sh.n;
+        double2 *ub = g->u.p + (size_t)buf * g->batch * g->sh.n * g->sh.n;
+        // stream 0: k points, interpolation, eigensystem of batch bi into buffer bi % 2
+        if (bi >= 2) WN_HIP(hipStreamWaitEvent(s0, g->ev_free[buf], 0));
+        mark(bi, 0, s0);
         if (!p.sep) {
-            WN_HIP(wn_launch_mesh_kpoints(g->kp.p, p.mesh, k0, cnt, rows, g->stream));
+            WN_HIP(wn_launch_mesh_kpoints(g->kp.p, p.mesh, k0, cnt, rows, s0));
             WN_HIP(launch_phases(g, rows));
         }
-        mark(bi, 1);
+        mark(bi, 1, s0);
         if (p.sep) {
             const double2 *boxc = (const double2 *)g->box.p;
             if (bt.k2b == 0)
                 WN_HIP(wn_launch_contract(boxc, g->y3.p, g->e3.p, 1, bt.k3, g->span[2], (int)m2,
-                                          g->span[0] * g->span[1], g->span[2] * m2, m2, m2, 0, g->stream));
+                                          g->span[0] * g->span[1], g->span[2] * m2, m2, m2, 0, s0));
             WN_HIP(wn_launch_contract(g->y3.p, g->y23.p, g->e2.p, bt.k2c, bt.k2b, g->span[1], (int)m2, g->span[0],
-                                      g->span[1] * m2, m2, (long long)bt.k2c * m2, m2, g->stream));
-            WN_HIP(wn_launch_contract(g->y23.p, (double2 *)g->x.p, g->e1.p, p.mesh[0], 0, g->span[0], (int)m2, bt.k2c,
-                                      m2, (long long)bt.k2c * m2, (long long)p.mesh[0] * m2, m2, g->stream));
+                                      g->span[1] * m2, m2, (long long)bt.k2c * m2, m2, s0));
+            WN_HIP(wn_launch_contract(g->y23.p, (double2 *)xb, g->e1.p, p.mesh[0], 0, g->span[0], (int)m2, bt.k2c, m2,
+                                      (long long)bt.k2c * m2, (long long)p.mesh[0] * m2, m2, s0));
             if (interp_bytes) *interp_bytes += (double)cnt * g->sh.ldo * 8.0;
         } else {
-            WN_HIP(launch_interp(g, rows));
+            WN_HIP(launch_interp(g, rows, xb));
         }
-        mark(bi, 2);
-        WN_HIP(wn_launch_eig_f64(g->x.p, g->e.p, g->u.p, g->flags.p + 1, cnt, g->sh, g->eig_tpb, g->stream));
-        mark(bi, 3);
-        WN_HIP(wn_launch_rotate_f64(g->x.p, g->u.p, cnt, g->sh, g->stream));
-        mark(bi, 4);
+        mark(bi, 2, s0);
+        WN_HIP(wn_launch_eig_f64(xb, eb, ub, g->flags.p + 1, cnt, g->sh, g->eig_tpb, s0));
+        mark(bi, 3, s0);
+        WN_HIP(hipEventRecord(g->ev_ready[buf], s0));
+        // stream 1: rotation, response and k sums of batch bi
+        WN_HIP(hipStreamWaitEvent(s1, g->ev_ready[buf], 0));
+        if (marks) hipEventRecord((*marks)[bi][7], s1);
+        WN_HIP(wn_launch_rotate_f64(xb, ub, cnt, g->sh, s1));
+        mark(bi, 4, s1);
         const int nch = (int)(p.bchunk[bi + 1] - p.bchunk[bi]);
         const int nln = (int)(p.blane[bi + 1] - p.blane[bi]);
         const int4 *chunks = p.dchunks.p + p.bchunk[bi];
         if (g->family == 0) {
-            WN_HIP(wn_launch_pairs_f64(g->x.p, g->e.p, g->omega.p, g->count.p, g->pmeta.p, g->pmd.p, g->q.p,
-                                       g->flags.p, cnt, g->sh, g->sc, g->stream));
-            mark(bi, 5);
+            WN_HIP(wn_launch_pairs_f64(xb, eb, g->omega.p, g->count.p, g->pmeta.p, g->pmd.p, g->q.p, g->flags.p, cnt,
+                                       g->sh, g->sc, s1));
+            mark(bi, 5, s1);
             WN_HIP(wn_launch_accum(chunks, nch, g->count.p, g->pmeta.p, g->pmd.p, g->q.p, g->omega.p, g->partial.p,
-                                   g->sh, g->sc, g->stream));
+                                   g->sh, g->sc, s1));
         } else if (g->family == 3) {
-            WN_HIP(wn_launch_shg(g->x.p, g->e.p, g->omega.p, chunks, 0, cnt, g->der.p, g->count.p, g->pairs.p,
-                                 g->pdiff.p, g->coef.p, g->one.p, g->partial.p, g->sh, g->sp, g->stream));
-            mark(bi, 5);
-            WN_HIP(wn_launch_shg(g->x.p, g->e.p, g->omega.p, chunks, nch, 0, g->der.p, g->count.p, g->pairs.p,
-                                 g->pdiff.p, g->coef.p, g->one.p, g->partial.p, g->sh, g->sp, g->stream));
+            WN_HIP(wn_launch_shg(xb, eb, g->omega.p, chunks, 0, cnt, g->der.p, g->count.p, g->pairs.p, g->pdiff.p,
+                                 g->coef.p, g->one.p, g->partial.p, g->sh, g->sp, s1));
+            mark(bi, 5, s1);
+            WN_HIP(wn_launch_shg(xb, eb, g->omega.p, chunks, nch, 0, g->der.p, g->count.p, g->pairs.p, g->pdiff.p,
+                                 g->coef.p, g->one.p, g->partial.p, g->sh, g->sp, s1));
         } else {
-            WN_HIP(wn_launch_geom_f64(g->x.p, g->e.p, g->omega.p, chunks, nch, g->partial.p, g->flags.p, k0, g->sh,
-                                      g->gp, g->stream));
-            mark(bi, 5);
+            WN_HIP(wn_launch_geom_f64(xb, eb, g->omega.p, chunks, nch, g->partial.p, g->flags.p, k0, g->sh, g->gp, s1));
+            mark(bi, 5, s1);
         }
-        WN_HIP(wn_launch_lane_reduce(p.dlanes.p + p.blane[bi], nln, g->partial.p, g->lanebuf.p, comp, nelem,
-                                     g->stream));
-        mark(bi, 6);
+        WN_HIP(wn_launch_lane_reduce(p.dlanes.p + p.blane[bi], nln, g->partial.p, g->lanebuf.p, comp, nelem, s1));
+        mark(bi, 6, s1);
+        WN_HIP(hipEventRecord(g->ev_free[buf], s1));
     }
+    // join
+    WN_HIP(hipEventRecord(g->ev_join, s1));
+    WN_HIP(hipStreamWaitEvent(s0, g->ev_join, 0));
     return WNLQG_OK;
 }
 
@@ -1022,7 +1051,7 @@ int32_t wnlqg_run_mesh(wnlqg_engine *g, const int32_t mesh[3], int32_t n_lanes, 
     int rc = plan_mesh(g, mesh, n_lanes);
     if (rc) return rc;
     const long long nbatch = (long long)g->plan.batches.size();
-    std::vector<std::array<hipEvent_t, 7>> marks(nbatch);
+    std::vector<std::array<hipEvent_t, 8>> marks(nbatch);
     bool events_ok = true;
     for (auto &mk : marks)
         for (auto &e : mk) {
@@ -1048,7 +1077,7 @@ int32_t wnlqg_run_mesh(wnlqg_engine *g, const int32_t mesh[3], int32_t n_lanes, 
             auto &mk = marks[bi];
             hipEventElapsedTime(&ms, mk[1], mk[2]); st.interp_ms += ms;
             hipEventElapsedTime(&ms, mk[2], mk[3]); st.eig_ms += ms;
-            hipEventElapsedTime(&ms, mk[3], mk[4]); st.rotate_ms += ms;
+            hipEventElapsedTime(&ms, mk[7], mk[4]); st.rotate_ms += ms;
             hipEventElapsedTime(&ms, mk[4], mk[5]); st.pairs_ms += ms;
             hipEventElapsedTime(&ms, mk[5], mk[6]); st.accum_ms += ms;
         }
