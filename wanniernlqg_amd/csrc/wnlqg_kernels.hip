@@ -787,7 +787,7 @@ hipError_t wn_launch_contract(const double2 *in, double2 *out, const double2 *E,
 }
 
 // ------------------------------------------------------------------------------------------------
-// Jacobi eigensolver: one wave64 workgroup per k point, H and V in LDS (n <= 64).
+// Jacobi eigensolver: one workgroup (64, 128 or 256 threads) per k point, H and V in LDS (n <= 64).
 //
 // The upper triangle of channel 0 defines the Hermitian matrix (LAPACK uplo = 'U' semantics;
 // the imaginary part of the diagonal is ignored).  Parallel cyclic Jacobi with a round-robin
@@ -813,25 +813,22 @@ __device__ __forceinline__ double wn_rcp(double x) {
     return y;
 }
 
-// Block form of the parallel Jacobi round on the full Hermitian matrix in LDS (both triangles, padded
-// rows).  A round with disjoint planes t = (p_t, q_t) maps every 2x2 block B = H[{p1,q1}][{p2,q2}],
-// t1 < t2, to J1^+ B J2 and writes it and its conjugate transpose; the diagonal blocks become
-// diag(a - t b, d + t b) exactly (written by the thread that computes the rotation).  Each thread owns
-// a fixed off-diagonal block (decoded once); V <- V J runs in the same phase, preferably on the threads
-// that own no block, so that every wave issues a similar number of instructions.
+// Block form of the parallel Jacobi round.  Only the upper triangle (and diagonal) of H is stored.
+// A round with disjoint planes t = (p_t, q_t) maps every 2x2 block B = H[{p1,q1}][{p2,q2}] to
+// J1^+ B J2, which reads and writes each stored element once; the diagonal blocks become
+// diag(a - t b, d + t b) exactly.  V <- V J runs in the same phase.  Element (i, j) with i > j is
+// read as conj(H[j][i]).
 template <int N, int TPB>
 __global__ __launch_bounds__(TPB) void k_eig_f64(const double *__restrict__ x, double *__restrict__ eout,
                                                 double2 *__restrict__ uout, int *__restrict__ sweeps_out, int n,
                                                 int ldo, double tol) {
     constexpr int LD = N + 1;
     constexpr int HALF = N / 2;
-    constexpr int NOFF = HALF * (HALF - 1) / 2;     // off-diagonal blocks t1 < t2
-    constexpr int NBT = (NOFF + TPB - 1) / TPB;     // blocks per thread
+    constexpr int NBLK = HALF * (HALF + 1) / 2;   // blocks t1 <= t2
     constexpr int VITEMS = N * HALF;
     constexpr int NW = TPB / 64;
-    // V items: on the threads without a block when there are enough of them
-    constexpr bool VSPLIT = NOFF < TPB && (TPB - NOFF) * 6 >= VITEMS;
-    __shared__ double2 H[N * LD];
+    constexpr int NP = N * (N + 1) / 2;
+    __shared__ double2 H[NP];       // packed upper triangle, row-major
     __shared__ double2 V[N * LD];
     __shared__ double4 rot[HALF];   // (c, s, cos phi, sin phi)
     __shared__ int2 pq[HALF];
@@ -841,30 +838,19 @@ __global__ __launch_bounds__(TPB) void k_eig_f64(const double *__restrict__ x, d
     const int k = blockIdx.x;
     const int t = threadIdx.x;
 
+    auto P = [](int i, int j) { return i * (2 * N - i + 1) / 2 + (j - i); };
     const double2 *xk = (const double2 *)(x + (size_t)k * ldo);
     for (int e = t; e < N * N; e += TPB) {
         const int i = e / N, j = e - (e / N) * N;
-        double2 hv = make_double2(0.0, 0.0);
-        if (i < n && j < n) {
-            // upper triangle defines the matrix (uplo = 'U'); the lower triangle mirrors it
-            hv = i <= j ? xk[i * n + j] : cconj(xk[j * n + i]);
-            if (i == j) hv.y = 0.0;
+        if (i <= j) {
+            double2 hv = make_double2(0.0, 0.0);
+            if (j < n) {
+                hv = xk[i * n + j];
+                if (i == j) hv.y = 0.0;
+            }
+            H[P(i, j)] = hv;
         }
-        H[i * LD + j] = hv;
         V[i * LD + j] = make_double2(i == j ? 1.0 : 0.0, 0.0);
-    }
-    // this thread's blocks (t1, t2), row-major over t1 < t2
-    int bt1[NBT], bt2[NBT];
-#pragma unroll
-    for (int u = 0; u < NBT; ++u) {
-        const int bidx = t + u * TPB;
-        int t1 = 0, rem = bidx;
-        while (t1 < HALF - 1 && rem >= HALF - 1 - t1) {
-            rem -= HALF - 1 - t1;
-            ++t1;
-        }
-        bt1[u] = bidx < NOFF ? t1 : -1;
-        bt2[u] = t1 + 1 + rem;
     }
     __syncthreads();
 
@@ -874,12 +860,13 @@ __global__ __launch_bounds__(TPB) void k_eig_f64(const double *__restrict__ x, d
         double off = 0.0, dia = 0.0;
         for (int e = t; e < N * N; e += TPB) {
             const int i = e / N, j = e - (e / N) * N;
-            const double2 hv = H[i * LD + j];
+            if (i > j) continue;
+            const double2 hv = H[P(i, j)];
             const double m2 = hv.x * hv.x + hv.y * hv.y;
             if (i == j)
                 dia += m2;
             else
-                off += m2;
+                off += 2.0 * m2;
         }
         for (int o = 32; o > 0; o >>= 1) {
             off += __shfl_xor(off, o);
@@ -915,13 +902,12 @@ __global__ __launch_bounds__(TPB) void k_eig_f64(const double *__restrict__ x, d
                 }
                 double c = 1.0, s = 0.0, cph = 1.0, sph = 0.0;
                 if (q < n) {
-                    const double a = H[p * LD + p].x, d = H[q * LD + q].x;
-                    const double2 h = H[p * LD + q];
+                    const double a = H[P(p, p)].x, d = H[P(q, q)].x;
+                    const double2 h = H[P(p, q)];
                     const double b2 = h.x * h.x + h.y * h.y;
                     if (b2 > 0.0) {
                         // t = tan(theta) = sign(d-a) 2b / (|d-a| + sqrt((d-a)^2 + 4b^2)), the smaller root of
-                        // t^2 + 2 tau t - 1 = 0 with tau = (d-a)/(2b); reciprocal square roots and
-                        // reciprocals from the hardware estimates refined by two Newton steps
+                        // t^2 + 2 tau t - 1 = 0 with tau = (d-a)/(2b)
                         const double ib = wn_rsqrt(b2);
                         const double b = b2 * ib;
                         const double diff = d - a;
@@ -932,68 +918,76 @@ __global__ __launch_bounds__(TPB) void k_eig_f64(const double *__restrict__ x, d
                         s = tt * c;
                         cph = h.x * ib;
                         sph = h.y * ib;
-                        H[p * LD + p] = make_double2(a - tt * b, 0.0);
-                        H[q * LD + q] = make_double2(d + tt * b, 0.0);
-                        H[p * LD + q] = make_double2(0.0, 0.0);
-                        H[q * LD + p] = make_double2(0.0, 0.0);
+                        H[P(p, p)] = make_double2(a - tt * b, 0.0);
+                        H[P(q, q)] = make_double2(d + tt * b, 0.0);
+                        H[P(p, q)] = make_double2(0.0, 0.0);
                     }
                 }
                 rot[t] = make_double4(c, s, cph, sph);
                 pq[t] = make_int2(p, q);
             }
             __syncthreads();
-#pragma unroll
-            for (int u = 0; u < NBT; ++u) {
-                const int t1 = bt1[u], t2 = bt2[u];
-                if (t1 < 0) continue;
+            // off-diagonal blocks t1 < t2 of H
+            for (int bidx = t; bidx < NBLK; bidx += TPB) {
+                // decode bidx -> (t1, t2) with t1 <= t2 (row-major over the upper triangle of blocks)
+                int t1 = 0, rem = bidx;
+                while (rem >= HALF - t1) {
+                    rem -= HALF - t1;
+                    ++t1;
+                }
+                const int t2 = t1 + rem;
+                if (t2 == t1) continue;
                 const double4 R1 = rot[t1], R2 = rot[t2];
                 if (R1.y == 0.0 && R2.y == 0.0) continue;
                 const int2 a1 = pq[t1], a2 = pq[t2];
-                const int r0 = a1.x * LD, r1 = a1.y * LD;
-                double2 B00 = H[r0 + a2.x], B01 = H[r0 + a2.y], B10 = H[r1 + a2.x], B11 = H[r1 + a2.y];
+                const int rr[2] = {a1.x, a1.y}, cc[2] = {a2.x, a2.y};
+                double2 B[2][2];
+#pragma unroll
+                for (int u = 0; u < 2; ++u)
+#pragma unroll
+                    for (int v = 0; v < 2; ++v) {
+                        const int i = rr[u], j = cc[v];
+                        B[u][v] = i < j ? H[P(i, j)] : cconj(H[P(j, i)]);
+                    }
                 // B J2: col0 = c2 B0 - s2 e^{-i phi2} B1, col1 = s2 B0 + c2 e^{-i phi2} B1
                 const double2 em2 = make_double2(R2.z, -R2.w);
-                {
-                    const double2 y0 = cmul(em2, B01), y1 = cmul(em2, B11);
-                    const double2 x0 = B00, x1 = B10;
-                    B00 = make_double2(R2.x * x0.x - R2.y * y0.x, R2.x * x0.y - R2.y * y0.y);
-                    B01 = make_double2(R2.y * x0.x + R2.x * y0.x, R2.y * x0.y + R2.x * y0.y);
-                    B10 = make_double2(R2.x * x1.x - R2.y * y1.x, R2.x * x1.y - R2.y * y1.y);
-                    B11 = make_double2(R2.y * x1.x + R2.x * y1.x, R2.y * x1.y + R2.x * y1.y);
+#pragma unroll
+                for (int u = 0; u < 2; ++u) {
+                    const double2 y = cmul(em2, B[u][1]);
+                    const double2 x0 = B[u][0];
+                    B[u][0] = make_double2(R2.x * x0.x - R2.y * y.x, R2.x * x0.y - R2.y * y.y);
+                    B[u][1] = make_double2(R2.y * x0.x + R2.x * y.x, R2.y * x0.y + R2.x * y.y);
                 }
                 // J1^+ M: row0 = c1 M0 - s1 e^{i phi1} M1, row1 = s1 M0 + c1 e^{i phi1} M1
                 const double2 ep1 = make_double2(R1.z, R1.w);
-                {
-                    const double2 y0 = cmul(ep1, B10), y1 = cmul(ep1, B11);
-                    const double2 x0 = B00, x1 = B01;
-                    B00 = make_double2(R1.x * x0.x - R1.y * y0.x, R1.x * x0.y - R1.y * y0.y);
-                    B10 = make_double2(R1.y * x0.x + R1.x * y0.x, R1.y * x0.y + R1.x * y0.y);
-                    B01 = make_double2(R1.x * x1.x - R1.y * y1.x, R1.x * x1.y - R1.y * y1.y);
-                    B11 = make_double2(R1.y * x1.x + R1.x * y1.x, R1.y * x1.y + R1.x * y1.y);
+#pragma unroll
+                for (int v = 0; v < 2; ++v) {
+                    const double2 y = cmul(ep1, B[1][v]);
+                    const double2 x0 = B[0][v];
+                    B[0][v] = make_double2(R1.x * x0.x - R1.y * y.x, R1.x * x0.y - R1.y * y.y);
+                    B[1][v] = make_double2(R1.y * x0.x + R1.x * y.x, R1.y * x0.y + R1.x * y.y);
                 }
-                H[r0 + a2.x] = B00;
-                H[r0 + a2.y] = B01;
-                H[r1 + a2.x] = B10;
-                H[r1 + a2.y] = B11;
-                H[a2.x * LD + a1.x] = cconj(B00);
-                H[a2.y * LD + a1.x] = cconj(B01);
-                H[a2.x * LD + a1.y] = cconj(B10);
-                H[a2.y * LD + a1.y] = cconj(B11);
+#pragma unroll
+                for (int u = 0; u < 2; ++u)
+#pragma unroll
+                    for (int v = 0; v < 2; ++v) {
+                        const int i = rr[u], j = cc[v];
+                        if (i < j)
+                            H[P(i, j)] = B[u][v];
+                        else
+                            H[P(j, i)] = cconj(B[u][v]);
+                    }
             }
             // V <- V J
-            const int v0 = VSPLIT ? t - NOFF : t;
-            const int vstep = VSPLIT ? TPB - NOFF : TPB;
-            if (v0 >= 0) {
-                for (int e = v0; e < VITEMS; e += vstep) {
-                    const int pr = e / N, i = e - (e / N) * N;
-                    const double4 R = rot[pr];
-                    if (R.y == 0.0 || i >= n) continue;
-                    const int2 ij = pq[pr];
-                    const double2 xp = V[i * LD + ij.x], yq = V[i * LD + ij.y];
-                    const double2 ey = make_double2(R.z * yq.x + R.w * yq.y, R.z * yq.y - R.w * yq.x);
-                    V[i * LD + ij.x] = make_double2(R.x * xp.x - R.y * ey.x, R.x * xp.y - R.y * ey.y);
-                    V[i * LD + ij.y] = make_double2(R.y * xp.x + R.x * ey.x, R.y * xp.y + R.x * ey.y);
-                }
+            for (int e = t; e < VITEMS; e += TPB) {
+                const int pr = e / N, i = e - (e / N) * N;
+                const double4 R = rot[pr];
+                if (R.y == 0.0 || i >= n) continue;
+                const int2 ij = pq[pr];
+                const double2 xp = V[i * LD + ij.x], yq = V[i * LD + ij.y];
+                const double2 ey = make_double2(R.z * yq.x + R.w * yq.y, R.z * yq.y - R.w * yq.x);
+                V[i * LD + ij.x] = make_double2(R.x * xp.x - R.y * ey.x, R.x * xp.y - R.y * ey.y);
+                V[i * LD + ij.y] = make_double2(R.y * xp.x + R.x * ey.x, R.y * xp.y + R.x * ey.y);
             }
             __syncthreads();
         }
@@ -1001,7 +995,7 @@ __global__ __launch_bounds__(TPB) void k_eig_f64(const double *__restrict__ x, d
     // sort ascending (stable by index); canonical phase per eigenvector column (first largest |v_i|
     // made real and non-negative), written in sorted order
     __shared__ double2 phase[N];
-    if (t < n) ev[t] = H[t * LD + t].x;
+    if (t < n) ev[t] = H[P(t, t)].x;
     __syncthreads();
     if (t < n) {
         const double et = ev[t];
