@@ -406,6 +406,119 @@ __global__ __launch_bounds__(WN_SHG_TPB) void k_shg_accum(const int4 *__restrict
     }
 }
 
+// ------------------------------------------------------------------------------------------------
+// Frequency contraction on FP64 MFMA.  For one chunk and one resonance kind (d1, d2, dn) the
+// partial sums form a complex GEMM out[w][c] = sum_p R(dE_p, w) coef[p][c], written as a real GEMM
+// with K = 2 x pairs:  A[w][2p] = Re R, A[w][2p+1] = Im R;  B[2p][2c] = Re C, B[2p][2c+1] = Im C,
+// B[2p+1][2c] = -Im C, B[2p+1][2c+1] = Re C.  v_mfma_f64_16x16x4f64 lane l holds A[l%16][l/16] (the
+// resonance of frequency l%16 and pair l/32 of the step, part (l/16)%2, generated in registers) and
+// B[l/16][l%16] (loaded from the coefficient rows).  Workgroup (chunk, 64-frequency tile, kind);
+// wave w owns 16 frequencies and every 16-column tile of its kind's block.  Kind 3 is the
+// frequency-independent low block: plain sums over the chunk's pairs in pair order.
+// ------------------------------------------------------------------------------------------------
+#define WN_SHG_MAXT 18   // 16-column real tiles of the widest block (8 x 18 components x 2 / 16)
+
+__global__ __launch_bounds__(WN_SHG_TPB) void k_shg_mfma(const int4 *__restrict__ chunks, const int *__restrict__ count,
+                                                        const double *__restrict__ pdiff,
+                                                        const double2 *__restrict__ coef,
+                                                        const double2 *__restrict__ one,
+                                                        const double *__restrict__ omega, double2 *__restrict__ partial,
+                                                        int pcap, int nwt, WnShg sp) {
+    const int nc = sp.ncomp, ncol = 19 * nc, nw = sp.nw;
+    const int per_chunk = nwt * 4;
+    const int chunk = blockIdx.x / per_chunk;
+    const int rem = blockIdx.x - chunk * per_chunk;
+    const int wt = rem >> 2, kind = rem & 3;
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int4 ch = chunks[chunk];
+    double2 *o = partial + (size_t)chunk * ((size_t)ncol * nw + nc);
+    if (kind == 3) {
+        // low block (R = 1) and the one-band sums, once per chunk
+        if (wt != 0) return;
+        for (int c = threadIdx.x; c < 3 * nc; c += WN_SHG_TPB) {
+            double2 acc = make_double2(0.0, 0.0);
+            for (int k = ch.x; k < ch.x + ch.y; ++k) {
+                const size_t base = (size_t)k * pcap;
+                for (int p = 0; p < count[k]; ++p) acc = cadd(acc, coef[(base + p) * ncol + 16 * nc + c]);
+            }
+            for (int w = 0; w < nw; ++w) o[(size_t)(16 * nc + c) * nw + w] = acc;
+        }
+        if (threadIdx.x < nc) {
+            double sb = 0.0, ss = 0.0;
+            for (int k = ch.x; k < ch.x + ch.y; ++k) {
+                const double2 v = one[(size_t)k * nc + threadIdx.x];
+                sb += v.x;
+                ss += v.y;
+            }
+            o[(size_t)ncol * nw + threadIdx.x] = make_double2(sb, ss);
+        }
+        return;
+    }
+    const int cbeg = kind == 0 ? 0 : (kind == 1 ? 8 * nc : 14 * nc);
+    const int cwid = kind == 0 ? 8 * nc : (kind == 1 ? 6 * nc : 2 * nc);   // complex columns
+    const int ntile = (2 * cwid + 15) / 16;
+    const int lr = lane & 15, lk = lane >> 4;           // A row (frequency) / K row of this lane
+    const int w_a = wt * 64 + wave * 16 + lr;           // frequency of this lane's A element
+    const double om = w_a < nw ? omega[w_a] : 0.0;
+    const int pair_sel = lk >> 1, part = lk & 1;        // pair of the step and re / im row
+    f64x4 acc[WN_SHG_MAXT];
+#pragma unroll
+    for (int j = 0; j < WN_SHG_MAXT; ++j) acc[j] = (f64x4){0.0, 0.0, 0.0, 0.0};
+    const double cut = 27.3 * sp.width;                 // exp(-(x/w)^2) is 0 in FP64 beyond this
+    for (int k = ch.x; k < ch.x + ch.y; ++k) {
+        const int cnt = count[k];
+        const size_t base = (size_t)k * pcap;
+        for (int p0 = 0; p0 < cnt; p0 += 2) {
+            const int p = p0 + pair_sel;
+            const bool valid = p < cnt;
+            // A: resonance of (frequency w_a, pair p)
+            const double diff = valid ? pdiff[base + p] : 0.0;
+            const double xx = kind == 0 ? diff - om : (kind == 1 ? diff - 2.0 * om : -diff - om);
+            const double pv = xx / (xx * xx + sp.width * sp.width);
+            double dl;
+            if (sp.gaussian) {
+                const double y = xx / sp.width;
+                dl = fabs(xx) < cut ? exp(-y * y) / (sqrt(M_PI) * sp.width) : 0.0;
+            } else {
+                const double y = xx / sp.width;
+                dl = 1.0 / (M_PI * sp.width * (1.0 + y * y));
+            }
+            const double a = valid ? (part == 0 ? pv : M_PI * dl) : 0.0;
+            // B: coefficient rows of the step's two pairs
+            const double2 *row = coef + (base + (valid ? p : 0)) * ncol + cbeg;
+#pragma unroll
+            for (int j = 0; j < WN_SHG_MAXT; ++j) {
+                if (j < ntile) {
+                    const int cr = j * 16 + lr;                  // real column
+                    double b = 0.0;
+                    if (valid && cr < 2 * cwid) {
+                        const double2 cv = row[cr >> 1];
+                        b = part == 0 ? ((cr & 1) ? cv.y : cv.x) : ((cr & 1) ? cv.x : -cv.y);
+                    }
+                    acc[j] = __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, acc[j], 0, 0, 0);
+                }
+            }
+        }
+    }
+    // D reg q -> row (frequency) lk + 4q, column lr of each 16-column tile
+#pragma unroll
+    for (int j = 0; j < WN_SHG_MAXT; ++j) {
+        if (j < ntile) {
+            const int cr = j * 16 + lr;
+            if (cr < 2 * cwid) {
+#pragma unroll
+                for (int q = 0; q < 4; ++q) {
+                    const int w = wt * 64 + wave * 16 + lk + 4 * q;
+                    if (w < nw) {
+                        double *dst = (double *)(o + (size_t)(cbeg + (cr >> 1)) * nw + w);
+                        dst[cr & 1] = acc[j][q];
+                    }
+                }
+            }
+        }
+    }
+}
+
 hipError_t wn_launch_shg(const double *x, const double *e, const double *omega, const int4 *chunks, int nchunks,
                          int count_k, double2 *der, int *count, int2 *pairs, double *pdiff, double2 *coef,
                          double2 *one, double2 *partial, const WnShape &sh, const WnShg &sp, hipStream_t st) {
@@ -417,9 +530,8 @@ hipError_t wn_launch_shg(const double *x, const double *e, const double *omega, 
     }
     if (nchunks > 0) {
         const int nwt = (sp.nw + 63) / 64;
-        const int nct = (19 * sp.ncomp + 127) / 128;
-        hipLaunchKernelGGL(k_shg_accum, dim3(nchunks * nwt * nct), dim3(WN_SHG_TPB), 0, st, chunks, count, pdiff, coef,
-                           one, omega, partial, sh.n * sh.n, nwt, nct, sp);
+        hipLaunchKernelGGL(k_shg_mfma, dim3(nchunks * nwt * 4), dim3(WN_SHG_TPB), 0, st, chunks, count, pdiff, coef, one,
+                           omega, partial, sh.n * sh.n, nwt, sp);
     }
     return hipGetLastError();
 }
