@@ -10,6 +10,7 @@ mesh interpolation); the plain FP64 k x R contraction and the split-FP16 mixed c
 for speed and relative spectrum error; the CPU baseline times the NumPy FP64 oracle on a bounded k sample.
 
   python bench.py [--gpus 1] [--steps K] [--warmup W] [--precision mixed|fp64] [--backend auto|direct|planes]
+  python bench.py --form geometry|shg|batch|learned     (the other forms, one JSON line each)
 """
 
 from __future__ import annotations
@@ -79,11 +80,15 @@ def main():
     ap.add_argument("--no-compare", action="store_true", help="skip the plain-FP64 and mixed direct comparison runs")
     ap.add_argument("--cpu-sample", type=int, default=1024)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--form", default="headline", choices=["headline", "geometry", "shg", "batch", "learned"])
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}); the k-mesh would shard by "
               "deterministic lanes across GPUs, which this build does not launch.", file=sys.stderr)
         sys.exit(2)
+    if args.form != "headline":
+        FORMS[args.form](args)
+        return
 
     from wanniernlqg_amd import engine as E
     from wanniernlqg_amd.model import config_model
@@ -202,6 +207,153 @@ def main():
     if compare:
         out["compare"] = compare
     print(json.dumps(out), flush=True)
+
+
+def _timed_runs(eng, mesh, steps, warmup):
+    for _ in range(warmup):
+        eng.run_mesh(mesh)
+    eng.synchronize()
+    t0 = time.perf_counter()
+    st = None
+    for _ in range(steps):
+        _, tot, st = eng.run_mesh(mesh)
+    eng.synchronize()
+    return (time.perf_counter() - t0) / steps, tot, st
+
+
+def form_geometry(args):
+    """BCD (Fermi surface) + intrinsic AHC in one engine on the n_w = 32 model, 3 chemical potentials."""
+    from wanniernlqg_amd import engine as E
+    from wanniernlqg_amd.model import config_model
+    model = config_model(1)
+    mesh = (args.mesh,) * 3 if args.mesh != 128 else (96, 96, 96)
+    task = E.EngineTask(berry_curvature_dipole=True, intrinsic_hall=True, fermi_energies=[-0.5, 0.0, 0.5],
+                        temperature=0.0, fermi_surface_broadening=0.02, denominator_regularization=0.0,
+                        degeneracy_threshold=0.0)
+    with E.Engine(model, task) as eng:
+        wall, _, st = _timed_runs(eng, mesh, args.steps, args.warmup)
+    nk = int(np.prod(mesh))
+    out = {"metric": "k-points/s (BCD + intrinsic AHC, n_w=32)", "value": nk / wall, "unit": "k-points/s",
+           "n_gpus": 1, "steps": args.steps, "warmup": args.warmup, "ms_per_step": 1e3 * wall,
+           "higher_is_better": True, "dtype": "f64", "data": "synthetic (seeded TB model)",
+           "config": {"workload": f"BCD (Fermi surface, eta_fs 0.02 eV) + AHC at mu = -0.5, 0, 0.5 eV, T = 0, "
+                                  f"n_w=32, N_R=1331, {mesh[0]}^3 mesh", "kpoints": nk},
+           "stages_ms": {k: st[k] for k in ("interp_ms", "eig_ms", "rotate_ms", "pairs_ms", "accum_ms", "total_ms")}}
+    print(json.dumps(out), flush=True)
+
+
+def form_shg(args):
+    """SHG chi/sigma(-2w; w, w), 3 components, 400 w in [0, 3] eV, on the n_w = 32 model."""
+    from wanniernlqg_amd import engine as E
+    from wanniernlqg_amd.model import config_model
+    model = config_model(1)
+    mesh = (args.mesh,) * 3 if args.mesh != 128 else (48, 48, 48)
+    task = E.EngineTask(second_harmonic=True, photon_energies=np.linspace(0.0, 3.0, 400), broadening=0.02,
+                        shg_components=[(0, 1, 1), (1, 0, 2), (2, 2, 2)], fermi_energy=0.0, temperature=0.0,
+                        shg_intermediate_regularization=1e-3)
+    with E.Engine(model, task) as eng:
+        wall, _, st = _timed_runs(eng, mesh, args.steps, args.warmup)
+    nk = int(np.prod(mesh))
+    out = {"metric": "k-points/s (SHG, 3 components, 400 omega, n_w=32)", "value": nk / wall, "unit": "k-points/s",
+           "n_gpus": 1, "steps": args.steps, "warmup": args.warmup, "ms_per_step": 1e3 * wall,
+           "higher_is_better": True, "dtype": "f64", "data": "synthetic (seeded TB model)",
+           "config": {"workload": f"SHG xyy, yxz, zzz; 400 omega in [0,3] eV; Gaussian 0.02 eV; n_w=32, N_R=1331; "
+                                  f"{mesh[0]}^3 mesh", "kpoints": nk},
+           "stages_ms": {k: st[k] for k in ("interp_ms", "eig_ms", "rotate_ms", "pairs_ms", "accum_ms", "total_ms")}}
+    print(json.dumps(out), flush=True)
+
+
+def form_batch(args):
+    """Batching form: 64 models (n_w = 4 (s mod 12 + 1), |R_i| <= 4, seeds 0..63) with meshes drawn from
+    {16,32,48,64,96}^3, BCD + AHC at 3 chemical potentials, one captured graph, against the same
+    integrals run one engine after another."""
+    from wanniernlqg_amd import engine as E
+    from wanniernlqg_amd.model import config_model
+    rng = np.random.default_rng(2)
+    sizes = [16, 32, 48, 64, 96]
+    entries = []
+    for seed in range(64):
+        m = config_model(2, seed)
+        n = sizes[int(rng.integers(0, len(sizes)))]
+        task = E.EngineTask(berry_curvature_dipole=True, intrinsic_hall=True, fermi_energies=[-0.5, 0.0, 0.5],
+                            temperature=0.0, fermi_surface_broadening=0.02, denominator_regularization=0.0,
+                            degeneracy_threshold=0.0)
+        entries.append((m, task, (n, n, n)))
+    total_k = sum(int(np.prod(e[2])) for e in entries)
+    t0 = time.perf_counter()
+    b = E.Batch(entries, autotune=True)
+    setup = time.perf_counter() - t0
+    b.run()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        _, dev_ms = b.run()
+    wall = (time.perf_counter() - t0) / args.steps
+    tuning = b.tuning()
+    b.close()
+    # the same integrals one engine at a time (plain path)
+    t0 = time.perf_counter()
+    seq_dev = 0.0
+    for m, task, mesh in entries:
+        with E.Engine(m, task) as eng:
+            _, _, st = eng.run_mesh(mesh)
+            seq_dev += st["total_ms"]
+    seq_wall = time.perf_counter() - t0
+    out = {"metric": "k-points/s (batching form: 64 models, BCD + AHC)", "value": total_k / wall,
+           "unit": "k-points/s", "n_gpus": 1, "steps": args.steps, "ms_per_step": 1e3 * wall, "higher_is_better": True,
+           "dtype": "f64", "data": "synthetic (seeded TB models)",
+           "config": {"workload": "64 models n_w in {4..48}, |R_i|<=4, meshes {16..96}^3, BCD + AHC at 3 mu",
+                      "kpoints": total_k},
+           "graph_device_ms": dev_ms, "setup_s": setup, "eig_threads_autotuned": sorted(set(tuning)),
+           "sequential_engines": {"wall_s_including_setup": seq_wall, "device_ms": seq_dev,
+                                  "kpoints_per_s_device": total_k / (1e-3 * seq_dev)}}
+    print(json.dumps(out), flush=True)
+
+
+def form_learned(args):
+    """Learned refinement of the BCD on the hot-spot model (n_w = 16, |R_i| <= 4, four Weyl-like pairs with
+    1-5 meV gaps, mu = 0, T = 50 K): network trained on plain 32^3 and 48^3 runs ranks the 64^3 base cells,
+    top cells refined by 5; judged against the uniform 320^3 mesh."""
+    from wanniernlqg_amd import engine as E
+    from wanniernlqg_amd import refine
+    from wanniernlqg_amd.model import config_model
+    m = config_model(3)
+    n, r = 64, 5
+    task = E.EngineTask(berry_curvature_dipole=True, fermi_energies=[0.0], temperature=50.0,
+                        denominator_regularization=0.0, degeneracy_threshold=0.0)
+    with E.Engine(m, task) as eng:
+        lr = refine.LearnedRefinement(eng, seed=1)
+        t0 = time.perf_counter()
+        lr.train(meshes=(32, 48), steps=6000, batch=4096, lr=2e-3, seed=3)
+        train_s = time.perf_counter() - t0
+        t0 = time.perf_counter()
+        _, dense, _ = eng.run_mesh((n * r,) * 3)
+        dense_s = time.perf_counter() - t0
+        ref = dense["bcd"]
+        base = lr.point_moments(refine.mesh_points(n))
+        heur = np.argsort(-refine.local_log_variance(refine.point_scalar(base), n), kind="stable")
+        err = lambda v: float(np.linalg.norm(v - ref) / np.linalg.norm(ref))
+        rows = []
+        for frac in (0.01, 0.02, 0.05, 0.1, 0.2):
+            budget = int(n ** 3 * frac)
+            t0 = time.perf_counter()
+            le = lr.estimate(n, r, budget, base=base)
+            est_s = time.perf_counter() - t0
+            he = lr.estimate(n, r, budget, order=heur, base=base)
+            nu = int(np.ceil(le.evaluations ** (1 / 3)))
+            _, uni, _ = eng.run_mesh((nu,) * 3)
+            rows.append({"refined_fraction": frac, "evaluations": le.evaluations,
+                         "evaluations_vs_dense": (n * r) ** 3 / le.evaluations, "err_learned": err(le.estimate),
+                         "err_base_variance": err(he.estimate), "uniform_mesh": nu, "err_uniform": err(uni["bcd"]),
+                         "adaptive_s": est_s})
+    out = {"metric": "k evaluations for dense-mesh accuracy (learned adaptive BCD)", "unit": "ratio",
+           "higher_is_better": True, "n_gpus": 1, "dtype": "f64 integrand, f32 network", "data": "synthetic hot-spot model",
+           "config": {"workload": "BCD mu=0 T=50K, n_w=16, |R_i|<=4, 4 near-degenerate pairs (1-5 meV); base 64^3, "
+                                  "refinement 5, dense 320^3", "dense_kpoints": (n * r) ** 3},
+           "train_s": train_s, "dense_s": dense_s, "rows": rows}
+    print(json.dumps(out), flush=True)
+
+
+FORMS = {"geometry": form_geometry, "shg": form_shg, "batch": form_batch, "learned": form_learned}
 
 
 if __name__ == "__main__":

@@ -30,6 +30,8 @@
 
 #include <math.h>
 
+#include <algorithm>
+
 #define WN_SHG_TPB 256
 
 namespace {
@@ -414,28 +416,33 @@ __global__ __launch_bounds__(WN_SHG_TPB) void k_shg_accum(const int4 *__restrict
 // resonance of frequency l%16 and pair l/32 of the step, part (l/16)%2, generated in registers) and
 // B[l/16][l%16] (loaded from the coefficient rows).  Workgroup (chunk, 64-frequency tile, kind);
 // wave w owns 16 frequencies and every 16-column tile of its kind's block.  Kind 3 is the
-// frequency-independent low block: plain sums over the chunk's pairs in pair order.
+// frequency-independent low block: plain sums over the chunk's pairs in pair order.  Coefficient rows
+// are staged through LDS in blocks of `pb` pairs (all k points of the chunk in order).
 // ------------------------------------------------------------------------------------------------
-#define WN_SHG_MAXT 18   // 16-column real tiles of the widest block (8 x 18 components x 2 / 16)
-
+#define WN_SHG_MAXK 64   // k points per chunk
+template <int MAXT>
 __global__ __launch_bounds__(WN_SHG_TPB) void k_shg_mfma(const int4 *__restrict__ chunks, const int *__restrict__ count,
                                                         const double *__restrict__ pdiff,
                                                         const double2 *__restrict__ coef,
                                                         const double2 *__restrict__ one,
                                                         const double *__restrict__ omega, double2 *__restrict__ partial,
-                                                        int pcap, int nwt, WnShg sp) {
+                                                        int pcap, int nwt, int pb, WnShg sp) {
+    extern __shared__ double2 lds_coef[];      // [pb][cwid] coefficient block
+    __shared__ double lds_diff[128];
+    __shared__ int pre[WN_SHG_MAXK + 1];       // prefix of the pair counts of the chunk's k points
     const int nc = sp.ncomp, ncol = 19 * nc, nw = sp.nw;
     const int per_chunk = nwt * 4;
     const int chunk = blockIdx.x / per_chunk;
     const int rem = blockIdx.x - chunk * per_chunk;
     const int wt = rem >> 2, kind = rem & 3;
-    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int t = threadIdx.x;
+    const int wave = t >> 6, lane = t & 63;
     const int4 ch = chunks[chunk];
     double2 *o = partial + (size_t)chunk * ((size_t)ncol * nw + nc);
     if (kind == 3) {
         // low block (R = 1) and the one-band sums, once per chunk
         if (wt != 0) return;
-        for (int c = threadIdx.x; c < 3 * nc; c += WN_SHG_TPB) {
+        for (int c = t; c < 3 * nc; c += WN_SHG_TPB) {
             double2 acc = make_double2(0.0, 0.0);
             for (int k = ch.x; k < ch.x + ch.y; ++k) {
                 const size_t base = (size_t)k * pcap;
@@ -443,14 +450,14 @@ __global__ __launch_bounds__(WN_SHG_TPB) void k_shg_mfma(const int4 *__restrict_
             }
             for (int w = 0; w < nw; ++w) o[(size_t)(16 * nc + c) * nw + w] = acc;
         }
-        if (threadIdx.x < nc) {
+        if (t < nc) {
             double sb = 0.0, ss = 0.0;
             for (int k = ch.x; k < ch.x + ch.y; ++k) {
-                const double2 v = one[(size_t)k * nc + threadIdx.x];
+                const double2 v = one[(size_t)k * nc + t];
                 sb += v.x;
                 ss += v.y;
             }
-            o[(size_t)ncol * nw + threadIdx.x] = make_double2(sb, ss);
+            o[(size_t)ncol * nw + t] = make_double2(sb, ss);
         }
         return;
     }
@@ -461,18 +468,39 @@ __global__ __launch_bounds__(WN_SHG_TPB) void k_shg_mfma(const int4 *__restrict_
     const int w_a = wt * 64 + wave * 16 + lr;           // frequency of this lane's A element
     const double om = w_a < nw ? omega[w_a] : 0.0;
     const int pair_sel = lk >> 1, part = lk & 1;        // pair of the step and re / im row
-    f64x4 acc[WN_SHG_MAXT];
+    if (t == 0) {
+        pre[0] = 0;
+        for (int i = 0; i < ch.y; ++i) pre[i + 1] = pre[i] + count[ch.x + i];
+    }
+    __syncthreads();
+    const int total = pre[ch.y];
+    f64x4 acc[MAXT];
 #pragma unroll
-    for (int j = 0; j < WN_SHG_MAXT; ++j) acc[j] = (f64x4){0.0, 0.0, 0.0, 0.0};
+    for (int j = 0; j < MAXT; ++j) acc[j] = (f64x4){0.0, 0.0, 0.0, 0.0};
     const double cut = 27.3 * sp.width;                 // exp(-(x/w)^2) is 0 in FP64 beyond this
-    for (int k = ch.x; k < ch.x + ch.y; ++k) {
-        const int cnt = count[k];
-        const size_t base = (size_t)k * pcap;
-        for (int p0 = 0; p0 < cnt; p0 += 2) {
-            const int p = p0 + pair_sel;
-            const bool valid = p < cnt;
-            // A: resonance of (frequency w_a, pair p)
-            const double diff = valid ? pdiff[base + p] : 0.0;
+    for (int f0 = 0; f0 < total; f0 += pb) {
+        const int nb = min(pb, total - f0);
+        // cooperative load of the block's coefficient rows and energy differences (pairs in chunk order)
+        for (int e = t; e < nb * cwid; e += WN_SHG_TPB) {
+            const int fp = e / cwid, c = e - (e / cwid) * cwid;
+            const int f = f0 + fp;
+            int ki = 0;
+            while (pre[ki + 1] <= f) ++ki;
+            const size_t row = (size_t)(ch.x + ki) * pcap + (f - pre[ki]);
+            lds_coef[fp * cwid + c] = coef[row * ncol + cbeg + c];
+        }
+        for (int e = t; e < nb; e += WN_SHG_TPB) {
+            const int f = f0 + e;
+            int ki = 0;
+            while (pre[ki + 1] <= f) ++ki;
+            lds_diff[e] = pdiff[(size_t)(ch.x + ki) * pcap + (f - pre[ki])];
+        }
+        __syncthreads();
+        for (int s0 = 0; s0 < nb; s0 += 2) {
+            const int pl = s0 + pair_sel;
+            const bool valid = pl < nb;
+            // A: resonance of (frequency w_a, pair pl)
+            const double diff = valid ? lds_diff[pl] : 0.0;
             const double xx = kind == 0 ? diff - om : (kind == 1 ? diff - 2.0 * om : -diff - om);
             const double pv = xx / (xx * xx + sp.width * sp.width);
             double dl;
@@ -484,10 +512,9 @@ __global__ __launch_bounds__(WN_SHG_TPB) void k_shg_mfma(const int4 *__restrict_
                 dl = 1.0 / (M_PI * sp.width * (1.0 + y * y));
             }
             const double a = valid ? (part == 0 ? pv : M_PI * dl) : 0.0;
-            // B: coefficient rows of the step's two pairs
-            const double2 *row = coef + (base + (valid ? p : 0)) * ncol + cbeg;
+            const double2 *row = lds_coef + (valid ? pl : 0) * cwid;
 #pragma unroll
-            for (int j = 0; j < WN_SHG_MAXT; ++j) {
+            for (int j = 0; j < MAXT; ++j) {
                 if (j < ntile) {
                     const int cr = j * 16 + lr;                  // real column
                     double b = 0.0;
@@ -499,10 +526,11 @@ __global__ __launch_bounds__(WN_SHG_TPB) void k_shg_mfma(const int4 *__restrict_
                 }
             }
         }
+        __syncthreads();
     }
     // D reg q -> row (frequency) lk + 4q, column lr of each 16-column tile
 #pragma unroll
-    for (int j = 0; j < WN_SHG_MAXT; ++j) {
+    for (int j = 0; j < MAXT; ++j) {
         if (j < ntile) {
             const int cr = j * 16 + lr;
             if (cr < 2 * cwid) {
@@ -530,8 +558,20 @@ hipError_t wn_launch_shg(const double *x, const double *e, const double *omega, 
     }
     if (nchunks > 0) {
         const int nwt = (sp.nw + 63) / 64;
-        hipLaunchKernelGGL(k_shg_mfma, dim3(nchunks * nwt * 4), dim3(WN_SHG_TPB), 0, st, chunks, count, pdiff, coef, one,
-                           omega, partial, sh.n * sh.n, nwt, sp);
+        // pairs per LDS block: the widest block (8 nc complex columns) times pb within 32 KB, even, <= 128
+        const int pb = std::max(2, std::min(128, (2048 / (8 * sp.ncomp)) & ~1));
+        const size_t lds = (size_t)pb * 8 * sp.ncomp * sizeof(double2);
+        const dim3 grid(nchunks * nwt * 4);
+#define WN_SHG_LAUNCH(T)                                                                                          \
+    hipLaunchKernelGGL((k_shg_mfma<T>), grid, dim3(WN_SHG_TPB), lds, st, chunks, count, pdiff, coef, one, omega, \
+                       partial, sh.n * sh.n, nwt, pb, sp)
+        if (sp.ncomp <= 4)
+            WN_SHG_LAUNCH(4);
+        else if (sp.ncomp <= 8)
+            WN_SHG_LAUNCH(8);
+        else
+            WN_SHG_LAUNCH(18);
+#undef WN_SHG_LAUNCH
     }
     return hipGetLastError();
 }
