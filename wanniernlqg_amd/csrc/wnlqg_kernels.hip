@@ -1146,15 +1146,16 @@ __global__ __launch_bounds__(256) void k_rotate_f64(double *__restrict__ x, cons
 //   Y = T U:    Yr = Tr Ur - Ti Ui,      Yi = Tr Ui + Ti Ur
 // with U, X (then T) in LDS as planar real arrays padded to np = 16*ceil(n/16) (zeros beyond n).
 // The (np/16)^2 output tiles are dealt to the 4 waves.
+template <int NP, int PF>
 __global__ __launch_bounds__(256) void k_rotate_mfma(double *__restrict__ x, const double2 *__restrict__ u, int n,
                                                     int C, int ldo) {
     extern __shared__ double smd[];
     const int k = blockIdx.x;
     const int t = threadIdx.x;
     const int lane = t & 63, wave = t >> 6;
-    const int np = (n + 15) & ~15;
-    const int ld = np + 1;
-    const int nt = np >> 4;
+    constexpr int np = NP;     // n rounded up to 16 (compile time: the k loops unroll)
+    constexpr int ld = np + 1;
+    constexpr int nt = np >> 4;
     double *Ur = smd;
     double *Ui = Ur + np * ld;
     double *Xr = Ui + np * ld;
@@ -1168,14 +1169,29 @@ __global__ __launch_bounds__(256) void k_rotate_mfma(double *__restrict__ x, con
     }
     double2 *xk = (double2 *)(x + (size_t)k * ldo);
     const int lr = lane & 15, lq = lane >> 4;
+    // channel c + 1 is loaded into registers while channel c's second product runs (PF = np^2 / 256)
+    double2 pf[PF];
+    auto prefetch = [&](int c) {
+        const double2 *xc = xk + (size_t)c * n * n;
+#pragma unroll
+        for (int q = 0; q < PF; ++q) {
+            const int e = t + 256 * q;
+            const int i = e / np, j = e - (e / np) * np;
+            pf[q] = (e < np * np && i < n && j < n) ? xc[i * n + j] : make_double2(0.0, 0.0);
+        }
+    };
+    if (C > 1) prefetch(1);
     for (int c = 1; c < C; ++c) {
         double2 *xc = xk + (size_t)c * n * n;
         __syncthreads();
-        for (int e = t; e < np * np; e += 256) {
-            int i = e / np, j = e - i * np;
-            double2 v = (i < n && j < n) ? xc[i * n + j] : make_double2(0.0, 0.0);
-            Xr[i * ld + j] = v.x;
-            Xi[i * ld + j] = v.y;
+#pragma unroll
+        for (int q = 0; q < PF; ++q) {
+            const int e = t + 256 * q;
+            if (e < np * np) {
+                const int i = e / np, j = e - (e / np) * np;
+                Xr[i * ld + j] = pf[q].x;
+                Xi[i * ld + j] = pf[q].y;
+            }
         }
         __syncthreads();
         // T = U^+ X, tiles kept in registers (at most 4 per wave for np <= 64)
@@ -1186,6 +1202,7 @@ __global__ __launch_bounds__(256) void k_rotate_mfma(double *__restrict__ x, con
             f64x4 ar = {0.0, 0.0, 0.0, 0.0}, ai = {0.0, 0.0, 0.0, 0.0};
             if (tile < nt * nt) {
                 const int r0 = (tile / nt) * 16, c0 = (tile % nt) * 16;
+#pragma unroll
                 for (int k0 = 0; k0 < np; k0 += 4) {
                     const int kr = k0 + lq;
                     const double ur = Ur[kr * ld + r0 + lr], ui = Ui[kr * ld + r0 + lr];
@@ -1212,11 +1229,13 @@ __global__ __launch_bounds__(256) void k_rotate_mfma(double *__restrict__ x, con
                 }
             }
         }
+        if (c + 1 < C) prefetch(c + 1);
         __syncthreads();
         // Y = T U, written to global
         for (int tile = wave; tile < nt * nt; tile += 4) {
             const int r0 = (tile / nt) * 16, c0 = (tile % nt) * 16;
             f64x4 yr = {0.0, 0.0, 0.0, 0.0}, yi = {0.0, 0.0, 0.0, 0.0};
+#pragma unroll
             for (int k0 = 0; k0 < np; k0 += 4) {
                 const int kc = k0 + lq;
                 const double tr_ = Xr[(r0 + lr) * ld + kc], ti_ = Xi[(r0 + lr) * ld + kc];
@@ -1240,7 +1259,14 @@ hipError_t wn_launch_rotate_f64(double *x, const double2 *u, int count, const Wn
     if (n >= 16) {
         const int np = (n + 15) & ~15;
         size_t lds = (size_t)4 * np * (np + 1) * sizeof(double);
-        hipLaunchKernelGGL(k_rotate_mfma, dim3(count), dim3(256), lds, st, x, u, n, sh.C, sh.ldo);
+        if (np == 16)
+            hipLaunchKernelGGL((k_rotate_mfma<16, 1>), dim3(count), dim3(256), lds, st, x, u, n, sh.C, sh.ldo);
+        else if (np == 32)
+            hipLaunchKernelGGL((k_rotate_mfma<32, 4>), dim3(count), dim3(256), lds, st, x, u, n, sh.C, sh.ldo);
+        else if (np == 48)
+            hipLaunchKernelGGL((k_rotate_mfma<48, 9>), dim3(count), dim3(256), lds, st, x, u, n, sh.C, sh.ldo);
+        else
+            hipLaunchKernelGGL((k_rotate_mfma<64, 16>), dim3(count), dim3(256), lds, st, x, u, n, sh.C, sh.ldo);
     } else {
         size_t lds = (size_t)2 * n * (n + 1) * sizeof(double2);
         hipLaunchKernelGGL(k_rotate_f64, dim3(count), dim3(256), lds, st, x, u, n, sh.C, sh.ldo);
