@@ -941,7 +941,7 @@ int enqueue_mesh(wnlqg_engine *g, std::vector<std::array<hipEvent_t, 8>> *marks,
             WN_HIP(launch_interp(g, rows, xb));
         }
         mark(bi, 2, s0);
-        WN_HIP(wn_launch_eig_f64(xb, eb, ub, g->flags.p + 1, cnt, g->sh, g->eig_tpb, s0));
+        WN_HIP(wn_launch_eig_mesh_f64(xb, eb, ub, g->flags.p + 1, cnt, g->sh, g->eig_tpb, s0));
         mark(bi, 3, s0);
         WN_HIP(hipEventRecord(g->ev_ready[buf], s0));
         // stream 1: rotation, response and k sums of batch bi

@@ -99,6 +99,10 @@ hipError_t wn_launch_contract(const double2 *in, double2 *out, const double2 *E,
 // tpb: threads per k point (64, 128, 256; 0 = default of the size class).
 hipError_t wn_launch_eig_f64(const double *x, double *e, double2 *u, int *sweeps, int count, const WnShape &sh,
                              int tpb, hipStream_t st);
+// The same for count consecutive points of a mesh row: anchors of groups of 4 first, the other points
+// started from their anchor's eigenvectors (channel 0 of those points is overwritten by U_a^+ H U_a).
+hipError_t wn_launch_eig_mesh_f64(double *x, double *e, double2 *u, int *sweeps, int count, const WnShape &sh,
+                                  int tpb, hipStream_t st);
 // Clears that are safe under stream capture: p[0 .. n) = 0; flags = {0, 0, 0, 0x7f7f7f7f}.
 hipError_t wn_launch_zero(double *p, long long n, hipStream_t st);
 hipError_t wn_launch_reset_flags(int *f, hipStream_t st);

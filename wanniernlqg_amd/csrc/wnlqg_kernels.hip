@@ -818,10 +818,32 @@ __device__ __forceinline__ double wn_rcp(double x) {
 // J1^+ B J2, which reads and writes each stored element once; the diagonal blocks become
 // diag(a - t b, d + t b) exactly.  V <- V J runs in the same phase.  Element (i, j) with i > j is
 // read as conj(H[j][i]).
+// Warm-started mesh eigensolver.  Consecutive k points (neighbours along the fastest mesh axis) form
+// groups of WN_WARM_G; the anchor of a group (position WN_WARM_A, or the last point of a short group)
+// is solved from the identity (pass 0).  Every other point k of the group is then solved from the
+// anchor's eigenvectors U_a (pass 1): its channel 0 is first replaced by U_a^+ H(k) U_a, which is
+// diagonal up to O(|k - k_a|), and V starts at U_a, so that V J1 J2 ... are eigenvectors of H(k).
+// The stopping rule is unchanged; a nearby start only removes sweeps.
+#define WN_WARM_G 4
+#define WN_WARM_A 1
+__device__ __forceinline__ int wn_warm_k(int b, int pass, int count, int *anchor) {
+    if (pass == 0) {
+        const int base = b * WN_WARM_G;
+        *anchor = -1;
+        return base + min(WN_WARM_A, count - 1 - base);
+    }
+    const int g = b / (WN_WARM_G - 1), j = b - g * (WN_WARM_G - 1);
+    const int base = g * WN_WARM_G;
+    const int al = min(WN_WARM_A, count - 1 - base);
+    *anchor = base + al;
+    return base + (j < al ? j : j + 1);
+}
+
+// pass < 0: every k point from the identity (grid = count); pass 0 / 1: the warm-start passes above
 template <int N, int TPB>
 __global__ __launch_bounds__(TPB) void k_eig_f64(const double *__restrict__ x, double *__restrict__ eout,
                                                 double2 *__restrict__ uout, int *__restrict__ sweeps_out, int n,
-                                                int ldo, double tol) {
+                                                int ldo, double tol, int pass, int count) {
     constexpr int LD = N + 1;
     constexpr int HALF = N / 2;
     constexpr int NBLK = HALF * (HALF + 1) / 2;   // blocks t1 <= t2
@@ -835,11 +857,14 @@ __global__ __launch_bounds__(TPB) void k_eig_f64(const double *__restrict__ x, d
     __shared__ double red[2 * NW];
     __shared__ double ev[N];
     __shared__ int rank[N];
-    const int k = blockIdx.x;
+    int anchor = -1;
+    const int k = pass < 0 ? (int)blockIdx.x : wn_warm_k(blockIdx.x, pass, count, &anchor);
+    if (k >= count) return;
     const int t = threadIdx.x;
 
     auto P = [](int i, int j) { return i * (2 * N - i + 1) / 2 + (j - i); };
     const double2 *xk = (const double2 *)(x + (size_t)k * ldo);
+    const double2 *ua = anchor >= 0 ? uout + (size_t)anchor * n * n : nullptr;
     for (int e = t; e < N * N; e += TPB) {
         const int i = e / N, j = e - (e / N) * N;
         if (i <= j) {
@@ -850,7 +875,7 @@ __global__ __launch_bounds__(TPB) void k_eig_f64(const double *__restrict__ x, d
             }
             H[P(i, j)] = hv;
         }
-        V[i * LD + j] = make_double2(i == j ? 1.0 : 0.0, 0.0);
+        V[i * LD + j] = (ua && i < n && j < n) ? ua[i * n + j] : make_double2(i == j ? 1.0 : 0.0, 0.0);
     }
     __syncthreads();
 
@@ -1032,13 +1057,15 @@ __global__ __launch_bounds__(TPB) void k_eig_f64(const double *__restrict__ x, d
     if (t == 0 && sweeps_out) atomicMax(sweeps_out, sweep);
 }
 
-hipError_t wn_launch_eig_f64(const double *x, double *e, double2 *u, int *sweeps, int count, const WnShape &sh,
-                             int tpb, hipStream_t st) {
+static hipError_t eig_pass(const double *x, double *e, double2 *u, int *sweeps, int count, const WnShape &sh, int tpb,
+                           int pass, hipStream_t st) {
     const int n = sh.n;
     const double tol = 1.0e-15;
+    const int groups = (count + WN_WARM_G - 1) / WN_WARM_G;
+    const int grid = pass < 0 ? count : pass == 0 ? groups : groups * (WN_WARM_G - 1);
     // tpb: threads per k point (64, 128 or 256); 0 selects the default of the size class
 #define WN_EIG_LAUNCH(NN, T) \
-    hipLaunchKernelGGL((k_eig_f64<NN, T>), dim3(count), dim3(T), 0, st, x, e, u, sweeps, n, sh.ldo, tol)
+    hipLaunchKernelGGL((k_eig_f64<NN, T>), dim3(grid), dim3(T), 0, st, x, e, u, sweeps, n, sh.ldo, tol, pass, count)
 #define WN_EIG_CASE(NN, DEF)                   \
     if (n <= NN) {                             \
         const int t = tpb ? tpb : DEF;         \
@@ -1146,11 +1173,16 @@ __global__ __launch_bounds__(256) void k_rotate_f64(double *__restrict__ x, cons
 //   Y = T U:    Yr = Tr Ur - Ti Ui,      Yi = Tr Ui + Ti Ur
 // with U, X (then T) in LDS as planar real arrays padded to np = 16*ceil(n/16) (zeros beyond n).
 // The (np/16)^2 output tiles are dealt to the 4 waves.
-template <int NP, int PF>
+// WARM: the warm-start transform of the mesh eigensolver: channel 0 of the pass-1 k points replaced by
+// U_a^+ H U_a with the group anchor's eigenvectors U_a.  Otherwise channels 1 .. C-1 of every k point.
+template <int NP, int PF, bool WARM>
 __global__ __launch_bounds__(256) void k_rotate_mfma(double *__restrict__ x, const double2 *__restrict__ u, int n,
-                                                    int C, int ldo) {
+                                                    int C, int ldo, int count) {
     extern __shared__ double smd[];
-    const int k = blockIdx.x;
+    int ka = blockIdx.x;
+    const int k = WARM ? wn_warm_k(blockIdx.x, 1, count, &ka) : (int)blockIdx.x;
+    if (WARM && k >= count) return;
+    const int cbeg = WARM ? 0 : 1, cend = WARM ? 1 : C;
     const int t = threadIdx.x;
     const int lane = t & 63, wave = t >> 6;
     constexpr int np = NP;     // n rounded up to 16 (compile time: the k loops unroll)
@@ -1160,7 +1192,7 @@ __global__ __launch_bounds__(256) void k_rotate_mfma(double *__restrict__ x, con
     double *Ui = Ur + np * ld;
     double *Xr = Ui + np * ld;
     double *Xi = Xr + np * ld;
-    const double2 *uk = u + (size_t)k * n * n;
+    const double2 *uk = u + (size_t)ka * n * n;
     for (int e = t; e < np * np; e += 256) {
         int i = e / np, j = e - i * np;
         double2 v = (i < n && j < n) ? uk[i * n + j] : make_double2(0.0, 0.0);
@@ -1180,8 +1212,8 @@ __global__ __launch_bounds__(256) void k_rotate_mfma(double *__restrict__ x, con
             pf[q] = (e < np * np && i < n && j < n) ? xc[i * n + j] : make_double2(0.0, 0.0);
         }
     };
-    if (C > 1) prefetch(1);
-    for (int c = 1; c < C; ++c) {
+    if (cbeg < cend) prefetch(cbeg);
+    for (int c = cbeg; c < cend; ++c) {
         double2 *xc = xk + (size_t)c * n * n;
         __syncthreads();
 #pragma unroll
@@ -1229,7 +1261,7 @@ __global__ __launch_bounds__(256) void k_rotate_mfma(double *__restrict__ x, con
                 }
             }
         }
-        if (c + 1 < C) prefetch(c + 1);
+        if (c + 1 < cend) prefetch(c + 1);
         __syncthreads();
         // Y = T U, written to global
         for (int tile = wave; tile < nt * nt; tile += 4) {
@@ -1254,19 +1286,45 @@ __global__ __launch_bounds__(256) void k_rotate_mfma(double *__restrict__ x, con
     }
 }
 
+template <bool WARM>
+static void launch_rotate_mfma(double *x, const double2 *u, int count, const WnShape &sh, hipStream_t st) {
+    const int n = sh.n;
+    const int np = (n + 15) & ~15;
+    const size_t lds = (size_t)4 * np * (np + 1) * sizeof(double);
+    const int grid = WARM ? (count + WN_WARM_G - 1) / WN_WARM_G * (WN_WARM_G - 1) : count;
+#define WN_ROT(NPV, PFV) \
+    hipLaunchKernelGGL((k_rotate_mfma<NPV, PFV, WARM>), dim3(grid), dim3(256), lds, st, x, u, n, sh.C, sh.ldo, count)
+    if (np == 16)
+        WN_ROT(16, 1);
+    else if (np == 32)
+        WN_ROT(32, 4);
+    else if (np == 48)
+        WN_ROT(48, 9);
+    else
+        WN_ROT(64, 16);
+#undef WN_ROT
+}
+
+hipError_t wn_launch_eig_f64(const double *x, double *e, double2 *u, int *sweeps, int count, const WnShape &sh,
+                             int tpb, hipStream_t st) {
+    return eig_pass(x, e, u, sweeps, count, sh, tpb, -1, st);
+}
+
+hipError_t wn_launch_eig_mesh_f64(double *x, double *e, double2 *u, int *sweeps, int count, const WnShape &sh,
+                                  int tpb, hipStream_t st) {
+    // the warm start needs the MFMA transform (n >= 16) and at least one full group
+    if (sh.n < 16 || count < WN_WARM_G) return eig_pass(x, e, u, sweeps, count, sh, tpb, -1, st);
+    hipError_t err = eig_pass(x, e, u, sweeps, count, sh, tpb, 0, st);
+    if (err != hipSuccess) return err;
+    launch_rotate_mfma<true>(x, u, count, sh, st);
+    if ((err = hipGetLastError()) != hipSuccess) return err;
+    return eig_pass(x, e, u, sweeps, count, sh, tpb, 1, st);
+}
+
 hipError_t wn_launch_rotate_f64(double *x, const double2 *u, int count, const WnShape &sh, hipStream_t st) {
     const int n = sh.n;
     if (n >= 16) {
-        const int np = (n + 15) & ~15;
-        size_t lds = (size_t)4 * np * (np + 1) * sizeof(double);
-        if (np == 16)
-            hipLaunchKernelGGL((k_rotate_mfma<16, 1>), dim3(count), dim3(256), lds, st, x, u, n, sh.C, sh.ldo);
-        else if (np == 32)
-            hipLaunchKernelGGL((k_rotate_mfma<32, 4>), dim3(count), dim3(256), lds, st, x, u, n, sh.C, sh.ldo);
-        else if (np == 48)
-            hipLaunchKernelGGL((k_rotate_mfma<48, 9>), dim3(count), dim3(256), lds, st, x, u, n, sh.C, sh.ldo);
-        else
-            hipLaunchKernelGGL((k_rotate_mfma<64, 16>), dim3(count), dim3(256), lds, st, x, u, n, sh.C, sh.ldo);
+        launch_rotate_mfma<false>(x, u, count, sh, st);
     } else {
         size_t lds = (size_t)2 * n * (n + 1) * sizeof(double2);
         hipLaunchKernelGGL(k_rotate_f64, dim3(count), dim3(256), lds, st, x, u, n, sh.C, sh.ldo);
