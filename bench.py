@@ -11,6 +11,7 @@ for speed and relative spectrum error; the CPU baseline times the NumPy FP64 ora
 
   python bench.py [--gpus 1] [--steps K] [--warmup W] [--precision mixed|fp64] [--backend auto|direct|planes]
   python bench.py --form geometry|shg|batch|learned     (the other forms, one JSON line each)
+  python bench.py --form copy                            (HBM copy used to calibrate the PMC byte counters)
 """
 
 from __future__ import annotations
@@ -69,6 +70,78 @@ def jacobi_flops_per_k(n: int, sweeps: float) -> float:
     return sweeps * (n * (n - 1) / 2) * 3 * n * 24.0
 
 
+TRAFFIC_PROFILE = os.path.join(ROOT, "profiles", "kloop_n32_64cube_hbm_traffic.md")
+
+
+def measured_traffic_per_k():
+    """HBM bytes per k point of each k-loop kernel, from the committed PMC summary (FETCH_SIZE and
+    WRITE_SIZE, each scaled by the reading of a plain copy of known size); {} when absent."""
+    try:
+        with open(TRAFFIC_PROFILE) as fh:
+            for line in fh:
+                if line.startswith("traffic_bytes_per_k:"):
+                    return {kv.split("=")[0]: float(kv.split("=")[1]) for kv in line.split(":", 1)[1].split()}
+    except OSError:
+        pass
+    return {}
+
+
+def roofline(st, n, nk, precision):
+    """Roofline of the kernel with the largest device time: algorithmic work per launch over the
+    average launch duration (HIP events around the launches on their own stream)."""
+    nb = max(1, st["batches"])
+    kern = {
+        "eig": st["eig_ms"],
+        "rotate": st["rotate_ms"],
+        "interp": st["interp_ms"],
+        "pairs": st["pairs_ms"],
+    }
+    dominant = max(kern, key=kern.get)
+    traffic = measured_traffic_per_k()
+    if dominant == "eig":
+        launches = max(1, st["eig_launches"])
+        avg_s = 1e-3 * st["eig_ms"] / launches
+        flops = jacobi_flops_per_k(n, st["mean_sweeps"]) * nk / launches
+        roof = {"kernel": "k_eig_f64 (Jacobi on FP64 VALU, both warm-start passes; FP64 vector peak = FP64 matrix "
+                          "peak)", "bound": "mfma", "achieved": flops / avg_s / 1e12, "peak": FP64_PEAK_TFLOPS,
+                "unit": "TFLOP/s", "mean_sweeps": st["mean_sweeps"]}
+        tkey = "k_eig_f64"
+    elif dominant == "rotate":
+        launches = nb
+        avg_s = 1e-3 * st["rotate_ms"] / launches
+        flops = 16.0 * n ** 3 * 21 * nk / launches
+        roof = {"kernel": "k_rotate_mfma (U^+ X U of 21 channels on FP64 MFMA)", "bound": "mfma",
+                "achieved": flops / avg_s / 1e12, "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s"}
+        tkey = "k_rotate_mfma"
+    elif dominant == "interp" and st["separable"]:
+        launches = 3 * nb
+        avg_s = 1e-3 * st["interp_ms"] / launches
+        roof = {"kernel": "k_contract (separable interpolation, writes every interpolated matrix)", "bound": "hbm",
+                "achieved": st["interp_bytes"] / launches / avg_s / 1e9, "peak": HBM_PEAK_GBS, "unit": "GB/s"}
+        tkey = "k_contract"
+    elif dominant == "interp":
+        launches = nb
+        avg_s = 1e-3 * st["interp_ms"] / launches
+        mult = 1.0 if precision == "fp64" else 3.0
+        roof = {"kernel": "k_interp (k x R contraction)", "bound": "mfma",
+                "achieved": mult * st["interp_flops"] / launches / avg_s / 1e12,
+                "peak": FP64_PEAK_TFLOPS if precision == "fp64" else BF16_PEAK_TFLOPS, "unit": "TFLOP/s"}
+        tkey = "k_interp"
+    else:
+        launches = nb
+        avg_s = 1e-3 * st["pairs_ms"] / launches
+        roof = {"kernel": "k_pairs_f64", "bound": "mfma", "achieved": None, "peak": FP64_PEAK_TFLOPS,
+                "unit": "TFLOP/s"}
+        tkey = "k_pairs_f64"
+    roof["frac"] = roof["achieved"] / roof["peak"] if roof["achieved"] is not None else None
+    roof["traffic"] = traffic[tkey] * nk / launches if tkey in traffic else None
+    if roof["traffic"] is not None:
+        roof["traffic_source"] = os.path.relpath(TRAFFIC_PROFILE, ROOT)
+    roof["avg_launch_ms"] = 1e3 * avg_s
+    roof["launches"] = launches
+    return roof
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -80,7 +153,7 @@ def main():
     ap.add_argument("--no-compare", action="store_true", help="skip the plain-FP64 and mixed direct comparison runs")
     ap.add_argument("--cpu-sample", type=int, default=1024)
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--form", default="headline", choices=["headline", "geometry", "shg", "batch", "learned"])
+    ap.add_argument("--form", default="headline", choices=["headline", "geometry", "shg", "batch", "learned", "copy"])
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}); the k-mesh would shard by "
@@ -118,8 +191,9 @@ def main():
         _, tot, st = eng.run_mesh(mesh)
         stats.append(st)
         log(f"[bench] step {s}: device {st['total_ms']:.1f} ms (interp {st['interp_ms']:.1f}, eig {st['eig_ms']:.1f}, "
-            f"rot {st['rotate_ms']:.1f}, pairs {st['pairs_ms']:.1f}, accum {st['accum_ms']:.1f}; "
-            f"separable={st['separable']}, max sweeps {st['max_sweeps']})")
+            f"warm {st['warm_ms']:.1f}, rot {st['rotate_ms']:.1f}, pairs {st['pairs_ms']:.1f}, "
+            f"accum {st['accum_ms']:.1f}; separable={st['separable']}, sweeps mean {st['mean_sweeps']:.2f} "
+            f"max {st['max_sweeps']})")
     eng.synchronize()
     wall = time.perf_counter() - t0
     ms_per_step = 1e3 * wall / args.steps
@@ -129,38 +203,7 @@ def main():
 
     # roofline of the dominant kernel, from its device event timings inside run_mesh
     st = stats[-1]
-    stages = {"interp": st["interp_ms"], "eig": st["eig_ms"], "rotate": st["rotate_ms"], "pairs": st["pairs_ms"],
-              "accum": st["accum_ms"]}
-    dominant = max(stages, key=stages.get)
-    launches = max(1, st["batches"])
-    if dominant == "eig":
-        avg_s = 1e-3 * st["eig_ms"] / launches
-        flops = jacobi_flops_per_k(n, st["max_sweeps"]) * nk / launches
-        roof = {"kernel": "k_eig_f64 (Jacobi, FP64 VALU; FP64 vector and matrix peaks are equal)", "bound": "mfma",
-                "achieved": flops / avg_s / 1e12, "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s"}
-    elif dominant == "interp" and st["separable"]:
-        avg_s = 1e-3 * st["interp_ms"] / launches
-        roof = {"kernel": "k_contract (separable interpolation, writes every interpolated matrix)", "bound": "hbm",
-                "achieved": st["interp_bytes"] / launches / avg_s / 1e9, "peak": HBM_PEAK_GBS, "unit": "GB/s"}
-    elif dominant == "interp":
-        avg_s = 1e-3 * st["interp_ms"] / launches
-        mult = 1.0 if args.precision == "fp64" else 3.0
-        roof = {"kernel": "k_interp (k x R contraction)", "bound": "mfma",
-                "achieved": mult * st["interp_flops"] / launches / avg_s / 1e12,
-                "peak": FP64_PEAK_TFLOPS if args.precision == "fp64" else BF16_PEAK_TFLOPS, "unit": "TFLOP/s"}
-    elif dominant == "rotate":
-        avg_s = 1e-3 * st["rotate_ms"] / launches
-        flops = 16.0 * n ** 3 * 21 * nk / launches
-        roof = {"kernel": "k_rotate_mfma (U^+ X U on FP64 MFMA)", "bound": "mfma", "achieved": flops / avg_s / 1e12,
-                "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s"}
-    else:
-        avg_s = 1e-3 * stages[dominant] / launches
-        roof = {"kernel": dominant, "bound": "mfma", "achieved": None, "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s"}
-    if roof["achieved"] is not None:
-        roof["frac"] = roof["achieved"] / roof["peak"]
-    roof["traffic"] = None
-    roof["avg_launch_ms"] = 1e3 * avg_s
-    roof["launches"] = launches
+    roof = roofline(st, n, nk, args.precision)
 
     compare = {}
     if not args.no_compare:
@@ -202,7 +245,9 @@ def main():
                    "separable": bool(st["separable"])},
         "roofline": roof,
         "cpu_baseline": cpu,
-        "stages_ms": {k: st[k] for k in ("interp_ms", "eig_ms", "rotate_ms", "pairs_ms", "accum_ms", "total_ms")},
+        "stages_ms": {k: st[k] for k in ("interp_ms", "eig_ms", "warm_ms", "rotate_ms", "pairs_ms", "accum_ms",
+                                         "total_ms")},
+        "stage_note": "stages run on two streams and overlap; each figure is the event-timed span of its kernels",
     }
     if compare:
         out["compare"] = compare
@@ -353,7 +398,32 @@ def form_learned(args):
     print(json.dumps(out), flush=True)
 
 
-FORMS = {"geometry": form_geometry, "shg": form_shg, "batch": form_batch, "learned": form_learned}
+def form_copy(args):
+    """HBM calibration: a plain device-to-device copy of 256 MiB (32 Mi doubles read, 32 Mi written), five
+    launches.  Its FETCH_SIZE / WRITE_SIZE counter readings against the known 256 MiB per launch give the
+    scale factors applied to the same counters of the k-loop kernels."""
+    import torch
+    if not torch.cuda.is_available():
+        raise SystemExit("the copy calibration runs on the GPU")
+    x = torch.ones(32 * 1024 * 1024, dtype=torch.float64, device="cuda")
+    y = torch.empty_like(x)
+    y.copy_(x)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(5):
+        y.copy_(x)
+    torch.cuda.synchronize()
+    dt = (time.perf_counter() - t0) / 5
+    nbytes = x.numel() * 8
+    out = {"metric": "HBM copy bandwidth (calibration)", "value": 2 * nbytes / dt / 1e9, "unit": "GB/s",
+           "n_gpus": 1, "steps": 5, "higher_is_better": True, "dtype": "f64", "data": "synthetic",
+           "config": {"workload": "device copy, 256 MiB read + 256 MiB written per launch", "bytes_per_launch":
+                      nbytes}}
+    print(json.dumps(out), flush=True)
+
+
+FORMS = {"geometry": form_geometry, "shg": form_shg, "batch": form_batch, "learned": form_learned,
+         "copy": form_copy}
 
 
 if __name__ == "__main__":

@@ -28,7 +28,7 @@
 extern "C" {
 #endif
 
-#define WNLQG_ABI_VERSION 2
+#define WNLQG_ABI_VERSION 3
 
 /* Status codes. */
 #define WNLQG_OK 0
@@ -127,6 +127,10 @@ typedef struct wnlqg_stats {
     int32_t max_sweeps;     /* largest Jacobi sweep count over the k points */
     int32_t separable;      /* 1 if the mesh used the separable plane contraction */
     double interp_bytes;    /* bytes of interpolated channels written (separable contraction) */
+    double mean_sweeps;     /* Jacobi sweeps per k point, averaged over the mesh */
+    double warm_ms;         /* warm-start transform U_a^+ H U_a of the mesh eigensolver */
+    int32_t eig_launches;   /* eigensolver kernel launches (2 per batch when warm-started) */
+    int32_t reserved;
 } wnlqg_stats;
 
 typedef struct wnlqg_engine wnlqg_engine;

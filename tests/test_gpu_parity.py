@@ -168,6 +168,32 @@ def test_mesh_integral(name, mesh):
     assert stats["kpoints"] == int(np.prod(mesh))
 
 
+@pytest.mark.parametrize("n,mesh", [(32, (5, 3, 1)), (16, (7, 1, 1)), (20, (5, 1, 2)), (32, (3, 1, 1)),
+                                    (24, (9, 2, 1)), (32, (48, 1, 1))])
+def test_warm_started_eigensolver(n, mesh):
+    """Mesh eigensolver warm start (groups of 4 consecutive k points; the anchor solved from the
+    identity, the others from the anchor's eigenvectors): short last groups (15, 7, 5 and 9 points per
+    row), padded sizes (n = 20, 24) and a row too short for a group (3 points, cold solve) all match the
+    oracle.  On the fine row (48 points) the warm-started points need fewer sweeps than the anchors."""
+    model = synthetic_model(n, 2, 40 + n, position_scale=0.1)
+    kw = dict(photon_energies=np.linspace(0, 5, 80), broadening=0.04)
+    say(f"[warm-eig] n={n} mesh={mesh}")
+    with engine(model, **kw) as eng:
+        lanes, total, st = eng.run_mesh(mesh)
+    ol, ot = kloop.integrate_mesh(model, oracle_task(**kw), mesh, batch=64)
+    for key in ("sc", "ic"):
+        r = kloop.spectrum_relative_error(total[key], ot[key])
+        say(f"  {key} worst component rel {r.max():.3e}")
+        assert r.max() < 1e-10
+        assert rel(lanes[key], ol[key]) < 1e-10
+    warm = mesh[0] * mesh[1] >= 4
+    say(f"  sweeps mean {st['mean_sweeps']:.2f} max {st['max_sweeps']} eig launches {st['eig_launches']}")
+    assert st["eig_launches"] == st["batches"] * (2 if warm else 1)
+    assert 0 < st["mean_sweeps"] <= st["max_sweeps"]
+    if mesh[0] >= 48:
+        assert st["mean_sweeps"] < st["max_sweeps"] - 0.5
+
+
 @pytest.mark.parametrize("backend,batch,mesh,nbatch", [("direct", 128, (6, 5, 7), 2), ("planes", 128, (6, 5, 7), 7),
                                                       ("planes", 128, (40, 6, 3), 6)])
 def test_mesh_lane_counts_and_batches(backend, batch, mesh, nbatch):

@@ -22,6 +22,8 @@
 #include <string>
 #include <vector>
 
+#define WN_MARKS 10
+
 namespace {
 
 thread_local std::string g_last_error;
@@ -607,7 +609,7 @@ int32_t wnlqg_create(const wnlqg_model *model, const wnlqg_task *task, wnlqg_eng
         rid2[r] = make_int2((R[0] - g->rmin[0]) * g->sh.S + (R[1] - g->rmin[1]), R[2] - g->rmin[2]);
     }
     if (g->hr.alloc(2 * n2 * nr) || g->rc.alloc(3 * (size_t)nr) || g->deg.alloc(nr) ||
-        g->ridx.alloc(rid.size()) || g->omega.alloc(g->sc.nw) || g->flags.alloc(4) ||
+        g->ridx.alloc(rid.size()) || g->omega.alloc(g->sc.nw) || g->flags.alloc(WN_FLAG_WORDS) ||
         g->bf.alloc((size_t)g->sh.Kb * (g->nterms ? g->h_tiles : g->sh.nNt) * 2048) ||
         (g->kind == 2 && (g->bh.alloc((size_t)g->sh.Kb * 16 * g->sh.ldo) || g->bm.alloc((size_t)g->sh.Kb * 16 * g->sh.ldo))) ||
         (g->kind == 2 && g->nterms >= 4 && g->bl.alloc((size_t)g->sh.Kb * 16 * g->sh.ldo)) ||
@@ -893,8 +895,10 @@ int plan_mesh(wnlqg_engine *g, const int32_t mesh[3], int32_t n_lanes) {
 
 // The kernel sequence of one planned mesh integral, ordered on g->stream (forked to g->stream2 and
 // joined back).  Capture-safe: no allocation, no host synchronization, no memset.  marks (optional)
-// holds 8 pre-created events per batch.
-int enqueue_mesh(wnlqg_engine *g, std::vector<std::array<hipEvent_t, 8>> *marks, double *interp_bytes) {
+// holds WN_MARKS pre-created events per batch: 0 start, 1 k points, 2 interpolation, 8 eigensolver
+// pass 0, 9 warm-start transform, 3 eigensolver done (stream 0); 7 start, 4 rotation, 5 response,
+// 6 k sums (stream 1).
+int enqueue_mesh(wnlqg_engine *g, std::vector<std::array<hipEvent_t, WN_MARKS>> *marks, double *interp_bytes) {
     const MeshPlan &p = g->plan;
     const int nelem = output_elems(g);
     const long long nbatch = (long long)p.batches.size();
@@ -941,7 +945,16 @@ int enqueue_mesh(wnlqg_engine *g, std::vector<std::array<hipEvent_t, 8>> *marks,
             WN_HIP(launch_interp(g, rows, xb));
         }
         mark(bi, 2, s0);
-        WN_HIP(wn_launch_eig_mesh_f64(xb, eb, ub, g->flags.p + 1, cnt, g->sh, g->eig_tpb, s0));
+        unsigned long long *sweep_total = (unsigned long long *)(g->flags.p + 4);
+        if (wn_warm_start(g->sh.n, cnt)) {
+            WN_HIP(wn_launch_eig_pass_f64(xb, eb, ub, g->flags.p + 1, sweep_total, cnt, g->sh, g->eig_tpb, 0, s0));
+            mark(bi, 8, s0);
+            WN_HIP(wn_launch_warm_transform_f64(xb, ub, cnt, g->sh, s0));
+            mark(bi, 9, s0);
+            WN_HIP(wn_launch_eig_pass_f64(xb, eb, ub, g->flags.p + 1, sweep_total, cnt, g->sh, g->eig_tpb, 1, s0));
+        } else {
+            WN_HIP(wn_launch_eig_pass_f64(xb, eb, ub, g->flags.p + 1, sweep_total, cnt, g->sh, g->eig_tpb, -1, s0));
+        }
         mark(bi, 3, s0);
         WN_HIP(hipEventRecord(g->ev_ready[buf], s0));
         // stream 1: rotation, response and k sums of batch bi
@@ -1051,7 +1064,7 @@ int32_t wnlqg_run_mesh(wnlqg_engine *g, const int32_t mesh[3], int32_t n_lanes, 
     int rc = plan_mesh(g, mesh, n_lanes);
     if (rc) return rc;
     const long long nbatch = (long long)g->plan.batches.size();
-    std::vector<std::array<hipEvent_t, 8>> marks(nbatch);
+    std::vector<std::array<hipEvent_t, WN_MARKS>> marks(nbatch);
     bool events_ok = true;
     for (auto &mk : marks)
         for (auto &e : mk) {
@@ -1076,7 +1089,15 @@ int32_t wnlqg_run_mesh(wnlqg_engine *g, const int32_t mesh[3], int32_t n_lanes, 
         for (long long bi = 0; bi < nbatch; ++bi) {
             auto &mk = marks[bi];
             hipEventElapsedTime(&ms, mk[1], mk[2]); st.interp_ms += ms;
-            hipEventElapsedTime(&ms, mk[2], mk[3]); st.eig_ms += ms;
+            if (wn_warm_start(g->sh.n, g->plan.batches[bi].cnt)) {
+                hipEventElapsedTime(&ms, mk[2], mk[8]); st.eig_ms += ms;
+                hipEventElapsedTime(&ms, mk[8], mk[9]); st.warm_ms += ms;
+                hipEventElapsedTime(&ms, mk[9], mk[3]); st.eig_ms += ms;
+                st.eig_launches += 2;
+            } else {
+                hipEventElapsedTime(&ms, mk[2], mk[3]); st.eig_ms += ms;
+                st.eig_launches += 1;
+            }
             hipEventElapsedTime(&ms, mk[7], mk[4]); st.rotate_ms += ms;
             hipEventElapsedTime(&ms, mk[4], mk[5]); st.pairs_ms += ms;
             hipEventElapsedTime(&ms, mk[5], mk[6]); st.accum_ms += ms;
@@ -1087,9 +1108,12 @@ int32_t wnlqg_run_mesh(wnlqg_engine *g, const int32_t mesh[3], int32_t n_lanes, 
         }
     }
     release_events();
-    int fl[2] = {0, 0};
-    WN_HIP(hipMemcpy(fl, g->flags.p, 2 * sizeof(int), hipMemcpyDeviceToHost));
+    int fl[WN_FLAG_WORDS] = {};
+    WN_HIP(hipMemcpy(fl, g->flags.p, sizeof(fl), hipMemcpyDeviceToHost));
     st.max_sweeps = fl[1];
+    unsigned long long swt = 0;
+    std::memcpy(&swt, fl + 4, sizeof(swt));
+    st.mean_sweeps = nk > 0 ? (double)swt / (double)nk : 0.0;
     st.kpoints = nk;
     st.batches = (int)nbatch;
     st.interp_launches = (int)nbatch;

@@ -99,12 +99,21 @@ hipError_t wn_launch_contract(const double2 *in, double2 *out, const double2 *E,
 // tpb: threads per k point (64, 128, 256; 0 = default of the size class).
 hipError_t wn_launch_eig_f64(const double *x, double *e, double2 *u, int *sweeps, int count, const WnShape &sh,
                              int tpb, hipStream_t st);
-// The same for count consecutive points of a mesh row: anchors of groups of 4 first, the other points
-// started from their anchor's eigenvectors (channel 0 of those points is overwritten by U_a^+ H U_a).
-hipError_t wn_launch_eig_mesh_f64(double *x, double *e, double2 *u, int *sweeps, int count, const WnShape &sh,
-                                  int tpb, hipStream_t st);
+// Warm-started solve of count consecutive points of a mesh row (groups of 4 along the fastest axis):
+//   pass 0  wn_launch_eig_pass_f64(..., pass = 0): the group anchors from the identity;
+//           wn_launch_warm_transform_f64: channel 0 of the other points <- U_a^+ H U_a;
+//   pass 1  wn_launch_eig_pass_f64(..., pass = 1): the other points, V started at U_a.
+// pass = -1 solves every point from the identity.  wn_warm_start(n, count) tells whether the warm
+// sequence applies (n >= 16, count >= 4).  sweep_total (nullable) accumulates the sweeps executed.
+hipError_t wn_launch_eig_pass_f64(const double *x, double *e, double2 *u, int *sweeps, unsigned long long *sweep_total,
+                                  int count, const WnShape &sh, int tpb, int pass, hipStream_t st);
+bool wn_warm_start(int n, int count);
+hipError_t wn_launch_warm_transform_f64(double *x, const double2 *u, int count, const WnShape &sh, hipStream_t st);
 // Clears that are safe under stream capture: p[0 .. n) = 0; flags = {0, 0, 0, 0x7f7f7f7f}.
 hipError_t wn_launch_zero(double *p, long long n, hipStream_t st);
+// status words: [0] pair-gap flag, [1] max sweeps, [2] geometry flags, [3] first failing k,
+// [4..5] total sweeps (64-bit)
+#define WN_FLAG_WORDS 8
 hipError_t wn_launch_reset_flags(int *f, hipStream_t st);
 // U^+ X U for channels 1 .. C-1, in place.
 hipError_t wn_launch_rotate_f64(double *x, const double2 *u, int count, const WnShape &sh, hipStream_t st);

@@ -842,8 +842,9 @@ __device__ __forceinline__ int wn_warm_k(int b, int pass, int count, int *anchor
 // pass < 0: every k point from the identity (grid = count); pass 0 / 1: the warm-start passes above
 template <int N, int TPB>
 __global__ __launch_bounds__(TPB) void k_eig_f64(const double *__restrict__ x, double *__restrict__ eout,
-                                                double2 *__restrict__ uout, int *__restrict__ sweeps_out, int n,
-                                                int ldo, double tol, int pass, int count) {
+                                                double2 *__restrict__ uout, int *__restrict__ sweeps_out,
+                                                unsigned long long *__restrict__ sweep_total, int n, int ldo,
+                                                double tol, int pass, int count) {
     constexpr int LD = N + 1;
     constexpr int HALF = N / 2;
     constexpr int NBLK = HALF * (HALF + 1) / 2;   // blocks t1 <= t2
@@ -1055,17 +1056,19 @@ __global__ __launch_bounds__(TPB) void k_eig_f64(const double *__restrict__ x, d
         uk[i * n + rank[j]] = cmul(V[i * LD + j], phase[j]);
     }
     if (t == 0 && sweeps_out) atomicMax(sweeps_out, sweep);
+    if (t == 0 && sweep_total) atomicAdd(sweep_total, (unsigned long long)sweep);
 }
 
-static hipError_t eig_pass(const double *x, double *e, double2 *u, int *sweeps, int count, const WnShape &sh, int tpb,
-                           int pass, hipStream_t st) {
+hipError_t wn_launch_eig_pass_f64(const double *x, double *e, double2 *u, int *sweeps, unsigned long long *sweep_total,
+                                  int count, const WnShape &sh, int tpb, int pass, hipStream_t st) {
     const int n = sh.n;
     const double tol = 1.0e-15;
     const int groups = (count + WN_WARM_G - 1) / WN_WARM_G;
     const int grid = pass < 0 ? count : pass == 0 ? groups : groups * (WN_WARM_G - 1);
     // tpb: threads per k point (64, 128 or 256); 0 selects the default of the size class
 #define WN_EIG_LAUNCH(NN, T) \
-    hipLaunchKernelGGL((k_eig_f64<NN, T>), dim3(grid), dim3(T), 0, st, x, e, u, sweeps, n, sh.ldo, tol, pass, count)
+    hipLaunchKernelGGL((k_eig_f64<NN, T>), dim3(grid), dim3(T), 0, st, x, e, u, sweeps, sweep_total, n, sh.ldo, tol, \
+                       pass, count)
 #define WN_EIG_CASE(NN, DEF)                   \
     if (n <= NN) {                             \
         const int t = tpb ? tpb : DEF;         \
@@ -1095,7 +1098,7 @@ __global__ void k_zero(double *__restrict__ p, long long n) {
 }
 
 __global__ void k_reset_flags(int *__restrict__ f) {
-    if (threadIdx.x < 4) f[threadIdx.x] = threadIdx.x == 3 ? 0x7f7f7f7f : 0;
+    if (threadIdx.x < WN_FLAG_WORDS) f[threadIdx.x] = threadIdx.x == 3 ? 0x7f7f7f7f : 0;
 }
 
 hipError_t wn_launch_zero(double *p, long long n, hipStream_t st) {
@@ -1307,18 +1310,15 @@ static void launch_rotate_mfma(double *x, const double2 *u, int count, const WnS
 
 hipError_t wn_launch_eig_f64(const double *x, double *e, double2 *u, int *sweeps, int count, const WnShape &sh,
                              int tpb, hipStream_t st) {
-    return eig_pass(x, e, u, sweeps, count, sh, tpb, -1, st);
+    return wn_launch_eig_pass_f64(x, e, u, sweeps, nullptr, count, sh, tpb, -1, st);
 }
 
-hipError_t wn_launch_eig_mesh_f64(double *x, double *e, double2 *u, int *sweeps, int count, const WnShape &sh,
-                                  int tpb, hipStream_t st) {
-    // the warm start needs the MFMA transform (n >= 16) and at least one full group
-    if (sh.n < 16 || count < WN_WARM_G) return eig_pass(x, e, u, sweeps, count, sh, tpb, -1, st);
-    hipError_t err = eig_pass(x, e, u, sweeps, count, sh, tpb, 0, st);
-    if (err != hipSuccess) return err;
+bool wn_warm_start(int n, int count) { return n >= 16 && count >= WN_WARM_G; }
+
+hipError_t wn_launch_warm_transform_f64(double *x, const double2 *u, int count, const WnShape &sh, hipStream_t st) {
+    if (!wn_warm_start(sh.n, count)) return hipErrorInvalidValue;
     launch_rotate_mfma<true>(x, u, count, sh, st);
-    if ((err = hipGetLastError()) != hipSuccess) return err;
-    return eig_pass(x, e, u, sweeps, count, sh, tpb, 1, st);
+    return hipGetLastError();
 }
 
 hipError_t wn_launch_rotate_f64(double *x, const double2 *u, int count, const WnShape &sh, hipStream_t st) {

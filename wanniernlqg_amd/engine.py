@@ -73,13 +73,17 @@ class Stats(C.Structure):
                 ("rotate_ms", C.c_double), ("pairs_ms", C.c_double), ("accum_ms", C.c_double),
                 ("kpoints", C.c_int64), ("active_pairs", C.c_int64), ("batches", C.c_int32),
                 ("interp_launches", C.c_int32), ("interp_flops", C.c_double), ("max_sweeps", C.c_int32),
-                ("separable", C.c_int32), ("interp_bytes", C.c_double)]
+                ("separable", C.c_int32), ("interp_bytes", C.c_double), ("mean_sweeps", C.c_double),
+                ("warm_ms", C.c_double), ("eig_launches", C.c_int32), ("reserved", C.c_int32)]
 
     def as_dict(self) -> dict:
         return {name: getattr(self, name) for name, _ in self._fields_}
 
 
 _lib = None
+
+
+ABI_VERSION = 3   # WNLQG_ABI_VERSION of include/wnlqg.h
 
 
 def load_library(path: str = LIB_PATH) -> C.CDLL:
@@ -121,6 +125,9 @@ def load_library(path: str = LIB_PATH) -> C.CDLL:
     lib.wnlqg_batch_destroy.argtypes = [C.c_void_p]
     for name in EXPORTED:
         getattr(lib, name).restype = C.c_int32
+    if lib.wnlqg_version() != ABI_VERSION:
+        raise RuntimeError(f"{path} reports ABI version {lib.wnlqg_version()}, these bindings expect {ABI_VERSION}: "
+                           "rebuild it with `python -m wanniernlqg_amd.build`")
     _lib = lib
     return lib
 
