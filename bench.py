@@ -356,8 +356,10 @@ def form_batch(args):
 
 def form_learned(args):
     """Learned refinement of the BCD on the hot-spot model (n_w = 16, |R_i| <= 4, four Weyl-like pairs with
-    1-5 meV gaps, mu = 0, T = 50 K): network trained on plain 32^3 and 48^3 runs ranks the 64^3 base cells,
-    top cells refined by 5; judged against the uniform 320^3 mesh."""
+    1-5 meV gaps, mu = 0, T = 50 K): the network trained on plain 32^3 and 48^3 runs ranks the 64^3 base
+    cells from their spectral features, top cells refined by 5; judged against the uniform 320^3 mesh,
+    whose own mesh error is estimated against the uniform 384^3 mesh.  value: the evaluation saving of
+    the smallest budget whose error is a tenth of that mesh error or less."""
     from wanniernlqg_amd import engine as E
     from wanniernlqg_amd import refine
     from wanniernlqg_amd.model import config_model
@@ -368,33 +370,46 @@ def form_learned(args):
     with E.Engine(m, task) as eng:
         lr = refine.LearnedRefinement(eng, seed=1)
         t0 = time.perf_counter()
-        lr.train(meshes=(32, 48), steps=6000, batch=4096, lr=2e-3, seed=3)
+        hist = lr.train(meshes=(32, 48), steps=6000, batch=4096, lr=2e-3, seed=3)
         train_s = time.perf_counter() - t0
         t0 = time.perf_counter()
         _, dense, _ = eng.run_mesh((n * r,) * 3)
         dense_s = time.perf_counter() - t0
+        _, finer, _ = eng.run_mesh((384,) * 3)
         ref = dense["bcd"]
-        base = lr.point_moments(refine.mesh_points(n))
-        heur = np.argsort(-refine.local_log_variance(refine.point_scalar(base), n), kind="stable")
         err = lambda v: float(np.linalg.norm(v - ref) / np.linalg.norm(ref))
+        base, ben = lr.point_data(refine.mesh_points(n))
+        t0 = time.perf_counter()
+        learned_order = np.argsort(-lr.scores(n, base, ben), kind="stable")
+        rank_s = time.perf_counter() - t0
+        heur = np.argsort(-refine.local_log_variance(refine.point_scalar(base), n), kind="stable")
+        rnd = np.random.default_rng(0).permutation(n ** 3)
         rows = []
         for frac in (0.01, 0.02, 0.05, 0.1, 0.2):
             budget = int(n ** 3 * frac)
             t0 = time.perf_counter()
-            le = lr.estimate(n, r, budget, base=base)
+            le = lr.estimate(n, r, budget, order=learned_order, base=base)
             est_s = time.perf_counter() - t0
             he = lr.estimate(n, r, budget, order=heur, base=base)
+            ra = lr.estimate(n, r, budget, order=rnd, base=base)
             nu = int(np.ceil(le.evaluations ** (1 / 3)))
             _, uni, _ = eng.run_mesh((nu,) * 3)
             rows.append({"refined_fraction": frac, "evaluations": le.evaluations,
                          "evaluations_vs_dense": (n * r) ** 3 / le.evaluations, "err_learned": err(le.estimate),
-                         "err_base_variance": err(he.estimate), "uniform_mesh": nu, "err_uniform": err(uni["bcd"]),
-                         "adaptive_s": est_s})
-    out = {"metric": "k evaluations for dense-mesh accuracy (learned adaptive BCD)", "unit": "ratio",
-           "higher_is_better": True, "n_gpus": 1, "dtype": "f64 integrand, f32 network", "data": "synthetic hot-spot model",
+                         "err_base_variance_heuristic": err(he.estimate), "err_random_cells": err(ra.estimate),
+                         "uniform_mesh": nu, "err_uniform": err(uni["bcd"]), "adaptive_s": est_s})
+    dense_mesh_err = err(finer["bcd"])
+    ok = [row for row in rows if row["err_learned"] <= 0.1 * dense_mesh_err]
+    out = {"metric": "k evaluations for dense-mesh accuracy (learned adaptive BCD)",
+           "value": ok[0]["evaluations_vs_dense"] if ok else None, "unit": "x fewer evaluations than the dense mesh",
+           "higher_is_better": True, "n_gpus": 1, "dtype": "f64 integrand, f32 network",
+           "data": "synthetic hot-spot model",
            "config": {"workload": "BCD mu=0 T=50K, n_w=16, |R_i|<=4, 4 near-degenerate pairs (1-5 meV); base 64^3, "
                                   "refinement 5, dense 320^3", "dense_kpoints": (n * r) ** 3},
-           "train_s": train_s, "dense_s": dense_s, "rows": rows}
+           "accuracy_criterion": "err vs the 320^3 result <= 0.1 x the 320^3 result's own deviation from 384^3",
+           "dense_mesh_err_vs_384": dense_mesh_err, "train_loss_first_last": [float(hist[:100].mean()),
+                                                                             float(hist[-100:].mean())],
+           "train_s": train_s, "dense_s": dense_s, "rank_s": rank_s, "rows": rows}
     print(json.dumps(out), flush=True)
 
 

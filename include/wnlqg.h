@@ -192,6 +192,14 @@ int32_t wnlqg_run_mesh(wnlqg_engine *engine, const int32_t mesh[3], int32_t n_la
 int32_t wnlqg_eval_points(wnlqg_engine *engine, const double *kpoints, int32_t m, double *out);
 
 /*
+ * wnlqg_eval_points that also returns the band energies of the same evaluation: energies [m][n],
+ * ascending per point (eV).  No reference counterpart: the per-point spectrum is what the learned
+ * refinement proposal reads as features of a cell, at no extra interpolation or eigensolve.
+ */
+int32_t wnlqg_eval_points_spectrum(wnlqg_engine *engine, const double *kpoints, int32_t m, double *out,
+                                   double *energies);
+
+/*
  * Wannier-gauge interpolated channels at m points (the Fourier sums of
  * HamiltonianMatrixElements.jl:42-58,427-462 and PositionMatrixElements.jl:239-409).
  * out: complex [m][C][n][n] with C = wnlqg_num_channels(); channel order:

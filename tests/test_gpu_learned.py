@@ -68,23 +68,37 @@ def test_full_refinement_reproduces_uniform_mesh():
     assert np.linalg.norm(res.estimate - tot["bcd"]) < 1e-10 * np.linalg.norm(tot["bcd"])
 
 
+def test_eval_points_with_energies_matches_separate_calls():
+    m = hotspot_model(3, num_orbitals=8, r_max=2, blocks=2, gap_range=(0.03, 0.04))
+    k = np.random.default_rng(4).uniform(0, 1, (300, 3))
+    say("[refine] eval_points energies")
+    with bcd_engine(m, 100.0) as eng:
+        vals, en = eng.eval_points(k, energies=True)
+        plain = eng.eval_points(k)
+        e2, _ = eng.spectrum_points(k)
+    assert np.array_equal(vals["bcd"], plain["bcd"])
+    assert en.shape == (300, 8) and np.all(np.diff(en, axis=1) >= 0)
+    assert np.max(np.abs(en - e2)) < 1e-12
+
+
 def test_learned_refinement_reaches_dense_accuracy_with_fewer_points():
     """Hot-spot model (Weyl-like pairs with 30-40 meV gaps at T = 100 K, background bands away from the
-    Fermi level): the network trained on plain 16^3 and 20^3 runs ranks the 24^3 base cells; refining
-    its top 20% by 5 reproduces the uniform 120^3 result to 1e-4 with >= 4x fewer k evaluations, while
-    a uniform mesh with at least as many evaluations, or a random choice of cells, is far off."""
+    Fermi level): the network trained on plain 16^3 and 20^3 runs ranks the 24^3 base cells from their
+    spectral features; refining its top 20% by 5 reproduces the uniform 120^3 result to 1e-4 with
+    >= 4x fewer k evaluations, while a uniform mesh with at least as many evaluations, or a random
+    choice of cells, is far off."""
     m = hotspot_model(3, num_orbitals=8, r_max=2, blocks=2, gap_range=(0.03, 0.04))
     n, r = 24, 5
     say("[refine] learned vs uniform")
     with bcd_engine(m, 100.0) as eng:
         lr = refine.LearnedRefinement(eng, seed=1)
-        lr.train(meshes=(16, 20), steps=6000, batch=2048, lr=2e-3, seed=3)
+        hist = lr.train(meshes=(16, 20), steps=6000, batch=2048, lr=2e-3, seed=3)
         _, dense, _ = eng.run_mesh((n * r,) * 3)
         ref = dense["bcd"]
-        base = lr.point_moments(refine.mesh_points(n))
-        scores = lr.scores(n)
+        base, ben = lr.point_data(refine.mesh_points(n))
+        scores = lr.scores(n, base, ben)
         budget = n ** 3 // 5
-        learned = lr.estimate(n, r, budget, base=base)
+        learned = lr.estimate(n, r, budget, base=base, base_energies=ben)
         rnd = lr.estimate(n, r, budget, order=np.random.default_rng(0).permutation(n ** 3), base=base)
         nu = int(np.ceil(learned.evaluations ** (1 / 3)))
         _, uni, _ = eng.run_mesh((nu,) * 3)
@@ -93,8 +107,10 @@ def test_learned_refinement_reaches_dense_accuracy_with_fewer_points():
                           .reshape(n ** 3, -1), axis=1)
     corr = np.corrcoef(scores, np.log10(gain + 1e-300))[0, 1]
     err = lambda v: np.linalg.norm(v - ref) / np.linalg.norm(ref)
-    say(f"  evaluations {learned.evaluations} vs dense {(n * r) ** 3}; errors learned {err(learned.estimate):.3e} "
-        f"random {err(rnd.estimate):.3e} uniform {nu}^3 {err(uni['bcd']):.3e}; corr(score, log gain) {corr:.2f}")
+    say(f"  loss {hist[:100].mean():.3f} -> {hist[-100:].mean():.3f}; evaluations {learned.evaluations} vs dense "
+        f"{(n * r) ** 3}; errors learned {err(learned.estimate):.3e} random {err(rnd.estimate):.3e} uniform {nu}^3 "
+        f"{err(uni['bcd']):.3e}; corr(score, log gain) {corr:.2f}")
+    assert hist[-100:].mean() < 0.5 * hist[:100].mean()
     assert corr > 0.5
     assert 4 * learned.evaluations <= (n * r) ** 3
     assert err(learned.estimate) < 1e-4

@@ -765,6 +765,11 @@ hipError_t launch_response(wnlqg_engine *g, const int4 *chunks, int nch, int cnt
 }  // namespace
 
 int32_t wnlqg_eval_points(wnlqg_engine *g, const double *kpoints, int32_t m, double *out) {
+    return wnlqg_eval_points_spectrum(g, kpoints, m, out, nullptr);
+}
+
+int32_t wnlqg_eval_points_spectrum(wnlqg_engine *g, const double *kpoints, int32_t m, double *out,
+                                   double *energies) {
     if (!g || !kpoints || !out || m < 0) return fail(WNLQG_E_INVALID, "invalid arguments");
     WN_HIP(hipSetDevice(g->device));
     const int nelem = output_elems(g);
@@ -777,6 +782,9 @@ int32_t wnlqg_eval_points(wnlqg_engine *g, const double *kpoints, int32_t m, dou
         if (rc) return rc;
         rc = run_front(g, cnt, rows, true);
         if (rc) return rc;
+        if (energies)
+            WN_HIP(hipMemcpyAsync(energies + (size_t)p0 * g->sh.n, g->e.p, (size_t)cnt * g->sh.n * sizeof(double),
+                                  hipMemcpyDeviceToHost, g->stream));
         WN_HIP(wn_launch_rotate_f64(g->x.p, g->u.p, cnt, g->sh, g->stream));
         rc = ensure_partial(g, cnt);
         if (rc) return rc;

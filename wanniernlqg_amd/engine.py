@@ -35,7 +35,8 @@ ERRORS = {-1: "WNLQG_E_INVALID", -2: "WNLQG_E_NONFINITE", -3: "WNLQG_E_GAP", -4:
           -5: "WNLQG_E_OOM", -6: "WNLQG_E_UNSUPPORTED"}
 
 EXPORTED = ["wnlqg_version", "wnlqg_last_error", "wnlqg_create", "wnlqg_destroy", "wnlqg_num_states",
-            "wnlqg_num_channels", "wnlqg_run_mesh", "wnlqg_eval_points", "wnlqg_fourier_points",
+            "wnlqg_num_channels", "wnlqg_run_mesh", "wnlqg_eval_points", "wnlqg_eval_points_spectrum",
+            "wnlqg_fourier_points",
             "wnlqg_spectrum_points", "wnlqg_synchronize", "wnlqg_mlp_create", "wnlqg_mlp_destroy",
             "wnlqg_mlp_last_error", "wnlqg_mlp_get_params", "wnlqg_mlp_set_params", "wnlqg_mlp_predict",
             "wnlqg_mlp_gradient", "wnlqg_mlp_train", "wnlqg_batch_create", "wnlqg_batch_run", "wnlqg_batch_tuning",
@@ -105,6 +106,7 @@ def load_library(path: str = LIB_PATH) -> C.CDLL:
     lib.wnlqg_num_channels.argtypes = [C.c_void_p]
     lib.wnlqg_run_mesh.argtypes = [C.c_void_p, C.POINTER(C.c_int32), C.c_int32, dp, dp, C.POINTER(Stats)]
     lib.wnlqg_eval_points.argtypes = [C.c_void_p, dp, C.c_int32, dp]
+    lib.wnlqg_eval_points_spectrum.argtypes = [C.c_void_p, dp, C.c_int32, dp, dp]
     lib.wnlqg_fourier_points.argtypes = [C.c_void_p, dp, C.c_int32, dp]
     lib.wnlqg_spectrum_points.argtypes = [C.c_void_p, dp, C.c_int32, dp, dp]
     lib.wnlqg_synchronize.argtypes = [C.c_void_p]
@@ -319,17 +321,26 @@ class Engine:
             out[s] = np.transpose(blk, tuple(range(L)) + perm).real.copy()
         return out
 
-    def eval_points(self, kpoints: np.ndarray):
+    def eval_points(self, kpoints: np.ndarray, energies: bool = False):
+        """Per-point values at arbitrary k; with energies=True also the ascending band energies
+        [m, n] of the same evaluation (returned as a second value)."""
         k = np.ascontiguousarray(kpoints, dtype=np.float64).reshape(-1, 3)
         nw = self._axis
         out = np.zeros(k.shape[0] * self._result_elems() * 2)
-        _check(self.lib, self.lib.wnlqg_eval_points(self.handle, _ptr(k), k.shape[0], _ptr(out)))
+        if energies:
+            en = np.zeros((k.shape[0], self.model.num_orbitals))
+            _check(self.lib, self.lib.wnlqg_eval_points_spectrum(self.handle, _ptr(k), k.shape[0], _ptr(out),
+                                                                 _ptr(en)))
+        else:
+            _check(self.lib, self.lib.wnlqg_eval_points(self.handle, _ptr(k), k.shape[0], _ptr(out)))
         if self.shg:
-            return {"shg": self._unpack_shg(out, (k.shape[0],))}
-        if self.geometry:
-            return self._unpack_geometry(out, (k.shape[0],))
-        vals = _unpack(out, (k.shape[0],), self.num_states, nw)
-        return {s: vals[:, i] for i, s in enumerate(self.states)}
+            res = {"shg": self._unpack_shg(out, (k.shape[0],))}
+        elif self.geometry:
+            res = self._unpack_geometry(out, (k.shape[0],))
+        else:
+            vals = _unpack(out, (k.shape[0],), self.num_states, nw)
+            res = {s: vals[:, i] for i, s in enumerate(self.states)}
+        return (res, en) if energies else res
 
     def fourier_points(self, kpoints: np.ndarray) -> np.ndarray:
         k = np.ascontiguousarray(kpoints, dtype=np.float64).reshape(-1, 3)
