@@ -147,7 +147,7 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--precision", default="fp64", choices=["mixed", "fp64"])
+    ap.add_argument("--precision", default="fp64", choices=["mixed", "fp64", "bf16x3"])
     ap.add_argument("--backend", default="auto", choices=["auto", "direct", "planes"])
     ap.add_argument("--mesh", type=int, default=128)
     ap.add_argument("--no-compare", action="store_true", help="skip the plain-FP64 and mixed direct comparison runs")
@@ -207,7 +207,8 @@ def main():
 
     compare = {}
     if not args.no_compare:
-        for label, prec, backend in (("plain_fp64_direct", "fp64", "direct"), ("mixed_f16_direct", "mixed", "direct")):
+        for label, prec, backend in (("plain_fp64_direct", "fp64", "direct"), ("mixed_f16_direct", "mixed", "direct"),
+                                     ("split_bf16x3_direct", "bf16x3", "direct")):
             ref = make(prec, backend)
             _, rtot, rst = ref.run_mesh(mesh)
             ref.close()
@@ -236,7 +237,7 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f64" if args.precision == "fp64" else "f16x2-split-mfma+f64",
+        "dtype": {"fp64": "f64", "mixed": "f16x2-split-mfma+f64", "bf16x3": "bf16x2-split-mfma+f64"}[args.precision],
         "data": "synthetic (seeded TB model, no external data)",
         "config": {"workload": f"shift+injection current, n_w=32, N_R=1331, {args.mesh}^3 Gamma-centred mesh, "
                                f"400 omega in [0,6] eV, Gaussian 0.01 eV, 27 components",

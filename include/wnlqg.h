@@ -57,6 +57,10 @@ extern "C" {
 #define WNLQG_PRECISION_MIXED 1 /* split-FP16 (3 products) MFMA interpolation with FP64 partial sums, FP64
                                    eigensolve, denominators and vertices, Kahan k-sum (SC / IC only) */
 #define WNLQG_PRECISION_MIXED_HIGH 2 /* split-FP16 with 4 products */
+#define WNLQG_PRECISION_SPLIT_BF16 3 /* split-BF16 (3 products) on BF16 MFMA with FP32 accumulation and FP64
+                                        partial sums; otherwise as MIXED.  Comparison form: the rounding of
+                                        the coefficient operand is the same at every k, so its error floor
+                                        (~1e-5 of the spectrum) does not average out over the mesh */
 
 /* Interpolation backends for wnlqg_run_mesh (the role of fourier_backend, FourierExecutionPlan.jl:394-480). */
 #define WNLQG_FOURIER_AUTO 0     /* separable plane contraction on meshes */

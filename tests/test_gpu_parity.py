@@ -215,9 +215,10 @@ def test_mesh_lane_counts_and_batches(backend, batch, mesh, nbatch):
         assert rel(l64[key], ol[key]) < 1e-10
 
 
-@pytest.mark.parametrize("prec,tol", [("mixed", 2e-4), ("mixed6", 2e-6)])
+@pytest.mark.parametrize("prec,tol", [("mixed", 2e-4), ("mixed6", 2e-6), ("bf16x3", 1e-3)])
 def test_mixed_fourier_channels(prec, tol):
-    """Split-BF16 interpolation: per-channel error relative to the channel's largest entry."""
+    """Split-FP16 (mixed, mixed6) and split-BF16 (bf16x3) interpolation: per-channel error relative to
+    the channel's largest entry."""
     model = MODELS["n32_pos"]()
     kp = np.random.default_rng(5).uniform(-1, 1, (64, 3))
     say(f"[fourier-{prec}] n32")
@@ -230,19 +231,25 @@ def test_mixed_fourier_channels(prec, tol):
     assert err < tol
 
 
-@pytest.mark.parametrize("prec,tol", [("mixed", 1e-3), ("mixed6", 1e-5)])
+@pytest.mark.parametrize("prec,tol", [("mixed", 1e-6), ("mixed6", 1e-6), ("bf16x3", 1e-4)])
 def test_mixed_mesh_vs_fp64(prec, tol):
+    """The mixed forms change only the direct k x R interpolation GEMM (meshes default to the separable
+    FP64 contraction), so both engines run the direct backend."""
     model = MODELS["n12_pos"]()
-    kw = dict(photon_energies=np.linspace(0, 5, 100), broadening=0.05)
-    say(f"[mesh-{prec}] n12 8x8x8")
+    kw = dict(photon_energies=np.linspace(0, 5, 100), broadening=0.05, fourier_backend="direct")
+    say(f"[mesh-{prec}] n12 8x8x8 direct")
     with engine(model, **kw) as e64:
-        _, t64, _ = e64.run_mesh((8, 8, 8))
+        _, t64, s64 = e64.run_mesh((8, 8, 8))
     with engine(model, precision=prec, **kw) as emx:
-        _, tmx, _ = emx.run_mesh((8, 8, 8))
+        _, tmx, smx = emx.run_mesh((8, 8, 8))
+    assert s64["separable"] == 0 and smx["separable"] == 0
+    worst = 0.0
     for key in ("sc", "ic"):
         r = kloop.spectrum_relative_error(tmx[key], t64[key]).max()
         say(f"  {key} worst component rel {r:.3e}")
+        worst = max(worst, r)
         assert r < tol
+    assert worst > 0.0      # the reduced-precision GEMM did run
 
 
 @pytest.mark.parametrize("name", ["fixture_n4", "n32_pos"])

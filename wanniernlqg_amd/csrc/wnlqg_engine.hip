@@ -226,7 +226,7 @@ int check_task(const wnlqg_task *t, int n) {
         return fail(WNLQG_E_UNSUPPORTED, "band_window_size %d selects a partial window; only all bands are implemented",
                     t->band_window_size);
     if (t->precision != WNLQG_PRECISION_FP64 && t->precision != WNLQG_PRECISION_MIXED &&
-        t->precision != WNLQG_PRECISION_MIXED_HIGH)
+        t->precision != WNLQG_PRECISION_MIXED_HIGH && t->precision != WNLQG_PRECISION_SPLIT_BF16)
         return fail(WNLQG_E_INVALID, "unknown precision %d", t->precision);
     if (t->batch_size < 0) return fail(WNLQG_E_INVALID, "batch_size must be >= 0");
     if (t->fourier_backend < WNLQG_FOURIER_AUTO || t->fourier_backend > WNLQG_FOURIER_PLANES)
@@ -565,19 +565,11 @@ int32_t wnlqg_create(const wnlqg_model *model, const wnlqg_task *task, wnlqg_eng
     g->sh.Kb = (nr + WN_RB - 1) / WN_RB;
     // columns padded to 256 so that both interpolation kernels tile the same output rows
     g->sh.nNt = (int)((2 * (size_t)g->sh.C * n2 + 255) / 256) * 2;
-    // mixed forms: split FP16 with 3 (MIXED) or 4 (MIXED_HIGH) products; WNLQG_MIXED_KERNEL=bf16 selects
-    // the split-BF16 kernel (WNLQG_BF16_TERMS = 3..6 products) for comparison runs
-    g->nterms = task->precision == WNLQG_PRECISION_MIXED ? 3 : (task->precision == WNLQG_PRECISION_MIXED_HIGH ? 4 : 0);
-    g->kind = g->nterms ? 1 : 0;
-    if (g->nterms) {
-        const char *kenv = getenv("WNLQG_MIXED_KERNEL");
-        if (kenv && strcmp(kenv, "bf16") == 0) {
-            g->kind = 2;
-            g->nterms = 6;
-            const char *env = getenv("WNLQG_BF16_TERMS");
-            if (env && atoi(env) >= 3 && atoi(env) <= 6) g->nterms = atoi(env);
-        }
-    }
+    // mixed forms: split FP16 with 3 (MIXED) or 4 (MIXED_HIGH) products, split BF16 with 3 (SPLIT_BF16)
+    g->nterms = task->precision == WNLQG_PRECISION_MIXED || task->precision == WNLQG_PRECISION_SPLIT_BF16
+                    ? 3
+                    : (task->precision == WNLQG_PRECISION_MIXED_HIGH ? 4 : 0);
+    g->kind = task->precision == WNLQG_PRECISION_SPLIT_BF16 ? 2 : (g->nterms ? 1 : 0);
     g->h_tiles = (int)((2 * n2 + WN_GN - 1) / WN_GN);
     g->sh.ldo = g->sh.nNt * WN_GN;
     int rmax[3];

@@ -482,9 +482,6 @@ hipError_t wn_launch_interp_bf16(const __bf16 *bh, const __bf16 *bm, const __bf1
         return hipGetLastError();                                                                              \
     }
     WN_BF16_CASE(3)
-    WN_BF16_CASE(4)
-    WN_BF16_CASE(5)
-    WN_BF16_CASE(6)
 #undef WN_BF16_CASE
     return hipErrorInvalidValue;
 }

@@ -11,8 +11,9 @@
 //                 resonances d1 = R(dE - w), d2 = R(dE - 2w), dn = R(-dE - w), R(x) = PV(x) + i pi delta(x),
 //                 and of the k-independent low-frequency factor; per band the one-band (Fermi-surface)
 //                 BCD and semiclassical sums at T > 0;
-//   k_shg_accum : sum over the chunk's k points and pairs of resonance x coefficient for every
-//                 frequency (lane = frequency), in a fixed order.
+//   k_shg_mfma  : sum over the chunk's k points and pairs of resonance x coefficient for every
+//                 frequency (lane = frequency), in a fixed order, as a real FP64 MFMA GEMM whose A
+//                 operand (the resonances) is generated in registers.
 // The remaining factors depend on the frequency only (low = PV(w, w_low), the one-band kernel
 // low - i delta(w, w_low)) and are applied to the lane sums on the host, which is exact because the
 // k sum is linear.
@@ -339,73 +340,11 @@ __global__ __launch_bounds__(WN_SHG_TPB) void k_shg_coef(const double *__restric
     }
 }
 
-// ------------------------------------------------------------------------------------------------
-// Frequency contraction of one chunk: workgroup (chunk, frequency tile of 64, column tile of 128);
-// wave w owns columns [ct*128 + 32w, +32), lane = frequency.  partial[chunk] = [ncol][nw] complex,
-// then [nc] (bcd, semi) sums over the chunk's k points (written by column tile 0, frequency tile 0).
-// ------------------------------------------------------------------------------------------------
+// R(x) = PV(x) + i pi delta(x) for the broadening of the task (SecondHarmonicKernels.jl:86-98)
 __device__ inline double2 shg_resonance(double xx, double width, int gaussian) {
     const double y = xx / width;
     const double dl = gaussian ? exp(-y * y) / (sqrt(M_PI) * width) : 1.0 / (M_PI * width * (1.0 + y * y));
     return make_double2(shg_principal(xx, width), M_PI * dl);
-}
-
-__global__ __launch_bounds__(WN_SHG_TPB) void k_shg_accum(const int4 *__restrict__ chunks, const int *__restrict__ count,
-                                                         const double *__restrict__ pdiff,
-                                                         const double2 *__restrict__ coef,
-                                                         const double2 *__restrict__ one,
-                                                         const double *__restrict__ omega, double2 *__restrict__ partial,
-                                                         int pcap, int nwt, int nct, WnShg sp) {
-    const int nc = sp.ncomp, ncol = 19 * nc, nw = sp.nw;
-    const int per_chunk = nwt * nct;
-    const int chunk = blockIdx.x / per_chunk;
-    const int rem = blockIdx.x - chunk * per_chunk;
-    const int wt = rem / nct, ct = rem - (rem / nct) * nct;
-    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
-    const int w = wt * 64 + lane;
-    const double om = w < nw ? omega[w] : 0.0;
-    const int c0 = ct * 128 + wave * 32;
-    // kinds of this wave's columns: 0 d1 [0, 8nc), 1 d2 [8nc, 14nc), 2 dn [14nc, 16nc), 3 low [16nc, 19nc)
-    const int b1 = 8 * nc, b2 = 14 * nc, b3 = 16 * nc;
-    const bool need1 = c0 < b1, need2 = c0 + 32 > b1 && c0 < b2, needn = c0 + 32 > b2 && c0 < b3;
-    double2 acc[32];
-#pragma unroll
-    for (int j = 0; j < 32; ++j) acc[j] = make_double2(0.0, 0.0);
-    const int4 ch = chunks[chunk];
-    for (int k = ch.x; k < ch.x + ch.y; ++k) {
-        const int cnt = count[k];
-        const size_t base = (size_t)k * pcap;
-        for (int p = 0; p < cnt; ++p) {
-            const double diff = pdiff[base + p];
-            const double2 r1 = need1 ? shg_resonance(diff - om, sp.width, sp.gaussian) : make_double2(0.0, 0.0);
-            const double2 r2 = need2 ? shg_resonance(diff - 2.0 * om, sp.width, sp.gaussian) : make_double2(0.0, 0.0);
-            const double2 rn = needn ? shg_resonance(-diff - om, sp.width, sp.gaussian) : make_double2(0.0, 0.0);
-            const double2 *row = coef + (base + p) * ncol;
-#pragma unroll
-            for (int j = 0; j < 32; ++j) {
-                const int col = c0 + j;
-                if (col < ncol) {
-                    const double2 r = col < b1 ? r1 : (col < b2 ? r2 : (col < b3 ? rn : make_double2(1.0, 0.0)));
-                    acc[j] = cfma(r, row[col], acc[j]);
-                }
-            }
-        }
-    }
-    double2 *o = partial + (size_t)chunk * ((size_t)ncol * nw + nc);
-    if (w < nw) {
-#pragma unroll
-        for (int j = 0; j < 32; ++j)
-            if (c0 + j < ncol) o[(size_t)(c0 + j) * nw + w] = acc[j];
-    }
-    if (wt == 0 && ct == 0 && threadIdx.x < nc) {
-        double sb = 0.0, ss = 0.0;
-        for (int k = ch.x; k < ch.x + ch.y; ++k) {
-            const double2 v = one[(size_t)k * nc + threadIdx.x];
-            sb += v.x;
-            ss += v.y;
-        }
-        o[(size_t)ncol * nw + threadIdx.x] = make_double2(sb, ss);
-    }
 }
 
 // ------------------------------------------------------------------------------------------------

@@ -30,6 +30,7 @@ WNLQG_LORENTZIAN = 1
 WNLQG_PRECISION_FP64 = 0
 WNLQG_PRECISION_MIXED = 1
 WNLQG_PRECISION_MIXED_HIGH = 2
+WNLQG_PRECISION_SPLIT_BF16 = 3
 
 ERRORS = {-1: "WNLQG_E_INVALID", -2: "WNLQG_E_NONFINITE", -3: "WNLQG_E_GAP", -4: "WNLQG_E_HIP",
           -5: "WNLQG_E_OOM", -6: "WNLQG_E_UNSUPPORTED"}
@@ -219,7 +220,7 @@ class Engine:
              (WNLQG_INTRINSIC_HALL if task.intrinsic_hall else 0) | (WNLQG_SECOND_HARMONIC if shg else 0))
         bt = {"Gaussian": WNLQG_GAUSSIAN, "Lorentzian": WNLQG_LORENTZIAN}.get(task.broadening_type, 99)
         prec = {"fp64": WNLQG_PRECISION_FP64, "mixed": WNLQG_PRECISION_MIXED,
-                "mixed6": WNLQG_PRECISION_MIXED_HIGH}[task.precision]
+                "mixed6": WNLQG_PRECISION_MIXED_HIGH, "bf16x3": WNLQG_PRECISION_SPLIT_BF16}[task.precision]
         t = _Task(q, self._om.shape[0], _ptr(self._om), bt, task.broadening, task.fermi_energy, task.temperature,
                   task.denominator_regularization, task.transition_window_factor, task.band_window_size, prec,
                   task.batch_size, {"auto": 0, "direct": 1, "planes": 2}[task.fourier_backend],
