@@ -10,7 +10,7 @@ mesh interpolation); the plain FP64 k x R contraction and the split-FP16 mixed c
 for speed and relative spectrum error; the CPU baseline times the NumPy FP64 oracle on a bounded k sample.
 
   python bench.py [--gpus 1] [--steps K] [--warmup W] [--precision mixed|fp64] [--backend auto|direct|planes]
-  python bench.py --form geometry|shg|batch|learned     (the other forms, one JSON line each)
+  python bench.py --form geometry|shg|batch|learned|large  (the other forms, one JSON line each)
   python bench.py --form copy                            (HBM copy used to calibrate the PMC byte counters)
 """
 
@@ -153,7 +153,7 @@ def main():
     ap.add_argument("--no-compare", action="store_true", help="skip the plain-FP64 and mixed direct comparison runs")
     ap.add_argument("--cpu-sample", type=int, default=1024)
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--form", default="headline", choices=["headline", "geometry", "shg", "batch", "learned", "copy"])
+    ap.add_argument("--form", default="headline", choices=["headline", "geometry", "shg", "batch", "learned", "large", "copy"])
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}); the k-mesh would shard by "
@@ -414,6 +414,32 @@ def form_learned(args):
     print(json.dumps(out), flush=True)
 
 
+def form_large(args):
+    """n_w = 96 SHG (|R_i| <= 6, 2197 R vectors, A amplitude 0.1 Angstrom; xyy, yxz, zzz; 400 omega in [0, 3] eV,
+    Gaussian 0.02 eV) on a --mesh^3 sub-mesh (default 16^3) of the 256^3 configuration.  The engine keeps
+    the Jacobi's eigenvectors and the rotation's U^+ X in global memory at this size."""
+    from wanniernlqg_amd import engine as E
+    from wanniernlqg_amd.model import config_model
+    model = config_model(4)
+    mesh = (args.mesh,) * 3 if args.mesh != 128 else (16, 16, 16)
+    task = E.EngineTask(second_harmonic=True, photon_energies=np.linspace(0.0, 3.0, 400), broadening=0.02,
+                        shg_components=[(0, 1, 1), (1, 0, 2), (2, 2, 2)], fermi_energy=0.0, temperature=0.0)
+    t0 = time.perf_counter()
+    eng = E.Engine(model, task)
+    setup = time.perf_counter() - t0
+    wall, _, st = _timed_runs(eng, mesh, args.steps, args.warmup)
+    eng.close()
+    nk = int(np.prod(mesh))
+    out = {"metric": "k-points/s (SHG, n_w=96, 3 components, 400 omega)", "value": nk / wall, "unit": "k-points/s",
+           "n_gpus": 1, "steps": args.steps, "warmup": args.warmup, "ms_per_step": 1e3 * wall,
+           "higher_is_better": True, "dtype": "f64", "data": "synthetic (seeded TB model)",
+           "config": {"workload": f"SHG xyy, yxz, zzz; 400 omega in [0,3] eV; Gaussian 0.02 eV; n_w=96, N_R=2197; "
+                                  f"{mesh[0]}^3 sub-mesh of the 256^3 configuration", "kpoints": nk},
+           "setup_s": setup, "projected_256cube_s": 256 ** 3 * wall / nk,
+           "stages_ms": {k: st[k] for k in ("interp_ms", "eig_ms", "rotate_ms", "pairs_ms", "accum_ms", "total_ms")}}
+    print(json.dumps(out), flush=True)
+
+
 def form_copy(args):
     """HBM calibration: a plain device-to-device copy of 256 MiB (32 Mi doubles read, 32 Mi written), five
     launches.  Its FETCH_SIZE / WRITE_SIZE counter readings against the known 256 MiB per launch give the
@@ -439,7 +465,7 @@ def form_copy(args):
 
 
 FORMS = {"geometry": form_geometry, "shg": form_shg, "batch": form_batch, "learned": form_learned,
-         "copy": form_copy}
+         "large": form_large, "copy": form_copy}
 
 
 if __name__ == "__main__":

@@ -67,6 +67,14 @@ def test_rejects_partial_band_window_and_large_n():
     with pytest.raises(E.WnlqgError) as ei:
         E.Engine(big, _task())
     assert ei.value.code == -6
+    # more than 64 orbitals: SHG only (up to 128)
+    with pytest.raises(E.WnlqgError) as ei:
+        E.Engine(big, _geom())
+    assert ei.value.code == -6
+    with pytest.raises(E.WnlqgError) as ei:
+        E.Engine(synthetic_model(129, 0, 0), E.EngineTask(photon_energies=np.linspace(0, 3, 10), second_harmonic=True,
+                                                          broadening=0.02))
+    assert ei.value.code == -6
 
 
 def _geom(**kw):

@@ -102,6 +102,7 @@ struct wnlqg_engine {
     WnGeom gp{};        // BCD / AHC parameters
     WnShg sp{};         // SHG parameters
     DevBuf<double2> der, coef, one;  // SHG optical position derivatives, pair coefficients, one-band sums
+    DevBuf<double2> escratch, rscratch;  // n > 64: Jacobi V and rotation U^+ X in global memory
     DevBuf<int2> pairs;               // SHG pair lists
     DevBuf<double> pdiff;             // SHG pair energy differences
     int device = 0;
@@ -260,6 +261,10 @@ int ensure_rows(wnlqg_engine *g, int rows) {
     WN_HIP(g->e.alloc((size_t)2 * rows * g->sh.n));
     WN_HIP(g->u.alloc((size_t)2 * rows * n2));
     WN_HIP(g->sweeps.alloc((size_t)rows));
+    if (g->sh.n > 64) {
+        WN_HIP(g->escratch.alloc(wn_eig_scratch_elems(g->sh.n, rows)));
+        WN_HIP(g->rscratch.alloc(wn_rotate_scratch_elems(g->sh.n, rows)));
+    }
     if (g->family == 3) {
         WN_HIP(g->count.alloc((size_t)rows));
         WN_HIP(g->der.alloc((size_t)rows * 9 * n2));
@@ -315,7 +320,9 @@ hipError_t launch_interp(wnlqg_engine *g, int rows, double *out) {
 int run_front(wnlqg_engine *g, int count, int rows, bool eig) {
     WN_HIP(launch_phases(g, rows));
     WN_HIP(launch_interp(g, rows, g->x.p));
-    if (eig) WN_HIP(wn_launch_eig_f64(g->x.p, g->e.p, g->u.p, g->sweeps.p, count, g->sh, g->eig_tpb, g->stream));
+    if (eig)
+        WN_HIP(wn_launch_eig_f64(g->x.p, g->e.p, g->u.p, g->sweeps.p, count, g->sh, g->eig_tpb, g->stream,
+                                 g->escratch.p));
     return WNLQG_OK;
 }
 
@@ -440,9 +447,11 @@ int32_t wnlqg_create(const wnlqg_model *model, const wnlqg_task *task, wnlqg_eng
         return fail(WNLQG_E_INVALID, "model pointers missing");
     const int n = model->num_orbitals, nr = model->num_r_vectors;
     if (n < 1 || nr < 1) return fail(WNLQG_E_INVALID, "num_orbitals and num_r_vectors must be positive");
-    if (n > 64) return fail(WNLQG_E_UNSUPPORTED, "num_orbitals %d > 64 is not supported by this build", n);
+    if (n > 128) return fail(WNLQG_E_UNSUPPORTED, "num_orbitals %d > 128 is not supported by this build", n);
     int rc = check_task(task, n);
     if (rc) return rc;
+    if (n > 64 && task_family(task->quantities) != 3)
+        return fail(WNLQG_E_UNSUPPORTED, "num_orbitals %d > 64 is supported for SHG only (SC/IC/BCD/AHC: <= 64)", n);
     const size_t n2 = (size_t)n * n;
     for (size_t i = 0; i < 2 * n2 * nr; ++i)
         if (!std::isfinite(model->hamiltonian_r[i])) return fail(WNLQG_E_NONFINITE, "non-finite H(R) entry %zu", i);
@@ -668,7 +677,8 @@ int32_t wnlqg_create(const wnlqg_model *model, const wnlqg_task *task, wnlqg_eng
     const size_t per_k = (size_t)g->sh.ldo * 8 +
                          (g->family == 0 ? n2 * (sizeof(int4) + sizeof(double2) + 27 * g->sc.nq * 16) : 0) +
                          (g->family == 3 ? n2 * (9 * 16 + 8 + 8 + 19 * (size_t)g->sp.ncomp * 16) : 0) +
-                         n2 * 16 + (size_t)3 * g->sh.S * 16;
+                         n2 * 16 + (size_t)3 * g->sh.S * 16 +
+                         (wn_eig_scratch_elems(n, 1) + wn_rotate_scratch_elems(n, 1)) * 16;
     long long b = task->batch_size > 0 ? task->batch_size : (long long)(24.0e9 / (double)per_k);
     b = std::max(128LL, std::min(b, 65536LL));
     g->batch = (int)((b + WN_GM - 1) / WN_GM * WN_GM);
@@ -777,7 +787,7 @@ int32_t wnlqg_eval_points_spectrum(wnlqg_engine *g, const double *kpoints, int32
         if (energies)
             WN_HIP(hipMemcpyAsync(energies + (size_t)p0 * g->sh.n, g->e.p, (size_t)cnt * g->sh.n * sizeof(double),
                                   hipMemcpyDeviceToHost, g->stream));
-        WN_HIP(wn_launch_rotate_f64(g->x.p, g->u.p, cnt, g->sh, g->stream));
+        WN_HIP(wn_launch_rotate_f64(g->x.p, g->u.p, cnt, g->sh, g->stream, g->rscratch.p));
         rc = ensure_partial(g, cnt);
         if (rc) return rc;
         std::vector<int4> ch(cnt);
@@ -953,14 +963,15 @@ int enqueue_mesh(wnlqg_engine *g, std::vector<std::array<hipEvent_t, WN_MARKS>> 
             mark(bi, 9, s0);
             WN_HIP(wn_launch_eig_pass_f64(xb, eb, ub, g->flags.p + 1, sweep_total, cnt, g->sh, g->eig_tpb, 1, s0));
         } else {
-            WN_HIP(wn_launch_eig_pass_f64(xb, eb, ub, g->flags.p + 1, sweep_total, cnt, g->sh, g->eig_tpb, -1, s0));
+            WN_HIP(wn_launch_eig_pass_f64(xb, eb, ub, g->flags.p + 1, sweep_total, cnt, g->sh, g->eig_tpb, -1, s0,
+                                          g->escratch.p));
         }
         mark(bi, 3, s0);
         WN_HIP(hipEventRecord(g->ev_ready[buf], s0));
         // stream 1: rotation, response and k sums of batch bi
         WN_HIP(hipStreamWaitEvent(s1, g->ev_ready[buf], 0));
         if (marks) hipEventRecord((*marks)[bi][7], s1);
-        WN_HIP(wn_launch_rotate_f64(xb, ub, cnt, g->sh, s1));
+        WN_HIP(wn_launch_rotate_f64(xb, ub, cnt, g->sh, s1, g->rscratch.p));
         mark(bi, 4, s1);
         const int nch = (int)(p.bchunk[bi + 1] - p.bchunk[bi]);
         const int nln = (int)(p.blane[bi + 1] - p.blane[bi]);
@@ -1176,7 +1187,7 @@ int autotune(wnlqg_batch *b) {
             float ms = 0.0f;
             for (int rep = 0; rep < 2; ++rep) {  // first launch warms up
                 hipEventRecord(a, g->stream);
-                wn_launch_eig_f64(g->x.p, g->e.p, g->u.p, g->sweeps.p, cnt, g->sh, tpb, g->stream);
+                wn_launch_eig_f64(g->x.p, g->e.p, g->u.p, g->sweeps.p, cnt, g->sh, tpb, g->stream, g->escratch.p);
                 hipEventRecord(z, g->stream);
                 if (hipEventSynchronize(z) != hipSuccess) {
                     rc = fail(WNLQG_E_HIP, "autotune timing failed");

@@ -97,8 +97,9 @@ hipError_t wn_launch_contract(const double2 *in, double2 *out, const double2 *E,
                               long long ib, long long iq, long long ob, long long op, hipStream_t st);
 // Jacobi eigensolver of the upper triangle of channel 0.
 // tpb: threads per k point (64, 128, 256; 0 = default of the size class).
+// vscratch: [count][N][N] complex (N = 96 or 128), required for n > 64 (the Jacobi's V in global memory).
 hipError_t wn_launch_eig_f64(const double *x, double *e, double2 *u, int *sweeps, int count, const WnShape &sh,
-                             int tpb, hipStream_t st);
+                             int tpb, hipStream_t st, double2 *vscratch = nullptr);
 // Warm-started solve of count consecutive points of a mesh row (groups of 4 along the fastest axis):
 //   pass 0  wn_launch_eig_pass_f64(..., pass = 0): the group anchors from the identity;
 //           wn_launch_warm_transform_f64: channel 0 of the other points <- U_a^+ H U_a;
@@ -106,7 +107,11 @@ hipError_t wn_launch_eig_f64(const double *x, double *e, double2 *u, int *sweeps
 // pass = -1 solves every point from the identity.  wn_warm_start(n, count) tells whether the warm
 // sequence applies (n >= 16, count >= 4).  sweep_total (nullable) accumulates the sweeps executed.
 hipError_t wn_launch_eig_pass_f64(const double *x, double *e, double2 *u, int *sweeps, unsigned long long *sweep_total,
-                                  int count, const WnShape &sh, int tpb, int pass, hipStream_t st);
+                                  int count, const WnShape &sh, int tpb, int pass, hipStream_t st,
+                                  double2 *vscratch = nullptr);
+// size in complex elements of the eigensolver / rotation scratch for count points (0 for n <= 64)
+size_t wn_eig_scratch_elems(int n, int count);
+size_t wn_rotate_scratch_elems(int n, int count);
 bool wn_warm_start(int n, int count);
 hipError_t wn_launch_warm_transform_f64(double *x, const double2 *u, int count, const WnShape &sh, hipStream_t st);
 // Clears that are safe under stream capture: p[0 .. n) = 0; flags = {0, 0, 0, 0x7f7f7f7f}.
@@ -116,7 +121,9 @@ hipError_t wn_launch_zero(double *p, long long n, hipStream_t st);
 #define WN_FLAG_WORDS 8
 hipError_t wn_launch_reset_flags(int *f, hipStream_t st);
 // U^+ X U for channels 1 .. C-1, in place.
-hipError_t wn_launch_rotate_f64(double *x, const double2 *u, int count, const WnShape &sh, hipStream_t st);
+// scratch: wn_rotate_scratch_elems(n, count) complex elements, required for n > 64
+hipError_t wn_launch_rotate_f64(double *x, const double2 *u, int count, const WnShape &sh, hipStream_t st,
+                                double2 *scratch = nullptr);
 // Screening, pair lists, generalized derivatives and SC/IC kernels.
 hipError_t wn_launch_pairs_f64(const double *x, const double *e, const double *omega, int *count, int4 *pmeta,
                                double2 *pmd, double2 *q, int *flags, int nk, const WnShape &sh, const WnScreen &sc,

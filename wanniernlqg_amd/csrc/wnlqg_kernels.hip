@@ -836,12 +836,14 @@ __device__ __forceinline__ int wn_warm_k(int b, int pass, int count, int *anchor
     return base + (j < al ? j : j + 1);
 }
 
-// pass < 0: every k point from the identity (grid = count); pass 0 / 1: the warm-start passes above
-template <int N, int TPB>
+// pass < 0: every k point from the identity (grid = count); pass 0 / 1: the warm-start passes above.
+// VG: V lives in global memory (vscratch, [count][N][N] column-major, so that the threads of one plane
+// update read consecutive rows) because H and V together exceed the LDS for N > 64.
+template <int N, int TPB, bool VG>
 __global__ __launch_bounds__(TPB) void k_eig_f64(const double *__restrict__ x, double *__restrict__ eout,
                                                 double2 *__restrict__ uout, int *__restrict__ sweeps_out,
                                                 unsigned long long *__restrict__ sweep_total, int n, int ldo,
-                                                double tol, int pass, int count) {
+                                                double tol, int pass, int count, double2 *__restrict__ vscratch) {
     constexpr int LD = N + 1;
     constexpr int HALF = N / 2;
     constexpr int NBLK = HALF * (HALF + 1) / 2;   // blocks t1 <= t2
@@ -849,7 +851,7 @@ __global__ __launch_bounds__(TPB) void k_eig_f64(const double *__restrict__ x, d
     constexpr int NW = TPB / 64;
     constexpr int NP = N * (N + 1) / 2;
     __shared__ double2 H[NP];       // packed upper triangle, row-major
-    __shared__ double2 V[N * LD];
+    __shared__ double2 Vs[VG ? 1 : N * LD];
     __shared__ double4 rot[HALF];   // (c, s, cos phi, sin phi)
     __shared__ int2 pq[HALF];
     __shared__ double red[2 * NW];
@@ -861,6 +863,9 @@ __global__ __launch_bounds__(TPB) void k_eig_f64(const double *__restrict__ x, d
     const int t = threadIdx.x;
 
     auto P = [](int i, int j) { return i * (2 * N - i + 1) / 2 + (j - i); };
+    double2 *vg = VG ? vscratch + (size_t)k * N * N : nullptr;
+    // V[i][j]: row i, column j (the eigenvector index)
+    auto V = [&](int i, int j) -> double2 & { return VG ? vg[(size_t)j * N + i] : Vs[i * LD + j]; };
     const double2 *xk = (const double2 *)(x + (size_t)k * ldo);
     const double2 *ua = anchor >= 0 ? uout + (size_t)anchor * n * n : nullptr;
     for (int e = t; e < N * N; e += TPB) {
@@ -873,8 +878,9 @@ __global__ __launch_bounds__(TPB) void k_eig_f64(const double *__restrict__ x, d
             }
             H[P(i, j)] = hv;
         }
-        V[i * LD + j] = (ua && i < n && j < n) ? ua[i * n + j] : make_double2(i == j ? 1.0 : 0.0, 0.0);
+        V(i, j) = (ua && i < n && j < n) ? ua[i * n + j] : make_double2(i == j ? 1.0 : 0.0, 0.0);
     }
+    if (VG) __threadfence_block();
     __syncthreads();
 
     int sweep = 0;
@@ -1007,11 +1013,12 @@ __global__ __launch_bounds__(TPB) void k_eig_f64(const double *__restrict__ x, d
                 const double4 R = rot[pr];
                 if (R.y == 0.0 || i >= n) continue;
                 const int2 ij = pq[pr];
-                const double2 xp = V[i * LD + ij.x], yq = V[i * LD + ij.y];
+                const double2 xp = V(i, ij.x), yq = V(i, ij.y);
                 const double2 ey = make_double2(R.z * yq.x + R.w * yq.y, R.z * yq.y - R.w * yq.x);
-                V[i * LD + ij.x] = make_double2(R.x * xp.x - R.y * ey.x, R.x * xp.y - R.y * ey.y);
-                V[i * LD + ij.y] = make_double2(R.y * xp.x + R.x * ey.x, R.y * xp.y + R.x * ey.y);
+                V(i, ij.x) = make_double2(R.x * xp.x - R.y * ey.x, R.x * xp.y - R.y * ey.y);
+                V(i, ij.y) = make_double2(R.y * xp.x + R.x * ey.x, R.y * xp.y + R.x * ey.y);
             }
+            if (VG) __threadfence_block();
             __syncthreads();
         }
     }
@@ -1030,9 +1037,9 @@ __global__ __launch_bounds__(TPB) void k_eig_f64(const double *__restrict__ x, d
         rank[t] = rk;
         eout[(size_t)k * n + rk] = et;
         int piv = 0;
-        double pm = hypot(V[t].x, V[t].y);
+        double pm = hypot(V(0, t).x, V(0, t).y);
         for (int i = 1; i < n; ++i) {
-            const double2 v = V[i * LD + t];
+            const double2 v = V(i, t);
             const double m = hypot(v.x, v.y);
             if (m > pm) {
                 pm = m;
@@ -1041,7 +1048,7 @@ __global__ __launch_bounds__(TPB) void k_eig_f64(const double *__restrict__ x, d
         }
         double2 ph = make_double2(1.0, 0.0);
         if (pm > 0.0) {
-            const double2 v = V[piv * LD + t];
+            const double2 v = V(piv, t);
             ph = make_double2(v.x / pm, -v.y / pm);
         }
         phase[t] = ph;
@@ -1050,22 +1057,22 @@ __global__ __launch_bounds__(TPB) void k_eig_f64(const double *__restrict__ x, d
     double2 *uk = uout + (size_t)k * n * n;
     for (int e = t; e < n * n; e += TPB) {
         const int i = e / n, j = e - i * n;
-        uk[i * n + rank[j]] = cmul(V[i * LD + j], phase[j]);
+        uk[i * n + rank[j]] = cmul(V(i, j), phase[j]);
     }
     if (t == 0 && sweeps_out) atomicMax(sweeps_out, sweep);
     if (t == 0 && sweep_total) atomicAdd(sweep_total, (unsigned long long)sweep);
 }
 
 hipError_t wn_launch_eig_pass_f64(const double *x, double *e, double2 *u, int *sweeps, unsigned long long *sweep_total,
-                                  int count, const WnShape &sh, int tpb, int pass, hipStream_t st) {
+                                  int count, const WnShape &sh, int tpb, int pass, hipStream_t st, double2 *vscratch) {
     const int n = sh.n;
     const double tol = 1.0e-15;
     const int groups = (count + WN_WARM_G - 1) / WN_WARM_G;
     const int grid = pass < 0 ? count : pass == 0 ? groups : groups * (WN_WARM_G - 1);
     // tpb: threads per k point (64, 128 or 256); 0 selects the default of the size class
 #define WN_EIG_LAUNCH(NN, T) \
-    hipLaunchKernelGGL((k_eig_f64<NN, T>), dim3(grid), dim3(T), 0, st, x, e, u, sweeps, sweep_total, n, sh.ldo, tol, \
-                       pass, count)
+    hipLaunchKernelGGL((k_eig_f64<NN, T, false>), dim3(grid), dim3(T), 0, st, x, e, u, sweeps, sweep_total, n, sh.ldo, \
+                       tol, pass, count, nullptr)
 #define WN_EIG_CASE(NN, DEF)                   \
     if (n <= NN) {                             \
         const int t = tpb ? tpb : DEF;         \
@@ -1085,6 +1092,17 @@ hipError_t wn_launch_eig_pass_f64(const double *x, double *e, double2 *u, int *s
     WN_EIG_CASE(64, 256)
 #undef WN_EIG_CASE
 #undef WN_EIG_LAUNCH
+    // n in (64, 128]: V in global memory, 512 threads, cold start only
+    if (n <= 128 && pass < 0) {
+        if (!vscratch) return hipErrorInvalidValue;
+        if (n <= 96)
+            hipLaunchKernelGGL((k_eig_f64<96, 512, true>), dim3(grid), dim3(512), 0, st, x, e, u, sweeps, sweep_total, n,
+                               sh.ldo, tol, pass, count, vscratch);
+        else
+            hipLaunchKernelGGL((k_eig_f64<128, 512, true>), dim3(grid), dim3(512), 0, st, x, e, u, sweeps, sweep_total,
+                               n, sh.ldo, tol, pass, count, vscratch);
+        return hipGetLastError();
+    }
     return hipErrorInvalidValue;
 }
 
@@ -1286,6 +1304,73 @@ __global__ __launch_bounds__(256) void k_rotate_mfma(double *__restrict__ x, con
     }
 }
 
+// Rotation for n in (64, 128]: U and X are read through the L2 as MFMA operands and U^+ X is staged
+// in a per-k scratch [NP][NP] (the LDS cannot hold U, X and U^+ X at these sizes).  One 256-thread
+// workgroup per k, channels 1 .. C-1 in turn; operand layouts as in k_rotate_mfma.
+template <int NP>
+__global__ __launch_bounds__(256) void k_rotate_big(double *__restrict__ x, const double2 *__restrict__ u, int n, int C,
+                                                   int ldo, double2 *__restrict__ scratch) {
+    constexpr int nt = NP / 16;
+    const int k = blockIdx.x;
+    const int t = threadIdx.x;
+    const int lane = t & 63, wave = t >> 6;
+    const int lr = lane & 15, lq = lane >> 4;
+    const double2 *uk = u + (size_t)k * n * n;
+    double2 *tk = scratch + (size_t)k * NP * NP;
+    double2 *xk = (double2 *)(x + (size_t)k * ldo);
+    const double2 zero = make_double2(0.0, 0.0);
+    for (int c = 1; c < C; ++c) {
+        double2 *xc = xk + (size_t)c * n * n;
+        // T = U^+ X into the scratch (rows and columns beyond n are zero)
+        for (int tile = wave; tile < nt * nt; tile += 4) {
+            const int r0 = (tile / nt) * 16, c0 = (tile % nt) * 16;
+            f64x4 ar = {0.0, 0.0, 0.0, 0.0}, ai = {0.0, 0.0, 0.0, 0.0};
+            for (int k0 = 0; k0 < NP; k0 += 4) {
+                const int kr = k0 + lq;
+                const double2 uv = (kr < n && r0 + lr < n) ? uk[kr * n + r0 + lr] : zero;
+                const double2 xv = (kr < n && c0 + lr < n) ? xc[kr * n + c0 + lr] : zero;
+                ar = __builtin_amdgcn_mfma_f64_16x16x4f64(uv.x, xv.x, ar, 0, 0, 0);
+                ar = __builtin_amdgcn_mfma_f64_16x16x4f64(uv.y, xv.y, ar, 0, 0, 0);
+                ai = __builtin_amdgcn_mfma_f64_16x16x4f64(uv.x, xv.y, ai, 0, 0, 0);
+                ai = __builtin_amdgcn_mfma_f64_16x16x4f64(-uv.y, xv.x, ai, 0, 0, 0);
+            }
+#pragma unroll
+            for (int q = 0; q < 4; ++q) tk[(r0 + lq + 4 * q) * NP + c0 + lr] = make_double2(ar[q], ai[q]);
+        }
+        __threadfence_block();
+        __syncthreads();
+        // X <- T U
+        for (int tile = wave; tile < nt * nt; tile += 4) {
+            const int r0 = (tile / nt) * 16, c0 = (tile % nt) * 16;
+            f64x4 yr = {0.0, 0.0, 0.0, 0.0}, yi = {0.0, 0.0, 0.0, 0.0};
+            for (int k0 = 0; k0 < NP; k0 += 4) {
+                const int kc = k0 + lq;
+                const double2 tv = tk[(r0 + lr) * NP + kc];
+                const double2 uv = (kc < n && c0 + lr < n) ? uk[kc * n + c0 + lr] : zero;
+                yr = __builtin_amdgcn_mfma_f64_16x16x4f64(tv.x, uv.x, yr, 0, 0, 0);
+                yr = __builtin_amdgcn_mfma_f64_16x16x4f64(-tv.y, uv.y, yr, 0, 0, 0);
+                yi = __builtin_amdgcn_mfma_f64_16x16x4f64(tv.x, uv.y, yi, 0, 0, 0);
+                yi = __builtin_amdgcn_mfma_f64_16x16x4f64(tv.y, uv.x, yi, 0, 0, 0);
+            }
+#pragma unroll
+            for (int q = 0; q < 4; ++q) {
+                const int row = r0 + lq + 4 * q, col = c0 + lr;
+                if (row < n && col < n) xc[row * n + col] = make_double2(yr[q], yi[q]);
+            }
+        }
+        __threadfence_block();
+        __syncthreads();
+    }
+}
+
+size_t wn_eig_scratch_elems(int n, int count) {
+    if (n <= 64) return 0;
+    const size_t N = n <= 96 ? 96 : 128;
+    return N * N * (size_t)count;
+}
+
+size_t wn_rotate_scratch_elems(int n, int count) { return wn_eig_scratch_elems(n, count); }
+
 template <bool WARM>
 static void launch_rotate_mfma(double *x, const double2 *u, int count, const WnShape &sh, hipStream_t st) {
     const int n = sh.n;
@@ -1306,11 +1391,11 @@ static void launch_rotate_mfma(double *x, const double2 *u, int count, const WnS
 }
 
 hipError_t wn_launch_eig_f64(const double *x, double *e, double2 *u, int *sweeps, int count, const WnShape &sh,
-                             int tpb, hipStream_t st) {
-    return wn_launch_eig_pass_f64(x, e, u, sweeps, nullptr, count, sh, tpb, -1, st);
+                             int tpb, hipStream_t st, double2 *vscratch) {
+    return wn_launch_eig_pass_f64(x, e, u, sweeps, nullptr, count, sh, tpb, -1, st, vscratch);
 }
 
-bool wn_warm_start(int n, int count) { return n >= 16 && count >= WN_WARM_G; }
+bool wn_warm_start(int n, int count) { return n >= 16 && n <= 64 && count >= WN_WARM_G; }
 
 hipError_t wn_launch_warm_transform_f64(double *x, const double2 *u, int count, const WnShape &sh, hipStream_t st) {
     if (!wn_warm_start(sh.n, count)) return hipErrorInvalidValue;
@@ -1318,9 +1403,17 @@ hipError_t wn_launch_warm_transform_f64(double *x, const double2 *u, int count, 
     return hipGetLastError();
 }
 
-hipError_t wn_launch_rotate_f64(double *x, const double2 *u, int count, const WnShape &sh, hipStream_t st) {
+hipError_t wn_launch_rotate_f64(double *x, const double2 *u, int count, const WnShape &sh, hipStream_t st,
+                                double2 *scratch) {
     const int n = sh.n;
-    if (n >= 16) {
+    if (n > 128) return hipErrorInvalidValue;
+    if (n > 64) {
+        if (!scratch) return hipErrorInvalidValue;
+        if (n <= 96)
+            hipLaunchKernelGGL(k_rotate_big<96>, dim3(count), dim3(256), 0, st, x, u, n, sh.C, sh.ldo, scratch);
+        else
+            hipLaunchKernelGGL(k_rotate_big<128>, dim3(count), dim3(256), 0, st, x, u, n, sh.C, sh.ldo, scratch);
+    } else if (n >= 16) {
         launch_rotate_mfma<false>(x, u, count, sh, st);
     } else {
         size_t lds = (size_t)2 * n * (n + 1) * sizeof(double2);

@@ -34,6 +34,7 @@
 #include <algorithm>
 
 #define WN_SHG_TPB 256
+#define WN_SHG_NMAX 128   // orbitals (<= WN_SHG_TPB: one thread per band in the per-band phases)
 
 namespace {
 
@@ -107,7 +108,7 @@ __device__ inline bool shg_active(const double *occ, int i, int j) { return i !=
 // ------------------------------------------------------------------------------------------------
 __global__ __launch_bounds__(WN_SHG_TPB) void k_shg_der(const double *__restrict__ x, const double *__restrict__ eg,
                                                        double2 *__restrict__ der, WnShape sh, WnShg sp) {
-    __shared__ double occ[64];
+    __shared__ double occ[WN_SHG_NMAX];
     const long long k = blockIdx.x;
     const int ab = blockIdx.y, a = ab / 3, b = ab - (ab / 3) * 3;
     const int n = sh.n, n2 = n * n;
@@ -177,9 +178,9 @@ __global__ __launch_bounds__(WN_SHG_TPB) void k_shg_coef(const double *__restric
                                                         int2 *__restrict__ pairs, double *__restrict__ pdiff,
                                                         double2 *__restrict__ coef, double2 *__restrict__ one,
                                                         WnShape sh, WnShg sp) {
-    __shared__ double occ[64];
-    __shared__ int rowcnt[64];
-    __shared__ int rowoff[65];
+    __shared__ double occ[WN_SHG_NMAX];
+    __shared__ int rowcnt[WN_SHG_NMAX];
+    __shared__ int rowoff[WN_SHG_NMAX + 1];
     extern __shared__ double lds_one[];  // [n][nc][2] one-band contributions
     const long long k = blockIdx.x;
     const int n = sh.n, n2 = n * n, nc = sp.ncomp;
@@ -489,7 +490,7 @@ __global__ __launch_bounds__(WN_SHG_TPB) void k_shg_mfma(const int4 *__restrict_
 hipError_t wn_launch_shg(const double *x, const double *e, const double *omega, const int4 *chunks, int nchunks,
                          int count_k, double2 *der, int *count, int2 *pairs, double *pdiff, double2 *coef,
                          double2 *one, double2 *partial, const WnShape &sh, const WnShg &sp, hipStream_t st) {
-    if (sh.n > 64 || sp.ncomp < 1 || sp.ncomp > 18) return hipErrorInvalidValue;
+    if (sh.n > WN_SHG_NMAX || sp.ncomp < 1 || sp.ncomp > 18) return hipErrorInvalidValue;
     if (count_k > 0) {
         hipLaunchKernelGGL(k_shg_der, dim3(count_k, 9), dim3(WN_SHG_TPB), 0, st, x, e, der, sh, sp);
         hipLaunchKernelGGL(k_shg_coef, dim3(count_k), dim3(WN_SHG_TPB), (size_t)sh.n * sp.ncomp * 2 * sizeof(double),
