@@ -883,6 +883,23 @@ __global__ __launch_bounds__(TPB) void k_eig_f64(const double *__restrict__ x, d
     if (VG) __threadfence_block();
     __syncthreads();
 
+    // the off-diagonal blocks (t1 < t2) this thread updates, decoded once (row-major over the upper
+    // triangle of blocks); -1 marks no block
+    constexpr int BPT = (NBLK + TPB - 1) / TPB;
+    int bt1[BPT], bt2[BPT];
+#pragma unroll
+    for (int q = 0; q < BPT; ++q) {
+        const int bidx = t + q * TPB;
+        int t1 = 0, rem = bidx;
+        while (t1 < HALF && rem >= HALF - t1) {
+            rem -= HALF - t1;
+            ++t1;
+        }
+        const bool ok = bidx < NBLK && rem > 0;
+        bt1[q] = ok ? t1 : -1;
+        bt2[q] = ok ? t1 + rem : -1;
+    }
+
     int sweep = 0;
     double prev_off = 0.0;
     for (; sweep < 40; ++sweep) {
@@ -957,15 +974,10 @@ __global__ __launch_bounds__(TPB) void k_eig_f64(const double *__restrict__ x, d
             }
             __syncthreads();
             // off-diagonal blocks t1 < t2 of H
-            for (int bidx = t; bidx < NBLK; bidx += TPB) {
-                // decode bidx -> (t1, t2) with t1 <= t2 (row-major over the upper triangle of blocks)
-                int t1 = 0, rem = bidx;
-                while (rem >= HALF - t1) {
-                    rem -= HALF - t1;
-                    ++t1;
-                }
-                const int t2 = t1 + rem;
-                if (t2 == t1) continue;
+#pragma unroll
+            for (int qb = 0; qb < BPT; ++qb) {
+                const int t1 = bt1[qb], t2 = bt2[qb];
+                if (t1 < 0) continue;
                 const double4 R1 = rot[t1], R2 = rot[t2];
                 if (R1.y == 0.0 && R2.y == 0.0) continue;
                 const int2 a1 = pq[t1], a2 = pq[t2];
@@ -1848,39 +1860,72 @@ hipError_t wn_launch_accum(const int4 *chunks, int nchunks, const int *count, co
 // Chunk partials -> lanes in fixed chunk order (Kahan-compensated when lanecomp != nullptr).
 // lanes[i] = (lane, chunk_begin, chunk_end, -).
 // ------------------------------------------------------------------------------------------------
-__global__ void k_lane_reduce(const int4 *__restrict__ lanes, const double2 *__restrict__ partial,
-                              double2 *__restrict__ lanebuf, double2 *__restrict__ lanecomp, int nelem) {
-    const int e = blockIdx.x * blockDim.x + threadIdx.x;
-    if (e >= nelem) return;
+// Chunk partials -> lane accumulators in a fixed order.  A workgroup owns WN_LR_E consecutive
+// elements of one lane; its WN_LR_S segments of threads sum the lane's chunks c = first + s,
+// first + s + WN_LR_S, ... in increasing c, and segment sums are added to the lane in segment order,
+// so the result depends only on the chunk plan.  With lanecomp (the mixed forms) every addition is
+// Kahan-compensated.
+#define WN_LR_E 16
+#define WN_LR_S 16
+__global__ __launch_bounds__(WN_LR_E * WN_LR_S) void k_lane_reduce(const int4 *__restrict__ lanes,
+                                                                  const double2 *__restrict__ partial,
+                                                                  double2 *__restrict__ lanebuf,
+                                                                  double2 *__restrict__ lanecomp, int nelem) {
+    __shared__ double2 ssum[WN_LR_S][WN_LR_E];
+    __shared__ double2 scomp[WN_LR_S][WN_LR_E];
+    const int j = threadIdx.x % WN_LR_E, sg = threadIdx.x / WN_LR_E;
+    const int e = blockIdx.x * WN_LR_E + j;
     const int4 li = lanes[blockIdx.y];
-    const size_t o = (size_t)li.x * nelem + e;
-    double2 acc = lanebuf[o];
-    if (lanecomp) {
-        double2 comp = lanecomp[o];
-        for (int c = li.y; c < li.z; ++c) {
-            double2 v = partial[(size_t)c * nelem + e];
-            double yx = v.x - comp.x, yy = v.y - comp.y;
-            double tx = acc.x + yx, ty = acc.y + yy;
-            comp.x = (tx - acc.x) - yx;
-            comp.y = (ty - acc.y) - yy;
-            acc.x = tx;
-            acc.y = ty;
-        }
-        lanecomp[o] = comp;
-    } else {
-        for (int c = li.y; c < li.z; ++c) {
-            double2 v = partial[(size_t)c * nelem + e];
-            acc.x += v.x;
-            acc.y += v.y;
+    double2 acc = make_double2(0.0, 0.0), comp = make_double2(0.0, 0.0);
+    if (e < nelem) {
+        for (int c = li.y + sg; c < li.z; c += WN_LR_S) {
+            const double2 v = partial[(size_t)c * nelem + e];
+            if (lanecomp) {
+                const double yx = v.x - comp.x, yy = v.y - comp.y;
+                const double tx = acc.x + yx, ty = acc.y + yy;
+                comp.x = (tx - acc.x) - yx;
+                comp.y = (ty - acc.y) - yy;
+                acc.x = tx;
+                acc.y = ty;
+            } else {
+                acc.x += v.x;
+                acc.y += v.y;
+            }
         }
     }
-    lanebuf[o] = acc;
+    ssum[sg][j] = acc;
+    scomp[sg][j] = comp;
+    __syncthreads();
+    if (sg != 0 || e >= nelem) return;
+    const size_t o = (size_t)li.x * nelem + e;
+    double2 lane = lanebuf[o];
+    if (lanecomp) {
+        double2 lc = lanecomp[o];
+        for (int q = 0; q < WN_LR_S; ++q) {
+            // segment value sum - compensation, added with compensation
+            const double2 sv = ssum[q][j], sc = scomp[q][j];
+            const double vx = sv.x - sc.x, vy = sv.y - sc.y;
+            const double yx = vx - lc.x, yy = vy - lc.y;
+            const double tx = lane.x + yx, ty = lane.y + yy;
+            lc.x = (tx - lane.x) - yx;
+            lc.y = (ty - lane.y) - yy;
+            lane.x = tx;
+            lane.y = ty;
+        }
+        lanecomp[o] = lc;
+    } else {
+        for (int q = 0; q < WN_LR_S; ++q) {
+            lane.x += ssum[q][j].x;
+            lane.y += ssum[q][j].y;
+        }
+    }
+    lanebuf[o] = lane;
 }
 
 hipError_t wn_launch_lane_reduce(const int4 *lanes, int nlanes, const double2 *partial, double2 *lanebuf,
                                  double2 *lanecomp, int nelem, hipStream_t st) {
     if (nlanes == 0) return hipSuccess;
-    hipLaunchKernelGGL(k_lane_reduce, dim3((nelem + 255) / 256, nlanes), dim3(256), 0, st, lanes, partial, lanebuf,
-                       lanecomp, nelem);
+    hipLaunchKernelGGL(k_lane_reduce, dim3((nelem + WN_LR_E - 1) / WN_LR_E, nlanes), dim3(WN_LR_E * WN_LR_S), 0, st,
+                       lanes, partial, lanebuf, lanecomp, nelem);
     return hipGetLastError();
 }
