@@ -846,7 +846,6 @@ __global__ __launch_bounds__(TPB) void k_eig_f64(const double *__restrict__ x, d
                                                 double tol, int pass, int count, double2 *__restrict__ vscratch) {
     constexpr int LD = N + 1;
     constexpr int HALF = N / 2;
-    constexpr int NBLK = HALF * (HALF + 1) / 2;   // blocks t1 <= t2
     constexpr int VITEMS = N * HALF;
     constexpr int NW = TPB / 64;
     constexpr int NP = N * (N + 1) / 2;
@@ -883,21 +882,30 @@ __global__ __launch_bounds__(TPB) void k_eig_f64(const double *__restrict__ x, d
     if (VG) __threadfence_block();
     __syncthreads();
 
-    // the off-diagonal blocks (t1 < t2) this thread updates, decoded once (row-major over the upper
-    // triangle of blocks); -1 marks no block
-    constexpr int BPT = (NBLK + TPB - 1) / TPB;
+    // Work split of a round: NOFF off-diagonal blocks (t1 < t2) and VITEMS updates of V.  When the
+    // blocks fit in whole waves with at least one wave to spare (N = 32 at 256 threads), the first
+    // BW threads take the blocks and the remaining waves take V, so that no wave carries both;
+    // otherwise every thread takes its share of both.
+    constexpr int NOFF = HALF * (HALF - 1) / 2;
+    constexpr int BW0 = (NOFF + 63) / 64 * 64;
+    constexpr bool SPLIT = BW0 + 64 <= TPB;
+    constexpr int BW = SPLIT ? BW0 : TPB;           // threads updating blocks
+    constexpr int VT0 = SPLIT ? BW0 : 0;            // first thread updating V
+    constexpr int VW = SPLIT ? TPB - BW0 : TPB;     // threads updating V
+    constexpr int BPT = (NOFF + BW - 1) / BW;
     int bt1[BPT], bt2[BPT];
 #pragma unroll
     for (int q = 0; q < BPT; ++q) {
-        const int bidx = t + q * TPB;
+        // bidx -> (t1, t2), t1 < t2, row-major over the strict upper triangle of blocks
+        const int bidx = t + q * BW;
         int t1 = 0, rem = bidx;
-        while (t1 < HALF && rem >= HALF - t1) {
-            rem -= HALF - t1;
+        while (t1 < HALF - 1 && rem >= HALF - 1 - t1) {
+            rem -= HALF - 1 - t1;
             ++t1;
         }
-        const bool ok = bidx < NBLK && rem > 0;
+        const bool ok = t < BW && bidx < NOFF;
         bt1[q] = ok ? t1 : -1;
-        bt2[q] = ok ? t1 + rem : -1;
+        bt2[q] = ok ? t1 + 1 + rem : -1;
     }
 
     int sweep = 0;
@@ -1020,7 +1028,7 @@ __global__ __launch_bounds__(TPB) void k_eig_f64(const double *__restrict__ x, d
                     }
             }
             // V <- V J
-            for (int e = t; e < VITEMS; e += TPB) {
+            for (int e = t - VT0; e >= 0 && e < VITEMS; e += VW) {
                 const int pr = e / N, i = e - (e / N) * N;
                 const double4 R = rot[pr];
                 if (R.y == 0.0 || i >= n) continue;
