@@ -188,10 +188,35 @@ def test_warm_started_eigensolver(n, mesh):
         assert rel(lanes[key], ol[key]) < 1e-10
     warm = mesh[0] * mesh[1] >= 4
     say(f"  sweeps mean {st['mean_sweeps']:.2f} max {st['max_sweeps']} eig launches {st['eig_launches']}")
-    assert st["eig_launches"] == st["batches"] * (2 if warm else 1)
+    # one plane per batch here: the first plane in groups (2 launches), later planes from the previous plane
+    assert st["batches"] == mesh[2]
+    assert st["eig_launches"] == (2 + (mesh[2] - 1) if warm else mesh[2])
     assert 0 < st["mean_sweeps"] <= st["max_sweeps"]
     if mesh[0] >= 48:
         assert st["mean_sweeps"] < st["max_sweeps"] - 0.5
+
+
+@pytest.mark.parametrize("n,mesh", [(32, (6, 5, 7)), (16, (4, 4, 9)), (48, (3, 3, 4))])
+def test_plane_warm_started_eigensolver(n, mesh):
+    """Planes after the first start every point from the eigenvectors of the same point one k3 plane
+    earlier (the other pipeline buffer): parity with the oracle, one eigensolver launch per later plane,
+    fewer sweeps than the cold anchors."""
+    model = synthetic_model(n, 2, 70 + n, position_scale=0.1)
+    kw = dict(photon_energies=np.linspace(0, 5, 80), broadening=0.04)
+    say(f"[plane-warm] n={n} mesh={mesh}")
+    with engine(model, **kw) as eng:
+        lanes, total, st = eng.run_mesh(mesh)
+        lanes2, _, _ = eng.run_mesh(mesh)
+    ol, ot = kloop.integrate_mesh(model, oracle_task(**kw), mesh, batch=64)
+    for key in ("sc", "ic"):
+        r = kloop.spectrum_relative_error(total[key], ot[key])
+        say(f"  {key} worst component rel {r.max():.3e}")
+        assert r.max() < 1e-10
+        assert rel(lanes[key], ol[key]) < 1e-10
+        assert np.array_equal(lanes[key], lanes2[key])
+    say(f"  sweeps mean {st['mean_sweeps']:.2f} max {st['max_sweeps']} eig launches {st['eig_launches']}")
+    assert st["batches"] == mesh[2] and st["eig_launches"] == 2 + (mesh[2] - 1)
+    assert st["mean_sweeps"] <= st["max_sweeps"]
 
 
 @pytest.mark.parametrize("backend,batch,mesh,nbatch", [("direct", 128, (6, 5, 7), 2), ("planes", 128, (6, 5, 7), 7),

@@ -903,6 +903,19 @@ int plan_mesh(wnlqg_engine *g, const int32_t mesh[3], int32_t n_lanes) {
     return WNLQG_OK;
 }
 
+// Eigensolver start of batch bi: 0 every point from the identity; 1 groups of 4 along the fastest
+// axis (anchors from the identity, the rest from their anchor); 2 every point from the same point of
+// the previous k3 plane (whole planes per batch, the previous plane in the other buffer).
+int eig_scheme(const wnlqg_engine *g, long long bi) {
+    const MeshPlan &p = g->plan;
+    const MeshBatch &bt = p.batches[bi];
+    if (!wn_warm_start(g->sh.n, bt.cnt)) return 0;
+    if (p.sep && bi > 0 && bt.k2c == p.mesh[1] && p.batches[bi - 1].k2c == p.mesh[1] && p.batches[bi - 1].cnt == bt.cnt &&
+        p.batches[bi - 1].k3 + 1 == bt.k3)
+        return 2;
+    return 1;
+}
+
 // The kernel sequence of one planned mesh integral, ordered on g->stream (forked to g->stream2 and
 // joined back).  Capture-safe: no allocation, no host synchronization, no memset.  marks (optional)
 // holds WN_MARKS pre-created events per batch: 0 start, 1 k points, 2 interpolation, 8 eigensolver
@@ -956,7 +969,15 @@ int enqueue_mesh(wnlqg_engine *g, std::vector<std::array<hipEvent_t, WN_MARKS>> 
         }
         mark(bi, 2, s0);
         unsigned long long *sweep_total = (unsigned long long *)(g->flags.p + 4);
-        if (wn_warm_start(g->sh.n, cnt)) {
+        const int scheme = eig_scheme(g, bi);
+        if (scheme == 2) {
+            const double2 *uprev = g->u.p + (size_t)((bi - 1) & 1) * g->batch * g->sh.n * g->sh.n;
+            mark(bi, 8, s0);
+            WN_HIP(wn_launch_warm_transform_f64(xb, ub, cnt, g->sh, s0, uprev));
+            mark(bi, 9, s0);
+            WN_HIP(wn_launch_eig_pass_f64(xb, eb, ub, g->flags.p + 1, sweep_total, cnt, g->sh, g->eig_tpb, 2, s0,
+                                          nullptr, uprev));
+        } else if (scheme == 1) {
             WN_HIP(wn_launch_eig_pass_f64(xb, eb, ub, g->flags.p + 1, sweep_total, cnt, g->sh, g->eig_tpb, 0, s0));
             mark(bi, 8, s0);
             WN_HIP(wn_launch_warm_transform_f64(xb, ub, cnt, g->sh, s0));
@@ -1100,11 +1121,12 @@ int32_t wnlqg_run_mesh(wnlqg_engine *g, const int32_t mesh[3], int32_t n_lanes, 
         for (long long bi = 0; bi < nbatch; ++bi) {
             auto &mk = marks[bi];
             hipEventElapsedTime(&ms, mk[1], mk[2]); st.interp_ms += ms;
-            if (wn_warm_start(g->sh.n, g->plan.batches[bi].cnt)) {
+            const int scheme = eig_scheme(g, bi);
+            if (scheme) {
                 hipEventElapsedTime(&ms, mk[2], mk[8]); st.eig_ms += ms;
                 hipEventElapsedTime(&ms, mk[8], mk[9]); st.warm_ms += ms;
                 hipEventElapsedTime(&ms, mk[9], mk[3]); st.eig_ms += ms;
-                st.eig_launches += 2;
+                st.eig_launches += scheme == 1 ? 2 : 1;
             } else {
                 hipEventElapsedTime(&ms, mk[2], mk[3]); st.eig_ms += ms;
                 st.eig_launches += 1;

@@ -106,14 +106,17 @@ hipError_t wn_launch_eig_f64(const double *x, double *e, double2 *u, int *sweeps
 //   pass 1  wn_launch_eig_pass_f64(..., pass = 1): the other points, V started at U_a.
 // pass = -1 solves every point from the identity.  wn_warm_start(n, count) tells whether the warm
 // sequence applies (n >= 16, count >= 4).  sweep_total (nullable) accumulates the sweeps executed.
+//   pass 2  after wn_launch_warm_transform_f64(..., uprev): every point started from uprev[k], the
+//           eigenvectors of the same point of the previous mesh plane.
 hipError_t wn_launch_eig_pass_f64(const double *x, double *e, double2 *u, int *sweeps, unsigned long long *sweep_total,
                                   int count, const WnShape &sh, int tpb, int pass, hipStream_t st,
-                                  double2 *vscratch = nullptr);
+                                  double2 *vscratch = nullptr, const double2 *uprev = nullptr);
 // size in complex elements of the eigensolver / rotation scratch for count points (0 for n <= 64)
 size_t wn_eig_scratch_elems(int n, int count);
 size_t wn_rotate_scratch_elems(int n, int count);
 bool wn_warm_start(int n, int count);
-hipError_t wn_launch_warm_transform_f64(double *x, const double2 *u, int count, const WnShape &sh, hipStream_t st);
+hipError_t wn_launch_warm_transform_f64(double *x, const double2 *u, int count, const WnShape &sh, hipStream_t st,
+                                        const double2 *uprev = nullptr);
 // Clears that are safe under stream capture: p[0 .. n) = 0; flags = {0, 0, 0, 0x7f7f7f7f}.
 hipError_t wn_launch_zero(double *p, long long n, hipStream_t st);
 // status words: [0] pair-gap flag, [1] max sweeps, [2] geometry flags, [3] first failing k,

@@ -836,14 +836,17 @@ __device__ __forceinline__ int wn_warm_k(int b, int pass, int count, int *anchor
     return base + (j < al ? j : j + 1);
 }
 
-// pass < 0: every k point from the identity (grid = count); pass 0 / 1: the warm-start passes above.
+// pass < 0: every k point from the identity (grid = count); pass 0 / 1: the warm-start passes above;
+// pass 2: every k point from uprev[k], the eigenvectors of the same point of the previous mesh plane
+// (channel 0 already transformed by them).
 // VG: V lives in global memory (vscratch, [count][N][N] column-major, so that the threads of one plane
 // update read consecutive rows) because H and V together exceed the LDS for N > 64.
 template <int N, int TPB, bool VG>
 __global__ __launch_bounds__(TPB) void k_eig_f64(const double *__restrict__ x, double *__restrict__ eout,
                                                 double2 *__restrict__ uout, int *__restrict__ sweeps_out,
                                                 unsigned long long *__restrict__ sweep_total, int n, int ldo,
-                                                double tol, int pass, int count, double2 *__restrict__ vscratch) {
+                                                double tol, int pass, int count, double2 *__restrict__ vscratch,
+                                                const double2 *__restrict__ uprev) {
     constexpr int LD = N + 1;
     constexpr int HALF = N / 2;
     constexpr int VITEMS = N * HALF;
@@ -857,7 +860,7 @@ __global__ __launch_bounds__(TPB) void k_eig_f64(const double *__restrict__ x, d
     __shared__ double ev[N];
     __shared__ int rank[N];
     int anchor = -1;
-    const int k = pass < 0 ? (int)blockIdx.x : wn_warm_k(blockIdx.x, pass, count, &anchor);
+    const int k = (pass < 0 || pass == 2) ? (int)blockIdx.x : wn_warm_k(blockIdx.x, pass, count, &anchor);
     if (k >= count) return;
     const int t = threadIdx.x;
 
@@ -866,7 +869,7 @@ __global__ __launch_bounds__(TPB) void k_eig_f64(const double *__restrict__ x, d
     // V[i][j]: row i, column j (the eigenvector index)
     auto V = [&](int i, int j) -> double2 & { return VG ? vg[(size_t)j * N + i] : Vs[i * LD + j]; };
     const double2 *xk = (const double2 *)(x + (size_t)k * ldo);
-    const double2 *ua = anchor >= 0 ? uout + (size_t)anchor * n * n : nullptr;
+    const double2 *ua = pass == 2 ? uprev + (size_t)k * n * n : (anchor >= 0 ? uout + (size_t)anchor * n * n : nullptr);
     for (int e = t; e < N * N; e += TPB) {
         const int i = e / N, j = e - (e / N) * N;
         if (i <= j) {
@@ -1084,15 +1087,17 @@ __global__ __launch_bounds__(TPB) void k_eig_f64(const double *__restrict__ x, d
 }
 
 hipError_t wn_launch_eig_pass_f64(const double *x, double *e, double2 *u, int *sweeps, unsigned long long *sweep_total,
-                                  int count, const WnShape &sh, int tpb, int pass, hipStream_t st, double2 *vscratch) {
+                                  int count, const WnShape &sh, int tpb, int pass, hipStream_t st, double2 *vscratch,
+                                  const double2 *uprev) {
+    if (pass == 2 && (!uprev || !wn_warm_start(sh.n, count))) return hipErrorInvalidValue;
     const int n = sh.n;
     const double tol = 1.0e-15;
     const int groups = (count + WN_WARM_G - 1) / WN_WARM_G;
-    const int grid = pass < 0 ? count : pass == 0 ? groups : groups * (WN_WARM_G - 1);
+    const int grid = (pass < 0 || pass == 2) ? count : pass == 0 ? groups : groups * (WN_WARM_G - 1);
     // tpb: threads per k point (64, 128 or 256); 0 selects the default of the size class
 #define WN_EIG_LAUNCH(NN, T) \
     hipLaunchKernelGGL((k_eig_f64<NN, T, false>), dim3(grid), dim3(T), 0, st, x, e, u, sweeps, sweep_total, n, sh.ldo, \
-                       tol, pass, count, nullptr)
+                       tol, pass, count, nullptr, uprev)
 #define WN_EIG_CASE(NN, DEF)                   \
     if (n <= NN) {                             \
         const int t = tpb ? tpb : DEF;         \
@@ -1117,10 +1122,10 @@ hipError_t wn_launch_eig_pass_f64(const double *x, double *e, double2 *u, int *s
         if (!vscratch) return hipErrorInvalidValue;
         if (n <= 96)
             hipLaunchKernelGGL((k_eig_f64<96, 512, true>), dim3(grid), dim3(512), 0, st, x, e, u, sweeps, sweep_total, n,
-                               sh.ldo, tol, pass, count, vscratch);
+                               sh.ldo, tol, pass, count, vscratch, nullptr);
         else
             hipLaunchKernelGGL((k_eig_f64<128, 512, true>), dim3(grid), dim3(512), 0, st, x, e, u, sweeps, sweep_total,
-                               n, sh.ldo, tol, pass, count, vscratch);
+                               n, sh.ldo, tol, pass, count, vscratch, nullptr);
         return hipGetLastError();
     }
     return hipErrorInvalidValue;
@@ -1211,15 +1216,18 @@ __global__ __launch_bounds__(256) void k_rotate_f64(double *__restrict__ x, cons
 //   Y = T U:    Yr = Tr Ur - Ti Ui,      Yi = Tr Ui + Ti Ur
 // with U, X (then T) in LDS as planar real arrays padded to np = 16*ceil(n/16) (zeros beyond n).
 // The (np/16)^2 output tiles are dealt to the 4 waves.
-// WARM: the warm-start transform of the mesh eigensolver: channel 0 of the pass-1 k points replaced by
-// U_a^+ H U_a with the group anchor's eigenvectors U_a.  Otherwise channels 1 .. C-1 of every k point.
+// WARM: the warm-start transform of the mesh eigensolver: channel 0 replaced by W^+ H W, with W the
+// group anchor's eigenvectors for the pass-1 points (uw = nullptr), or W = uw[k], the eigenvectors of
+// the same point of the previous mesh plane, for every point (uw set).  Otherwise channels 1 .. C-1 of
+// every k point with its own U.
 template <int NP, int PF, bool WARM>
 __global__ __launch_bounds__(256) void k_rotate_mfma(double *__restrict__ x, const double2 *__restrict__ u, int n,
-                                                    int C, int ldo, int count) {
+                                                    int C, int ldo, int count, const double2 *__restrict__ uw) {
     extern __shared__ double smd[];
     int ka = blockIdx.x;
-    const int k = WARM ? wn_warm_k(blockIdx.x, 1, count, &ka) : (int)blockIdx.x;
+    const int k = (WARM && !uw) ? wn_warm_k(blockIdx.x, 1, count, &ka) : (int)blockIdx.x;
     if (WARM && k >= count) return;
+    if (WARM && uw) u = uw;
     const int cbeg = WARM ? 0 : 1, cend = WARM ? 1 : C;
     const int t = threadIdx.x;
     const int lane = t & 63, wave = t >> 6;
@@ -1392,13 +1400,15 @@ size_t wn_eig_scratch_elems(int n, int count) {
 size_t wn_rotate_scratch_elems(int n, int count) { return wn_eig_scratch_elems(n, count); }
 
 template <bool WARM>
-static void launch_rotate_mfma(double *x, const double2 *u, int count, const WnShape &sh, hipStream_t st) {
+static void launch_rotate_mfma(double *x, const double2 *u, int count, const WnShape &sh, hipStream_t st,
+                               const double2 *uw = nullptr) {
     const int n = sh.n;
     const int np = (n + 15) & ~15;
     const size_t lds = (size_t)4 * np * (np + 1) * sizeof(double);
-    const int grid = WARM ? (count + WN_WARM_G - 1) / WN_WARM_G * (WN_WARM_G - 1) : count;
-#define WN_ROT(NPV, PFV) \
-    hipLaunchKernelGGL((k_rotate_mfma<NPV, PFV, WARM>), dim3(grid), dim3(256), lds, st, x, u, n, sh.C, sh.ldo, count)
+    const int grid = (WARM && !uw) ? (count + WN_WARM_G - 1) / WN_WARM_G * (WN_WARM_G - 1) : count;
+#define WN_ROT(NPV, PFV)                                                                                       \
+    hipLaunchKernelGGL((k_rotate_mfma<NPV, PFV, WARM>), dim3(grid), dim3(256), lds, st, x, u, n, sh.C, sh.ldo, count, \
+                       uw)
     if (np == 16)
         WN_ROT(16, 1);
     else if (np == 32)
@@ -1412,14 +1422,15 @@ static void launch_rotate_mfma(double *x, const double2 *u, int count, const WnS
 
 hipError_t wn_launch_eig_f64(const double *x, double *e, double2 *u, int *sweeps, int count, const WnShape &sh,
                              int tpb, hipStream_t st, double2 *vscratch) {
-    return wn_launch_eig_pass_f64(x, e, u, sweeps, nullptr, count, sh, tpb, -1, st, vscratch);
+    return wn_launch_eig_pass_f64(x, e, u, sweeps, nullptr, count, sh, tpb, -1, st, vscratch, nullptr);
 }
 
 bool wn_warm_start(int n, int count) { return n >= 16 && n <= 64 && count >= WN_WARM_G; }
 
-hipError_t wn_launch_warm_transform_f64(double *x, const double2 *u, int count, const WnShape &sh, hipStream_t st) {
+hipError_t wn_launch_warm_transform_f64(double *x, const double2 *u, int count, const WnShape &sh, hipStream_t st,
+                                        const double2 *uprev) {
     if (!wn_warm_start(sh.n, count)) return hipErrorInvalidValue;
-    launch_rotate_mfma<true>(x, u, count, sh, st);
+    launch_rotate_mfma<true>(x, u, count, sh, st, uprev);
     return hipGetLastError();
 }
 
