@@ -1465,90 +1465,18 @@ hipError_t wn_launch_rotate_f64(double *x, const double2 *u, int count, const Wn
 //   SC: Q[a+3b+9c] = -i (G_nm[c][a] B_mn[b] - G_mn[b][a] B_nm[c]),   B = A_H + i D
 //   IC: Q[27*sc + a+3b+9c] = (V_mm^a - V_nn^a) B_nm[c] B_mn[b].
 // ------------------------------------------------------------------------------------------------
+// Channels of one k point (channel c at xk + c n^2), addressed by arithmetic rather than a pointer
+// table, so that runtime component indices do not force the table into scratch memory.
 struct PairView {
-    const double2 *V[3];
-    const double2 *W[6];
-    const double2 *A[3];
-    const double2 *dA[9];
+    const double2 *xk;
+    int n2;
     int has_pos;
+    __device__ __forceinline__ const double2 *V(int y) const { return xk + (size_t)(WN_CH_DH + y) * n2; }
+    __device__ __forceinline__ const double2 *W(int s) const { return xk + (size_t)(WN_CH_DDH + s) * n2; }
+    __device__ __forceinline__ const double2 *A(int y) const { return xk + (size_t)(WN_CH_A + y) * n2; }
+    __device__ __forceinline__ const double2 *dA(int i) const { return xk + (size_t)(WN_CH_DA + i) * n2; }
 };
 
-__device__ __forceinline__ double2 ld2(const double2 *p, int idx) { return p[idx]; }
-
-__device__ void wn_gderiv(const PairView &pv, const double *inv, int n, int nb, int m, double2 G[3][3]) {
-    // r^b_{nm;a} for all (a = x, b = y); ConventionalShiftCurrent.jl:191-229
-    double2 sAD[3][3], sHD[3][3];
-#pragma unroll
-    for (int x = 0; x < 3; ++x)
-#pragma unroll
-        for (int y = 0; y < 3; ++y) {
-            sAD[x][y] = make_double2(0.0, 0.0);
-            sHD[x][y] = make_double2(0.0, 0.0);
-        }
-    for (int l = 0; l < nb; ++l) {
-        if (l == n || l == m) continue;
-        const double inl = inv[n * nb + l];
-        const double ilm = inv[l * nb + m];
-        double2 Vnl[3], Vlm[3], Anl[3], Alm[3], Dnl[3], Dlm[3];
-#pragma unroll
-        for (int x = 0; x < 3; ++x) {
-            Vnl[x] = pv.V[x][n * nb + l];
-            Vlm[x] = pv.V[x][l * nb + m];
-            Dnl[x] = cscale(Vnl[x], -inl);
-            Dlm[x] = cscale(Vlm[x], -ilm);
-            if (pv.has_pos) {
-                Anl[x] = pv.A[x][n * nb + l];
-                Alm[x] = pv.A[x][l * nb + m];
-            } else {
-                Anl[x] = make_double2(0.0, 0.0);
-                Alm[x] = make_double2(0.0, 0.0);
-            }
-        }
-#pragma unroll
-        for (int x = 0; x < 3; ++x)
-#pragma unroll
-            for (int y = 0; y < 3; ++y) {
-                sAD[x][y] = csub(cfma(Anl[x], Dlm[y], sAD[x][y]), cmul(Dnl[y], Alm[x]));
-                sHD[x][y] = csub(cfma(Vnl[x], Dlm[y], sHD[x][y]), cmul(Dnl[y], Vlm[x]));
-            }
-    }
-    const double inm = inv[n * nb + m];
-    double2 Dnm[3], Ann[3], Amm[3], Anm[3], Vnn[3], Vmm[3];
-#pragma unroll
-    for (int x = 0; x < 3; ++x) {
-        double2 v = pv.V[x][n * nb + m];
-        Dnm[x] = cscale(v, -inm);
-        Vnn[x] = pv.V[x][n * nb + n];
-        Vmm[x] = pv.V[x][m * nb + m];
-        if (pv.has_pos) {
-            Ann[x] = pv.A[x][n * nb + n];
-            Amm[x] = pv.A[x][m * nb + m];
-            Anm[x] = pv.A[x][n * nb + m];
-        } else {
-            Ann[x] = Amm[x] = Anm[x] = make_double2(0.0, 0.0);
-        }
-    }
-#pragma unroll
-    for (int x = 0; x < 3; ++x)
-#pragma unroll
-        for (int y = 0; y < 3; ++y) {
-            double2 dAy = csub(Ann[y], Amm[y]), dAx = csub(Ann[x], Amm[x]);
-            double2 dVy = csub(Vnn[y], Vmm[y]), dVx = csub(Vnn[x], Vmm[x]);
-            double2 ad_bit = cadd(cmul(dAy, Dnm[x]), cmul(dAx, Dnm[y]));
-            double2 aa_bit = cmul(dAy, Anm[x]);
-            double2 dv_bit = cadd(cmul(dVy, Dnm[x]), cmul(dVx, Dnm[y]));
-            double2 da = pv.has_pos ? pv.dA[3 * x + y][n * nb + m] : make_double2(0.0, 0.0);
-            double2 w = pv.W[wn_sym(x, y)][n * nb + m];
-            double2 g = cadd(da, ad_bit);
-            g = csub(g, cmuli(aa_bit));
-            g = cadd(g, sAD[x][y]);
-            double2 h = cadd(cadd(w, sHD[x][y]), dv_bit);
-            g = cadd(g, cscale(cmuli(h), -inm));
-            G[x][y] = g;
-        }
-}
-
-// r^y_{nm;x} for y = 0..2 at fixed x (one row of wn_gderiv); ConventionalShiftCurrent.jl:191-229.
 __device__ void wn_gderiv_row(const PairView &pv, const double *inv, int n, int nb, int m, int x, double2 g[3]) {
     double2 sAD[3], sHD[3];
 #pragma unroll
@@ -1556,8 +1484,8 @@ __device__ void wn_gderiv_row(const PairView &pv, const double *inv, int n, int 
         sAD[y] = make_double2(0.0, 0.0);
         sHD[y] = make_double2(0.0, 0.0);
     }
-    const double2 *Vx = pv.V[x];
-    const double2 *Ax = pv.has_pos ? pv.A[x] : nullptr;
+    const double2 *Vx = pv.V(x);
+    const double2 *Ax = pv.has_pos ? pv.A(x) : nullptr;
     for (int l = 0; l < nb; ++l) {
         if (l == n || l == m) continue;
         const double inl = inv[n * nb + l];
@@ -1567,8 +1495,8 @@ __device__ void wn_gderiv_row(const PairView &pv, const double *inv, int n, int 
         const double2 axlm = Ax ? Ax[l * nb + m] : make_double2(0.0, 0.0);
 #pragma unroll
         for (int y = 0; y < 3; ++y) {
-            const double2 dnl = cscale(pv.V[y][n * nb + l], -inl);
-            const double2 dlm = cscale(pv.V[y][l * nb + m], -ilm);
+            const double2 dnl = cscale(pv.V(y)[n * nb + l], -inl);
+            const double2 dlm = cscale(pv.V(y)[l * nb + m], -ilm);
             sAD[y] = csub(cfma(axnl, dlm, sAD[y]), cmul(dnl, axlm));
             sHD[y] = csub(cfma(vxnl, dlm, sHD[y]), cmul(dnl, vxlm));
         }
@@ -1577,18 +1505,23 @@ __device__ void wn_gderiv_row(const PairView &pv, const double *inv, int n, int 
     double2 Dnm[3], dA[3], dV[3];
 #pragma unroll
     for (int y = 0; y < 3; ++y) {
-        Dnm[y] = cscale(pv.V[y][n * nb + m], -inm);
-        dV[y] = csub(pv.V[y][n * nb + n], pv.V[y][m * nb + m]);
-        dA[y] = pv.has_pos ? csub(pv.A[y][n * nb + n], pv.A[y][m * nb + m]) : make_double2(0.0, 0.0);
+        Dnm[y] = cscale(pv.V(y)[n * nb + m], -inm);
+        dV[y] = csub(pv.V(y)[n * nb + n], pv.V(y)[m * nb + m]);
+        dA[y] = pv.has_pos ? csub(pv.A(y)[n * nb + n], pv.A(y)[m * nb + m]) : make_double2(0.0, 0.0);
     }
-    const double2 Anm = pv.has_pos ? pv.A[x][n * nb + m] : make_double2(0.0, 0.0);
+    const double2 Anm = pv.has_pos ? pv.A(x)[n * nb + m] : make_double2(0.0, 0.0);
+    // the x entries, read again rather than indexed with the runtime x (which would put the arrays in
+    // scratch memory)
+    const double2 Dnm_x = cscale(Vx[n * nb + m], -inm);
+    const double2 dV_x = csub(Vx[n * nb + n], Vx[m * nb + m]);
+    const double2 dA_x = pv.has_pos ? csub(Ax[n * nb + n], Ax[m * nb + m]) : make_double2(0.0, 0.0);
 #pragma unroll
     for (int y = 0; y < 3; ++y) {
-        const double2 ad_bit = cadd(cmul(dA[y], Dnm[x]), cmul(dA[x], Dnm[y]));
+        const double2 ad_bit = cadd(cmul(dA[y], Dnm_x), cmul(dA_x, Dnm[y]));
         const double2 aa_bit = cmul(dA[y], Anm);
-        const double2 dv_bit = cadd(cmul(dV[y], Dnm[x]), cmul(dV[x], Dnm[y]));
-        const double2 da = pv.has_pos ? pv.dA[3 * x + y][n * nb + m] : make_double2(0.0, 0.0);
-        const double2 w = pv.W[wn_sym(x, y)][n * nb + m];
+        const double2 dv_bit = cadd(cmul(dV[y], Dnm_x), cmul(dV_x, Dnm[y]));
+        const double2 da = pv.has_pos ? pv.dA(3 * x + y)[n * nb + m] : make_double2(0.0, 0.0);
+        const double2 w = pv.W(wn_sym(x, y))[n * nb + m];
         double2 gg = cadd(da, ad_bit);
         gg = csub(gg, cmuli(aa_bit));
         gg = cadd(gg, sAD[y]);
@@ -1597,7 +1530,7 @@ __device__ void wn_gderiv_row(const PairView &pv, const double *inv, int n, int 
     }
 }
 
-__global__ __launch_bounds__(256) void k_pairs_f64(const double *__restrict__ x, const double *__restrict__ eg,
+__global__ __launch_bounds__(256, 4) void k_pairs_f64(const double *__restrict__ x, const double *__restrict__ eg,
                                                   const double *__restrict__ omega, int *__restrict__ count,
                                                   int4 *__restrict__ pmeta, double2 *__restrict__ pmd,
                                                   double2 *__restrict__ qout, int *__restrict__ flags, WnShape sh,
@@ -1736,15 +1669,9 @@ __global__ __launch_bounds__(256) void k_pairs_f64(const double *__restrict__ x,
 
     const double2 *xk = (const double2 *)(x + (size_t)k * sh.ldo);
     PairView pv;
+    pv.xk = xk;
+    pv.n2 = n2;
     pv.has_pos = sh.has_pos;
-#pragma unroll
-    for (int a = 0; a < 3; ++a) pv.V[a] = xk + (size_t)(WN_CH_DH + a) * n2;
-#pragma unroll
-    for (int a = 0; a < 6; ++a) pv.W[a] = xk + (size_t)(WN_CH_DDH + a) * n2;
-#pragma unroll
-    for (int a = 0; a < 3; ++a) pv.A[a] = sh.has_pos ? xk + (size_t)(WN_CH_A + a) * n2 : nullptr;
-#pragma unroll
-    for (int a = 0; a < 9; ++a) pv.dA[a] = sh.has_pos ? xk + (size_t)(WN_CH_DA + a) * n2 : nullptr;
     const int ncomp = 27 * sc.nq;
     // Chunks of WN_PCH pairs.  Phase A: one item per (pair, direction d, component x) forms
     // r^y_{n'm';x} for y = 0..2 with (n', m') = (n, m) or (m, n); one item per pair also stages the
@@ -1768,13 +1695,13 @@ __global__ __launch_bounds__(256) void k_pairs_f64(const double *__restrict__ x,
                 const double imn = inv[m * nb + n];
 #pragma unroll
                 for (int a = 0; a < 3; ++a) {
-                    const double2 anm = sh.has_pos ? pv.A[a][n * nb + m] : make_double2(0.0, 0.0);
-                    const double2 amn = sh.has_pos ? pv.A[a][m * nb + n] : make_double2(0.0, 0.0);
-                    const double2 dnm = cscale(pv.V[a][n * nb + m], -inm);
-                    const double2 dmn = cscale(pv.V[a][m * nb + n], -imn);
+                    const double2 anm = sh.has_pos ? pv.A(a)[n * nb + m] : make_double2(0.0, 0.0);
+                    const double2 amn = sh.has_pos ? pv.A(a)[m * nb + n] : make_double2(0.0, 0.0);
+                    const double2 dnm = cscale(pv.V(a)[n * nb + m], -inm);
+                    const double2 dmn = cscale(pv.V(a)[m * nb + n], -imn);
                     Bs[i][0][a] = cadd(anm, cmuli(dnm));
                     Bs[i][1][a] = cadd(amn, cmuli(dmn));
-                    Vd[i][a] = csub(pv.V[a][m * nb + m], pv.V[a][n * nb + n]);
+                    Vd[i][a] = csub(pv.V(a)[m * nb + m], pv.V(a)[n * nb + n]);
                 }
             }
         }
