@@ -53,20 +53,24 @@ __device__ inline double shg_principal(double x, double w) { return x / (x * x +
 
 __device__ inline double shg_weight(double d, double eta) { return (eta * eta) / (d * d + eta * eta); }
 
+// Channels of one k point (channel c at xk + c n^2), addressed by arithmetic rather than a pointer
+// table, so that runtime component indices do not force the table into scratch memory.
 struct ShgView {
-    const double2 *A[3];   // A_H (nullptr without a position operator)
-    const double2 *V[3];   // dH_H
-    const double2 *dA[9];  // d_b A_a at 3a + b
-    const double2 *W[6];   // ddH_H, unique (xx, yy, zz, yz, xz, xy)
+    const double2 *xk;
     const double *E;
-    int n;
+    int n, n2;
+    int pos;               // position operator present (A and dA channels)
     double eta;
+    __device__ __forceinline__ const double2 *A(int a) const { return xk + (size_t)(WN_CH_A + a) * n2; }
+    __device__ __forceinline__ const double2 *V(int a) const { return xk + (size_t)(WN_CH_DH + a) * n2; }
+    __device__ __forceinline__ const double2 *dA(int i) const { return xk + (size_t)(WN_CH_DA + i) * n2; }  // d_b A_a: 3a + b
+    __device__ __forceinline__ const double2 *W(int s) const { return xk + (size_t)(WN_CH_DDH + s) * n2; }  // xx yy zz yz xz xy
 };
 
 __device__ inline double2 sv_A(const ShgView &s, int a, int i, int j) {
-    return s.A[0] ? s.A[a][i * s.n + j] : make_double2(0.0, 0.0);
+    return s.pos ? s.A(a)[i * s.n + j] : make_double2(0.0, 0.0);
 }
-__device__ inline double2 sv_V(const ShgView &s, int a, int i, int j) { return s.V[a][i * s.n + j]; }
+__device__ inline double2 sv_V(const ShgView &s, int a, int i, int j) { return s.V(a)[i * s.n + j]; }
 // bare vertex part V_ij / (E_j - E_i) (cut) and regularized part
 __device__ inline double2 sv_bare(const ShgView &s, int a, int i, int j) {
     return cscale(sv_V(s, a, i, j), shg_bare_inverse(s.E[j] - s.E[i]));
@@ -84,16 +88,12 @@ __device__ inline double2 sv_rdip(const ShgView &s, int a, int i, int j) {
 
 __device__ inline ShgView make_view(const double *x, const double *e, long long k, const WnShape &sh, double eta) {
     ShgView s;
-    const int n = sh.n, n2 = n * n;
-    const double2 *xk = (const double2 *)(x + (size_t)k * sh.ldo);
-    for (int a = 0; a < 3; ++a) {
-        s.V[a] = xk + (size_t)(WN_CH_DH + a) * n2;
-        s.A[a] = sh.has_pos ? xk + (size_t)(WN_CH_A + a) * n2 : nullptr;
-    }
-    for (int a = 0; a < 9; ++a) s.dA[a] = sh.has_pos ? xk + (size_t)(WN_CH_DA + a) * n2 : nullptr;
-    for (int a = 0; a < 6; ++a) s.W[a] = xk + (size_t)(WN_CH_DDH + a) * n2;
+    const int n = sh.n;
+    s.xk = (const double2 *)(x + (size_t)k * sh.ldo);
     s.E = e + (size_t)k * n;
     s.n = n;
+    s.n2 = n * n;
+    s.pos = sh.has_pos;
     s.eta = eta;
     return s;
 }
@@ -117,7 +117,7 @@ __global__ __launch_bounds__(WN_SHG_TPB) void k_shg_der(const double *__restrict
     if (t < n) occ[t] = shg_occupation(s.E[t], sp.mu, sp.temperature);
     __syncthreads();
     double2 *out = der + ((size_t)k * 9 + ab) * n2;
-    const bool pos = s.A[0] != nullptr;
+    const bool pos = s.pos != 0;
     for (int e = t; e < n2; e += WN_SHG_TPB) {
         const int nn = e / n, m = e - (e / n) * n;
         if (!shg_active(occ, nn, m)) {
@@ -154,12 +154,12 @@ __global__ __launch_bounds__(WN_SHG_TPB) void k_shg_der(const double *__restrict
         const double2 dAa = csub(Aan, Aam), dAb = csub(Abn, Abm);
         const double2 sum_a = csub(ca, cmul(dAa, reg_b));
         const double2 sum_h = csub(ch, cmul(csub(Van, Vam), reg_b));
-        const double2 dAab = pos ? s.dA[3 * a + b][nn * n + m] : make_double2(0.0, 0.0);
+        const double2 dAab = pos ? s.dA(3 * a + b)[nn * n + m] : make_double2(0.0, 0.0);
         double2 finite = cadd(dAab, sum_a);
         finite = cadd(finite, cmul(dAa, bare_b));
         finite = cadd(finite, cmul(dAb, bare_a));
         finite = csub(finite, cmuli(cmul(sv_A(s, a, nn, m), dAb)));
-        const double2 Wab = s.W[wn_sym(a, b)][nn * n + m];
+        const double2 Wab = s.W(wn_sym(a, b))[nn * n + m];
         double2 inner = cadd(Wab, sum_h);
         inner = cadd(inner, cscale(bare_a, Vbn.x - Vbm.x));
         inner = cadd(inner, cscale(bare_b, Van.x - Vam.x));
@@ -296,7 +296,7 @@ __global__ __launch_bounds__(WN_SHG_TPB) void k_shg_coef(const double *__restric
             const bool isolated = (nn == 0 || s.E[nn] - s.E[nn - 1] > sp.tol) && (nn == n - 1 || s.E[nn + 1] - s.E[nn] > sp.tol);
             const int cc = sp.comp[ci];
             const int a = cc % 3, b = (cc / 3) % 3, c = cc / 9;
-            double hess = s.W[wn_sym(b, c)][nn * n + nn].x;
+            double hess = s.W(wn_sym(b, c))[nn * n + nn].x;
             for (int q = 0; q < n; ++q) {
                 if (q == nn) continue;
                 const double2 pr = cmul(sv_V(s, b, nn, q), sv_V(s, c, q, nn));
@@ -311,7 +311,7 @@ __global__ __launch_bounds__(WN_SHG_TPB) void k_shg_coef(const double *__restric
                 for (int u = 0; u < 2; ++u) {
                     const int y = ys[u];
                     if (y == a) continue;
-                    double2 val = s.dA[0] ? csub(s.dA[3 * y + a][nn * n + nn], s.dA[3 * a + y][nn * n + nn])
+                    double2 val = s.pos ? csub(s.dA(3 * y + a)[nn * n + nn], s.dA(3 * a + y)[nn * n + nn])
                                           : make_double2(0.0, 0.0);
                     for (int q = 0; q < n; ++q) {
                         const double2 ba_qn = sv_bare(s, a, q, nn), ba_nq = sv_bare(s, a, nn, q);
