@@ -188,26 +188,27 @@ def test_warm_started_eigensolver(n, mesh):
         assert rel(lanes[key], ol[key]) < 1e-10
     warm = mesh[0] * mesh[1] >= 4
     say(f"  sweeps mean {st['mean_sweeps']:.2f} max {st['max_sweeps']} eig launches {st['eig_launches']}")
-    # one plane per batch here: the first plane in groups (2 launches), later planes from the previous plane
-    assert st["batches"] == mesh[2]
-    assert st["eig_launches"] == (2 + (mesh[2] - 1) if warm else mesh[2])
+    # these small meshes fit one batch (planes grouped): every batch in groups of 4
+    assert st["batches"] == 1
+    assert st["eig_launches"] == (2 if warm else 1)
     assert 0 < st["mean_sweeps"] <= st["max_sweeps"]
     if mesh[0] >= 48:
         assert st["mean_sweeps"] < st["max_sweeps"] - 0.5
 
 
-@pytest.mark.parametrize("n,mesh", [(32, (6, 5, 7)), (16, (4, 4, 9)), (48, (3, 3, 4))])
+@pytest.mark.parametrize("n,mesh", [(32, (10, 10, 5)), (16, (12, 8, 6)), (48, (9, 9, 4))])
 def test_plane_warm_started_eigensolver(n, mesh):
-    """Planes after the first start every point from the eigenvectors of the same point one k3 plane
-    earlier (the other pipeline buffer): parity with the oracle, one eigensolver launch per later plane,
-    fewer sweeps than the cold anchors."""
+    """One plane per batch (planes of 64-128 points with 128-point batches): planes after the first start
+    every point from the eigenvectors of the same point one k3 plane earlier (the other pipeline buffer):
+    parity with the oracle, one eigensolver launch per later plane."""
     model = synthetic_model(n, 2, 70 + n, position_scale=0.1)
-    kw = dict(photon_energies=np.linspace(0, 5, 80), broadening=0.04)
+    kw = dict(photon_energies=np.linspace(0, 5, 80), broadening=0.04, batch_size=128)
     say(f"[plane-warm] n={n} mesh={mesh}")
     with engine(model, **kw) as eng:
         lanes, total, st = eng.run_mesh(mesh)
         lanes2, _, _ = eng.run_mesh(mesh)
-    ol, ot = kloop.integrate_mesh(model, oracle_task(**kw), mesh, batch=64)
+    okw = {k: v for k, v in kw.items() if k != "batch_size"}
+    ol, ot = kloop.integrate_mesh(model, oracle_task(**okw), mesh, batch=64)
     for key in ("sc", "ic"):
         r = kloop.spectrum_relative_error(total[key], ot[key])
         say(f"  {key} worst component rel {r.max():.3e}")
@@ -219,11 +220,12 @@ def test_plane_warm_started_eigensolver(n, mesh):
     assert st["mean_sweeps"] <= st["max_sweeps"]
 
 
-@pytest.mark.parametrize("backend,batch,mesh,nbatch", [("direct", 128, (6, 5, 7), 2), ("planes", 128, (6, 5, 7), 7),
+@pytest.mark.parametrize("backend,batch,mesh,nbatch", [("direct", 128, (6, 5, 7), 2), ("planes", 128, (6, 5, 7), 2),
                                                       ("planes", 128, (40, 6, 3), 6)])
 def test_mesh_lane_counts_and_batches(backend, batch, mesh, nbatch):
     """Results do not depend on the lane count beyond rounding; small batches exercise the
-    multi-batch chunk plan (lanes straddling batches, k2 sub-ranges of a plane)."""
+    multi-batch chunk plan (lanes straddling batches, several planes per batch (4 + 3 planes of 30
+    points), k2 sub-ranges of a plane)."""
     model = MODELS["cfg0_n8"]()
     kw = dict(photon_energies=np.linspace(0, 5, 60), broadening=0.04, batch_size=batch, fourier_backend=backend)
     say(f"[mesh] cfg0 {mesh} backend {backend} batch {batch}, lanes 64 vs 7")

@@ -75,11 +75,16 @@ struct DevBuf {
     }
 };
 
-// One batch of a mesh: k range [k0, k0 + cnt); separable: k3 plane and k2 range [k2b, k2b + k2c).
+// One batch of a mesh: k range [k0, k0 + cnt); separable: planes [k3, k3 + nk3) with k2 range
+// [k2b, k2b + k2c) (several whole planes, or part of one plane).
 struct MeshBatch {
     long long k0;
-    int cnt, k3, k2b, k2c;
+    int cnt, k3, k2b, k2c, nk3;
 };
+
+// Separable meshes group small k3 planes into batches of at least this many points, so that the
+// one-workgroup-per-k kernels fill the GPU.
+#define WN_MIN_MESH_BATCH 16384
 
 // Device tables of one mesh integral, kept between calls with the same mesh and lane count.
 struct MeshPlan {
@@ -845,11 +850,21 @@ int plan_mesh(wnlqg_engine *g, const int32_t mesh[3], int32_t n_lanes) {
     p.sep = g->separable && mesh[0] <= g->batch;
     if (p.sep) {
         const int k2step = std::max(1, std::min(mesh[1], g->batch / mesh[0]));
-        for (int k3 = 0; k3 < mesh[2]; ++k3)
-            for (int k2b = 0; k2b < mesh[1]; k2b += k2step) {
-                const int k2c = std::min(k2step, mesh[1] - k2b);
-                p.batches.push_back({((long long)k3 * mesh[1] + k2b) * mesh[0], k2c * mesh[0], k3, k2b, k2c});
+        const long long plane = (long long)mesh[0] * mesh[1];
+        if (k2step == mesh[1]) {
+            const int pp = (int)std::max<long long>(1, std::min<long long>(g->batch / plane,
+                                                                           (WN_MIN_MESH_BATCH + plane - 1) / plane));
+            for (int k3 = 0; k3 < mesh[2]; k3 += pp) {
+                const int nk3 = std::min(pp, mesh[2] - k3);
+                p.batches.push_back({(long long)k3 * plane, (int)(nk3 * plane), k3, 0, mesh[1], nk3});
             }
+        } else {
+            for (int k3 = 0; k3 < mesh[2]; ++k3)
+                for (int k2b = 0; k2b < mesh[1]; k2b += k2step) {
+                    const int k2c = std::min(k2step, mesh[1] - k2b);
+                    p.batches.push_back({((long long)k3 * mesh[1] + k2b) * mesh[0], k2c * mesh[0], k3, k2b, k2c, 1});
+                }
+        }
         const size_t m2 = g->sh.ldo / 2;
         if (g->y23.n < (size_t)g->span[0] * k2step * m2) WN_HIP(g->y23.alloc((size_t)g->span[0] * k2step * m2));
         WN_HIP(g->e1.alloc((size_t)mesh[0] * g->span[0]));
@@ -860,7 +875,7 @@ int plan_mesh(wnlqg_engine *g, const int32_t mesh[3], int32_t n_lanes) {
         WN_HIP(wn_launch_axis_phases(g->e3.p, mesh[2], g->span[2], g->rmin[2], g->stream));
     } else {
         for (long long k0 = 0; k0 < nk; k0 += g->batch)
-            p.batches.push_back({k0, (int)std::min<long long>(g->batch, nk - k0), 0, 0, 0});
+            p.batches.push_back({k0, (int)std::min<long long>(g->batch, nk - k0), 0, 0, 0, 0});
     }
     const long long nbatch = (long long)p.batches.size();
     std::vector<int4> all_chunks, all_lanes;
@@ -910,8 +925,8 @@ int eig_scheme(const wnlqg_engine *g, long long bi) {
     const MeshPlan &p = g->plan;
     const MeshBatch &bt = p.batches[bi];
     if (!wn_warm_start(g->sh.n, bt.cnt)) return 0;
-    if (p.sep && bi > 0 && bt.k2c == p.mesh[1] && p.batches[bi - 1].k2c == p.mesh[1] && p.batches[bi - 1].cnt == bt.cnt &&
-        p.batches[bi - 1].k3 + 1 == bt.k3)
+    if (p.sep && bi > 0 && bt.nk3 == 1 && p.batches[bi - 1].nk3 == 1 && bt.k2c == p.mesh[1] &&
+        p.batches[bi - 1].k2c == p.mesh[1] && p.batches[bi - 1].cnt == bt.cnt && p.batches[bi - 1].k3 + 1 == bt.k3)
         return 2;
     return 1;
 }
@@ -956,13 +971,17 @@ int enqueue_mesh(wnlqg_engine *g, std::vector<std::array<hipEvent_t, WN_MARKS>> 
         mark(bi, 1, s0);
         if (p.sep) {
             const double2 *boxc = (const double2 *)g->box.p;
-            if (bt.k2b == 0)
-                WN_HIP(wn_launch_contract(boxc, g->y3.p, g->e3.p, 1, bt.k3, g->span[2], (int)m2,
-                                          g->span[0] * g->span[1], g->span[2] * m2, m2, m2, 0, s0));
-            WN_HIP(wn_launch_contract(g->y3.p, g->y23.p, g->e2.p, bt.k2c, bt.k2b, g->span[1], (int)m2, g->span[0],
-                                      g->span[1] * m2, m2, (long long)bt.k2c * m2, m2, s0));
-            WN_HIP(wn_launch_contract(g->y23.p, (double2 *)xb, g->e1.p, p.mesh[0], 0, g->span[0], (int)m2, bt.k2c, m2,
-                                      (long long)bt.k2c * m2, (long long)p.mesh[0] * m2, m2, s0));
+            for (int j = 0; j < bt.nk3; ++j) {
+                // plane k3 + j of the batch: its k2 range [k2b, k2b + k2c) starts at row j k2c N1
+                double2 *xj = (double2 *)xb + (size_t)j * bt.k2c * p.mesh[0] * m2;
+                if (bt.k2b == 0)
+                    WN_HIP(wn_launch_contract(boxc, g->y3.p, g->e3.p, 1, bt.k3 + j, g->span[2], (int)m2,
+                                              g->span[0] * g->span[1], g->span[2] * m2, m2, m2, 0, s0));
+                WN_HIP(wn_launch_contract(g->y3.p, g->y23.p, g->e2.p, bt.k2c, bt.k2b, g->span[1], (int)m2, g->span[0],
+                                          g->span[1] * m2, m2, (long long)bt.k2c * m2, m2, s0));
+                WN_HIP(wn_launch_contract(g->y23.p, xj, g->e1.p, p.mesh[0], 0, g->span[0], (int)m2, bt.k2c, m2,
+                                          (long long)bt.k2c * m2, (long long)p.mesh[0] * m2, m2, s0));
+            }
             if (interp_bytes) *interp_bytes += (double)cnt * g->sh.ldo * 8.0;
         } else {
             WN_HIP(launch_interp(g, rows, xb));

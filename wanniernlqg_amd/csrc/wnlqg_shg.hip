@@ -360,7 +360,7 @@ __device__ inline double2 shg_resonance(double xx, double width, int gaussian) {
 // are staged through LDS in blocks of `pb` pairs (all k points of the chunk in order).
 // ------------------------------------------------------------------------------------------------
 #define WN_SHG_MAXK 64   // k points per chunk
-template <int MAXT>
+template <int MAXT, int RTW>
 __global__ __launch_bounds__(WN_SHG_TPB) void k_shg_mfma(const int4 *__restrict__ chunks, const int *__restrict__ count,
                                                         const double *__restrict__ pdiff,
                                                         const double2 *__restrict__ coef,
@@ -369,6 +369,7 @@ __global__ __launch_bounds__(WN_SHG_TPB) void k_shg_mfma(const int4 *__restrict_
                                                         int pcap, int nwt, int pb, WnShg sp) {
     extern __shared__ double2 lds_coef[];      // [pb][cwid] coefficient block
     __shared__ double lds_diff[128];
+    __shared__ int lds_row[128];
     __shared__ int pre[WN_SHG_MAXK + 1];       // prefix of the pair counts of the chunk's k points
     const int nc = sp.ncomp, ncol = 19 * nc, nw = sp.nw;
     const int per_chunk = nwt * 4;
@@ -380,15 +381,32 @@ __global__ __launch_bounds__(WN_SHG_TPB) void k_shg_mfma(const int4 *__restrict_
     const int4 ch = chunks[chunk];
     double2 *o = partial + (size_t)chunk * ((size_t)ncol * nw + nc);
     if (kind == 3) {
-        // low block (R = 1) and the one-band sums, once per chunk
+        // low block (R = 1) and the one-band sums, once per chunk: thread t sums the pairs t, t + 256,
+        // ... of every column, then the 256 partial sums are combined in a fixed tree order
         if (wt != 0) return;
-        for (int c = t; c < 3 * nc; c += WN_SHG_TPB) {
+        __shared__ double2 red[WN_SHG_TPB];
+        if (t == 0) {
+            pre[0] = 0;
+            for (int i = 0; i < ch.y; ++i) pre[i + 1] = pre[i] + count[ch.x + i];
+        }
+        __syncthreads();
+        const int tot = pre[ch.y];
+        for (int c = 0; c < 3 * nc; ++c) {
             double2 acc = make_double2(0.0, 0.0);
-            for (int k = ch.x; k < ch.x + ch.y; ++k) {
-                const size_t base = (size_t)k * pcap;
-                for (int p = 0; p < count[k]; ++p) acc = cadd(acc, coef[(base + p) * ncol + 16 * nc + c]);
+            int ki = 0;
+            for (int f = t; f < tot; f += WN_SHG_TPB) {
+                while (pre[ki + 1] <= f) ++ki;
+                acc = cadd(acc, coef[((size_t)(ch.x + ki) * pcap + (f - pre[ki])) * ncol + 16 * nc + c]);
             }
-            for (int w = 0; w < nw; ++w) o[(size_t)(16 * nc + c) * nw + w] = acc;
+            red[t] = acc;
+            __syncthreads();
+            for (int h = WN_SHG_TPB / 2; h > 0; h >>= 1) {
+                if (t < h) red[t] = cadd(red[t], red[t + h]);
+                __syncthreads();
+            }
+            const double2 sum = red[0];
+            for (int w = t; w < nw; w += WN_SHG_TPB) o[(size_t)(16 * nc + c) * nw + w] = sum;
+            __syncthreads();
         }
         if (t < nc) {
             double sb = 0.0, ss = 0.0;
@@ -405,8 +423,14 @@ __global__ __launch_bounds__(WN_SHG_TPB) void k_shg_mfma(const int4 *__restrict_
     const int cwid = kind == 0 ? 8 * nc : (kind == 1 ? 6 * nc : 2 * nc);   // complex columns
     const int ntile = (2 * cwid + 15) / 16;
     const int lr = lane & 15, lk = lane >> 4;           // A row (frequency) / K row of this lane
-    const int w_a = wt * 64 + wave * 16 + lr;           // frequency of this lane's A element
-    const double om = w_a < nw ? omega[w_a] : 0.0;
+    // row tiles (16 frequencies each) of this wave: wt * 4 RTW + wave RTW + r
+    const int rt0 = (wt * 4 + wave) * RTW;
+    double om[RTW];
+#pragma unroll
+    for (int r = 0; r < RTW; ++r) {
+        const int w_a = (rt0 + r) * 16 + lr;            // frequency of this lane's A element in tile r
+        om[r] = w_a < nw ? omega[w_a] : 0.0;
+    }
     const int pair_sel = lk >> 1, part = lk & 1;        // pair of the step and re / im row
     if (t == 0) {
         pre[0] = 0;
@@ -414,44 +438,51 @@ __global__ __launch_bounds__(WN_SHG_TPB) void k_shg_mfma(const int4 *__restrict_
     }
     __syncthreads();
     const int total = pre[ch.y];
-    f64x4 acc[MAXT];
+    f64x4 acc[RTW][MAXT];
 #pragma unroll
-    for (int j = 0; j < MAXT; ++j) acc[j] = (f64x4){0.0, 0.0, 0.0, 0.0};
+    for (int r = 0; r < RTW; ++r)
+#pragma unroll
+        for (int j = 0; j < MAXT; ++j) acc[r][j] = (f64x4){0.0, 0.0, 0.0, 0.0};
     const double cut = 27.3 * sp.width;                 // exp(-(x/w)^2) is 0 in FP64 beyond this
     for (int f0 = 0; f0 < total; f0 += pb) {
         const int nb = min(pb, total - f0);
-        // cooperative load of the block's coefficient rows and energy differences (pairs in chunk order)
-        for (int e = t; e < nb * cwid; e += WN_SHG_TPB) {
-            const int fp = e / cwid, c = e - (e / cwid) * cwid;
-            const int f = f0 + fp;
-            int ki = 0;
-            while (pre[ki + 1] <= f) ++ki;
-            const size_t row = (size_t)(ch.x + ki) * pcap + (f - pre[ki]);
-            lds_coef[fp * cwid + c] = coef[row * ncol + cbeg + c];
-        }
+        // cooperative load of the block's coefficient rows and energy differences (pairs in chunk order);
+        // the row of every pair of the block is located once
         for (int e = t; e < nb; e += WN_SHG_TPB) {
             const int f = f0 + e;
             int ki = 0;
             while (pre[ki + 1] <= f) ++ki;
-            lds_diff[e] = pdiff[(size_t)(ch.x + ki) * pcap + (f - pre[ki])];
+            const int row = (ch.x + ki) * pcap + (f - pre[ki]);
+            lds_row[e] = row;
+            lds_diff[e] = pdiff[row];
+        }
+        __syncthreads();
+        for (int e = t; e < nb * cwid; e += WN_SHG_TPB) {
+            const int fp = e / cwid, c = e - (e / cwid) * cwid;
+            lds_coef[fp * cwid + c] = coef[(size_t)lds_row[fp] * ncol + cbeg + c];
         }
         __syncthreads();
         for (int s0 = 0; s0 < nb; s0 += 2) {
             const int pl = s0 + pair_sel;
             const bool valid = pl < nb;
-            // A: resonance of (frequency w_a, pair pl)
+            // A: resonances of (frequency of row tile r, pair pl)
             const double diff = valid ? lds_diff[pl] : 0.0;
-            const double xx = kind == 0 ? diff - om : (kind == 1 ? diff - 2.0 * om : -diff - om);
-            const double pv = xx / (xx * xx + sp.width * sp.width);
-            double dl;
-            if (sp.gaussian) {
-                const double y = xx / sp.width;
-                dl = fabs(xx) < cut ? exp(-y * y) / (sqrt(M_PI) * sp.width) : 0.0;
-            } else {
-                const double y = xx / sp.width;
-                dl = 1.0 / (M_PI * sp.width * (1.0 + y * y));
+            double a[RTW];
+#pragma unroll
+            for (int r = 0; r < RTW; ++r) {
+                const double xx = kind == 0 ? diff - om[r] : (kind == 1 ? diff - 2.0 * om[r] : -diff - om[r]);
+                double v;
+                if (part == 0) {
+                    v = xx / (xx * xx + sp.width * sp.width);
+                } else if (sp.gaussian) {
+                    const double y = xx / sp.width;
+                    v = fabs(xx) < cut ? M_PI * (exp(-y * y) / (sqrt(M_PI) * sp.width)) : 0.0;
+                } else {
+                    const double y = xx / sp.width;
+                    v = M_PI * (1.0 / (M_PI * sp.width * (1.0 + y * y)));
+                }
+                a[r] = valid ? v : 0.0;
             }
-            const double a = valid ? (part == 0 ? pv : M_PI * dl) : 0.0;
             const double2 *row = lds_coef + (valid ? pl : 0) * cwid;
 #pragma unroll
             for (int j = 0; j < MAXT; ++j) {
@@ -462,7 +493,9 @@ __global__ __launch_bounds__(WN_SHG_TPB) void k_shg_mfma(const int4 *__restrict_
                         const double2 cv = row[cr >> 1];
                         b = part == 0 ? ((cr & 1) ? cv.y : cv.x) : ((cr & 1) ? cv.x : -cv.y);
                     }
-                    acc[j] = __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, acc[j], 0, 0, 0);
+#pragma unroll
+                    for (int r = 0; r < RTW; ++r)
+                        acc[r][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[r], b, acc[r][j], 0, 0, 0);
                 }
             }
         }
@@ -470,16 +503,18 @@ __global__ __launch_bounds__(WN_SHG_TPB) void k_shg_mfma(const int4 *__restrict_
     }
     // D reg q -> row (frequency) lk + 4q, column lr of each 16-column tile
 #pragma unroll
+    for (int r = 0; r < RTW; ++r)
+#pragma unroll
     for (int j = 0; j < MAXT; ++j) {
         if (j < ntile) {
             const int cr = j * 16 + lr;
             if (cr < 2 * cwid) {
 #pragma unroll
                 for (int q = 0; q < 4; ++q) {
-                    const int w = wt * 64 + wave * 16 + lk + 4 * q;
+                    const int w = (rt0 + r) * 16 + lk + 4 * q;
                     if (w < nw) {
                         double *dst = (double *)(o + (size_t)(cbeg + (cr >> 1)) * nw + w);
-                        dst[cr & 1] = acc[j][q];
+                        dst[cr & 1] = acc[r][j][q];
                     }
                 }
             }
@@ -497,20 +532,22 @@ hipError_t wn_launch_shg(const double *x, const double *e, const double *omega, 
                            st, x, e, der, count, pairs, pdiff, coef, one, sh, sp);
     }
     if (nchunks > 0) {
-        const int nwt = (sp.nw + 63) / 64;
+        // RTW row tiles of 16 frequencies per wave: a workgroup covers 64 RTW frequencies
+        const int rtw = sp.ncomp <= 4 ? 3 : (sp.ncomp <= 8 ? 2 : 1);
+        const int nwt = (sp.nw + 64 * rtw - 1) / (64 * rtw);
         // pairs per LDS block: the widest block (8 nc complex columns) times pb within 32 KB, even, <= 128
         const int pb = std::max(2, std::min(128, (2048 / (8 * sp.ncomp)) & ~1));
         const size_t lds = (size_t)pb * 8 * sp.ncomp * sizeof(double2);
         const dim3 grid(nchunks * nwt * 4);
-#define WN_SHG_LAUNCH(T)                                                                                          \
-    hipLaunchKernelGGL((k_shg_mfma<T>), grid, dim3(WN_SHG_TPB), lds, st, chunks, count, pdiff, coef, one, omega, \
+#define WN_SHG_LAUNCH(T, R)                                                                                         \
+    hipLaunchKernelGGL((k_shg_mfma<T, R>), grid, dim3(WN_SHG_TPB), lds, st, chunks, count, pdiff, coef, one, omega, \
                        partial, sh.n * sh.n, nwt, pb, sp)
         if (sp.ncomp <= 4)
-            WN_SHG_LAUNCH(4);
+            WN_SHG_LAUNCH(4, 3);
         else if (sp.ncomp <= 8)
-            WN_SHG_LAUNCH(8);
+            WN_SHG_LAUNCH(8, 2);
         else
-            WN_SHG_LAUNCH(18);
+            WN_SHG_LAUNCH(18, 1);
 #undef WN_SHG_LAUNCH
     }
     return hipGetLastError();
