@@ -227,7 +227,7 @@ def main():
         log(f"[bench] cpu baseline {kps:.1f} k/s on {threads} threads")
 
     out = {
-        "metric": "k-points/s (n_w=32,128^3 shift current) mixed vs FP64 GPU at rel err <=1e-6",
+        "metric": "k-points/s (n_w=32,128\u00b3 shift current) mixed vs FP64 GPU at rel err \u22641e-6",
         "value": value,
         "unit": "k-points/s",
         "n_gpus": 1,
