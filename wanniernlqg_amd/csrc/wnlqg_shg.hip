@@ -341,13 +341,6 @@ __global__ __launch_bounds__(WN_SHG_TPB) void k_shg_coef(const double *__restric
     }
 }
 
-// R(x) = PV(x) + i pi delta(x) for the broadening of the task (SecondHarmonicKernels.jl:86-98)
-__device__ inline double2 shg_resonance(double xx, double width, int gaussian) {
-    const double y = xx / width;
-    const double dl = gaussian ? exp(-y * y) / (sqrt(M_PI) * width) : 1.0 / (M_PI * width * (1.0 + y * y));
-    return make_double2(shg_principal(xx, width), M_PI * dl);
-}
-
 // ------------------------------------------------------------------------------------------------
 // Frequency contraction on FP64 MFMA.  For one chunk and one resonance kind (d1, d2, dn) the
 // partial sums form a complex GEMM out[w][c] = sum_p R(dE_p, w) coef[p][c], written as a real GEMM
@@ -465,7 +458,8 @@ __global__ __launch_bounds__(WN_SHG_TPB) void k_shg_mfma(const int4 *__restrict_
         for (int s0 = 0; s0 < nb; s0 += 2) {
             const int pl = s0 + pair_sel;
             const bool valid = pl < nb;
-            // A: resonances of (frequency of row tile r, pair pl)
+            // A: resonances R(x) = PV(x) + i pi delta(x) (SecondHarmonicKernels.jl:86-98) of (frequency of row
+            // tile r, pair pl); part 0 of the lane carries Re R, part 1 Im R
             const double diff = valid ? lds_diff[pl] : 0.0;
             double a[RTW];
 #pragma unroll
