@@ -56,6 +56,22 @@ __device__ __forceinline__ double2 cfma(double2 a, double2 b, double2 c) {
 
 // d/(d^2+eta^2) with the scaled branch for over/underflow
 // (reference PositionVertexRegularization.jl:17-29); eta == 0 selects the bare 1/d (0 at d == 0).
+// 1/sqrt(x) and 1/x for finite x > 0: hardware estimate, then two Newton steps (quadratic
+// convergence takes the estimate's error far below the FP64 rounding level).
+__device__ __forceinline__ double wn_rsqrt(double x) {
+    double y = __builtin_amdgcn_rsq(x);
+    const double hx = 0.5 * x;
+    y = y * (1.5 - hx * y * y);
+    y = y * (1.5 - hx * y * y);
+    return y;
+}
+__device__ __forceinline__ double wn_rcp(double x) {
+    double y = __builtin_amdgcn_rcp(x);
+    y = y * (2.0 - x * y);
+    y = y * (2.0 - x * y);
+    return y;
+}
+
 __device__ __forceinline__ double wn_reg_inverse(double d, double eta) {
     if (eta == 0.0) return d == 0.0 ? 0.0 : 1.0 / d;
     double den = d * d + eta * eta;

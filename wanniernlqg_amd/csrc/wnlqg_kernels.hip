@@ -794,21 +794,6 @@ hipError_t wn_launch_contract(const double2 *in, double2 *out, const double2 *E,
 // ||offdiag(H)||_F <= tol ||H||_F.  Eigenvalues are sorted ascending and every eigenvector
 // column is rotated so that its first largest-magnitude entry is real and non-negative.
 // ------------------------------------------------------------------------------------------------
-// 1/sqrt(x) and 1/x for finite x > 0: hardware estimate, then two Newton steps (quadratic
-// convergence takes the estimate's error far below the FP64 rounding level).
-__device__ __forceinline__ double wn_rsqrt(double x) {
-    double y = __builtin_amdgcn_rsq(x);
-    const double hx = 0.5 * x;
-    y = y * (1.5 - hx * y * y);
-    y = y * (1.5 - hx * y * y);
-    return y;
-}
-__device__ __forceinline__ double wn_rcp(double x) {
-    double y = __builtin_amdgcn_rcp(x);
-    y = y * (2.0 - x * y);
-    y = y * (2.0 - x * y);
-    return y;
-}
 
 // Block form of the parallel Jacobi round.  Only the upper triangle (and diagonal) of H is stored.
 // A round with disjoint planes t = (p_t, q_t) maps every 2x2 block B = H[{p1,q1}][{p2,q2}] to

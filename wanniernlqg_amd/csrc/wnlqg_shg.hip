@@ -354,7 +354,7 @@ __global__ __launch_bounds__(WN_SHG_TPB) void k_shg_coef(const double *__restric
 // ------------------------------------------------------------------------------------------------
 #define WN_SHG_MAXK 64   // k points per chunk
 template <int MAXT, int RTW>
-__global__ __launch_bounds__(WN_SHG_TPB) void k_shg_mfma(const int4 *__restrict__ chunks, const int *__restrict__ count,
+__global__ __launch_bounds__(WN_SHG_TPB, MAXT == 4 ? 3 : 2) void k_shg_mfma(const int4 *__restrict__ chunks, const int *__restrict__ count,
                                                         const double *__restrict__ pdiff,
                                                         const double2 *__restrict__ coef,
                                                         const double2 *__restrict__ one,
@@ -437,6 +437,8 @@ __global__ __launch_bounds__(WN_SHG_TPB) void k_shg_mfma(const int4 *__restrict_
 #pragma unroll
         for (int j = 0; j < MAXT; ++j) acc[r][j] = (f64x4){0.0, 0.0, 0.0, 0.0};
     const double cut = 27.3 * sp.width;                 // exp(-(x/w)^2) is 0 in FP64 beyond this
+    const double w2 = sp.width * sp.width, iw = 1.0 / sp.width;
+    const double gnorm = sqrt(M_PI) * iw;               // pi / (sqrt(pi) w)
     for (int f0 = 0; f0 < total; f0 += pb) {
         const int nb = min(pb, total - f0);
         // cooperative load of the block's coefficient rows and energy differences (pairs in chunk order);
@@ -467,13 +469,13 @@ __global__ __launch_bounds__(WN_SHG_TPB) void k_shg_mfma(const int4 *__restrict_
                 const double xx = kind == 0 ? diff - om[r] : (kind == 1 ? diff - 2.0 * om[r] : -diff - om[r]);
                 double v;
                 if (part == 0) {
-                    v = xx / (xx * xx + sp.width * sp.width);
+                    v = xx * wn_rcp(xx * xx + w2);                      // PV: x / (x^2 + w^2)
                 } else if (sp.gaussian) {
-                    const double y = xx / sp.width;
-                    v = fabs(xx) < cut ? M_PI * (exp(-y * y) / (sqrt(M_PI) * sp.width)) : 0.0;
+                    const double y = xx * iw;                           // pi exp(-y^2) / (sqrt(pi) w)
+                    v = fabs(xx) < cut ? gnorm * exp(-y * y) : 0.0;
                 } else {
-                    const double y = xx / sp.width;
-                    v = M_PI * (1.0 / (M_PI * sp.width * (1.0 + y * y)));
+                    const double y = xx * iw;                           // pi / (pi w (1 + y^2))
+                    v = iw * wn_rcp(1.0 + y * y);
                 }
                 a[r] = valid ? v : 0.0;
             }
