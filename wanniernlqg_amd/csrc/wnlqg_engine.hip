@@ -272,7 +272,7 @@ int ensure_rows(wnlqg_engine *g, int rows) {
     }
     if (g->family == 3) {
         WN_HIP(g->count.alloc((size_t)rows));
-        WN_HIP(g->der.alloc((size_t)rows * 9 * n2));
+        WN_HIP(g->der.alloc((size_t)rows * 12 * n2));   // 9 vertex derivative blocks + 3 r_eta blocks
         WN_HIP(g->pairs.alloc((size_t)rows * n2));
         WN_HIP(g->pdiff.alloc((size_t)rows * n2));
         WN_HIP(g->coef.alloc((size_t)rows * n2 * 19 * g->sp.ncomp));
@@ -681,7 +681,7 @@ int32_t wnlqg_create(const wnlqg_model *model, const wnlqg_task *task, wnlqg_eng
     // batch size: interpolation rows + pair buffers within ~24 GB unless the caller fixed it
     const size_t per_k = (size_t)g->sh.ldo * 8 +
                          (g->family == 0 ? n2 * (sizeof(int4) + sizeof(double2) + 27 * g->sc.nq * 16) : 0) +
-                         (g->family == 3 ? n2 * (9 * 16 + 8 + 8 + 19 * (size_t)g->sp.ncomp * 16) : 0) +
+                         (g->family == 3 ? n2 * (12 * 16 + 8 + 8 + 19 * (size_t)g->sp.ncomp * 16) : 0) +
                          n2 * 16 + (size_t)3 * g->sh.S * 16 +
                          (wn_eig_scratch_elems(n, 1) + wn_rotate_scratch_elems(n, 1)) * 16;
     long long b = task->batch_size > 0 ? task->batch_size : (long long)(24.0e9 / (double)per_k);

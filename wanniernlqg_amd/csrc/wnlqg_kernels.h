@@ -141,7 +141,8 @@ hipError_t wn_launch_accum(const int4 *chunks, int nchunks, const int *count, co
 hipError_t wn_launch_geom_f64(const double *x, const double *e, const double *mu, const int4 *chunks, int nchunks,
                               double2 *partial, int *flags, long long kbase, const WnShape &sh, const WnGeom &gp,
                               hipStream_t st);
-// SHG: optical position derivatives der[k][9][n][n], pair lists and coefficient rows for `count_k` k
+// SHG: optical position derivatives der[k][0..8][n][n] and regularized dipoles der[k][9..11][n][n]
+// (der holds 12 n^2 complex per k point), pair lists and coefficient rows for `count_k` k
 // points, then the frequency contraction of `nchunks` chunks into partial[chunk] = [19 nc][nw] complex
 // followed by [nc] (one-band BCD, semiclassical) sums.
 hipError_t wn_launch_shg(const double *x, const double *e, const double *omega, const int4 *chunks, int nchunks,

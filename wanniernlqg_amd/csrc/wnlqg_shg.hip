@@ -35,6 +35,8 @@
 
 #define WN_SHG_TPB 256
 #define WN_SHG_NMAX 128   // orbitals (<= WN_SHG_TPB: one thread per band in the per-band phases)
+// per-k blocks of the vertex buffer: der[3a+b] (9), then the regularized dipoles r_eta[a] (3)
+#define WN_SHG_DER_BLOCKS 12
 
 namespace {
 
@@ -114,9 +116,15 @@ __global__ __launch_bounds__(WN_SHG_TPB) void k_shg_der(const double *__restrict
     const int n = sh.n, n2 = n * n;
     const ShgView s = make_view(x, eg, k, sh, sp.eta);
     const int t = threadIdx.x;
+    if (ab >= 9) {
+        // r_eta[a'] = A + i V reg(E_j - E_i) of every element, read by the pair coefficients' band sums
+        double2 *rd = der + ((size_t)k * WN_SHG_DER_BLOCKS + ab) * n2;
+        for (int e = t; e < n2; e += WN_SHG_TPB) rd[e] = sv_rdip(s, ab - 9, e / n, e - (e / n) * n);
+        return;
+    }
     if (t < n) occ[t] = shg_occupation(s.E[t], sp.mu, sp.temperature);
     __syncthreads();
-    double2 *out = der + ((size_t)k * 9 + ab) * n2;
+    double2 *out = der + ((size_t)k * WN_SHG_DER_BLOCKS + ab) * n2;
     const bool pos = s.pos != 0;
     for (int e = t; e < n2; e += WN_SHG_TPB) {
         const int nn = e / n, m = e - (e / n) * n;
@@ -186,7 +194,8 @@ __global__ __launch_bounds__(WN_SHG_TPB) void k_shg_coef(const double *__restric
     const int n = sh.n, n2 = n * n, nc = sp.ncomp;
     const int ncol = 19 * nc;
     const ShgView s = make_view(x, eg, k, sh, sp.eta);
-    const double2 *dk = der + (size_t)k * 9 * n2;
+    const double2 *dk = der + (size_t)k * WN_SHG_DER_BLOCKS * n2;
+    const double2 *rdk = dk + (size_t)9 * n2;   // r_eta[a][i][j]
     const int t = threadIdx.x;
     if (t < n) occ[t] = shg_occupation(s.E[t], sp.mu, sp.temperature);
     __syncthreads();
@@ -240,15 +249,15 @@ __global__ __launch_bounds__(WN_SHG_TPB) void k_shg_coef(const double *__restric
         double2 tf = make_double2(0.0, 0.0), to = make_double2(0.0, 0.0);
         for (int q = 0; q < n; ++q) {
             if (q == nn || q == m) continue;
-            const double2 rb_mq = sv_rdip(s, b, m, q), rc_qn = sv_rdip(s, c, q, nn);
-            const double2 rc_mq = sv_rdip(s, c, m, q), rb_qn = sv_rdip(s, b, q, nn);
+            const double2 rb_mq = rdk[(size_t)b * n2 + m * n + q], rc_qn = rdk[(size_t)c * n2 + q * n + nn];
+            const double2 rc_mq = rdk[(size_t)c * n2 + m * n + q], rb_qn = rdk[(size_t)b * n2 + q * n + nn];
             const double pv0 = shg_principal(2.0 * s.E[q] - s.E[nn] - s.E[m], sp.eta);
             tf = cadd(tf, cscale(cmul(dip_a_nm, cadd(cmul(rb_mq, rc_qn), cmul(rc_mq, rb_qn))), pv0));
             const double pv1 = shg_principal(2.0 * s.E[nn] - s.E[m] - s.E[q], sp.eta);
-            const double2 ra_mq = sv_rdip(s, a, m, q);
+            const double2 ra_mq = rdk[(size_t)a * n2 + m * n + q];
             to = cadd(to, cscale(cmul(ra_mq, cadd(cmul(rb_qn, dip_c_nm), cmul(rc_qn, dip_b_nm))), pv1));
             const double pv2 = shg_principal(2.0 * s.E[m] - s.E[q] - s.E[nn], sp.eta);
-            const double2 ra_qn = sv_rdip(s, a, q, nn);
+            const double2 ra_qn = rdk[(size_t)a * n2 + q * n + nn];
             to = cadd(to, cscale(cmul(ra_qn, cadd(cmul(dip_b_nm, rc_mq), cmul(dip_c_nm, rb_mq))), pv2));
         }
         // f = i o / 2;  f * inv * z = i (o inv / 2) z
@@ -523,7 +532,7 @@ hipError_t wn_launch_shg(const double *x, const double *e, const double *omega, 
                          double2 *one, double2 *partial, const WnShape &sh, const WnShg &sp, hipStream_t st) {
     if (sh.n > WN_SHG_NMAX || sp.ncomp < 1 || sp.ncomp > 18) return hipErrorInvalidValue;
     if (count_k > 0) {
-        hipLaunchKernelGGL(k_shg_der, dim3(count_k, 9), dim3(WN_SHG_TPB), 0, st, x, e, der, sh, sp);
+        hipLaunchKernelGGL(k_shg_der, dim3(count_k, WN_SHG_DER_BLOCKS), dim3(WN_SHG_TPB), 0, st, x, e, der, sh, sp);
         hipLaunchKernelGGL(k_shg_coef, dim3(count_k), dim3(WN_SHG_TPB), (size_t)sh.n * sp.ncomp * 2 * sizeof(double),
                            st, x, e, der, count, pairs, pdiff, coef, one, sh, sp);
     }
