@@ -369,7 +369,7 @@ __global__ __launch_bounds__(WN_SHG_TPB, MAXT == 4 ? 3 : 2) void k_shg_mfma(cons
                                                         const double2 *__restrict__ one,
                                                         const double *__restrict__ omega, double2 *__restrict__ partial,
                                                         int pcap, int nwt, int pb, WnShg sp) {
-    extern __shared__ double2 lds_coef[];      // [pb][cwid] coefficient block
+    extern __shared__ double lds_b[];          // [pb][2][2 cwid] coefficient block in the real B layout
     __shared__ double lds_diff[128];
     __shared__ int lds_row[128];
     __shared__ int pre[WN_SHG_MAXK + 1];       // prefix of the pair counts of the chunk's k points
@@ -461,11 +461,19 @@ __global__ __launch_bounds__(WN_SHG_TPB, MAXT == 4 ? 3 : 2) void k_shg_mfma(cons
             lds_diff[e] = pdiff[row];
         }
         __syncthreads();
+        // pair fp, coefficient c = (cr, ci): B rows 2fp = (cr, ci), 2fp+1 = (-ci, cr)
+        const int bw = 2 * cwid;
         for (int e = t; e < nb * cwid; e += WN_SHG_TPB) {
             const int fp = e / cwid, c = e - (e / cwid) * cwid;
-            lds_coef[fp * cwid + c] = coef[(size_t)lds_row[fp] * ncol + cbeg + c];
+            const double2 cv = coef[(size_t)lds_row[fp] * ncol + cbeg + c];
+            double *b0 = lds_b + (size_t)(2 * fp) * bw + 2 * c;
+            b0[0] = cv.x;
+            b0[1] = cv.y;
+            b0[bw] = -cv.y;
+            b0[bw + 1] = cv.x;
         }
         __syncthreads();
+#pragma unroll 2
         for (int s0 = 0; s0 < nb; s0 += 2) {
             const int pl = s0 + pair_sel;
             const bool valid = pl < nb;
@@ -488,16 +496,12 @@ __global__ __launch_bounds__(WN_SHG_TPB, MAXT == 4 ? 3 : 2) void k_shg_mfma(cons
                 }
                 a[r] = valid ? v : 0.0;
             }
-            const double2 *row = lds_coef + (valid ? pl : 0) * cwid;
+            const double *brow = lds_b + (size_t)(2 * (valid ? pl : 0) + part) * bw;
 #pragma unroll
             for (int j = 0; j < MAXT; ++j) {
                 if (j < ntile) {
                     const int cr = j * 16 + lr;                  // real column
-                    double b = 0.0;
-                    if (valid && cr < 2 * cwid) {
-                        const double2 cv = row[cr >> 1];
-                        b = part == 0 ? ((cr & 1) ? cv.y : cv.x) : ((cr & 1) ? cv.x : -cv.y);
-                    }
+                    const double b = (valid && cr < bw) ? brow[cr] : 0.0;
 #pragma unroll
                     for (int r = 0; r < RTW; ++r)
                         acc[r][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[r], b, acc[r][j], 0, 0, 0);
@@ -541,8 +545,10 @@ hipError_t wn_launch_shg(const double *x, const double *e, const double *omega, 
         const int rtw = sp.ncomp <= 4 ? 3 : (sp.ncomp <= 8 ? 2 : 1);
         const int nwt = (sp.nw + 64 * rtw - 1) / (64 * rtw);
         // pairs per LDS block: the widest block (8 nc complex columns) times pb within 32 KB, even, <= 128
-        const int pb = std::max(2, std::min(128, (2048 / (8 * sp.ncomp)) & ~1));
-        const size_t lds = (size_t)pb * 8 * sp.ncomp * sizeof(double2);
+        // pairs per LDS block: the widest block (8 nc complex columns, 4 doubles each in the B layout) within
+        // 32 KB, even, <= 128
+        const int pb = std::max(2, std::min(128, (1024 / (8 * sp.ncomp)) & ~1));
+        const size_t lds = (size_t)pb * 8 * sp.ncomp * 4 * sizeof(double);
         const dim3 grid(nchunks * nwt * 4);
 #define WN_SHG_LAUNCH(T, R)                                                                                         \
     hipLaunchKernelGGL((k_shg_mfma<T, R>), grid, dim3(WN_SHG_TPB), lds, st, chunks, count, pdiff, coef, one, omega, \
