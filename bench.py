@@ -70,6 +70,15 @@ def jacobi_flops_per_k(n: int, sweeps: float) -> float:
     return sweeps * (n * (n - 1) / 2) * 3 * n * 24.0
 
 
+def pairs_flops_per_pair(n: int) -> float:
+    """Algorithmic flops of one active band pair in the SC/IC pair kernel: six generalized-derivative rows
+    (two directions x three components), each an intermediate-state sum over n - 2 bands of, for the three
+    derivative directions, two scaled dipoles (4 flops) and two complex multiply-add/subtract pairs for the
+    A and dH commutator terms (32 flops), plus ~200 flops of closing terms per row; then the 27 shift
+    kernels (two complex products and a difference, 14 flops) and 27 injection kernels (12 flops)."""
+    return 6.0 * (108.0 * (n - 2) + 200.0) + 27 * 14.0 + 27 * 12.0
+
+
 TRAFFIC_PROFILE = os.path.join(ROOT, "profiles", "kloop_n32_64cube_hbm_traffic.md")
 
 
@@ -130,8 +139,10 @@ def roofline(st, n, nk, precision):
     else:
         launches = nb
         avg_s = 1e-3 * st["pairs_ms"] / launches
-        roof = {"kernel": "k_pairs_f64", "bound": "mfma", "achieved": None, "peak": FP64_PEAK_TFLOPS,
-                "unit": "TFLOP/s"}
+        flops = pairs_flops_per_pair(n) * st["active_pairs"] / launches
+        roof = {"kernel": "k_pairs_f64 (generalized derivatives and SC/IC pair kernels on FP64 VALU; FP64 vector "
+                          "peak = FP64 matrix peak)", "bound": "mfma", "achieved": flops / avg_s / 1e12,
+                "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s", "active_pairs_per_k": st["active_pairs"] / nk}
         tkey = "k_pairs_f64"
     roof["frac"] = roof["achieved"] / roof["peak"] if roof["achieved"] is not None else None
     roof["traffic"] = traffic[tkey] * nk / launches if tkey in traffic else None

@@ -753,7 +753,13 @@ int check_flags(wnlqg_engine *g) {
     return WNLQG_OK;
 }
 
-// clear the status words: [0] pair-gap flag, [1] max sweeps, [2] geometry flags, [3] first failing k
+// MIXED and SPLIT_BF16 rotate channels 1 .. C-1 on FP32 matrix cores; MIXED_HIGH and FP64 stay FP64.
+bool rotate_fp32(const wnlqg_engine *g) {
+    return g->precision == WNLQG_PRECISION_MIXED || g->precision == WNLQG_PRECISION_SPLIT_BF16;
+}
+
+// clear the status words: [0] pair-gap flag, [1] max sweeps, [2] geometry flags, [3] first failing k,
+// [4-5] total Jacobi sweeps, [6-7] total active SC/IC pairs
 hipError_t clear_flags(wnlqg_engine *g) { return wn_launch_reset_flags(g->flags.p, g->stream); }
 
 // per-chunk stage after the rotation: SC / IC pairs + frequency contraction, or BCD / AHC
@@ -792,7 +798,7 @@ int32_t wnlqg_eval_points_spectrum(wnlqg_engine *g, const double *kpoints, int32
         if (energies)
             WN_HIP(hipMemcpyAsync(energies + (size_t)p0 * g->sh.n, g->e.p, (size_t)cnt * g->sh.n * sizeof(double),
                                   hipMemcpyDeviceToHost, g->stream));
-        WN_HIP(wn_launch_rotate_f64(g->x.p, g->u.p, cnt, g->sh, g->stream, g->rscratch.p));
+        WN_HIP(wn_launch_rotate_f64(g->x.p, g->u.p, cnt, g->sh, g->stream, g->rscratch.p, rotate_fp32(g)));
         rc = ensure_partial(g, cnt);
         if (rc) return rc;
         std::vector<int4> ch(cnt);
@@ -841,6 +847,7 @@ int plan_mesh(wnlqg_engine *g, const int32_t mesh[3], int32_t n_lanes) {
     if (p.valid && p.mesh[0] == mesh[0] && p.mesh[1] == mesh[1] && p.mesh[2] == mesh[2] && p.n_lanes == n_lanes)
         return WNLQG_OK;
     p.valid = false;
+    if (g->kpc < 1 || g->kpc > WN_ACC_KMAX) return fail(WNLQG_E_INVALID, "chunk size %d outside [1, %d]", g->kpc, WN_ACC_KMAX);
     const long long nk = (long long)mesh[0] * mesh[1] * mesh[2];
     const int nelem = output_elems(g);
     std::vector<long long> lstart(n_lanes + 1);
@@ -1011,7 +1018,7 @@ int enqueue_mesh(wnlqg_engine *g, std::vector<std::array<hipEvent_t, WN_MARKS>> 
         // stream 1: rotation, response and k sums of batch bi
         WN_HIP(hipStreamWaitEvent(s1, g->ev_ready[buf], 0));
         if (marks) hipEventRecord((*marks)[bi][7], s1);
-        WN_HIP(wn_launch_rotate_f64(xb, ub, cnt, g->sh, s1, g->rscratch.p));
+        WN_HIP(wn_launch_rotate_f64(xb, ub, cnt, g->sh, s1, g->rscratch.p, rotate_fp32(g)));
         mark(bi, 4, s1);
         const int nch = (int)(p.bchunk[bi + 1] - p.bchunk[bi]);
         const int nln = (int)(p.blane[bi + 1] - p.blane[bi]);
@@ -1166,6 +1173,9 @@ int32_t wnlqg_run_mesh(wnlqg_engine *g, const int32_t mesh[3], int32_t n_lanes, 
     unsigned long long swt = 0;
     std::memcpy(&swt, fl + 4, sizeof(swt));
     st.mean_sweeps = nk > 0 ? (double)swt / (double)nk : 0.0;
+    unsigned long long npairs = 0;
+    std::memcpy(&npairs, fl + 6, sizeof(npairs));
+    st.active_pairs = (int64_t)npairs;
     st.kpoints = nk;
     st.batches = (int)nbatch;
     st.interp_launches = (int)nbatch;

@@ -120,18 +120,20 @@ hipError_t wn_launch_warm_transform_f64(double *x, const double2 *u, int count, 
 // Clears that are safe under stream capture: p[0 .. n) = 0; flags = {0, 0, 0, 0x7f7f7f7f}.
 hipError_t wn_launch_zero(double *p, long long n, hipStream_t st);
 // status words: [0] pair-gap flag, [1] max sweeps, [2] geometry flags, [3] first failing k,
-// [4..5] total sweeps (64-bit)
+// [4..5] total sweeps (64-bit), [6..7] total active SC/IC pairs (64-bit)
 #define WN_FLAG_WORDS 8
 hipError_t wn_launch_reset_flags(int *f, hipStream_t st);
 // U^+ X U for channels 1 .. C-1, in place.
 // scratch: wn_rotate_scratch_elems(n, count) complex elements, required for n > 64
+// fp32: FP32 matrix-core products with FP32 accumulation (mixed forms, 16 <= n <= 64; FP64 otherwise)
 hipError_t wn_launch_rotate_f64(double *x, const double2 *u, int count, const WnShape &sh, hipStream_t st,
-                                double2 *scratch = nullptr);
+                                double2 *scratch = nullptr, bool fp32 = false);
 // Screening, pair lists, generalized derivatives and SC/IC kernels.
 hipError_t wn_launch_pairs_f64(const double *x, const double *e, const double *omega, int *count, int4 *pmeta,
                                double2 *pmd, double2 *q, int *flags, int nk, const WnShape &sh, const WnScreen &sc,
                                hipStream_t st);
-// Frequency contraction of the pairs of each chunk.
+// Frequency contraction of the pairs of each chunk (chunks of at most WN_ACC_KMAX k points).
+#define WN_ACC_KMAX 64
 hipError_t wn_launch_accum(const int4 *chunks, int nchunks, const int *count, const int4 *pmeta, const double2 *pmd,
                            const double2 *q, const double *omega, double2 *partial, const WnShape &sh,
                            const WnScreen &sc, hipStream_t st);

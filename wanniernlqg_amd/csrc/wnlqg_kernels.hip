@@ -849,7 +849,8 @@ __global__ __launch_bounds__(TPB) void k_eig_f64(const double *__restrict__ x, d
     if (k >= count) return;
     const int t = threadIdx.x;
 
-    auto P = [](int i, int j) { return i * (2 * N - i + 1) / 2 + (j - i); };
+    // packed row-major index of (i, j), i <= j (24-bit multiply: full-rate on the VALU)
+    auto P = [](int i, int j) { return (int)(__umul24((unsigned)i, (unsigned)(2 * N - i + 1)) >> 1) + (j - i); };
     double2 *vg = VG ? vscratch + (size_t)k * N * N : nullptr;
     // V[i][j]: row i, column j (the eigenvector index)
     auto V = [&](int i, int j) -> double2 & { return VG ? vg[(size_t)j * N + i] : Vs[i * LD + j]; };
@@ -881,6 +882,7 @@ __global__ __launch_bounds__(TPB) void k_eig_f64(const double *__restrict__ x, d
     constexpr int VT0 = SPLIT ? BW0 : 0;            // first thread updating V
     constexpr int VW = SPLIT ? TPB - BW0 : TPB;     // threads updating V
     constexpr int BPT = (NOFF + BW - 1) / BW;
+    constexpr int VPT = (VITEMS + VW - 1) / VW;     // V items per thread
     int bt1[BPT], bt2[BPT];
 #pragma unroll
     for (int q = 0; q < BPT; ++q) {
@@ -978,13 +980,25 @@ __global__ __launch_bounds__(TPB) void k_eig_f64(const double *__restrict__ x, d
                 if (R1.y == 0.0 && R2.y == 0.0) continue;
                 const int2 a1 = pq[t1], a2 = pq[t2];
                 const int rr[2] = {a1.x, a1.y}, cc[2] = {a2.x, a2.y};
+                // element (i, j) is stored at (min, max) and conjugated when i > j: one 16-byte read each and
+                // a sign flip of the imaginary part, without branches
                 double2 B[2][2];
+                int off[2][2];
+                unsigned long long flip[2][2];
 #pragma unroll
                 for (int u = 0; u < 2; ++u)
 #pragma unroll
                     for (int v = 0; v < 2; ++v) {
                         const int i = rr[u], j = cc[v];
-                        B[u][v] = i < j ? H[P(i, j)] : cconj(H[P(j, i)]);
+                        off[u][v] = P(min(i, j), max(i, j));
+                        flip[u][v] = (unsigned long long)(i > j) << 63;
+                    }
+#pragma unroll
+                for (int u = 0; u < 2; ++u)
+#pragma unroll
+                    for (int v = 0; v < 2; ++v) {
+                        const double2 h = H[off[u][v]];
+                        B[u][v] = make_double2(h.x, __longlong_as_double(__double_as_longlong(h.y) ^ flip[u][v]));
                     }
                 // B J2: col0 = c2 B0 - s2 e^{-i phi2} B1, col1 = s2 B0 + c2 e^{-i phi2} B1
                 const double2 em2 = make_double2(R2.z, -R2.w);
@@ -1007,24 +1021,27 @@ __global__ __launch_bounds__(TPB) void k_eig_f64(const double *__restrict__ x, d
 #pragma unroll
                 for (int u = 0; u < 2; ++u)
 #pragma unroll
-                    for (int v = 0; v < 2; ++v) {
-                        const int i = rr[u], j = cc[v];
-                        if (i < j)
-                            H[P(i, j)] = B[u][v];
-                        else
-                            H[P(j, i)] = cconj(B[u][v]);
-                    }
+                    for (int v = 0; v < 2; ++v)
+                        H[off[u][v]] = make_double2(B[u][v].x, __longlong_as_double(__double_as_longlong(B[u][v].y) ^
+                                                                                   flip[u][v]));
             }
-            // V <- V J
-            for (int e = t - VT0; e >= 0 && e < VITEMS; e += VW) {
-                const int pr = e / N, i = e - (e / N) * N;
-                const double4 R = rot[pr];
-                if (R.y == 0.0 || i >= n) continue;
-                const int2 ij = pq[pr];
-                const double2 xp = V(i, ij.x), yq = V(i, ij.y);
-                const double2 ey = make_double2(R.z * yq.x + R.w * yq.y, R.z * yq.y - R.w * yq.x);
-                V(i, ij.x) = make_double2(R.x * xp.x - R.y * ey.x, R.x * xp.y - R.y * ey.y);
-                V(i, ij.y) = make_double2(R.y * xp.x + R.x * ey.x, R.y * xp.y + R.x * ey.y);
+            // V <- V J.  Items (pair slot, row) are fixed per thread; a pair without rotation has
+            // (c, s, cos phi, sin phi) = (1, 0, 1, 0), which maps the two entries to themselves exactly, so
+            // every item is applied unconditionally (no branch per item, all items' reads in flight together).
+            if (t >= VT0) {
+#pragma unroll
+                for (int qv = 0; qv < VPT; ++qv) {
+                    const int e = t - VT0 + qv * VW;
+                    if (e >= VITEMS) break;
+                    const int pr = e / N, i = e - (e / N) * N;
+                    if (i >= n) continue;
+                    const double4 R = rot[pr];
+                    const int2 ij = pq[pr];
+                    const double2 xp = V(i, ij.x), yq = V(i, ij.y);
+                    const double2 ey = make_double2(R.z * yq.x + R.w * yq.y, R.z * yq.y - R.w * yq.x);
+                    V(i, ij.x) = make_double2(R.x * xp.x - R.y * ey.x, R.x * xp.y - R.y * ey.y);
+                    V(i, ij.y) = make_double2(R.y * xp.x + R.x * ey.x, R.y * xp.y + R.x * ey.y);
+                }
             }
             if (VG) __threadfence_block();
             __syncthreads();
@@ -1196,9 +1213,10 @@ __global__ __launch_bounds__(256) void k_rotate_f64(double *__restrict__ x, cons
     }
 }
 
-// MFMA form of the rotation: complex products as four real products on v_mfma_f64_16x16x4f64,
-//   T = U^+ X:  Tr = Ur^T Xr + Ui^T Xi,  Ti = Ur^T Xi - Ui^T Xr
-//   Y = T U:    Yr = Tr Ur - Ti Ui,      Yi = Tr Ui + Ti Ur
+// MFMA form of the rotation: complex products as three real products on v_mfma_f64_16x16x4f64
+// (Gauss's three-multiplication form; the operand sums are formed in registers from the LDS reads),
+//   T = U^+ X:  P1 = Ur^T Xr, P2 = Ui^T Xi, P3 = (Ur + Ui)^T (Xi - Xr);  Tr = P1 + P2, Ti = P3 + P1 - P2
+//   Y = T U:    P1 = Tr Ur,   P2 = Ti Ui,   P3 = (Tr + Ti) (Ur + Ui);    Yr = P1 - P2, Yi = P3 - P1 - P2
 // with U, X (then T) in LDS as planar real arrays padded to np = 16*ceil(n/16) (zeros beyond n).
 // The (np/16)^2 output tiles are dealt to the 4 waves.
 // WARM: the warm-start transform of the mesh eigensolver: channel 0 replaced by W^+ H W, with W the
@@ -1262,7 +1280,7 @@ __global__ __launch_bounds__(256) void k_rotate_mfma(double *__restrict__ x, con
 #pragma unroll
         for (int s4 = 0; s4 < 4; ++s4) {
             const int tile = wave + 4 * s4;
-            f64x4 ar = {0.0, 0.0, 0.0, 0.0}, ai = {0.0, 0.0, 0.0, 0.0};
+            f64x4 p1 = {0.0, 0.0, 0.0, 0.0}, p2 = {0.0, 0.0, 0.0, 0.0}, p3 = {0.0, 0.0, 0.0, 0.0};
             if (tile < nt * nt) {
                 const int r0 = (tile / nt) * 16, c0 = (tile % nt) * 16;
 #pragma unroll
@@ -1270,14 +1288,17 @@ __global__ __launch_bounds__(256) void k_rotate_mfma(double *__restrict__ x, con
                     const int kr = k0 + lq;
                     const double ur = Ur[kr * ld + r0 + lr], ui = Ui[kr * ld + r0 + lr];
                     const double xr = Xr[kr * ld + c0 + lr], xi = Xi[kr * ld + c0 + lr];
-                    ar = __builtin_amdgcn_mfma_f64_16x16x4f64(ur, xr, ar, 0, 0, 0);
-                    ar = __builtin_amdgcn_mfma_f64_16x16x4f64(ui, xi, ar, 0, 0, 0);
-                    ai = __builtin_amdgcn_mfma_f64_16x16x4f64(ur, xi, ai, 0, 0, 0);
-                    ai = __builtin_amdgcn_mfma_f64_16x16x4f64(-ui, xr, ai, 0, 0, 0);
+                    p1 = __builtin_amdgcn_mfma_f64_16x16x4f64(ur, xr, p1, 0, 0, 0);
+                    p2 = __builtin_amdgcn_mfma_f64_16x16x4f64(ui, xi, p2, 0, 0, 0);
+                    p3 = __builtin_amdgcn_mfma_f64_16x16x4f64(ur + ui, xi - xr, p3, 0, 0, 0);
                 }
             }
-            tr[s4] = ar;
-            ti[s4] = ai;
+            // Tr = P1 + P2, Ti = P3 + P1 - P2
+#pragma unroll
+            for (int q = 0; q < 4; ++q) {
+                tr[s4][q] = p1[q] + p2[q];
+                ti[s4][q] = p3[q] + (p1[q] - p2[q]);
+            }
         }
         __syncthreads();
 #pragma unroll
@@ -1297,21 +1318,133 @@ __global__ __launch_bounds__(256) void k_rotate_mfma(double *__restrict__ x, con
         // Y = T U, written to global
         for (int tile = wave; tile < nt * nt; tile += 4) {
             const int r0 = (tile / nt) * 16, c0 = (tile % nt) * 16;
-            f64x4 yr = {0.0, 0.0, 0.0, 0.0}, yi = {0.0, 0.0, 0.0, 0.0};
+            f64x4 p1 = {0.0, 0.0, 0.0, 0.0}, p2 = {0.0, 0.0, 0.0, 0.0}, p3 = {0.0, 0.0, 0.0, 0.0};
 #pragma unroll
             for (int k0 = 0; k0 < np; k0 += 4) {
                 const int kc = k0 + lq;
                 const double tr_ = Xr[(r0 + lr) * ld + kc], ti_ = Xi[(r0 + lr) * ld + kc];
                 const double ur = Ur[kc * ld + c0 + lr], ui = Ui[kc * ld + c0 + lr];
-                yr = __builtin_amdgcn_mfma_f64_16x16x4f64(tr_, ur, yr, 0, 0, 0);
-                yr = __builtin_amdgcn_mfma_f64_16x16x4f64(-ti_, ui, yr, 0, 0, 0);
-                yi = __builtin_amdgcn_mfma_f64_16x16x4f64(tr_, ui, yi, 0, 0, 0);
-                yi = __builtin_amdgcn_mfma_f64_16x16x4f64(ti_, ur, yi, 0, 0, 0);
+                p1 = __builtin_amdgcn_mfma_f64_16x16x4f64(tr_, ur, p1, 0, 0, 0);
+                p2 = __builtin_amdgcn_mfma_f64_16x16x4f64(ti_, ui, p2, 0, 0, 0);
+                p3 = __builtin_amdgcn_mfma_f64_16x16x4f64(tr_ + ti_, ur + ui, p3, 0, 0, 0);
             }
+            // Yr = P1 - P2, Yi = P3 - P1 - P2
 #pragma unroll
             for (int q = 0; q < 4; ++q) {
                 const int row = r0 + lq + 4 * q, col = c0 + lr;
-                if (row < n && col < n) xc[row * n + col] = make_double2(yr[q], yi[q]);
+                if (row < n && col < n) xc[row * n + col] = make_double2(p1[q] - p2[q], p3[q] - (p1[q] + p2[q]));
+            }
+        }
+    }
+}
+
+// Mixed-precision form of the rotation (channels 1 .. C-1): U and X rounded to FP32 in LDS, complex
+// products as four real products on v_mfma_f32_16x16x4f32 (twice the FP64 matrix rate) with FP32
+// accumulation, results written back as FP64.  The eigensystem, the warm-start transform and channel 0
+// stay FP64.  Layouts as in k_rotate_mfma except the accumulator: v_mfma_f32_16x16x4f32 lane l, register q
+// holds D[4 (l / 16) + q][l % 16].
+template <int NP, int PF>
+__global__ __launch_bounds__(256) void k_rotate_mfma32(double *__restrict__ x, const double2 *__restrict__ u, int n,
+                                                      int C, int ldo) {
+    extern __shared__ float smf[];
+    const int k = blockIdx.x;
+    const int t = threadIdx.x;
+    const int lane = t & 63, wave = t >> 6;
+    constexpr int np = NP;
+    constexpr int ld = np + 1;
+    constexpr int nt = np >> 4;
+    float *Ur = smf;
+    float *Ui = Ur + np * ld;
+    float *Xr = Ui + np * ld;
+    float *Xi = Xr + np * ld;
+    const double2 *uk = u + (size_t)k * n * n;
+    for (int e = t; e < np * np; e += 256) {
+        const int i = e / np, j = e - i * np;
+        const double2 v = (i < n && j < n) ? uk[i * n + j] : make_double2(0.0, 0.0);
+        Ur[i * ld + j] = (float)v.x;
+        Ui[i * ld + j] = (float)v.y;
+    }
+    double2 *xk = (double2 *)(x + (size_t)k * ldo);
+    const int lr = lane & 15, lq = lane >> 4;
+    double2 pf[PF];
+    auto prefetch = [&](int c) {
+        const double2 *xc = xk + (size_t)c * n * n;
+#pragma unroll
+        for (int q = 0; q < PF; ++q) {
+            const int e = t + 256 * q;
+            const int i = e / np, j = e - (e / np) * np;
+            pf[q] = (e < np * np && i < n && j < n) ? xc[i * n + j] : make_double2(0.0, 0.0);
+        }
+    };
+    if (C > 1) prefetch(1);
+    for (int c = 1; c < C; ++c) {
+        double2 *xc = xk + (size_t)c * n * n;
+        __syncthreads();
+#pragma unroll
+        for (int q = 0; q < PF; ++q) {
+            const int e = t + 256 * q;
+            if (e < np * np) {
+                const int i = e / np, j = e - (e / np) * np;
+                Xr[i * ld + j] = (float)pf[q].x;
+                Xi[i * ld + j] = (float)pf[q].y;
+            }
+        }
+        __syncthreads();
+        // T = U^+ X:  Tr = Ur^T Xr + Ui^T Xi,  Ti = Ur^T Xi - Ui^T Xr
+        f32x4 tr[4], ti[4];
+#pragma unroll
+        for (int s4 = 0; s4 < 4; ++s4) {
+            const int tile = wave + 4 * s4;
+            f32x4 ar = {0.f, 0.f, 0.f, 0.f}, ai = {0.f, 0.f, 0.f, 0.f};
+            if (tile < nt * nt) {
+                const int r0 = (tile / nt) * 16, c0 = (tile % nt) * 16;
+#pragma unroll
+                for (int k0 = 0; k0 < np; k0 += 4) {
+                    const int kr = k0 + lq;
+                    const float ur = Ur[kr * ld + r0 + lr], ui = Ui[kr * ld + r0 + lr];
+                    const float xr = Xr[kr * ld + c0 + lr], xi = Xi[kr * ld + c0 + lr];
+                    ar = __builtin_amdgcn_mfma_f32_16x16x4f32(ur, xr, ar, 0, 0, 0);
+                    ar = __builtin_amdgcn_mfma_f32_16x16x4f32(ui, xi, ar, 0, 0, 0);
+                    ai = __builtin_amdgcn_mfma_f32_16x16x4f32(ur, xi, ai, 0, 0, 0);
+                    ai = __builtin_amdgcn_mfma_f32_16x16x4f32(-ui, xr, ai, 0, 0, 0);
+                }
+            }
+            tr[s4] = ar;
+            ti[s4] = ai;
+        }
+        __syncthreads();
+#pragma unroll
+        for (int s4 = 0; s4 < 4; ++s4) {
+            const int tile = wave + 4 * s4;
+            if (tile < nt * nt) {
+                const int r0 = (tile / nt) * 16, c0 = (tile % nt) * 16;
+#pragma unroll
+                for (int q = 0; q < 4; ++q) {
+                    Xr[(r0 + 4 * lq + q) * ld + c0 + lr] = tr[s4][q];
+                    Xi[(r0 + 4 * lq + q) * ld + c0 + lr] = ti[s4][q];
+                }
+            }
+        }
+        if (c + 1 < C) prefetch(c + 1);
+        __syncthreads();
+        // Y = T U:  Yr = Tr Ur - Ti Ui,  Yi = Tr Ui + Ti Ur
+        for (int tile = wave; tile < nt * nt; tile += 4) {
+            const int r0 = (tile / nt) * 16, c0 = (tile % nt) * 16;
+            f32x4 yr = {0.f, 0.f, 0.f, 0.f}, yi = {0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+            for (int k0 = 0; k0 < np; k0 += 4) {
+                const int kc = k0 + lq;
+                const float tr_ = Xr[(r0 + lr) * ld + kc], ti_ = Xi[(r0 + lr) * ld + kc];
+                const float ur = Ur[kc * ld + c0 + lr], ui = Ui[kc * ld + c0 + lr];
+                yr = __builtin_amdgcn_mfma_f32_16x16x4f32(tr_, ur, yr, 0, 0, 0);
+                yr = __builtin_amdgcn_mfma_f32_16x16x4f32(-ti_, ui, yr, 0, 0, 0);
+                yi = __builtin_amdgcn_mfma_f32_16x16x4f32(tr_, ui, yi, 0, 0, 0);
+                yi = __builtin_amdgcn_mfma_f32_16x16x4f32(ti_, ur, yi, 0, 0, 0);
+            }
+#pragma unroll
+            for (int q = 0; q < 4; ++q) {
+                const int row = r0 + 4 * lq + q, col = c0 + lr;
+                if (row < n && col < n) xc[row * n + col] = make_double2((double)yr[q], (double)yi[q]);
             }
         }
     }
@@ -1420,9 +1553,25 @@ hipError_t wn_launch_warm_transform_f64(double *x, const double2 *u, int count, 
 }
 
 hipError_t wn_launch_rotate_f64(double *x, const double2 *u, int count, const WnShape &sh, hipStream_t st,
-                                double2 *scratch) {
+                                double2 *scratch, bool fp32) {
     const int n = sh.n;
     if (n > 128) return hipErrorInvalidValue;
+    if (fp32 && n >= 16 && n <= 64) {
+        const int np = (n + 15) & ~15;
+        const size_t lds = (size_t)4 * np * (np + 1) * sizeof(float);
+#define WN_ROT32(NPV, PFV) \
+    hipLaunchKernelGGL((k_rotate_mfma32<NPV, PFV>), dim3(count), dim3(256), lds, st, x, u, n, sh.C, sh.ldo)
+        if (np == 16)
+            WN_ROT32(16, 1);
+        else if (np == 32)
+            WN_ROT32(32, 4);
+        else if (np == 48)
+            WN_ROT32(48, 9);
+        else
+            WN_ROT32(64, 16);
+#undef WN_ROT32
+        return hipGetLastError();
+    }
     if (n > 64) {
         if (!scratch) return hipErrorInvalidValue;
         if (n <= 96)
@@ -1462,30 +1611,74 @@ struct PairView {
     __device__ __forceinline__ const double2 *dA(int i) const { return xk + (size_t)(WN_CH_DA + i) * n2; }
 };
 
-__device__ void wn_gderiv_row(const PairView &pv, const double *inv, int n, int nb, int m, int x, double2 g[3]) {
+// Lanes per generalized-derivative row (gl = 1, 2, 4 or 8, uniform over the workgroup): the
+// intermediate-state sum over l is split into gl strided parts (one per lane of an aligned lane group) and
+// added by a butterfly over the group.  Few active pairs get wide groups, so that the items of a chunk
+// fill the workgroup in one pass.
+#define WN_GL_MAX 8
+
+// Intermediate-state sums of the generalized derivative over this lane's l = sub, sub + gl, ... (all
+// bands but n and m):  sAD[y] = sum_l A^x_nl D^y_lm - D^y_nl A^x_lm,  sHD[y] = sum_l V^x_nl D^y_lm - D^y_nl V^x_lm.
+// Excluded l get zero weights (D^y_nl = D^y_lm = 0: their products add exact zeros) instead of a branch, so
+// the loads of several l are in flight together.  Column m is read as stored: the interpolated matrices
+// need not be Hermitian (the eigensolver reads only the upper triangle of H).
+template <bool POS>
+__device__ __forceinline__ void wn_gderiv_part(const PairView &pv, const double *inv, int n, int nb, int m, int x,
+                                               int sub, int gl, bool ok, double2 sAD[3], double2 sHD[3]) {
+    const double2 *Vx = pv.V(x);
+    const double2 *Ax = pv.A(x);
+    const double2 *V0 = pv.V(0), *V1 = pv.V(1), *V2 = pv.V(2);
+    const int lend = ok ? nb : 0;
+#pragma unroll 4
+    for (int l = sub; l < lend; l += gl) {
+        const bool skip = (l == n) || (l == m);
+        const double inl = skip ? 0.0 : inv[n * nb + l];
+        const double ilm = skip ? 0.0 : inv[l * nb + m];
+        const int nl = n * nb + l, lm = l * nb + m;
+        const double2 vxnl = Vx[nl], vxlm = Vx[lm];
+        const double2 vnl[3] = {V0[nl], V1[nl], V2[nl]};
+        const double2 vlm[3] = {V0[lm], V1[lm], V2[lm]};
+        double2 axnl = make_double2(0.0, 0.0), axlm = make_double2(0.0, 0.0);
+        if (POS) {
+            axnl = Ax[nl];
+            axlm = Ax[lm];
+        }
+#pragma unroll
+        for (int y = 0; y < 3; ++y) {
+            const double2 dnl = cscale(vnl[y], -inl);
+            const double2 dlm = cscale(vlm[y], -ilm);
+            if (POS) sAD[y] = csub(cfma(axnl, dlm, sAD[y]), cmul(dnl, axlm));
+            sHD[y] = csub(cfma(vxnl, dlm, sHD[y]), cmul(dnl, vxlm));
+        }
+    }
+}
+
+// Row r^y_{nm;x}, y = 0..2, of the generalized derivative, computed by the gl lanes of an aligned lane
+// group (sub = lane index in the group; every lane of the group must call it; lanes with ok = false add
+// nothing).  Every lane of the group returns the row; the caller stores it from one lane.
+__device__ void wn_gderiv_row(const PairView &pv, const double *inv, int n, int nb, int m, int x, double2 g[3],
+                              int sub, int gl, bool ok) {
     double2 sAD[3], sHD[3];
 #pragma unroll
     for (int y = 0; y < 3; ++y) {
         sAD[y] = make_double2(0.0, 0.0);
         sHD[y] = make_double2(0.0, 0.0);
     }
-    const double2 *Vx = pv.V(x);
-    const double2 *Ax = pv.has_pos ? pv.A(x) : nullptr;
-    for (int l = 0; l < nb; ++l) {
-        if (l == n || l == m) continue;
-        const double inl = inv[n * nb + l];
-        const double ilm = inv[l * nb + m];
-        const double2 vxnl = Vx[n * nb + l], vxlm = Vx[l * nb + m];
-        const double2 axnl = Ax ? Ax[n * nb + l] : make_double2(0.0, 0.0);
-        const double2 axlm = Ax ? Ax[l * nb + m] : make_double2(0.0, 0.0);
+    if (pv.has_pos)
+        wn_gderiv_part<true>(pv, inv, n, nb, m, x, sub, gl, ok, sAD, sHD);
+    else
+        wn_gderiv_part<false>(pv, inv, n, nb, m, x, sub, gl, ok, sAD, sHD);
+    for (int o = gl / 2; o > 0; o >>= 1) {
 #pragma unroll
         for (int y = 0; y < 3; ++y) {
-            const double2 dnl = cscale(pv.V(y)[n * nb + l], -inl);
-            const double2 dlm = cscale(pv.V(y)[l * nb + m], -ilm);
-            sAD[y] = csub(cfma(axnl, dlm, sAD[y]), cmul(dnl, axlm));
-            sHD[y] = csub(cfma(vxnl, dlm, sHD[y]), cmul(dnl, vxlm));
+            sAD[y].x += __shfl_xor(sAD[y].x, o);
+            sAD[y].y += __shfl_xor(sAD[y].y, o);
+            sHD[y].x += __shfl_xor(sHD[y].x, o);
+            sHD[y].y += __shfl_xor(sHD[y].y, o);
         }
     }
+    const double2 *Vx = pv.V(x);
+    const double2 *Ax = pv.A(x);
     const double inm = inv[n * nb + m];
     double2 Dnm[3], dA[3], dV[3];
 #pragma unroll
@@ -1545,6 +1738,7 @@ __global__ __launch_bounds__(256, 4) void k_pairs_f64(const double *__restrict__
     }
     __syncthreads();
     const double cutoff = sc.width * sqrt(200.0);
+    const double om_first = sc.nw > 0 ? omega[0] : 0.0, om_last = sc.nw > 0 ? omega[sc.nw - 1] : 0.0;
     // candidate pairs in n-major order, contiguous chunk per thread
     const int cpt = (n2 + 255) / 256;
     const int c0 = t * cpt;
@@ -1556,6 +1750,9 @@ __global__ __launch_bounds__(256, 4) void k_pairs_f64(const double *__restrict__
         if (!(fabs(df) > 1.0e-10)) continue;
         double de = E[m] - E[n];
         if (sc.gaussian) {
+            // no photon energy within the cutoff: (w_0 - de) >= cutoff or (de - w_last) >= cutoff rules out
+            // every j exactly (rounded differences are monotone in w_j)
+            if (om_first - de >= cutoff || de - om_last >= cutoff) continue;
             // first j with w_j > de - cutoff
             int lo = 0, hi = sc.nw;
             while (lo < hi) {
@@ -1612,6 +1809,7 @@ __global__ __launch_bounds__(256, 4) void k_pairs_f64(const double *__restrict__
             double de = E[m] - E[n];
             int first = 0, last = sc.nw - 1;
             if (sc.gaussian) {
+                if (om_first - de >= cutoff || de - om_last >= cutoff) continue;
                 int lo = 0, hi = sc.nw;
                 while (lo < hi) {
                     int mid = (lo + hi) >> 1;
@@ -1638,6 +1836,8 @@ __global__ __launch_bounds__(256, 4) void k_pairs_f64(const double *__restrict__
         }
     }
     if (t == 0) count[k] = active ? total : 0;
+    // running count of active pairs (status words 6-7) for the work figures of the response stage
+    if (t == 0 && active) atomicAdd((unsigned long long *)(flags + 6), (unsigned long long)total);
     __syncthreads();
     if (!active) return;
 
@@ -1666,13 +1866,18 @@ __global__ __launch_bounds__(256, 4) void k_pairs_f64(const double *__restrict__
     __shared__ double2 Vd[WN_PCH][3];
     for (int c0 = 0; c0 < total; c0 += WN_PCH) {
         const int cn = min(WN_PCH, total - c0);
-        for (int item = t; item < cn * 6; item += 256) {
-            const int i = item % cn, dx = item / cn;
+        // gl lanes per item; the loop bound is uniform so that every lane group reaches the butterfly
+        const int gl = cn * 6 <= 256 / WN_GL_MAX ? WN_GL_MAX : (cn * 6 <= 64 ? 4 : (cn * 6 <= 128 ? 2 : 1));
+        for (int ib = 0; ib < cn * 6; ib += 256 / gl) {
+            const int item = ib + t / gl, sub = t % gl;
+            const bool ok = item < cn * 6;
+            const int i = ok ? item % cn : 0, dx = ok ? item / cn : 0;
             const int d = dx / 3, xcomp = dx - 3 * d;
             const int4 mt = pmeta[pbase + c0 + i];
             const int n = d ? mt.y : mt.x, m = d ? mt.x : mt.y;
             double2 g[3];
-            wn_gderiv_row(pv, inv, n, nb, m, xcomp, g);
+            wn_gderiv_row(pv, inv, n, nb, m, xcomp, g, sub, gl, ok);
+            if (!ok || sub != 0) continue;
 #pragma unroll
             for (int y = 0; y < 3; ++y) Gs[i][d][xcomp][y] = g[y];
             if (dx == 0) {
@@ -1720,18 +1925,24 @@ hipError_t wn_launch_pairs_f64(const double *x, const double *e, const double *o
 
 // ------------------------------------------------------------------------------------------------
 // Frequency contraction per chunk of k points.  Workgroup (chunk, 64-frequency tile): wave w
-// owns components [14w, 14w+14), lane = frequency.  For every pair whose window meets the
-// tile, weight = df * delta(dE - w) (exact Gaussian cutoff test per frequency) and
-// acc[c] += weight * Q[c]; the per-chunk sums go to partial[chunk][component][frequency].
-// ------------------------------------------------------------------------------------------------
+// owns components [14w, 14w+14), lane = frequency.  The chunk's pairs are staged in LDS in blocks of
+// 256 in (k, pair) order, those whose frequency window meets the tile are compacted (order kept), and
+// for each of them weight = df * delta(dE - w) (exact Gaussian cutoff test per frequency) and
+// acc[c] += weight * Q[c], with the Q rows of several pairs in flight; the per-chunk sums go to
+// partial[chunk][component][frequency].  The summation order is the (k, pair) order.
 __global__ __launch_bounds__(256) void k_accum(const int4 *__restrict__ chunks, const int *__restrict__ count,
                                               const int4 *__restrict__ pmeta, const double2 *__restrict__ pmd,
                                               const double2 *__restrict__ q, const double *__restrict__ omega,
                                               double2 *__restrict__ partial, int pcap, int ncomp, int nw, int nwt,
                                               int gaussian, double width) {
+    __shared__ int scnt[WN_ACC_KMAX + 1];
+    __shared__ int wsel[4];
+    __shared__ double2 smd[256];
+    __shared__ long long srow[256];
     const int chunk = blockIdx.x / nwt;
     const int wt = blockIdx.x - chunk * nwt;
-    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int t = threadIdx.x;
+    const int wave = t >> 6, lane = t & 63;
     const int w = wt * WN_WT + lane;
     const double om = w < nw ? omega[w] : 0.0;
     const int cbase = wave * 14;
@@ -1743,31 +1954,66 @@ __global__ __launch_bounds__(256) void k_accum(const int4 *__restrict__ chunks, 
 #pragma unroll
     for (int c = 0; c < 14; ++c) acc[c] = make_double2(0.0, 0.0);
     const int4 ch = chunks[chunk];
-    for (int k = ch.x; k < ch.x + ch.y; ++k) {
-        const int cnt = count[k];
-        const size_t base = (size_t)k * pcap;
-        for (int s = 0; s < cnt; ++s) {
-            const int4 mt = pmeta[base + s];
-            if (mt.w <= tlo || mt.z >= thi) continue;
-            const double2 md = pmd[base + s];
-            const double xx = md.x - om;
+    if (t == 0) {
+        int run = 0;
+        for (int j = 0; j < ch.y; ++j) {
+            scnt[j] = run;
+            run += count[ch.x + j];
+        }
+        scnt[ch.y] = run;
+    }
+    __syncthreads();
+    const int total = scnt[ch.y];
+    for (int b0 = 0; b0 < total; b0 += 256) {
+        const int nbk = min(256, total - b0);
+        // stage pair b0 + t, keep it if its window [z, w) meets the tile
+        bool keep = false;
+        long long row = 0;
+        double2 md = make_double2(0.0, 0.0);
+        if (t < nbk) {
+            const int j = b0 + t;
+            int kk = 0;
+            while (scnt[kk + 1] <= j) ++kk;
+            row = (long long)(ch.x + kk) * pcap + (j - scnt[kk]);
+            const int4 mt = pmeta[row];
+            keep = !(mt.w <= tlo || mt.z >= thi);
+            if (keep) md = pmd[row];
+        }
+        // ordered compaction of the kept pairs
+        const unsigned long long bal = __ballot(keep);
+        if (lane == 0) wsel[wave] = __popcll(bal);
+        __syncthreads();
+        int off = 0;
+        for (int v = 0; v < wave; ++v) off += wsel[v];
+        const int nsel = wsel[0] + wsel[1] + wsel[2] + wsel[3];
+        if (keep) {
+            const int pos = off + __popcll(bal & ((1ull << lane) - 1ull));
+            smd[pos] = md;
+            srow[pos] = row;
+        }
+        __syncthreads();
+#pragma unroll 4
+        for (int sidx = 0; sidx < nsel; ++sidx) {
+            const double2 pm = smd[sidx];
+            const double2 *qq = q + srow[sidx] * ncomp + cbase;
+            double2 v[14];
+#pragma unroll
+            for (int c = 0; c < 14; ++c) v[c] = cbase + c < ncomp ? qq[c] : make_double2(0.0, 0.0);
+            const double xx = pm.x - om;
             double dl;
             if (gaussian) {
                 dl = fabs(xx) < cutoff ? gscale * exp(-(xx / width) * (xx / width)) : 0.0;
             } else {
                 dl = lscale / (xx * xx + width * width);
             }
-            const double wgt = w < nw ? md.y * dl : 0.0;
-            const double2 *qq = q + (base + s) * ncomp + cbase;
+            const double wgt = w < nw ? pm.y * dl : 0.0;
 #pragma unroll
             for (int c = 0; c < 14; ++c) {
-                if (cbase + c < ncomp) {
-                    double2 v = qq[c];
-                    acc[c].x += wgt * v.x;
-                    acc[c].y += wgt * v.y;
-                }
+                acc[c].x += wgt * v[c].x;
+                acc[c].y += wgt * v[c].y;
             }
         }
+        __syncthreads();
     }
     if (w < nw) {
         double2 *o = partial + (size_t)chunk * ncomp * nw;
@@ -1782,6 +2028,7 @@ hipError_t wn_launch_accum(const int4 *chunks, int nchunks, const int *count, co
                            const WnScreen &sc, hipStream_t st) {
     const int nwt = (sc.nw + WN_WT - 1) / WN_WT;
     const int ncomp = 27 * sc.nq;
+    if (ncomp > 56) return hipErrorInvalidValue;   // four waves of 14 components
     hipLaunchKernelGGL(k_accum, dim3(nchunks * nwt), dim3(256), 0, st, chunks, count, pmeta, pmd, q, omega, partial,
                        sh.n * sh.n, ncomp, sc.nw, nwt, sc.gaussian, sc.width);
     return hipGetLastError();
