@@ -758,8 +758,8 @@ bool rotate_fp32(const wnlqg_engine *g) {
     return g->precision == WNLQG_PRECISION_MIXED || g->precision == WNLQG_PRECISION_SPLIT_BF16;
 }
 
-// clear the status words: [0] pair-gap flag, [1] max sweeps, [2] geometry flags, [3] first failing k,
-// [4-5] total Jacobi sweeps, [6-7] total active SC/IC pairs
+// clear the status words: [0] pair-gap flag, [2] geometry flags, [3] first failing k, and the counter
+// buckets (max and total Jacobi sweeps, total active SC/IC pairs)
 hipError_t clear_flags(wnlqg_engine *g) { return wn_launch_reset_flags(g->flags.p, g->stream); }
 
 // per-chunk stage after the rotation: SC / IC pairs + frequency contraction, or BCD / AHC
@@ -994,24 +994,27 @@ int enqueue_mesh(wnlqg_engine *g, std::vector<std::array<hipEvent_t, WN_MARKS>> 
             WN_HIP(launch_interp(g, rows, xb));
         }
         mark(bi, 2, s0);
-        unsigned long long *sweep_total = (unsigned long long *)(g->flags.p + 4);
+        int *sweep_max = g->flags.p + WN_CTR_BASE;
+        unsigned long long *sweep_total = (unsigned long long *)(g->flags.p + WN_CTR_BASE + 2);
         const int scheme = eig_scheme(g, bi);
         if (scheme == 2) {
             const double2 *uprev = g->u.p + (size_t)((bi - 1) & 1) * g->batch * g->sh.n * g->sh.n;
             mark(bi, 8, s0);
             WN_HIP(wn_launch_warm_transform_f64(xb, ub, cnt, g->sh, s0, uprev));
             mark(bi, 9, s0);
-            WN_HIP(wn_launch_eig_pass_f64(xb, eb, ub, g->flags.p + 1, sweep_total, cnt, g->sh, g->eig_tpb, 2, s0,
-                                          nullptr, uprev));
+            WN_HIP(wn_launch_eig_pass_f64(xb, eb, ub, sweep_max, sweep_total, cnt, g->sh, g->eig_tpb, 2, s0,
+                                          nullptr, uprev, WN_CTR_STRIDE));
         } else if (scheme == 1) {
-            WN_HIP(wn_launch_eig_pass_f64(xb, eb, ub, g->flags.p + 1, sweep_total, cnt, g->sh, g->eig_tpb, 0, s0));
+            WN_HIP(wn_launch_eig_pass_f64(xb, eb, ub, sweep_max, sweep_total, cnt, g->sh, g->eig_tpb, 0, s0, nullptr,
+                                          nullptr, WN_CTR_STRIDE));
             mark(bi, 8, s0);
             WN_HIP(wn_launch_warm_transform_f64(xb, ub, cnt, g->sh, s0));
             mark(bi, 9, s0);
-            WN_HIP(wn_launch_eig_pass_f64(xb, eb, ub, g->flags.p + 1, sweep_total, cnt, g->sh, g->eig_tpb, 1, s0));
+            WN_HIP(wn_launch_eig_pass_f64(xb, eb, ub, sweep_max, sweep_total, cnt, g->sh, g->eig_tpb, 1, s0, nullptr,
+                                          nullptr, WN_CTR_STRIDE));
         } else {
-            WN_HIP(wn_launch_eig_pass_f64(xb, eb, ub, g->flags.p + 1, sweep_total, cnt, g->sh, g->eig_tpb, -1, s0,
-                                          g->escratch.p));
+            WN_HIP(wn_launch_eig_pass_f64(xb, eb, ub, sweep_max, sweep_total, cnt, g->sh, g->eig_tpb, -1, s0,
+                                          g->escratch.p, nullptr, WN_CTR_STRIDE));
         }
         mark(bi, 3, s0);
         WN_HIP(hipEventRecord(g->ev_ready[buf], s0));
@@ -1167,14 +1170,20 @@ int32_t wnlqg_run_mesh(wnlqg_engine *g, const int32_t mesh[3], int32_t n_lanes, 
         }
     }
     release_events();
-    int fl[WN_FLAG_WORDS] = {};
-    WN_HIP(hipMemcpy(fl, g->flags.p, sizeof(fl), hipMemcpyDeviceToHost));
-    st.max_sweeps = fl[1];
-    unsigned long long swt = 0;
-    std::memcpy(&swt, fl + 4, sizeof(swt));
+    std::vector<int> fl(WN_FLAG_WORDS);
+    WN_HIP(hipMemcpy(fl.data(), g->flags.p, fl.size() * sizeof(int), hipMemcpyDeviceToHost));
+    unsigned long long swt = 0, npairs = 0;
+    st.max_sweeps = 0;
+    for (int b = 0; b < WN_CTR_BUCKETS; ++b) {
+        const int *bk = fl.data() + WN_CTR_BASE + WN_CTR_STRIDE * b;
+        unsigned long long v = 0;
+        st.max_sweeps = std::max(st.max_sweeps, bk[0]);
+        std::memcpy(&v, bk + 2, sizeof(v));
+        swt += v;
+        std::memcpy(&v, bk + 4, sizeof(v));
+        npairs += v;
+    }
     st.mean_sweeps = nk > 0 ? (double)swt / (double)nk : 0.0;
-    unsigned long long npairs = 0;
-    std::memcpy(&npairs, fl + 6, sizeof(npairs));
     st.active_pairs = (int64_t)npairs;
     st.kpoints = nk;
     st.batches = (int)nbatch;

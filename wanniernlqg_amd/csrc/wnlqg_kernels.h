@@ -108,9 +108,11 @@ hipError_t wn_launch_eig_f64(const double *x, double *e, double2 *u, int *sweeps
 // sequence applies (n >= 16, count >= 4).  sweep_total (nullable) accumulates the sweeps executed.
 //   pass 2  after wn_launch_warm_transform_f64(..., uprev): every point started from uprev[k], the
 //           eigenvectors of the same point of the previous mesh plane.
+// ctr_stride: 0 updates sweeps[0] / sweep_total[0]; WN_CTR_STRIDE spreads the updates over the counter
+// buckets of the status words (sweeps = words + WN_CTR_BASE, sweep_total = words + WN_CTR_BASE + 2).
 hipError_t wn_launch_eig_pass_f64(const double *x, double *e, double2 *u, int *sweeps, unsigned long long *sweep_total,
                                   int count, const WnShape &sh, int tpb, int pass, hipStream_t st,
-                                  double2 *vscratch = nullptr, const double2 *uprev = nullptr);
+                                  double2 *vscratch = nullptr, const double2 *uprev = nullptr, int ctr_stride = 0);
 // size in complex elements of the eigensolver / rotation scratch for count points (0 for n <= 64)
 size_t wn_eig_scratch_elems(int n, int count);
 size_t wn_rotate_scratch_elems(int n, int count);
@@ -119,9 +121,14 @@ hipError_t wn_launch_warm_transform_f64(double *x, const double2 *u, int count, 
                                         const double2 *uprev = nullptr);
 // Clears that are safe under stream capture: p[0 .. n) = 0; flags = {0, 0, 0, 0x7f7f7f7f}.
 hipError_t wn_launch_zero(double *p, long long n, hipStream_t st);
-// status words: [0] pair-gap flag, [1] max sweeps, [2] geometry flags, [3] first failing k,
-// [4..5] total sweeps (64-bit), [6..7] total active SC/IC pairs (64-bit)
-#define WN_FLAG_WORDS 8
+// status words: [0] pair-gap flag, [2] geometry flags, [3] first failing k; then WN_CTR_BUCKETS counter
+// buckets of WN_CTR_STRIDE words (128 bytes apart, so that the per-workgroup atomics of a launch spread
+// over many addresses): bucket word [0] max sweeps, [2..3] total sweeps (64-bit), [4..5] total active
+// SC/IC pairs (64-bit); workgroup g adds to bucket g % WN_CTR_BUCKETS.
+#define WN_CTR_BASE 8
+#define WN_CTR_STRIDE 32
+#define WN_CTR_BUCKETS 64
+#define WN_FLAG_WORDS (WN_CTR_BASE + WN_CTR_BUCKETS * WN_CTR_STRIDE)
 hipError_t wn_launch_reset_flags(int *f, hipStream_t st);
 // U^+ X U for channels 1 .. C-1, in place.
 // scratch: wn_rotate_scratch_elems(n, count) complex elements, required for n > 64

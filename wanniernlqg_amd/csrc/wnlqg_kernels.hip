@@ -831,7 +831,7 @@ __global__ __launch_bounds__(TPB) void k_eig_f64(const double *__restrict__ x, d
                                                 double2 *__restrict__ uout, int *__restrict__ sweeps_out,
                                                 unsigned long long *__restrict__ sweep_total, int n, int ldo,
                                                 double tol, int pass, int count, double2 *__restrict__ vscratch,
-                                                const double2 *__restrict__ uprev) {
+                                                const double2 *__restrict__ uprev, int ctr_stride) {
     constexpr int LD = N + 1;
     constexpr int HALF = N / 2;
     constexpr int VITEMS = N * HALF;
@@ -1084,13 +1084,15 @@ __global__ __launch_bounds__(TPB) void k_eig_f64(const double *__restrict__ x, d
         const int i = e / n, j = e - i * n;
         uk[i * n + rank[j]] = cmul(V(i, j), phase[j]);
     }
-    if (t == 0 && sweeps_out) atomicMax(sweeps_out, sweep);
-    if (t == 0 && sweep_total) atomicAdd(sweep_total, (unsigned long long)sweep);
+    // counters: bucket blockIdx % 64, ctr_stride words apart (0: one address)
+    const int bucket = ctr_stride * (int)(blockIdx.x % WN_CTR_BUCKETS);
+    if (t == 0 && sweeps_out) atomicMax(sweeps_out + bucket, sweep);
+    if (t == 0 && sweep_total) atomicAdd(sweep_total + bucket / 2, (unsigned long long)sweep);
 }
 
 hipError_t wn_launch_eig_pass_f64(const double *x, double *e, double2 *u, int *sweeps, unsigned long long *sweep_total,
                                   int count, const WnShape &sh, int tpb, int pass, hipStream_t st, double2 *vscratch,
-                                  const double2 *uprev) {
+                                  const double2 *uprev, int ctr_stride) {
     if (pass == 2 && (!uprev || !wn_warm_start(sh.n, count))) return hipErrorInvalidValue;
     const int n = sh.n;
     const double tol = 1.0e-15;
@@ -1099,7 +1101,7 @@ hipError_t wn_launch_eig_pass_f64(const double *x, double *e, double2 *u, int *s
     // tpb: threads per k point (64, 128 or 256); 0 selects the default of the size class
 #define WN_EIG_LAUNCH(NN, T) \
     hipLaunchKernelGGL((k_eig_f64<NN, T, false>), dim3(grid), dim3(T), 0, st, x, e, u, sweeps, sweep_total, n, sh.ldo, \
-                       tol, pass, count, nullptr, uprev)
+                       tol, pass, count, nullptr, uprev, ctr_stride)
 #define WN_EIG_CASE(NN, DEF)                   \
     if (n <= NN) {                             \
         const int t = tpb ? tpb : DEF;         \
@@ -1124,10 +1126,10 @@ hipError_t wn_launch_eig_pass_f64(const double *x, double *e, double2 *u, int *s
         if (!vscratch) return hipErrorInvalidValue;
         if (n <= 96)
             hipLaunchKernelGGL((k_eig_f64<96, 512, true>), dim3(grid), dim3(512), 0, st, x, e, u, sweeps, sweep_total, n,
-                               sh.ldo, tol, pass, count, vscratch, nullptr);
+                               sh.ldo, tol, pass, count, vscratch, nullptr, ctr_stride);
         else
             hipLaunchKernelGGL((k_eig_f64<128, 512, true>), dim3(grid), dim3(512), 0, st, x, e, u, sweeps, sweep_total,
-                               n, sh.ldo, tol, pass, count, vscratch, nullptr);
+                               n, sh.ldo, tol, pass, count, vscratch, nullptr, ctr_stride);
         return hipGetLastError();
     }
     return hipErrorInvalidValue;
@@ -1140,7 +1142,7 @@ __global__ void k_zero(double *__restrict__ p, long long n) {
 }
 
 __global__ void k_reset_flags(int *__restrict__ f) {
-    if (threadIdx.x < WN_FLAG_WORDS) f[threadIdx.x] = threadIdx.x == 3 ? 0x7f7f7f7f : 0;
+    for (int i = threadIdx.x; i < WN_FLAG_WORDS; i += blockDim.x) f[i] = i == 3 ? 0x7f7f7f7f : 0;
 }
 
 hipError_t wn_launch_zero(double *p, long long n, hipStream_t st) {
@@ -1619,8 +1621,8 @@ struct PairView {
 
 // Intermediate-state sums of the generalized derivative over this lane's l = sub, sub + gl, ... (all
 // bands but n and m):  sAD[y] = sum_l A^x_nl D^y_lm - D^y_nl A^x_lm,  sHD[y] = sum_l V^x_nl D^y_lm - D^y_nl V^x_lm.
-// Excluded l get zero weights (D^y_nl = D^y_lm = 0: their products add exact zeros) instead of a branch, so
-// the loads of several l are in flight together.  Column m is read as stored: the interpolated matrices
+// Excluded l get zero weights (D^y_nl = D^y_lm = 0: their products add exact zeros) instead of a branch.
+// Column m is read as stored: the interpolated matrices
 // need not be Hermitian (the eigensolver reads only the upper triangle of H).
 template <bool POS>
 __device__ __forceinline__ void wn_gderiv_part(const PairView &pv, const double *inv, int n, int nb, int m, int x,
@@ -1629,7 +1631,6 @@ __device__ __forceinline__ void wn_gderiv_part(const PairView &pv, const double 
     const double2 *Ax = pv.A(x);
     const double2 *V0 = pv.V(0), *V1 = pv.V(1), *V2 = pv.V(2);
     const int lend = ok ? nb : 0;
-#pragma unroll 4
     for (int l = sub; l < lend; l += gl) {
         const bool skip = (l == n) || (l == m);
         const double inl = skip ? 0.0 : inv[n * nb + l];
@@ -1708,149 +1709,136 @@ __device__ void wn_gderiv_row(const PairView &pv, const double *inv, int n, int 
     }
 }
 
+// Photon-energy window of a transition energy de: the first and last j with |de - w_j| < cutoff
+// (Gaussian), found from the first w_j above de - cutoff; first = -1 when there is none.  A transition
+// with (w_0 - de) >= cutoff or (de - w_last) >= cutoff has none (rounded differences are monotone in w_j).
+__device__ __forceinline__ void wn_gauss_window(double de, const double *__restrict__ omega, int nw, double cutoff,
+                                                double om_first, double om_last, int &first, int &last) {
+    first = -1;
+    last = -1;
+    if (om_first - de >= cutoff || de - om_last >= cutoff) return;
+    int lo = 0, hi = nw;
+    while (lo < hi) {
+        const int mid = (lo + hi) >> 1;
+        if (omega[mid] > de - cutoff)
+            hi = mid;
+        else
+            lo = mid + 1;
+    }
+    for (int j = lo > 0 ? lo - 1 : 0; j < nw; ++j) {
+        const double xx = de - omega[j];
+        if (fabs(xx) < cutoff) {
+            if (first < 0) first = j;
+            last = j;
+        } else if (omega[j] > de) {
+            break;
+        }
+    }
+}
+
+// Screening and active-pair lists, one wave per k point (four k points per workgroup, no block
+// barriers).  Occupations (CalculationUtils.jl:21-45), df_nm = f_n - f_m, dE = E_m - E_n; candidates
+// (n, m) in n-major order, 64 per round, kept pairs compacted in order with a ballot:
+//   Gaussian: |df| > 1e-10 and some w_j with |dE - w_j| < sqrt(200) width; window [first, last + 1).
+//   Lorentzian: |df| > 1e-10, window all j; the k point is active only if some |dE - w_j| < window.
+// (TransitionScreening.jl:32-94.)  count[k] = number of pairs (0 for an inactive point); with
+// denominator_regularization = 0 an active point with two levels closer than 1e-10 sets flags[0].
+__global__ __launch_bounds__(256) void k_screen(const double *__restrict__ eg, const double *__restrict__ omega,
+                                               int *__restrict__ count, int4 *__restrict__ pmeta,
+                                               double2 *__restrict__ pmd, int *__restrict__ flags, int nk, WnShape sh,
+                                               WnScreen sc) {
+    const int lane = threadIdx.x & 63;
+    const int k = blockIdx.x * 4 + (threadIdx.x >> 6);
+    if (k >= nk) return;
+    const int nb = sh.n;
+    const int n2 = nb * nb;
+    const double el = lane < nb ? eg[(size_t)k * nb + lane] : 0.0;
+    double fl = 0.0;
+    {
+        const double kt = 8.617333262e-5 * sc.temperature;
+        if (kt == 0.0) {
+            fl = el <= sc.mu ? 1.0 : 0.0;
+        } else {
+            const double arg = fmin(fmax((el - sc.mu) / kt, -700.0), 700.0);
+            fl = 1.0 / (exp(arg) + 1.0);
+        }
+    }
+    const double cutoff = sc.width * sqrt(200.0);
+    const double om_first = sc.nw > 0 ? omega[0] : 0.0, om_last = sc.nw > 0 ? omega[sc.nw - 1] : 0.0;
+    const size_t pbase = (size_t)k * n2;
+    int total = 0;
+    bool active = false;
+    for (int c0 = 0; c0 < n2; c0 += 64) {
+        const int c = c0 + lane;
+        const int n = c < n2 ? c / nb : 0, m = c < n2 ? c - n * nb : 0;
+        const double en = __shfl(el, n), em = __shfl(el, m);
+        const double df = __shfl(fl, n) - __shfl(fl, m);
+        const double de = em - en;
+        bool keep = c < n2 && fabs(df) > 1.0e-10;
+        bool flag = false;
+        int first = 0, last = sc.nw - 1;
+        if (keep) {
+            if (sc.gaussian) {
+                wn_gauss_window(de, omega, sc.nw, cutoff, om_first, om_last, first, last);
+                keep = first >= 0;
+                flag = keep;
+            } else if (sc.window > 0.0) {
+                for (int j = 0; j < sc.nw; ++j)
+                    if (fabs(de - omega[j]) < sc.window) {
+                        flag = true;
+                        break;
+                    }
+            } else {
+                flag = true;
+            }
+        }
+        active = active || __ballot(flag) != 0ull;
+        const unsigned long long kept = __ballot(keep);
+        if (keep) {
+            const int slot = total + __popcll(kept & ((1ull << lane) - 1ull));
+            pmeta[pbase + slot] = make_int4(n, m, first, last + 1);
+            pmd[pbase + slot] = make_double2(de, df);
+        }
+        total += __popcll(kept);
+    }
+    if (!active) total = 0;
+    if (lane == 0) count[k] = total;
+    // running count of active pairs (counter buckets) for the work figures of the response stage
+    if (lane == 0 && total > 0)
+        atomicAdd((unsigned long long *)(flags + WN_CTR_BASE + 4 + WN_CTR_STRIDE * (k % WN_CTR_BUCKETS)),
+                  (unsigned long long)total);
+    // eta == 0: bare inverse requires isolated levels (OrthogonalBandDerivatives.jl:66-82)
+    if (active && sc.eta == 0.0) {
+        bool bad = false;
+        for (int j = 0; j < nb; ++j) {
+            const double ej = __shfl(el, j);
+            if (lane < nb && j > lane && !(fabs(ej - el) > 1.0e-10)) bad = true;
+        }
+        if (bad) atomicOr(flags, 1);
+    }
+}
+
+// Generalized derivatives and SC/IC pair kernels of the pairs k_screen listed, one 256-thread
+// workgroup per k point.
 __global__ __launch_bounds__(256, 4) void k_pairs_f64(const double *__restrict__ x, const double *__restrict__ eg,
-                                                  const double *__restrict__ omega, int *__restrict__ count,
-                                                  int4 *__restrict__ pmeta, double2 *__restrict__ pmd,
-                                                  double2 *__restrict__ qout, int *__restrict__ flags, WnShape sh,
-                                                  WnScreen sc) {
+                                                  const int *__restrict__ count, const int4 *__restrict__ pmeta,
+                                                  double2 *__restrict__ qout, WnShape sh, WnScreen sc) {
     __shared__ double E[64];
-    __shared__ double F[64];
-    __shared__ int scan[256];
-    __shared__ int kactive;
     extern __shared__ double inv[];   // [n][n] regularized inverse energy differences
     const int k = blockIdx.x;
     const int t = threadIdx.x;
     const int nb = sh.n;
     const int n2 = nb * nb;
+    const int total = count[k];
+    if (total == 0) return;
     if (t < nb) E[t] = eg[(size_t)k * nb + t];
-    if (t == 0) kactive = 0;
     __syncthreads();
-    if (t < nb) {
-        const double kt = 8.617333262e-5 * sc.temperature;
-        double f;
-        if (kt == 0.0) {
-            f = E[t] <= sc.mu ? 1.0 : 0.0;
-        } else {
-            double arg = fmin(fmax((E[t] - sc.mu) / kt, -700.0), 700.0);
-            f = 1.0 / (exp(arg) + 1.0);
-        }
-        F[t] = f;
-    }
-    __syncthreads();
-    const double cutoff = sc.width * sqrt(200.0);
-    const double om_first = sc.nw > 0 ? omega[0] : 0.0, om_last = sc.nw > 0 ? omega[sc.nw - 1] : 0.0;
-    // candidate pairs in n-major order, contiguous chunk per thread
-    const int cpt = (n2 + 255) / 256;
-    const int c0 = t * cpt;
-    int mycount = 0;
-    int myflag = 0;
-    for (int c = c0; c < c0 + cpt && c < n2; ++c) {
-        int n = c / nb, m = c - n * nb;
-        double df = F[n] - F[m];
-        if (!(fabs(df) > 1.0e-10)) continue;
-        double de = E[m] - E[n];
-        if (sc.gaussian) {
-            // no photon energy within the cutoff: (w_0 - de) >= cutoff or (de - w_last) >= cutoff rules out
-            // every j exactly (rounded differences are monotone in w_j)
-            if (om_first - de >= cutoff || de - om_last >= cutoff) continue;
-            // first j with w_j > de - cutoff
-            int lo = 0, hi = sc.nw;
-            while (lo < hi) {
-                int mid = (lo + hi) >> 1;
-                if (omega[mid] > de - cutoff) hi = mid;
-                else lo = mid + 1;
-            }
-            int j0 = lo > 0 ? lo - 1 : 0;
-            int first = -1, last = -1;
-            for (int j = j0; j < sc.nw; ++j) {
-                double xx = de - omega[j];
-                if (fabs(xx) < cutoff) {
-                    if (first < 0) first = j;
-                    last = j;
-                } else if (omega[j] > de) {
-                    break;
-                }
-            }
-            if (first >= 0) {
-                ++mycount;
-                myflag = 1;
-            }
-        } else {
-            ++mycount;
-            if (sc.window > 0.0) {
-                for (int j = 0; j < sc.nw; ++j)
-                    if (fabs(de - omega[j]) < sc.window) {
-                        myflag = 1;
-                        break;
-                    }
-            } else {
-                myflag = 1;
-            }
-        }
-    }
-    scan[t] = mycount;
-    if (myflag) atomicOr(&kactive, 1);
-    __syncthreads();
-    for (int o = 1; o < 256; o <<= 1) {
-        int v = t >= o ? scan[t - o] : 0;
-        __syncthreads();
-        scan[t] += v;
-        __syncthreads();
-    }
-    const int total = scan[255];
-    const int active = kactive;
-    int slot = scan[t] - mycount;
     const size_t pbase = (size_t)k * n2;
-    if (active) {
-        for (int c = c0; c < c0 + cpt && c < n2; ++c) {
-            int n = c / nb, m = c - n * nb;
-            double df = F[n] - F[m];
-            if (!(fabs(df) > 1.0e-10)) continue;
-            double de = E[m] - E[n];
-            int first = 0, last = sc.nw - 1;
-            if (sc.gaussian) {
-                if (om_first - de >= cutoff || de - om_last >= cutoff) continue;
-                int lo = 0, hi = sc.nw;
-                while (lo < hi) {
-                    int mid = (lo + hi) >> 1;
-                    if (omega[mid] > de - cutoff) hi = mid;
-                    else lo = mid + 1;
-                }
-                int j0 = lo > 0 ? lo - 1 : 0;
-                first = -1;
-                last = -1;
-                for (int j = j0; j < sc.nw; ++j) {
-                    double xx = de - omega[j];
-                    if (fabs(xx) < cutoff) {
-                        if (first < 0) first = j;
-                        last = j;
-                    } else if (omega[j] > de) {
-                        break;
-                    }
-                }
-                if (first < 0) continue;
-            }
-            pmeta[pbase + slot] = make_int4(n, m, first, last + 1);
-            pmd[pbase + slot] = make_double2(de, df);
-            ++slot;
-        }
-    }
-    if (t == 0) count[k] = active ? total : 0;
-    // running count of active pairs (status words 6-7) for the work figures of the response stage
-    if (t == 0 && active) atomicAdd((unsigned long long *)(flags + 6), (unsigned long long)total);
-    __syncthreads();
-    if (!active) return;
-
     for (int e = t; e < n2; e += 256) {
         const int i = e / nb, j = e - i * nb;
         inv[e] = wn_reg_inverse(E[i] - E[j], sc.eta);
     }
     __syncthreads();
-    // eta == 0: bare inverse requires isolated levels (OrthogonalBandDerivatives.jl:66-82)
-    if (sc.eta == 0.0 && t < nb) {
-        for (int j = t + 1; j < nb; ++j)
-            if (!(fabs(E[j] - E[t]) > 1.0e-10)) atomicOr(flags, 1);
-    }
 
     const double2 *xk = (const double2 *)(x + (size_t)k * sh.ldo);
     PairView pv;
@@ -1918,8 +1906,10 @@ __global__ __launch_bounds__(256, 4) void k_pairs_f64(const double *__restrict__
 hipError_t wn_launch_pairs_f64(const double *x, const double *e, const double *omega, int *count, int4 *pmeta,
                                double2 *pmd, double2 *q, int *flags, int nk, const WnShape &sh, const WnScreen &sc,
                                hipStream_t st) {
-    hipLaunchKernelGGL(k_pairs_f64, dim3(nk), dim3(256), (size_t)sh.n * sh.n * sizeof(double), st, x, e, omega, count,
-                       pmeta, pmd, q, flags, sh, sc);
+    if (sh.n > 64) return hipErrorInvalidValue;
+    hipLaunchKernelGGL(k_screen, dim3((nk + 3) / 4), dim3(256), 0, st, e, omega, count, pmeta, pmd, flags, nk, sh, sc);
+    hipLaunchKernelGGL(k_pairs_f64, dim3(nk), dim3(256), (size_t)sh.n * sh.n * sizeof(double), st, x, e, count, pmeta, q,
+                       sh, sc);
     return hipGetLastError();
 }
 
@@ -1928,7 +1918,7 @@ hipError_t wn_launch_pairs_f64(const double *x, const double *e, const double *o
 // owns components [14w, 14w+14), lane = frequency.  The chunk's pairs are staged in LDS in blocks of
 // 256 in (k, pair) order, those whose frequency window meets the tile are compacted (order kept), and
 // for each of them weight = df * delta(dE - w) (exact Gaussian cutoff test per frequency) and
-// acc[c] += weight * Q[c], with the Q rows of several pairs in flight; the per-chunk sums go to
+// acc[c] += weight * Q[c]; the per-chunk sums go to
 // partial[chunk][component][frequency].  The summation order is the (k, pair) order.
 __global__ __launch_bounds__(256) void k_accum(const int4 *__restrict__ chunks, const int *__restrict__ count,
                                               const int4 *__restrict__ pmeta, const double2 *__restrict__ pmd,
@@ -1992,7 +1982,6 @@ __global__ __launch_bounds__(256) void k_accum(const int4 *__restrict__ chunks, 
             srow[pos] = row;
         }
         __syncthreads();
-#pragma unroll 4
         for (int sidx = 0; sidx < nsel; ++sidx) {
             const double2 pm = smd[sidx];
             const double2 *qq = q + srow[sidx] * ncomp + cbase;
