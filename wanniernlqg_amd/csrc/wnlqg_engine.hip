@@ -124,7 +124,7 @@ struct wnlqg_engine {
     DevBuf<double> hr, ar, rc, bf, kp, x, e, omega;
     DevBuf<int> deg, sweeps, count, flags;
     DevBuf<int4> ridx, pmeta, chunks, lanes;
-    DevBuf<double2> tab, u, pmd, q, partial, lanebuf, lanecomp;
+    DevBuf<double2> tab, u, pmd, q, pwd, partial, lanebuf, lanecomp;
     DevBuf<float2> tab32;
     DevBuf<__bf16> bh, bm, bl;  // split-BF16 coefficient parts (alternative mixed kernel)
     DevBuf<_Float16> fh, fm;    // split-FP16 coefficient parts (mixed forms)
@@ -283,6 +283,7 @@ int ensure_rows(wnlqg_engine *g, int rows) {
         WN_HIP(g->pmeta.alloc((size_t)rows * n2));
         WN_HIP(g->pmd.alloc((size_t)rows * n2));
         WN_HIP(g->q.alloc((size_t)rows * n2 * 27 * g->sc.nq));
+        WN_HIP(g->pwd.alloc((size_t)rows * WN_PCAP * 30));
     }
     return WNLQG_OK;
 }
@@ -758,19 +759,29 @@ bool rotate_fp32(const wnlqg_engine *g) {
     return g->precision == WNLQG_PRECISION_MIXED || g->precision == WNLQG_PRECISION_SPLIT_BF16;
 }
 
+// Hamiltonian gauge of cnt points in (x, e, u).  SC / IC: screening first, then the rotation of what the
+// pairs use, then the pair kernels (the frequency contraction follows in launch_response); other
+// families: every channel.
+hipError_t launch_rotation(wnlqg_engine *g, double *x, const double *e, const double2 *u, int cnt, hipStream_t st) {
+    const bool fp32 = rotate_fp32(g);
+    if (g->family != 0) return wn_launch_rotate_f64(x, u, cnt, g->sh, st, g->rscratch.p, fp32);
+    hipError_t r = wn_launch_screen(e, g->omega.p, g->count.p, g->pmeta.p, g->pmd.p, g->flags.p, cnt, g->sh, g->sc, st);
+    if (r != hipSuccess) return r;
+    r = wn_launch_rotate_sc(x, u, g->count.p, g->pmeta.p, g->pwd.p, cnt, g->sh, st, g->rscratch.p, fp32);
+    if (r != hipSuccess) return r;
+    return wn_launch_pairs_f64(x, e, g->count.p, g->pmeta.p, wn_rotate_sc_elements(g->sh, fp32) ? g->pwd.p : nullptr,
+                               g->q.p, cnt, g->sh, g->sc, st);
+}
+
 // clear the status words: [0] pair-gap flag, [2] geometry flags, [3] first failing k, and the counter
 // buckets (max and total Jacobi sweeps, total active SC/IC pairs)
 hipError_t clear_flags(wnlqg_engine *g) { return wn_launch_reset_flags(g->flags.p, g->stream); }
 
 // per-chunk stage after the rotation: SC / IC pairs + frequency contraction, or BCD / AHC
 hipError_t launch_response(wnlqg_engine *g, const int4 *chunks, int nch, int cnt, long long kbase) {
-    if (g->family == 0) {
-        hipError_t e = wn_launch_pairs_f64(g->x.p, g->e.p, g->omega.p, g->count.p, g->pmeta.p, g->pmd.p, g->q.p,
-                                           g->flags.p, cnt, g->sh, g->sc, g->stream);
-        if (e != hipSuccess) return e;
+    if (g->family == 0)
         return wn_launch_accum(chunks, nch, g->count.p, g->pmeta.p, g->pmd.p, g->q.p, g->omega.p, g->partial.p, g->sh,
                                g->sc, g->stream);
-    }
     return wn_launch_geom_f64(g->x.p, g->e.p, g->omega.p, chunks, nch, g->partial.p, g->flags.p, kbase, g->sh, g->gp,
                               g->stream);
 }
@@ -798,7 +809,7 @@ int32_t wnlqg_eval_points_spectrum(wnlqg_engine *g, const double *kpoints, int32
         if (energies)
             WN_HIP(hipMemcpyAsync(energies + (size_t)p0 * g->sh.n, g->e.p, (size_t)cnt * g->sh.n * sizeof(double),
                                   hipMemcpyDeviceToHost, g->stream));
-        WN_HIP(wn_launch_rotate_f64(g->x.p, g->u.p, cnt, g->sh, g->stream, g->rscratch.p, rotate_fp32(g)));
+        WN_HIP(launch_rotation(g, g->x.p, g->e.p, g->u.p, cnt, g->stream));
         rc = ensure_partial(g, cnt);
         if (rc) return rc;
         std::vector<int4> ch(cnt);
@@ -1021,14 +1032,24 @@ int enqueue_mesh(wnlqg_engine *g, std::vector<std::array<hipEvent_t, WN_MARKS>> 
         // stream 1: rotation, response and k sums of batch bi
         WN_HIP(hipStreamWaitEvent(s1, g->ev_ready[buf], 0));
         if (marks) hipEventRecord((*marks)[bi][7], s1);
-        WN_HIP(wn_launch_rotate_f64(xb, ub, cnt, g->sh, s1, g->rscratch.p, rotate_fp32(g)));
+        const bool fp32 = rotate_fp32(g);
+        if (g->family == 0) {
+            // SC / IC: screening first, so that the rotation skips points without pairs and reduces dd H and
+            // dA to the pairs' elements
+            WN_HIP(wn_launch_screen(eb, g->omega.p, g->count.p, g->pmeta.p, g->pmd.p, g->flags.p, cnt, g->sh, g->sc,
+                                    s1));
+            WN_HIP(wn_launch_rotate_sc(xb, ub, g->count.p, g->pmeta.p, g->pwd.p, cnt, g->sh, s1, g->rscratch.p, fp32));
+        } else {
+            WN_HIP(wn_launch_rotate_f64(xb, ub, cnt, g->sh, s1, g->rscratch.p, fp32));
+        }
         mark(bi, 4, s1);
         const int nch = (int)(p.bchunk[bi + 1] - p.bchunk[bi]);
         const int nln = (int)(p.blane[bi + 1] - p.blane[bi]);
         const int4 *chunks = p.dchunks.p + p.bchunk[bi];
         if (g->family == 0) {
-            WN_HIP(wn_launch_pairs_f64(xb, eb, g->omega.p, g->count.p, g->pmeta.p, g->pmd.p, g->q.p, g->flags.p, cnt,
-                                       g->sh, g->sc, s1));
+            WN_HIP(wn_launch_pairs_f64(xb, eb, g->count.p, g->pmeta.p,
+                                       wn_rotate_sc_elements(g->sh, fp32) ? g->pwd.p : nullptr, g->q.p, cnt, g->sh,
+                                       g->sc, s1));
             mark(bi, 5, s1);
             WN_HIP(wn_launch_accum(chunks, nch, g->count.p, g->pmeta.p, g->pmd.p, g->q.p, g->omega.p, g->partial.p,
                                    g->sh, g->sc, s1));

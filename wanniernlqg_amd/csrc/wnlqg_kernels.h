@@ -136,8 +136,20 @@ hipError_t wn_launch_reset_flags(int *f, hipStream_t st);
 hipError_t wn_launch_rotate_f64(double *x, const double2 *u, int count, const WnShape &sh, hipStream_t st,
                                 double2 *scratch = nullptr, bool fp32 = false);
 // Screening, pair lists, generalized derivatives and SC/IC kernels.
-hipError_t wn_launch_pairs_f64(const double *x, const double *e, const double *omega, int *count, int4 *pmeta,
-                               double2 *pmd, double2 *q, int *flags, int nk, const WnShape &sh, const WnScreen &sc,
+// (k_screen: occupations, windows and the active pair list of every k point, before the rotation.)
+hipError_t wn_launch_screen(const double *e, const double *omega, int *count, int4 *pmeta, double2 *pmd, int *flags,
+                            int nk, const WnShape &sh, const WnScreen &sc, hipStream_t st);
+// SC/IC rotation: points without pairs skipped; points with at most WN_PCAP pairs get dH and A rotated in
+// full and the pairs' dd H / dA elements in pwd[k][pair][direction][15] (wn_rotate_sc_elements true);
+// otherwise, or when wn_rotate_sc_elements is false, every channel is rotated (wn_launch_rotate_f64).
+#define WN_PCAP 8
+bool wn_rotate_sc_elements(const WnShape &sh, bool fp32);
+hipError_t wn_launch_rotate_sc(double *x, const double2 *u, const int *count, const int4 *pmeta, double2 *pwd,
+                               int nk, const WnShape &sh, hipStream_t st, double2 *scratch, bool fp32);
+// Generalized derivatives and SC/IC kernels of the screened pairs (pwd: nullptr to read every element from
+// the rotated channels).
+hipError_t wn_launch_pairs_f64(const double *x, const double *e, const int *count, const int4 *pmeta,
+                               const double2 *pwd, double2 *q, int nk, const WnShape &sh, const WnScreen &sc,
                                hipStream_t st);
 // Frequency contraction of the pairs of each chunk (chunks of at most WN_ACC_KMAX k points).
 #define WN_ACC_KMAX 64

@@ -1452,6 +1452,206 @@ __global__ __launch_bounds__(256) void k_rotate_mfma32(double *__restrict__ x, c
     }
 }
 
+// Rotation for the shift/injection-current pair kernels (16 <= n <= 64, FP64).  A point without active
+// pairs is skipped (nothing reads its channels).  A point with at most WN_PCAP active pairs rotates the
+// velocity and position channels (dH_a, A_a: the pair kernels sum their rows and columns over all
+// intermediate bands) in full, and of dd H and dA only the matrix elements the pairs use: with B the bands
+// of the pairs (at most 16), T_B = U_B^+ X (one 16-row MFMA tile per column tile), then
+// (U^+ X U)_ab = sum_q T_B[a][q] U[q][b] for (a, b) = (n, m) and (m, n), written to
+// pwd[k][pair][direction][15] (ddH channels 0..5, then dA 6..14) and not back to the channel.  Points with
+// more pairs rotate every channel.  Products as in k_rotate_mfma (three real MFMA products per complex
+// product).
+template <int NP, int PF>
+__global__ __launch_bounds__(256) void k_rotate_sc(double *__restrict__ x, const double2 *__restrict__ u,
+                                                  const int *__restrict__ count, const int4 *__restrict__ pmeta,
+                                                  double2 *__restrict__ pwd, int n, int C, int ldo) {
+    extern __shared__ double smd[];
+    __shared__ int bl[16];            // bands of the pairs
+    __shared__ int prow[WN_PCAP][2];  // row of n and of m in bl, per pair
+    __shared__ int pband[WN_PCAP][2]; // n and m
+    __shared__ int nbl;
+    const int k = blockIdx.x;
+    const int cnt = count[k];
+    if (cnt == 0) return;
+    const bool sparse = cnt <= WN_PCAP;
+    const int t = threadIdx.x;
+    const int lane = t & 63, wave = t >> 6;
+    constexpr int np = NP;
+    constexpr int ld = np + 1;
+    constexpr int nt = np >> 4;
+    double *Ur = smd;
+    double *Ui = Ur + np * ld;
+    double *Xr = Ui + np * ld;
+    double *Xi = Xr + np * ld;
+    const double2 *uk = u + (size_t)k * n * n;
+    if (sparse && t == 0) {
+        int nb_ = 0;
+        for (int p = 0; p < cnt; ++p) {
+            const int4 mt = pmeta[(size_t)k * n * n + p];
+            const int ab[2] = {mt.x, mt.y};
+            for (int d = 0; d < 2; ++d) {
+                int r = 0;
+                while (r < nb_ && bl[r] != ab[d]) ++r;
+                if (r == nb_) bl[nb_++] = ab[d];
+                prow[p][d] = r;
+                pband[p][d] = ab[d];
+            }
+        }
+        nbl = nb_;
+    }
+    for (int e = t; e < np * np; e += 256) {
+        const int i = e / np, j = e - i * np;
+        const double2 v = (i < n && j < n) ? uk[i * n + j] : make_double2(0.0, 0.0);
+        Ur[i * ld + j] = v.x;
+        Ui[i * ld + j] = v.y;
+    }
+    double2 *xk = (double2 *)(x + (size_t)k * ldo);
+    const int lr = lane & 15, lq = lane >> 4;
+    double2 pf[PF];
+    auto prefetch = [&](int c) {
+        const double2 *xc = xk + (size_t)c * n * n;
+#pragma unroll
+        for (int q = 0; q < PF; ++q) {
+            const int e = t + 256 * q;
+            const int i = e / np, j = e - (e / np) * np;
+            pf[q] = (e < np * np && i < n && j < n) ? xc[i * n + j] : make_double2(0.0, 0.0);
+        }
+    };
+    prefetch(1);
+    for (int c = 1; c < C; ++c) {
+        double2 *xc = xk + (size_t)c * n * n;
+        const bool full = !sparse || (c >= WN_CH_DH && c < WN_CH_DDH) || (c >= WN_CH_A && c < WN_CH_DA);
+        __syncthreads();
+#pragma unroll
+        for (int q = 0; q < PF; ++q) {
+            const int e = t + 256 * q;
+            if (e < np * np) {
+                const int i = e / np, j = e - (e / np) * np;
+                Xr[i * ld + j] = pf[q].x;
+                Xi[i * ld + j] = pf[q].y;
+            }
+        }
+        __syncthreads();
+        if (full) {
+            // T = U^+ X in registers, then to LDS
+            f64x4 tr[4], ti[4];
+#pragma unroll
+            for (int s4 = 0; s4 < 4; ++s4) {
+                const int tile = wave + 4 * s4;
+                f64x4 p1 = {0.0, 0.0, 0.0, 0.0}, p2 = {0.0, 0.0, 0.0, 0.0}, p3 = {0.0, 0.0, 0.0, 0.0};
+                if (tile < nt * nt) {
+                    const int r0 = (tile / nt) * 16, c0 = (tile % nt) * 16;
+#pragma unroll
+                    for (int k0 = 0; k0 < np; k0 += 4) {
+                        const int kr = k0 + lq;
+                        const double ur = Ur[kr * ld + r0 + lr], ui = Ui[kr * ld + r0 + lr];
+                        const double xr = Xr[kr * ld + c0 + lr], xi = Xi[kr * ld + c0 + lr];
+                        p1 = __builtin_amdgcn_mfma_f64_16x16x4f64(ur, xr, p1, 0, 0, 0);
+                        p2 = __builtin_amdgcn_mfma_f64_16x16x4f64(ui, xi, p2, 0, 0, 0);
+                        p3 = __builtin_amdgcn_mfma_f64_16x16x4f64(ur + ui, xi - xr, p3, 0, 0, 0);
+                    }
+                }
+#pragma unroll
+                for (int q = 0; q < 4; ++q) {
+                    tr[s4][q] = p1[q] + p2[q];
+                    ti[s4][q] = p3[q] + (p1[q] - p2[q]);
+                }
+            }
+            __syncthreads();
+#pragma unroll
+            for (int s4 = 0; s4 < 4; ++s4) {
+                const int tile = wave + 4 * s4;
+                if (tile < nt * nt) {
+                    const int r0 = (tile / nt) * 16, c0 = (tile % nt) * 16;
+#pragma unroll
+                    for (int q = 0; q < 4; ++q) {
+                        Xr[(r0 + lq + 4 * q) * ld + c0 + lr] = tr[s4][q];
+                        Xi[(r0 + lq + 4 * q) * ld + c0 + lr] = ti[s4][q];
+                    }
+                }
+            }
+            if (c + 1 < C) prefetch(c + 1);
+            __syncthreads();
+            // Y = T U, written to global
+            for (int tile = wave; tile < nt * nt; tile += 4) {
+                const int r0 = (tile / nt) * 16, c0 = (tile % nt) * 16;
+                f64x4 p1 = {0.0, 0.0, 0.0, 0.0}, p2 = {0.0, 0.0, 0.0, 0.0}, p3 = {0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+                for (int k0 = 0; k0 < np; k0 += 4) {
+                    const int kc = k0 + lq;
+                    const double tr_ = Xr[(r0 + lr) * ld + kc], ti_ = Xi[(r0 + lr) * ld + kc];
+                    const double ur = Ur[kc * ld + c0 + lr], ui = Ui[kc * ld + c0 + lr];
+                    p1 = __builtin_amdgcn_mfma_f64_16x16x4f64(tr_, ur, p1, 0, 0, 0);
+                    p2 = __builtin_amdgcn_mfma_f64_16x16x4f64(ti_, ui, p2, 0, 0, 0);
+                    p3 = __builtin_amdgcn_mfma_f64_16x16x4f64(tr_ + ti_, ur + ui, p3, 0, 0, 0);
+                }
+#pragma unroll
+                for (int q = 0; q < 4; ++q) {
+                    const int row = r0 + lq + 4 * q, col = c0 + lr;
+                    if (row < n && col < n) xc[row * n + col] = make_double2(p1[q] - p2[q], p3[q] - (p1[q] + p2[q]));
+                }
+            }
+        } else {
+            // T_B = U_B^+ X: one 16-row tile (row r <-> band bl[r], zero rows beyond nbl), nt column tiles
+            const int nb_ = nbl;
+            f64x4 tr = {0.0, 0.0, 0.0, 0.0}, ti = {0.0, 0.0, 0.0, 0.0};
+            if (wave < nt) {
+                const int c0 = wave * 16;
+                const int br = lr < nb_ ? bl[lr] : 0;
+                const double rm = lr < nb_ ? 1.0 : 0.0;
+                f64x4 p1 = {0.0, 0.0, 0.0, 0.0}, p2 = {0.0, 0.0, 0.0, 0.0}, p3 = {0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+                for (int k0 = 0; k0 < np; k0 += 4) {
+                    const int kr = k0 + lq;
+                    const double ur = rm * Ur[kr * ld + br], ui = rm * Ui[kr * ld + br];
+                    const double xr = Xr[kr * ld + c0 + lr], xi = Xi[kr * ld + c0 + lr];
+                    p1 = __builtin_amdgcn_mfma_f64_16x16x4f64(ur, xr, p1, 0, 0, 0);
+                    p2 = __builtin_amdgcn_mfma_f64_16x16x4f64(ui, xi, p2, 0, 0, 0);
+                    p3 = __builtin_amdgcn_mfma_f64_16x16x4f64(ur + ui, xi - xr, p3, 0, 0, 0);
+                }
+#pragma unroll
+                for (int q = 0; q < 4; ++q) {
+                    tr[q] = p1[q] + p2[q];
+                    ti[q] = p3[q] + (p1[q] - p2[q]);
+                }
+            }
+            __syncthreads();
+            if (wave < nt) {
+                const int c0 = wave * 16;
+#pragma unroll
+                for (int q = 0; q < 4; ++q) {
+                    Xr[(lq + 4 * q) * ld + c0 + lr] = tr[q];
+                    Xi[(lq + 4 * q) * ld + c0 + lr] = ti[q];
+                }
+            }
+            if (c + 1 < C) prefetch(c + 1);
+            __syncthreads();
+            // matrix elements: 16 lanes per (pair, direction), q strided over the lanes, butterfly sum
+            const int item = t >> 4, sub = t & 15;
+            const int p = item >> 1, d = item & 1;
+            const bool ok = p < cnt;
+            const int ra = ok ? prow[p][d] : 0, bb = ok ? pband[p][d ^ 1] : 0;
+            double sr = 0.0, si = 0.0;
+            if (ok) {
+                for (int q = sub; q < np; q += 16) {
+                    const double tr_ = Xr[ra * ld + q], ti_ = Xi[ra * ld + q];
+                    const double ur = Ur[q * ld + bb], ui = Ui[q * ld + bb];
+                    sr += tr_ * ur - ti_ * ui;
+                    si += tr_ * ui + ti_ * ur;
+                }
+            }
+            for (int o = 8; o > 0; o >>= 1) {
+                sr += __shfl_xor(sr, o);
+                si += __shfl_xor(si, o);
+            }
+            if (ok && sub == 0) {
+                const int slot = c < WN_CH_A ? c - WN_CH_DDH : 6 + (c - WN_CH_DA);
+                pwd[(((size_t)k * WN_PCAP + p) * 2 + d) * 15 + slot] = make_double2(sr, si);
+            }
+        }
+    }
+}
+
 // Rotation for n in (64, 128]: U and X are read through the L2 as MFMA operands and U^+ X is staged
 // in a per-k scratch [NP][NP] (the LDS cannot hold U, X and U^+ X at these sizes).  One 256-thread
 // workgroup per k, channels 1 .. C-1 in turn; operand layouts as in k_rotate_mfma.
@@ -1540,6 +1740,28 @@ static void launch_rotate_mfma(double *x, const double2 *u, int count, const WnS
 #undef WN_ROT
 }
 
+bool wn_rotate_sc_elements(const WnShape &sh, bool fp32) { return !fp32 && sh.n >= 16 && sh.n <= 64; }
+
+hipError_t wn_launch_rotate_sc(double *x, const double2 *u, const int *count, const int4 *pmeta, double2 *pwd,
+                               int nk, const WnShape &sh, hipStream_t st, double2 *scratch, bool fp32) {
+    if (!wn_rotate_sc_elements(sh, fp32)) return wn_launch_rotate_f64(x, u, nk, sh, st, scratch, fp32);
+    const int np = (sh.n + 15) & ~15;
+    const size_t lds = (size_t)4 * np * (np + 1) * sizeof(double);
+#define WN_ROTSC(NPV, PFV)                                                                                      \
+    hipLaunchKernelGGL((k_rotate_sc<NPV, PFV>), dim3(nk), dim3(256), lds, st, x, u, count, pmeta, pwd, sh.n, sh.C, \
+                       sh.ldo)
+    if (np == 16)
+        WN_ROTSC(16, 1);
+    else if (np == 32)
+        WN_ROTSC(32, 4);
+    else if (np == 48)
+        WN_ROTSC(48, 9);
+    else
+        WN_ROTSC(64, 16);
+#undef WN_ROTSC
+    return hipGetLastError();
+}
+
 hipError_t wn_launch_eig_f64(const double *x, double *e, double2 *u, int *sweeps, int count, const WnShape &sh,
                              int tpb, hipStream_t st, double2 *vscratch) {
     return wn_launch_eig_pass_f64(x, e, u, sweeps, nullptr, count, sh, tpb, -1, st, vscratch, nullptr);
@@ -1607,6 +1829,7 @@ struct PairView {
     const double2 *xk;
     int n2;
     int has_pos;
+    const double2 *pw;   // the pair's dd H and dA elements (k_rotate_sc), per direction; nullptr: read the channels
     __device__ __forceinline__ const double2 *V(int y) const { return xk + (size_t)(WN_CH_DH + y) * n2; }
     __device__ __forceinline__ const double2 *W(int s) const { return xk + (size_t)(WN_CH_DDH + s) * n2; }
     __device__ __forceinline__ const double2 *A(int y) const { return xk + (size_t)(WN_CH_A + y) * n2; }
@@ -1699,8 +1922,9 @@ __device__ void wn_gderiv_row(const PairView &pv, const double *inv, int n, int 
         const double2 ad_bit = cadd(cmul(dA[y], Dnm_x), cmul(dA_x, Dnm[y]));
         const double2 aa_bit = cmul(dA[y], Anm);
         const double2 dv_bit = cadd(cmul(dV[y], Dnm_x), cmul(dV_x, Dnm[y]));
-        const double2 da = pv.has_pos ? pv.dA(3 * x + y)[n * nb + m] : make_double2(0.0, 0.0);
-        const double2 w = pv.W(wn_sym(x, y))[n * nb + m];
+        const double2 da = pv.has_pos ? (pv.pw ? pv.pw[6 + 3 * x + y] : pv.dA(3 * x + y)[n * nb + m])
+                                      : make_double2(0.0, 0.0);
+        const double2 w = pv.pw ? pv.pw[wn_sym(x, y)] : pv.W(wn_sym(x, y))[n * nb + m];
         double2 gg = cadd(da, ad_bit);
         gg = csub(gg, cmuli(aa_bit));
         gg = cadd(gg, sAD[y]);
@@ -1822,7 +2046,8 @@ __global__ __launch_bounds__(256) void k_screen(const double *__restrict__ eg, c
 // workgroup per k point.
 __global__ __launch_bounds__(256, 4) void k_pairs_f64(const double *__restrict__ x, const double *__restrict__ eg,
                                                   const int *__restrict__ count, const int4 *__restrict__ pmeta,
-                                                  double2 *__restrict__ qout, WnShape sh, WnScreen sc) {
+                                                  const double2 *__restrict__ pwd, double2 *__restrict__ qout,
+                                                  WnShape sh, WnScreen sc) {
     __shared__ double E[64];
     extern __shared__ double inv[];   // [n][n] regularized inverse energy differences
     const int k = blockIdx.x;
@@ -1845,6 +2070,9 @@ __global__ __launch_bounds__(256, 4) void k_pairs_f64(const double *__restrict__
     pv.xk = xk;
     pv.n2 = n2;
     pv.has_pos = sh.has_pos;
+    pv.pw = nullptr;
+    // dd H and dA elements precomputed by k_rotate_sc for points with at most WN_PCAP pairs
+    const bool compact = pwd != nullptr && total <= WN_PCAP;
     const int ncomp = 27 * sc.nq;
     // Chunks of WN_PCH pairs.  Phase A: one item per (pair, direction d, component x) forms
     // r^y_{n'm';x} for y = 0..2 with (n', m') = (n, m) or (m, n); one item per pair also stages the
@@ -1864,7 +2092,9 @@ __global__ __launch_bounds__(256, 4) void k_pairs_f64(const double *__restrict__
             const int4 mt = pmeta[pbase + c0 + i];
             const int n = d ? mt.y : mt.x, m = d ? mt.x : mt.y;
             double2 g[3];
-            wn_gderiv_row(pv, inv, n, nb, m, xcomp, g, sub, gl, ok);
+            PairView pvi = pv;
+            if (compact) pvi.pw = pwd + (((size_t)k * WN_PCAP + c0 + i) * 2 + d) * 15;
+            wn_gderiv_row(pvi, inv, n, nb, m, xcomp, g, sub, gl, ok);
             if (!ok || sub != 0) continue;
 #pragma unroll
             for (int y = 0; y < 3; ++y) Gs[i][d][xcomp][y] = g[y];
@@ -1903,13 +2133,19 @@ __global__ __launch_bounds__(256, 4) void k_pairs_f64(const double *__restrict__
     }
 }
 
-hipError_t wn_launch_pairs_f64(const double *x, const double *e, const double *omega, int *count, int4 *pmeta,
-                               double2 *pmd, double2 *q, int *flags, int nk, const WnShape &sh, const WnScreen &sc,
-                               hipStream_t st) {
+hipError_t wn_launch_screen(const double *e, const double *omega, int *count, int4 *pmeta, double2 *pmd, int *flags,
+                            int nk, const WnShape &sh, const WnScreen &sc, hipStream_t st) {
     if (sh.n > 64) return hipErrorInvalidValue;
     hipLaunchKernelGGL(k_screen, dim3((nk + 3) / 4), dim3(256), 0, st, e, omega, count, pmeta, pmd, flags, nk, sh, sc);
-    hipLaunchKernelGGL(k_pairs_f64, dim3(nk), dim3(256), (size_t)sh.n * sh.n * sizeof(double), st, x, e, count, pmeta, q,
-                       sh, sc);
+    return hipGetLastError();
+}
+
+hipError_t wn_launch_pairs_f64(const double *x, const double *e, const int *count, const int4 *pmeta,
+                               const double2 *pwd, double2 *q, int nk, const WnShape &sh, const WnScreen &sc,
+                               hipStream_t st) {
+    if (sh.n > 64) return hipErrorInvalidValue;
+    hipLaunchKernelGGL(k_pairs_f64, dim3(nk), dim3(256), (size_t)sh.n * sh.n * sizeof(double), st, x, e, count, pmeta,
+                       pwd, q, sh, sc);
     return hipGetLastError();
 }
 
