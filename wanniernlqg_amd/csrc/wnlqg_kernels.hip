@@ -827,7 +827,7 @@ __device__ __forceinline__ int wn_warm_k(int b, int pass, int count, int *anchor
 // VG: V lives in global memory (vscratch, [count][N][N] column-major, so that the threads of one plane
 // update read consecutive rows) because H and V together exceed the LDS for N > 64.
 template <int N, int TPB, bool VG>
-__global__ __launch_bounds__(TPB) void k_eig_f64(const double *__restrict__ x, double *__restrict__ eout,
+__global__ __launch_bounds__(TPB, (N <= 32 && TPB == 256) ? 6 : 1) void k_eig_f64(const double *__restrict__ x, double *__restrict__ eout,
                                                 double2 *__restrict__ uout, int *__restrict__ sweeps_out,
                                                 unsigned long long *__restrict__ sweep_total, int n, int ldo,
                                                 double tol, int pass, int count, double2 *__restrict__ vscratch,
@@ -842,8 +842,6 @@ __global__ __launch_bounds__(TPB) void k_eig_f64(const double *__restrict__ x, d
     __shared__ double4 rot[HALF];   // (c, s, cos phi, sin phi)
     __shared__ int2 pq[HALF];
     __shared__ double red[2 * NW];
-    __shared__ double ev[N];
-    __shared__ int rank[N];
     int anchor = -1;
     const int k = (pass < 0 || pass == 2) ? (int)blockIdx.x : wn_warm_k(blockIdx.x, pass, count, &anchor);
     if (k >= count) return;
@@ -1049,7 +1047,10 @@ __global__ __launch_bounds__(TPB) void k_eig_f64(const double *__restrict__ x, d
     }
     // sort ascending (stable by index); canonical phase per eigenvector column (first largest |v_i|
     // made real and non-negative), written in sorted order
-    __shared__ double2 phase[N];
+    // rot / pq are free after the sweeps and H after its diagonal is read: ev, rank and phase reuse them
+    double *ev = (double *)&rot[0];
+    int *rank = (int *)&pq[0];
+    double2 *phase = H;
     if (t < n) ev[t] = H[P(t, t)].x;
     __syncthreads();
     if (t < n) {
@@ -1531,6 +1532,8 @@ __global__ __launch_bounds__(256) void k_rotate_sc(double *__restrict__ x, const
                 Xi[i * ld + j] = pf[q].y;
             }
         }
+        // the next channel's loads are in flight during this channel's products
+        if (c + 1 < C) prefetch(c + 1);
         __syncthreads();
         if (full) {
             // T = U^+ X in registers, then to LDS
@@ -1570,7 +1573,6 @@ __global__ __launch_bounds__(256) void k_rotate_sc(double *__restrict__ x, const
                     }
                 }
             }
-            if (c + 1 < C) prefetch(c + 1);
             __syncthreads();
             // Y = T U, written to global
             for (int tile = wave; tile < nt * nt; tile += 4) {
@@ -1624,7 +1626,6 @@ __global__ __launch_bounds__(256) void k_rotate_sc(double *__restrict__ x, const
                     Xi[(lq + 4 * q) * ld + c0 + lr] = ti[q];
                 }
             }
-            if (c + 1 < C) prefetch(c + 1);
             __syncthreads();
             // matrix elements: 16 lanes per (pair, direction), q strided over the lanes, butterfly sum
             const int item = t >> 4, sub = t & 15;
