@@ -70,6 +70,15 @@ def jacobi_flops_per_k(n: int, sweeps: float) -> float:
     return sweeps * (n * (n - 1) / 2) * 3 * n * 24.0
 
 
+def rotate_sc_flops(st, n: int) -> float:
+    """Algorithmic flops of the SC/IC rotation over a mesh (8 flops per complex multiply-add): points rotated
+    in full transform 21 channels (16 n^3 each); element points transform dH and A (6 channels) in full and
+    form U_B^+ X (8 |B| n^2) for the 15 dd H and dA channels, |B| the bands of the point's pairs (the final
+    dot products are not counted); points without pairs are skipped."""
+    return (st["rotate_full_points"] * 21 * 16.0 * n ** 3 + st["rotate_element_points"] * 6 * 16.0 * n ** 3
+            + st["rotate_element_bands"] * 15 * 8.0 * n ** 2)
+
+
 def pairs_flops_per_pair(n: int) -> float:
     """Algorithmic flops of one active band pair in the SC/IC pair kernel: six generalized-derivative rows
     (two directions x three components), each an intermediate-state sum over n - 2 bands of, for the three
@@ -118,10 +127,10 @@ def roofline(st, n, nk, precision):
     elif dominant == "rotate":
         launches = nb
         avg_s = 1e-3 * st["rotate_ms"] / launches
-        flops = 16.0 * n ** 3 * 21 * nk / launches
-        roof = {"kernel": "k_rotate_mfma (U^+ X U of 21 channels on FP64 MFMA)", "bound": "mfma",
-                "achieved": flops / avg_s / 1e12, "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s"}
-        tkey = "k_rotate_mfma"
+        flops = rotate_sc_flops(st, n) / launches
+        roof = {"kernel": "k_rotate_sc (U^+ X U of dH and A; dd H and dA reduced to the pairs' elements; FP64 MFMA)",
+                "bound": "mfma", "achieved": flops / avg_s / 1e12, "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s"}
+        tkey = "k_rotate_sc"
     elif dominant == "interp" and st["separable"]:
         launches = 3 * nb
         avg_s = 1e-3 * st["interp_ms"] / launches
@@ -259,7 +268,10 @@ def main():
         "cpu_baseline": cpu,
         "stages_ms": {k: st[k] for k in ("interp_ms", "eig_ms", "warm_ms", "rotate_ms", "pairs_ms", "accum_ms",
                                          "total_ms")},
-        "stage_note": "stages run on two streams and overlap; each figure is the event-timed span of its kernels",
+        "stage_note": "stages run in order on one stream; each figure is the event-timed span of its kernels",
+        "rotation_points": {"elements": st["rotate_element_points"], "full": st["rotate_full_points"],
+                            "mean_bands": st["rotate_element_bands"] / max(1, st["rotate_element_points"])},
+        "active_pairs_per_k": st["active_pairs"] / nk,
     }
     if compare:
         out["compare"] = compare

@@ -28,7 +28,7 @@
 extern "C" {
 #endif
 
-#define WNLQG_ABI_VERSION 3
+#define WNLQG_ABI_VERSION 4
 
 /* Status codes. */
 #define WNLQG_OK 0
@@ -135,6 +135,9 @@ typedef struct wnlqg_stats {
     double warm_ms;         /* warm-start transform U_a^+ H U_a of the mesh eigensolver */
     int32_t eig_launches;   /* eigensolver kernel launches (2 per batch when warm-started) */
     int32_t reserved;
+    int64_t rotate_element_points; /* SC / IC points whose dd H and dA were reduced to the pairs' elements */
+    int64_t rotate_full_points;    /* SC / IC points with every channel rotated (points without pairs: neither) */
+    int64_t rotate_element_bands;  /* sum over the element points of the number of bands in their pairs */
 } wnlqg_stats;
 
 typedef struct wnlqg_engine wnlqg_engine;

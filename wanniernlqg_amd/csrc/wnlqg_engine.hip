@@ -754,22 +754,16 @@ int check_flags(wnlqg_engine *g) {
     return WNLQG_OK;
 }
 
-// MIXED and SPLIT_BF16 rotate channels 1 .. C-1 on FP32 matrix cores; MIXED_HIGH and FP64 stay FP64.
-bool rotate_fp32(const wnlqg_engine *g) {
-    return g->precision == WNLQG_PRECISION_MIXED || g->precision == WNLQG_PRECISION_SPLIT_BF16;
-}
-
 // Hamiltonian gauge of cnt points in (x, e, u).  SC / IC: screening first, then the rotation of what the
 // pairs use, then the pair kernels (the frequency contraction follows in launch_response); other
 // families: every channel.
 hipError_t launch_rotation(wnlqg_engine *g, double *x, const double *e, const double2 *u, int cnt, hipStream_t st) {
-    const bool fp32 = rotate_fp32(g);
-    if (g->family != 0) return wn_launch_rotate_f64(x, u, cnt, g->sh, st, g->rscratch.p, fp32);
+    if (g->family != 0) return wn_launch_rotate_f64(x, u, cnt, g->sh, st, g->rscratch.p);
     hipError_t r = wn_launch_screen(e, g->omega.p, g->count.p, g->pmeta.p, g->pmd.p, g->flags.p, cnt, g->sh, g->sc, st);
     if (r != hipSuccess) return r;
-    r = wn_launch_rotate_sc(x, u, g->count.p, g->pmeta.p, g->pwd.p, cnt, g->sh, st, g->rscratch.p, fp32);
+    r = wn_launch_rotate_sc(x, u, g->count.p, g->pmeta.p, g->pwd.p, cnt, g->sh, st, g->rscratch.p, g->flags.p);
     if (r != hipSuccess) return r;
-    return wn_launch_pairs_f64(x, e, g->count.p, g->pmeta.p, wn_rotate_sc_elements(g->sh, fp32) ? g->pwd.p : nullptr,
+    return wn_launch_pairs_f64(x, e, g->count.p, g->pmeta.p, wn_rotate_sc_elements(g->sh) ? g->pwd.p : nullptr,
                                g->q.p, cnt, g->sh, g->sc, st);
 }
 
@@ -954,7 +948,8 @@ int eig_scheme(const wnlqg_engine *g, long long bi) {
 // holds WN_MARKS pre-created events per batch: 0 start, 1 k points, 2 interpolation, 8 eigensolver
 // pass 0, 9 warm-start transform, 3 eigensolver done (stream 0); 7 start, 4 rotation, 5 response,
 // 6 k sums (stream 1).
-int enqueue_mesh(wnlqg_engine *g, std::vector<std::array<hipEvent_t, WN_MARKS>> *marks, double *interp_bytes) {
+int enqueue_mesh(wnlqg_engine *g, std::vector<std::array<hipEvent_t, WN_MARKS>> *marks, double *interp_bytes,
+                 bool overlap) {
     const MeshPlan &p = g->plan;
     const int nelem = output_elems(g);
     const long long nbatch = (long long)p.batches.size();
@@ -963,7 +958,11 @@ int enqueue_mesh(wnlqg_engine *g, std::vector<std::array<hipEvent_t, WN_MARKS>> 
     WN_HIP(clear_flags(g));
     double2 *comp = g->nterms ? g->lanecomp.p : nullptr;
     const long long m2 = g->sh.ldo / 2;
-    hipStream_t s0 = g->stream, s1 = g->stream2;
+    // overlap: rotation and response of batch b - 1 on the second stream while batch b is interpolated and
+    // diagonalized (the captured batching graph); otherwise one stream, in program order (every stage fills
+    // the GPU by itself: on the 128^3 mesh both orders take the same time, and one stream times each
+    // kernel alone)
+    hipStream_t s0 = g->stream, s1 = overlap ? g->stream2 : g->stream;
     // fork the second stream from the first (this also brings it into a stream capture)
     WN_HIP(hipEventRecord(g->ev_fork, s0));
     WN_HIP(hipStreamWaitEvent(s1, g->ev_fork, 0));
@@ -1032,15 +1031,15 @@ int enqueue_mesh(wnlqg_engine *g, std::vector<std::array<hipEvent_t, WN_MARKS>> 
         // stream 1: rotation, response and k sums of batch bi
         WN_HIP(hipStreamWaitEvent(s1, g->ev_ready[buf], 0));
         if (marks) hipEventRecord((*marks)[bi][7], s1);
-        const bool fp32 = rotate_fp32(g);
         if (g->family == 0) {
             // SC / IC: screening first, so that the rotation skips points without pairs and reduces dd H and
             // dA to the pairs' elements
             WN_HIP(wn_launch_screen(eb, g->omega.p, g->count.p, g->pmeta.p, g->pmd.p, g->flags.p, cnt, g->sh, g->sc,
                                     s1));
-            WN_HIP(wn_launch_rotate_sc(xb, ub, g->count.p, g->pmeta.p, g->pwd.p, cnt, g->sh, s1, g->rscratch.p, fp32));
+            WN_HIP(wn_launch_rotate_sc(xb, ub, g->count.p, g->pmeta.p, g->pwd.p, cnt, g->sh, s1, g->rscratch.p,
+                                       g->flags.p));
         } else {
-            WN_HIP(wn_launch_rotate_f64(xb, ub, cnt, g->sh, s1, g->rscratch.p, fp32));
+            WN_HIP(wn_launch_rotate_f64(xb, ub, cnt, g->sh, s1, g->rscratch.p));
         }
         mark(bi, 4, s1);
         const int nch = (int)(p.bchunk[bi + 1] - p.bchunk[bi]);
@@ -1048,7 +1047,7 @@ int enqueue_mesh(wnlqg_engine *g, std::vector<std::array<hipEvent_t, WN_MARKS>> 
         const int4 *chunks = p.dchunks.p + p.bchunk[bi];
         if (g->family == 0) {
             WN_HIP(wn_launch_pairs_f64(xb, eb, g->count.p, g->pmeta.p,
-                                       wn_rotate_sc_elements(g->sh, fp32) ? g->pwd.p : nullptr, g->q.p, cnt, g->sh,
+                                       wn_rotate_sc_elements(g->sh) ? g->pwd.p : nullptr, g->q.p, cnt, g->sh,
                                        g->sc, s1));
             mark(bi, 5, s1);
             WN_HIP(wn_launch_accum(chunks, nch, g->count.p, g->pmeta.p, g->pmd.p, g->q.p, g->omega.p, g->partial.p,
@@ -1159,7 +1158,7 @@ int32_t wnlqg_run_mesh(wnlqg_engine *g, const int32_t mesh[3], int32_t n_lanes, 
                 if (e) hipEventDestroy(e);
     };
     double interp_bytes = 0.0;
-    rc = enqueue_mesh(g, events_ok ? &marks : nullptr, &interp_bytes);
+    rc = enqueue_mesh(g, events_ok ? &marks : nullptr, &interp_bytes, false);
     if (!rc) rc = finish_mesh(g, lane_out, total_out);
     if (rc) {
         release_events();
@@ -1203,6 +1202,9 @@ int32_t wnlqg_run_mesh(wnlqg_engine *g, const int32_t mesh[3], int32_t n_lanes, 
         swt += v;
         std::memcpy(&v, bk + 4, sizeof(v));
         npairs += v;
+        st.rotate_element_points += bk[6];
+        st.rotate_full_points += bk[7];
+        st.rotate_element_bands += bk[8];
     }
     st.mean_sweeps = nk > 0 ? (double)swt / (double)nk : 0.0;
     st.active_pairs = (int64_t)npairs;
@@ -1379,7 +1381,7 @@ int32_t wnlqg_batch_create(int32_t count, const wnlqg_model *models, const wnlqg
         wnlqg_engine *g = b->engines[i];
         hipStream_t own = g->stream;
         g->stream = b->streams[i % ns];
-        rc = enqueue_mesh(g, nullptr, nullptr);
+        rc = enqueue_mesh(g, nullptr, nullptr, true);
         g->stream = own;
     }
     for (int s = 0; s < ns && he == hipSuccess; ++s) {

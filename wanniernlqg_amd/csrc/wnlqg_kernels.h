@@ -124,7 +124,8 @@ hipError_t wn_launch_zero(double *p, long long n, hipStream_t st);
 // status words: [0] pair-gap flag, [2] geometry flags, [3] first failing k; then WN_CTR_BUCKETS counter
 // buckets of WN_CTR_STRIDE words (128 bytes apart, so that the per-workgroup atomics of a launch spread
 // over many addresses): bucket word [0] max sweeps, [2..3] total sweeps (64-bit), [4..5] total active
-// SC/IC pairs (64-bit); workgroup g adds to bucket g % WN_CTR_BUCKETS.
+// SC/IC pairs (64-bit), [6] SC/IC points rotated to pair elements, [7] SC/IC points rotated in full,
+// [8] bands of the element points; workgroup g adds to bucket g % WN_CTR_BUCKETS.
 #define WN_CTR_BASE 8
 #define WN_CTR_STRIDE 32
 #define WN_CTR_BUCKETS 64
@@ -132,9 +133,8 @@ hipError_t wn_launch_zero(double *p, long long n, hipStream_t st);
 hipError_t wn_launch_reset_flags(int *f, hipStream_t st);
 // U^+ X U for channels 1 .. C-1, in place.
 // scratch: wn_rotate_scratch_elems(n, count) complex elements, required for n > 64
-// fp32: FP32 matrix-core products with FP32 accumulation (mixed forms, 16 <= n <= 64; FP64 otherwise)
 hipError_t wn_launch_rotate_f64(double *x, const double2 *u, int count, const WnShape &sh, hipStream_t st,
-                                double2 *scratch = nullptr, bool fp32 = false);
+                                double2 *scratch = nullptr);
 // Screening, pair lists, generalized derivatives and SC/IC kernels.
 // (k_screen: occupations, windows and the active pair list of every k point, before the rotation.)
 hipError_t wn_launch_screen(const double *e, const double *omega, int *count, int4 *pmeta, double2 *pmd, int *flags,
@@ -143,9 +143,9 @@ hipError_t wn_launch_screen(const double *e, const double *omega, int *count, in
 // full and the pairs' dd H / dA elements in pwd[k][pair][direction][15] (wn_rotate_sc_elements true);
 // otherwise, or when wn_rotate_sc_elements is false, every channel is rotated (wn_launch_rotate_f64).
 #define WN_PCAP 8
-bool wn_rotate_sc_elements(const WnShape &sh, bool fp32);
+bool wn_rotate_sc_elements(const WnShape &sh);
 hipError_t wn_launch_rotate_sc(double *x, const double2 *u, const int *count, const int4 *pmeta, double2 *pwd,
-                               int nk, const WnShape &sh, hipStream_t st, double2 *scratch, bool fp32);
+                               int nk, const WnShape &sh, hipStream_t st, double2 *scratch, int *flags);
 // Generalized derivatives and SC/IC kernels of the screened pairs (pwd: nullptr to read every element from
 // the rotated channels).
 hipError_t wn_launch_pairs_f64(const double *x, const double *e, const int *count, const int4 *pmeta,

@@ -1341,118 +1341,6 @@ __global__ __launch_bounds__(256) void k_rotate_mfma(double *__restrict__ x, con
     }
 }
 
-// Mixed-precision form of the rotation (channels 1 .. C-1): U and X rounded to FP32 in LDS, complex
-// products as four real products on v_mfma_f32_16x16x4f32 (twice the FP64 matrix rate) with FP32
-// accumulation, results written back as FP64.  The eigensystem, the warm-start transform and channel 0
-// stay FP64.  Layouts as in k_rotate_mfma except the accumulator: v_mfma_f32_16x16x4f32 lane l, register q
-// holds D[4 (l / 16) + q][l % 16].
-template <int NP, int PF>
-__global__ __launch_bounds__(256) void k_rotate_mfma32(double *__restrict__ x, const double2 *__restrict__ u, int n,
-                                                      int C, int ldo) {
-    extern __shared__ float smf[];
-    const int k = blockIdx.x;
-    const int t = threadIdx.x;
-    const int lane = t & 63, wave = t >> 6;
-    constexpr int np = NP;
-    constexpr int ld = np + 1;
-    constexpr int nt = np >> 4;
-    float *Ur = smf;
-    float *Ui = Ur + np * ld;
-    float *Xr = Ui + np * ld;
-    float *Xi = Xr + np * ld;
-    const double2 *uk = u + (size_t)k * n * n;
-    for (int e = t; e < np * np; e += 256) {
-        const int i = e / np, j = e - i * np;
-        const double2 v = (i < n && j < n) ? uk[i * n + j] : make_double2(0.0, 0.0);
-        Ur[i * ld + j] = (float)v.x;
-        Ui[i * ld + j] = (float)v.y;
-    }
-    double2 *xk = (double2 *)(x + (size_t)k * ldo);
-    const int lr = lane & 15, lq = lane >> 4;
-    double2 pf[PF];
-    auto prefetch = [&](int c) {
-        const double2 *xc = xk + (size_t)c * n * n;
-#pragma unroll
-        for (int q = 0; q < PF; ++q) {
-            const int e = t + 256 * q;
-            const int i = e / np, j = e - (e / np) * np;
-            pf[q] = (e < np * np && i < n && j < n) ? xc[i * n + j] : make_double2(0.0, 0.0);
-        }
-    };
-    if (C > 1) prefetch(1);
-    for (int c = 1; c < C; ++c) {
-        double2 *xc = xk + (size_t)c * n * n;
-        __syncthreads();
-#pragma unroll
-        for (int q = 0; q < PF; ++q) {
-            const int e = t + 256 * q;
-            if (e < np * np) {
-                const int i = e / np, j = e - (e / np) * np;
-                Xr[i * ld + j] = (float)pf[q].x;
-                Xi[i * ld + j] = (float)pf[q].y;
-            }
-        }
-        __syncthreads();
-        // T = U^+ X:  Tr = Ur^T Xr + Ui^T Xi,  Ti = Ur^T Xi - Ui^T Xr
-        f32x4 tr[4], ti[4];
-#pragma unroll
-        for (int s4 = 0; s4 < 4; ++s4) {
-            const int tile = wave + 4 * s4;
-            f32x4 ar = {0.f, 0.f, 0.f, 0.f}, ai = {0.f, 0.f, 0.f, 0.f};
-            if (tile < nt * nt) {
-                const int r0 = (tile / nt) * 16, c0 = (tile % nt) * 16;
-#pragma unroll
-                for (int k0 = 0; k0 < np; k0 += 4) {
-                    const int kr = k0 + lq;
-                    const float ur = Ur[kr * ld + r0 + lr], ui = Ui[kr * ld + r0 + lr];
-                    const float xr = Xr[kr * ld + c0 + lr], xi = Xi[kr * ld + c0 + lr];
-                    ar = __builtin_amdgcn_mfma_f32_16x16x4f32(ur, xr, ar, 0, 0, 0);
-                    ar = __builtin_amdgcn_mfma_f32_16x16x4f32(ui, xi, ar, 0, 0, 0);
-                    ai = __builtin_amdgcn_mfma_f32_16x16x4f32(ur, xi, ai, 0, 0, 0);
-                    ai = __builtin_amdgcn_mfma_f32_16x16x4f32(-ui, xr, ai, 0, 0, 0);
-                }
-            }
-            tr[s4] = ar;
-            ti[s4] = ai;
-        }
-        __syncthreads();
-#pragma unroll
-        for (int s4 = 0; s4 < 4; ++s4) {
-            const int tile = wave + 4 * s4;
-            if (tile < nt * nt) {
-                const int r0 = (tile / nt) * 16, c0 = (tile % nt) * 16;
-#pragma unroll
-                for (int q = 0; q < 4; ++q) {
-                    Xr[(r0 + 4 * lq + q) * ld + c0 + lr] = tr[s4][q];
-                    Xi[(r0 + 4 * lq + q) * ld + c0 + lr] = ti[s4][q];
-                }
-            }
-        }
-        if (c + 1 < C) prefetch(c + 1);
-        __syncthreads();
-        // Y = T U:  Yr = Tr Ur - Ti Ui,  Yi = Tr Ui + Ti Ur
-        for (int tile = wave; tile < nt * nt; tile += 4) {
-            const int r0 = (tile / nt) * 16, c0 = (tile % nt) * 16;
-            f32x4 yr = {0.f, 0.f, 0.f, 0.f}, yi = {0.f, 0.f, 0.f, 0.f};
-#pragma unroll
-            for (int k0 = 0; k0 < np; k0 += 4) {
-                const int kc = k0 + lq;
-                const float tr_ = Xr[(r0 + lr) * ld + kc], ti_ = Xi[(r0 + lr) * ld + kc];
-                const float ur = Ur[kc * ld + c0 + lr], ui = Ui[kc * ld + c0 + lr];
-                yr = __builtin_amdgcn_mfma_f32_16x16x4f32(tr_, ur, yr, 0, 0, 0);
-                yr = __builtin_amdgcn_mfma_f32_16x16x4f32(-ti_, ui, yr, 0, 0, 0);
-                yi = __builtin_amdgcn_mfma_f32_16x16x4f32(tr_, ui, yi, 0, 0, 0);
-                yi = __builtin_amdgcn_mfma_f32_16x16x4f32(ti_, ur, yi, 0, 0, 0);
-            }
-#pragma unroll
-            for (int q = 0; q < 4; ++q) {
-                const int row = r0 + 4 * lq + q, col = c0 + lr;
-                if (row < n && col < n) xc[row * n + col] = make_double2((double)yr[q], (double)yi[q]);
-            }
-        }
-    }
-}
-
 // Rotation for the shift/injection-current pair kernels (16 <= n <= 64, FP64).  A point without active
 // pairs is skipped (nothing reads its channels).  A point with at most WN_PCAP active pairs rotates the
 // velocity and position channels (dH_a, A_a: the pair kernels sum their rows and columns over all
@@ -1465,7 +1353,8 @@ __global__ __launch_bounds__(256) void k_rotate_mfma32(double *__restrict__ x, c
 template <int NP, int PF>
 __global__ __launch_bounds__(256) void k_rotate_sc(double *__restrict__ x, const double2 *__restrict__ u,
                                                   const int *__restrict__ count, const int4 *__restrict__ pmeta,
-                                                  double2 *__restrict__ pwd, int n, int C, int ldo) {
+                                                  double2 *__restrict__ pwd, int n, int C, int ldo,
+                                                  int *__restrict__ flags) {
     extern __shared__ double smd[];
     __shared__ int bl[16];            // bands of the pairs
     __shared__ int prow[WN_PCAP][2];  // row of n and of m in bl, per pair
@@ -1499,6 +1388,15 @@ __global__ __launch_bounds__(256) void k_rotate_sc(double *__restrict__ x, const
             }
         }
         nbl = nb_;
+    }
+    if (t == 0) {
+        int *bk = flags + WN_CTR_BASE + WN_CTR_STRIDE * (k % WN_CTR_BUCKETS);
+        if (sparse) {
+            atomicAdd(bk + 6, 1);
+            atomicAdd(bk + 8, nbl);
+        } else {
+            atomicAdd(bk + 7, 1);
+        }
     }
     for (int e = t; e < np * np; e += 256) {
         const int i = e / np, j = e - i * np;
@@ -1741,16 +1639,16 @@ static void launch_rotate_mfma(double *x, const double2 *u, int count, const WnS
 #undef WN_ROT
 }
 
-bool wn_rotate_sc_elements(const WnShape &sh, bool fp32) { return !fp32 && sh.n >= 16 && sh.n <= 64; }
+bool wn_rotate_sc_elements(const WnShape &sh) { return sh.n >= 16 && sh.n <= 64; }
 
 hipError_t wn_launch_rotate_sc(double *x, const double2 *u, const int *count, const int4 *pmeta, double2 *pwd,
-                               int nk, const WnShape &sh, hipStream_t st, double2 *scratch, bool fp32) {
-    if (!wn_rotate_sc_elements(sh, fp32)) return wn_launch_rotate_f64(x, u, nk, sh, st, scratch, fp32);
+                               int nk, const WnShape &sh, hipStream_t st, double2 *scratch, int *flags) {
+    if (!wn_rotate_sc_elements(sh)) return wn_launch_rotate_f64(x, u, nk, sh, st, scratch);
     const int np = (sh.n + 15) & ~15;
     const size_t lds = (size_t)4 * np * (np + 1) * sizeof(double);
 #define WN_ROTSC(NPV, PFV)                                                                                      \
     hipLaunchKernelGGL((k_rotate_sc<NPV, PFV>), dim3(nk), dim3(256), lds, st, x, u, count, pmeta, pwd, sh.n, sh.C, \
-                       sh.ldo)
+                       sh.ldo, flags)
     if (np == 16)
         WN_ROTSC(16, 1);
     else if (np == 32)
@@ -1778,25 +1676,9 @@ hipError_t wn_launch_warm_transform_f64(double *x, const double2 *u, int count, 
 }
 
 hipError_t wn_launch_rotate_f64(double *x, const double2 *u, int count, const WnShape &sh, hipStream_t st,
-                                double2 *scratch, bool fp32) {
+                                double2 *scratch) {
     const int n = sh.n;
     if (n > 128) return hipErrorInvalidValue;
-    if (fp32 && n >= 16 && n <= 64) {
-        const int np = (n + 15) & ~15;
-        const size_t lds = (size_t)4 * np * (np + 1) * sizeof(float);
-#define WN_ROT32(NPV, PFV) \
-    hipLaunchKernelGGL((k_rotate_mfma32<NPV, PFV>), dim3(count), dim3(256), lds, st, x, u, n, sh.C, sh.ldo)
-        if (np == 16)
-            WN_ROT32(16, 1);
-        else if (np == 32)
-            WN_ROT32(32, 4);
-        else if (np == 48)
-            WN_ROT32(48, 9);
-        else
-            WN_ROT32(64, 16);
-#undef WN_ROT32
-        return hipGetLastError();
-    }
     if (n > 64) {
         if (!scratch) return hipErrorInvalidValue;
         if (n <= 96)

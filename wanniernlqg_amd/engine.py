@@ -76,7 +76,9 @@ class Stats(C.Structure):
                 ("kpoints", C.c_int64), ("active_pairs", C.c_int64), ("batches", C.c_int32),
                 ("interp_launches", C.c_int32), ("interp_flops", C.c_double), ("max_sweeps", C.c_int32),
                 ("separable", C.c_int32), ("interp_bytes", C.c_double), ("mean_sweeps", C.c_double),
-                ("warm_ms", C.c_double), ("eig_launches", C.c_int32), ("reserved", C.c_int32)]
+                ("warm_ms", C.c_double), ("eig_launches", C.c_int32), ("reserved", C.c_int32),
+                ("rotate_element_points", C.c_int64), ("rotate_full_points", C.c_int64),
+                ("rotate_element_bands", C.c_int64)]
 
     def as_dict(self) -> dict:
         return {name: getattr(self, name) for name, _ in self._fields_}
@@ -85,7 +87,7 @@ class Stats(C.Structure):
 _lib = None
 
 
-ABI_VERSION = 3   # WNLQG_ABI_VERSION of include/wnlqg.h
+ABI_VERSION = 4   # WNLQG_ABI_VERSION of include/wnlqg.h
 
 
 def load_library(path: str = LIB_PATH) -> C.CDLL:
