@@ -168,6 +168,29 @@ def test_mesh_integral(name, mesh):
     assert stats["kpoints"] == int(np.prod(mesh))
 
 
+@pytest.mark.parametrize("omega_max,paths", [(6.0, "elements"), (7.0, "both"), (24.0, "full")])
+def test_element_rotation_paths(omega_max, paths):
+    """The SC/IC rotation reduces dd H and dA to the active pairs' elements for points with at most 8 pairs
+    and rotates every channel otherwise.  On the benchmark model (bands over +-30 eV) photon energies up to
+    6 eV give 2-7 pairs per point (element path only), up to 7 eV a mix, up to 24 eV 70-95 (full path only);
+    each case against the oracle at the FP64 bar, with the device counts of both paths."""
+    model = config_model(1)
+    kw = dict(photon_energies=np.linspace(0, omega_max, 200), broadening=0.01)
+    mesh = (4, 3, 3)
+    say(f"[element-rotation] benchmark model, omega <= {omega_max} eV, mesh {mesh}")
+    with engine(model, **kw) as eng:
+        _, total, stats = eng.run_mesh(mesh)
+    _, ot = kloop.integrate_mesh(model, oracle_task(**kw), mesh, batch=36)
+    for key in ("sc", "ic"):
+        r = kloop.spectrum_relative_error(total[key], ot[key]).max()
+        say(f"  {key} worst component rel {r:.3e}")
+        assert r < 1e-10
+    el, full = stats["rotate_element_points"], stats["rotate_full_points"]
+    say(f"  element points {el}, full points {full}, bands {stats['rotate_element_bands']}")
+    assert el + full <= int(np.prod(mesh))
+    assert (el > 0) == (paths in ("elements", "both")) and (full > 0) == (paths in ("full", "both"))
+
+
 @pytest.mark.parametrize("n,mesh", [(32, (5, 3, 1)), (16, (7, 1, 1)), (20, (5, 1, 2)), (32, (3, 1, 1)),
                                     (24, (9, 2, 1)), (32, (48, 1, 1))])
 def test_warm_started_eigensolver(n, mesh):
