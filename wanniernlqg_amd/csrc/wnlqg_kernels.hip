@@ -881,6 +881,8 @@ __global__ __launch_bounds__(TPB, (N <= 32 && TPB == 256) ? 6 : 1) void k_eig_f6
     constexpr int VW = SPLIT ? TPB - BW0 : TPB;     // threads updating V
     constexpr int BPT = (NOFF + BW - 1) / BW;
     constexpr int VPT = (VITEMS + VW - 1) / VW;     // V items per thread
+    constexpr int VTPS = VW / HALF;                  // V threads per pair slot (slot-major mapping)
+    constexpr bool VSLOT = VW % HALF == 0 && VTPS > 0 && N % VTPS == 0 && N / VTPS == VPT;
     int bt1[BPT], bt2[BPT];
 #pragma unroll
     for (int q = 0; q < BPT; ++q) {
@@ -933,10 +935,15 @@ __global__ __launch_bounds__(TPB, (N <= 32 && TPB == 256) ? 6 : 1) void k_eig_f6
         if (off <= tol * tol * (dia + off)) break;
         if (sweep > 0 && off > 0.25 * prev_off && off <= 1e-20 * (dia + off)) break;
         prev_off = off;
+        // round-robin slot t pairs p = (t - 1 + r) mod (N - 1) + 1 (0 for t = 0) with
+        // q = (N - 2 - t + r) mod (N - 1) + 1: both residues advance by one per round
+        int xp = t >= 1 && t < HALF ? t - 1 : 0, xq = t < HALF ? N - 2 - t : 0;
         for (int r = 0; r < N - 1; ++r) {
             if (t < HALF) {
-                int p = (t == 0) ? 0 : ((t - 1 + r) % (N - 1)) + 1;
-                int q = ((N - 2 - t + r) % (N - 1)) + 1;
+                int p = (t == 0) ? 0 : xp + 1;
+                int q = xq + 1;
+                xp = xp == N - 2 ? 0 : xp + 1;
+                xq = xq == N - 2 ? 0 : xq + 1;
                 if (p > q) {
                     const int tmp = p;
                     p = q;
@@ -1025,20 +1032,35 @@ __global__ __launch_bounds__(TPB, (N <= 32 && TPB == 256) ? 6 : 1) void k_eig_f6
             }
             // V <- V J.  Items (pair slot, row) are fixed per thread; a pair without rotation has
             // (c, s, cos phi, sin phi) = (1, 0, 1, 0), which maps the two entries to themselves exactly, so
-            // every item is applied unconditionally (no branch per item, all items' reads in flight together).
+            // every item is applied unconditionally; padded rows (i >= n) hold zeros in the columns of
+            // rotated pairs (p, q < n) and stay zero.  When the slots divide the V threads, a thread keeps
+            // one slot (one rotation read per round) and takes VPT rows of it.
             if (t >= VT0) {
-#pragma unroll
-                for (int qv = 0; qv < VPT; ++qv) {
-                    const int e = t - VT0 + qv * VW;
-                    if (e >= VITEMS) break;
-                    const int pr = e / N, i = e - (e / N) * N;
-                    if (i >= n) continue;
+                if constexpr (VSLOT) {
+                    const int pr = (t - VT0) / VTPS, i0 = (t - VT0) - pr * VTPS;
                     const double4 R = rot[pr];
                     const int2 ij = pq[pr];
-                    const double2 xp = V(i, ij.x), yq = V(i, ij.y);
-                    const double2 ey = make_double2(R.z * yq.x + R.w * yq.y, R.z * yq.y - R.w * yq.x);
-                    V(i, ij.x) = make_double2(R.x * xp.x - R.y * ey.x, R.x * xp.y - R.y * ey.y);
-                    V(i, ij.y) = make_double2(R.y * xp.x + R.x * ey.x, R.y * xp.y + R.x * ey.y);
+#pragma unroll
+                    for (int qv = 0; qv < VPT; ++qv) {
+                        const int i = i0 + qv * VTPS;
+                        const double2 xp = V(i, ij.x), yq = V(i, ij.y);
+                        const double2 ey = make_double2(R.z * yq.x + R.w * yq.y, R.z * yq.y - R.w * yq.x);
+                        V(i, ij.x) = make_double2(R.x * xp.x - R.y * ey.x, R.x * xp.y - R.y * ey.y);
+                        V(i, ij.y) = make_double2(R.y * xp.x + R.x * ey.x, R.y * xp.y + R.x * ey.y);
+                    }
+                } else {
+#pragma unroll
+                    for (int qv = 0; qv < VPT; ++qv) {
+                        const int e = t - VT0 + qv * VW;
+                        if (e >= VITEMS) break;
+                        const int pr = e / N, i = e - (e / N) * N;
+                        const double4 R = rot[pr];
+                        const int2 ij = pq[pr];
+                        const double2 xp = V(i, ij.x), yq = V(i, ij.y);
+                        const double2 ey = make_double2(R.z * yq.x + R.w * yq.y, R.z * yq.y - R.w * yq.x);
+                        V(i, ij.x) = make_double2(R.x * xp.x - R.y * ey.x, R.x * xp.y - R.y * ey.y);
+                        V(i, ij.y) = make_double2(R.y * xp.x + R.x * ey.x, R.y * xp.y + R.x * ey.y);
+                    }
                 }
             }
             if (VG) __threadfence_block();
