@@ -10,7 +10,7 @@ import pytest
 
 from oracle import kloop
 from wanniernlqg_amd import engine as E
-from wanniernlqg_amd.model import config_model, read_wannier_tb, synthetic_model
+from wanniernlqg_amd.model import TightBindingModel, config_model, read_wannier_tb, synthetic_model
 
 pytestmark = pytest.mark.gpu
 HERE = os.path.dirname(os.path.abspath(__file__))
@@ -139,6 +139,34 @@ def test_point_values_finite_temperature_and_eta():
     exp = kloop.point_values(model, oracle_task(**kw), kp)
     for key in ("sc", "ic"):
         assert rel(got[key], exp[key]) < 1e-10
+
+
+def test_bare_denominators_and_gap_error():
+    """denominator_regularization = 0 selects the bare 1/dE (OrthogonalBandDerivatives.jl:66-82): on a model
+    with isolated levels it matches the oracle; on the direct sum of a model with itself every level is
+    exactly doubly degenerate, and the screening kernel's isolation check returns WNLQG_E_GAP."""
+    model = MODELS["n12_pos"]()
+    kp = np.random.default_rng(9).uniform(0, 1, (17, 3))
+    kw = dict(denominator_regularization=0.0, photon_energies=np.linspace(0.0, 5.0, 80), broadening=0.05)
+    say("[points] eta = 0, n12")
+    with engine(model, **kw) as eng:
+        got = eng.eval_points(kp)
+    exp = kloop.point_values(model, oracle_task(**kw), kp)
+    for key in ("sc", "ic"):
+        r = rel(got[key], exp[key])
+        say(f"  {key} rel {r:.3e}")
+        assert r < 1e-10
+    n = model.num_orbitals
+    h = np.zeros((model.num_r_vectors, 2 * n, 2 * n), dtype=np.complex128)
+    a = np.zeros((model.num_r_vectors, 3, 2 * n, 2 * n), dtype=np.complex128)
+    h[:, :n, :n] = h[:, n:, n:] = model.hamiltonian_r
+    a[:, :, :n, :n] = a[:, :, n:, n:] = model.position_r
+    doubled = TightBindingModel(model.lattice, model.r_vectors, model.r_degeneracies, h, a, name="doubled")
+    say("[points] eta = 0, doubly degenerate n24")
+    with engine(doubled, **kw) as eng:
+        with pytest.raises(E.WnlqgError) as err:
+            eng.eval_points(kp[:4])
+    assert err.value.code == -3
 
 
 @pytest.mark.parametrize("name,mesh", [("cfg0_n8", (8, 8, 8)), ("fixture_n4", (5, 3, 2)), ("n32_pos", (4, 3, 3)),
