@@ -387,6 +387,7 @@ def form_learned(args):
     cells from their spectral features, top cells refined by 5; judged against the uniform 320^3 mesh,
     whose own mesh error is estimated against the uniform 384^3 mesh.  value: the evaluation saving of
     the smallest budget whose error is a tenth of that mesh error or less."""
+    from wanniernlqg_amd import cubature
     from wanniernlqg_amd import engine as E
     from wanniernlqg_amd import refine
     from wanniernlqg_amd.model import config_model
@@ -425,6 +426,19 @@ def form_learned(args):
                          "evaluations_vs_dense": (n * r) ** 3 / le.evaluations, "err_learned": err(le.estimate),
                          "err_base_variance_heuristic": err(he.estimate), "err_random_cells": err(ra.estimate),
                          "uniform_mesh": nu, "err_uniform": err(uni["bcd"]), "adaptive_s": est_s})
+        # the non-learned comparator: h-adaptive Genz-Malik cubature over the zone, GPU-evaluated nodes
+        err384 = lambda v: float(np.linalg.norm(v - finer["bcd"]) / np.linalg.norm(finer["bcd"]))
+        hc_rows = []
+        for budget in (1_000_000, 4_000_000, 16_000_000):
+            t0 = time.perf_counter()
+            res = cubature.integrate_engine(eng, cubature.HAdaptiveCubature(atol=0.0, rtol=1e-3,
+                                                                            max_evaluations=budget),
+                                            boxes_per_step=128)
+            hc_rows.append({"rtol": 1e-3, "max_evaluations": budget, "evaluations": res.evaluations,
+                            "converged": res.estimate_converged,
+                            "estimate_rel": res.embedded_estimate / float(np.max(np.abs(res.value))),
+                            "err_vs_320": err(res.value[0]), "err_vs_384": err384(res.value[0]),
+                            "evaluator_calls": res.steps, "s": time.perf_counter() - t0})
     dense_mesh_err = err(finer["bcd"])
     ok = [row for row in rows if row["err_learned"] <= 0.1 * dense_mesh_err]
     out = {"metric": "k evaluations for dense-mesh accuracy (learned adaptive BCD)",
@@ -436,7 +450,8 @@ def form_learned(args):
            "accuracy_criterion": "err vs the 320^3 result <= 0.1 x the 320^3 result's own deviation from 384^3",
            "dense_mesh_err_vs_384": dense_mesh_err, "train_loss_first_last": [float(hist[:100].mean()),
                                                                              float(hist[-100:].mean())],
-           "train_s": train_s, "dense_s": dense_s, "rank_s": rank_s, "rows": rows}
+           "train_s": train_s, "dense_s": dense_s, "rank_s": rank_s, "rows": rows,
+           "hadaptive_cubature_rows": hc_rows}
     print(json.dumps(out), flush=True)
 
 
