@@ -10,7 +10,7 @@ mesh interpolation); the plain FP64 k x R contraction and the split-FP16 mixed c
 for speed and relative spectrum error; the CPU baseline times the NumPy FP64 oracle on a bounded k sample.
 
   python bench.py [--gpus 1] [--steps K] [--warmup W] [--precision mixed|fp64] [--backend auto|direct|planes]
-  python bench.py --form geometry|shg|batch|learned|large  (the other forms, one JSON line each)
+  python bench.py --form geometry|shg|batch|learned|large|symmetry  (the other forms, one JSON line each)
   python bench.py --form copy                            (HBM copy used to calibrate the PMC byte counters)
 """
 
@@ -173,7 +173,7 @@ def main():
     ap.add_argument("--no-compare", action="store_true", help="skip the plain-FP64 and mixed direct comparison runs")
     ap.add_argument("--cpu-sample", type=int, default=1024)
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--form", default="headline", choices=["headline", "geometry", "shg", "batch", "learned", "large", "copy"])
+    ap.add_argument("--form", default="headline", choices=["headline", "geometry", "shg", "batch", "learned", "large", "symmetry", "copy"])
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}); the k-mesh would shard by "
@@ -455,6 +455,63 @@ def form_learned(args):
     print(json.dumps(out), flush=True)
 
 
+def form_symmetry(args):
+    """Symmetry-reduced mesh: the headline model (n_w = 32, |R_i| <= 5, A amplitude 0.1 Angstrom) averaged over
+    C4v (8 operations, orbitals at the origin), shift + injection current, 400 omega in [0, 6] eV, Gaussian
+    0.01 eV, on the --mesh^3 mesh: the full separable mesh against the orbit representatives (multiplicity
+    weights; whole planes interpolated, representatives gathered: wnlqg_run_mesh_reduced) followed by the
+    tensor projection; the representatives interpolated alone (wnlqg_run_points) reported beside.  value: full-mesh k-points per second of
+    the reduced integral (orbit planning on the host timed apart)."""
+    from wanniernlqg_amd import engine as E
+    from wanniernlqg_amd import symmetry as S
+    from wanniernlqg_amd.model import config_model
+    base = config_model(1)
+    c4 = np.array([[0, -1, 0], [1, 0, 0], [0, 0, 1]])
+    mx = np.diag([-1, 1, 1])
+    ops = S.group_from_generators([(c4, False), (mx, False)], np.asarray(base.lattice))
+    model = S.symmetrize_model(base, ops)
+    mesh = (args.mesh,) * 3
+    nk = int(np.prod(mesh))
+    task = E.EngineTask(photon_energies=np.linspace(0.0, 6.0, 400), broadening=0.01, shift_current=True,
+                        injection_current=True)
+    with E.Engine(model, task) as eng:
+        wall_full, full, _ = _timed_runs(eng, mesh, args.steps, args.warmup)
+        t0 = time.perf_counter()
+        orb = S.kmesh_orbits(mesh, ops)
+        kp = (S.mesh_indices(mesh)[:, orb.representatives] / np.array(mesh, dtype=float)[:, None]).T
+        w = orb.multiplicities / float(orb.full_count)
+        proj = {q: S.response_projector(q, ops) for q in ("sc", "ic")}
+        plan_s = time.perf_counter() - t0
+        for _ in range(args.warmup):
+            eng.run_mesh_reduced(mesh, orb.representatives, w)
+        eng.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(args.steps):
+            _, tot = eng.run_mesh_reduced(mesh, orb.representatives, w)
+            red = {q: S.project(v, proj[q]) for q, v in tot.items()}
+        eng.synchronize()
+        wall_red = (time.perf_counter() - t0) / args.steps
+        # the representatives' points interpolated alone (direct k x R products)
+        t0 = time.perf_counter()
+        _, tot_pts = eng.run_points(kp, w)
+        eng.synchronize()
+        wall_pts = time.perf_counter() - t0
+        err_pts = max(float(np.linalg.norm(S.project(tot_pts[q], proj[q]) - full[q]) / np.linalg.norm(full[q]))
+                      for q in ("sc", "ic"))
+        cov = S.check_covariance(lambda kk: eng.eval_points(kk), ops, samples=2)
+    err = max(float(np.linalg.norm(red[q] - full[q]) / np.linalg.norm(full[q])) for q in ("sc", "ic"))
+    out = {"metric": "k-points/s (symmetry-reduced mesh: C4v, n_w=32, SC + IC, 400 omega)", "value": nk / wall_red,
+           "unit": "full-mesh k-points/s", "higher_is_better": True, "n_gpus": 1, "steps": args.steps,
+           "warmup": args.warmup, "ms_per_step": 1e3 * wall_red, "dtype": "f64", "data": "synthetic C4v model",
+           "config": {"workload": "SC+IC, n_w=32, |R_i|<=5 averaged over C4v, 400 omega in [0,6] eV, Gaussian 0.01 eV",
+                      "mesh": list(mesh)},
+           "representatives": int(orb.representatives.shape[0]), "full_kpoints": nk,
+           "full_mesh_kps": nk / wall_full, "full_mesh_ms": 1e3 * wall_full, "speedup_vs_full_mesh": wall_full / wall_red,
+           "orbit_planning_s": plan_s, "rel_err_vs_full_mesh": err, "covariance_residual": cov,
+           "points_path_ms": 1e3 * wall_pts, "points_path_rel_err": err_pts}
+    print(json.dumps(out), flush=True)
+
+
 def form_large(args):
     """n_w = 96 SHG (|R_i| <= 6, 2197 R vectors, A amplitude 0.1 Angstrom; xyy, yxz, zzz; 400 omega in [0, 3] eV,
     Gaussian 0.02 eV) on a --mesh^3 sub-mesh (default 16^3) of the 256^3 configuration.  The engine keeps
@@ -506,7 +563,7 @@ def form_copy(args):
 
 
 FORMS = {"geometry": form_geometry, "shg": form_shg, "batch": form_batch, "learned": form_learned,
-         "large": form_large, "copy": form_copy}
+         "large": form_large, "symmetry": form_symmetry, "copy": form_copy}
 
 
 if __name__ == "__main__":

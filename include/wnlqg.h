@@ -28,7 +28,7 @@
 extern "C" {
 #endif
 
-#define WNLQG_ABI_VERSION 4
+#define WNLQG_ABI_VERSION 5
 
 /* Status codes. */
 #define WNLQG_OK 0
@@ -189,6 +189,35 @@ int32_t wnlqg_num_channels(const wnlqg_engine *engine);
  */
 int32_t wnlqg_run_mesh(wnlqg_engine *engine, const int32_t mesh[3], int32_t n_lanes,
                        double *lane_out, double *total_out, wnlqg_stats *stats);
+
+/*
+ * Weighted point list for SC / IC engines: the k-loop over the evaluation items of a symmetry-reduced
+ * mesh (src/Runtime/Execution/IntegralDriver.jl:822-831: kpoint_index = representatives[i],
+ * kpoint_weight = multiplicities[i], scale = kpoint_weight / num_kpoints).  kpoints: column-major
+ * [3,m] fractional; weights[i]: the scale of item i (multiplicity / N_k for a reduced mesh).  Lanes
+ * partition the m items as wnlqg_run_mesh partitions the mesh: lane L covers items
+ * floor(L*m/n_lanes) .. floor((L+1)*m/n_lanes)-1, 1 <= n_lanes <= m.  lane_out: complex
+ * [n_lanes][n_states][n_w*27], each block sum_{i in lane} weights[i] * value_i with the state's
+ * prefactor; total_out (optional): the lanes summed in lane order.  The tensor projection of
+ * apply_response_symmetry! (ResponseSymmetryRuntime.jl:1121-1146) is the caller's.  Other engine
+ * families return WNLQG_E_INVALID.
+ */
+int32_t wnlqg_run_points(wnlqg_engine *engine, const double *kpoints, const double *weights, int64_t m,
+                         int32_t n_lanes, double *lane_out, double *total_out);
+
+/*
+ * Symmetry-reduced dense mesh for SC / IC engines with the separable interpolation (fourier_backend
+ * auto or planes): the same integral as wnlqg_run_points over the mesh points reps[0..m-1] (strictly
+ * increasing linear indices of the Gamma-centred mesh, axis 1 fastest: the orbit representatives of
+ * response_symmetry_kmesh_orbits, ResponseSymmetryRuntime.jl:463-496, 0-based) with weights[i]
+ * (multiplicity / N_k), but each batch of k3 planes is interpolated whole by the plane contraction and
+ * only the representatives' rows go on to the eigensolver and the response kernels.  Lanes, lane_out
+ * and total_out as in wnlqg_run_points; the tensor projection is the caller's.  WNLQG_E_INVALID for
+ * other families or a mesh the separable interpolation does not plan.
+ */
+int32_t wnlqg_run_mesh_reduced(wnlqg_engine *engine, const int32_t mesh[3], const int64_t *reps,
+                               const double *weights, int64_t m, int32_t n_lanes, double *lane_out,
+                               double *total_out);
 
 /*
  * Per-point values (point_value!, IntegralDriver.jl:596, with num_kpoints = 1) at m arbitrary

@@ -22,7 +22,7 @@ def header_functions():
 
 def test_library_loads_and_version():
     lib = E.load_library()
-    assert lib.wnlqg_version() == 4
+    assert lib.wnlqg_version() == 5
 
 
 def test_exports_match_header():

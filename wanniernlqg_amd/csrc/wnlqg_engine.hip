@@ -121,9 +121,9 @@ struct wnlqg_engine {
     int batch = 0;      // k rows per batch (multiple of 128)
     int kpc = 16;       // k points per accumulation chunk
 
-    DevBuf<double> hr, ar, rc, bf, kp, x, e, omega;
-    DevBuf<int> deg, sweeps, count, flags;
-    DevBuf<int4> ridx, pmeta, chunks, lanes;
+    DevBuf<double> hr, ar, rc, bf, kp, x, e, omega, wts;
+    DevBuf<int> deg, sweeps, count, flags, ridxs;  // ridxs: batch-local rows of a reduced mesh's representatives
+    DevBuf<int4> ridx, pmeta, chunks, lanes, rchunks, rlanes;
     DevBuf<double2> tab, u, pmd, q, pwd, partial, lanebuf, lanecomp;
     DevBuf<float2> tab32;
     DevBuf<__bf16> bh, bm, bl;  // split-BF16 coefficient parts (alternative mixed kernel)
@@ -432,6 +432,7 @@ int32_t wnlqg_destroy(wnlqg_engine *g) {
     g->hr.release(); g->ar.release(); g->rc.release(); g->bf.release(); g->kp.release(); g->x.release();
     g->e.release(); g->omega.release(); g->deg.release(); g->sweeps.release(); g->count.release();
     g->flags.release(); g->ridx.release(); g->pmeta.release(); g->chunks.release(); g->lanes.release();
+    g->ridxs.release(); g->rchunks.release(); g->rlanes.release(); g->wts.release();
     g->tab.release(); g->u.release(); g->pmd.release(); g->q.release(); g->partial.release();
     g->lanebuf.release(); g->lanecomp.release(); g->tab32.release(); g->bh.release(); g->bm.release();
     g->bl.release(); g->fh.release(); g->fm.release(); g->cup.release(); g->cdown.release(); g->t12.release();
@@ -943,6 +944,28 @@ int eig_scheme(const wnlqg_engine *g, long long bi) {
     return 1;
 }
 
+// Separable plane contraction of one batch into xb: plane k3 + j of the batch, k2 range [k2b, k2b + k2c),
+// lands at rows j k2c N1 + (k2 - k2b) N1 + k1.
+hipError_t launch_contract_batch(wnlqg_engine *g, const MeshBatch &bt, double *xb, hipStream_t s0) {
+    const MeshPlan &p = g->plan;
+    const long long m2 = g->sh.ldo / 2;
+    const double2 *boxc = (const double2 *)g->box.p;
+    hipError_t e = hipSuccess;
+    for (int j = 0; j < bt.nk3 && e == hipSuccess; ++j) {
+        double2 *xj = (double2 *)xb + (size_t)j * bt.k2c * p.mesh[0] * m2;
+        if (bt.k2b == 0)
+            e = wn_launch_contract(boxc, g->y3.p, g->e3.p, 1, bt.k3 + j, g->span[2], (int)m2, g->span[0] * g->span[1],
+                                   g->span[2] * m2, m2, m2, 0, s0);
+        if (e == hipSuccess)
+            e = wn_launch_contract(g->y3.p, g->y23.p, g->e2.p, bt.k2c, bt.k2b, g->span[1], (int)m2, g->span[0],
+                                   g->span[1] * m2, m2, (long long)bt.k2c * m2, m2, s0);
+        if (e == hipSuccess)
+            e = wn_launch_contract(g->y23.p, xj, g->e1.p, p.mesh[0], 0, g->span[0], (int)m2, bt.k2c, m2,
+                                   (long long)bt.k2c * m2, (long long)p.mesh[0] * m2, m2, s0);
+    }
+    return e;
+}
+
 // The kernel sequence of one planned mesh integral, ordered on g->stream (forked to g->stream2 and
 // joined back).  Capture-safe: no allocation, no host synchronization, no memset.  marks (optional)
 // holds WN_MARKS pre-created events per batch: 0 start, 1 k points, 2 interpolation, 8 eigensolver
@@ -987,18 +1010,7 @@ int enqueue_mesh(wnlqg_engine *g, std::vector<std::array<hipEvent_t, WN_MARKS>> 
         }
         mark(bi, 1, s0);
         if (p.sep) {
-            const double2 *boxc = (const double2 *)g->box.p;
-            for (int j = 0; j < bt.nk3; ++j) {
-                // plane k3 + j of the batch: its k2 range [k2b, k2b + k2c) starts at row j k2c N1
-                double2 *xj = (double2 *)xb + (size_t)j * bt.k2c * p.mesh[0] * m2;
-                if (bt.k2b == 0)
-                    WN_HIP(wn_launch_contract(boxc, g->y3.p, g->e3.p, 1, bt.k3 + j, g->span[2], (int)m2,
-                                              g->span[0] * g->span[1], g->span[2] * m2, m2, m2, 0, s0));
-                WN_HIP(wn_launch_contract(g->y3.p, g->y23.p, g->e2.p, bt.k2c, bt.k2b, g->span[1], (int)m2, g->span[0],
-                                          g->span[1] * m2, m2, (long long)bt.k2c * m2, m2, s0));
-                WN_HIP(wn_launch_contract(g->y23.p, xj, g->e1.p, p.mesh[0], 0, g->span[0], (int)m2, bt.k2c, m2,
-                                          (long long)bt.k2c * m2, (long long)p.mesh[0] * m2, m2, s0));
-            }
+            WN_HIP(launch_contract_batch(g, bt, xb, s0));
             if (interp_bytes) *interp_bytes += (double)cnt * g->sh.ldo * 8.0;
         } else {
             WN_HIP(launch_interp(g, rows, xb));
@@ -1072,12 +1084,10 @@ int enqueue_mesh(wnlqg_engine *g, std::vector<std::array<hipEvent_t, WN_MARKS>> 
     return WNLQG_OK;
 }
 
-// Lane sums to the caller after the enqueued integral: flags, prefactors, totals.
-int finish_mesh(wnlqg_engine *g, double *lane_out, double *total_out) {
-    const MeshPlan &p = g->plan;
+// Lane sums to the caller after the enqueued integral: flags, prefactors, totals.  nk: the divisor of a
+// mesh integral (N_k); 1 for a weighted point list, whose weights carry the normalization.
+int finish_lanes(wnlqg_engine *g, int n_lanes, long long nk, double *lane_out, double *total_out) {
     const int nelem = output_elems(g);
-    const int n_lanes = p.n_lanes;
-    const long long nk = p.nk;
     std::vector<double> lanes((size_t)n_lanes * nelem * 2);
     WN_HIP(hipMemcpyAsync(lanes.data(), g->lanebuf.p, lanes.size() * sizeof(double), hipMemcpyDeviceToHost,
                           g->stream));
@@ -1159,7 +1169,7 @@ int32_t wnlqg_run_mesh(wnlqg_engine *g, const int32_t mesh[3], int32_t n_lanes, 
     };
     double interp_bytes = 0.0;
     rc = enqueue_mesh(g, events_ok ? &marks : nullptr, &interp_bytes, false);
-    if (!rc) rc = finish_mesh(g, lane_out, total_out);
+    if (!rc) rc = finish_lanes(g, g->plan.n_lanes, g->plan.nk, lane_out, total_out);
     if (rc) {
         release_events();
         return rc;
@@ -1216,6 +1226,189 @@ int32_t wnlqg_run_mesh(wnlqg_engine *g, const int32_t mesh[3], int32_t n_lanes, 
     st.separable = g->plan.sep ? 1 : 0;
     if (stats) *stats = st;
     return WNLQG_OK;
+}
+
+// Weighted point list (SC / IC): the k-loop of a symmetry-reduced mesh, whose evaluation items are the
+// orbit representatives with weight multiplicity / N_k (consume_point!, IntegralDriver.jl:822-831).  Per
+// batch: interpolation and eigensystem of the listed points, screening, weights folded into the pairs'
+// occupation differences, rotation and pair kernels, frequency contraction per lane-aligned chunk, lane
+// sums in chunk order.
+int32_t wnlqg_run_points(wnlqg_engine *g, const double *kpoints, const double *weights, int64_t m, int32_t n_lanes,
+                         double *lane_out, double *total_out) {
+    if (!g || !kpoints || !weights || !lane_out || m < 1) return fail(WNLQG_E_INVALID, "invalid arguments");
+    if (n_lanes < 1 || n_lanes > m) return fail(WNLQG_E_INVALID, "n_lanes must be in [1, m]");
+    if (g->family != 0)
+        return fail(WNLQG_E_INVALID, "weighted point lists integrate shift / injection current engines only");
+    if (g->kpc < 1 || g->kpc > WN_ACC_KMAX) return fail(WNLQG_E_INVALID, "chunk size %d outside [1, %d]", g->kpc, WN_ACC_KMAX);
+    for (int64_t i = 0; i < m; ++i)
+        if (!std::isfinite(weights[i])) return fail(WNLQG_E_NONFINITE, "non-finite weight %lld", (long long)i);
+    WN_HIP(hipSetDevice(g->device));
+    const int nelem = output_elems(g);
+    if (g->lanebuf.n < (size_t)n_lanes * nelem) {
+        WN_HIP(g->lanebuf.alloc((size_t)n_lanes * nelem));
+        WN_HIP(g->lanecomp.alloc((size_t)n_lanes * nelem));
+    }
+    if (g->wts.n < (size_t)g->batch) WN_HIP(g->wts.alloc((size_t)g->batch));
+    WN_HIP(wn_launch_zero((double *)g->lanebuf.p, (long long)n_lanes * nelem * 2, g->stream));
+    if (g->nterms) WN_HIP(wn_launch_zero((double *)g->lanecomp.p, (long long)n_lanes * nelem * 2, g->stream));
+    WN_HIP(clear_flags(g));
+    double2 *comp = g->nterms ? g->lanecomp.p : nullptr;
+    std::vector<long long> lstart(n_lanes + 1);
+    for (int l = 0; l <= n_lanes; ++l) lstart[l] = (long long)((__int128)l * m / n_lanes);
+    std::vector<int4> ch, ln;
+    for (long long p0 = 0; p0 < m; p0 += g->batch) {
+        const int cnt = (int)std::min<long long>(g->batch, m - p0);
+        const int rows = (cnt + WN_GM - 1) / WN_GM * WN_GM;
+        // chunks of at most kpc points inside each lane part of this batch, in point order
+        ch.clear();
+        ln.clear();
+        int l = (int)(std::upper_bound(lstart.begin(), lstart.end(), p0) - lstart.begin()) - 1;
+        for (; l < n_lanes && lstart[l] < p0 + cnt; ++l) {
+            const long long a = std::max(p0, lstart[l]), b = std::min(p0 + cnt, lstart[l + 1]);
+            if (a >= b) continue;
+            const int cb = (int)ch.size();
+            for (long long c = a; c < b; c += g->kpc)
+                ch.push_back(make_int4((int)(c - p0), (int)std::min<long long>(g->kpc, b - c), l, 0));
+            ln.push_back(make_int4(l, cb, (int)ch.size(), 0));
+        }
+        // upload_points drains the stream first, so the previous batch no longer reads kp, wts, chunks, lanes
+        int rc = upload_points(g, kpoints + 3 * (size_t)p0, cnt, rows);
+        if (rc) return rc;
+        rc = ensure_partial(g, (int)ch.size());
+        if (rc) return rc;
+        if (g->lanes.n < ln.size()) WN_HIP(g->lanes.alloc(std::max<size_t>(ln.size(), (size_t)n_lanes)));
+        WN_HIP(hipMemcpy(g->wts.p, weights + p0, (size_t)cnt * sizeof(double), hipMemcpyHostToDevice));
+        WN_HIP(hipMemcpy(g->chunks.p, ch.data(), ch.size() * sizeof(int4), hipMemcpyHostToDevice));
+        WN_HIP(hipMemcpy(g->lanes.p, ln.data(), ln.size() * sizeof(int4), hipMemcpyHostToDevice));
+        rc = run_front(g, cnt, rows, true);
+        if (rc) return rc;
+        WN_HIP(wn_launch_screen(g->e.p, g->omega.p, g->count.p, g->pmeta.p, g->pmd.p, g->flags.p, cnt, g->sh, g->sc,
+                                g->stream));
+        WN_HIP(wn_launch_weight_pairs(g->pmd.p, g->count.p, g->wts.p, cnt, g->sh, g->stream));
+        WN_HIP(wn_launch_rotate_sc(g->x.p, g->u.p, g->count.p, g->pmeta.p, g->pwd.p, cnt, g->sh, g->stream,
+                                   g->rscratch.p, g->flags.p));
+        WN_HIP(wn_launch_pairs_f64(g->x.p, g->e.p, g->count.p, g->pmeta.p,
+                                   wn_rotate_sc_elements(g->sh) ? g->pwd.p : nullptr, g->q.p, cnt, g->sh, g->sc,
+                                   g->stream));
+        WN_HIP(wn_launch_accum(g->chunks.p, (int)ch.size(), g->count.p, g->pmeta.p, g->pmd.p, g->q.p, g->omega.p,
+                               g->partial.p, g->sh, g->sc, g->stream));
+        WN_HIP(wn_launch_lane_reduce(g->lanes.p, (int)ln.size(), g->partial.p, g->lanebuf.p, comp, nelem, g->stream));
+    }
+    return finish_lanes(g, n_lanes, 1, lane_out, total_out);
+}
+
+// Symmetry-reduced mesh (SC / IC, separable interpolation): every batch of k3 planes is interpolated
+// whole by the plane contraction, the rows of the batch's orbit representatives are gathered into the
+// other half of the double buffer, and only they are diagonalized (warm-started in groups of 4 consecutive
+// representatives where the size allows, as the mesh rows are), screened,
+// weighted, rotated and contracted.  reps: sorted linear mesh indices (axis 1 fastest); weights: the
+// items' scales (multiplicity / N_k); lanes partition the m items as in wnlqg_run_points.
+int32_t wnlqg_run_mesh_reduced(wnlqg_engine *g, const int32_t mesh[3], const int64_t *reps, const double *weights,
+                               int64_t m, int32_t n_lanes, double *lane_out, double *total_out) {
+    if (!g || !mesh || !reps || !weights || !lane_out || m < 1) return fail(WNLQG_E_INVALID, "invalid arguments");
+    if (mesh[0] < 1 || mesh[1] < 1 || mesh[2] < 1) return fail(WNLQG_E_INVALID, "mesh sizes must be positive");
+    if (n_lanes < 1 || n_lanes > m) return fail(WNLQG_E_INVALID, "n_lanes must be in [1, m]");
+    if (g->family != 0)
+        return fail(WNLQG_E_INVALID, "reduced meshes integrate shift / injection current engines only");
+    const long long nk = (long long)mesh[0] * mesh[1] * mesh[2];
+    if (m > nk) return fail(WNLQG_E_INVALID, "more representatives than mesh points");
+    for (int64_t i = 0; i < m; ++i) {
+        if (reps[i] < 0 || reps[i] >= nk || (i > 0 && reps[i] <= reps[i - 1]))
+            return fail(WNLQG_E_INVALID, "representatives must be strictly increasing mesh indices");
+        if (!std::isfinite(weights[i])) return fail(WNLQG_E_NONFINITE, "non-finite weight %lld", (long long)i);
+    }
+    WN_HIP(hipSetDevice(g->device));
+    int rc = plan_mesh(g, mesh, 1);
+    if (rc) return rc;
+    const MeshPlan &p = g->plan;
+    if (!p.sep)
+        return fail(WNLQG_E_INVALID, "reduced meshes need the separable interpolation (fourier_backend auto or planes)");
+    const long long nbatch = (long long)p.batches.size();
+    // representatives of each batch: a contiguous range of the sorted list
+    std::vector<long long> rb(nbatch + 1, m);
+    for (long long bi = 0; bi < nbatch; ++bi)
+        rb[bi] = std::lower_bound(reps, reps + m, (int64_t)p.batches[bi].k0) - reps;
+    std::vector<int> local(m);
+    for (long long bi = 0; bi < nbatch; ++bi)
+        for (long long i = rb[bi]; i < rb[bi + 1]; ++i) local[i] = (int)(reps[i] - p.batches[bi].k0);
+    std::vector<long long> lstart(n_lanes + 1);
+    for (int l = 0; l <= n_lanes; ++l) lstart[l] = (long long)((__int128)l * m / n_lanes);
+    std::vector<int4> ch, ln;
+    std::vector<long long> bch(nbatch + 1, 0), bln(nbatch + 1, 0);
+    int max_chunks = 0;
+    for (long long bi = 0; bi < nbatch; ++bi) {
+        bch[bi] = (long long)ch.size();
+        bln[bi] = (long long)ln.size();
+        const long long r0 = rb[bi], r1 = rb[bi + 1];
+        int l = (int)(std::upper_bound(lstart.begin(), lstart.end(), r0) - lstart.begin()) - 1;
+        for (; r0 < r1 && l < n_lanes && lstart[l] < r1; ++l) {
+            const long long a = std::max(r0, lstart[l]), b = std::min(r1, lstart[l + 1]);
+            if (a >= b) continue;
+            const int cb = (int)(ch.size() - bch[bi]);
+            for (long long c = a; c < b; c += g->kpc)
+                ch.push_back(make_int4((int)(c - r0), (int)std::min<long long>(g->kpc, b - c), l, 0));
+            ln.push_back(make_int4(l, cb, (int)(ch.size() - bch[bi]), 0));
+        }
+        max_chunks = std::max(max_chunks, (int)(ch.size() - bch[bi]));
+    }
+    bch[nbatch] = (long long)ch.size();
+    bln[nbatch] = (long long)ln.size();
+    rc = ensure_partial(g, std::max(1, max_chunks));
+    if (rc) return rc;
+    const int nelem = output_elems(g);
+    if (g->lanebuf.n < (size_t)n_lanes * nelem) {
+        WN_HIP(g->lanebuf.alloc((size_t)n_lanes * nelem));
+        WN_HIP(g->lanecomp.alloc((size_t)n_lanes * nelem));
+    }
+    if (g->wts.n < (size_t)m) WN_HIP(g->wts.alloc((size_t)m));
+    if (g->ridxs.n < (size_t)m) WN_HIP(g->ridxs.alloc((size_t)m));
+    if (g->rchunks.n < ch.size()) WN_HIP(g->rchunks.alloc(ch.size()));
+    if (g->rlanes.n < ln.size()) WN_HIP(g->rlanes.alloc(ln.size()));
+    WN_HIP(hipStreamSynchronize(g->stream));
+    WN_HIP(hipMemcpy(g->wts.p, weights, (size_t)m * sizeof(double), hipMemcpyHostToDevice));
+    WN_HIP(hipMemcpy(g->ridxs.p, local.data(), (size_t)m * sizeof(int), hipMemcpyHostToDevice));
+    WN_HIP(hipMemcpy(g->rchunks.p, ch.data(), ch.size() * sizeof(int4), hipMemcpyHostToDevice));
+    WN_HIP(hipMemcpy(g->rlanes.p, ln.data(), ln.size() * sizeof(int4), hipMemcpyHostToDevice));
+    hipStream_t s0 = g->stream;
+    WN_HIP(wn_launch_zero((double *)g->lanebuf.p, (long long)n_lanes * nelem * 2, s0));
+    if (g->nterms) WN_HIP(wn_launch_zero((double *)g->lanecomp.p, (long long)n_lanes * nelem * 2, s0));
+    WN_HIP(clear_flags(g));
+    double2 *comp = g->nterms ? g->lanecomp.p : nullptr;
+    int *sweep_max = g->flags.p + WN_CTR_BASE;
+    unsigned long long *sweep_total = (unsigned long long *)(g->flags.p + WN_CTR_BASE + 2);
+    // whole batches in buffer 0, the gathered representatives in buffer 1
+    double *xb = g->x.p;
+    double *xo = g->x.p + (size_t)g->batch * g->sh.ldo;
+    double *eo = g->e.p + (size_t)g->batch * g->sh.n;
+    double2 *uo = g->u.p + (size_t)g->batch * g->sh.n * g->sh.n;
+    for (long long bi = 0; bi < nbatch; ++bi) {
+        const int cr = (int)(rb[bi + 1] - rb[bi]);
+        if (cr == 0) continue;
+        WN_HIP(launch_contract_batch(g, p.batches[bi], xb, s0));
+        WN_HIP(wn_launch_gather_rows(xb, xo, g->ridxs.p + rb[bi], cr, g->sh.ldo, s0));
+        if (wn_warm_start(g->sh.n, cr)) {
+            // groups of 4 consecutive representatives (mostly neighbours along axis 1)
+            WN_HIP(wn_launch_eig_pass_f64(xo, eo, uo, sweep_max, sweep_total, cr, g->sh, g->eig_tpb, 0, s0, nullptr,
+                                          nullptr, WN_CTR_STRIDE));
+            WN_HIP(wn_launch_warm_transform_f64(xo, uo, cr, g->sh, s0));
+            WN_HIP(wn_launch_eig_pass_f64(xo, eo, uo, sweep_max, sweep_total, cr, g->sh, g->eig_tpb, 1, s0, nullptr,
+                                          nullptr, WN_CTR_STRIDE));
+        } else {
+            WN_HIP(wn_launch_eig_pass_f64(xo, eo, uo, sweep_max, sweep_total, cr, g->sh, g->eig_tpb, -1, s0,
+                                          g->escratch.p, nullptr, WN_CTR_STRIDE));
+        }
+        WN_HIP(wn_launch_screen(eo, g->omega.p, g->count.p, g->pmeta.p, g->pmd.p, g->flags.p, cr, g->sh, g->sc, s0));
+        WN_HIP(wn_launch_weight_pairs(g->pmd.p, g->count.p, g->wts.p + rb[bi], cr, g->sh, s0));
+        WN_HIP(wn_launch_rotate_sc(xo, uo, g->count.p, g->pmeta.p, g->pwd.p, cr, g->sh, s0, g->rscratch.p,
+                                   g->flags.p));
+        WN_HIP(wn_launch_pairs_f64(xo, eo, g->count.p, g->pmeta.p, wn_rotate_sc_elements(g->sh) ? g->pwd.p : nullptr,
+                                   g->q.p, cr, g->sh, g->sc, s0));
+        WN_HIP(wn_launch_accum(g->rchunks.p + bch[bi], (int)(bch[bi + 1] - bch[bi]), g->count.p, g->pmeta.p, g->pmd.p,
+                               g->q.p, g->omega.p, g->partial.p, g->sh, g->sc, s0));
+        WN_HIP(wn_launch_lane_reduce(g->rlanes.p + bln[bi], (int)(bln[bi + 1] - bln[bi]), g->partial.p, g->lanebuf.p,
+                                     comp, nelem, s0));
+    }
+    return finish_lanes(g, n_lanes, 1, lane_out, total_out);
 }
 
 // ------------------------------------------------------------------------------------------------
@@ -1410,7 +1603,8 @@ int32_t wnlqg_batch_run(wnlqg_batch *b, double *const *lane_out, double *const *
     WN_HIP(hipEventElapsedTime(&ms, b->t0, b->t1));
     if (elapsed_ms) *elapsed_ms = ms;
     for (size_t i = 0; i < b->engines.size(); ++i) {
-        int rc = finish_mesh(b->engines[i], lane_out[i], total_out ? total_out[i] : nullptr);
+        int rc = finish_lanes(b->engines[i], b->engines[i]->plan.n_lanes, b->engines[i]->plan.nk, lane_out[i],
+                              total_out ? total_out[i] : nullptr);
         if (rc) return fail(rc, "entry %zu: %s", i, g_last_error.c_str());
     }
     return WNLQG_OK;

@@ -151,6 +151,13 @@ hipError_t wn_launch_rotate_sc(double *x, const double2 *u, const int *count, co
 hipError_t wn_launch_pairs_f64(const double *x, const double *e, const int *count, const int4 *pmeta,
                                const double2 *pwd, double2 *q, int nk, const WnShape &sh, const WnScreen &sc,
                                hipStream_t st);
+// Gather rows of ldo doubles (ldo even): dst row i = src row idx[i], i < count.
+hipError_t wn_launch_gather_rows(const double *src, double *dst, const int *idx, int count, long long ldo,
+                                 hipStream_t st);
+// Per-point weights of a point-list integral: the occupation difference of every active pair of point k
+// (pmd[k n^2 + j].y, j < count[k]) times w[k], so that the frequency contraction sums w_k value_k.
+hipError_t wn_launch_weight_pairs(double2 *pmd, const int *count, const double *w, int nk, const WnShape &sh,
+                                  hipStream_t st);
 // Frequency contraction of the pairs of each chunk (chunks of at most WN_ACC_KMAX k points).
 #define WN_ACC_KMAX 64
 hipError_t wn_launch_accum(const int4 *chunks, int nchunks, const int *count, const int4 *pmeta, const double2 *pmd,

@@ -2055,6 +2055,42 @@ hipError_t wn_launch_pairs_f64(const double *x, const double *e, const int *coun
 }
 
 // ------------------------------------------------------------------------------------------------
+// Row gather: dst row i = src row idx[i] (rows of m2 complex values; one workgroup per row).
+__global__ __launch_bounds__(256) void k_gather_rows(const double2 *__restrict__ src, double2 *__restrict__ dst,
+                                                     const int *__restrict__ idx, long long m2) {
+    const size_t i = blockIdx.x;
+    const double2 *s = src + (size_t)idx[i] * m2;
+    double2 *d = dst + i * m2;
+    for (long long c = threadIdx.x; c < m2; c += 256) d[c] = s[c];
+}
+
+hipError_t wn_launch_gather_rows(const double *src, double *dst, const int *idx, int count, long long ldo,
+                                 hipStream_t st) {
+    if (count <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_gather_rows, dim3(count), dim3(256), 0, st, (const double2 *)src, (double2 *)dst, idx,
+                       ldo / 2);
+    return hipGetLastError();
+}
+
+// ------------------------------------------------------------------------------------------------
+// Point weights folded into the pairs' occupation differences (one workgroup of 64 per point).
+__global__ __launch_bounds__(64) void k_weight_pairs(double2 *__restrict__ pmd, const int *__restrict__ count,
+                                                     const double *__restrict__ w, int pcap) {
+    const int k = blockIdx.x;
+    const int c = count[k];
+    const double wk = w[k];
+    double2 *row = pmd + (size_t)k * pcap;
+    for (int j = threadIdx.x; j < c; j += 64) row[j].y *= wk;
+}
+
+hipError_t wn_launch_weight_pairs(double2 *pmd, const int *count, const double *w, int nk, const WnShape &sh,
+                                  hipStream_t st) {
+    if (nk <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_weight_pairs, dim3(nk), dim3(64), 0, st, pmd, count, w, sh.n * sh.n);
+    return hipGetLastError();
+}
+
+// ------------------------------------------------------------------------------------------------
 // Frequency contraction per chunk of k points.  Workgroup (chunk, 64-frequency tile): wave w
 // owns components [14w, 14w+14), lane = frequency.  The chunk's pairs are staged in LDS in blocks of
 // 256 in (k, pair) order, those whose frequency window meets the tile are compacted (order kept), and

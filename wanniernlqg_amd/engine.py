@@ -36,7 +36,9 @@ ERRORS = {-1: "WNLQG_E_INVALID", -2: "WNLQG_E_NONFINITE", -3: "WNLQG_E_GAP", -4:
           -5: "WNLQG_E_OOM", -6: "WNLQG_E_UNSUPPORTED"}
 
 EXPORTED = ["wnlqg_version", "wnlqg_last_error", "wnlqg_create", "wnlqg_destroy", "wnlqg_num_states",
-            "wnlqg_num_channels", "wnlqg_run_mesh", "wnlqg_eval_points", "wnlqg_eval_points_spectrum",
+            "wnlqg_num_channels", "wnlqg_run_mesh", "wnlqg_run_points", "wnlqg_run_mesh_reduced",
+            "wnlqg_eval_points",
+            "wnlqg_eval_points_spectrum",
             "wnlqg_fourier_points",
             "wnlqg_spectrum_points", "wnlqg_synchronize", "wnlqg_mlp_create", "wnlqg_mlp_destroy",
             "wnlqg_mlp_last_error", "wnlqg_mlp_get_params", "wnlqg_mlp_set_params", "wnlqg_mlp_predict",
@@ -87,7 +89,7 @@ class Stats(C.Structure):
 _lib = None
 
 
-ABI_VERSION = 4   # WNLQG_ABI_VERSION of include/wnlqg.h
+ABI_VERSION = 5   # WNLQG_ABI_VERSION of include/wnlqg.h
 
 
 def load_library(path: str = LIB_PATH) -> C.CDLL:
@@ -108,6 +110,9 @@ def load_library(path: str = LIB_PATH) -> C.CDLL:
     lib.wnlqg_num_states.argtypes = [C.c_void_p]
     lib.wnlqg_num_channels.argtypes = [C.c_void_p]
     lib.wnlqg_run_mesh.argtypes = [C.c_void_p, C.POINTER(C.c_int32), C.c_int32, dp, dp, C.POINTER(Stats)]
+    lib.wnlqg_run_points.argtypes = [C.c_void_p, dp, dp, C.c_int64, C.c_int32, dp, dp]
+    lib.wnlqg_run_mesh_reduced.argtypes = [C.c_void_p, C.POINTER(C.c_int32), C.POINTER(C.c_int64), dp, C.c_int64,
+                                           C.c_int32, dp, dp]
     lib.wnlqg_eval_points.argtypes = [C.c_void_p, dp, C.c_int32, dp]
     lib.wnlqg_eval_points_spectrum.argtypes = [C.c_void_p, dp, C.c_int32, dp, dp]
     lib.wnlqg_fourier_points.argtypes = [C.c_void_p, dp, C.c_int32, dp]
@@ -293,6 +298,40 @@ class Engine:
                                                  _ptr(total_out), C.byref(st)))
         lanes, total = self._unpack_mesh(lane_out, total_out, nl)
         return lanes, total, st.as_dict()
+
+    def run_points(self, kpoints: np.ndarray, weights: np.ndarray, n_lanes: Optional[int] = None):
+        """Weighted point list (SC / IC engines): returns (lanes, total) like run_mesh, each lane the sum
+        of weights[i] * value(k_i) over its items (the k-loop of a symmetry-reduced mesh)."""
+        k = np.ascontiguousarray(kpoints, dtype=np.float64).reshape(-1, 3)
+        w = np.ascontiguousarray(weights, dtype=np.float64).reshape(-1)
+        if w.shape[0] != k.shape[0]:
+            raise ValueError("one weight per k point required")
+        m = k.shape[0]
+        nl = min(m, 64) if n_lanes is None else int(n_lanes)
+        per = self._result_elems()
+        lane_out = np.zeros(nl * per * 2)
+        total_out = np.zeros(per * 2)
+        _check(self.lib, self.lib.wnlqg_run_points(self.handle, _ptr(k), _ptr(w), m, nl, _ptr(lane_out),
+                                                   _ptr(total_out)))
+        return self._unpack_mesh(lane_out, total_out, nl)
+
+    def run_mesh_reduced(self, mesh: Sequence[int], reps: np.ndarray, weights: np.ndarray,
+                         n_lanes: Optional[int] = None):
+        """The mesh points reps (sorted 0-based linear indices) with weights, interpolated by whole planes
+        (SC / IC engines, separable backend): returns (lanes, total) like run_points."""
+        mesh_a = np.ascontiguousarray(mesh, dtype=np.int32)
+        r = np.ascontiguousarray(reps, dtype=np.int64).reshape(-1)
+        w = np.ascontiguousarray(weights, dtype=np.float64).reshape(-1)
+        if w.shape[0] != r.shape[0]:
+            raise ValueError("one weight per representative required")
+        m = r.shape[0]
+        nl = min(m, 64) if n_lanes is None else int(n_lanes)
+        per = self._result_elems()
+        lane_out = np.zeros(nl * per * 2)
+        total_out = np.zeros(per * 2)
+        _check(self.lib, self.lib.wnlqg_run_mesh_reduced(self.handle, _ptr(mesh_a, C.c_int32), _ptr(r, C.c_int64), _ptr(w),
+                                                         m, nl, _ptr(lane_out), _ptr(total_out)))
+        return self._unpack_mesh(lane_out, total_out, nl)
 
     def _result_elems(self) -> int:
         if self.shg:
