@@ -459,7 +459,7 @@ def form_symmetry(args):
     """Symmetry-reduced mesh: the headline model (n_w = 32, |R_i| <= 5, A amplitude 0.1 Angstrom) averaged over
     C4v (8 operations, orbitals at the origin), shift + injection current, 400 omega in [0, 6] eV, Gaussian
     0.01 eV, on the --mesh^3 mesh: the full separable mesh against the orbit representatives (multiplicity
-    weights; whole planes interpolated, representatives gathered: wnlqg_run_mesh_reduced) followed by the
+    weights; planes contracted, the last stage only for the representatives: wnlqg_run_mesh_reduced) followed by the
     tensor projection; the representatives interpolated alone (wnlqg_run_points) reported beside.  value: full-mesh k-points per second of
     the reduced integral (orbit planning on the host timed apart)."""
     from wanniernlqg_amd import engine as E

@@ -210,8 +210,8 @@ int32_t wnlqg_run_points(wnlqg_engine *engine, const double *kpoints, const doub
  * auto or planes): the same integral as wnlqg_run_points over the mesh points reps[0..m-1] (strictly
  * increasing linear indices of the Gamma-centred mesh, axis 1 fastest: the orbit representatives of
  * response_symmetry_kmesh_orbits, ResponseSymmetryRuntime.jl:463-496, 0-based) with weights[i]
- * (multiplicity / N_k), but each batch of k3 planes is interpolated whole by the plane contraction and
- * only the representatives' rows go on to the eigensolver and the response kernels.  Lanes, lane_out
+ * (multiplicity / N_k), but interpolated by the plane contraction: its first two stages run for every
+ * plane, its last stage and everything after it only for the representatives.  Lanes, lane_out
  * and total_out as in wnlqg_run_points; the tensor projection is the caller's.  WNLQG_E_INVALID for
  * other families or a mesh the separable interpolation does not plan.
  */

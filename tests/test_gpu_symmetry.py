@@ -55,14 +55,18 @@ def test_run_points_against_oracle():
         assert rel(total[key], np.einsum("k,k...->...", w, vals[key])) < 1e-10
 
 
-@pytest.mark.parametrize("group,n,rmax,mesh", [("C4v", 8, 2, (8, 8, 6)), ("C4v", 16, 1, (6, 6, 4)),
-                                               ("C2v+T", 10, 1, (6, 4, 4))])
-def test_reduced_mesh_equals_full_mesh(group, n, rmax, mesh):
+@pytest.mark.parametrize("group,n,rmax,mesh,batch", [("C4v", 8, 2, (8, 8, 6), 0), ("C4v", 16, 1, (6, 6, 4), 0),
+                                                     ("C2v+T", 10, 1, (6, 4, 4), 0), ("C4v", 8, 1, (16, 16, 3), 128),
+                                                     ("C4v", 16, 1, (16, 16, 4), 256)])
+def test_reduced_mesh_equals_full_mesh(group, n, rmax, mesh, batch):
+    """batch 0: whole planes per batch (several planes in one); batch 128: a 16 x 16 plane split over
+    two batches (the second reuses the first's first contraction stage); batch 256 with n = 16: one plane
+    per batch, eigensolver started from the previous plane's representatives."""
     base = synthetic_model(n, rmax, 9, position_scale=0.1)
     ops = S.group_from_generators(GENERATORS[group], np.asarray(base.lattice))
     model = S.symmetrize_model(base, ops)
-    say(f"[symmetry] {group} n={n} mesh={mesh}")
-    with E.Engine(model, E.EngineTask(**task_kw())) as eng:
+    say(f"[symmetry] {group} n={n} mesh={mesh} batch={batch}")
+    with E.Engine(model, E.EngineTask(batch_size=batch, **task_kw())) as eng:
         cov = S.check_covariance(lambda kk: eng.eval_points(kk), ops, samples=3)
         _, full, _ = eng.run_mesh(mesh)
         lp, red, orb = S.integrate_symmetric(eng, mesh, ops, n_lanes=5, path="planes")

@@ -122,8 +122,8 @@ struct wnlqg_engine {
     int kpc = 16;       // k points per accumulation chunk
 
     DevBuf<double> hr, ar, rc, bf, kp, x, e, omega, wts;
-    DevBuf<int> deg, sweeps, count, flags, ridxs;  // ridxs: batch-local rows of a reduced mesh's representatives
-    DevBuf<int4> ridx, pmeta, chunks, lanes, rchunks, rlanes;
+    DevBuf<int> deg, sweeps, count, flags, ridxs;  // ridxs: k1 of a reduced mesh's representatives
+    DevBuf<int4> ridx, pmeta, chunks, lanes, rchunks, rlanes, rgroups;
     DevBuf<double2> tab, u, pmd, q, pwd, partial, lanebuf, lanecomp;
     DevBuf<float2> tab32;
     DevBuf<__bf16> bh, bm, bl;  // split-BF16 coefficient parts (alternative mixed kernel)
@@ -432,7 +432,7 @@ int32_t wnlqg_destroy(wnlqg_engine *g) {
     g->hr.release(); g->ar.release(); g->rc.release(); g->bf.release(); g->kp.release(); g->x.release();
     g->e.release(); g->omega.release(); g->deg.release(); g->sweeps.release(); g->count.release();
     g->flags.release(); g->ridx.release(); g->pmeta.release(); g->chunks.release(); g->lanes.release();
-    g->ridxs.release(); g->rchunks.release(); g->rlanes.release(); g->wts.release();
+    g->ridxs.release(); g->rchunks.release(); g->rlanes.release(); g->rgroups.release(); g->wts.release();
     g->tab.release(); g->u.release(); g->pmd.release(); g->q.release(); g->partial.release();
     g->lanebuf.release(); g->lanecomp.release(); g->tab32.release(); g->bh.release(); g->bm.release();
     g->bl.release(); g->fh.release(); g->fm.release(); g->cup.release(); g->cdown.release(); g->t12.release();
@@ -1297,9 +1297,9 @@ int32_t wnlqg_run_points(wnlqg_engine *g, const double *kpoints, const double *w
     return finish_lanes(g, n_lanes, 1, lane_out, total_out);
 }
 
-// Symmetry-reduced mesh (SC / IC, separable interpolation): every batch of k3 planes is interpolated
-// whole by the plane contraction, the rows of the batch's orbit representatives are gathered into the
-// other half of the double buffer, and only they are diagonalized (warm-started in groups of 4 consecutive
+// Symmetry-reduced mesh (SC / IC, separable interpolation): the first two contraction stages of every
+// k3 plane run whole (they are shared by all points of the plane), the last stage writes only the orbit
+// representatives' rows, compactly, and only they are diagonalized (warm-started in groups of 4 consecutive
 // representatives where the size allows, as the mesh rows are), screened,
 // weighted, rotated and contracted.  reps: sorted linear mesh indices (axis 1 fastest); weights: the
 // items' scales (multiplicity / N_k); lanes partition the m items as in wnlqg_run_points.
@@ -1328,9 +1328,34 @@ int32_t wnlqg_run_mesh_reduced(wnlqg_engine *g, const int32_t mesh[3], const int
     std::vector<long long> rb(nbatch + 1, m);
     for (long long bi = 0; bi < nbatch; ++bi)
         rb[bi] = std::lower_bound(reps, reps + m, (int64_t)p.batches[bi].k0) - reps;
-    std::vector<int> local(m);
-    for (long long bi = 0; bi < nbatch; ++bi)
-        for (long long i = rb[bi]; i < rb[bi + 1]; ++i) local[i] = (int)(reps[i] - p.batches[bi].k0);
+    // per representative its k1 (the phase row of the last contraction stage); per (batch, plane) the runs
+    // of representatives sharing a mesh row: (row b within the plane's k2 range, first, end) with indices
+    // relative to the batch's first representative
+    std::vector<int> sel(m);
+    std::vector<int4> groups;
+    std::vector<long long> gplane;   // groups of plane j of batch bi: [gplane[gp0[bi] + j], gplane[gp0[bi] + j + 1])
+    std::vector<long long> gp0(nbatch + 1, 0);
+    for (long long bi = 0; bi < nbatch; ++bi) {
+        const MeshBatch &bt = p.batches[bi];
+        const long long rowsz = (long long)bt.k2c * mesh[0];
+        gp0[bi] = (long long)gplane.size();
+        long long i = rb[bi];
+        for (int j = 0; j < bt.nk3; ++j) {
+            gplane.push_back((long long)groups.size());
+            while (i < rb[bi + 1] && (reps[i] - bt.k0) / rowsz == j) {
+                const long long r = (reps[i] - bt.k0) % rowsz;
+                const int b = (int)(r / mesh[0]);
+                const long long t0 = i;
+                while (i < rb[bi + 1] && (reps[i] - bt.k0) / rowsz == j && ((reps[i] - bt.k0) % rowsz) / mesh[0] == b) {
+                    sel[i] = (int)((reps[i] - bt.k0) % mesh[0]);
+                    ++i;
+                }
+                groups.push_back(make_int4(b, (int)(t0 - rb[bi]), (int)(i - rb[bi]), 0));
+            }
+        }
+        gplane.push_back((long long)groups.size());
+    }
+    gp0[nbatch] = (long long)gplane.size();
     std::vector<long long> lstart(n_lanes + 1);
     for (int l = 0; l <= n_lanes; ++l) lstart[l] = (long long)((__int128)l * m / n_lanes);
     std::vector<int4> ch, ln;
@@ -1362,11 +1387,13 @@ int32_t wnlqg_run_mesh_reduced(wnlqg_engine *g, const int32_t mesh[3], const int
     }
     if (g->wts.n < (size_t)m) WN_HIP(g->wts.alloc((size_t)m));
     if (g->ridxs.n < (size_t)m) WN_HIP(g->ridxs.alloc((size_t)m));
+    if (g->rgroups.n < groups.size()) WN_HIP(g->rgroups.alloc(groups.size()));
     if (g->rchunks.n < ch.size()) WN_HIP(g->rchunks.alloc(ch.size()));
     if (g->rlanes.n < ln.size()) WN_HIP(g->rlanes.alloc(ln.size()));
     WN_HIP(hipStreamSynchronize(g->stream));
     WN_HIP(hipMemcpy(g->wts.p, weights, (size_t)m * sizeof(double), hipMemcpyHostToDevice));
-    WN_HIP(hipMemcpy(g->ridxs.p, local.data(), (size_t)m * sizeof(int), hipMemcpyHostToDevice));
+    WN_HIP(hipMemcpy(g->ridxs.p, sel.data(), (size_t)m * sizeof(int), hipMemcpyHostToDevice));
+    WN_HIP(hipMemcpy(g->rgroups.p, groups.data(), groups.size() * sizeof(int4), hipMemcpyHostToDevice));
     WN_HIP(hipMemcpy(g->rchunks.p, ch.data(), ch.size() * sizeof(int4), hipMemcpyHostToDevice));
     WN_HIP(hipMemcpy(g->rlanes.p, ln.data(), ln.size() * sizeof(int4), hipMemcpyHostToDevice));
     hipStream_t s0 = g->stream;
@@ -1376,17 +1403,48 @@ int32_t wnlqg_run_mesh_reduced(wnlqg_engine *g, const int32_t mesh[3], const int
     double2 *comp = g->nterms ? g->lanecomp.p : nullptr;
     int *sweep_max = g->flags.p + WN_CTR_BASE;
     unsigned long long *sweep_total = (unsigned long long *)(g->flags.p + WN_CTR_BASE + 2);
-    // whole batches in buffer 0, the gathered representatives in buffer 1
-    double *xb = g->x.p;
-    double *xo = g->x.p + (size_t)g->batch * g->sh.ldo;
-    double *eo = g->e.p + (size_t)g->batch * g->sh.n;
-    double2 *uo = g->u.p + (size_t)g->batch * g->sh.n * g->sh.n;
+    // the representatives' rows, compact, alternately in the two halves of the double buffer, so that a
+    // batch whose representatives sit at the same places of the previous batch's plane (point groups that
+    // keep k3) starts its eigensolver from the previous plane's eigenvectors, as the full mesh does
+    const long long m2 = g->sh.ldo / 2;
+    const double2 *boxc = (const double2 *)g->box.p;
+    long long prev = -1;
+    int buf = 0;
+    auto same_pattern = [&](long long a, long long b) {
+        const MeshBatch &ba = p.batches[a], &bb = p.batches[b];
+        if (ba.nk3 != 1 || bb.nk3 != 1 || ba.k2c != mesh[1] || bb.k2c != mesh[1]) return false;
+        if (rb[a + 1] - rb[a] != rb[b + 1] - rb[b]) return false;
+        for (long long i = 0; i < rb[a + 1] - rb[a]; ++i)
+            if (reps[rb[a] + i] - ba.k0 != reps[rb[b] + i] - bb.k0) return false;
+        return true;
+    };
     for (long long bi = 0; bi < nbatch; ++bi) {
         const int cr = (int)(rb[bi + 1] - rb[bi]);
+        const MeshBatch &bt = p.batches[bi];
+        double *xo = g->x.p + (size_t)buf * g->batch * g->sh.ldo;
+        double *eo = g->e.p + (size_t)buf * g->batch * g->sh.n;
+        double2 *uo = g->u.p + (size_t)buf * g->batch * g->sh.n * g->sh.n;
+        for (int j = 0; j < bt.nk3; ++j) {
+            // the first stage whenever a plane starts (batches holding the rest of the plane reuse it), the
+            // second for the batch's k2 range, the last only for the representatives' rows
+            if (bt.k2b == 0)
+                WN_HIP(wn_launch_contract(boxc, g->y3.p, g->e3.p, 1, bt.k3 + j, g->span[2], (int)m2,
+                                          g->span[0] * g->span[1], g->span[2] * m2, m2, m2, 0, s0));
+            const long long ga = gplane[gp0[bi] + j], gb = gplane[gp0[bi] + j + 1];
+            if (ga == gb) continue;
+            WN_HIP(wn_launch_contract(g->y3.p, g->y23.p, g->e2.p, bt.k2c, bt.k2b, g->span[1], (int)m2, g->span[0],
+                                      g->span[1] * m2, m2, (long long)bt.k2c * m2, m2, s0));
+            WN_HIP(wn_launch_contract_sel(g->y23.p, (double2 *)xo, g->e1.p, g->rgroups.p + ga, (int)(gb - ga),
+                                          g->ridxs.p + rb[bi], g->span[0], (int)m2, m2, (long long)bt.k2c * m2, m2,
+                                          s0));
+        }
         if (cr == 0) continue;
-        WN_HIP(launch_contract_batch(g, p.batches[bi], xb, s0));
-        WN_HIP(wn_launch_gather_rows(xb, xo, g->ridxs.p + rb[bi], cr, g->sh.ldo, s0));
-        if (wn_warm_start(g->sh.n, cr)) {
+        if (wn_warm_start(g->sh.n, cr) && prev >= 0 && prev == bi - 1 && same_pattern(prev, bi)) {
+            const double2 *uprev = g->u.p + (size_t)(buf ^ 1) * g->batch * g->sh.n * g->sh.n;
+            WN_HIP(wn_launch_warm_transform_f64(xo, uo, cr, g->sh, s0, uprev));
+            WN_HIP(wn_launch_eig_pass_f64(xo, eo, uo, sweep_max, sweep_total, cr, g->sh, g->eig_tpb, 2, s0, nullptr,
+                                          uprev, WN_CTR_STRIDE));
+        } else if (wn_warm_start(g->sh.n, cr)) {
             // groups of 4 consecutive representatives (mostly neighbours along axis 1)
             WN_HIP(wn_launch_eig_pass_f64(xo, eo, uo, sweep_max, sweep_total, cr, g->sh, g->eig_tpb, 0, s0, nullptr,
                                           nullptr, WN_CTR_STRIDE));
@@ -1407,6 +1465,8 @@ int32_t wnlqg_run_mesh_reduced(wnlqg_engine *g, const int32_t mesh[3], const int
                                g->q.p, g->omega.p, g->partial.p, g->sh, g->sc, s0));
         WN_HIP(wn_launch_lane_reduce(g->rlanes.p + bln[bi], (int)(bln[bi + 1] - bln[bi]), g->partial.p, g->lanebuf.p,
                                      comp, nelem, s0));
+        prev = bi;
+        buf ^= 1;
     }
     return finish_lanes(g, n_lanes, 1, lane_out, total_out);
 }

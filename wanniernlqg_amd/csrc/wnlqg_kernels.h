@@ -95,6 +95,11 @@ hipError_t wn_launch_build_box(const double *hr, const double *ar, const double 
 hipError_t wn_launch_axis_phases(double2 *E, int N, int span, int rmin, hipStream_t st);
 hipError_t wn_launch_contract(const double2 *in, double2 *out, const double2 *E, int P, int p0, int Q, int M, int nb,
                               long long ib, long long iq, long long ob, long long op, hipStream_t st);
+// Selected outputs of a contraction: group g = (b, t0, t1, -) writes out[t*ot + m] for t0 <= t < t1 from input
+// row b and phase row sel[t] (one mesh row's chosen points, written compactly).
+hipError_t wn_launch_contract_sel(const double2 *in, double2 *out, const double2 *E, const int4 *groups, int ngroups,
+                                  const int *sel, int Q, int M, long long ib, long long iq, long long ot,
+                                  hipStream_t st);
 // Jacobi eigensolver of the upper triangle of channel 0.
 // tpb: threads per k point (64, 128, 256; 0 = default of the size class).
 // vscratch: [count][N][N] complex (N = 96 or 128), required for n > 64 (the Jacobi's V in global memory).
@@ -151,9 +156,6 @@ hipError_t wn_launch_rotate_sc(double *x, const double2 *u, const int *count, co
 hipError_t wn_launch_pairs_f64(const double *x, const double *e, const int *count, const int4 *pmeta,
                                const double2 *pwd, double2 *q, int nk, const WnShape &sh, const WnScreen &sc,
                                hipStream_t st);
-// Gather rows of ldo doubles (ldo even): dst row i = src row idx[i], i < count.
-hipError_t wn_launch_gather_rows(const double *src, double *dst, const int *idx, int count, long long ldo,
-                                 hipStream_t st);
 // Per-point weights of a point-list integral: the occupation difference of every active pair of point k
 // (pmd[k n^2 + j].y, j < count[k]) times w[k], so that the frequency contraction sums w_k value_k.
 hipError_t wn_launch_weight_pairs(double2 *pmd, const int *count, const double *w, int nk, const WnShape &sh,

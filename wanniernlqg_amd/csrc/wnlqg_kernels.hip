@@ -783,6 +783,40 @@ hipError_t wn_launch_contract(const double2 *in, double2 *out, const double2 *E,
     return hipGetLastError();
 }
 
+// Selected outputs of the last contraction stage: group g = (b, t0, t1) writes out row t, t0 <= t < t1, as
+// sum_{q<Q} E[sel[t] * Q + q] * in[b*ib + q*iq + m].  Each workgroup loads its Q input values per m once and
+// produces the row's selected outputs (the representatives of one mesh row, written compactly).
+__global__ __launch_bounds__(256) void k_contract_sel(const double2 *__restrict__ in, double2 *__restrict__ out,
+                                                     const double2 *__restrict__ E, const int4 *__restrict__ groups,
+                                                     const int *__restrict__ sel, int Q, int M, long long ib,
+                                                     long long iq, long long ot) {
+    const int m = blockIdx.x * blockDim.x + threadIdx.x;
+    if (m >= M) return;
+    const int4 gr = groups[blockIdx.y];
+    const double2 *src = in + (long long)gr.x * ib + m;
+    double2 v[WN_QMAX];
+#pragma unroll
+    for (int q = 0; q < WN_QMAX; ++q) v[q] = q < Q ? src[q * iq] : make_double2(0.0, 0.0);
+    for (int t = gr.y; t < gr.z; ++t) {
+        const double2 *ep = E + (size_t)sel[t] * Q;
+        double2 acc = make_double2(0.0, 0.0);
+#pragma unroll
+        for (int q = 0; q < WN_QMAX; ++q)
+            if (q < Q) acc = cfma(ep[q], v[q], acc);
+        out[(long long)t * ot + m] = acc;
+    }
+}
+
+hipError_t wn_launch_contract_sel(const double2 *in, double2 *out, const double2 *E, const int4 *groups, int ngroups,
+                                  const int *sel, int Q, int M, long long ib, long long iq, long long ot,
+                                  hipStream_t st) {
+    if (Q > WN_QMAX) return hipErrorInvalidValue;
+    if (ngroups <= 0) return hipSuccess;
+    dim3 grid((M + 255) / 256, ngroups);
+    hipLaunchKernelGGL(k_contract_sel, grid, dim3(256), 0, st, in, out, E, groups, sel, Q, M, ib, iq, ot);
+    return hipGetLastError();
+}
+
 // ------------------------------------------------------------------------------------------------
 // Jacobi eigensolver: one workgroup (64, 128 or 256 threads) per k point, H and V in LDS (n <= 64).
 //
@@ -2051,24 +2085,6 @@ hipError_t wn_launch_pairs_f64(const double *x, const double *e, const int *coun
     if (sh.n > 64) return hipErrorInvalidValue;
     hipLaunchKernelGGL(k_pairs_f64, dim3(nk), dim3(256), (size_t)sh.n * sh.n * sizeof(double), st, x, e, count, pmeta,
                        pwd, q, sh, sc);
-    return hipGetLastError();
-}
-
-// ------------------------------------------------------------------------------------------------
-// Row gather: dst row i = src row idx[i] (rows of m2 complex values; one workgroup per row).
-__global__ __launch_bounds__(256) void k_gather_rows(const double2 *__restrict__ src, double2 *__restrict__ dst,
-                                                     const int *__restrict__ idx, long long m2) {
-    const size_t i = blockIdx.x;
-    const double2 *s = src + (size_t)idx[i] * m2;
-    double2 *d = dst + i * m2;
-    for (long long c = threadIdx.x; c < m2; c += 256) d[c] = s[c];
-}
-
-hipError_t wn_launch_gather_rows(const double *src, double *dst, const int *idx, int count, long long ldo,
-                                 hipStream_t st) {
-    if (count <= 0) return hipSuccess;
-    hipLaunchKernelGGL(k_gather_rows, dim3(count), dim3(256), 0, st, (const double2 *)src, (double2 *)dst, idx,
-                       ldo / 2);
     return hipGetLastError();
 }
 

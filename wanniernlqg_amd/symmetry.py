@@ -210,7 +210,8 @@ def integrate_symmetric(engine, mesh: Sequence[int], ops: Sequence[SymmetryOpera
     """Integrate the mesh over its orbit representatives only.  Returns (lanes, total, orbits): lanes over
     the representatives (unprojected, weights multiplicity / N_k), and totals projected onto the allowed
     tensors, equal to the full-mesh totals for a covariant integrand.  path "planes": whole k3 planes
-    interpolated by the separable contraction, representatives gathered (``Engine.run_mesh_reduced``);
+    contracted by the separable interpolation, the last stage only for the representatives
+    (``Engine.run_mesh_reduced``);
     "points": the representatives' k points interpolated alone (``Engine.run_points``); "auto": planes
     unless the engine's Fourier backend is "direct"."""
     validate_group(ops)
