@@ -917,6 +917,12 @@ __global__ __launch_bounds__(TPB, (N <= 32 && TPB == 256) ? 6 : 1) void k_eig_f6
     constexpr int VPT = (VITEMS + VW - 1) / VW;     // V items per thread
     constexpr int VTPS = VW / HALF;                  // V threads per pair slot (slot-major mapping)
     constexpr bool VSLOT = VW % HALF == 0 && VTPS > 0 && N % VTPS == 0 && N / VTPS == VPT;
+    // Bank-conflict-free V mapping: 16 consecutive lanes share one pair slot and take rows i0, i0 + 16, ...
+    // (i0 = lane mod 16), so a 16-byte LDS access of the 16 lanes touches the 16 distinct 16-byte bank
+    // groups (i0 (N + 1) + p) mod 16 (N + 1 odd); the VG lane groups take slots g, g + VG, ...
+    constexpr int VG16 = VW / 16;
+    constexpr bool VQUAD = !VG && VW % 16 == 0 && N % 16 == 0 && VG16 > 0 && HALF % VG16 == 0 &&
+                           (HALF / VG16) * (N / 16) == VPT && VT0 % 64 == 0;
     int bt1[BPT], bt2[BPT];
 #pragma unroll
     for (int q = 0; q < BPT; ++q) {
@@ -1070,7 +1076,23 @@ __global__ __launch_bounds__(TPB, (N <= 32 && TPB == 256) ? 6 : 1) void k_eig_f6
             // rotated pairs (p, q < n) and stay zero.  When the slots divide the V threads, a thread keeps
             // one slot (one rotation read per round) and takes VPT rows of it.
             if (t >= VT0) {
-                if constexpr (VSLOT) {
+                if constexpr (VQUAD) {
+                    const int l = t - VT0, i0 = l & 15, g = l >> 4;
+#pragma unroll
+                    for (int sp = 0; sp < HALF / VG16; ++sp) {
+                        const int pr = g + sp * VG16;
+                        const double4 R = rot[pr];
+                        const int2 ij = pq[pr];
+#pragma unroll
+                        for (int rq = 0; rq < N / 16; ++rq) {
+                            const int i = i0 + rq * 16;
+                            const double2 xp = V(i, ij.x), yq = V(i, ij.y);
+                            const double2 ey = make_double2(R.z * yq.x + R.w * yq.y, R.z * yq.y - R.w * yq.x);
+                            V(i, ij.x) = make_double2(R.x * xp.x - R.y * ey.x, R.x * xp.y - R.y * ey.y);
+                            V(i, ij.y) = make_double2(R.y * xp.x + R.x * ey.x, R.y * xp.y + R.x * ey.y);
+                        }
+                    }
+                } else if constexpr (VSLOT) {
                     const int pr = (t - VT0) / VTPS, i0 = (t - VT0) - pr * VTPS;
                     const double4 R = rot[pr];
                     const int2 ij = pq[pr];
