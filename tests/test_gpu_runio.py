@@ -48,6 +48,7 @@ method = "Conventional"
 [response.tasks.shg]
 kind = "SHG"
 method = "Conventional"
+output = "both"
 [response.tasks.shg.numerics]
 broadening = 0.03
 intermediate_regularization = 0.02
@@ -112,3 +113,7 @@ def test_run_file_to_result_files(tmp_path):
             want = st[:, c, :, q].sum(axis=1)
             got = f[:, 1] + 1j * f[:, 2]
             assert rel(got, want) < 1e-10, (name, label, rel(got, want))
+            # output = "both": the seven terms follow the total, each against the oracle
+            terms = f[:, 3::2] + 1j * f[:, 4::2]
+            assert terms.shape[1] == 7
+            assert rel(terms, st[:, c, :, q]) < 1e-10

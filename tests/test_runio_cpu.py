@@ -222,6 +222,10 @@ def test_load_run_config(tmp_path):
     assert [(t.task_id, t.quantity) for t in cfg.tasks] == [
         ("ic", "injection_current"), ("sc", "shift_current"), ("shg", "second_harmonic_generation")]
     assert cfg.tasks[2].numerics == {"broadening": 0.03}
+    assert cfg.tasks[2].selectors == {} and cfg.tasks[0].selectors == {}
+    body = BASE.replace('kind = "SHG"', 'kind = "SHG"\noutput = "both"\nresponse = "susceptibility"')
+    shg = R.load_run_config(_write_cfg(tmp_path, body)).tasks[2]
+    assert shg.selectors == {"output": "both", "response": "susceptibility"}
     # explicit list, explicit system name, default output root = the run file's directory
     body = BASE.replace("photon_energy_min_ev = 0.0\nphoton_energy_max_ev = 1.0\nphoton_energy_step_ev = 0.3",
                         "photon_energies_ev = [0.5, 1.5]")
@@ -241,6 +245,8 @@ def test_load_run_config(tmp_path):
     ("photon_energy_step_ev = 0.3", "photon_energy_step_ev = 0.3\nphoton_energy_num_points = 4"),
     ('source = "wannier"', 'source = "openmx"'),
     ("temperature_k = 0.0", "temperature_k = -1.0"),
+    ('kind = "SC"', 'kind = "SC"\noutput = "terms"'),
+    ('kind = "SHG"', 'kind = "SHG"\noutput = "all"'),
     ("k_mesh = [3, 2, 2]", "k_mesh = [3, 0, 2]"),
     ("photon_energy_min_ev = 0.0\n", ""),
 ])

@@ -19,7 +19,8 @@ Supported run-file keys (others are rejected, so that a run never silently ignor
 ``fermi_energy_ev``, ``temperature_k``; ``[response.sampling] kind = "bz_mesh", k_mesh,
 spatial_dimension = 3``; ``[response.execution] fourier_backend`` (any value: meshes always use the
 engine's separable contraction); ``[response.tasks.<id>] kind`` in SC / shift_current, IC /
-injection_current, SHG / second_harmonic_generation, ``method = "Conventional"``, optional
+injection_current, SHG / second_harmonic_generation (with the selectors ``output`` = total / terms /
+both and ``response`` = conductivity / susceptibility / both), ``method = "Conventional"``, optional
 ``[response.tasks.<id>.numerics]`` with ``broadening``, ``broadening_type``,
 ``denominator_regularization``, ``transition_window_factor``, ``band_window_size``,
 ``degeneracy_threshold``, ``low_frequency_broadening``, ``intermediate_regularization``, ``eta_correction``; ``[output]
@@ -221,6 +222,7 @@ class RunTask:
     task_id: str
     quantity: str
     numerics: Dict[str, object] = field(default_factory=dict)
+    selectors: Dict[str, str] = field(default_factory=dict)   # SHG output / response
 
 
 @dataclass
@@ -286,15 +288,25 @@ def load_run_config(path: str) -> RunConfig:
     _check_keys(resp.get("execution", {}), {"fourier_backend"}, "response.execution")
     tasks = []
     for tid, tab in sorted(resp.get("tasks", {}).items()):
-        _check_keys(tab, {"kind", "method", "numerics"}, f"response.tasks.{tid}")
+        _check_keys(tab, {"kind", "method", "numerics", "output", "response"}, f"response.tasks.{tid}")
         kind = str(tab.get("kind", "")).lower()
         if kind not in TASK_KINDS:
             raise ValueError(f"response.tasks.{tid}: unsupported kind {tab.get('kind')!r}")
+        # SHG selectors (_task_input_explicit_selectors, src/Runtime/TaskInputParameters.jl:50-68)
+        selectors = {}
+        for key, choices in (("output", ("total", "terms", "both")),
+                             ("response", ("conductivity", "susceptibility", "both"))):
+            if key in tab:
+                if TASK_KINDS[kind] != "second_harmonic_generation":
+                    raise ValueError(f"response.tasks.{tid}.{key} is not used by this observation.")
+                if tab[key] not in choices:
+                    raise ValueError(f"response.tasks.{tid}.{key} has an invalid explicit selector.")
+                selectors[key] = tab[key]
         if str(tab.get("method", "Conventional")).lower() != "conventional":
             raise ValueError(f"response.tasks.{tid}: only method = \"Conventional\" is supported")
         num = dict(tab.get("numerics", {}))
         _check_keys(num, NUMERICS_KEYS, f"response.tasks.{tid}.numerics")
-        tasks.append(RunTask(tid, TASK_KINDS[kind], num))
+        tasks.append(RunTask(tid, TASK_KINDS[kind], num, selectors))
     if not tasks:
         raise ValueError("response.tasks: at least one task is required")
     out = doc.get("output", {})
@@ -355,6 +367,8 @@ def run(path: str) -> List[str]:
                 for t in tasks:
                     written += write_second_harmonic(os.path.join(cfg.output_root, t.task_id),
                                                      cfg.photon_energies[:, None], total["shg"], comps,
+                                                     output=t.selectors.get("output", "total"),
+                                                     response=t.selectors.get("response", "both"),
                                                      digits=cfg.digits or 12,
                                                      domain_metadata=full_zone_metadata(cfg.k_mesh))
                 continue
