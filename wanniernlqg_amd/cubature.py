@@ -200,22 +200,21 @@ def apply_rule(rule: BoxRule, lower: np.ndarray, upper: np.ndarray, values: np.n
     lo = np.einsum("m,bmt->bt", rule.embedded, flat)
     I = (vol[:, None] * hi).reshape((values.shape[0],) + tail)
     Ie = (vol[:, None] * lo).reshape((values.shape[0],) + tail)
-    E = np.array([_norm(I[b] - Ie[b]) for b in range(values.shape[0])])
+    B = values.shape[0]
+    E = np.max(np.abs((I - Ie).reshape(B, -1)), axis=1) if I[0].size else np.zeros(B)
     if rule.dimension == 1:
-        kdiv = np.zeros(values.shape[0], dtype=np.int64)
+        kdiv = np.zeros(B, dtype=np.int64)
     else:
         d = rule.dimension
         f1 = flat[:, 0]
-        diffs = np.empty((values.shape[0], d))
+        diffs = np.empty((B, d))
         for i in range(d):
             f2 = flat[:, 1 + 2 * i] + flat[:, 2 + 2 * i]
             f3 = flat[:, 1 + 2 * d + 2 * i] + flat[:, 2 + 2 * d + 2 * i]
             diffs[:, i] = np.max(np.abs(f3 + 12.0 * f1 - 7.0 * f2), axis=1)
-        kdiv = np.empty(values.shape[0], dtype=np.int64)
-        for b in range(values.shape[0]):
-            top = diffs[b].max()
-            near = np.nonzero(diffs[b] >= top * (1.0 - 1e-10))[0]
-            kdiv[b] = near[np.argmax(half[b, near])]           # widest; first on equal widths
+        # largest fourth difference; equal to a relative 1e-10: the widest axis, then the lowest index
+        near = diffs >= diffs.max(axis=1, keepdims=True) * (1.0 - 1e-10)
+        kdiv = np.argmax(np.where(near, half, -np.inf), axis=1).astype(np.int64)
     return I, E, kdiv
 
 
