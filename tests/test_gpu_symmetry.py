@@ -89,3 +89,25 @@ def test_run_points_rejects_other_families():
                                       denominator_regularization=0.0)) as eng:
         with pytest.raises(E.WnlqgError):
             eng.run_points(np.zeros((4, 3)), np.ones(4))
+
+
+def test_identity_group_reproduces_mesh_lanes():
+    """With the identity alone every point represents itself with weight 1/N_k: the reduced path's lanes
+    are the mesh lanes (same partition), up to where the weight enters the product."""
+    model = synthetic_model(16, 1, 2, position_scale=0.1)
+    ops = S.group_from_generators([], np.asarray(model.lattice))
+    mesh = (6, 5, 4)
+    say("[symmetry] identity group, n=16, mesh (6, 5, 4)")
+    with E.Engine(model, E.EngineTask(**task_kw())) as eng:
+        lanes, total, _ = eng.run_mesh(mesh)
+        lr, tr, orb = S.integrate_symmetric(eng, mesh, ops, n_lanes=lanes["sc"].shape[0], path="planes")
+        lp, tp, _ = S.integrate_symmetric(eng, mesh, ops, n_lanes=lanes["sc"].shape[0], path="points")
+        # one point, one lane
+        l1, t1 = eng.run_points(np.array([[0.1, 0.2, 0.3]]), np.array([0.5]), 1)
+        v1 = eng.eval_points(np.array([[0.1, 0.2, 0.3]]))
+    assert len(orb.representatives) == int(np.prod(mesh))
+    for key in ("sc", "ic"):
+        assert rel(lr[key], lanes[key]) < 1e-12
+        assert rel(lp[key], lanes[key]) < 1e-12
+        assert rel(tr[key], total[key]) < 1e-12
+        assert rel(t1[key], 0.5 * v1[key][0]) < 1e-12
