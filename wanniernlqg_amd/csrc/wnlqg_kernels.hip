@@ -2005,7 +2005,7 @@ __global__ __launch_bounds__(256) void k_screen(const double *__restrict__ eg, c
 
 // Generalized derivatives and SC/IC pair kernels of the pairs k_screen listed, one 256-thread
 // workgroup per k point.
-__global__ __launch_bounds__(256, 4) void k_pairs_f64(const double *__restrict__ x, const double *__restrict__ eg,
+__global__ __launch_bounds__(256, 3) void k_pairs_f64(const double *__restrict__ x, const double *__restrict__ eg,
                                                   const int *__restrict__ count, const int4 *__restrict__ pmeta,
                                                   const double2 *__restrict__ pwd, double2 *__restrict__ qout,
                                                   WnShape sh, WnScreen sc) {
