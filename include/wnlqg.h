@@ -28,7 +28,7 @@
 extern "C" {
 #endif
 
-#define WNLQG_ABI_VERSION 5
+#define WNLQG_ABI_VERSION 6
 
 /* Status codes. */
 #define WNLQG_OK 0
@@ -38,6 +38,8 @@ extern "C" {
 #define WNLQG_E_HIP (-4)         /* HIP runtime error */
 #define WNLQG_E_OOM (-5)         /* device allocation failed */
 #define WNLQG_E_UNSUPPORTED (-6) /* configuration outside what this build implements */
+#define WNLQG_E_NOT_CONVERGED (-7) /* the Jacobi eigensolver reached its sweep cap (40) at some k point without
+                                      meeting its stopping test (LAPACK's zheevr reports INFO > 0 there) */
 
 /* Response quantities (bitmask). Output states are ordered by increasing bit.  One engine serves one
  * reference driver family: SC and/or IC (IntegralDriver.jl), or BCD alone (BerryMultipoleDriver.jl),

@@ -22,7 +22,17 @@ def header_functions():
 
 def test_library_loads_and_version():
     lib = E.load_library()
-    assert lib.wnlqg_version() == 5
+    declared = int(re.search(r"#define WNLQG_ABI_VERSION (\d+)", open(HEADER).read()).group(1))
+    assert lib.wnlqg_version() == declared == E.ABI_VERSION
+
+
+def test_status_codes_match_header():
+    """Every WNLQG_E_* code of the header (including the eigensolver's sweep-cap failure) has the same
+    value and name in the binding's error map."""
+    codes = dict((int(v), name) for name, v in
+                 re.findall(r"#define (WNLQG_E_\w+) \((-\d+)\)", open(HEADER).read()))
+    assert codes == E.ERRORS
+    assert codes[-7] == "WNLQG_E_NOT_CONVERGED"
 
 
 def test_exports_match_header():

@@ -156,3 +156,18 @@ def test_bcd_and_ahc_in_one_engine_match_separate_engines():
         _, t1, _ = b.run_mesh((4, 3, 5))
         _, t2, _ = a.run_mesh((4, 3, 5))
     assert np.array_equal(tb["bcd"], t1["bcd"]) and np.array_equal(tb["ahc"], t2["ahc"])
+
+
+def test_bcd_periodic_model_matches_reference_closed_form():
+    """The reference's periodic two-band model (test/berry_multipole_periodic_unit.jl:1-68) on the engine:
+    every moment component against the closed-form curvature and band gradients (no oracle in between)."""
+    from test_oracle_response_pins import (MU_PERIODIC, T_PERIODIC, periodic_bcd_closed_form,
+                                           periodic_bcd_model)
+    say("bcd periodic two-band model, 32x32x1, mu -1.3 eV, T 800 K")
+    model = periodic_bcd_model()
+    with bcd_engine(model, dict(fermi_energies=[MU_PERIODIC], temperature=T_PERIODIC),
+                    degeneracy_threshold=0.002) as eng:
+        _, total, _ = eng.run_mesh((32, 32, 1))
+    ref = periodic_bcd_closed_form(32)
+    got = total["bcd"][0]
+    assert np.max(np.abs(got - ref)) <= 1e-10 * np.max(np.abs(ref))

@@ -361,3 +361,39 @@ def test_planes_equal_direct(name):
     assert sd["separable"] == 0 and sp["separable"] == 1
     for key in ("sc", "ic"):
         assert rel(tp[key], td[key]) < 1e-11
+
+
+@pytest.mark.parametrize("with_positions", [False, True])
+def test_point_values_equal_projector_formulation(with_positions):
+    """Per-point SC and IC of the engine (eta = 0) against the projector / band-velocity formulation of the
+    theory notes in SI units (tests/test_oracle_response_pins.py), with no oracle in between; with
+    positions the model carries A(0) = diag(tau) and the reference side uses the Convention-I matrix."""
+    import test_oracle_response_pins as pins
+    model, tau = pins._tau_model() if with_positions else (pins._sc_model(), None)
+    kappa = pins.KAPPAS[0]
+    omega, width, sigma, eta, _ = pins.si_point_values(model, kappa, tau=tau)
+    say(f"projector pin on the GPU: positions={with_positions}")
+    with E.Engine(model, E.EngineTask(photon_energies=omega, broadening=width, shift_current=True,
+                                      injection_current=True, denominator_regularization=0.0)) as eng:
+        vals = eng.eval_points(pins._frac(model, kappa)[None])
+    assert np.max(np.abs(vals["sc"][0] - sigma)) <= 1e-6 * np.max(np.abs(sigma))
+    assert np.max(np.abs(vals["ic"][0] - eta)) <= 1e-6 * np.max(np.abs(eta))
+
+
+def test_eigensolver_sweep_cap_reports_not_converged(monkeypatch):
+    """A sweep cap of 1 (set from the host before the engine is created) leaves every cold solve
+    unconverged: the point and mesh entry points fail with WNLQG_E_NOT_CONVERGED instead of returning the
+    unconverged eigenpairs; without the cap the same engine settings succeed."""
+    model = config_model(0)
+    kp = np.random.default_rng(5).uniform(0, 1, (16, 3))
+    say("sweep cap 1: spectrum, eval and mesh must report non-convergence")
+    monkeypatch.setenv("WNLQG_JACOBI_MAX_SWEEPS", "1")
+    with engine(model) as eng:
+        for call in (lambda: eng.spectrum_points(kp), lambda: eng.eval_points(kp), lambda: eng.run_mesh((4, 4, 4))):
+            with pytest.raises(E.WnlqgError) as ei:
+                call()
+            assert ei.value.code == -7
+    monkeypatch.delenv("WNLQG_JACOBI_MAX_SWEEPS")
+    with engine(model) as eng:
+        eng.eval_points(kp)
+        eng.run_mesh((4, 4, 4))

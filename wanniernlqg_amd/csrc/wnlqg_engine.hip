@@ -328,9 +328,12 @@ int run_front(wnlqg_engine *g, int count, int rows, bool eig) {
     WN_HIP(launch_interp(g, rows, g->x.p));
     if (eig)
         WN_HIP(wn_launch_eig_f64(g->x.p, g->e.p, g->u.p, g->sweeps.p, count, g->sh, g->eig_tpb, g->stream,
-                                 g->escratch.p));
+                                 g->escratch.p, g->flags.p));
     return WNLQG_OK;
 }
+
+int check_flags(wnlqg_engine *g);
+hipError_t clear_flags(wnlqg_engine *g);
 
 int upload_points(wnlqg_engine *g, const double *kpoints, int m, int rows) {
     std::vector<double> k((size_t)rows * 3, 0.0);
@@ -598,6 +601,13 @@ int32_t wnlqg_create(const wnlqg_model *model, const wnlqg_task *task, wnlqg_eng
             g->rmin[a] = std::min(g->rmin[a], model->r_vectors[3 * r + a]);
             rmax[a] = std::max(rmax[a], model->r_vectors[3 * r + a]);
         }
+    // Jacobi sweep cap (a point that reaches it fails the call with WNLQG_E_NOT_CONVERGED); converged
+    // solves take 4-8 sweeps.  WNLQG_JACOBI_MAX_SWEEPS lowers it (tests force the failure path with it).
+    g->sh.max_sweeps = 40;
+    if (const char *ms = std::getenv("WNLQG_JACOBI_MAX_SWEEPS")) {
+        const int v = std::atoi(ms);
+        if (v >= 1 && v <= 40) g->sh.max_sweeps = v;
+    }
     g->sh.S = 1;
     for (int a = 0; a < 3; ++a) {
         g->span[a] = rmax[a] - g->rmin[a] + 1;
@@ -722,6 +732,7 @@ int32_t wnlqg_fourier_points(wnlqg_engine *g, const double *kpoints, int32_t m, 
 int32_t wnlqg_spectrum_points(wnlqg_engine *g, const double *kpoints, int32_t m, double *energies, double *vectors) {
     if (!g || !kpoints || !energies || !vectors || m < 0) return fail(WNLQG_E_INVALID, "invalid arguments");
     WN_HIP(hipSetDevice(g->device));
+    WN_HIP(clear_flags(g));
     const size_t n = g->sh.n;
     for (int p0 = 0; p0 < m; p0 += g->batch) {
         int cnt = std::min(g->batch, m - p0);
@@ -736,7 +747,7 @@ int32_t wnlqg_spectrum_points(wnlqg_engine *g, const double *kpoints, int32_t m,
                               hipMemcpyDeviceToHost, g->stream));
         WN_HIP(hipStreamSynchronize(g->stream));
     }
-    return WNLQG_OK;
+    return check_flags(g);
 }
 
 namespace {
@@ -744,6 +755,9 @@ namespace {
 int check_flags(wnlqg_engine *g) {
     int f[4] = {0, 0, 0, 0};
     WN_HIP(hipMemcpy(f, g->flags.p, sizeof(f), hipMemcpyDeviceToHost));
+    if (f[1] & 1)
+        return fail(WNLQG_E_NOT_CONVERGED, "Jacobi eigensolver reached %d sweeps without converging at some k point",
+                    g->sh.max_sweeps);
     if (f[0] & 1)
         return fail(WNLQG_E_GAP,
                     "GEOMETRY_SINGLE_BAND_SPECTRUM_NOT_ISOLATED: denominator_regularization = 0 needs gaps > 1e-10 eV");
@@ -1025,18 +1039,18 @@ int enqueue_mesh(wnlqg_engine *g, std::vector<std::array<hipEvent_t, WN_MARKS>> 
             WN_HIP(wn_launch_warm_transform_f64(xb, ub, cnt, g->sh, s0, uprev));
             mark(bi, 9, s0);
             WN_HIP(wn_launch_eig_pass_f64(xb, eb, ub, sweep_max, sweep_total, cnt, g->sh, g->eig_tpb, 2, s0,
-                                          nullptr, uprev, WN_CTR_STRIDE));
+                                          nullptr, uprev, WN_CTR_STRIDE, g->flags.p));
         } else if (scheme == 1) {
             WN_HIP(wn_launch_eig_pass_f64(xb, eb, ub, sweep_max, sweep_total, cnt, g->sh, g->eig_tpb, 0, s0, nullptr,
-                                          nullptr, WN_CTR_STRIDE));
+                                          nullptr, WN_CTR_STRIDE, g->flags.p));
             mark(bi, 8, s0);
             WN_HIP(wn_launch_warm_transform_f64(xb, ub, cnt, g->sh, s0));
             mark(bi, 9, s0);
             WN_HIP(wn_launch_eig_pass_f64(xb, eb, ub, sweep_max, sweep_total, cnt, g->sh, g->eig_tpb, 1, s0, nullptr,
-                                          nullptr, WN_CTR_STRIDE));
+                                          nullptr, WN_CTR_STRIDE, g->flags.p));
         } else {
             WN_HIP(wn_launch_eig_pass_f64(xb, eb, ub, sweep_max, sweep_total, cnt, g->sh, g->eig_tpb, -1, s0,
-                                          g->escratch.p, nullptr, WN_CTR_STRIDE));
+                                          g->escratch.p, nullptr, WN_CTR_STRIDE, g->flags.p));
         }
         mark(bi, 3, s0);
         WN_HIP(hipEventRecord(g->ev_ready[buf], s0));
@@ -1443,17 +1457,17 @@ int32_t wnlqg_run_mesh_reduced(wnlqg_engine *g, const int32_t mesh[3], const int
             const double2 *uprev = g->u.p + (size_t)(buf ^ 1) * g->batch * g->sh.n * g->sh.n;
             WN_HIP(wn_launch_warm_transform_f64(xo, uo, cr, g->sh, s0, uprev));
             WN_HIP(wn_launch_eig_pass_f64(xo, eo, uo, sweep_max, sweep_total, cr, g->sh, g->eig_tpb, 2, s0, nullptr,
-                                          uprev, WN_CTR_STRIDE));
+                                          uprev, WN_CTR_STRIDE, g->flags.p));
         } else if (wn_warm_start(g->sh.n, cr)) {
             // groups of 4 consecutive representatives (mostly neighbours along axis 1)
             WN_HIP(wn_launch_eig_pass_f64(xo, eo, uo, sweep_max, sweep_total, cr, g->sh, g->eig_tpb, 0, s0, nullptr,
-                                          nullptr, WN_CTR_STRIDE));
+                                          nullptr, WN_CTR_STRIDE, g->flags.p));
             WN_HIP(wn_launch_warm_transform_f64(xo, uo, cr, g->sh, s0));
             WN_HIP(wn_launch_eig_pass_f64(xo, eo, uo, sweep_max, sweep_total, cr, g->sh, g->eig_tpb, 1, s0, nullptr,
-                                          nullptr, WN_CTR_STRIDE));
+                                          nullptr, WN_CTR_STRIDE, g->flags.p));
         } else {
             WN_HIP(wn_launch_eig_pass_f64(xo, eo, uo, sweep_max, sweep_total, cr, g->sh, g->eig_tpb, -1, s0,
-                                          g->escratch.p, nullptr, WN_CTR_STRIDE));
+                                          g->escratch.p, nullptr, WN_CTR_STRIDE, g->flags.p));
         }
         WN_HIP(wn_launch_screen(eo, g->omega.p, g->count.p, g->pmeta.p, g->pmd.p, g->flags.p, cr, g->sh, g->sc, s0));
         WN_HIP(wn_launch_weight_pairs(g->pmd.p, g->count.p, g->wts.p + rb[bi], cr, g->sh, s0));
@@ -1497,14 +1511,20 @@ int channels_for(int family, bool pos) {
 
 // Autotuner: Jacobi threads per k point for each distinct n_w, timed on the first batch of each size
 // class's first engine (interpolation of that batch, then the eigensolver with 64, 128, 256 threads).
+// n_w > 64 has a single eigensolver form (512 threads, V in global memory) and is not tuned.  Only
+// configurations whose launch and timing succeeded are eligible.
 int autotune(wnlqg_batch *b) {
-    std::vector<int> done(65, 0);
+    std::vector<int> done(129, 0);
     hipEvent_t a, z;
     WN_HIP(hipEventCreate(&a));
-    WN_HIP(hipEventCreate(&z));
+    if (hipEventCreate(&z) != hipSuccess) {
+        hipEventDestroy(a);
+        return fail(WNLQG_E_HIP, "autotune event creation failed");
+    }
     int rc = WNLQG_OK;
     for (wnlqg_engine *g : b->engines) {
         const int n = g->sh.n;
+        if (n < 1 || n > 64) continue;
         if (done[n]) {
             g->eig_tpb = done[n];
             continue;
@@ -1521,18 +1541,20 @@ int autotune(wnlqg_batch *b) {
         int best_tpb = 0;
         for (int tpb : {64, 128, 256}) {
             float ms = 0.0f;
-            for (int rep = 0; rep < 2; ++rep) {  // first launch warms up
-                hipEventRecord(a, g->stream);
-                wn_launch_eig_f64(g->x.p, g->e.p, g->u.p, g->sweeps.p, cnt, g->sh, tpb, g->stream, g->escratch.p);
-                hipEventRecord(z, g->stream);
+            bool ok = true;
+            for (int rep = 0; ok && rep < 2; ++rep) {  // first launch warms up
+                ok = hipEventRecord(a, g->stream) == hipSuccess &&
+                     wn_launch_eig_f64(g->x.p, g->e.p, g->u.p, g->sweeps.p, cnt, g->sh, tpb, g->stream, g->escratch.p,
+                                       g->flags.p) == hipSuccess &&
+                     hipEventRecord(z, g->stream) == hipSuccess;
                 if (hipEventSynchronize(z) != hipSuccess) {
                     rc = fail(WNLQG_E_HIP, "autotune timing failed");
                     break;
                 }
-                hipEventElapsedTime(&ms, a, z);
+                ok = ok && hipEventElapsedTime(&ms, a, z) == hipSuccess;
             }
             if (rc) break;
-            if (ms < best) {
+            if (ok && ms < best) {
                 best = ms;
                 best_tpb = tpb;
             }

@@ -15,6 +15,7 @@ struct WnShape {
     int nNt;      // 128-column tiles of the interpolation output
     int ldo;      // per-k row stride (doubles) of the interpolation output = nNt * 128
     int S;        // span of one phase table axis
+    int max_sweeps;  // Jacobi sweep cap; a solve that reaches it sets status word 1
     // stored channel s holds logical channel lch[s] of the full map (WN_CH_H .. WN_CH_DA + 8)
     signed char lch[WN_CH_FULL];
 };
@@ -103,8 +104,9 @@ hipError_t wn_launch_contract_sel(const double2 *in, double2 *out, const double2
 // Jacobi eigensolver of the upper triangle of channel 0.
 // tpb: threads per k point (64, 128, 256; 0 = default of the size class).
 // vscratch: [count][N][N] complex (N = 96 or 128), required for n > 64 (the Jacobi's V in global memory).
+// status (nullable): word 1 |= 1 when a point reaches sh.max_sweeps without meeting the stopping test.
 hipError_t wn_launch_eig_f64(const double *x, double *e, double2 *u, int *sweeps, int count, const WnShape &sh,
-                             int tpb, hipStream_t st, double2 *vscratch = nullptr);
+                             int tpb, hipStream_t st, double2 *vscratch = nullptr, int *status = nullptr);
 // Warm-started solve of count consecutive points of a mesh row (groups of 4 along the fastest axis):
 //   pass 0  wn_launch_eig_pass_f64(..., pass = 0): the group anchors from the identity;
 //           wn_launch_warm_transform_f64: channel 0 of the other points <- U_a^+ H U_a;
@@ -117,7 +119,8 @@ hipError_t wn_launch_eig_f64(const double *x, double *e, double2 *u, int *sweeps
 // buckets of the status words (sweeps = words + WN_CTR_BASE, sweep_total = words + WN_CTR_BASE + 2).
 hipError_t wn_launch_eig_pass_f64(const double *x, double *e, double2 *u, int *sweeps, unsigned long long *sweep_total,
                                   int count, const WnShape &sh, int tpb, int pass, hipStream_t st,
-                                  double2 *vscratch = nullptr, const double2 *uprev = nullptr, int ctr_stride = 0);
+                                  double2 *vscratch = nullptr, const double2 *uprev = nullptr, int ctr_stride = 0,
+                                  int *status = nullptr);
 // size in complex elements of the eigensolver / rotation scratch for count points (0 for n <= 64)
 size_t wn_eig_scratch_elems(int n, int count);
 size_t wn_rotate_scratch_elems(int n, int count);
@@ -126,7 +129,7 @@ hipError_t wn_launch_warm_transform_f64(double *x, const double2 *u, int count, 
                                         const double2 *uprev = nullptr);
 // Clears that are safe under stream capture: p[0 .. n) = 0; flags = {0, 0, 0, 0x7f7f7f7f}.
 hipError_t wn_launch_zero(double *p, long long n, hipStream_t st);
-// status words: [0] pair-gap flag, [2] geometry flags, [3] first failing k; then WN_CTR_BUCKETS counter
+// status words: [0] pair-gap flag, [1] eigensolver sweep cap reached, [2] geometry flags, [3] first failing k; then WN_CTR_BUCKETS counter
 // buckets of WN_CTR_STRIDE words (128 bytes apart, so that the per-workgroup atomics of a launch spread
 // over many addresses): bucket word [0] max sweeps, [2..3] total sweeps (64-bit), [4..5] total active
 // SC/IC pairs (64-bit), [6] SC/IC points rotated to pair elements, [7] SC/IC points rotated in full,

@@ -865,7 +865,8 @@ __global__ __launch_bounds__(TPB, (N <= 32 && TPB == 256) ? 6 : 1) void k_eig_f6
                                                 double2 *__restrict__ uout, int *__restrict__ sweeps_out,
                                                 unsigned long long *__restrict__ sweep_total, int n, int ldo,
                                                 double tol, int pass, int count, double2 *__restrict__ vscratch,
-                                                const double2 *__restrict__ uprev, int ctr_stride) {
+                                                const double2 *__restrict__ uprev, int ctr_stride,
+                                                int *__restrict__ status, int max_sweeps) {
     constexpr int LD = N + 1;
     constexpr int HALF = N / 2;
     constexpr int VITEMS = N * HALF;
@@ -939,8 +940,9 @@ __global__ __launch_bounds__(TPB, (N <= 32 && TPB == 256) ? 6 : 1) void k_eig_f6
     }
 
     int sweep = 0;
+    bool converged = false;
     double prev_off = 0.0;
-    for (; sweep < 40; ++sweep) {
+    for (; sweep < max_sweeps; ++sweep) {
         double off = 0.0, dia = 0.0;
         for (int e = t; e < N * N; e += TPB) {
             const int i = e / N, j = e - (e / N) * N;
@@ -972,8 +974,11 @@ __global__ __launch_bounds__(TPB, (N <= 32 && TPB == 256) ? 6 : 1) void k_eig_f6
         }
         // converged, or at the rounding floor (a sweep no longer reduces the off-diagonal mass
         // quadratically once it is below ~1e-20 of the total)
-        if (off <= tol * tol * (dia + off)) break;
-        if (sweep > 0 && off > 0.25 * prev_off && off <= 1e-20 * (dia + off)) break;
+        if (off <= tol * tol * (dia + off) ||
+            (sweep > 0 && off > 0.25 * prev_off && off <= 1e-20 * (dia + off))) {
+            converged = true;
+            break;
+        }
         prev_off = off;
         // round-robin slot t pairs p = (t - 1 + r) mod (N - 1) + 1 (0 for t = 0) with
         // q = (N - 2 - t + r) mod (N - 1) + 1: both residues advance by one per round
@@ -1167,11 +1172,14 @@ __global__ __launch_bounds__(TPB, (N <= 32 && TPB == 256) ? 6 : 1) void k_eig_f6
     const int bucket = ctr_stride * (int)(blockIdx.x % WN_CTR_BUCKETS);
     if (t == 0 && sweeps_out) atomicMax(sweeps_out + bucket, sweep);
     if (t == 0 && sweep_total) atomicAdd(sweep_total + bucket / 2, (unsigned long long)sweep);
+    // the sweep cap ended the solve before the stopping test passed: status word 1
+    if (t == 0 && !converged && status) atomicOr(status + 1, 1);
 }
 
 hipError_t wn_launch_eig_pass_f64(const double *x, double *e, double2 *u, int *sweeps, unsigned long long *sweep_total,
                                   int count, const WnShape &sh, int tpb, int pass, hipStream_t st, double2 *vscratch,
-                                  const double2 *uprev, int ctr_stride) {
+                                  const double2 *uprev, int ctr_stride, int *status) {
+    if (sh.max_sweeps < 1) return hipErrorInvalidValue;
     if (pass == 2 && (!uprev || !wn_warm_start(sh.n, count))) return hipErrorInvalidValue;
     const int n = sh.n;
     const double tol = 1.0e-15;
@@ -1180,7 +1188,7 @@ hipError_t wn_launch_eig_pass_f64(const double *x, double *e, double2 *u, int *s
     // tpb: threads per k point (64, 128 or 256); 0 selects the default of the size class
 #define WN_EIG_LAUNCH(NN, T) \
     hipLaunchKernelGGL((k_eig_f64<NN, T, false>), dim3(grid), dim3(T), 0, st, x, e, u, sweeps, sweep_total, n, sh.ldo, \
-                       tol, pass, count, nullptr, uprev, ctr_stride)
+                       tol, pass, count, nullptr, uprev, ctr_stride, status, sh.max_sweeps)
 #define WN_EIG_CASE(NN, DEF)                   \
     if (n <= NN) {                             \
         const int t = tpb ? tpb : DEF;         \
@@ -1205,10 +1213,10 @@ hipError_t wn_launch_eig_pass_f64(const double *x, double *e, double2 *u, int *s
         if (!vscratch) return hipErrorInvalidValue;
         if (n <= 96)
             hipLaunchKernelGGL((k_eig_f64<96, 512, true>), dim3(grid), dim3(512), 0, st, x, e, u, sweeps, sweep_total, n,
-                               sh.ldo, tol, pass, count, vscratch, nullptr, ctr_stride);
+                               sh.ldo, tol, pass, count, vscratch, nullptr, ctr_stride, status, sh.max_sweeps);
         else
             hipLaunchKernelGGL((k_eig_f64<128, 512, true>), dim3(grid), dim3(512), 0, st, x, e, u, sweeps, sweep_total,
-                               n, sh.ldo, tol, pass, count, vscratch, nullptr, ctr_stride);
+                               n, sh.ldo, tol, pass, count, vscratch, nullptr, ctr_stride, status, sh.max_sweeps);
         return hipGetLastError();
     }
     return hipErrorInvalidValue;
@@ -1740,8 +1748,8 @@ hipError_t wn_launch_rotate_sc(double *x, const double2 *u, const int *count, co
 }
 
 hipError_t wn_launch_eig_f64(const double *x, double *e, double2 *u, int *sweeps, int count, const WnShape &sh,
-                             int tpb, hipStream_t st, double2 *vscratch) {
-    return wn_launch_eig_pass_f64(x, e, u, sweeps, nullptr, count, sh, tpb, -1, st, vscratch, nullptr);
+                             int tpb, hipStream_t st, double2 *vscratch, int *status) {
+    return wn_launch_eig_pass_f64(x, e, u, sweeps, nullptr, count, sh, tpb, -1, st, vscratch, nullptr, 0, status);
 }
 
 bool wn_warm_start(int n, int count) { return n >= 16 && n <= 64 && count >= WN_WARM_G; }

@@ -33,7 +33,7 @@ WNLQG_PRECISION_MIXED_HIGH = 2
 WNLQG_PRECISION_SPLIT_BF16 = 3
 
 ERRORS = {-1: "WNLQG_E_INVALID", -2: "WNLQG_E_NONFINITE", -3: "WNLQG_E_GAP", -4: "WNLQG_E_HIP",
-          -5: "WNLQG_E_OOM", -6: "WNLQG_E_UNSUPPORTED"}
+          -5: "WNLQG_E_OOM", -6: "WNLQG_E_UNSUPPORTED", -7: "WNLQG_E_NOT_CONVERGED"}
 
 EXPORTED = ["wnlqg_version", "wnlqg_last_error", "wnlqg_create", "wnlqg_destroy", "wnlqg_num_states",
             "wnlqg_num_channels", "wnlqg_run_mesh", "wnlqg_run_points", "wnlqg_run_mesh_reduced",
@@ -89,7 +89,7 @@ class Stats(C.Structure):
 _lib = None
 
 
-ABI_VERSION = 5   # WNLQG_ABI_VERSION of include/wnlqg.h
+ABI_VERSION = 6   # WNLQG_ABI_VERSION of include/wnlqg.h
 
 
 def load_library(path: str = LIB_PATH) -> C.CDLL:
