@@ -1,0 +1,547 @@
+// Batched Hermitian Jacobi eigensolver with register-resident eigenvectors (n_w <= 32).
+//
+// Same mathematics and sweep order as the LDS kernel k_eig_f64 (wnlqg_kernels.hip): the upper triangle
+// of channel 0 is the Hermitian matrix (DenseSpectra.hermitian_spectrum, reference
+// src/Core/ESKernel/DenseSpectra.jl:14-16, uplo 'U'), parallel cyclic Jacobi over a round-robin
+// tournament (element 0 fixed, the others moving one position per round), the 2x2 block form of a round
+// on the packed upper triangle, the same stopping rule, ascending sort and canonical column phase
+// (canonicalize_eigenvectors!, HamiltonianMatrixElements.jl:6-25).  What differs is where the work lives:
+//
+//   * A workgroup of two waves serves KPW = 64 / N k points (N = 8, 16, 32: the padded size).
+//   * Wave 0 owns H: both packed triangles in LDS, N lanes per k point.  It computes the rotation
+//     parameters of a round (lanes 0 .. N/2-1 of each k point's lane group), then the off-diagonal block
+//     updates of that round, then the parameters of the next round: every read of H follows the writes
+//     it depends on in the same wave's program order, so H needs no workgroup barrier.
+//   * Wave 1 owns V: lane i of a k point's group holds row i of V (N complex values) in registers.
+//     V <- V J combines two columns of the same row, so a row never leaves its lane; the round loop is
+//     unrolled, which makes every round's column pairs compile-time register indices.
+//   * One barrier per round hands the round's rotation parameters (double-buffered in LDS) from wave 0
+//     to wave 1.  A k point that has converged gets identity rotations, which leave H and V unchanged
+//     exactly, so its result is what a solve of that point alone gives.
+//
+// V's LDS traffic (every rotated column pair read and written each round) is gone; what remains in
+// LDS is the packed H (read and written once per block per round) and the parameter broadcast.
+
+#include "wnlqg_common.h"
+#include "wnlqg_kernels.h"
+
+namespace {
+
+// Orders this wave's LDS accesses in program order for the compiler (the LDS executes one wave's
+// instructions in issue order, so cross-lane reads after writes of the same wave need no wait).
+__device__ __forceinline__ void wn_wave_order() { __asm__ volatile("" ::: "memory"); }
+
+__device__ __forceinline__ void wn_wg_barrier() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+    __builtin_amdgcn_s_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
+}
+
+// Round-robin tournament of N players in N - 1 rounds (circle method): slot t of round r holds
+//   left a = 0 (t = 0) or (t - 1 + r) mod (N - 1) + 1,   right b = (N - 2 - t + r) mod (N - 1) + 1,
+// the LDS kernel's pairs.  The rotation of a slot acts on (left, right) in that order.
+__host__ __device__ constexpr int wn_rr_a(int N, int t, int r) {
+    return t == 0 ? 0 : (t - 1 + r) % (N - 1) + 1;
+}
+__host__ __device__ constexpr int wn_rr_b(int N, int t, int r) { return (N - 2 - t + r) % (N - 1) + 1; }
+// (left, right) of slot t in round r at run time: one conditional subtraction per residue
+__device__ __forceinline__ int2 wn_rr_rt(int N, int t, int r) {
+    int a = t - 1 + r;
+    a = a >= N - 1 ? a - (N - 1) : a;
+    int b = N - 2 - t + r;
+    b = b >= N - 1 ? b - (N - 1) : b;
+    return make_int2(t == 0 ? 0 : a + 1, b + 1);
+}
+
+// A row of V held as register tuples of 8 complex values (16 doubles): element updates are writes into the
+// same tuple, so the row keeps its registers from one sweep to the next.
+typedef double f64x16 __attribute__((ext_vector_type(16)));
+template <int N>
+struct WnRow {
+    f64x16 t[(N + 7) / 8];
+    template <int C>
+    __device__ __forceinline__ double2 get() const {
+        return make_double2(t[C / 8][2 * (C % 8)], t[C / 8][2 * (C % 8) + 1]);
+    }
+    template <int C>
+    __device__ __forceinline__ void set(double2 v) {
+        t[C / 8][2 * (C % 8)] = v.x;
+        t[C / 8][2 * (C % 8) + 1] = v.y;
+    }
+};
+
+// V <- V J for one row: columns (p, q) with R = (c, s, cos phi, sin phi)
+template <int N, int P, int Q>
+__device__ __forceinline__ void wn_rotate_row(WnRow<N> &V, const double4 R) {
+    const double2 xp = V.template get<P>(), yq = V.template get<Q>();
+    const double2 ey = make_double2(R.z * yq.x + R.w * yq.y, R.z * yq.y - R.w * yq.x);
+    V.template set<P>(make_double2(R.x * xp.x - R.y * ey.x, R.x * xp.y - R.y * ey.y));
+    V.template set<Q>(make_double2(R.y * xp.x + R.x * ey.x, R.y * xp.y + R.x * ey.y));
+}
+
+// One unrolled round of the V wave: every slot's rotation applied to this lane's row.
+// The parameters are loaded four slots at a time (a compiler fence between groups), so that the row and
+// the parameters in flight stay within the register budget of two waves per SIMD.
+template <int N, int R, int T>
+struct WnVSlots {
+    __device__ __forceinline__ static void run(WnRow<N> &V, const double4 *__restrict__ rot) {
+        // parameters loaded four slots at a time (a compiler fence between groups)
+        if constexpr (T % 4 == 0) wn_wave_order();
+        wn_rotate_row<N, wn_rr_a(N, T, R), wn_rr_b(N, T, R)>(V, rot[T]);
+        WnVSlots<N, R, T + 1>::run(V, rot);
+    }
+};
+template <int N, int R>
+struct WnVSlots<N, R, N / 2> {
+    __device__ __forceinline__ static void run(WnRow<N> &, const double4 *__restrict__) {}
+};
+
+// The V wave's rounds of one sweep; the round's parameters are in rot[(cb + R) & 1].  The barrier of round 0
+// also publishes the go flag, read there once (the H wave rewrites it for the next sweep while this wave may
+// still be in its last round): without another sweep the V wave stops there and run returns false.
+template <int N, int R>
+struct WnVRounds {
+    __device__ __forceinline__ static bool run(WnRow<N> &V, const double4 (*rot)[64 / N][N / 2], int j, int cb,
+                                               const volatile int &go) {
+        wn_wg_barrier();
+        if (R == 0 && !go) return false;
+        WnVSlots<N, R, 0>::run(V, rot[(cb + R) & 1][j]);
+        return WnVRounds<N, R + 1>::run(V, rot, j, cb, go);
+    }
+};
+template <int N>
+struct WnVRounds<N, N - 1> {
+    __device__ __forceinline__ static bool run(WnRow<N> &, const double4 (*)[64 / N][N / 2], int, int,
+                                               const volatile int &) {
+        return true;
+    }
+};
+
+template <int N, int C>
+struct WnRowIo {
+    __device__ __forceinline__ static void load(WnRow<N> &V, const double2 *__restrict__ ua, int n, int l) {
+        V.template set<C>((ua && l < n && C < n) ? ua[l * n + C] : make_double2(l == C ? 1.0 : 0.0, 0.0));
+        WnRowIo<N, C + 1>::load(V, ua, n, l);
+    }
+    // canonical phase of column C (first largest |V[i][C]| over the rows i < n made real and >= 0), written
+    // to column rank[C] of the output
+    __device__ __forceinline__ static void store(const WnRow<N> &V, double2 *__restrict__ uk, const int *rank, int n,
+                                                 int l, int base) {
+        if (C < n) {   // uniform over the wave
+            const double2 v = V.template get<C>();
+            double m = l < n ? hypot(v.x, v.y) : -1.0;
+            int idx = l;
+#pragma unroll
+            for (int o = N / 2; o > 0; o >>= 1) {
+                const double om = __shfl_xor(m, o);
+                const int oi = __shfl_xor(idx, o);
+                if (om > m || (om == m && oi < idx)) {
+                    m = om;
+                    idx = oi;
+                }
+            }
+            const double2 pv = make_double2(__shfl(v.x, base + idx), __shfl(v.y, base + idx));
+            const double2 ph = m > 0.0 ? make_double2(pv.x / m, -pv.y / m) : make_double2(1.0, 0.0);
+            if (l < n) uk[l * n + rank[C]] = cmul(v, ph);
+        }
+        WnRowIo<N, C + 1>::store(V, uk, rank, n, l, base);
+    }
+};
+template <int N>
+struct WnRowIo<N, N> {
+    __device__ __forceinline__ static void load(WnRow<N> &, const double2 *__restrict__, int, int) {}
+    __device__ __forceinline__ static void store(const WnRow<N> &, double2 *__restrict__, const int *, int, int, int) {}
+};
+
+}  // namespace
+
+// Slot-ordered storage of H.  Round r of the tournament pairs the bands (a_t(r), b_t(r)) in slot t; H is
+// kept with the bands of slot t at positions (2t, 2t+1) (left, right), so that slot t's 2x2 diagonal block
+// and every off-diagonal block (t1, t2) sit at the same addresses in every round.  Between rounds the
+// circle method moves every band one place along the cycle R_0 -> R_1 -> ... -> R_{H-1} -> L_{H-1} -> ...
+// -> L_1 -> R_0 (L_t = 2t, R_t = 2t + 1, band 0 fixed at L_0): the same position map sigma every round.
+// A round reads its blocks from one buffer and writes each updated element to (sigma(x), sigma(y)) of the
+// other buffer (conjugated when that lands below the diagonal), so every address and conjugation of a
+// lane is fixed for the whole solve.  Storage: upper-triangle 2x2 blocks (t1 <= t2) row-major, element
+// (u, v) of block b at [2u + v][b] (each element plane contiguous over blocks).  After a whole sweep
+// (N - 1 rounds) every band is back at its round-0 position pos0(b).
+template <int N>
+struct WnSlots {
+    static constexpr int HALF = N / 2;
+    static constexpr int NB = HALF * (HALF + 1) / 2;
+    __host__ __device__ static constexpr int blk(int t1, int t2) { return t1 * HALF - t1 * (t1 - 1) / 2 + (t2 - t1); }
+    __host__ __device__ static constexpr int sigma(int x) {
+        return x == 0 ? 0
+                      : ((x & 1) ? ((x >> 1) < HALF - 1 ? x + 2 : x - 1) : ((x >> 1) == 1 ? 1 : x - 2));
+    }
+    __host__ __device__ static constexpr int pos0(int b) { return b < HALF ? 2 * b : 2 * (N - 1 - b) + 1; }
+    // storage index of element (x, y), x <= y
+    __host__ __device__ static constexpr int at(int x, int y) { return (2 * (x & 1) + (y & 1)) * NB + blk(x >> 1, y >> 1); }
+    // destination of the element now at (x, y), x < y: index, with bit 31 set when stored conjugated
+    __host__ __device__ static constexpr int dest(int x, int y) {
+        return sigma(x) < sigma(y) ? at(sigma(x), sigma(y)) : (at(sigma(y), sigma(x)) | (int)0x80000000);
+    }
+};
+
+template <int N>
+struct WnEigShared {
+    double2 Hs[2][64 / N][4 * WnSlots<N>::NB];   // slot-ordered H, two buffers
+    double4 rot[2][64 / N][N / 2];               // (c, s, cos phi, sin phi) of the round, two buffers
+    int rank[64 / N][N];                          // position of band c in the ascending spectrum
+    int go_flag;                                  // another sweep follows
+};
+
+// Rotation parameters of one slot (branch-free, so that the compiler can interleave the dependent chain
+// with independent block updates): H's 2x2 diagonal block [[a, h], [h*, d]] of the (left, right) pair.
+// rotate = false gives the identity (1, 0, 1, 0) and leaves the block as it is.
+struct WnRotParams {
+    double4 R;       // (c, s, cos phi, sin phi)
+    double an, dn;   // new diagonal entries
+    double2 h;       // new (left, right) entry: 0 after a rotation
+};
+__device__ __forceinline__ WnRotParams wn_rot_params(double a, double d, double2 h, bool rotate) {
+    // smaller root t of t^2 + 2 tau t - 1 = 0, tau = (d - a)/(2b): with root = sqrt((d-a)^2 + 4b^2) and
+    // u = |d-a| + root: c = u / sqrt(2 root u), s = sign(d-a) 2b / sqrt(2 root u), t b = sign(d-a) 2b^2 / u
+    const double b2 = h.x * h.x + h.y * h.y;
+    rotate = rotate && b2 > 0.0;
+    const double b2s = rotate ? b2 : 1.0;
+    const double ib = wn_rsqrt(b2s);
+    const double diff = d - a;
+    const double rr = diff * diff + 4.0 * b2s;
+    const double root = rr * wn_rsqrt(rr);
+    const double u = fabs(diff) + root;
+    const double iw = wn_rsqrt(2.0 * root * u);
+    const double sg = diff >= 0.0 ? 1.0 : -1.0;
+    const double tb = sg * 2.0 * b2s * wn_rcp(u);
+    WnRotParams o;
+    o.R = rotate ? make_double4(u * iw, sg * 2.0 * (b2s * ib) * iw, h.x * ib, h.y * ib)
+                 : make_double4(1.0, 0.0, 1.0, 0.0);
+    o.an = rotate ? a - tb : a;
+    o.dn = rotate ? d + tb : d;
+    o.h = rotate ? make_double2(0.0, 0.0) : h;
+    return o;
+}
+
+// One off-diagonal block B (t1 < t2) of round buffer Hr: B <- J1^+ B J2, its four elements written to the
+// next round's positions dst[e] (bit 31: stored conjugated) of Hw.
+__device__ __forceinline__ void wn_block_update(const double2 *Hr, double2 *Hw, int src, const int (&dst)[4], int NB,
+                                                const double4 R1, const double4 R2) {
+    double2 B[2][2];
+#pragma unroll
+    for (int e = 0; e < 4; ++e) B[e >> 1][e & 1] = Hr[e * NB + src];
+    // B J2: col0 = c2 B0 - s2 e^{-i phi2} B1, col1 = s2 B0 + c2 e^{-i phi2} B1
+    const double2 em2 = make_double2(R2.z, -R2.w);
+#pragma unroll
+    for (int u = 0; u < 2; ++u) {
+        const double2 y = cmul(em2, B[u][1]);
+        const double2 x0 = B[u][0];
+        B[u][0] = make_double2(R2.x * x0.x - R2.y * y.x, R2.x * x0.y - R2.y * y.y);
+        B[u][1] = make_double2(R2.y * x0.x + R2.x * y.x, R2.y * x0.y + R2.x * y.y);
+    }
+    // J1^+ M: row0 = c1 M0 - s1 e^{i phi1} M1, row1 = s1 M0 + c1 e^{i phi1} M1
+    const double2 ep1 = make_double2(R1.z, R1.w);
+#pragma unroll
+    for (int v = 0; v < 2; ++v) {
+        const double2 y = cmul(ep1, B[1][v]);
+        const double2 x0 = B[0][v];
+        B[0][v] = make_double2(R1.x * x0.x - R1.y * y.x, R1.x * x0.y - R1.y * y.y);
+        B[1][v] = make_double2(R1.y * x0.x + R1.x * y.x, R1.y * x0.y + R1.x * y.y);
+    }
+#pragma unroll
+    for (int e = 0; e < 4; ++e) {
+        const int dd = dst[e];
+        const double2 val = B[e >> 1][e & 1];
+        Hw[dd & 0x7fffffff] = make_double2(
+            val.x, __longlong_as_double(__double_as_longlong(val.y) ^ ((unsigned long long)(dd < 0) << 63)));
+    }
+}
+
+// Wave 0: H, the stopping test, the rotation parameters, the eigenvalues.
+//
+// Look-ahead: the diagonal blocks of round r + 1 come from round r's diagonal entries and from one
+// off-diagonal block per slot, the "critical" block whose (L_{t+1}, R_{t-1}) element becomes slot t's
+// (left, right) entry: (0, 1) for slot 0, (t-1, t+1) for 1 <= t <= H-2, (H-2, H-1) for slot H-1.  A round
+// updates the critical blocks first, then computes the next round's parameters (published before the
+// next barrier), then the other blocks; the V wave can start round r + 1 while those finish.  The
+// parameter step writes only diagonal entries and critical-block positions of the buffer being read,
+// which the other blocks do not read.  Barriers: N - 1 per sweep (the first also carries the go flag),
+// one at the end, as in wn_eig_v_role.
+template <int N>
+__device__ __forceinline__ void wn_eig_h_role(WnEigShared<N> &sm, const double *__restrict__ x,
+                                              double *__restrict__ eout, int *__restrict__ sweeps_out,
+                                              unsigned long long *__restrict__ sweep_total, int n, int ldo,
+                                              double tol, int k, bool active, int j, int l, int lane, int bucket,
+                                              int *__restrict__ status, int max_sweeps) {
+    using S = WnSlots<N>;
+    constexpr int HALF = N / 2;
+    constexpr int NB = S::NB;
+    constexpr int NOFF = HALF * (HALF - 1) / 2;
+    constexpr int NREST = NOFF - HALF;                  // off-diagonal blocks that are not critical
+    // block slots of a lane: slot 0 is slot cs's critical block on lanes l < HALF and one of the first
+    // HALF non-critical blocks on the others; the remaining non-critical blocks fill BPL more slots
+    constexpr int BPL = (NREST - HALF + N - 1) / N;
+    // H(k) upper triangle in band order -> slot order of round 0
+    const double2 *xk = active ? (const double2 *)(x + (size_t)k * ldo) : nullptr;
+    for (int e = l; e < N * N; e += N) {
+        const int i = e / N, jj = e - (e / N) * N;
+        if (i <= jj) {
+            double2 hv = make_double2(0.0, 0.0);
+            if (active && jj < n) {
+                hv = xk[i * n + jj];
+                if (i == jj) hv.y = 0.0;
+            }
+            const int pi = S::pos0(i), pj = S::pos0(jj);
+            if (pi <= pj)
+                sm.Hs[0][j][S::at(pi, pj)] = hv;
+            else
+                sm.Hs[0][j][S::at(pj, pi)] = make_double2(hv.x, -hv.y);
+        }
+    }
+    // fixed addresses of this lane.  Critical block of next-round slot cs = l mod HALF (lanes l < HALF);
+    // the other blocks in row-major order over the strict upper triangle of blocks, critical ones skipped.
+    auto critical = [](int t1, int t2) {
+        return (t1 == 0 && t2 == 1) || (t2 == t1 + 2) || (t1 == HALF - 2 && t2 == HALF - 1);
+    };
+    const int cs = l & (HALF - 1);
+    // the want-th non-critical block (t1, t2), or (-1, -1)
+    auto rest_block = [&](int want, int &t1, int &t2) {
+        t1 = -1;
+        t2 = -1;
+        if (want < 0 || want >= NREST) return;
+        int seen = 0;
+        for (int a = 0; a < HALF - 1 && t1 < 0; ++a)
+            for (int b = a + 1; b < HALF; ++b) {
+                if (critical(a, b)) continue;
+                if (seen == want) {
+                    t1 = a;
+                    t2 = b;
+                    break;
+                }
+                ++seen;
+            }
+    };
+    int ct1, ct2;
+    if (l < HALF) {
+        ct1 = cs == 0 ? 0 : (cs == HALF - 1 ? HALF - 2 : cs - 1);
+        ct2 = cs == 0 ? 1 : (cs == HALF - 1 ? HALF - 1 : cs + 1);
+    } else {
+        rest_block(l - HALF, ct1, ct2);
+    }
+    const int csrc = ct1 >= 0 ? S::blk(ct1, ct2) : 0;
+    int cdst[4];
+#pragma unroll
+    for (int e = 0; e < 4; ++e) cdst[e] = ct1 >= 0 ? S::dest(2 * ct1 + (e >> 1), 2 * ct2 + (e & 1)) : 0;
+    int bt1[BPL > 0 ? BPL : 1], bt2[BPL > 0 ? BPL : 1], bsrc[BPL > 0 ? BPL : 1], bdst[BPL > 0 ? BPL : 1][4];
+#pragma unroll
+    for (int qb = 0; qb < BPL; ++qb) {
+        const int want = HALF + l + qb * N;   // index among the non-critical blocks
+        int t1 = -1, t2 = -1;
+        rest_block(want, t1, t2);
+        bt1[qb] = t1;
+        bt2[qb] = t2;
+        bsrc[qb] = t1 >= 0 ? S::blk(t1, t2) : 0;
+#pragma unroll
+        for (int e = 0; e < 4; ++e) bdst[qb][e] = t1 >= 0 ? S::dest(2 * t1 + (e >> 1), 2 * t2 + (e & 1)) : 0;
+    }
+    // slot cs's diagonal block and where its entries go
+    const int dblk = S::blk(cs, cs);
+    const int dst_a = S::at(S::sigma(2 * cs), S::sigma(2 * cs));
+    const int dst_d = S::at(S::sigma(2 * cs + 1), S::sigma(2 * cs + 1));
+    const int dst_h = S::dest(2 * cs, 2 * cs + 1) & 0x7fffffff;
+    const bool writer = l < HALF;
+
+    // parameters of round r (buffer Hr) for slot cs, published in rot[g & 1], the diagonal block written to Hw
+    auto params = [&](const double2 *Hr, double2 *Hw, int r, int g, bool frz) {
+        const int2 pq = wn_rr_rt(N, cs, r);
+        const WnRotParams o = wn_rot_params(Hr[dblk].x, Hr[3 * NB + dblk].x, Hr[NB + dblk],
+                                            !frz && pq.x < n && pq.y < n);
+        if (writer) {
+            if (!frz) {
+                Hw[dst_a] = make_double2(o.an, 0.0);
+                Hw[dst_d] = make_double2(o.dn, 0.0);
+                Hw[dst_h] = o.h;
+            }
+            sm.rot[g & 1][j][cs] = o.R;
+        }
+    };
+
+    int sweep = 0, done_sweep = 0;
+    bool frozen = !active, converged = !active;
+    double prev_off = 0.0;
+    for (;;) {
+        const int cb = sweep & 1;   // buffer holding H at the start of this sweep (N - 1 rounds per sweep, odd)
+        const int g0 = sweep * (N - 1);
+        // stopping test of every k point (its N lanes): ||offdiag||_F against tol ||H||_F, or the rounding
+        // floor (a sweep no longer reduces the off-diagonal mass once it is ~1e-20 of the total)
+        double off = 0.0, dia = 0.0;
+        if (!frozen) {
+            for (int b = l; b < NB; b += N) {
+                int t1 = 0, rem = b;
+                while (rem >= HALF - t1) {
+                    rem -= HALF - t1;
+                    ++t1;
+                }
+                const double2 e0 = sm.Hs[cb][j][b], e1 = sm.Hs[cb][j][NB + b], e3 = sm.Hs[cb][j][3 * NB + b];
+                const double m0 = e0.x * e0.x + e0.y * e0.y, m1 = e1.x * e1.x + e1.y * e1.y;
+                const double m3 = e3.x * e3.x + e3.y * e3.y;
+                if (rem == 0) {   // diagonal block: (0,0), (1,1) on the diagonal, (0,1) twice off it
+                    dia += m0 + m3;
+                    off += 2.0 * m1;
+                } else {
+                    const double2 e2 = sm.Hs[cb][j][2 * NB + b];
+                    off += 2.0 * (m0 + m1 + m3 + e2.x * e2.x + e2.y * e2.y);
+                }
+            }
+        }
+#pragma unroll
+        for (int o = N / 2; o > 0; o >>= 1) {
+            off += __shfl_xor(off, o);
+            dia += __shfl_xor(dia, o);
+        }
+        if (!frozen && (off <= tol * tol * (dia + off) ||
+                        (sweep > 0 && off > 0.25 * prev_off && off <= 1e-20 * (dia + off)))) {
+            frozen = true;
+            converged = true;
+            done_sweep = sweep;
+        }
+        prev_off = off;
+        const bool more = __ballot(!frozen) != 0ull && sweep < max_sweeps;
+        if (lane == 0) sm.go_flag = more ? 1 : 0;
+        if (more) {
+            wn_wave_order();
+            params(sm.Hs[cb][j], sm.Hs[cb ^ 1][j], 0, g0, frozen);
+        }
+        for (int r = 0; r < N - 1; ++r) {
+            wn_wg_barrier();
+            if (r == 0 && !more) break;
+            const int rb = (cb + r) & 1;   // buffer read in this round; the other one is written
+            const double2 *Hr = sm.Hs[rb][j];
+            double2 *Hw = sm.Hs[rb ^ 1][j];
+            const double4 *R = sm.rot[(g0 + r) & 1][j];
+            if (!frozen) {
+                if (ct1 >= 0) wn_block_update(Hr, Hw, csrc, cdst, NB, R[ct1], R[ct2]);
+                wn_wave_order();
+            }
+            if (r + 1 < N - 1) params(Hw, sm.Hs[rb][j], r + 1, g0 + r + 1, frozen);
+            if (!frozen) {
+#pragma unroll
+                for (int qb = 0; qb < BPL; ++qb)
+                    if (bt1[qb] >= 0) wn_block_update(Hr, Hw, bsrc[qb], bdst[qb], NB, R[bt1[qb]], R[bt2[qb]]);
+            }
+            wn_wave_order();
+        }
+        if (!more) break;
+        ++sweep;
+    }
+    // eigenvalues ascending (stable by index) and their ranks.  A point's H stopped moving at the sweep it
+    // converged (or at the cap), with every band at pos0.
+    const int fb = (converged ? done_sweep : sweep) & 1;
+    if (active && l < n) {
+        auto diag = [&](int b) {
+            const int p = S::pos0(b);
+            return sm.Hs[fb][j][S::at(p, p)].x;
+        };
+        const double et = diag(l);
+        int rk = 0;
+        for (int jj = 0; jj < n; ++jj) {
+            const double ej = diag(jj);
+            rk += (ej < et) || (ej == et && jj < l);
+        }
+        sm.rank[j][l] = rk;
+        eout[(size_t)k * n + rk] = et;
+    }
+    if (active && l == 0) {
+        const int sw = converged ? done_sweep : sweep;
+        if (sweeps_out) atomicMax(sweeps_out + bucket, sw);
+        if (sweep_total) atomicAdd(sweep_total + bucket / 2, (unsigned long long)sw);
+        if (!converged && status) atomicOr(status + 1, 1);
+    }
+    wn_wg_barrier();
+}
+
+// Wave 1: the rows of V in registers; applies every round's rotations, then writes the canonical
+// eigenvectors in ascending order.
+template <int N>
+__device__ __forceinline__ void wn_eig_v_role(WnEigShared<N> &sm, double2 *__restrict__ uout,
+                                              const double2 *__restrict__ ua, int n, int k, bool active, int j,
+                                              int l) {
+    WnRow<N> V;
+    WnRowIo<N, 0>::load(V, ua, n, l);
+    for (int sweep = 0;; ++sweep)
+        if (!WnVRounds<N, 0>::run(V, sm.rot, j, sweep & 1, sm.go_flag)) break;
+    wn_wg_barrier();
+    if (!active) return;
+    WnRowIo<N, 0>::store(V, uout + (size_t)k * n * n, sm.rank[j], n, l, j * N);
+}
+
+// pass < 0: every point from the identity; pass 0 / 1: group anchors, then the other points of each group
+// from their anchor's eigenvectors (wn_warm_k); pass 2: every point from uprev[k].
+template <int N>
+__global__ __launch_bounds__(128, 2) void k_eig_rv(const double *__restrict__ x, double *__restrict__ eout,
+                                                   double2 *__restrict__ uout, int *__restrict__ sweeps_out,
+                                                   unsigned long long *__restrict__ sweep_total, int n, int ldo,
+                                                   double tol, int pass, int count, int items,
+                                                   const double2 *__restrict__ uprev, int ctr_stride,
+                                                   int *__restrict__ status, int max_sweeps) {
+    constexpr int KPW = 64 / N;   // k points per workgroup
+    __shared__ WnEigShared<N> sm;
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int j = lane / N, l = lane - (lane / N) * N;   // k slot of the workgroup, lane within its group
+    const int item = blockIdx.x * KPW + j;
+    int anchor = -1, k = -1;
+    if (item < items) {
+        if (pass < 0 || pass == 2) {
+            k = item;
+        } else {
+            // wn_warm_k (wnlqg_kernels.hip): anchors first, then the other points of each group of 4
+            constexpr int G = 4, A = 1;
+            if (pass == 0) {
+                const int base = item * G;
+                k = base + min(A, count - 1 - base);
+            } else {
+                const int g = item / (G - 1), jj = item - g * (G - 1);
+                const int base = g * G;
+                const int al = min(A, count - 1 - base);
+                anchor = base + al;
+                k = base + (jj < al ? jj : jj + 1);
+            }
+        }
+    }
+    const bool active = k >= 0 && k < count;
+    if (wave == 0) {
+        const int bucket = ctr_stride * (int)(item % WN_CTR_BUCKETS);
+        wn_eig_h_role<N>(sm, x, eout, sweeps_out, sweep_total, n, ldo, tol, k, active, j, l, lane, bucket, status,
+                         max_sweeps);
+    } else {
+        const double2 *ua = !active ? nullptr
+                                    : (pass == 2 ? uprev + (size_t)k * n * n
+                                                 : (anchor >= 0 ? uout + (size_t)anchor * n * n : nullptr));
+        wn_eig_v_role<N>(sm, uout, ua, n, k, active, j, l);
+    }
+}
+
+bool wn_eig_rv_supported(int n) { return n >= 1 && n <= 32; }
+
+hipError_t wn_launch_eig_rv(const double *x, double *e, double2 *u, int *sweeps, unsigned long long *sweep_total,
+                            int count, const WnShape &sh, int pass, hipStream_t st, const double2 *uprev,
+                            int ctr_stride, int *status) {
+    const int n = sh.n;
+    if (!wn_eig_rv_supported(n) || sh.max_sweeps < 1) return hipErrorInvalidValue;
+    if (pass == 2 && !uprev) return hipErrorInvalidValue;
+    const double tol = 1.0e-15;
+    constexpr int G = 4;
+    const int groups = (count + G - 1) / G;
+    const int items = (pass < 0 || pass == 2) ? count : pass == 0 ? groups : groups * (G - 1);
+    if (items <= 0) return hipSuccess;
+#define WN_RV_LAUNCH(NN)                                                                                          \
+    hipLaunchKernelGGL((k_eig_rv<NN>), dim3((items + 64 / NN - 1) / (64 / NN)), dim3(128), 0, st, x, e, u, sweeps, \
+                       sweep_total, n, sh.ldo, tol, pass, count, items, uprev, ctr_stride, status, sh.max_sweeps)
+    if (n <= 8)
+        WN_RV_LAUNCH(8);
+    else if (n <= 16)
+        WN_RV_LAUNCH(16);
+    else
+        WN_RV_LAUNCH(32);
+#undef WN_RV_LAUNCH
+    return hipGetLastError();
+}

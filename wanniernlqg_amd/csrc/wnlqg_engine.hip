@@ -608,6 +608,14 @@ int32_t wnlqg_create(const wnlqg_model *model, const wnlqg_task *task, wnlqg_eng
         const int v = std::atoi(ms);
         if (v >= 1 && v <= 40) g->sh.max_sweeps = v;
     }
+    // eigensolver form: WNLQG_EIG_KERNEL = "lds64" / "lds128" / "lds256" (LDS kernel with that many threads per
+    // point) or "regv" (register-resident eigenvectors, n_w <= 32); default: the size class's default
+    if (const char *ek = std::getenv("WNLQG_EIG_KERNEL")) {
+        if (!strcmp(ek, "lds64")) g->eig_tpb = 64;
+        else if (!strcmp(ek, "lds128")) g->eig_tpb = 128;
+        else if (!strcmp(ek, "lds256")) g->eig_tpb = 256;
+        else if (!strcmp(ek, "regv")) g->eig_tpb = WN_EIG_RV;
+    }
     g->sh.S = 1;
     for (int a = 0; a < 3; ++a) {
         g->span[a] = rmax[a] - g->rmin[a] + 1;
@@ -1509,8 +1517,9 @@ int channels_for(int family, bool pos) {
     return pos ? WN_CH_FULL : WN_CH_BASE;
 }
 
-// Autotuner: Jacobi threads per k point for each distinct n_w, timed on the first batch of each size
-// class's first engine (interpolation of that batch, then the eigensolver with 64, 128, 256 threads).
+// Autotuner: eigensolver form for each distinct n_w, timed on the first batch of each size class's first
+// engine (interpolation of that batch, then the register-resident-V kernel and the LDS kernel with 64, 128,
+// 256 threads per k point).
 // n_w > 64 has a single eigensolver form (512 threads, V in global memory) and is not tuned.  Only
 // configurations whose launch and timing succeeded are eligible.
 int autotune(wnlqg_batch *b) {
@@ -1539,7 +1548,11 @@ int autotune(wnlqg_batch *b) {
         if (rc) break;
         float best = 1e30f;
         int best_tpb = 0;
-        for (int tpb : {64, 128, 256}) {
+        // candidates: the register-resident-V kernel (n_w <= 32), then the LDS kernel at 64, 128, 256 threads
+        std::vector<int> cands;
+        if (wn_eig_rv_supported(n)) cands.push_back(WN_EIG_RV);
+        for (int tpb : {64, 128, 256}) cands.push_back(tpb);
+        for (int tpb : cands) {
             float ms = 0.0f;
             bool ok = true;
             for (int rep = 0; ok && rep < 2; ++rep) {  // first launch warms up

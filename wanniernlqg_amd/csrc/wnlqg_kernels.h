@@ -121,6 +121,14 @@ hipError_t wn_launch_eig_pass_f64(const double *x, double *e, double2 *u, int *s
                                   int count, const WnShape &sh, int tpb, int pass, hipStream_t st,
                                   double2 *vscratch = nullptr, const double2 *uprev = nullptr, int ctr_stride = 0,
                                   int *status = nullptr);
+// Register-resident-V Jacobi (wnlqg_eig.hip) for n <= 32: the same passes and stopping rule as the LDS
+// kernel, two waves per workgroup (H in LDS, V rows in registers), 64 / N points per workgroup.  It is the
+// default of wn_launch_eig_pass_f64 for n <= 32 (tpb 0 or WN_EIG_RV); tpb 64, 128, 256 select the LDS kernel.
+#define WN_EIG_RV 1
+bool wn_eig_rv_supported(int n);
+hipError_t wn_launch_eig_rv(const double *x, double *e, double2 *u, int *sweeps, unsigned long long *sweep_total,
+                            int count, const WnShape &sh, int pass, hipStream_t st, const double2 *uprev,
+                            int ctr_stride, int *status);
 // size in complex elements of the eigensolver / rotation scratch for count points (0 for n <= 64)
 size_t wn_eig_scratch_elems(int n, int count);
 size_t wn_rotate_scratch_elems(int n, int count);

@@ -1182,6 +1182,8 @@ hipError_t wn_launch_eig_pass_f64(const double *x, double *e, double2 *u, int *s
     if (sh.max_sweeps < 1) return hipErrorInvalidValue;
     if (pass == 2 && (!uprev || !wn_warm_start(sh.n, count))) return hipErrorInvalidValue;
     const int n = sh.n;
+    if (wn_eig_rv_supported(n) && (tpb == 0 || tpb == WN_EIG_RV))
+        return wn_launch_eig_rv(x, e, u, sweeps, sweep_total, count, sh, pass, st, uprev, ctr_stride, status);
     const double tol = 1.0e-15;
     const int groups = (count + WN_WARM_G - 1) / WN_WARM_G;
     const int grid = (pass < 0 || pass == 2) ? count : pass == 0 ? groups : groups * (WN_WARM_G - 1);
