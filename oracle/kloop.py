@@ -147,6 +147,26 @@ def interpolate(model, kpts: np.ndarray, channels: Optional[dict] = None) -> dic
     return {key: contract(val) for key, val in ch.items()}
 
 
+def interpolate_factored(model, kpts: np.ndarray) -> dict:
+    """The channels of :func:`interpolate` with the R-dependent factors moved onto the Fourier factors:
+    dH_a(k) = sum_R (i R_a f_R) H(R), ddH_ab(k) = sum_R (-R_a R_b f_R) H(R), dA_ab(k) = sum_R (i R_b f_R) A_a(R).
+    The same sums, without materialising the R-space channels (large models: n = 96, N_R = 2197)."""
+    f = fourier_factors(model, kpts)                   # [K, N_R]
+    rc = r_cartesian(model)                            # [N_R, 3]
+    n, nr = model.num_orbitals, model.num_r_vectors
+    hr = model.hamiltonian_r.reshape(nr, n * n)
+    K = f.shape[0]
+    out = {"H": (f @ hr).reshape(K, n, n)}
+    out["dH"] = np.stack([((1j * rc[None, :, a] * f) @ hr).reshape(K, n, n) for a in range(3)], axis=1)
+    out["ddH"] = np.stack([np.stack([((-rc[None, :, a] * rc[None, :, b] * f) @ hr).reshape(K, n, n)
+                                     for b in range(3)], axis=1) for a in range(3)], axis=1)
+    ar = model.position_or_zero()                      # [N_R, 3, n, n]
+    out["A"] = np.stack([(f @ ar[:, a].reshape(nr, n * n)).reshape(K, n, n) for a in range(3)], axis=1)
+    out["dA"] = np.stack([np.stack([((1j * rc[None, :, b] * f) @ ar[:, a].reshape(nr, n * n)).reshape(K, n, n)
+                                    for b in range(3)], axis=1) for a in range(3)], axis=1)
+    return out
+
+
 # ----------------------------------------------------------------------------------------
 # Eigensystem
 # ----------------------------------------------------------------------------------------

@@ -244,3 +244,13 @@ def test_lane_partition(nk):
 def test_mesh_order_axis1_fastest():
     k = kloop.integral_kpoints((4, 3, 2))
     assert np.allclose(k[1], [0.25, 0, 0]) and np.allclose(k[4], [0, 1 / 3, 0]) and np.allclose(k[12], [0, 0, 0.5])
+
+
+def test_factored_interpolation_equals_channel_sums():
+    """interpolate_factored (R factors on the Fourier factors) == interpolate (R-space channels) to rounding."""
+    m = synthetic_model(6, 2, 13, position_scale=0.1, lattice_constant=3.3)
+    k = np.random.default_rng(8).uniform(0, 1, (3, 3))
+    a, b = kloop.interpolate(m, k), kloop.interpolate_factored(m, k)
+    for key in ("H", "dH", "ddH", "A", "dA"):
+        assert a[key].shape == b[key].shape
+        assert np.max(np.abs(a[key] - b[key])) <= 1e-12 * max(1.0, np.max(np.abs(a[key]))), key
