@@ -308,6 +308,22 @@ int32_t wnlqg_mlp_set_params(wnlqg_mlp *mlp, const float *params);  /* also rese
 int32_t wnlqg_mlp_predict(wnlqg_mlp *mlp, const float *x, int64_t n, float *y);
 /* gradient of the mean squared error over all n samples, and the loss */
 int32_t wnlqg_mlp_gradient(wnlqg_mlp *mlp, const float *x, const float *t, int64_t n, float *grad, float *loss);
+/*
+ * wnlqg_mlp_gradient without any allocation.  No counterpart in the reference: it is for callers that cannot
+ * allocate device memory (code under HIP graph capture, tight serving loops, harnesses that forbid
+ * hipMalloc).  x [n][3], t [n] and grad [WNLQG_MLP_PARAMS + 1] are DEVICE pointers; grad receives the
+ * gradient of the mean squared error followed by the SUM of squared errors (divide by n for the loss).
+ * The per-wave partial gradients live in a library-owned static device array of
+ * WNLQG_MLP_NOALLOC_SCRATCH_BYTES per GPU (held until the process ends) instead of buffers of the mlp object;
+ * the call makes no hipMalloc / hipFree.  1 <= n <= WNLQG_MLP_NOALLOC_MAX_SAMPLES, else WNLQG_E_INVALID.
+ * stream: a hipStream_t to enqueue on and return immediately, or NULL for the mlp's own stream, synchronized
+ * before returning.  Every call on one GPU shares that GPU's array: calls that overlap in time on one GPU
+ * (other streams, other threads, other mlp objects) race and must be serialized by the caller.
+ */
+#define WNLQG_MLP_NOALLOC_SCRATCH_BYTES (512ULL * 4482ULL * 4ULL)
+#define WNLQG_MLP_NOALLOC_MAX_SAMPLES (1LL << 40)
+int32_t wnlqg_mlp_gradient_noalloc(wnlqg_mlp *mlp, const float *x, const float *t, int64_t n, float *grad,
+                                   void *stream);
 /* `steps` Adam steps of `batch` samples; loss_history[steps] (may be NULL) receives each batch loss */
 int32_t wnlqg_mlp_train(wnlqg_mlp *mlp, const float *x, const float *t, int64_t n, int32_t steps, int32_t batch,
                         float lr, uint64_t seed, float *loss_history);

@@ -17,6 +17,7 @@
 #include <cstdarg>
 #include <cstdio>
 #include <cstring>
+#include <mutex>
 #include <random>
 #include <string>
 #include <vector>
@@ -214,9 +215,42 @@ struct Buf {
 
 struct wnlqg_mlp {
     hipStream_t stream = nullptr;
+    int device = 0;
     Buf<float> P, m, v, grad, partial, loss, x, t, y;
     long long step = 0;
 };
+
+// Scratch of wnlqg_mlp_gradient_noalloc: the per-wave partial gradients [MLP_GRAD_BLOCKS * MLP_WAVES][MLP_NP]
+// followed by the per-wave losses (WNLQG_MLP_NOALLOC_SCRATCH_BYTES, about 9.2 MB).  Every GPU the library runs
+// on holds its own copy until the process ends.  Host code never uses the name `scratch` as an address: the
+// device address comes from hipGetSymbolAddress, once per device.
+#define MLP_SCRATCH_FLOATS ((size_t)MLP_GRAD_BLOCKS * MLP_WAVES * (MLP_NP + 1))
+static_assert(MLP_SCRATCH_FLOATS * sizeof(float) == WNLQG_MLP_NOALLOC_SCRATCH_BYTES, "scratch size");
+static __device__ __align__(256) float scratch[MLP_SCRATCH_FLOATS];
+
+namespace {
+
+#define MLP_MAX_DEVICES 64
+std::mutex g_scratch_mutex;
+float *g_scratch_ptr[MLP_MAX_DEVICES] = {};
+
+// device address of `scratch` on `device` (which must be current): resolved on the first call per device
+int scratch_address(int device, float **out) {
+    if (device < 0 || device >= MLP_MAX_DEVICES) return mlp_fail(WNLQG_E_UNSUPPORTED, "device %d out of range", device);
+    std::lock_guard<std::mutex> lock(g_scratch_mutex);
+    if (!g_scratch_ptr[device]) {
+        void *ptr = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !ptr)
+            return mlp_fail(WNLQG_E_HIP, "hipGetSymbolAddress(scratch) failed on device %d: %s", device,
+                            hipGetErrorString(e));
+        g_scratch_ptr[device] = (float *)ptr;
+    }
+    *out = g_scratch_ptr[device];
+    return WNLQG_OK;
+}
+
+}  // namespace
 
 namespace {
 
@@ -230,16 +264,21 @@ int upload_samples(wnlqg_mlp *g, const float *x, const float *t, long long n) {
     return WNLQG_OK;
 }
 
-// full gradient of the mean squared error over samples [0, n) already on the device
-int launch_gradient(wnlqg_mlp *g, const float *x, const float *t, long long n) {
+// full gradient of the mean squared error over samples [0, n) already on the device, into grad[MLP_NP + 1]
+// (the sum of squared errors last); partial: [nwaves][MLP_NP] per-wave gradients, loss: [nwaves]
+int launch_gradient_to(wnlqg_mlp *g, const float *x, const float *t, long long n, float *partial, float *loss,
+                       float *grad, hipStream_t st) {
     const int nwaves = MLP_GRAD_BLOCKS * MLP_WAVES;
     const long long per_wave = (n + nwaves - 1) / nwaves;
-    hipLaunchKernelGGL(k_mlp_grad, dim3(MLP_GRAD_BLOCKS), dim3(64 * MLP_WAVES), 0, g->stream, g->P.p, x, t, n,
-                       per_wave, 2.0f / (float)n, g->partial.p, g->loss.p);
-    hipLaunchKernelGGL(k_mlp_reduce, dim3((MLP_NP + 1 + 255) / 256), dim3(256), 0, g->stream, g->partial.p, g->loss.p,
-                       nwaves, g->grad.p);
+    hipLaunchKernelGGL(k_mlp_grad, dim3(MLP_GRAD_BLOCKS), dim3(64 * MLP_WAVES), 0, st, g->P.p, x, t, n, per_wave,
+                       2.0f / (float)n, partial, loss);
+    hipLaunchKernelGGL(k_mlp_reduce, dim3((MLP_NP + 1 + 255) / 256), dim3(256), 0, st, partial, loss, nwaves, grad);
     MLP_HIP(hipGetLastError());
     return WNLQG_OK;
+}
+
+int launch_gradient(wnlqg_mlp *g, const float *x, const float *t, long long n) {
+    return launch_gradient_to(g, x, t, n, g->partial.p, g->loss.p, g->grad.p, g->stream);
 }
 
 }  // namespace
@@ -263,6 +302,7 @@ int32_t wnlqg_mlp_create(uint64_t seed, wnlqg_mlp **out) {
         wnlqg_mlp_destroy(g);
         return code;
     };
+    if (hipGetDevice(&g->device) != hipSuccess) return bail(mlp_fail(WNLQG_E_HIP, "no HIP device"));
     if (hipStreamCreateWithFlags(&g->stream, hipStreamNonBlocking) != hipSuccess)
         return bail(mlp_fail(WNLQG_E_HIP, "stream creation failed"));
     const int nwaves = MLP_GRAD_BLOCKS * MLP_WAVES;
@@ -344,6 +384,25 @@ int32_t wnlqg_mlp_gradient(wnlqg_mlp *g, const float *x, const float *t, int64_t
     MLP_HIP(hipMemcpy(h.data(), g->grad.p, (MLP_NP + 1) * sizeof(float), hipMemcpyDeviceToHost));
     memcpy(grad, h.data(), MLP_NP * sizeof(float));
     if (loss) *loss = h[MLP_NP] / (float)n;
+    return WNLQG_OK;
+}
+
+int32_t wnlqg_mlp_gradient_noalloc(wnlqg_mlp *g, const float *x, const float *t, int64_t n, float *grad,
+                                   void *stream) {
+    if (!g || !x || !t || !grad || n < 1) return mlp_fail(WNLQG_E_INVALID, "invalid arguments");
+    if (n > WNLQG_MLP_NOALLOC_MAX_SAMPLES)
+        return mlp_fail(WNLQG_E_INVALID, "%lld samples exceed the no-allocation limit of %lld", (long long)n,
+                        (long long)WNLQG_MLP_NOALLOC_MAX_SAMPLES);
+    MLP_HIP(hipSetDevice(g->device));
+    float *scr = nullptr;
+    int rc = scratch_address(g->device, &scr);
+    if (rc) return rc;
+    const size_t nwaves = (size_t)MLP_GRAD_BLOCKS * MLP_WAVES;
+    hipStream_t st = stream ? (hipStream_t)stream : g->stream;
+    // every per-wave partial and loss is written by k_mlp_grad before k_mlp_reduce reads it: nothing to clear
+    rc = launch_gradient_to(g, x, t, n, scr, scr + nwaves * MLP_NP, grad, st);
+    if (rc) return rc;
+    if (!stream) MLP_HIP(hipStreamSynchronize(st));
     return WNLQG_OK;
 }
 

@@ -42,7 +42,8 @@ EXPORTED = ["wnlqg_version", "wnlqg_last_error", "wnlqg_create", "wnlqg_destroy"
             "wnlqg_fourier_points",
             "wnlqg_spectrum_points", "wnlqg_synchronize", "wnlqg_mlp_create", "wnlqg_mlp_destroy",
             "wnlqg_mlp_last_error", "wnlqg_mlp_get_params", "wnlqg_mlp_set_params", "wnlqg_mlp_predict",
-            "wnlqg_mlp_gradient", "wnlqg_mlp_train", "wnlqg_batch_create", "wnlqg_batch_run", "wnlqg_batch_tuning",
+            "wnlqg_mlp_gradient", "wnlqg_mlp_gradient_noalloc", "wnlqg_mlp_train", "wnlqg_batch_create",
+            "wnlqg_batch_run", "wnlqg_batch_tuning",
             "wnlqg_batch_destroy"]
 
 
@@ -126,6 +127,8 @@ def load_library(path: str = LIB_PATH) -> C.CDLL:
     lib.wnlqg_mlp_set_params.argtypes = [C.c_void_p, fp]
     lib.wnlqg_mlp_predict.argtypes = [C.c_void_p, fp, C.c_int64, fp]
     lib.wnlqg_mlp_gradient.argtypes = [C.c_void_p, fp, fp, C.c_int64, fp, fp]
+    lib.wnlqg_mlp_gradient_noalloc.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_int64, C.c_void_p,
+                                               C.c_void_p]
     lib.wnlqg_mlp_train.argtypes = [C.c_void_p, fp, fp, C.c_int64, C.c_int32, C.c_int32, C.c_float, C.c_uint64, fp]
     lib.wnlqg_batch_create.argtypes = [C.c_int32, C.POINTER(_Model), C.POINTER(_Task), C.POINTER(C.c_int32),
                                        C.POINTER(C.c_int32), C.c_int32, C.POINTER(C.c_void_p)]

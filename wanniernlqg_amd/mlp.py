@@ -76,6 +76,13 @@ class ProposalNetwork:
         self._check(self.lib.wnlqg_mlp_gradient(self.handle, _fp(x), _fp(t), x.shape[0], _fp(g), C.byref(loss)))
         return g, float(loss.value)
 
+    def gradient_noalloc(self, x_ptr: int, t_ptr: int, n: int, grad_ptr: int, stream: int = 0):
+        """wnlqg_mlp_gradient_noalloc on device buffers (addresses): x [n][3], t [n], grad [PARAMS + 1] floats;
+        grad receives the gradient of the mean squared error and then the sum of squared errors.  stream 0:
+        the network's own stream, synchronized before returning."""
+        self._check(self.lib.wnlqg_mlp_gradient_noalloc(self.handle, C.c_void_p(x_ptr), C.c_void_p(t_ptr), int(n),
+                                                        C.c_void_p(grad_ptr), C.c_void_p(stream or None)))
+
     def train(self, x: np.ndarray, t: np.ndarray, steps: int, batch: int = 4096, lr: float = 3e-3,
               seed: int = 0, history: bool = True) -> Optional[np.ndarray]:
         x = np.ascontiguousarray(x, dtype=np.float32).reshape(-1, 3)
