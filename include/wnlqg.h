@@ -56,8 +56,11 @@ extern "C" {
 
 /* Arithmetic forms. */
 #define WNLQG_PRECISION_FP64 0  /* FP64 everywhere (plain GPU path) */
-#define WNLQG_PRECISION_MIXED 1 /* split-FP16 (3 products) MFMA interpolation with FP64 partial sums, FP64
-                                   eigensolve, denominators and vertices, Kahan k-sum (SC / IC only) */
+#define WNLQG_PRECISION_MIXED 1 /* SC / IC only.  Arbitrary k points and non-separable meshes: split-FP16
+                                   (3 products) MFMA interpolation with FP64 partial sums.  Separable meshes
+                                   (even n_w in [16, 64]): FP64 plane contraction whose channels after H are
+                                   stored in FP32 (half the channel traffic).  Always: FP64 eigensolve,
+                                   rotation arithmetic, denominators and vertices, Kahan-compensated k-sum */
 #define WNLQG_PRECISION_MIXED_HIGH 2 /* split-FP16 with 4 products */
 #define WNLQG_PRECISION_SPLIT_BF16 3 /* split-BF16 (3 products) on BF16 MFMA with FP32 accumulation and FP64
                                         partial sums; otherwise as MIXED.  Comparison form: the rounding of

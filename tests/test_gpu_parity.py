@@ -330,10 +330,12 @@ def test_mixed_mesh_vs_fp64(prec, tol):
     assert worst > 0.0      # the reduced-precision GEMM did run
 
 
-@pytest.mark.parametrize("prec", ["mixed", "mixed6"])
-def test_mixed_separable_mesh_vs_fp64(prec):
-    """On the separable mesh path the mixed forms run the FP64 contraction, eigensolver and rotation and
-    differ from FP64 only by the compensated lane sums: they agree to rounding."""
+@pytest.mark.parametrize("prec,tol", [("mixed", 1e-6), ("mixed6", 1e-11)])
+def test_mixed_separable_mesh_vs_fp64(prec, tol):
+    """On the separable mesh path the mixed form stores the interpolated channels after H in FP32 (the
+    contraction computes in FP64; the rotation, eigensolver, vertices and compensated lane sums are FP64):
+    within the metric's 1e-6 relative spectrum error of FP64, and visibly not FP64.  mixed6 keeps FP64
+    channels and agrees to rounding."""
     model = MODELS["n32_pos"]()
     kw = dict(photon_energies=np.linspace(0, 6, 120), broadening=0.03)
     say(f"[mesh-sep-{prec}] n32 6x6x6 planes")
@@ -342,10 +344,14 @@ def test_mixed_separable_mesh_vs_fp64(prec):
     with engine(model, precision=prec, **kw) as emx:
         _, tmx, smx = emx.run_mesh((6, 6, 6))
     assert s64["separable"] == 1 and smx["separable"] == 1
+    worst = 0.0
     for key in ("sc", "ic"):
         r = kloop.spectrum_relative_error(tmx[key], t64[key]).max()
         say(f"  {key} worst component rel {r:.3e}")
-        assert r < 1e-11
+        worst = max(worst, r)
+        assert r < tol
+    if prec == "mixed":
+        assert worst > 1e-12      # the FP32 channels did run
 
 
 @pytest.mark.parametrize("name", ["fixture_n4", "n32_pos"])

@@ -135,6 +135,7 @@ struct wnlqg_engine {
     int kind = 0;               // 0 FP64, 1 split FP16, 2 split BF16
     int backend = WNLQG_FOURIER_AUTO;
     bool separable = false;     // meshes use the separable plane contraction
+    bool ch32 = false;          // mixed form, SC / IC on separable meshes: channels after H stored in FP32
     int span[3] = {1, 1, 1};    // R box extent per axis
     DevBuf<double> box;         // [S1*S2*S3][ldo] channel rows of the dense R box (FP64)
     DevBuf<double2> y3, y23, e1, e2, e3;
@@ -695,6 +696,10 @@ int32_t wnlqg_create(const wnlqg_model *model, const wnlqg_task *task, wnlqg_eng
                 hipStreamSynchronize(g->stream) != hipSuccess)
                 return bail(fail(WNLQG_E_HIP, "R box build failed"));
             g->separable = true;
+            // the mixed form keeps the interpolated channels after H in FP32 on its mesh path (SC / IC, where the
+            // element rotation reads them and writes back FP64)
+            g->ch32 = g->family == 0 && g->precision == WNLQG_PRECISION_MIXED && wn_rotate_sc_elements(g->sh) &&
+                      n % 2 == 0;
         }
     }
 
@@ -983,7 +988,8 @@ hipError_t launch_contract_batch(wnlqg_engine *g, const MeshBatch &bt, double *x
                                    g->span[1] * m2, m2, (long long)bt.k2c * m2, m2, s0);
         if (e == hipSuccess)
             e = wn_launch_contract(g->y23.p, xj, g->e1.p, p.mesh[0], 0, g->span[0], (int)m2, bt.k2c, m2,
-                                   (long long)bt.k2c * m2, (long long)p.mesh[0] * m2, m2, s0);
+                                   (long long)bt.k2c * m2, (long long)p.mesh[0] * m2, m2, s0,
+                                   g->ch32 ? g->sh.n * g->sh.n : 0);
     }
     return e;
 }
@@ -1071,7 +1077,7 @@ int enqueue_mesh(wnlqg_engine *g, std::vector<std::array<hipEvent_t, WN_MARKS>> 
             WN_HIP(wn_launch_screen(eb, g->omega.p, g->count.p, g->pmeta.p, g->pmd.p, g->flags.p, cnt, g->sh, g->sc,
                                     s1));
             WN_HIP(wn_launch_rotate_sc(xb, ub, g->count.p, g->pmeta.p, g->pwd.p, cnt, g->sh, s1, g->rscratch.p,
-                                       g->flags.p));
+                                       g->flags.p, g->ch32 && p.sep));
         } else {
             WN_HIP(wn_launch_rotate_f64(xb, ub, cnt, g->sh, s1, g->rscratch.p));
         }

@@ -94,8 +94,10 @@ hipError_t wn_launch_interp_f16(const _Float16 *bh, const _Float16 *bm, const do
 hipError_t wn_launch_build_box(const double *hr, const double *ar, const double *rc, const int *deg, const int *boxmap,
                                double *box, long long nbox, const WnShape &sh, hipStream_t st);
 hipError_t wn_launch_axis_phases(double2 *E, int N, int span, int rmin, hipStream_t st);
+// n2c32 > 0 (last stage of the mixed form): entries m >= n2c32 of an output row stored as FP32, channel
+// c = m / n2c32 in the first half of its FP64 slot.
 hipError_t wn_launch_contract(const double2 *in, double2 *out, const double2 *E, int P, int p0, int Q, int M, int nb,
-                              long long ib, long long iq, long long ob, long long op, hipStream_t st);
+                              long long ib, long long iq, long long ob, long long op, hipStream_t st, int n2c32 = 0);
 // Selected outputs of a contraction: group g = (b, t0, t1, -) writes out[t*ot + m] for t0 <= t < t1 from input
 // row b and phase row sel[t] (one mesh row's chosen points, written compactly).
 hipError_t wn_launch_contract_sel(const double2 *in, double2 *out, const double2 *E, const int4 *groups, int ngroups,
@@ -160,8 +162,10 @@ hipError_t wn_launch_screen(const double *e, const double *omega, int *count, in
 // otherwise, or when wn_rotate_sc_elements is false, every channel is rotated (wn_launch_rotate_f64).
 #define WN_PCAP 8
 bool wn_rotate_sc_elements(const WnShape &sh);
+// ch32: channels 1 .. C-1 stored in FP32 by the mixed form's contraction (requires wn_rotate_sc_elements)
 hipError_t wn_launch_rotate_sc(double *x, const double2 *u, const int *count, const int4 *pmeta, double2 *pwd,
-                               int nk, const WnShape &sh, hipStream_t st, double2 *scratch, int *flags);
+                               int nk, const WnShape &sh, hipStream_t st, double2 *scratch, int *flags,
+                               bool ch32 = false);
 // Generalized derivatives and SC/IC kernels of the screened pairs (pwd: nullptr to read every element from
 // the rotated channels).
 hipError_t wn_launch_pairs_f64(const double *x, const double *e, const int *count, const int4 *pmeta,

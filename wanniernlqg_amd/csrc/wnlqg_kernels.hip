@@ -748,11 +748,13 @@ hipError_t wn_launch_axis_phases(double2 *E, int N, int span, int rmin, hipStrea
 }
 
 // out[b*ob + p*op + m] = sum_{q<Q} E[(p0 + p)*Q + q] * in[b*ib + q*iq + m]   (complex, m < M, p < P)
+// n2c32 > 0: entries m >= n2c32 (channels 1 .. of a k row) are stored in FP32, channel c = m / n2c32 as
+// n2c32 float2 values in the first half of its FP64 slot (float2 index m + c n2c32 of the row).
 #define WN_QMAX 16
 #define WN_PCH 64
 __global__ __launch_bounds__(256) void k_contract(const double2 *__restrict__ in, double2 *__restrict__ out,
                                                  const double2 *__restrict__ E, int P, int p0, int Q, int M,
-                                                 long long ib, long long iq, long long ob, long long op) {
+                                                 long long ib, long long iq, long long ob, long long op, int n2c32) {
     extern __shared__ double2 es[];
     for (int e = threadIdx.x; e < P * Q; e += blockDim.x) es[e] = E[(size_t)p0 * Q + e];
     __syncthreads();
@@ -764,22 +766,27 @@ __global__ __launch_bounds__(256) void k_contract(const double2 *__restrict__ in
 #pragma unroll
     for (int q = 0; q < WN_QMAX; ++q) v[q] = q < Q ? src[q * iq] : make_double2(0.0, 0.0);
     double2 *dst = out + b * ob + m;
+    const bool f32 = n2c32 > 0 && m >= n2c32;
+    const long long m32 = f32 ? m + (long long)(m / n2c32) * n2c32 : 0;
     for (int p = 0; p < P; ++p) {
         const double2 *ep = es + p * Q;
         double2 acc = make_double2(0.0, 0.0);
 #pragma unroll
         for (int q = 0; q < WN_QMAX; ++q)
             if (q < Q) acc = cfma(ep[q], v[q], acc);
-        dst[p * op] = acc;
+        if (f32)
+            ((float2 *)(out + b * ob + p * op))[m32] = make_float2((float)acc.x, (float)acc.y);
+        else
+            dst[p * op] = acc;
     }
 }
 
 hipError_t wn_launch_contract(const double2 *in, double2 *out, const double2 *E, int P, int p0, int Q, int M, int nb,
-                              long long ib, long long iq, long long ob, long long op, hipStream_t st) {
+                              long long ib, long long iq, long long ob, long long op, hipStream_t st, int n2c32) {
     if (Q > WN_QMAX) return hipErrorInvalidValue;
     dim3 grid((M + 255) / 256, nb);
     hipLaunchKernelGGL(k_contract, grid, dim3(256), (size_t)P * Q * sizeof(double2), st, in, out, E, P, p0, Q, M, ib,
-                       iq, ob, op);
+                       iq, ob, op, n2c32);
     return hipGetLastError();
 }
 
@@ -1438,7 +1445,9 @@ __global__ __launch_bounds__(256) void k_rotate_mfma(double *__restrict__ x, con
 // pwd[k][pair][direction][15] (ddH channels 0..5, then dA 6..14) and not back to the channel.  Points with
 // more pairs rotate every channel.  Products as in k_rotate_mfma (three real MFMA products per complex
 // product).
-template <int NP, int PF>
+// C32: channels 1 .. C-1 arrive in FP32 (the first half of each channel's FP64 slot, n^2 float2 values; the
+// mixed form's mesh channels); the products are FP64 and every channel written back is FP64.
+template <int NP, int PF, bool C32>
 __global__ __launch_bounds__(256) void k_rotate_sc(double *__restrict__ x, const double2 *__restrict__ u,
                                                   const int *__restrict__ count, const int4 *__restrict__ pmeta,
                                                   double2 *__restrict__ pwd, int n, int C, int ldo,
@@ -1494,14 +1503,34 @@ __global__ __launch_bounds__(256) void k_rotate_sc(double *__restrict__ x, const
     }
     double2 *xk = (double2 *)(x + (size_t)k * ldo);
     const int lr = lane & 15, lq = lane >> 4;
+    // FP32 channels (n even): pairs of consecutive elements per 16-byte load when the tile splits into pairs
+    // per thread
+    constexpr bool C32P = PF % 2 == 0;
     double2 pf[PF];
     auto prefetch = [&](int c) {
         const double2 *xc = xk + (size_t)c * n * n;
+        const float2 *xc32 = (const float2 *)xc;
 #pragma unroll
         for (int q = 0; q < PF; ++q) {
             const int e = t + 256 * q;
             const int i = e / np, j = e - (e / np) * np;
-            pf[q] = (e < np * np && i < n && j < n) ? xc[i * n + j] : make_double2(0.0, 0.0);
+            if constexpr (C32 && C32P) {
+                // two consecutive elements (e0, e0 + 1 of the row-major tile) per 16-byte load; n is even, so
+                // every pair lies in one row and is 16-byte aligned
+                if ((q & 1) == 0) {
+                    const int e0 = 2 * t + 512 * (q >> 1);
+                    const int i0 = e0 / np, j0 = e0 - (e0 / np) * np;
+                    float4 v = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+                    if (e0 < np * np && i0 < n && j0 < n) v = *(const float4 *)(xc32 + i0 * n + j0);
+                    pf[q] = make_double2((double)v.x, (double)v.y);
+                    pf[q + 1] = make_double2((double)v.z, (double)v.w);
+                }
+            } else if constexpr (C32) {
+                const float2 v = (e < np * np && i < n && j < n) ? xc32[i * n + j] : make_float2(0.0f, 0.0f);
+                pf[q] = make_double2((double)v.x, (double)v.y);
+            } else {
+                pf[q] = (e < np * np && i < n && j < n) ? xc[i * n + j] : make_double2(0.0, 0.0);
+            }
         }
     };
     prefetch(1);
@@ -1511,7 +1540,7 @@ __global__ __launch_bounds__(256) void k_rotate_sc(double *__restrict__ x, const
         __syncthreads();
 #pragma unroll
         for (int q = 0; q < PF; ++q) {
-            const int e = t + 256 * q;
+            const int e = C32 && C32P ? 2 * t + 512 * (q >> 1) + (q & 1) : t + 256 * q;
             if (e < np * np) {
                 const int i = e / np, j = e - (e / np) * np;
                 Xr[i * ld + j] = pf[q].x;
@@ -1730,13 +1759,22 @@ static void launch_rotate_mfma(double *x, const double2 *u, int count, const WnS
 bool wn_rotate_sc_elements(const WnShape &sh) { return sh.n >= 16 && sh.n <= 64; }
 
 hipError_t wn_launch_rotate_sc(double *x, const double2 *u, const int *count, const int4 *pmeta, double2 *pwd,
-                               int nk, const WnShape &sh, hipStream_t st, double2 *scratch, int *flags) {
-    if (!wn_rotate_sc_elements(sh)) return wn_launch_rotate_f64(x, u, nk, sh, st, scratch);
+                               int nk, const WnShape &sh, hipStream_t st, double2 *scratch, int *flags, bool ch32) {
+    if (!wn_rotate_sc_elements(sh) || (ch32 && sh.n % 2 != 0)) {
+        if (ch32) return hipErrorInvalidValue;
+        return wn_launch_rotate_f64(x, u, nk, sh, st, scratch);
+    }
     const int np = (sh.n + 15) & ~15;
     const size_t lds = (size_t)4 * np * (np + 1) * sizeof(double);
-#define WN_ROTSC(NPV, PFV)                                                                                      \
-    hipLaunchKernelGGL((k_rotate_sc<NPV, PFV>), dim3(nk), dim3(256), lds, st, x, u, count, pmeta, pwd, sh.n, sh.C, \
-                       sh.ldo, flags)
+#define WN_ROTSC(NPV, PFV)                                                                                          \
+    do {                                                                                                            \
+        if (ch32)                                                                                                   \
+            hipLaunchKernelGGL((k_rotate_sc<NPV, PFV, true>), dim3(nk), dim3(256), lds, st, x, u, count, pmeta, pwd, \
+                               sh.n, sh.C, sh.ldo, flags);                                                          \
+        else                                                                                                        \
+            hipLaunchKernelGGL((k_rotate_sc<NPV, PFV, false>), dim3(nk), dim3(256), lds, st, x, u, count, pmeta,     \
+                               pwd, sh.n, sh.C, sh.ldo, flags);                                                     \
+    } while (0)
     if (np == 16)
         WN_ROTSC(16, 1);
     else if (np == 32)
