@@ -273,7 +273,7 @@ int ensure_rows(wnlqg_engine *g, int rows) {
     }
     if (g->family == 3) {
         WN_HIP(g->count.alloc((size_t)rows));
-        WN_HIP(g->der.alloc((size_t)rows * 12 * n2));   // 9 vertex derivative blocks + 3 r_eta blocks
+        WN_HIP(g->der.alloc((size_t)rows * WN_SHG_DER_BLOCKS * n2));   // 9 derivative blocks + 12 operand blocks
         WN_HIP(g->pairs.alloc((size_t)rows * n2));
         WN_HIP(g->pdiff.alloc((size_t)rows * n2));
         WN_HIP(g->coef.alloc((size_t)rows * n2 * 19 * g->sp.ncomp));
@@ -523,6 +523,9 @@ int32_t wnlqg_create(const wnlqg_model *model, const wnlqg_task *task, wnlqg_eng
         sp.mu = task->fermi_energy;
         sp.temperature = task->temperature;
         sp.tol = task->degeneracy_threshold;
+        // about 8k active pairs (n^2 / 2 per point) per accumulation chunk, so that large n still gives
+        // the contraction many more chunk tasks than the chip has SIMDs
+        g->kpc = std::max(1, std::min(16, (int)std::lround(16384.0 / ((double)n * n))));
     }
     g->sc.nw = g->family == 1 ? task->num_fermi_energies : task->num_photon_energies;
     g->sc.gaussian = task->broadening_type == WNLQG_GAUSSIAN;
@@ -706,7 +709,7 @@ int32_t wnlqg_create(const wnlqg_model *model, const wnlqg_task *task, wnlqg_eng
     // batch size: interpolation rows + pair buffers within ~24 GB unless the caller fixed it
     const size_t per_k = (size_t)g->sh.ldo * 8 +
                          (g->family == 0 ? n2 * (sizeof(int4) + sizeof(double2) + 27 * g->sc.nq * 16) : 0) +
-                         (g->family == 3 ? n2 * (12 * 16 + 8 + 8 + 19 * (size_t)g->sp.ncomp * 16) : 0) +
+                         (g->family == 3 ? n2 * (WN_SHG_DER_BLOCKS * 16 + 8 + 8 + 19 * (size_t)g->sp.ncomp * 16) : 0) +
                          n2 * 16 + (size_t)3 * g->sh.S * 16 +
                          (wn_eig_scratch_elems(n, 1) + wn_rotate_scratch_elems(n, 1)) * 16;
     long long b = task->batch_size > 0 ? task->batch_size : (long long)(24.0e9 / (double)per_k);

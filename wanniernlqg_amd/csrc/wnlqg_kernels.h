@@ -186,10 +186,11 @@ hipError_t wn_launch_accum(const int4 *chunks, int nchunks, const int *count, co
 hipError_t wn_launch_geom_f64(const double *x, const double *e, const double *mu, const int4 *chunks, int nchunks,
                               double2 *partial, int *flags, long long kbase, const WnShape &sh, const WnGeom &gp,
                               hipStream_t st);
-// SHG: optical position derivatives der[k][0..8][n][n] and regularized dipoles der[k][9..11][n][n]
-// (der holds 12 n^2 complex per k point), pair lists and coefficient rows for `count_k` k
+// SHG: optical position derivatives der[k][0..8][n][n] and the operands of their commutator products
+// (der holds WN_SHG_DER_BLOCKS n^2 complex per k point), pair lists and coefficient rows for `count_k` k
 // points, then the frequency contraction of `nchunks` chunks into partial[chunk] = [19 nc][nw] complex
 // followed by [nc] (one-band BCD, semiclassical) sums.
+#define WN_SHG_DER_BLOCKS 21
 hipError_t wn_launch_shg(const double *x, const double *e, const double *omega, const int4 *chunks, int nchunks,
                          int count_k, double2 *der, int *count, int2 *pairs, double *pdiff, double2 *coef,
                          double2 *one, double2 *partial, const WnShape &sh, const WnShg &sp, hipStream_t st);

@@ -11,9 +11,9 @@
 //                 resonances d1 = R(dE - w), d2 = R(dE - 2w), dn = R(-dE - w), R(x) = PV(x) + i pi delta(x),
 //                 and of the k-independent low-frequency factor; per band the one-band (Fermi-surface)
 //                 BCD and semiclassical sums at T > 0;
-//   k_shg_mfma  : sum over the chunk's k points and pairs of resonance x coefficient for every
-//                 frequency (lane = frequency), in a fixed order, as a real FP64 MFMA GEMM whose A
-//                 operand (the resonances) is generated in registers.
+//   k_shg_acc   : sum over the chunk's k points and pairs of resonance x coefficient for every
+//                 frequency, in a fixed order, as real FP64 MFMA GEMMs whose A operand (the resonances)
+//                 is generated in registers; k_shg_low: the frequency-independent block and one-band sums.
 // The remaining factors depend on the frequency only (low = PV(w, w_low), the one-band kernel
 // low - i delta(w, w_low)) and are applied to the lane sums on the host, which is exact because the
 // k sum is linear.
@@ -35,8 +35,6 @@
 
 #define WN_SHG_TPB 256
 #define WN_SHG_NMAX 128   // orbitals (<= WN_SHG_TPB: one thread per band in the per-band phases)
-// per-k blocks of the vertex buffer: der[3a+b] (9), then the regularized dipoles r_eta[a] (3)
-#define WN_SHG_DER_BLOCKS 12
 
 namespace {
 
@@ -105,75 +103,176 @@ __device__ inline bool shg_active(const double *occ, int i, int j) { return i !=
 }  // namespace
 
 // ------------------------------------------------------------------------------------------------
-// der[k][3a+b][n][m] (OpticalPositionDerivatives.jl:13-130) for active pairs, 0 elsewhere.
-// Grid (count, 9): block (k, ab).
+// Optical position derivative (OpticalPositionDerivatives.jl:13-130) as commutator GEMMs.
+// Per k point the sums over the intermediate band p of the reference are matrix products:
+//   F_ab = [A_a, R_b]                                     (sum_a of the reference, before its diagonal term)
+//   G_ab = i [V_a, R_b] + [A'_a, X_b] - [Q'_a, Y_b]        (i sum_h + the eta-correction numerator)
+// with R_b = V_b o r(E_j - E_i) the regularized vertex, X_b = w o V_b, Y_b = w o A_b (w_ij = eta^2 /
+// ((E_i - E_j)^2 + eta^2)), Q'_a = V_a + i (E_i - E_j) o A_a, and A'_a = A_a; X, Y, Q', A' with a zero
+// diagonal, which removes exactly the reference's p = n and p = m terms of the eta correction (each of
+// its four products has the index n on one factor's diagonal and m on the other's).  At T = 0 the
+// active pairs are (occupied, empty) and (empty, occupied), and the occupied bands are a prefix of the
+// ascending spectrum: only the two off-diagonal blocks rows [0, n_occ) x columns [n_occ, n) and rows
+// [n_occ, n) x columns [0, n_occ) are computed (no Hermitian symmetry is assumed: the channels of a
+// model whose H(-R) is not H(R)^+ are not Hermitian, and the reference takes them as they are).  At T > 0
+// every element is computed.
+// Per k point the derivative buffer holds WN_SHG_DER_BLOCKS n^2 blocks: der[3a+b] (9), R_b (3), X_b (3),
+// Y_b (3), Q'_a (3).
 // ------------------------------------------------------------------------------------------------
-__global__ __launch_bounds__(WN_SHG_TPB) void k_shg_der(const double *__restrict__ x, const double *__restrict__ eg,
-                                                       double2 *__restrict__ der, WnShape sh, WnShg sp) {
-    __shared__ double occ[WN_SHG_NMAX];
+namespace {
+constexpr int kShgR = 9, kShgX = 12, kShgY = 15, kShgQ = 18;   // first block of each operand family
+
+// element (i, j) of the T = 0 block geometry: occupied bands (E <= mu) of one k point
+__device__ inline int shg_occupied(const double *E, int n, double mu) {
+    int c = 0;
+    for (int i = 0; i < n; ++i) c += E[i] <= mu;
+    return c;
+}
+
+// largest tile count (16 x 16 tiles) of one derivative matrix: both off-diagonal blocks at T = 0, the
+// whole matrix at T > 0
+int shg_vtx_tiles(int n, bool block) {
+    const int t = (n + 15) / 16;
+    if (!block) return t * t;
+    int best = 0;
+    for (int no = 1; no < n; ++no) best = std::max(best, 2 * ((no + 15) / 16) * ((n - no + 15) / 16));
+    return std::max(best, 1);
+}
+}  // namespace
+
+// Derived operands R_b, X_b, Y_b, Q'_a of one k point.  Grid (count, 12): block (k, operand).
+__global__ __launch_bounds__(WN_SHG_TPB) void k_shg_prep(const double *__restrict__ x, const double *__restrict__ eg,
+                                                        double2 *__restrict__ der, WnShape sh, WnShg sp) {
     const long long k = blockIdx.x;
-    const int ab = blockIdx.y, a = ab / 3, b = ab - (ab / 3) * 3;
+    const int op = blockIdx.y, fam = op / 3, d = op - (op / 3) * 3;
     const int n = sh.n, n2 = n * n;
     const ShgView s = make_view(x, eg, k, sh, sp.eta);
-    const int t = threadIdx.x;
-    if (ab >= 9) {
-        // r_eta[a'] = A + i V reg(E_j - E_i) of every element, read by the pair coefficients' band sums
-        double2 *rd = der + ((size_t)k * WN_SHG_DER_BLOCKS + ab) * n2;
-        for (int e = t; e < n2; e += WN_SHG_TPB) rd[e] = sv_rdip(s, ab - 9, e / n, e - (e / n) * n);
-        return;
+    double2 *out = der + ((size_t)k * WN_SHG_DER_BLOCKS + kShgR + op) * n2;
+    for (int e = threadIdx.x; e < n2; e += WN_SHG_TPB) {
+        const int i = e / n, j = e - (e / n) * n;
+        double2 v;
+        if (fam == 0) {
+            v = sv_reg(s, d, i, j);
+        } else if (i == j) {
+            v = make_double2(0.0, 0.0);
+        } else if (fam == 3) {
+            v = cadd(sv_V(s, d, i, j), cmuli(cscale(sv_A(s, d, i, j), s.E[i] - s.E[j])));
+        } else {
+            const double w = shg_weight(s.E[i] - s.E[j], sp.eta);
+            v = cscale(fam == 1 ? sv_V(s, d, i, j) : sv_A(s, d, i, j), w);
+        }
+        out[e] = v;
     }
-    if (t < n) occ[t] = shg_occupation(s.E[t], sp.mu, sp.temperature);
-    __syncthreads();
-    double2 *out = der + ((size_t)k * WN_SHG_DER_BLOCKS + ab) * n2;
-    const bool pos = s.pos != 0;
-    for (int e = t; e < n2; e += WN_SHG_TPB) {
-        const int nn = e / n, m = e - (e / n) * n;
-        if (!shg_active(occ, nn, m)) {
-            out[e] = make_double2(0.0, 0.0);
-            continue;
+}
+
+// One 16 x 16 tile of F_ab and G_ab per wave on v_mfma_f64_16x16x4f64, complex products as three real
+// products (t1 = Lr Rr, t2 = Li Ri, t3 = (Lr + Li)(Rr + Ri); C = (t1 - t2, t3 - t1 - t2)), then the
+// derivative of every element of the tile.  Lane l holds the left operand (row l%16, band p0 + l/16) and
+// the right operand (band p0 + l/16, column l%16).  Tasks of one k point run on one XCD (workgroup b on
+// XCD b % 8), so that its operands are read from that XCD's L2.
+__global__ __launch_bounds__(64) void k_shg_vtx(const double *__restrict__ x, const double *__restrict__ eg,
+                                               double2 *__restrict__ der, int count_k, int tiles, WnShape sh,
+                                               WnShg sp) {
+    const int tpk = 9 * tiles;
+    const int b = blockIdx.x, xcd = b & 7, s8 = b >> 3;
+    const long long k = (long long)(s8 / tpk) * 8 + xcd;
+    if (k >= count_k) return;
+    const int task = s8 - (s8 / tpk) * tpk;
+    const int ab = task / tiles, tile = task - (task / tiles) * tiles;
+    const int a = ab / 3, bb = ab - (ab / 3) * 3;
+    const int n = sh.n, n2 = n * n;
+    const ShgView s = make_view(x, eg, k, sh, sp.eta);
+    int r_lo = 0, r_hi = n, c_lo = 0, c_hi = n, t = tile;
+    if (sp.temperature <= 1.0e-7) {
+        // block (occupied rows, empty columns), then block (empty rows, occupied columns)
+        const int no = shg_occupied(s.E, n, sp.mu);
+        const int t1 = ((no + 15) / 16) * ((n - no + 15) / 16);
+        if (t < t1) {
+            r_hi = no;
+            c_lo = no;
+        } else {
+            t -= t1;
+            r_lo = no;
+            c_hi = no;
         }
-        // commutators [A_a, reg_b] and [V_a, reg_b]
-        double2 ca = make_double2(0.0, 0.0), ch = make_double2(0.0, 0.0);
-        double2 eta_num = make_double2(0.0, 0.0);
-        for (int p = 0; p < n; ++p) {
-            const double2 rpm = sv_reg(s, b, p, m), rnp = sv_reg(s, b, nn, p);
-            if (pos) ca = csub(cadd(ca, cmul(sv_A(s, a, nn, p), rpm)), cmul(rnp, sv_A(s, a, p, m)));
-            ch = csub(cadd(ch, cmul(sv_V(s, a, nn, p), rpm)), cmul(rnp, sv_V(s, a, p, m)));
-            if (sp.eta_correction && p != nn && p != m && pos) {
-                const double wpm = shg_weight(s.E[p] - s.E[m], sp.eta);
-                const double wnp = shg_weight(s.E[nn] - s.E[p], sp.eta);
-                const double2 Aanp = sv_A(s, a, nn, p), Vbpm = sv_V(s, b, p, m), Abpm = sv_A(s, b, p, m);
-                const double2 Vanp = sv_V(s, a, nn, p);
-                // V_a,np + i (E_n - E_p) A_a,np
-                const double2 van = cadd(Vanp, cmuli(cscale(Aanp, s.E[nn] - s.E[p])));
-                eta_num = cadd(eta_num, cscale(csub(cmul(Aanp, Vbpm), cmul(van, Abpm)), wpm));
-                const double2 Vbnp = sv_V(s, b, nn, p), Aapm = sv_A(s, a, p, m), Abnp = sv_A(s, b, nn, p);
-                const double2 Vapm = sv_V(s, a, p, m);
-                const double2 vap = cadd(Vapm, cmuli(cscale(Aapm, s.E[p] - s.E[m])));
-                eta_num = csub(eta_num, cscale(csub(cmul(Vbnp, Aapm), cmul(Abnp, vap)), wnp));
-            }  // without a position operator every eta-correction product carries an A factor
+    }
+    const int ntc = (c_hi - c_lo + 15) / 16, ntr = (r_hi - r_lo + 15) / 16;
+    if (t >= ntr * ntc) return;
+    const int ti = t / ntc, tj = t - (t / ntc) * ntc;
+    const int r0 = r_lo + 16 * ti, c0 = c_lo + 16 * tj;
+    const int lane = threadIdx.x, lr = lane & 15, lk = lane >> 4;
+    const int row = min(r0 + lr, n - 1), col = min(c0 + lr, n - 1);   // operand element (clamped in bounds)
+    const double2 *dk = der + (size_t)k * WN_SHG_DER_BLOCKS * n2;
+    const bool pos = s.pos != 0, eta = pos && sp.eta_correction;
+    f64x4 f1 = {0.0, 0.0, 0.0, 0.0}, f2 = f1, f3 = f1, g1 = f1, g2 = f1, g3 = f1;
+    // products L R: (left matrix, its factor, zero-diagonal), (right matrix, factor, zero-diagonal);
+    // factors 1, -1, i, -i as (re, im)
+    for (int blk = 0; blk < 8; ++blk) {
+        if (blk < 2 && !pos) continue;
+        if (blk >= 4 && !eta) continue;
+        const double2 *L, *R;
+        double2 fl = make_double2(1.0, 0.0), fr = make_double2(1.0, 0.0);
+        bool zl = false, zr = false;
+        const double2 *Aa = s.A(a), *Va = s.V(a), *Rb = dk + (size_t)(kShgR + bb) * n2;
+        switch (blk) {
+            case 0: L = Aa; R = Rb; break;                                          // F: A_a R_b
+            case 1: L = Rb; R = Aa; fr = make_double2(-1.0, 0.0); break;            //    - R_b A_a
+            case 2: L = Va; R = Rb; fl = make_double2(0.0, 1.0); break;             // G: i V_a R_b
+            case 3: L = Rb; R = Va; fr = make_double2(0.0, -1.0); break;            //    - i R_b V_a
+            case 4: L = Aa; zl = true; R = dk + (size_t)(kShgX + bb) * n2; break;   //    A'_a X_b
+            case 5: L = dk + (size_t)(kShgX + bb) * n2; R = Aa; zr = true; fr = make_double2(-1.0, 0.0); break;
+            case 6: L = dk + (size_t)(kShgQ + a) * n2; R = dk + (size_t)(kShgY + bb) * n2;
+                    fl = make_double2(-1.0, 0.0); break;                            //    - Q'_a Y_b
+            default: L = dk + (size_t)(kShgY + bb) * n2; R = dk + (size_t)(kShgQ + a) * n2; break;   // Y_b Q'_a
         }
+        for (int p0 = 0; p0 < n; p0 += 4) {
+            const int p = p0 + lk, pc = min(p, n - 1);
+            double2 lv = L[row * n + pc], rv = R[pc * n + col];
+            const bool lz = p >= n || (zl && p == row), rz = p >= n || (zr && p == col);
+            lv = lz ? make_double2(0.0, 0.0) : make_double2(lv.x * fl.x - lv.y * fl.y, lv.x * fl.y + lv.y * fl.x);
+            rv = rz ? make_double2(0.0, 0.0) : make_double2(rv.x * fr.x - rv.y * fr.y, rv.x * fr.y + rv.y * fr.x);
+            if (blk < 2) {
+                f1 = __builtin_amdgcn_mfma_f64_16x16x4f64(lv.x, rv.x, f1, 0, 0, 0);
+                f2 = __builtin_amdgcn_mfma_f64_16x16x4f64(lv.y, rv.y, f2, 0, 0, 0);
+                f3 = __builtin_amdgcn_mfma_f64_16x16x4f64(lv.x + lv.y, rv.x + rv.y, f3, 0, 0, 0);
+            } else {
+                g1 = __builtin_amdgcn_mfma_f64_16x16x4f64(lv.x, rv.x, g1, 0, 0, 0);
+                g2 = __builtin_amdgcn_mfma_f64_16x16x4f64(lv.y, rv.y, g2, 0, 0, 0);
+                g3 = __builtin_amdgcn_mfma_f64_16x16x4f64(lv.x + lv.y, rv.x + rv.y, g3, 0, 0, 0);
+            }
+        }
+    }
+    // element (nn, m) = (r0 + lk + 4q, c0 + lr) of the tile
+    const int m = c0 + lr;
+    double2 *outb = der + ((size_t)k * WN_SHG_DER_BLOCKS + ab) * n2;
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+        const int nn = r0 + lk + 4 * q;
+        if (nn >= r_hi || m >= c_hi) continue;
+        const double2 ca = make_double2(f1[q] - f2[q], f3[q] - f1[q] - f2[q]);
+        const double2 gg = make_double2(g1[q] - g2[q], g3[q] - g1[q] - g2[q]);
         const double2 Aan = sv_A(s, a, nn, nn), Aam = sv_A(s, a, m, m);
-        const double2 Abn = sv_A(s, b, nn, nn), Abm = sv_A(s, b, m, m);
+        const double2 Abn = sv_A(s, bb, nn, nn), Abm = sv_A(s, bb, m, m);
         const double2 Van = sv_V(s, a, nn, nn), Vam = sv_V(s, a, m, m);
-        const double2 Vbn = sv_V(s, b, nn, nn), Vbm = sv_V(s, b, m, m);
-        const double2 reg_b = sv_reg(s, b, nn, m);
-        const double2 bare_a = sv_bare(s, a, nn, m), bare_b = sv_bare(s, b, nn, m);
+        const double2 Vbn = sv_V(s, bb, nn, nn), Vbm = sv_V(s, bb, m, m);
+        const double2 reg_b = dk[(size_t)(kShgR + bb) * n2 + nn * n + m];
+        const double2 bare_a = sv_bare(s, a, nn, m), bare_b = sv_bare(s, bb, nn, m);
         const double2 dAa = csub(Aan, Aam), dAb = csub(Abn, Abm);
         const double2 sum_a = csub(ca, cmul(dAa, reg_b));
-        const double2 sum_h = csub(ch, cmul(csub(Van, Vam), reg_b));
-        const double2 dAab = pos ? s.dA(3 * a + b)[nn * n + m] : make_double2(0.0, 0.0);
+        const double2 dAab = pos ? s.dA(3 * a + bb)[nn * n + m] : make_double2(0.0, 0.0);
         double2 finite = cadd(dAab, sum_a);
         finite = cadd(finite, cmul(dAa, bare_b));
         finite = cadd(finite, cmul(dAb, bare_a));
         finite = csub(finite, cmuli(cmul(sv_A(s, a, nn, m), dAb)));
-        const double2 Wab = s.W(wn_sym(a, b))[nn * n + m];
-        double2 inner = cadd(Wab, sum_h);
+        // numerator = i (W_ab + [V_a, R_b] - dV_a reg_b + bare_a dV_b + bare_b dV_a) + eta numerator
+        const double2 Wab = s.W(wn_sym(a, bb))[nn * n + m];
+        double2 inner = csub(Wab, cmul(csub(Van, Vam), reg_b));
         inner = cadd(inner, cscale(bare_a, Vbn.x - Vbm.x));
         inner = cadd(inner, cscale(bare_b, Van.x - Vam.x));
-        double2 numerator = cadd(cmuli(inner), eta_num);
+        const double2 numerator = cadd(cmuli(inner), gg);
         const double iv = shg_bare_inverse(s.E[m] - s.E[nn]);
-        out[e] = iv == 0.0 ? finite : cadd(finite, cscale(numerator, iv));
+        const double2 v = iv == 0.0 ? finite : cadd(finite, cscale(numerator, iv));
+        outb[nn * n + m] = v;
     }
 }
 
@@ -195,7 +294,7 @@ __global__ __launch_bounds__(WN_SHG_TPB) void k_shg_coef(const double *__restric
     const int ncol = 19 * nc;
     const ShgView s = make_view(x, eg, k, sh, sp.eta);
     const double2 *dk = der + (size_t)k * WN_SHG_DER_BLOCKS * n2;
-    const double2 *rdk = dk + (size_t)9 * n2;   // r_eta[a][i][j]
+    const double2 *rk = dk + (size_t)kShgR * n2;   // R[d][i][j]; r_eta = A + i R
     const int t = threadIdx.x;
     if (t < n) occ[t] = shg_occupation(s.E[t], sp.mu, sp.temperature);
     __syncthreads();
@@ -225,6 +324,7 @@ __global__ __launch_bounds__(WN_SHG_TPB) void k_shg_coef(const double *__restric
     __syncthreads();
     const int np = rowoff[n];
     auto D = [&](int aa, int bb, int i, int j) { return dk[(size_t)(3 * aa + bb) * n2 + i * n + j]; };
+    auto RE = [&](int d, int i, int j) { return cadd(sv_A(s, d, i, j), cmuli(rk[(size_t)d * n2 + i * n + j])); };
     for (int item = t; item < np * nc; item += WN_SHG_TPB) {
         const int p = item / nc, ci = item - (item / nc) * nc;
         const int2 pr = pairs[pbase + p];
@@ -249,15 +349,15 @@ __global__ __launch_bounds__(WN_SHG_TPB) void k_shg_coef(const double *__restric
         double2 tf = make_double2(0.0, 0.0), to = make_double2(0.0, 0.0);
         for (int q = 0; q < n; ++q) {
             if (q == nn || q == m) continue;
-            const double2 rb_mq = rdk[(size_t)b * n2 + m * n + q], rc_qn = rdk[(size_t)c * n2 + q * n + nn];
-            const double2 rc_mq = rdk[(size_t)c * n2 + m * n + q], rb_qn = rdk[(size_t)b * n2 + q * n + nn];
+            const double2 rb_mq = RE(b, m, q), rc_qn = RE(c, q, nn);
+            const double2 rc_mq = RE(c, m, q), rb_qn = RE(b, q, nn);
             const double pv0 = shg_principal(2.0 * s.E[q] - s.E[nn] - s.E[m], sp.eta);
             tf = cadd(tf, cscale(cmul(dip_a_nm, cadd(cmul(rb_mq, rc_qn), cmul(rc_mq, rb_qn))), pv0));
             const double pv1 = shg_principal(2.0 * s.E[nn] - s.E[m] - s.E[q], sp.eta);
-            const double2 ra_mq = rdk[(size_t)a * n2 + m * n + q];
+            const double2 ra_mq = RE(a, m, q);
             to = cadd(to, cscale(cmul(ra_mq, cadd(cmul(rb_qn, dip_c_nm), cmul(rc_qn, dip_b_nm))), pv1));
             const double pv2 = shg_principal(2.0 * s.E[m] - s.E[q] - s.E[nn], sp.eta);
-            const double2 ra_qn = rdk[(size_t)a * n2 + q * n + nn];
+            const double2 ra_qn = RE(a, q, nn);
             to = cadd(to, cscale(cmul(ra_qn, cadd(cmul(dip_b_nm, rc_mq), cmul(dip_c_nm, rb_mq))), pv2));
         }
         // f = i o / 2;  f * inv * z = i (o inv / 2) z
@@ -351,183 +451,197 @@ __global__ __launch_bounds__(WN_SHG_TPB) void k_shg_coef(const double *__restric
 }
 
 // ------------------------------------------------------------------------------------------------
-// Frequency contraction on FP64 MFMA.  For one chunk and one resonance kind (d1, d2, dn) the
-// partial sums form a complex GEMM out[w][c] = sum_p R(dE_p, w) coef[p][c], written as a real GEMM
-// with K = 2 x pairs:  A[w][2p] = Re R, A[w][2p+1] = Im R;  B[2p][2c] = Re C, B[2p][2c+1] = Im C,
-// B[2p+1][2c] = -Im C, B[2p+1][2c+1] = Re C.  v_mfma_f64_16x16x4f64 lane l holds A[l%16][l/16] (the
-// resonance of frequency l%16 and pair l/32 of the step, part (l/16)%2, generated in registers) and
-// B[l/16][l%16] (loaded from the coefficient rows).  Workgroup (chunk, 64-frequency tile, kind);
-// wave w owns 16 frequencies and every 16-column tile of its kind's block.  Kind 3 is the
-// frequency-independent low block: plain sums over the chunk's pairs in pair order.  Coefficient rows
-// are staged through LDS in blocks of `pb` pairs (all k points of the chunk in order).
+// Frequency contraction on FP64 MFMA.  For one chunk the partial sums of the three resonance kinds
+// d1 = R(dE - w), d2 = R(dE - 2w), dn = R(-dE - w) form complex GEMMs out[w][c] = sum_p R_kind(dE_p, w)
+// coef[p][c] over the kind's column block, written as real GEMMs with K = 2 x pairs.  Four pairs make
+// two K steps of v_mfma_f64_16x16x4f64: lane l evaluates the complex resonance of frequency l%16 and
+// pair l/16 of the group once and passes Re R as its A element of the first step and Im R in the
+// second; its B elements are (Cr, Ci) of the pair's coefficient in the first step and (-Ci, Cr) in the
+// second (real column 2c = Re, 2c+1 = Im of complex column c), both from one 16-byte load:
+//   Re out = sum Re R Cr - Im R Ci,   Im out = sum Re R Ci + Im R Cr.
+// A column tile is 16 real columns (8 complex) of one kind; the kinds' tiles are numbered d1 (nc tiles),
+// d2 (ceil(6 nc / 8)), dn (ceil(2 nc / 8)).  One wave = one task: RTW row tiles of 16 frequencies times
+// up to MAXT consecutive column tiles, over all pairs of its chunk in pair order (k points in order).
+// Tasks of one chunk run on one XCD (workgroup b on XCD b % 8) so that its coefficient rows are read
+// from that XCD's L2.  The frequency-independent low block and the one-band sums go to k_shg_low.
 // ------------------------------------------------------------------------------------------------
-#define WN_SHG_MAXK 64   // k points per chunk
-template <int MAXT, int RTW>
-__global__ __launch_bounds__(WN_SHG_TPB, MAXT == 4 ? 3 : 2) void k_shg_mfma(const int4 *__restrict__ chunks, const int *__restrict__ count,
-                                                        const double *__restrict__ pdiff,
-                                                        const double2 *__restrict__ coef,
-                                                        const double2 *__restrict__ one,
-                                                        const double *__restrict__ omega, double2 *__restrict__ partial,
-                                                        int pcap, int nwt, int pb, WnShg sp) {
-    extern __shared__ double lds_b[];          // [pb][2][2 cwid] coefficient block in the real B layout
-    __shared__ double lds_diff[128];
-    __shared__ int lds_row[128];
-    __shared__ int pre[WN_SHG_MAXK + 1];       // prefix of the pair counts of the chunk's k points
-    const int nc = sp.ncomp, ncol = 19 * nc, nw = sp.nw;
-    const int per_chunk = nwt * 4;
-    const int chunk = blockIdx.x / per_chunk;
-    const int rem = blockIdx.x - chunk * per_chunk;
-    const int wt = rem >> 2, kind = rem & 3;
-    const int t = threadIdx.x;
-    const int wave = t >> 6, lane = t & 63;
-    const int4 ch = chunks[chunk];
-    double2 *o = partial + (size_t)chunk * ((size_t)ncol * nw + nc);
-    if (kind == 3) {
-        // low block (R = 1) and the one-band sums, once per chunk: thread t sums the pairs t, t + 256,
-        // ... of every column, then the 256 partial sums are combined in a fixed tree order
-        if (wt != 0) return;
-        __shared__ double2 red[WN_SHG_TPB];
-        if (t == 0) {
-            pre[0] = 0;
-            for (int i = 0; i < ch.y; ++i) pre[i + 1] = pre[i] + count[ch.x + i];
+namespace {
+
+__device__ __forceinline__ double2 shg_resonance(double x, double w2, double iw, double gnorm, double cut,
+                                                 int gaussian) {
+    // R(x) = PV(x) + i pi delta(x) (SecondHarmonicKernels.jl:86-98): PV = x / (x^2 + w^2); Gaussian
+    // pi delta = sqrt(pi) exp(-(x/w)^2) / w (0 beyond 27.3 w, where exp underflows in FP64), Lorentzian
+    // pi delta = w / (x^2 + w^2)
+    const double pv = x * wn_rcp(x * x + w2);
+    double d = 0.0;
+    if (gaussian) {
+        if (fabs(x) < cut) {
+            const double y = x * iw;
+            d = gnorm * exp(-y * y);
         }
-        __syncthreads();
-        const int tot = pre[ch.y];
-        for (int c = 0; c < 3 * nc; ++c) {
-            double2 acc = make_double2(0.0, 0.0);
-            int ki = 0;
-            for (int f = t; f < tot; f += WN_SHG_TPB) {
-                while (pre[ki + 1] <= f) ++ki;
-                acc = cadd(acc, coef[((size_t)(ch.x + ki) * pcap + (f - pre[ki])) * ncol + 16 * nc + c]);
-            }
-            red[t] = acc;
-            __syncthreads();
-            for (int h = WN_SHG_TPB / 2; h > 0; h >>= 1) {
-                if (t < h) red[t] = cadd(red[t], red[t + h]);
-                __syncthreads();
-            }
-            const double2 sum = red[0];
-            for (int w = t; w < nw; w += WN_SHG_TPB) o[(size_t)(16 * nc + c) * nw + w] = sum;
-            __syncthreads();
-        }
-        if (t < nc) {
-            double sb = 0.0, ss = 0.0;
-            for (int k = ch.x; k < ch.x + ch.y; ++k) {
-                const double2 v = one[(size_t)k * nc + t];
-                sb += v.x;
-                ss += v.y;
-            }
-            o[(size_t)ncol * nw + t] = make_double2(sb, ss);
-        }
-        return;
+    } else {
+        const double y = x * iw;
+        d = iw * wn_rcp(1.0 + y * y);
     }
-    const int cbeg = kind == 0 ? 0 : (kind == 1 ? 8 * nc : 14 * nc);
-    const int cwid = kind == 0 ? 8 * nc : (kind == 1 ? 6 * nc : 2 * nc);   // complex columns
-    const int ntile = (2 * cwid + 15) / 16;
-    const int lr = lane & 15, lk = lane >> 4;           // A row (frequency) / K row of this lane
-    // row tiles (16 frequencies each) of this wave: wt * 4 RTW + wave RTW + r
-    const int rt0 = (wt * 4 + wave) * RTW;
+    return make_double2(pv, d);
+}
+
+struct ShgTileSet {
+    int t0, t1, t2;   // tiles of d1, d2, dn
+    __device__ __host__ static ShgTileSet of(int nc) { return {nc, (6 * nc + 7) / 8, (2 * nc + 7) / 8}; }
+    __device__ __host__ int total() const { return t0 + t1 + t2; }
+};
+
+}  // namespace
+
+template <int RTW, int N0, int N1, int N2>
+__global__ __launch_bounds__(64, 2) void k_shg_acc(const int4 *__restrict__ chunks, int nchunks,
+                                                const int *__restrict__ count, const double *__restrict__ pdiff,
+                                                const double2 *__restrict__ coef, const double *__restrict__ omega,
+                                                double2 *__restrict__ partial, int pcap, int nrg, int ncg, WnShg sp) {
+    constexpr int NT = N0 + N1 + N2;
+    const int ntask = nrg * ncg;
+    const int b = blockIdx.x, xcd = b & 7, s = b >> 3;
+    const int chunk = (s / ntask) * 8 + xcd;
+    if (chunk >= nchunks) return;
+    const int task = s - (s / ntask) * ntask;
+    const int rg = task / ncg, cg = task - (task / ncg) * ncg;
+    const int lane = threadIdx.x, lr = lane & 15, lg = lane >> 4, part = lr & 1;
+    const int nc = sp.ncomp, ncol = 19 * nc, nw = sp.nw;
+    // this lane's complex column of every tile of the task (B operand and output): tiles 0..N0-1 of kind
+    // d1 from tile cg N0 of that kind, then N1 of d2 from cg N1, then N2 of dn from cg N2
+    int col[NT];
+    bool cv[NT];
+#pragma unroll
+    for (int j = 0; j < NT; ++j) {
+        const int kind = j < N0 ? 0 : (j < N0 + N1 ? 1 : 2);
+        const int tt = kind == 0 ? cg * N0 + j : (kind == 1 ? cg * N1 + j - N0 : cg * N2 + j - N0 - N1);
+        const int cbeg = kind == 0 ? 0 : (kind == 1 ? 8 * nc : 14 * nc);
+        const int cw = kind == 0 ? 8 * nc : (kind == 1 ? 6 * nc : 2 * nc);
+        const int cc = tt * 8 + (lr >> 1);
+        cv[j] = cc < cw;
+        col[j] = cv[j] ? cbeg + cc : 0;
+    }
     double om[RTW];
+    bool wv[RTW];
 #pragma unroll
     for (int r = 0; r < RTW; ++r) {
-        const int w_a = (rt0 + r) * 16 + lr;            // frequency of this lane's A element in tile r
-        om[r] = w_a < nw ? omega[w_a] : 0.0;
+        const int w = (rg * RTW + r) * 16 + lr;
+        wv[r] = w < nw;
+        om[r] = wv[r] ? omega[w] : 0.0;
     }
-    const int pair_sel = lk >> 1, part = lk & 1;        // pair of the step and re / im row
-    if (t == 0) {
-        pre[0] = 0;
-        for (int i = 0; i < ch.y; ++i) pre[i + 1] = pre[i] + count[ch.x + i];
-    }
-    __syncthreads();
-    const int total = pre[ch.y];
-    f64x4 acc[RTW][MAXT];
+    f64x4 acc[RTW][NT];
 #pragma unroll
     for (int r = 0; r < RTW; ++r)
 #pragma unroll
-        for (int j = 0; j < MAXT; ++j) acc[r][j] = (f64x4){0.0, 0.0, 0.0, 0.0};
-    const double cut = 27.3 * sp.width;                 // exp(-(x/w)^2) is 0 in FP64 beyond this
+        for (int j = 0; j < NT; ++j) acc[r][j] = (f64x4){0.0, 0.0, 0.0, 0.0};
+    const double cut = 27.3 * sp.width;
     const double w2 = sp.width * sp.width, iw = 1.0 / sp.width;
-    const double gnorm = sqrt(M_PI) * iw;               // pi / (sqrt(pi) w)
-    for (int f0 = 0; f0 < total; f0 += pb) {
-        const int nb = min(pb, total - f0);
-        // cooperative load of the block's coefficient rows and energy differences (pairs in chunk order);
-        // the row of every pair of the block is located once
-        for (int e = t; e < nb; e += WN_SHG_TPB) {
-            const int f = f0 + e;
-            int ki = 0;
-            while (pre[ki + 1] <= f) ++ki;
-            const int row = (ch.x + ki) * pcap + (f - pre[ki]);
-            lds_row[e] = row;
-            lds_diff[e] = pdiff[row];
-        }
-        __syncthreads();
-        // pair fp, coefficient c = (cr, ci): B rows 2fp = (cr, ci), 2fp+1 = (-ci, cr)
-        const int bw = 2 * cwid;
-        for (int e = t; e < nb * cwid; e += WN_SHG_TPB) {
-            const int fp = e / cwid, c = e - (e / cwid) * cwid;
-            const double2 cv = coef[(size_t)lds_row[fp] * ncol + cbeg + c];
-            double *b0 = lds_b + (size_t)(2 * fp) * bw + 2 * c;
-            b0[0] = cv.x;
-            b0[1] = cv.y;
-            b0[bw] = -cv.y;
-            b0[bw + 1] = cv.x;
-        }
-        __syncthreads();
-#pragma unroll 2
-        for (int s0 = 0; s0 < nb; s0 += 2) {
-            const int pl = s0 + pair_sel;
-            const bool valid = pl < nb;
-            // A: resonances R(x) = PV(x) + i pi delta(x) (SecondHarmonicKernels.jl:86-98) of (frequency of row
-            // tile r, pair pl); part 0 of the lane carries Re R, part 1 Im R
-            const double diff = valid ? lds_diff[pl] : 0.0;
-            double a[RTW];
+    const double gnorm = sqrt(M_PI) * iw;
+    const int4 ch = chunks[chunk];
+    const int kend = ch.x + ch.y;
+    // groups of four pairs in chunk order
+    for (int k = ch.x; k < kend; ++k) {
+        const int cnt = count[k];
+        for (int f0 = 0; f0 < cnt; f0 += 4) {
+            const bool pv = f0 + lg < cnt;
+            // a lane past the last pair reads the group's first pair (finite) and contributes R = 0
+            const size_t row = (size_t)k * pcap + f0 + (pv ? lg : 0);
+            const double dE = pdiff[row];
+            double2 bv[NT];
+#pragma unroll
+            for (int j = 0; j < NT; ++j) bv[j] = coef[row * ncol + col[j]];
+            double2 R0[RTW], R1[RTW], R2[RTW];
 #pragma unroll
             for (int r = 0; r < RTW; ++r) {
-                const double xx = kind == 0 ? diff - om[r] : (kind == 1 ? diff - 2.0 * om[r] : -diff - om[r]);
-                double v;
-                if (part == 0) {
-                    v = xx * wn_rcp(xx * xx + w2);                      // PV: x / (x^2 + w^2)
-                } else if (sp.gaussian) {
-                    const double y = xx * iw;                           // pi exp(-y^2) / (sqrt(pi) w)
-                    v = fabs(xx) < cut ? gnorm * exp(-y * y) : 0.0;
-                } else {
-                    const double y = xx * iw;                           // pi / (pi w (1 + y^2))
-                    v = iw * wn_rcp(1.0 + y * y);
+                const double on = (pv && wv[r]) ? 1.0 : 0.0;
+                if (N0) {
+                    const double2 v = shg_resonance(dE - om[r], w2, iw, gnorm, cut, sp.gaussian);
+                    R0[r] = make_double2(on * v.x, on * v.y);
                 }
-                a[r] = valid ? v : 0.0;
+                if (N1) {
+                    const double2 v = shg_resonance(dE - 2.0 * om[r], w2, iw, gnorm, cut, sp.gaussian);
+                    R1[r] = make_double2(on * v.x, on * v.y);
+                }
+                if (N2) {
+                    const double2 v = shg_resonance(-dE - om[r], w2, iw, gnorm, cut, sp.gaussian);
+                    R2[r] = make_double2(on * v.x, on * v.y);
+                }
             }
-            const double *brow = lds_b + (size_t)(2 * (valid ? pl : 0) + part) * bw;
+            // first K step: Re R against (Cr, Ci); second: Im R against (-Ci, Cr)
 #pragma unroll
-            for (int j = 0; j < MAXT; ++j) {
-                if (j < ntile) {
-                    const int cr = j * 16 + lr;                  // real column
-                    const double b = (valid && cr < bw) ? brow[cr] : 0.0;
+            for (int j = 0; j < NT; ++j) {
+                const double bA = cv[j] ? (part ? bv[j].y : bv[j].x) : 0.0;
 #pragma unroll
-                    for (int r = 0; r < RTW; ++r)
-                        acc[r][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[r], b, acc[r][j], 0, 0, 0);
+                for (int r = 0; r < RTW; ++r) {
+                    const double a = j < N0 ? R0[r].x : (j < N0 + N1 ? R1[r].x : R2[r].x);
+                    acc[r][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(a, bA, acc[r][j], 0, 0, 0);
+                }
+            }
+#pragma unroll
+            for (int j = 0; j < NT; ++j) {
+                const double bB = cv[j] ? (part ? bv[j].x : -bv[j].y) : 0.0;
+#pragma unroll
+                for (int r = 0; r < RTW; ++r) {
+                    const double a = j < N0 ? R0[r].y : (j < N0 + N1 ? R1[r].y : R2[r].y);
+                    acc[r][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(a, bB, acc[r][j], 0, 0, 0);
                 }
             }
         }
-        __syncthreads();
     }
-    // D reg q -> row (frequency) lk + 4q, column lr of each 16-column tile
+    // D register q of lane (lg, lr) -> frequency row lg + 4q, real column lr of the tile
+    double2 *o = partial + (size_t)chunk * ((size_t)ncol * nw + nc);
 #pragma unroll
-    for (int r = 0; r < RTW; ++r)
+    for (int j = 0; j < NT; ++j) {
+        if (cv[j]) {
 #pragma unroll
-    for (int j = 0; j < MAXT; ++j) {
-        if (j < ntile) {
-            const int cr = j * 16 + lr;
-            if (cr < 2 * cwid) {
+            for (int r = 0; r < RTW; ++r)
 #pragma unroll
                 for (int q = 0; q < 4; ++q) {
-                    const int w = (rt0 + r) * 16 + lk + 4 * q;
-                    if (w < nw) {
-                        double *dst = (double *)(o + (size_t)(cbeg + (cr >> 1)) * nw + w);
-                        dst[cr & 1] = acc[r][j][q];
-                    }
+                    const int w = (rg * RTW + r) * 16 + lg + 4 * q;
+                    if (w < nw) ((double *)(o + (size_t)col[j] * nw + w))[part] = acc[r][j][q];
                 }
-            }
         }
+    }
+}
+
+// Frequency-independent low block (R = 1) and the one-band sums of one chunk: thread t sums the pairs
+// t, t + 256, ... of every column, then the 256 partial sums are combined in a fixed tree order; the low
+// sums are stored for every frequency.
+__global__ __launch_bounds__(WN_SHG_TPB) void k_shg_low(const int4 *__restrict__ chunks, const int *__restrict__ count,
+                                                       const double2 *__restrict__ coef,
+                                                       const double2 *__restrict__ one, double2 *__restrict__ partial,
+                                                       int pcap, WnShg sp) {
+    __shared__ double2 red[WN_SHG_TPB];
+    const int nc = sp.ncomp, ncol = 19 * nc, nw = sp.nw;
+    const int chunk = blockIdx.x, t = threadIdx.x;
+    const int4 ch = chunks[chunk];
+    double2 *o = partial + (size_t)chunk * ((size_t)ncol * nw + nc);
+    for (int c = 0; c < 3 * nc; ++c) {
+        double2 acc = make_double2(0.0, 0.0);
+        for (int ki = 0; ki < ch.y; ++ki) {
+            const int k = ch.x + ki, cnt = count[k];
+            for (int f = t; f < cnt; f += WN_SHG_TPB)
+                acc = cadd(acc, coef[((size_t)k * pcap + f) * ncol + 16 * nc + c]);
+        }
+        red[t] = acc;
+        __syncthreads();
+        for (int h = WN_SHG_TPB / 2; h > 0; h >>= 1) {
+            if (t < h) red[t] = cadd(red[t], red[t + h]);
+            __syncthreads();
+        }
+        const double2 sum = red[0];
+        for (int w = t; w < nw; w += WN_SHG_TPB) o[(size_t)(16 * nc + c) * nw + w] = sum;
+        __syncthreads();
+    }
+    if (t < nc) {
+        double sb = 0.0, ss = 0.0;
+        for (int k = ch.x; k < ch.x + ch.y; ++k) {
+            const double2 v = one[(size_t)k * nc + t];
+            sb += v.x;
+            ss += v.y;
+        }
+        o[(size_t)ncol * nw + t] = make_double2(sb, ss);
     }
 }
 
@@ -536,30 +650,37 @@ hipError_t wn_launch_shg(const double *x, const double *e, const double *omega, 
                          double2 *one, double2 *partial, const WnShape &sh, const WnShg &sp, hipStream_t st) {
     if (sh.n > WN_SHG_NMAX || sp.ncomp < 1 || sp.ncomp > 18) return hipErrorInvalidValue;
     if (count_k > 0) {
-        hipLaunchKernelGGL(k_shg_der, dim3(count_k, WN_SHG_DER_BLOCKS), dim3(WN_SHG_TPB), 0, st, x, e, der, sh, sp);
+        hipLaunchKernelGGL(k_shg_prep, dim3(count_k, 12), dim3(WN_SHG_TPB), 0, st, x, e, der, sh, sp);
+        const int tiles = shg_vtx_tiles(sh.n, sp.temperature <= 1.0e-7);
+        const unsigned nb = (unsigned)((count_k + 7) / 8 * 8) * 9u * (unsigned)tiles;
+        hipLaunchKernelGGL(k_shg_vtx, dim3(nb), dim3(64), 0, st, x, e, der, count_k, tiles, sh, sp);
         hipLaunchKernelGGL(k_shg_coef, dim3(count_k), dim3(WN_SHG_TPB), (size_t)sh.n * sp.ncomp * 2 * sizeof(double),
                            st, x, e, der, count, pairs, pdiff, coef, one, sh, sp);
     }
     if (nchunks > 0) {
-        // RTW row tiles of 16 frequencies per wave: a workgroup covers 64 RTW frequencies
-        const int rtw = sp.ncomp <= 4 ? 3 : (sp.ncomp <= 8 ? 2 : 1);
-        const int nwt = (sp.nw + 64 * rtw - 1) / (64 * rtw);
-        // pairs per LDS block: the widest block (8 nc complex columns) times pb within 32 KB, even, <= 128
-        // pairs per LDS block: the widest block (8 nc complex columns, 4 doubles each in the B layout) within
-        // 32 KB, even, <= 128
-        const int pb = std::max(2, std::min(128, (1024 / (8 * sp.ncomp)) & ~1));
-        const size_t lds = (size_t)pb * 8 * sp.ncomp * 4 * sizeof(double);
-        const dim3 grid(nchunks * nwt * 4);
-#define WN_SHG_LAUNCH(T, R)                                                                                         \
-    hipLaunchKernelGGL((k_shg_mfma<T, R>), grid, dim3(WN_SHG_TPB), lds, st, chunks, count, pdiff, coef, one, omega, \
-                       partial, sh.n * sh.n, nwt, pb, sp)
-        if (sp.ncomp <= 4)
-            WN_SHG_LAUNCH(4, 3);
-        else if (sp.ncomp <= 8)
-            WN_SHG_LAUNCH(8, 2);
-        else
-            WN_SHG_LAUNCH(18, 1);
-#undef WN_SHG_LAUNCH
+        // two row tiles of 16 frequencies per task (three spill at eight column tiles); nc <= 4: one task
+        // covers every column tile of the three kinds; larger nc: tasks of eight tiles of one kind (tiles past
+        // the kind's columns contribute zeros)
+        const int nrt = (sp.nw + 15) / 16;
+        const int nrg = (nrt + 1) / 2;
+        const ShgTileSet ts = ShgTileSet::of(sp.ncomp);
+        auto grid = [&](int ncg) { return dim3((unsigned)((nchunks + 7) / 8 * 8 * nrg * ncg)); };
+#define WN_SHG_ACC(A, B, C, NCG)                                                                                    \
+    hipLaunchKernelGGL((k_shg_acc<2, A, B, C>), grid(NCG), dim3(64), 0, st, chunks, nchunks, count, pdiff, coef, omega, \
+                       partial, sh.n * sh.n, nrg, NCG, sp)
+        switch (sp.ncomp) {
+            case 1: WN_SHG_ACC(1, 1, 1, 1); break;
+            case 2: WN_SHG_ACC(2, 2, 1, 1); break;
+            case 3: WN_SHG_ACC(3, 3, 1, 1); break;
+            case 4: WN_SHG_ACC(4, 3, 1, 1); break;
+            default:
+                WN_SHG_ACC(8, 0, 0, (ts.t0 + 7) / 8);
+                WN_SHG_ACC(0, 8, 0, (ts.t1 + 7) / 8);
+                WN_SHG_ACC(0, 0, 8, (ts.t2 + 7) / 8);
+        }
+#undef WN_SHG_ACC
+        hipLaunchKernelGGL(k_shg_low, dim3(nchunks), dim3(WN_SHG_TPB), 0, st, chunks, count, coef, one, partial,
+                           sh.n * sh.n, sp);
     }
     return hipGetLastError();
 }
