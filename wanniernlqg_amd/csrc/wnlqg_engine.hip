@@ -108,7 +108,7 @@ struct wnlqg_engine {
     WnShg sp{};         // SHG parameters
     DevBuf<double2> der, coef, one;  // SHG optical position derivatives, pair coefficients, one-band sums
     DevBuf<double2> escratch, rscratch;  // n > 64: Jacobi V and rotation U^+ X in global memory
-    DevBuf<int2> pairs;               // SHG pair lists
+    DevBuf<int> pidx;                 // SHG pair index of every (n, m) (-1: inactive)
     DevBuf<double> pdiff;             // SHG pair energy differences
     int device = 0;
     hipStream_t stream = nullptr;
@@ -274,7 +274,7 @@ int ensure_rows(wnlqg_engine *g, int rows) {
     if (g->family == 3) {
         WN_HIP(g->count.alloc((size_t)rows));
         WN_HIP(g->der.alloc((size_t)rows * WN_SHG_DER_BLOCKS * n2));   // 9 derivative blocks + 12 operand blocks
-        WN_HIP(g->pairs.alloc((size_t)rows * n2));
+        WN_HIP(g->pidx.alloc((size_t)rows * n2));
         WN_HIP(g->pdiff.alloc((size_t)rows * n2));
         WN_HIP(g->coef.alloc((size_t)rows * n2 * 19 * g->sp.ncomp));
         WN_HIP(g->one.alloc((size_t)rows * g->sp.ncomp));
@@ -442,7 +442,7 @@ int32_t wnlqg_destroy(wnlqg_engine *g) {
     g->bl.release(); g->fh.release(); g->fm.release(); g->cup.release(); g->cdown.release(); g->t12.release();
     g->t3.release(); g->ridx2.release(); g->box.release(); g->y3.release(); g->y23.release(); g->e1.release();
     g->e2.release(); g->e3.release(); g->plan.dchunks.release(); g->plan.dlanes.release();
-    g->der.release(); g->coef.release(); g->one.release(); g->pairs.release(); g->pdiff.release();
+    g->der.release(); g->coef.release(); g->one.release(); g->pidx.release(); g->pdiff.release();
     for (hipEvent_t e : {g->ev_fork, g->ev_join, g->ev_ready[0], g->ev_ready[1], g->ev_free[0], g->ev_free[1]})
         if (e) (void)hipEventDestroy(e);
     if (g->stream2) (void)hipStreamDestroy(g->stream2);
@@ -842,7 +842,7 @@ int32_t wnlqg_eval_points_spectrum(wnlqg_engine *g, const double *kpoints, int32
         WN_HIP(hipStreamSynchronize(g->stream));
         WN_HIP(hipMemcpy(g->chunks.p, ch.data(), cnt * sizeof(int4), hipMemcpyHostToDevice));
         if (g->family == 3) {
-            WN_HIP(wn_launch_shg(g->x.p, g->e.p, g->omega.p, g->chunks.p, cnt, cnt, g->der.p, g->count.p, g->pairs.p,
+            WN_HIP(wn_launch_shg(g->x.p, g->e.p, g->omega.p, g->chunks.p, cnt, cnt, g->der.p, g->count.p, g->pidx.p,
                                  g->pdiff.p, g->coef.p, g->one.p, g->partial.p, g->sh, g->sp, g->stream));
         } else {
             WN_HIP(launch_response(g, g->chunks.p, cnt, cnt, p0));
@@ -1096,10 +1096,10 @@ int enqueue_mesh(wnlqg_engine *g, std::vector<std::array<hipEvent_t, WN_MARKS>> 
             WN_HIP(wn_launch_accum(chunks, nch, g->count.p, g->pmeta.p, g->pmd.p, g->q.p, g->omega.p, g->partial.p,
                                    g->sh, g->sc, s1));
         } else if (g->family == 3) {
-            WN_HIP(wn_launch_shg(xb, eb, g->omega.p, chunks, 0, cnt, g->der.p, g->count.p, g->pairs.p, g->pdiff.p,
+            WN_HIP(wn_launch_shg(xb, eb, g->omega.p, chunks, 0, cnt, g->der.p, g->count.p, g->pidx.p, g->pdiff.p,
                                  g->coef.p, g->one.p, g->partial.p, g->sh, g->sp, s1));
             mark(bi, 5, s1);
-            WN_HIP(wn_launch_shg(xb, eb, g->omega.p, chunks, nch, 0, g->der.p, g->count.p, g->pairs.p, g->pdiff.p,
+            WN_HIP(wn_launch_shg(xb, eb, g->omega.p, chunks, nch, 0, g->der.p, g->count.p, g->pidx.p, g->pdiff.p,
                                  g->coef.p, g->one.p, g->partial.p, g->sh, g->sp, s1));
         } else {
             WN_HIP(wn_launch_geom_f64(xb, eb, g->omega.p, chunks, nch, g->partial.p, g->flags.p, k0, g->sh, g->gp, s1));

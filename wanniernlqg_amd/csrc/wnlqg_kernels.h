@@ -192,7 +192,7 @@ hipError_t wn_launch_geom_f64(const double *x, const double *e, const double *mu
 // followed by [nc] (one-band BCD, semiclassical) sums.
 #define WN_SHG_DER_BLOCKS 21
 hipError_t wn_launch_shg(const double *x, const double *e, const double *omega, const int4 *chunks, int nchunks,
-                         int count_k, double2 *der, int *count, int2 *pairs, double *pdiff, double2 *coef,
+                         int count_k, double2 *der, int *count, int *pidx, double *pdiff, double2 *coef,
                          double2 *one, double2 *partial, const WnShape &sh, const WnShg &sp, hipStream_t st);
 // Fixed-order chunk -> lane sums.
 hipError_t wn_launch_lane_reduce(const int4 *lanes, int nlanes, const double2 *partial, double2 *lanebuf,

@@ -277,28 +277,25 @@ __global__ __launch_bounds__(64) void k_shg_vtx(const double *__restrict__ x, co
 }
 
 // ------------------------------------------------------------------------------------------------
-// Pair list, coefficient rows and one-band sums of one k point (one workgroup per k).
-// pairs[k][p] = (n, m), pdiff[k][p] = E_m - E_n, coef[k][p][col], count[k]; one[k][comp][2] = (bcd, semi).
+// Pair list of one k point (one workgroup per k): active pairs n != m with |f_n - f_m| >= 1e-10 in
+// n-major order (SecondHarmonicKernels.jl:140-141): count[k], pdiff[k][p] = E_m - E_n and the index map
+// pidx[k][n][m] = p (-1 for an inactive pair); and at T > 0 the one-band (Fermi-surface) BCD and
+// semiclassical sums one[k][comp] = (bcd, semi) (SecondHarmonicKernels.jl:245-294).
 // ------------------------------------------------------------------------------------------------
-__global__ __launch_bounds__(WN_SHG_TPB) void k_shg_coef(const double *__restrict__ x, const double *__restrict__ eg,
-                                                        const double2 *__restrict__ der, int *__restrict__ count,
-                                                        int2 *__restrict__ pairs, double *__restrict__ pdiff,
-                                                        double2 *__restrict__ coef, double2 *__restrict__ one,
-                                                        WnShape sh, WnShg sp) {
+__global__ __launch_bounds__(WN_SHG_TPB) void k_shg_pairs(const double *__restrict__ x, const double *__restrict__ eg,
+                                                         int *__restrict__ count, int *__restrict__ pidx,
+                                                         double *__restrict__ pdiff, double2 *__restrict__ one,
+                                                         WnShape sh, WnShg sp) {
     __shared__ double occ[WN_SHG_NMAX];
     __shared__ int rowcnt[WN_SHG_NMAX];
     __shared__ int rowoff[WN_SHG_NMAX + 1];
     extern __shared__ double lds_one[];  // [n][nc][2] one-band contributions
     const long long k = blockIdx.x;
     const int n = sh.n, n2 = n * n, nc = sp.ncomp;
-    const int ncol = 19 * nc;
     const ShgView s = make_view(x, eg, k, sh, sp.eta);
-    const double2 *dk = der + (size_t)k * WN_SHG_DER_BLOCKS * n2;
-    const double2 *rk = dk + (size_t)kShgR * n2;   // R[d][i][j]; r_eta = A + i R
     const int t = threadIdx.x;
     if (t < n) occ[t] = shg_occupation(s.E[t], sp.mu, sp.temperature);
     __syncthreads();
-    // n-major pair order (SecondHarmonicKernels.jl:140-141)
     if (t < n) {
         int c = 0;
         for (int m = 0; m < n; ++m) c += shg_active(occ, t, m);
@@ -314,81 +311,16 @@ __global__ __launch_bounds__(WN_SHG_TPB) void k_shg_coef(const double *__restric
     const size_t pbase = (size_t)k * n2;
     if (t < n) {
         int p = rowoff[t];
-        for (int m = 0; m < n; ++m)
+        for (int m = 0; m < n; ++m) {
             if (shg_active(occ, t, m)) {
-                pairs[pbase + p] = make_int2(t, m);
+                pidx[pbase + t * n + m] = p;
                 pdiff[pbase + p] = s.E[m] - s.E[t];
                 ++p;
+            } else {
+                pidx[pbase + t * n + m] = -1;
             }
-    }
-    __syncthreads();
-    const int np = rowoff[n];
-    auto D = [&](int aa, int bb, int i, int j) { return dk[(size_t)(3 * aa + bb) * n2 + i * n + j]; };
-    auto RE = [&](int d, int i, int j) { return cadd(sv_A(s, d, i, j), cmuli(rk[(size_t)d * n2 + i * n + j])); };
-    for (int item = t; item < np * nc; item += WN_SHG_TPB) {
-        const int p = item / nc, ci = item - (item / nc) * nc;
-        const int2 pr = pairs[pbase + p];
-        const int nn = pr.x, m = pr.y;
-        const int cc = sp.comp[ci];
-        const int a = cc % 3, b = (cc / 3) % 3, c = cc / 9;
-        const double diff = s.E[m] - s.E[nn];
-        const double inverse = shg_bare_inverse(diff);
-        const double o = occ[nn] - occ[m];
-        const double2 dip_a_nm = sv_dip(s, a, nn, m);
-        const double2 dip_b_mn = sv_dip(s, b, m, nn), dip_c_mn = sv_dip(s, c, m, nn);
-        const double2 dip_b_nm = sv_dip(s, b, nn, m), dip_c_nm = sv_dip(s, c, nn, m);
-        const double2 first = cmul(dip_a_nm, cadd(D(b, c, m, nn), D(c, b, m, nn)));
-        const double2 second = cadd(cmul(D(a, b, nn, m), dip_c_mn), cmul(D(a, c, nn, m), dip_b_mn));
-        const double vc = sv_V(s, c, m, m).x - sv_V(s, c, nn, nn).x;
-        const double vb = sv_V(s, b, m, m).x - sv_V(s, b, nn, nn).x;
-        const double2 third =
-            cscale(cmul(dip_a_nm, cadd(cscale(dip_b_mn, vc), cscale(dip_c_mn, vb))), inverse);
-        const double2 fourth = cadd(cmul(D(b, a, nn, m), dip_c_mn), cmul(D(c, a, nn, m), dip_b_mn));
-        const double va = sv_V(s, a, nn, nn).x - sv_V(s, a, m, m).x;
-        const double2 fifth = cscale(cadd(cmul(dip_b_nm, dip_c_mn), cmul(dip_c_nm, dip_b_mn)), va);
-        double2 tf = make_double2(0.0, 0.0), to = make_double2(0.0, 0.0);
-        for (int q = 0; q < n; ++q) {
-            if (q == nn || q == m) continue;
-            const double2 rb_mq = RE(b, m, q), rc_qn = RE(c, q, nn);
-            const double2 rc_mq = RE(c, m, q), rb_qn = RE(b, q, nn);
-            const double pv0 = shg_principal(2.0 * s.E[q] - s.E[nn] - s.E[m], sp.eta);
-            tf = cadd(tf, cscale(cmul(dip_a_nm, cadd(cmul(rb_mq, rc_qn), cmul(rc_mq, rb_qn))), pv0));
-            const double pv1 = shg_principal(2.0 * s.E[nn] - s.E[m] - s.E[q], sp.eta);
-            const double2 ra_mq = RE(a, m, q);
-            to = cadd(to, cscale(cmul(ra_mq, cadd(cmul(rb_qn, dip_c_nm), cmul(rc_qn, dip_b_nm))), pv1));
-            const double pv2 = shg_principal(2.0 * s.E[m] - s.E[q] - s.E[nn], sp.eta);
-            const double2 ra_qn = RE(a, q, nn);
-            to = cadd(to, cscale(cmul(ra_qn, cadd(cmul(dip_b_nm, rc_mq), cmul(dip_c_nm, rb_mq))), pv2));
         }
-        // f = i o / 2;  f * inv * z = i (o inv / 2) z
-        const double h = 0.5 * o;
-        const double hi = h * inverse;
-        double2 *row = coef + ((size_t)pbase + p) * ncol;
-        double2 *d1 = row + ci * 8;
-        double2 *d2 = row + nc * 8 + ci * 6;
-        double2 *dn = row + nc * 14 + ci * 2;
-        double2 *lw = row + nc * 16 + ci * 3;
-        d1[0] = cmuli(cscale(second, hi));
-        d1[1] = cmuli(cscale(third, hi));
-        d1[2] = cmuli(cscale(fourth, -0.5 * hi));
-        d1[3] = cmuli(cscale(fifth, -0.25 * h));
-        d1[4] = cscale(second, o);
-        d1[5] = cscale(third, o);
-        d1[6] = cscale(fourth, -0.5 * o);
-        d1[7] = cscale(fifth, -0.25 * o);
-        d2[0] = cmuli(cscale(first, 2.0 * hi));
-        d2[1] = cmuli(cscale(third, -4.0 * hi));
-        d2[2] = cscale(tf, o);
-        d2[3] = cscale(first, o);
-        d2[4] = cscale(third, -2.0 * o);
-        d2[5] = cmuli(cscale(tf, -o * diff));
-        dn[0] = cscale(to, 0.5 * o);
-        dn[1] = cmuli(cscale(to, o * diff));
-        lw[0] = cmuli(cscale(cadd(first, second), hi));
-        lw[1] = cmuli(cscale(third, -hi));
-        lw[2] = cmuli(cscale(fourth, -0.5 * hi));
     }
-    // one-band terms (SecondHarmonicKernels.jl:245-294), T > 1e-7 only
     double2 *ok = one + (size_t)k * nc;
     if (sp.temperature <= 1.0e-7) {
         for (int ci = t; ci < nc; ci += WN_SHG_TPB) ok[ci] = make_double2(0.0, 0.0);
@@ -447,6 +379,180 @@ __global__ __launch_bounds__(WN_SHG_TPB) void k_shg_coef(const double *__restric
             ss += lds_one[(nn * nc + ci) * 2 + 1];
         }
         ok[ci] = make_double2(sb, ss);
+    }
+}
+
+// ------------------------------------------------------------------------------------------------
+// Coefficient rows of the active pairs (SecondHarmonicKernels.jl:115-236): one thread per pair (n, m) of a
+// 16 x 16 tile over the same blocks as the vertex kernel.  The three-band sums over q != n, m of the
+// reference,
+//   tf = r0_a,nm sum_q PV(2E_q - E_n - E_m) (r_b,mq r_c,qn + r_c,mq r_b,qn)
+//   to = sum_q PV(2E_n - E_m - E_q) r_a,mq (r_b,qn r0_c,nm + r_c,qn r0_b,nm)
+//        + PV(2E_m - E_q - E_n) r_a,qn (r0_b,nm r_c,mq + r0_c,nm r_b,mq)        (r = r_eta = A + i R),
+// are regrouped with P_xy = r_x,mq r_y,qn as tf = r0_a,nm T_bc, to = r0_c,nm Z_ab + r0_b,nm Z_ac, where
+// T_xy = sum_q PV0 (P_xy + P_yx) and Z_xy = sum_q PV1 P_xy + PV2 P_yx: fifteen sums per pair serve every
+// component, and the three principal values are evaluated once per (pair, q).  Rows m (r_x,m.) and
+// columns n (r_y,.n) of the tile are staged through LDS in chunks of WN_SHG_QC bands.
+// Coefficient row layout (complex columns of component ci): d1 block ci 8 .. ci 8 + 7, d2 block
+// 8 nc + ci 6 .., dn block 14 nc + ci 2 .., low block 16 nc + ci 3 ..
+// ------------------------------------------------------------------------------------------------
+#define WN_SHG_QC 32
+namespace {
+template <int N>
+__device__ __forceinline__ double2 shg_pick(const double2 (&v)[N], int idx) {
+    unsigned long long re = 0, im = 0;
+#pragma unroll
+    for (int i = 0; i < N; ++i) {
+        const unsigned long long msk = 0ull - (unsigned long long)(i == idx);
+        re |= (unsigned long long)__double_as_longlong(v[i].x) & msk;
+        im |= (unsigned long long)__double_as_longlong(v[i].y) & msk;
+    }
+    return make_double2(__longlong_as_double((long long)re), __longlong_as_double((long long)im));
+}
+}  // namespace
+__global__ __launch_bounds__(WN_SHG_TPB) void k_shg_coef(const double *__restrict__ x, const double *__restrict__ eg,
+                                                        const double2 *__restrict__ der, const int *__restrict__ pidx,
+                                                        double2 *__restrict__ coef, int count_k, int tiles,
+                                                        WnShape sh, WnShg sp) {
+    __shared__ double2 rm[3][WN_SHG_QC][16];   // r_x[m][q] of the tile's columns m
+    __shared__ double2 cn[3][WN_SHG_QC][16];   // r_y[q][n] of the tile's rows n
+    __shared__ double es[WN_SHG_NMAX];
+    const int b8 = blockIdx.x, xcd = b8 & 7, s8 = b8 >> 3;
+    const long long k = (long long)(s8 / tiles) * 8 + xcd;
+    if (k >= count_k) return;
+    int tile = s8 - (s8 / tiles) * tiles;
+    const int n = sh.n, n2 = n * n, nc = sp.ncomp, ncol = 19 * nc;
+    const ShgView s = make_view(x, eg, k, sh, sp.eta);
+    int r_lo = 0, r_hi = n, c_lo = 0, c_hi = n;
+    if (sp.temperature <= 1.0e-7) {
+        const int no = shg_occupied(s.E, n, sp.mu);
+        const int t1 = ((no + 15) / 16) * ((n - no + 15) / 16);
+        if (tile < t1) {
+            r_hi = no;
+            c_lo = no;
+        } else {
+            tile -= t1;
+            r_lo = no;
+            c_hi = no;
+        }
+    }
+    const int ntc = (c_hi - c_lo + 15) / 16, ntr = (r_hi - r_lo + 15) / 16;
+    if (tile >= ntr * ntc) return;   // uniform over the workgroup
+    const int r0 = r_lo + 16 * (tile / ntc), c0 = c_lo + 16 * (tile - (tile / ntc) * ntc);
+    const int t = threadIdx.x, ln = t >> 4, lm = t & 15;
+    const int nn = r0 + ln, m = c0 + lm;
+    const bool inside = nn < r_hi && m < c_hi;
+    const double2 *dk = der + (size_t)k * WN_SHG_DER_BLOCKS * n2;
+    const double2 *rk = dk + (size_t)kShgR * n2;
+    auto RE = [&](int d, int i, int j) { return cadd(sv_A(s, d, i, j), cmuli(rk[(size_t)d * n2 + i * n + j])); };
+    for (int i = t; i < n; i += WN_SHG_TPB) es[i] = s.E[i];
+    double2 T[6], Z[9];
+#pragma unroll
+    for (int i = 0; i < 6; ++i) T[i] = make_double2(0.0, 0.0);
+#pragma unroll
+    for (int i = 0; i < 9; ++i) Z[i] = make_double2(0.0, 0.0);
+    const double eta = sp.eta;
+    const double en = s.E[min(nn, n - 1)], em = s.E[min(m, n - 1)];
+    for (int q0 = 0; q0 < n; q0 += WN_SHG_QC) {
+        __syncthreads();
+        for (int e = t; e < 3 * WN_SHG_QC * 16; e += WN_SHG_TPB) {
+            const int d = e / (WN_SHG_QC * 16), rem = e - d * (WN_SHG_QC * 16);
+            {   // row element: q fastest (contiguous in memory)
+                const int q = rem % WN_SHG_QC, ml = rem / WN_SHG_QC;
+                rm[d][q][ml] = RE(d, min(c0 + ml, n - 1), min(q0 + q, n - 1));
+            }
+            {   // column element: n fastest
+                const int nl = rem & 15, q = rem >> 4;
+                cn[d][q][nl] = RE(d, min(q0 + q, n - 1), min(r0 + nl, n - 1));
+            }
+        }
+        __syncthreads();
+        const int qn = min(WN_SHG_QC, n - q0);
+        for (int q = 0; q < qn; ++q) {
+            const int qq = q0 + q;
+            const double eq = es[qq];
+            const bool skip = qq == nn || qq == m;
+            const double pv0 = skip ? 0.0 : shg_principal(2.0 * eq - en - em, eta);
+            const double pv1 = skip ? 0.0 : shg_principal(2.0 * en - em - eq, eta);
+            const double pv2 = skip ? 0.0 : shg_principal(2.0 * em - eq - en, eta);
+            double2 rx[3], ry[3];
+#pragma unroll
+            for (int d = 0; d < 3; ++d) {
+                rx[d] = rm[d][q][lm];
+                ry[d] = cn[d][q][ln];
+            }
+            double2 P[3][3];
+#pragma unroll
+            for (int xd = 0; xd < 3; ++xd)
+#pragma unroll
+                for (int yd = 0; yd < 3; ++yd) P[xd][yd] = cmul(rx[xd], ry[yd]);
+#pragma unroll
+            for (int xd = 0; xd < 3; ++xd)
+#pragma unroll
+                for (int yd = 0; yd < 3; ++yd) {
+                    Z[3 * xd + yd] = cadd(Z[3 * xd + yd], cadd(cscale(P[xd][yd], pv1), cscale(P[yd][xd], pv2)));
+                    if (xd <= yd) {
+                        const int si = wn_sym(xd, yd);
+                        T[si] = cadd(T[si], cscale(cadd(P[xd][yd], P[yd][xd]), pv0));
+                    }
+                }
+        }
+    }
+    if (!inside) return;
+    const int p = pidx[(size_t)k * n2 + nn * n + m];
+    if (p < 0) return;
+    const double diff = s.E[m] - s.E[nn];
+    const double inverse = shg_bare_inverse(diff);
+    const double o = shg_occupation(s.E[nn], sp.mu, sp.temperature) - shg_occupation(s.E[m], sp.mu, sp.temperature);
+    auto D = [&](int aa, int bb, int i, int j) { return dk[(size_t)(3 * aa + bb) * n2 + i * n + j]; };
+    double2 *row = coef + ((size_t)k * n2 + p) * ncol;
+    for (int ci = 0; ci < nc; ++ci) {
+        const int cc = sp.comp[ci];
+        const int a = cc % 3, b = (cc / 3) % 3, c = cc / 9;
+        const double2 dip_a_nm = sv_dip(s, a, nn, m);
+        const double2 dip_b_mn = sv_dip(s, b, m, nn), dip_c_mn = sv_dip(s, c, m, nn);
+        const double2 dip_b_nm = sv_dip(s, b, nn, m), dip_c_nm = sv_dip(s, c, nn, m);
+        const double2 first = cmul(dip_a_nm, cadd(D(b, c, m, nn), D(c, b, m, nn)));
+        const double2 second = cadd(cmul(D(a, b, nn, m), dip_c_mn), cmul(D(a, c, nn, m), dip_b_mn));
+        const double vc = sv_V(s, c, m, m).x - sv_V(s, c, nn, nn).x;
+        const double vb = sv_V(s, b, m, m).x - sv_V(s, b, nn, nn).x;
+        const double2 third =
+            cscale(cmul(dip_a_nm, cadd(cscale(dip_b_mn, vc), cscale(dip_c_mn, vb))), inverse);
+        const double2 fourth = cadd(cmul(D(b, a, nn, m), dip_c_mn), cmul(D(c, a, nn, m), dip_b_mn));
+        const double va = sv_V(s, a, nn, nn).x - sv_V(s, a, m, m).x;
+        const double2 fifth = cscale(cadd(cmul(dip_b_nm, dip_c_mn), cmul(dip_c_nm, dip_b_mn)), va);
+        // the runtime component indices pick among the sums by bit masks (a dynamic register index, or
+        // selects that the compiler turns into one, would place the sums in scratch memory)
+        const double2 Tbc = shg_pick<6>(T, wn_sym(b, c)), Zab = shg_pick<9>(Z, 3 * a + b);
+        const double2 Zac = shg_pick<9>(Z, 3 * a + c);
+        const double2 tf = cmul(dip_a_nm, Tbc);
+        const double2 to = cadd(cmul(dip_c_nm, Zab), cmul(dip_b_nm, Zac));
+        // f = i o / 2;  f * inv * z = i (o inv / 2) z
+        const double h = 0.5 * o;
+        const double hi = h * inverse;
+        double2 *d1 = row + ci * 8;
+        double2 *d2 = row + nc * 8 + ci * 6;
+        double2 *dn = row + nc * 14 + ci * 2;
+        double2 *lw = row + nc * 16 + ci * 3;
+        d1[0] = cmuli(cscale(second, hi));
+        d1[1] = cmuli(cscale(third, hi));
+        d1[2] = cmuli(cscale(fourth, -0.5 * hi));
+        d1[3] = cmuli(cscale(fifth, -0.25 * h));
+        d1[4] = cscale(second, o);
+        d1[5] = cscale(third, o);
+        d1[6] = cscale(fourth, -0.5 * o);
+        d1[7] = cscale(fifth, -0.25 * o);
+        d2[0] = cmuli(cscale(first, 2.0 * hi));
+        d2[1] = cmuli(cscale(third, -4.0 * hi));
+        d2[2] = cscale(tf, o);
+        d2[3] = cscale(first, o);
+        d2[4] = cscale(third, -2.0 * o);
+        d2[5] = cmuli(cscale(tf, -o * diff));
+        dn[0] = cscale(to, 0.5 * o);
+        dn[1] = cmuli(cscale(to, o * diff));
+        lw[0] = cmuli(cscale(cadd(first, second), hi));
+        lw[1] = cmuli(cscale(third, -hi));
+        lw[2] = cmuli(cscale(fourth, -0.5 * hi));
     }
 }
 
@@ -646,7 +752,7 @@ __global__ __launch_bounds__(WN_SHG_TPB) void k_shg_low(const int4 *__restrict__
 }
 
 hipError_t wn_launch_shg(const double *x, const double *e, const double *omega, const int4 *chunks, int nchunks,
-                         int count_k, double2 *der, int *count, int2 *pairs, double *pdiff, double2 *coef,
+                         int count_k, double2 *der, int *count, int *pidx, double *pdiff, double2 *coef,
                          double2 *one, double2 *partial, const WnShape &sh, const WnShg &sp, hipStream_t st) {
     if (sh.n > WN_SHG_NMAX || sp.ncomp < 1 || sp.ncomp > 18) return hipErrorInvalidValue;
     if (count_k > 0) {
@@ -654,8 +760,10 @@ hipError_t wn_launch_shg(const double *x, const double *e, const double *omega, 
         const int tiles = shg_vtx_tiles(sh.n, sp.temperature <= 1.0e-7);
         const unsigned nb = (unsigned)((count_k + 7) / 8 * 8) * 9u * (unsigned)tiles;
         hipLaunchKernelGGL(k_shg_vtx, dim3(nb), dim3(64), 0, st, x, e, der, count_k, tiles, sh, sp);
-        hipLaunchKernelGGL(k_shg_coef, dim3(count_k), dim3(WN_SHG_TPB), (size_t)sh.n * sp.ncomp * 2 * sizeof(double),
-                           st, x, e, der, count, pairs, pdiff, coef, one, sh, sp);
+        hipLaunchKernelGGL(k_shg_pairs, dim3(count_k), dim3(WN_SHG_TPB), (size_t)sh.n * sp.ncomp * 2 * sizeof(double),
+                           st, x, e, count, pidx, pdiff, one, sh, sp);
+        hipLaunchKernelGGL(k_shg_coef, dim3((unsigned)((count_k + 7) / 8 * 8) * (unsigned)tiles), dim3(WN_SHG_TPB), 0,
+                           st, x, e, der, pidx, coef, count_k, tiles, sh, sp);
     }
     if (nchunks > 0) {
         // two row tiles of 16 frequencies per task (three spill at eight column tiles); nc <= 4: one task
