@@ -558,7 +558,7 @@ def form_large(args):
            "higher_is_better": True, "dtype": "f64", "data": "synthetic (seeded TB model)",
            "config": {"workload": f"SHG xyy, yxz, zzz; 400 omega in [0,3] eV; Gaussian 0.02 eV; n_w=96, N_R=2197; "
                                   f"{mesh[0]}^3 sub-mesh of the 256^3 configuration", "kpoints": nk},
-           "setup_s": setup, "projected_256cube_s": 256 ** 3 * wall / nk,
+           "setup_s": setup, "projected_256cube_s": 256 ** 3 * wall / nk, "mean_sweeps": st["mean_sweeps"],
            "stages_ms": {k: st[k] for k in ("interp_ms", "eig_ms", "rotate_ms", "pairs_ms", "accum_ms", "total_ms")}}
     print(json.dumps(out), flush=True)
 

@@ -13,7 +13,7 @@ import sys
 HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "libwnlqg_hip.so")
-SOURCES = ["wnlqg_kernels.hip", "wnlqg_eig.hip", "wnlqg_geometry.hip", "wnlqg_shg.hip", "wnlqg_mlp.hip", "wnlqg_engine.hip"]
+SOURCES = ["wnlqg_kernels.hip", "wnlqg_eig.hip", "wnlqg_eigbig.hip", "wnlqg_geometry.hip", "wnlqg_shg.hip", "wnlqg_mlp.hip", "wnlqg_engine.hip"]
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 
 

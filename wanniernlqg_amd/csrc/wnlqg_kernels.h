@@ -105,7 +105,7 @@ hipError_t wn_launch_contract_sel(const double2 *in, double2 *out, const double2
                                   hipStream_t st);
 // Jacobi eigensolver of the upper triangle of channel 0.
 // tpb: threads per k point (64, 128, 256; 0 = default of the size class).
-// vscratch: [count][N][N] complex (N = 96 or 128), required for n > 64 (the Jacobi's V in global memory).
+// vscratch: wn_eig_scratch_elems(n, count) complex elements, required for n > 64 (deferred-eigenvector solver).
 // status (nullable): word 1 |= 1 when a point reaches sh.max_sweeps without meeting the stopping test.
 hipError_t wn_launch_eig_f64(const double *x, double *e, double2 *u, int *sweeps, int count, const WnShape &sh,
                              int tpb, hipStream_t st, double2 *vscratch = nullptr, int *status = nullptr);
@@ -132,6 +132,11 @@ hipError_t wn_launch_eig_rv(const double *x, double *e, double2 *u, int *sweeps,
                             int count, const WnShape &sh, int pass, hipStream_t st, const double2 *uprev,
                             int ctr_stride, int *status);
 // size in complex elements of the eigensolver / rotation scratch for count points (0 for n <= 64)
+// 64 < n <= 128 (wnlqg_eigbig.hip): sweep chunks on H with recorded rotations, rotations applied to V by
+// row-owning waves, then sort and phase; scratch: wn_eig_big_scratch_elems(n, count) complex elements.
+size_t wn_eig_big_scratch_elems(int n, int count);
+hipError_t wn_launch_eig_big(const double *x, double *e, double2 *u, int *sweeps, unsigned long long *sweep_total,
+                             int count, const WnShape &sh, hipStream_t st, double2 *scr, int ctr_stride, int *status);
 size_t wn_eig_scratch_elems(int n, int count);
 size_t wn_rotate_scratch_elems(int n, int count);
 bool wn_warm_start(int n, int count);

@@ -865,13 +865,12 @@ __device__ __forceinline__ int wn_warm_k(int b, int pass, int count, int *anchor
 // pass < 0: every k point from the identity (grid = count); pass 0 / 1: the warm-start passes above;
 // pass 2: every k point from uprev[k], the eigenvectors of the same point of the previous mesh plane
 // (channel 0 already transformed by them).
-// VG: V lives in global memory (vscratch, [count][N][N] column-major, so that the threads of one plane
-// update read consecutive rows) because H and V together exceed the LDS for N > 64.
-template <int N, int TPB, bool VG>
+// n > 64 runs the deferred-eigenvector solver of wnlqg_eigbig.hip instead (H and V exceed the LDS).
+template <int N, int TPB>
 __global__ __launch_bounds__(TPB, (N <= 32 && TPB == 256) ? 6 : 1) void k_eig_f64(const double *__restrict__ x, double *__restrict__ eout,
                                                 double2 *__restrict__ uout, int *__restrict__ sweeps_out,
                                                 unsigned long long *__restrict__ sweep_total, int n, int ldo,
-                                                double tol, int pass, int count, double2 *__restrict__ vscratch,
+                                                double tol, int pass, int count,
                                                 const double2 *__restrict__ uprev, int ctr_stride,
                                                 int *__restrict__ status, int max_sweeps) {
     constexpr int LD = N + 1;
@@ -880,7 +879,7 @@ __global__ __launch_bounds__(TPB, (N <= 32 && TPB == 256) ? 6 : 1) void k_eig_f6
     constexpr int NW = TPB / 64;
     constexpr int NP = N * (N + 1) / 2;
     __shared__ double2 H[NP];       // packed upper triangle, row-major
-    __shared__ double2 Vs[VG ? 1 : N * LD];
+    __shared__ double2 Vs[N * LD];
     __shared__ double4 rot[HALF];   // (c, s, cos phi, sin phi)
     __shared__ int2 pq[HALF];
     __shared__ double red[2 * NW];
@@ -891,9 +890,8 @@ __global__ __launch_bounds__(TPB, (N <= 32 && TPB == 256) ? 6 : 1) void k_eig_f6
 
     // packed row-major index of (i, j), i <= j (24-bit multiply: full-rate on the VALU)
     auto P = [](int i, int j) { return (int)(__umul24((unsigned)i, (unsigned)(2 * N - i + 1)) >> 1) + (j - i); };
-    double2 *vg = VG ? vscratch + (size_t)k * N * N : nullptr;
     // V[i][j]: row i, column j (the eigenvector index)
-    auto V = [&](int i, int j) -> double2 & { return VG ? vg[(size_t)j * N + i] : Vs[i * LD + j]; };
+    auto V = [&](int i, int j) -> double2 & { return Vs[i * LD + j]; };
     const double2 *xk = (const double2 *)(x + (size_t)k * ldo);
     const double2 *ua = pass == 2 ? uprev + (size_t)k * n * n : (anchor >= 0 ? uout + (size_t)anchor * n * n : nullptr);
     for (int e = t; e < N * N; e += TPB) {
@@ -908,7 +906,6 @@ __global__ __launch_bounds__(TPB, (N <= 32 && TPB == 256) ? 6 : 1) void k_eig_f6
         }
         V(i, j) = (ua && i < n && j < n) ? ua[i * n + j] : make_double2(i == j ? 1.0 : 0.0, 0.0);
     }
-    if (VG) __threadfence_block();
     __syncthreads();
 
     // Work split of a round: NOFF off-diagonal blocks (t1 < t2) and VITEMS updates of V.  When the
@@ -927,9 +924,9 @@ __global__ __launch_bounds__(TPB, (N <= 32 && TPB == 256) ? 6 : 1) void k_eig_f6
     constexpr bool VSLOT = VW % HALF == 0 && VTPS > 0 && N % VTPS == 0 && N / VTPS == VPT;
     // Bank-conflict-free V mapping: 16 consecutive lanes share one pair slot and take rows i0, i0 + 16, ...
     // (i0 = lane mod 16), so a 16-byte LDS access of the 16 lanes touches the 16 distinct 16-byte bank
-    // groups (i0 (N + 1) + p) mod 16 (N + 1 odd); the VG lane groups take slots g, g + VG, ...
+    // groups (i0 (N + 1) + p) mod 16 (N + 1 odd); the VG16 lane groups take slots g, g + VG16, ...
     constexpr int VG16 = VW / 16;
-    constexpr bool VQUAD = !VG && VW % 16 == 0 && N % 16 == 0 && VG16 > 0 && HALF % VG16 == 0 &&
+    constexpr bool VQUAD = VW % 16 == 0 && N % 16 == 0 && VG16 > 0 && HALF % VG16 == 0 &&
                            (HALF / VG16) * (N / 16) == VPT && VT0 % 64 == 0;
     int bt1[BPT], bt2[BPT];
 #pragma unroll
@@ -1131,7 +1128,6 @@ __global__ __launch_bounds__(TPB, (N <= 32 && TPB == 256) ? 6 : 1) void k_eig_f6
                     }
                 }
             }
-            if (VG) __threadfence_block();
             __syncthreads();
         }
     }
@@ -1196,8 +1192,8 @@ hipError_t wn_launch_eig_pass_f64(const double *x, double *e, double2 *u, int *s
     const int grid = (pass < 0 || pass == 2) ? count : pass == 0 ? groups : groups * (WN_WARM_G - 1);
     // tpb: threads per k point (64, 128 or 256); 0 selects the default of the size class
 #define WN_EIG_LAUNCH(NN, T) \
-    hipLaunchKernelGGL((k_eig_f64<NN, T, false>), dim3(grid), dim3(T), 0, st, x, e, u, sweeps, sweep_total, n, sh.ldo, \
-                       tol, pass, count, nullptr, uprev, ctr_stride, status, sh.max_sweeps)
+    hipLaunchKernelGGL((k_eig_f64<NN, T>), dim3(grid), dim3(T), 0, st, x, e, u, sweeps, sweep_total, n, sh.ldo, tol, \
+                       pass, count, uprev, ctr_stride, status, sh.max_sweeps)
 #define WN_EIG_CASE(NN, DEF)                   \
     if (n <= NN) {                             \
         const int t = tpb ? tpb : DEF;         \
@@ -1217,17 +1213,9 @@ hipError_t wn_launch_eig_pass_f64(const double *x, double *e, double2 *u, int *s
     WN_EIG_CASE(64, 256)
 #undef WN_EIG_CASE
 #undef WN_EIG_LAUNCH
-    // n in (64, 128]: V in global memory, 512 threads, cold start only
-    if (n <= 128 && pass < 0) {
-        if (!vscratch) return hipErrorInvalidValue;
-        if (n <= 96)
-            hipLaunchKernelGGL((k_eig_f64<96, 512, true>), dim3(grid), dim3(512), 0, st, x, e, u, sweeps, sweep_total, n,
-                               sh.ldo, tol, pass, count, vscratch, nullptr, ctr_stride, status, sh.max_sweeps);
-        else
-            hipLaunchKernelGGL((k_eig_f64<128, 512, true>), dim3(grid), dim3(512), 0, st, x, e, u, sweeps, sweep_total,
-                               n, sh.ldo, tol, pass, count, vscratch, nullptr, ctr_stride, status, sh.max_sweeps);
-        return hipGetLastError();
-    }
+    // n in (64, 128]: deferred-eigenvector solver, cold start only
+    if (n <= 128 && pass < 0) return wn_launch_eig_big(x, e, u, sweeps, sweep_total, count, sh, st, vscratch, ctr_stride,
+                                                       status);
     return hipErrorInvalidValue;
 }
 
@@ -1727,13 +1715,13 @@ __global__ __launch_bounds__(256) void k_rotate_big(double *__restrict__ x, cons
     }
 }
 
-size_t wn_eig_scratch_elems(int n, int count) {
+size_t wn_eig_scratch_elems(int n, int count) { return wn_eig_big_scratch_elems(n, count); }
+
+size_t wn_rotate_scratch_elems(int n, int count) {
     if (n <= 64) return 0;
     const size_t N = n <= 96 ? 96 : 128;
     return N * N * (size_t)count;
 }
-
-size_t wn_rotate_scratch_elems(int n, int count) { return wn_eig_scratch_elems(n, count); }
 
 template <bool WARM>
 static void launch_rotate_mfma(double *x, const double2 *u, int count, const WnShape &sh, hipStream_t st,
