@@ -1658,7 +1658,63 @@ __global__ __launch_bounds__(256) void k_rotate_sc(double *__restrict__ x, const
 
 // Rotation for n in (64, 128]: U and X are read through the L2 as MFMA operands and U^+ X is staged
 // in a per-k scratch [NP][NP] (the LDS cannot hold U, X and U^+ X at these sizes).  One 256-thread
-// workgroup per k, channels 1 .. C-1 in turn; operand layouts as in k_rotate_mfma.
+// workgroup per k, channels 1 .. C-1 in turn; the (NP/16)^2 output tiles of each product are dealt to
+// the 4 waves.  Complex products as three real products (Gauss): for A B with A = (ar, ai), B = (br, bi),
+// P1 = ar br, P2 = ai bi, P3 = (ar + ai)(br + bi); real part P1 - P2, imaginary part P3 - P1 - P2.  Four K
+// steps of operands are loaded per iteration, the next iteration's before this one's products.
+// One 16 x 16 tile of A B over K = NP for k_rotate_big.  Lane (lr, lq) reads A at a[kk * a_ks] (this lane's
+// row; valid while a_ok and kk < a_kmax; imaginary part times a_im) and B at b[kk * b_ks] (this lane's
+// column; valid while b_ok and kk < b_kmax), kk = k0 + lq.
+template <int NP>
+__device__ __forceinline__ void big_tile(const double2 *__restrict__ a, int a_ks, bool a_ok, int a_kmax, double a_im,
+                                         const double2 *__restrict__ b, int b_ks, bool b_ok, int b_kmax, int lq,
+                                         f64x4 &p1, f64x4 &p2, f64x4 &p3) {
+    const double2 zero = make_double2(0.0, 0.0);
+    p1 = (f64x4){0.0, 0.0, 0.0, 0.0};
+    p2 = p1;
+    p3 = p1;
+    double2 av[4], bv[4];
+#pragma unroll
+    for (int s4 = 0; s4 < 4; ++s4) {
+        const int kk = 4 * s4 + lq;
+        av[s4] = (a_ok && kk < a_kmax) ? a[kk * a_ks] : zero;
+        bv[s4] = (b_ok && kk < b_kmax) ? b[kk * b_ks] : zero;
+    }
+#pragma unroll
+    for (int k0 = 0; k0 < NP; k0 += 16) {
+        double2 an[4], bn[4];
+        if (k0 + 16 < NP) {
+#pragma unroll
+            for (int s4 = 0; s4 < 4; ++s4) {
+                const int kk = k0 + 16 + 4 * s4 + lq;
+                an[s4] = (a_ok && kk < a_kmax) ? a[kk * a_ks] : zero;
+                bn[s4] = (b_ok && kk < b_kmax) ? b[kk * b_ks] : zero;
+            }
+        }
+#pragma unroll
+        for (int s4 = 0; s4 < 4; ++s4) {
+            const double ar = av[s4].x, ai = a_im * av[s4].y;
+            const double2 bq = bv[s4];
+            p1 = __builtin_amdgcn_mfma_f64_16x16x4f64(ar, bq.x, p1, 0, 0, 0);
+            p2 = __builtin_amdgcn_mfma_f64_16x16x4f64(ai, bq.y, p2, 0, 0, 0);
+            p3 = __builtin_amdgcn_mfma_f64_16x16x4f64(ar + ai, bq.x + bq.y, p3, 0, 0, 0);
+        }
+        if (k0 + 16 < NP) {
+#pragma unroll
+            for (int s4 = 0; s4 < 4; ++s4) {
+                av[s4] = an[s4];
+                bv[s4] = bn[s4];
+            }
+        }
+    }
+}
+
+// Rotation for n in (64, 128]: U and X are read through the L2 as MFMA operands and U^+ X is staged
+// in a per-k scratch [NP][NP] (the LDS cannot hold U, X and U^+ X at these sizes).  One 256-thread
+// workgroup per k, channels 1 .. C-1 in turn; the (NP/16)^2 output tiles of each product are dealt to
+// the 4 waves.  Complex products as three real products (Gauss): for A B with A = (ar, ai), B = (br, bi),
+// P1 = ar br, P2 = ai bi, P3 = (ar + ai)(br + bi); real part P1 - P2, imaginary part P3 - P1 - P2.  Four K
+// steps of operands are loaded per iteration, the next iteration's before this one's products.
 template <int NP>
 __global__ __launch_bounds__(256) void k_rotate_big(double *__restrict__ x, const double2 *__restrict__ u, int n, int C,
                                                    int ldo, double2 *__restrict__ scratch) {
@@ -1670,44 +1726,30 @@ __global__ __launch_bounds__(256) void k_rotate_big(double *__restrict__ x, cons
     const double2 *uk = u + (size_t)k * n * n;
     double2 *tk = scratch + (size_t)k * NP * NP;
     double2 *xk = (double2 *)(x + (size_t)k * ldo);
-    const double2 zero = make_double2(0.0, 0.0);
     for (int c = 1; c < C; ++c) {
         double2 *xc = xk + (size_t)c * n * n;
-        // T = U^+ X into the scratch (rows and columns beyond n are zero)
+        // T = U^+ X into the scratch (rows and columns beyond n are zero); (U^+)(r, kk) = conj U(kk, r)
         for (int tile = wave; tile < nt * nt; tile += 4) {
             const int r0 = (tile / nt) * 16, c0 = (tile % nt) * 16;
-            f64x4 ar = {0.0, 0.0, 0.0, 0.0}, ai = {0.0, 0.0, 0.0, 0.0};
-            for (int k0 = 0; k0 < NP; k0 += 4) {
-                const int kr = k0 + lq;
-                const double2 uv = (kr < n && r0 + lr < n) ? uk[kr * n + r0 + lr] : zero;
-                const double2 xv = (kr < n && c0 + lr < n) ? xc[kr * n + c0 + lr] : zero;
-                ar = __builtin_amdgcn_mfma_f64_16x16x4f64(uv.x, xv.x, ar, 0, 0, 0);
-                ar = __builtin_amdgcn_mfma_f64_16x16x4f64(uv.y, xv.y, ar, 0, 0, 0);
-                ai = __builtin_amdgcn_mfma_f64_16x16x4f64(uv.x, xv.y, ai, 0, 0, 0);
-                ai = __builtin_amdgcn_mfma_f64_16x16x4f64(-uv.y, xv.x, ai, 0, 0, 0);
-            }
+            f64x4 p1, p2, p3;
+            big_tile<NP>(uk + min(r0 + lr, n - 1), n, r0 + lr < n, n, -1.0, xc + min(c0 + lr, n - 1), n, c0 + lr < n,
+                         n, lq, p1, p2, p3);
 #pragma unroll
-            for (int q = 0; q < 4; ++q) tk[(r0 + lq + 4 * q) * NP + c0 + lr] = make_double2(ar[q], ai[q]);
+            for (int q = 0; q < 4; ++q)
+                tk[(r0 + lq + 4 * q) * NP + c0 + lr] = make_double2(p1[q] - p2[q], p3[q] - p1[q] - p2[q]);
         }
         __threadfence_block();
         __syncthreads();
         // X <- T U
         for (int tile = wave; tile < nt * nt; tile += 4) {
             const int r0 = (tile / nt) * 16, c0 = (tile % nt) * 16;
-            f64x4 yr = {0.0, 0.0, 0.0, 0.0}, yi = {0.0, 0.0, 0.0, 0.0};
-            for (int k0 = 0; k0 < NP; k0 += 4) {
-                const int kc = k0 + lq;
-                const double2 tv = tk[(r0 + lr) * NP + kc];
-                const double2 uv = (kc < n && c0 + lr < n) ? uk[kc * n + c0 + lr] : zero;
-                yr = __builtin_amdgcn_mfma_f64_16x16x4f64(tv.x, uv.x, yr, 0, 0, 0);
-                yr = __builtin_amdgcn_mfma_f64_16x16x4f64(-tv.y, uv.y, yr, 0, 0, 0);
-                yi = __builtin_amdgcn_mfma_f64_16x16x4f64(tv.x, uv.y, yi, 0, 0, 0);
-                yi = __builtin_amdgcn_mfma_f64_16x16x4f64(tv.y, uv.x, yi, 0, 0, 0);
-            }
+            f64x4 p1, p2, p3;
+            big_tile<NP>(tk + (r0 + lr) * NP, 1, true, NP, 1.0, uk + min(c0 + lr, n - 1), n, c0 + lr < n, n, lq, p1, p2,
+                         p3);
 #pragma unroll
             for (int q = 0; q < 4; ++q) {
                 const int row = r0 + lq + 4 * q, col = c0 + lr;
-                if (row < n && col < n) xc[row * n + col] = make_double2(yr[q], yi[q]);
+                if (row < n && col < n) xc[row * n + col] = make_double2(p1[q] - p2[q], p3[q] - p1[q] - p2[q]);
             }
         }
         __threadfence_block();

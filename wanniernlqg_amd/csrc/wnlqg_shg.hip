@@ -225,20 +225,42 @@ __global__ __launch_bounds__(64) void k_shg_vtx(const double *__restrict__ x, co
                     fl = make_double2(-1.0, 0.0); break;                            //    - Q'_a Y_b
             default: L = dk + (size_t)(kShgY + bb) * n2; R = dk + (size_t)(kShgQ + a) * n2; break;   // Y_b Q'_a
         }
-        for (int p0 = 0; p0 < n; p0 += 4) {
-            const int p = p0 + lk, pc = min(p, n - 1);
-            double2 lv = L[row * n + pc], rv = R[pc * n + col];
-            const bool lz = p >= n || (zl && p == row), rz = p >= n || (zr && p == col);
-            lv = lz ? make_double2(0.0, 0.0) : make_double2(lv.x * fl.x - lv.y * fl.y, lv.x * fl.y + lv.y * fl.x);
-            rv = rz ? make_double2(0.0, 0.0) : make_double2(rv.x * fr.x - rv.y * fr.y, rv.x * fr.y + rv.y * fr.x);
-            if (blk < 2) {
-                f1 = __builtin_amdgcn_mfma_f64_16x16x4f64(lv.x, rv.x, f1, 0, 0, 0);
-                f2 = __builtin_amdgcn_mfma_f64_16x16x4f64(lv.y, rv.y, f2, 0, 0, 0);
-                f3 = __builtin_amdgcn_mfma_f64_16x16x4f64(lv.x + lv.y, rv.x + rv.y, f3, 0, 0, 0);
-            } else {
-                g1 = __builtin_amdgcn_mfma_f64_16x16x4f64(lv.x, rv.x, g1, 0, 0, 0);
-                g2 = __builtin_amdgcn_mfma_f64_16x16x4f64(lv.y, rv.y, g2, 0, 0, 0);
-                g3 = __builtin_amdgcn_mfma_f64_16x16x4f64(lv.x + lv.y, rv.x + rv.y, g3, 0, 0, 0);
+        // four K steps per iteration; the next iteration's operands are loaded before this one's products
+        double2 lv[4], rv[4];
+        auto load = [&](int p0, double2 (&lo)[4], double2 (&ro)[4]) {
+#pragma unroll
+            for (int u = 0; u < 4; ++u) {
+                const int pc = min(p0 + 4 * u + lk, n - 1);
+                lo[u] = L[row * n + pc];
+                ro[u] = R[pc * n + col];
+            }
+        };
+        load(0, lv, rv);
+        for (int p0 = 0; p0 < n; p0 += 16) {
+            double2 ln[4], rn[4];
+            if (p0 + 16 < n) load(p0 + 16, ln, rn);
+#pragma unroll
+            for (int u = 0; u < 4; ++u) {
+                const int p = p0 + 4 * u + lk;
+                const bool lz = p >= n || (zl && p == row), rz = p >= n || (zr && p == col);
+                const double2 a = lz ? make_double2(0.0, 0.0)
+                                     : make_double2(lv[u].x * fl.x - lv[u].y * fl.y, lv[u].x * fl.y + lv[u].y * fl.x);
+                const double2 c = rz ? make_double2(0.0, 0.0)
+                                     : make_double2(rv[u].x * fr.x - rv[u].y * fr.y, rv[u].x * fr.y + rv[u].y * fr.x);
+                if (blk < 2) {
+                    f1 = __builtin_amdgcn_mfma_f64_16x16x4f64(a.x, c.x, f1, 0, 0, 0);
+                    f2 = __builtin_amdgcn_mfma_f64_16x16x4f64(a.y, c.y, f2, 0, 0, 0);
+                    f3 = __builtin_amdgcn_mfma_f64_16x16x4f64(a.x + a.y, c.x + c.y, f3, 0, 0, 0);
+                } else {
+                    g1 = __builtin_amdgcn_mfma_f64_16x16x4f64(a.x, c.x, g1, 0, 0, 0);
+                    g2 = __builtin_amdgcn_mfma_f64_16x16x4f64(a.y, c.y, g2, 0, 0, 0);
+                    g3 = __builtin_amdgcn_mfma_f64_16x16x4f64(a.x + a.y, c.x + c.y, g3, 0, 0, 0);
+                }
+            }
+#pragma unroll
+            for (int u = 0; u < 4; ++u) {
+                lv[u] = ln[u];
+                rv[u] = rn[u];
             }
         }
     }
