@@ -95,13 +95,14 @@ def pairs_flops_per_pair(n: int) -> float:
 TRAFFIC_PROFILE = os.path.join(ROOT, "profiles", "kloop_n32_64cube_hbm_traffic.md")
 
 
-def measured_traffic_per_k():
-    """HBM bytes per k point of each k-loop kernel, from the committed PMC summary (FETCH_SIZE and
-    WRITE_SIZE, each scaled by the reading of a plain copy of known size); {} when absent."""
+def measured_traffic_per_k(precision="fp64"):
+    """HBM bytes per k point of each k-loop kernel of the given precision form, from the committed PMC summary
+    (FETCH_SIZE and WRITE_SIZE, each scaled by the reading of a plain copy of known size); {} when absent."""
+    tag = "traffic_bytes_per_k_%s:" % ("mixed" if precision == "mixed" else "fp64")
     try:
         with open(TRAFFIC_PROFILE) as fh:
             for line in fh:
-                if line.startswith("traffic_bytes_per_k:"):
+                if line.startswith(tag):
                     return {kv.split("=")[0]: float(kv.split("=")[1]) for kv in line.split(":", 1)[1].split()}
     except OSError:
         pass
@@ -119,15 +120,17 @@ def roofline(st, n, nk, precision):
         "pairs": st["pairs_ms"],
     }
     dominant = max(kern, key=kern.get)
-    traffic = measured_traffic_per_k()
+    traffic = measured_traffic_per_k(precision)
     if dominant == "eig":
         launches = max(1, st["eig_launches"])
         avg_s = 1e-3 * st["eig_ms"] / launches
         flops = jacobi_flops_per_k(n, st["mean_sweeps"]) * nk / launches
-        roof = {"kernel": "k_eig_f64 (Jacobi on the FP64 VALU, both warm-start passes; peak = FP64 vector peak)",
+        # the default eigensolver: register-resident eigenvectors for n <= 32, the LDS kernel up to 64
+        kname = "k_eig_rv" if n <= 32 else "k_eig_f64"
+        roof = {"kernel": kname + " (Jacobi on the FP64 VALU, all warm-start passes; peak = FP64 vector peak)",
                 "bound": "valu", "achieved": flops / avg_s / 1e12, "peak": FP64_PEAK_TFLOPS,
                 "unit": "TFLOP/s", "mean_sweeps": st["mean_sweeps"]}
-        tkey = "k_eig_f64"
+        tkey = kname
     elif dominant == "rotate":
         launches = nb
         avg_s = 1e-3 * st["rotate_ms"] / launches
@@ -177,7 +180,8 @@ def main():
     ap.add_argument("--no-compare", action="store_true", help="skip the plain-FP64 and mixed direct comparison runs")
     ap.add_argument("--cpu-sample", type=int, default=1024)
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--form", default="headline", choices=["headline", "geometry", "shg", "batch", "learned", "large", "symmetry", "copy"])
+    ap.add_argument("--form", default="headline", choices=["headline", "geometry", "shg", "batch", "learned", "large", "symmetry", "copy",
+                                                         "fft"])
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}); the k-mesh would shard by "
@@ -587,8 +591,63 @@ def form_copy(args):
     print(json.dumps(out), flush=True)
 
 
+def form_fft(args):
+    """Interpolation by Mixed-FFT against the separable contraction on the headline mesh (n_w = 32, |R_i| <= 5,
+    22 channels, 128^3): the reference's Mixed-FFT (MixedFourierMatrixElements.jl:954-1123) splits the mesh
+    into NKdiv^3 = 8^3 offsets of an NKFFT^3 = 16^3 grid; per offset it multiplies the R-space channels by
+    the offset's phases, places them on the 16^3 grid (R mod 16) and transforms.  Timed here on the GPU with
+    torch.fft (rocFFT) per offset, including the transposition of the result to per-k matrices that the
+    eigensolver reads (the separable contraction writes them in that layout directly), and multiplied by the
+    512 offsets; the separable contraction is the engine's own interpolation time on the same mesh (FP64)."""
+    import torch
+    from wanniernlqg_amd import engine as E
+    from wanniernlqg_amd.model import config_model
+    if not torch.cuda.is_available():
+        raise SystemExit("the FFT comparison runs on the GPU")
+    model = config_model(1)
+    n, nr = model.num_orbitals, model.num_r_vectors
+    nkfft, nkdiv = 16, 8
+    chans = 22
+    dev = torch.device("cuda")
+    rv = torch.as_tensor(model.r_vectors, dtype=torch.int64, device=dev)
+    cell = ((rv[:, 0] % nkfft) * nkfft + (rv[:, 1] % nkfft)) * nkfft + (rv[:, 2] % nkfft)
+    g = torch.Generator(device="cpu").manual_seed(5)
+    fr = torch.randn(chans * n * n, nr, dtype=torch.complex128, generator=g).to(dev)
+    grid = torch.empty(chans * n * n, nkfft ** 3, dtype=torch.complex128, device=dev)
+
+    def offset(d):
+        ph = torch.exp(2j * np.pi * (rv.to(torch.float64) @ torch.tensor(d, dtype=torch.float64, device=dev))
+                       / (nkdiv * nkfft))
+        grid.zero_()
+        grid[:, cell] = fr * ph[None, :]
+        out = torch.fft.fftn(grid.view(-1, nkfft, nkfft, nkfft), dim=(1, 2, 3))
+        return out.view(chans * n * n, -1).transpose(0, 1).contiguous()     # per-k matrices
+
+    offsets = [(i, j, k) for i in range(2) for j in range(2) for k in range(2)]
+    offset(offsets[0])
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for d in offsets:
+        offset(d)
+    torch.cuda.synchronize()
+    per_offset = (time.perf_counter() - t0) / len(offsets)
+    fft_mesh_ms = 1e3 * per_offset * nkdiv ** 3
+    task = E.EngineTask(photon_energies=np.linspace(0.0, 6.0, 400), broadening=0.01, shift_current=True,
+                        injection_current=True, precision="fp64")
+    with E.Engine(model, task) as eng:
+        _, _, st = _timed_runs(eng, (128, 128, 128), 1, 1)
+    out = {"metric": "interpolation ms per 128^3 mesh (n_w=32, N_R=1331, 22 channels)", "value": fft_mesh_ms,
+           "unit": "ms", "higher_is_better": False, "n_gpus": 1, "dtype": "f64", "data": "synthetic",
+           "config": {"workload": "Mixed-FFT NKdiv=8 x NKFFT=16 per axis (torch.fft / rocFFT, 8 offsets timed, x512) "
+                                  "against the engine's separable contraction", "kpoints": 128 ** 3},
+           "fft_ms_per_offset": 1e3 * per_offset, "fft_mesh_ms": fft_mesh_ms,
+           "separable_mesh_ms": st["interp_ms"], "ratio_fft_over_separable": fft_mesh_ms / st["interp_ms"]}
+    print(json.dumps(out), flush=True)
+
+
 FORMS = {"geometry": form_geometry, "shg": form_shg, "batch": form_batch, "learned": form_learned,
-         "large": form_large, "symmetry": form_symmetry, "copy": form_copy}
+         "large": form_large, "symmetry": form_symmetry, "copy": form_copy,
+         "fft": form_fft}
 
 
 if __name__ == "__main__":

@@ -28,7 +28,7 @@
 extern "C" {
 #endif
 
-#define WNLQG_ABI_VERSION 6
+#define WNLQG_ABI_VERSION 7
 
 /* Status codes. */
 #define WNLQG_OK 0
@@ -68,9 +68,16 @@ extern "C" {
                                         (~1e-5 of the spectrum) does not average out over the mesh */
 
 /* Interpolation backends for wnlqg_run_mesh (the role of fourier_backend, FourierExecutionPlan.jl:394-480). */
-#define WNLQG_FOURIER_AUTO 0     /* separable plane contraction on meshes */
+#define WNLQG_FOURIER_AUTO 0     /* separable plane contraction on meshes where it plans, else direct */
 #define WNLQG_FOURIER_DIRECT 1   /* k x R contraction on MFMA (the only form for arbitrary k points) */
 #define WNLQG_FOURIER_PLANES 2   /* separable: R3 sum per k3 plane, R2 sum per k2 line, R1 sum per k point */
+#define WNLQG_FOURIER_MIXED 3    /* the reference's fourier_backend = "mixed" (NKdiv x NKFFT Mixed-FFT, the same
+                                    uniform-mesh Fourier sums): the separable contraction, with the reference's
+                                    errors: WNLQG_E_UNSUPPORTED from wnlqg_create when the contraction cannot
+                                    plan the model, from wnlqg_run_mesh when it cannot plan the mesh, and from
+                                    wnlqg_run_points (weighted k-point lists are adaptive / region nodes,
+                                    which the reference evaluates with direct sums only,
+                                    FourierExecutionPlan.jl:425-428) */
 
 /*
  * Tight-binding model (TightBindingModel, reference src/Core/Models.jl:11-75; the fields of

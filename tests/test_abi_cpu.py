@@ -43,6 +43,18 @@ def test_exports_match_header():
     assert exported == declared
 
 
+def test_fourier_backends_match_header():
+    """The binding's fourier_backend names map to the header's WNLQG_FOURIER_* values (the reference's auto,
+    direct and mixed, FourierExecutionPlan.jl:63-67, plus planes); an unknown name is rejected by the library."""
+    text = open(HEADER).read()
+    values = {name.lower(): int(v) for name, v in re.findall(r"#define WNLQG_FOURIER_(\w+) (\d+)", text)}
+    assert values == E.FOURIER_BACKENDS
+    with pytest.raises(E.WnlqgError) as ei:
+        E.Engine(synthetic_model(4, 1, 0), E.EngineTask(photon_energies=np.linspace(0, 4, 20), broadening=0.02,
+                                                         fourier_backend="fft"))
+    assert ei.value.code == -1
+
+
 def _task(**kw):
     base = dict(photon_energies=np.linspace(0, 4, 20), broadening=0.02)
     base.update(kw)

@@ -369,6 +369,29 @@ def test_planes_equal_direct(name):
         assert rel(tp[key], td[key]) < 1e-11
 
 
+def test_mixed_backend_runs_the_separable_contraction():
+    """fourier_backend = "mixed" (the reference's Mixed-FFT name) runs the separable contraction on meshes,
+    equal to the direct sums to rounding, and keeps the reference's errors: a weighted k-point list (adaptive /
+    region nodes, FourierExecutionPlan.jl:425-428) and a mesh the contraction cannot plan are rejected with
+    WNLQG_E_UNSUPPORTED."""
+    model = MODELS["n32_pos"]()
+    kw = dict(photon_energies=np.linspace(0, 5, 80), broadening=0.04)
+    say("[mixed-backend] n32_pos 6x4x5 against direct")
+    with engine(model, fourier_backend="direct", **kw) as ed:
+        _, td, _ = ed.run_mesh((6, 4, 5))
+    with engine(model, fourier_backend="mixed", batch_size=128, **kw) as em:
+        _, tm, sm = em.run_mesh((6, 4, 5))
+        assert sm["separable"] == 1
+        for key in ("sc", "ic"):
+            assert rel(tm[key], td[key]) < 1e-11
+        with pytest.raises(E.WnlqgError) as ei:
+            em.run_points(np.array([[0.1, 0.2, 0.3]]), np.array([1.0]))
+        assert ei.value.code == -6
+        with pytest.raises(E.WnlqgError) as ei:
+            em.run_mesh((200, 1, 1))          # axis 0 longer than the 128-point batch
+        assert ei.value.code == -6
+
+
 @pytest.mark.parametrize("with_positions", [False, True])
 def test_point_values_equal_projector_formulation(with_positions):
     """Per-point SC and IC of the engine (eta = 0) against the projector / band-velocity formulation of the

@@ -249,11 +249,24 @@ def test_load_run_config(tmp_path):
     ('kind = "SHG"', 'kind = "SHG"\noutput = "all"'),
     ("k_mesh = [3, 2, 2]", "k_mesh = [3, 0, 2]"),
     ("photon_energy_min_ev = 0.0\n", ""),
+    ('fourier_backend = "direct"', 'fourier_backend = "fft"'),
 ])
 def test_load_run_config_rejects(tmp_path, old, new):
     assert old in BASE
     with pytest.raises(ValueError):
         R.load_run_config(_write_cfg(tmp_path, BASE.replace(old, new, 1)))
+
+
+@pytest.mark.parametrize("value,expected", [('"direct"', "direct"), ('"Mixed"', "mixed"), ('" AUTO "', "auto"),
+                                            ('"-mixed-"', "mixed")])
+def test_fourier_backend_names(tmp_path, value, expected):
+    """fourier_backend is canonicalized like the reference's keys (Config.jl:437-442: lower case, blank and hyphen
+    runs to one underscore, outer underscores stripped) and must then be direct, mixed or auto
+    (FourierExecutionPlan.jl:63-67); without the key the backend is auto."""
+    body = BASE.replace('fourier_backend = "direct"', f"fourier_backend = {value}")
+    assert R.load_run_config(_write_cfg(tmp_path, body)).fourier_backend == expected
+    body = BASE.replace('[response.execution]\nfourier_backend = "direct"\n', "")
+    assert R.load_run_config(_write_cfg(tmp_path, body)).fourier_backend == "auto"
 
 
 def test_cli_usage():

@@ -177,7 +177,7 @@ int check_geometry_task(const wnlqg_task *t) {
     if (t->precision != WNLQG_PRECISION_FP64)
         return fail(WNLQG_E_UNSUPPORTED, "BCD / AHC run in FP64 only");
     if (t->batch_size < 0) return fail(WNLQG_E_INVALID, "batch_size must be >= 0");
-    if (t->fourier_backend < WNLQG_FOURIER_AUTO || t->fourier_backend > WNLQG_FOURIER_PLANES)
+    if (t->fourier_backend < WNLQG_FOURIER_AUTO || t->fourier_backend > WNLQG_FOURIER_MIXED)
         return fail(WNLQG_E_INVALID, "unknown fourier_backend %d", t->fourier_backend);
     if (bcd) {
         if (!(t->degeneracy_threshold >= 0.0) || !std::isfinite(t->degeneracy_threshold))
@@ -236,7 +236,7 @@ int check_task(const wnlqg_task *t, int n) {
         t->precision != WNLQG_PRECISION_MIXED_HIGH && t->precision != WNLQG_PRECISION_SPLIT_BF16)
         return fail(WNLQG_E_INVALID, "unknown precision %d", t->precision);
     if (t->batch_size < 0) return fail(WNLQG_E_INVALID, "batch_size must be >= 0");
-    if (t->fourier_backend < WNLQG_FOURIER_AUTO || t->fourier_backend > WNLQG_FOURIER_PLANES)
+    if (t->fourier_backend < WNLQG_FOURIER_AUTO || t->fourier_backend > WNLQG_FOURIER_MIXED)
         return fail(WNLQG_E_INVALID, "unknown fourier_backend %d", t->fourier_backend);
     return WNLQG_OK;
 }
@@ -686,7 +686,7 @@ int32_t wnlqg_create(const wnlqg_model *model, const wnlqg_task *task, wnlqg_eng
             if (boxmap[b] >= 0) ok = false;
             boxmap[b] = r;
         }
-        if (!ok && g->backend == WNLQG_FOURIER_PLANES)
+        if (!ok && (g->backend == WNLQG_FOURIER_PLANES || g->backend == WNLQG_FOURIER_MIXED))
             return bail(fail(WNLQG_E_UNSUPPORTED, "separable contraction needs distinct R vectors in a box of span <= 16"));
         if (ok) {
             DevBuf<int> dmap;
@@ -891,6 +891,9 @@ int plan_mesh(wnlqg_engine *g, const int32_t mesh[3], int32_t n_lanes) {
     // batches: contiguous k ranges.  Separable: k2 ranges of one k3 plane (rows k2*N1 + k1).
     p.batches.clear();
     p.sep = g->separable && mesh[0] <= g->batch;
+    if (!p.sep && g->backend == WNLQG_FOURIER_MIXED)
+        return fail(WNLQG_E_UNSUPPORTED, "fourier_backend mixed: the separable contraction needs mesh axis 0 (%d) "
+                    "within the batch (%d)", mesh[0], g->batch);
     if (p.sep) {
         const int k2step = std::max(1, std::min(mesh[1], g->batch / mesh[0]));
         const long long plane = (long long)mesh[0] * mesh[1];
@@ -1270,6 +1273,9 @@ int32_t wnlqg_run_points(wnlqg_engine *g, const double *kpoints, const double *w
     if (n_lanes < 1 || n_lanes > m) return fail(WNLQG_E_INVALID, "n_lanes must be in [1, m]");
     if (g->family != 0)
         return fail(WNLQG_E_INVALID, "weighted point lists integrate shift / injection current engines only");
+    if (g->backend == WNLQG_FOURIER_MIXED)
+        return fail(WNLQG_E_UNSUPPORTED, "weighted k-point lists (adaptive / region nodes) support direct evaluation; "
+                    "use fourier_backend direct or auto");
     if (g->kpc < 1 || g->kpc > WN_ACC_KMAX) return fail(WNLQG_E_INVALID, "chunk size %d outside [1, %d]", g->kpc, WN_ACC_KMAX);
     for (int64_t i = 0; i < m; ++i)
         if (!std::isfinite(weights[i])) return fail(WNLQG_E_NONFINITE, "non-finite weight %lld", (long long)i);
@@ -1353,7 +1359,7 @@ int32_t wnlqg_run_mesh_reduced(wnlqg_engine *g, const int32_t mesh[3], const int
     if (rc) return rc;
     const MeshPlan &p = g->plan;
     if (!p.sep)
-        return fail(WNLQG_E_INVALID, "reduced meshes need the separable interpolation (fourier_backend auto or planes)");
+        return fail(WNLQG_E_INVALID, "reduced meshes need the separable interpolation (fourier_backend auto, planes or mixed)");
     const long long nbatch = (long long)p.batches.size();
     // representatives of each batch: a contiguous range of the sorted list
     std::vector<long long> rb(nbatch + 1, m);

@@ -90,7 +90,9 @@ class Stats(C.Structure):
 _lib = None
 
 
-ABI_VERSION = 6   # WNLQG_ABI_VERSION of include/wnlqg.h
+ABI_VERSION = 7   # WNLQG_ABI_VERSION of include/wnlqg.h
+# fourier_backend names -> WNLQG_FOURIER_*: the reference's "auto", "direct", "mixed" and this build's "planes"
+FOURIER_BACKENDS = {"auto": 0, "direct": 1, "planes": 2, "mixed": 3}
 
 
 def load_library(path: str = LIB_PATH) -> C.CDLL:
@@ -185,7 +187,7 @@ class EngineTask:
     band_window_size: int = -1
     precision: str = "fp64"
     batch_size: int = 0
-    fourier_backend: str = "auto"      # meshes: "auto"/"planes" (separable) or "direct"
+    fourier_backend: str = "auto"      # meshes: "auto"/"planes"/"mixed" (separable) or "direct"
 
 
 class Engine:
@@ -233,7 +235,7 @@ class Engine:
                 "mixed6": WNLQG_PRECISION_MIXED_HIGH, "bf16x3": WNLQG_PRECISION_SPLIT_BF16}[task.precision]
         t = _Task(q, self._om.shape[0], _ptr(self._om), bt, task.broadening, task.fermi_energy, task.temperature,
                   task.denominator_regularization, task.transition_window_factor, task.band_window_size, prec,
-                  task.batch_size, {"auto": 0, "direct": 1, "planes": 2}[task.fourier_backend],
+                  task.batch_size, FOURIER_BACKENDS.get(task.fourier_backend, -1),
                   self._mu.shape[0] if geometry else 0, _ptr(self._mu), task.fermi_surface_broadening,
                   task.degeneracy_threshold, 0 if task.shg_components is None else len(comps),
                   _ptr(self._comps, C.c_int32), task.shg_low_frequency_broadening,

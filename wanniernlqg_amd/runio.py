@@ -17,8 +17,9 @@ Supported run-file keys (others are rejected, so that a run never silently ignor
 ``[model] source = "wannier", model_file``; ``[response] photon_energies_ev`` or
 ``photon_energy_{min,max}_ev`` with ``photon_energy_step_ev`` or ``photon_energy_num_points``,
 ``fermi_energy_ev``, ``temperature_k``; ``[response.sampling] kind = "bz_mesh", k_mesh,
-spatial_dimension = 3``; ``[response.execution] fourier_backend`` (any value: meshes always use the
-engine's separable contraction); ``[response.tasks.<id>] kind`` in SC / shift_current, IC /
+spatial_dimension = 3``; ``[response.execution] fourier_backend`` = "auto", "direct" or "mixed" (the reference's
+names, FourierExecutionPlan.jl:63-67; "mixed" runs the engine's separable contraction, which evaluates the same
+uniform-mesh Fourier sums as the reference's Mixed-FFT); ``[response.tasks.<id>] kind`` in SC / shift_current, IC /
 injection_current, SHG / second_harmonic_generation (with the selectors ``output`` = total / terms /
 both and ``response`` = conductivity / susceptibility / both), ``method = "Conventional"``, optional
 ``[response.tasks.<id>.numerics]`` with ``broadening``, ``broadening_type``,
@@ -237,6 +238,7 @@ class RunConfig:
     spatial_dimension: int
     tasks: List[RunTask]
     digits: Optional[int] = None
+    fourier_backend: str = "auto"
 
 
 def _check_keys(table: dict, allowed: set, path: str):
@@ -286,6 +288,12 @@ def load_run_config(path: str) -> RunConfig:
     if dim != 3:
         raise ValueError("response.sampling.spatial_dimension must be 3 for this runner (the reference's default is 2)")
     _check_keys(resp.get("execution", {}), {"fourier_backend"}, "response.execution")
+    # normalized_fourier_backend (FourierExecutionPlan.jl:63-67) of the key canonicalized as _canonical_key
+    # (Config.jl:437-442): lower case, runs of blanks and hyphens to one underscore, outer underscores removed
+    backend = re.sub(r"_+", "_", re.sub(r"[\s\-]+", "_", str(resp.get("execution", {}).get("fourier_backend", "auto"))
+                                       .strip().lower())).strip("_")
+    if backend not in ("auto", "direct", "mixed"):
+        raise ValueError(f"fourier_backend={backend!r} is invalid; use direct, mixed, or auto.")
     tasks = []
     for tid, tab in sorted(resp.get("tasks", {}).items()):
         _check_keys(tab, {"kind", "method", "numerics", "output", "response"}, f"response.tasks.{tid}")
@@ -325,7 +333,7 @@ def load_run_config(path: str) -> RunConfig:
                      output_root=os.path.normpath(os.path.join(base, out.get("output_root", "") or ".")),
                      photon_energies=omega, fermi_energy=float(resp.get("fermi_energy_ev", 0.0)),
                      temperature=temperature, k_mesh=tuple(int(v) for v in samp["k_mesh"]), spatial_dimension=dim, tasks=tasks,
-                     digits=None if digits is None else int(digits))
+                     digits=None if digits is None else int(digits), fourier_backend=backend)
 
 
 def run(path: str) -> List[str]:
@@ -349,13 +357,13 @@ def run(path: str) -> List[str]:
                       broadening=float(num["broadening"]), broadening_type=str(num["broadening_type"]),
                       degeneracy_threshold=float(num["degeneracy_threshold"]))
         if family == "shg":
-            task = E.EngineTask(second_harmonic=True,
+            task = E.EngineTask(second_harmonic=True, fourier_backend=cfg.fourier_backend,
                                 shg_low_frequency_broadening=float(num["low_frequency_broadening"]),
                                 shg_intermediate_regularization=float(num["intermediate_regularization"]),
                                 shg_eta_correction=bool(num["eta_correction"]), **common)
         else:
             quantities = {t.quantity for t in tasks}
-            task = E.EngineTask(shift_current="shift_current" in quantities,
+            task = E.EngineTask(shift_current="shift_current" in quantities, fourier_backend=cfg.fourier_backend,
                                 injection_current="injection_current" in quantities,
                                 denominator_regularization=float(num["denominator_regularization"]),
                                 transition_window_factor=float(num["transition_window_factor"]),
