@@ -69,20 +69,24 @@ def test_large_spectrum_and_channels(n):
         assert np.max(np.abs(ch[:, c] - ref)) < 1e-12 * max(np.max(np.abs(ref)), 1.0)
 
 
-@pytest.mark.parametrize("n1,n2", [(48, 48), (40, 40), (64, 64)])
-def test_large_shg_direct_sum(n1, n2):
+@pytest.mark.parametrize("n1,n2,mesh,batch", [(48, 48, (3, 2, 2), 0), (40, 40, (3, 2, 2), 0), (64, 64, (3, 2, 2), 0),
+                                               (48, 48, (16, 8, 3), 128)])
+def test_large_shg_direct_sum(n1, n2, mesh, batch):
+    """SHG of a direct sum of two decoupled models equals the sum of the parts (n > 64 against the n <= 64
+    path).  The 3 x 2 x 2 mesh runs in one batch (the eigensolver's warm-start groups of 4); the 16 x 8 x 3
+    mesh with 128-point batches runs one k3 plane per batch, so planes 2 and 3 start every point from the same
+    point of the previous plane."""
     m1 = synthetic_model(n1, 1, 3, position_scale=0.1)
     m2 = synthetic_model(n2, 1, 4, position_scale=0.1)
     m = direct_sum(m1, m2)
-    mesh = (3, 2, 2)
     k = np.random.default_rng(9).uniform(0, 1, (5, 3))
-    say(f"[large] SHG direct sum {n1}+{n2}")
-    with E.Engine(m, shg_task()) as eng:
+    say(f"[large] SHG direct sum {n1}+{n2}, mesh {mesh}, batch {batch}")
+    with E.Engine(m, shg_task(batch_size=batch)) as eng:
         lanes, total, st = eng.run_mesh(mesh)
         pts = eng.eval_points(k)["shg"]
     parts = []
     for part in (m1, m2):
-        with E.Engine(part, shg_task()) as eng:
+        with E.Engine(part, shg_task(batch_size=batch)) as eng:
             parts.append((eng.run_mesh(mesh), eng.eval_points(k)["shg"]))
     lane_sum = parts[0][0][0]["shg"] + parts[1][0][0]["shg"]
     total_sum = parts[0][0][1]["shg"] + parts[1][0][1]["shg"]

@@ -80,3 +80,50 @@ __device__ __forceinline__ double wn_reg_inverse(double d, double eta) {
     double sd = d / s, sw = eta / s;
     return (sd / (sd * sd + sw * sw)) / s;
 }
+
+// One 16 x 16 tile of a complex product A B over K = NP on FP64 MFMA (three real products per complex product).  Lane (lr, lq) reads A at a[kk * a_ks] (this lane's
+// row; valid while a_ok and kk < a_kmax; imaginary part times a_im) and B at b[kk * b_ks] (this lane's
+// column; valid while b_ok and kk < b_kmax), kk = k0 + lq.
+template <int NP>
+__device__ __forceinline__ void wn_mfma_tile_c64(const double2 *__restrict__ a, int a_ks, bool a_ok, int a_kmax, double a_im,
+                                         const double2 *__restrict__ b, int b_ks, bool b_ok, int b_kmax, int lq,
+                                         f64x4 &p1, f64x4 &p2, f64x4 &p3) {
+    const double2 zero = make_double2(0.0, 0.0);
+    p1 = (f64x4){0.0, 0.0, 0.0, 0.0};
+    p2 = p1;
+    p3 = p1;
+    double2 av[4], bv[4];
+#pragma unroll
+    for (int s4 = 0; s4 < 4; ++s4) {
+        const int kk = 4 * s4 + lq;
+        av[s4] = (a_ok && kk < a_kmax) ? a[kk * a_ks] : zero;
+        bv[s4] = (b_ok && kk < b_kmax) ? b[kk * b_ks] : zero;
+    }
+#pragma unroll
+    for (int k0 = 0; k0 < NP; k0 += 16) {
+        double2 an[4], bn[4];
+        if (k0 + 16 < NP) {
+#pragma unroll
+            for (int s4 = 0; s4 < 4; ++s4) {
+                const int kk = k0 + 16 + 4 * s4 + lq;
+                an[s4] = (a_ok && kk < a_kmax) ? a[kk * a_ks] : zero;
+                bn[s4] = (b_ok && kk < b_kmax) ? b[kk * b_ks] : zero;
+            }
+        }
+#pragma unroll
+        for (int s4 = 0; s4 < 4; ++s4) {
+            const double ar = av[s4].x, ai = a_im * av[s4].y;
+            const double2 bq = bv[s4];
+            p1 = __builtin_amdgcn_mfma_f64_16x16x4f64(ar, bq.x, p1, 0, 0, 0);
+            p2 = __builtin_amdgcn_mfma_f64_16x16x4f64(ai, bq.y, p2, 0, 0, 0);
+            p3 = __builtin_amdgcn_mfma_f64_16x16x4f64(ar + ai, bq.x + bq.y, p3, 0, 0, 0);
+        }
+        if (k0 + 16 < NP) {
+#pragma unroll
+            for (int s4 = 0; s4 < 4; ++s4) {
+                av[s4] = an[s4];
+                bv[s4] = bn[s4];
+            }
+        }
+    }
+}

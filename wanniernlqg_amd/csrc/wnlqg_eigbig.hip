@@ -50,19 +50,33 @@ __device__ __forceinline__ int eigb_p(int i, int j) {
     return (int)(__umul24((unsigned)i, (unsigned)(2 * N - i + 1)) >> 1) + (j - i);
 }
 
+// k point of slot s of a pass: pass -1 / 2 every point; pass 0 the warm-start group anchors; pass 1 the other
+// points of each group (anchor set); a slot past the last point of a short group maps to k >= count.
+__device__ __forceinline__ int eigb_point(int s, int pass, int count, int *anchor) {
+    if (pass == 0 || pass == 1) return wn_warm_k(s, pass, count, anchor);
+    *anchor = -1;
+    return s;
+}
+
+int eigb_slots(int pass, int count) {
+    const int groups = (count + WN_WARM_G - 1) / WN_WARM_G;
+    return pass == 0 ? groups : (pass == 1 ? groups * (WN_WARM_G - 1) : count);
+}
+
 }  // namespace
 
 template <int N, int TPB>
 __global__ __launch_bounds__(TPB, 1) void k_eigb_sweeps(const double *__restrict__ x, int n, int ldo, double tol,
                                                         int count, double2 *__restrict__ scr, int chunk,
-                                                        int max_sweeps) {
+                                                        int max_sweeps, int pass) {
     using L = EigbLayout<N>;
     constexpr int HALF = L::HALF, NP = L::NP, NW = TPB / 64;
     __shared__ double2 H[NP];
     __shared__ double4 rot[HALF];   // (c, s, cos phi, sin phi)
     __shared__ int2 pq[HALF];
     __shared__ double red[2 * NW];
-    const int k = blockIdx.x;
+    int anchor;
+    const int k = eigb_point((int)blockIdx.x, pass, count, &anchor);
     if (k >= count) return;
     const int t = threadIdx.x;
     double2 *sk = scr + (size_t)k * L::PER_K;
@@ -257,18 +271,24 @@ __global__ __launch_bounds__(TPB, 1) void k_eigb_sweeps(const double *__restrict
     }
 }
 
-// V <- V J for the rounds recorded by the last sweep chunk.  Workgroup b of 4 waves: k point and row group
-// from b (all row groups of one k point on XCD b % 8); wave w owns rows (4 g + w) RW .. + RW - 1.
+// V <- V J for the rounds recorded by the last sweep chunk.  Workgroup b of 4 waves: slot and row group from b
+// (all row groups of one point on XCD b % 8); wave w owns rows (4 g + w) RW .. + RW - 1.  The first chunk starts
+// V at the identity, or at the warm start's eigenvectors: the group anchor's (pass 1, from u) or the same
+// point's of the previous mesh plane (pass 2, uprev).
 template <int N, int RW>
 __global__ __launch_bounds__(256) void k_eigb_vapply(int n, int count, double2 *__restrict__ scr, int chunk,
-                                                     int groups) {
+                                                     int groups, int pass, int nslots, const double2 *__restrict__ u,
+                                                     const double2 *__restrict__ uprev) {
     using L = EigbLayout<N>;
     constexpr int HALF = L::HALF;
     constexpr int ITEMS = RW * HALF;
     constexpr int IPL = (ITEMS + 63) / 64;
     __shared__ double2 rows[4][RW][N];
     const int b = blockIdx.x, xcd = b & 7, s8 = b >> 3;
-    const int k = (s8 / groups) * 8 + xcd;
+    const int slot = (s8 / groups) * 8 + xcd;
+    if (slot >= nslots) return;
+    int anchor;
+    const int k = eigb_point(slot, pass, count, &anchor);
     if (k >= count) return;
     const int g = s8 - (s8 / groups) * groups;
     const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
@@ -280,9 +300,17 @@ __global__ __launch_bounds__(256) void k_eigb_vapply(int n, int count, double2 *
     const double4 *rec = (const double4 *)(sk + L::REC);
     double2 *vg = sk;   // V row-major [N][N]
     double2 (*rw)[N] = rows[w];
+    const double2 *w0 = pass == 1 ? u + (size_t)anchor * n * n : (pass == 2 ? uprev + (size_t)k * n * n : nullptr);
     for (int e = lane; e < RW * N; e += 64) {
         const int j = e / N, c = e - (e / N) * N, i = row0 + j;
-        rw[j][c] = chunk == 0 ? make_double2(i == c ? 1.0 : 0.0, 0.0) : vg[(size_t)i * N + c];
+        double2 v;
+        if (chunk != 0)
+            v = vg[(size_t)i * N + c];
+        else if (w0)
+            v = (i < n && c < n) ? w0[(size_t)i * n + c] : make_double2(0.0, 0.0);
+        else
+            v = make_double2(i == c ? 1.0 : 0.0, 0.0);
+        rw[j][c] = v;
     }
     // items of this lane: (row j, pair slot s); the slot's pair follows the round-robin of the sweep kernel
     int js[IPL], ss[IPL], xp[IPL], xq[IPL];
@@ -346,12 +374,13 @@ __global__ __launch_bounds__(256) void k_eigb_finish(double *__restrict__ eout, 
                                                      int *__restrict__ sweeps_out,
                                                      unsigned long long *__restrict__ sweep_total, int n, int count,
                                                      const double2 *__restrict__ scr, int ctr_stride,
-                                                     int *__restrict__ status) {
+                                                     int *__restrict__ status, int pass) {
     using L = EigbLayout<N>;
     __shared__ double ev[N];
     __shared__ int rank[N];
     __shared__ double2 phase[N];
-    const int k = blockIdx.x;
+    int anchor;
+    const int k = eigb_point((int)blockIdx.x, pass, count, &anchor);
     if (k >= count) return;
     const int t = threadIdx.x;
     const double2 *sk = scr + (size_t)k * L::PER_K;
@@ -403,29 +432,106 @@ size_t wn_eig_big_scratch_elems(int n, int count) {
     return (n <= 96 ? EigbLayout<96>::PER_K : EigbLayout<128>::PER_K) * (size_t)count;
 }
 
+// Warm-start transform for 64 < n <= 128: channel 0 of a point <- W^+ H W, H the Hermitian matrix of its upper
+// triangle (the eigensolver's input), W the group anchor's eigenvectors (pass-1 points, uprev null) or the same
+// point's of the previous mesh plane (uprev).  One 256-thread workgroup per point; the Hermitian matrix and
+// W^+ H are staged in the point's solver scratch (its record and V regions, free before the solve).
+template <int N>
+__global__ __launch_bounds__(256) void k_eigb_warm(double *__restrict__ x, const double2 *__restrict__ u,
+                                                   const double2 *__restrict__ uprev, int n, int ldo, int count,
+                                                   double2 *__restrict__ scr) {
+    using L = EigbLayout<N>;
+    constexpr int nt = N / 16;
+    int anchor = -1;
+    const int k = uprev ? (int)blockIdx.x : wn_warm_k((int)blockIdx.x, 1, count, &anchor);
+    if (k >= count) return;
+    const double2 *w = uprev ? uprev + (size_t)k * n * n : u + (size_t)anchor * n * n;
+    double2 *sk = scr + (size_t)k * L::PER_K;
+    double2 *hs = sk + L::REC;   // Hermitian H [N][N]
+    double2 *ts = sk;            // W^+ H [N][N]
+    double2 *x0 = (double2 *)(x + (size_t)k * ldo);
+    const int t = threadIdx.x, lane = t & 63, wave = t >> 6, lr = lane & 15, lq = lane >> 4;
+    for (int e = t; e < N * N; e += 256) {
+        const int i = e / N, j = e - (e / N) * N;
+        double2 v = make_double2(0.0, 0.0);
+        if (i < n && j < n) {
+            if (j >= i) {
+                v = x0[i * n + j];
+                if (i == j) v.y = 0.0;
+            } else {
+                v = x0[j * n + i];
+                v.y = -v.y;
+            }
+        }
+        hs[e] = v;
+    }
+    __threadfence_block();
+    __syncthreads();
+    // T = W^+ H: (W^+)(r, kk) = conj W(kk, r)
+    for (int tile = wave; tile < nt * nt; tile += 4) {
+        const int r0 = (tile / nt) * 16, c0 = (tile % nt) * 16;
+        f64x4 p1, p2, p3;
+        wn_mfma_tile_c64<N>(w + min(r0 + lr, n - 1), n, r0 + lr < n, n, -1.0, hs + c0 + lr, N, true, N, lq, p1, p2, p3);
+#pragma unroll
+        for (int q = 0; q < 4; ++q)
+            ts[(r0 + lq + 4 * q) * N + c0 + lr] = make_double2(p1[q] - p2[q], p3[q] - p1[q] - p2[q]);
+    }
+    __threadfence_block();
+    __syncthreads();
+    // channel 0 <- T W
+    for (int tile = wave; tile < nt * nt; tile += 4) {
+        const int r0 = (tile / nt) * 16, c0 = (tile % nt) * 16;
+        f64x4 p1, p2, p3;
+        wn_mfma_tile_c64<N>(ts + (r0 + lr) * N, 1, true, N, 1.0, w + min(c0 + lr, n - 1), n, c0 + lr < n, n, lq, p1, p2,
+                            p3);
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            const int row = r0 + lq + 4 * q, col = c0 + lr;
+            if (row < n && col < n) x0[row * n + col] = make_double2(p1[q] - p2[q], p3[q] - p1[q] - p2[q]);
+        }
+    }
+}
+
 template <int N>
 static hipError_t launch_eig_big(const double *x, double *e, double2 *u, int *sweeps,
                                  unsigned long long *sweep_total, int count, const WnShape &sh, hipStream_t st,
-                                 double2 *scr, int ctr_stride, int *status) {
+                                 double2 *scr, int ctr_stride, int *status, int pass, const double2 *uprev) {
     const double tol = 1.0e-15;
     const int n = sh.n;
     const int groups = (n + 4 * WN_EIGB_RW - 1) / (4 * WN_EIGB_RW);
     const int chunks = (sh.max_sweeps + WN_EIGB_SB - 1) / WN_EIGB_SB;
-    const unsigned vgrid = (unsigned)((count + 7) / 8 * 8) * (unsigned)groups;
+    const int nslots = eigb_slots(pass, count);
+    const unsigned vgrid = (unsigned)((nslots + 7) / 8 * 8) * (unsigned)groups;
     for (int c = 0; c < chunks; ++c) {
-        hipLaunchKernelGGL((k_eigb_sweeps<N, 512>), dim3(count), dim3(512), 0, st, x, n, sh.ldo, tol, count, scr, c,
-                           sh.max_sweeps);
-        hipLaunchKernelGGL((k_eigb_vapply<N, WN_EIGB_RW>), dim3(vgrid), dim3(256), 0, st, n, count, scr, c, groups);
+        hipLaunchKernelGGL((k_eigb_sweeps<N, 512>), dim3(nslots), dim3(512), 0, st, x, n, sh.ldo, tol, count, scr, c,
+                           sh.max_sweeps, pass);
+        hipLaunchKernelGGL((k_eigb_vapply<N, WN_EIGB_RW>), dim3(vgrid), dim3(256), 0, st, n, count, scr, c, groups, pass,
+                           nslots, u, uprev);
     }
-    hipLaunchKernelGGL((k_eigb_finish<N>), dim3(count), dim3(256), 0, st, e, u, sweeps, sweep_total, n, count, scr,
-                       ctr_stride, status);
+    hipLaunchKernelGGL((k_eigb_finish<N>), dim3(nslots), dim3(256), 0, st, e, u, sweeps, sweep_total, n, count, scr,
+                       ctr_stride, status, pass);
     return hipGetLastError();
 }
 
 hipError_t wn_launch_eig_big(const double *x, double *e, double2 *u, int *sweeps, unsigned long long *sweep_total,
-                             int count, const WnShape &sh, hipStream_t st, double2 *scr, int ctr_stride, int *status) {
+                             int count, const WnShape &sh, hipStream_t st, double2 *scr, int ctr_stride, int *status,
+                             int pass, const double2 *uprev) {
     if (sh.n <= 64 || sh.n > 128 || !scr || sh.max_sweeps < 1) return hipErrorInvalidValue;
+    if ((pass == 0 || pass == 1 || pass == 2) && !wn_warm_start(sh.n, count)) return hipErrorInvalidValue;
+    if (pass == 2 && !uprev) return hipErrorInvalidValue;
     if (count <= 0) return hipSuccess;
-    if (sh.n <= 96) return launch_eig_big<96>(x, e, u, sweeps, sweep_total, count, sh, st, scr, ctr_stride, status);
-    return launch_eig_big<128>(x, e, u, sweeps, sweep_total, count, sh, st, scr, ctr_stride, status);
+    if (sh.n <= 96)
+        return launch_eig_big<96>(x, e, u, sweeps, sweep_total, count, sh, st, scr, ctr_stride, status, pass, uprev);
+    return launch_eig_big<128>(x, e, u, sweeps, sweep_total, count, sh, st, scr, ctr_stride, status, pass, uprev);
+}
+
+hipError_t wn_launch_warm_big(double *x, const double2 *u, int count, const WnShape &sh, hipStream_t st,
+                              const double2 *uprev, double2 *scr) {
+    if (sh.n <= 64 || sh.n > 128 || !scr || !wn_warm_start(sh.n, count)) return hipErrorInvalidValue;
+    const int grid = uprev ? count : (count + WN_WARM_G - 1) / WN_WARM_G * (WN_WARM_G - 1);
+    if (sh.n <= 96)
+        hipLaunchKernelGGL((k_eigb_warm<96>), dim3(grid), dim3(256), 0, st, x, u, uprev, sh.n, sh.ldo, count, scr);
+    else
+        hipLaunchKernelGGL((k_eigb_warm<128>), dim3(grid), dim3(256), 0, st, x, u, uprev, sh.n, sh.ldo, count, scr);
+    return hipGetLastError();
 }

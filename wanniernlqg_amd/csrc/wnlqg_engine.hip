@@ -1056,18 +1056,18 @@ int enqueue_mesh(wnlqg_engine *g, std::vector<std::array<hipEvent_t, WN_MARKS>> 
         if (scheme == 2) {
             const double2 *uprev = g->u.p + (size_t)((bi - 1) & 1) * g->batch * g->sh.n * g->sh.n;
             mark(bi, 8, s0);
-            WN_HIP(wn_launch_warm_transform_f64(xb, ub, cnt, g->sh, s0, uprev));
+            WN_HIP(wn_launch_warm_transform_f64(xb, ub, cnt, g->sh, s0, uprev, g->escratch.p));
             mark(bi, 9, s0);
             WN_HIP(wn_launch_eig_pass_f64(xb, eb, ub, sweep_max, sweep_total, cnt, g->sh, g->eig_tpb, 2, s0,
-                                          nullptr, uprev, WN_CTR_STRIDE, g->flags.p));
+                                          g->escratch.p, uprev, WN_CTR_STRIDE, g->flags.p));
         } else if (scheme == 1) {
-            WN_HIP(wn_launch_eig_pass_f64(xb, eb, ub, sweep_max, sweep_total, cnt, g->sh, g->eig_tpb, 0, s0, nullptr,
-                                          nullptr, WN_CTR_STRIDE, g->flags.p));
+            WN_HIP(wn_launch_eig_pass_f64(xb, eb, ub, sweep_max, sweep_total, cnt, g->sh, g->eig_tpb, 0, s0,
+                                          g->escratch.p, nullptr, WN_CTR_STRIDE, g->flags.p));
             mark(bi, 8, s0);
-            WN_HIP(wn_launch_warm_transform_f64(xb, ub, cnt, g->sh, s0));
+            WN_HIP(wn_launch_warm_transform_f64(xb, ub, cnt, g->sh, s0, nullptr, g->escratch.p));
             mark(bi, 9, s0);
-            WN_HIP(wn_launch_eig_pass_f64(xb, eb, ub, sweep_max, sweep_total, cnt, g->sh, g->eig_tpb, 1, s0, nullptr,
-                                          nullptr, WN_CTR_STRIDE, g->flags.p));
+            WN_HIP(wn_launch_eig_pass_f64(xb, eb, ub, sweep_max, sweep_total, cnt, g->sh, g->eig_tpb, 1, s0,
+                                          g->escratch.p, nullptr, WN_CTR_STRIDE, g->flags.p));
         } else {
             WN_HIP(wn_launch_eig_pass_f64(xb, eb, ub, sweep_max, sweep_total, cnt, g->sh, g->eig_tpb, -1, s0,
                                           g->escratch.p, nullptr, WN_CTR_STRIDE, g->flags.p));
@@ -1478,15 +1478,15 @@ int32_t wnlqg_run_mesh_reduced(wnlqg_engine *g, const int32_t mesh[3], const int
         if (cr == 0) continue;
         if (wn_warm_start(g->sh.n, cr) && prev >= 0 && prev == bi - 1 && same_pattern(prev, bi)) {
             const double2 *uprev = g->u.p + (size_t)(buf ^ 1) * g->batch * g->sh.n * g->sh.n;
-            WN_HIP(wn_launch_warm_transform_f64(xo, uo, cr, g->sh, s0, uprev));
-            WN_HIP(wn_launch_eig_pass_f64(xo, eo, uo, sweep_max, sweep_total, cr, g->sh, g->eig_tpb, 2, s0, nullptr,
+            WN_HIP(wn_launch_warm_transform_f64(xo, uo, cr, g->sh, s0, uprev, g->escratch.p));
+            WN_HIP(wn_launch_eig_pass_f64(xo, eo, uo, sweep_max, sweep_total, cr, g->sh, g->eig_tpb, 2, s0, g->escratch.p,
                                           uprev, WN_CTR_STRIDE, g->flags.p));
         } else if (wn_warm_start(g->sh.n, cr)) {
             // groups of 4 consecutive representatives (mostly neighbours along axis 1)
-            WN_HIP(wn_launch_eig_pass_f64(xo, eo, uo, sweep_max, sweep_total, cr, g->sh, g->eig_tpb, 0, s0, nullptr,
+            WN_HIP(wn_launch_eig_pass_f64(xo, eo, uo, sweep_max, sweep_total, cr, g->sh, g->eig_tpb, 0, s0, g->escratch.p,
                                           nullptr, WN_CTR_STRIDE, g->flags.p));
-            WN_HIP(wn_launch_warm_transform_f64(xo, uo, cr, g->sh, s0));
-            WN_HIP(wn_launch_eig_pass_f64(xo, eo, uo, sweep_max, sweep_total, cr, g->sh, g->eig_tpb, 1, s0, nullptr,
+            WN_HIP(wn_launch_warm_transform_f64(xo, uo, cr, g->sh, s0, nullptr, g->escratch.p));
+            WN_HIP(wn_launch_eig_pass_f64(xo, eo, uo, sweep_max, sweep_total, cr, g->sh, g->eig_tpb, 1, s0, g->escratch.p,
                                           nullptr, WN_CTR_STRIDE, g->flags.p));
         } else {
             WN_HIP(wn_launch_eig_pass_f64(xo, eo, uo, sweep_max, sweep_total, cr, g->sh, g->eig_tpb, -1, s0,

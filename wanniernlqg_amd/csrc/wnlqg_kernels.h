@@ -136,12 +136,35 @@ hipError_t wn_launch_eig_rv(const double *x, double *e, double2 *u, int *sweeps,
 // row-owning waves, then sort and phase; scratch: wn_eig_big_scratch_elems(n, count) complex elements.
 size_t wn_eig_big_scratch_elems(int n, int count);
 hipError_t wn_launch_eig_big(const double *x, double *e, double2 *u, int *sweeps, unsigned long long *sweep_total,
-                             int count, const WnShape &sh, hipStream_t st, double2 *scr, int ctr_stride, int *status);
+                             int count, const WnShape &sh, hipStream_t st, double2 *scr, int ctr_stride, int *status,
+                             int pass = -1, const double2 *uprev = nullptr);
 size_t wn_eig_scratch_elems(int n, int count);
 size_t wn_rotate_scratch_elems(int n, int count);
 bool wn_warm_start(int n, int count);
+// vscratch: the eigensolver scratch, required for n > 64 (the transform stages its products there)
 hipError_t wn_launch_warm_transform_f64(double *x, const double2 *u, int count, const WnShape &sh, hipStream_t st,
-                                        const double2 *uprev = nullptr);
+                                        const double2 *uprev = nullptr, double2 *vscratch = nullptr);
+hipError_t wn_launch_warm_big(double *x, const double2 *u, int count, const WnShape &sh, hipStream_t st,
+                              const double2 *uprev, double2 *scr);
+
+// Warm-started mesh eigensolver groups: consecutive k points (neighbours along the fastest mesh axis) form
+// groups of WN_WARM_G; the anchor of a group (position WN_WARM_A, or the last point of a short group) is solved
+// from the identity (pass 0); every other point of the group is then solved from the anchor's eigenvectors
+// (pass 1).  wn_warm_k maps the workgroup (or slot) b of a pass to its k point and, in pass 1, its anchor.
+#define WN_WARM_G 4
+#define WN_WARM_A 1
+__device__ __forceinline__ int wn_warm_k(int b, int pass, int count, int *anchor) {
+    if (pass == 0) {
+        const int base = b * WN_WARM_G;
+        *anchor = -1;
+        return base + min(WN_WARM_A, count - 1 - base);
+    }
+    const int g = b / (WN_WARM_G - 1), j = b - g * (WN_WARM_G - 1);
+    const int base = g * WN_WARM_G;
+    const int al = min(WN_WARM_A, count - 1 - base);
+    *anchor = base + al;
+    return base + (j < al ? j : j + 1);
+}
 // Clears that are safe under stream capture: p[0 .. n) = 0; flags = {0, 0, 0, 0x7f7f7f7f}.
 hipError_t wn_launch_zero(double *p, long long n, hipStream_t st);
 // status words: [0] pair-gap flag, [1] eigensolver sweep cap reached, [2] geometry flags, [3] first failing k; then WN_CTR_BUCKETS counter

@@ -847,21 +847,6 @@ hipError_t wn_launch_contract_sel(const double2 *in, double2 *out, const double2
 // anchor's eigenvectors U_a (pass 1): its channel 0 is first replaced by U_a^+ H(k) U_a, which is
 // diagonal up to O(|k - k_a|), and V starts at U_a, so that V J1 J2 ... are eigenvectors of H(k).
 // The stopping rule is unchanged; a nearby start only removes sweeps.
-#define WN_WARM_G 4
-#define WN_WARM_A 1
-__device__ __forceinline__ int wn_warm_k(int b, int pass, int count, int *anchor) {
-    if (pass == 0) {
-        const int base = b * WN_WARM_G;
-        *anchor = -1;
-        return base + min(WN_WARM_A, count - 1 - base);
-    }
-    const int g = b / (WN_WARM_G - 1), j = b - g * (WN_WARM_G - 1);
-    const int base = g * WN_WARM_G;
-    const int al = min(WN_WARM_A, count - 1 - base);
-    *anchor = base + al;
-    return base + (j < al ? j : j + 1);
-}
-
 // pass < 0: every k point from the identity (grid = count); pass 0 / 1: the warm-start passes above;
 // pass 2: every k point from uprev[k], the eigenvectors of the same point of the previous mesh plane
 // (channel 0 already transformed by them).
@@ -1213,9 +1198,9 @@ hipError_t wn_launch_eig_pass_f64(const double *x, double *e, double2 *u, int *s
     WN_EIG_CASE(64, 256)
 #undef WN_EIG_CASE
 #undef WN_EIG_LAUNCH
-    // n in (64, 128]: deferred-eigenvector solver, cold start only
-    if (n <= 128 && pass < 0) return wn_launch_eig_big(x, e, u, sweeps, sweep_total, count, sh, st, vscratch, ctr_stride,
-                                                       status);
+    // n in (64, 128]: deferred-eigenvector solver, every pass
+    if (n <= 128) return wn_launch_eig_big(x, e, u, sweeps, sweep_total, count, sh, st, vscratch, ctr_stride, status,
+                                           pass, uprev);
     return hipErrorInvalidValue;
 }
 
@@ -1344,7 +1329,19 @@ __global__ __launch_bounds__(256) void k_rotate_mfma(double *__restrict__ x, con
         for (int q = 0; q < PF; ++q) {
             const int e = t + 256 * q;
             const int i = e / np, j = e - (e / np) * np;
-            pf[q] = (e < np * np && i < n && j < n) ? xc[i * n + j] : make_double2(0.0, 0.0);
+            double2 v = make_double2(0.0, 0.0);
+            if (e < np * np && i < n && j < n) {
+                if (!WARM) {
+                    v = xc[i * n + j];
+                } else if (j >= i) {   // the Hermitian matrix of the upper triangle (the eigensolver's input)
+                    v = xc[i * n + j];
+                    if (i == j) v.y = 0.0;
+                } else {
+                    v = xc[j * n + i];
+                    v.y = -v.y;
+                }
+            }
+            pf[q] = v;
         }
     };
     if (cbeg < cend) prefetch(cbeg);
@@ -1662,59 +1659,6 @@ __global__ __launch_bounds__(256) void k_rotate_sc(double *__restrict__ x, const
 // the 4 waves.  Complex products as three real products (Gauss): for A B with A = (ar, ai), B = (br, bi),
 // P1 = ar br, P2 = ai bi, P3 = (ar + ai)(br + bi); real part P1 - P2, imaginary part P3 - P1 - P2.  Four K
 // steps of operands are loaded per iteration, the next iteration's before this one's products.
-// One 16 x 16 tile of A B over K = NP for k_rotate_big.  Lane (lr, lq) reads A at a[kk * a_ks] (this lane's
-// row; valid while a_ok and kk < a_kmax; imaginary part times a_im) and B at b[kk * b_ks] (this lane's
-// column; valid while b_ok and kk < b_kmax), kk = k0 + lq.
-template <int NP>
-__device__ __forceinline__ void big_tile(const double2 *__restrict__ a, int a_ks, bool a_ok, int a_kmax, double a_im,
-                                         const double2 *__restrict__ b, int b_ks, bool b_ok, int b_kmax, int lq,
-                                         f64x4 &p1, f64x4 &p2, f64x4 &p3) {
-    const double2 zero = make_double2(0.0, 0.0);
-    p1 = (f64x4){0.0, 0.0, 0.0, 0.0};
-    p2 = p1;
-    p3 = p1;
-    double2 av[4], bv[4];
-#pragma unroll
-    for (int s4 = 0; s4 < 4; ++s4) {
-        const int kk = 4 * s4 + lq;
-        av[s4] = (a_ok && kk < a_kmax) ? a[kk * a_ks] : zero;
-        bv[s4] = (b_ok && kk < b_kmax) ? b[kk * b_ks] : zero;
-    }
-#pragma unroll
-    for (int k0 = 0; k0 < NP; k0 += 16) {
-        double2 an[4], bn[4];
-        if (k0 + 16 < NP) {
-#pragma unroll
-            for (int s4 = 0; s4 < 4; ++s4) {
-                const int kk = k0 + 16 + 4 * s4 + lq;
-                an[s4] = (a_ok && kk < a_kmax) ? a[kk * a_ks] : zero;
-                bn[s4] = (b_ok && kk < b_kmax) ? b[kk * b_ks] : zero;
-            }
-        }
-#pragma unroll
-        for (int s4 = 0; s4 < 4; ++s4) {
-            const double ar = av[s4].x, ai = a_im * av[s4].y;
-            const double2 bq = bv[s4];
-            p1 = __builtin_amdgcn_mfma_f64_16x16x4f64(ar, bq.x, p1, 0, 0, 0);
-            p2 = __builtin_amdgcn_mfma_f64_16x16x4f64(ai, bq.y, p2, 0, 0, 0);
-            p3 = __builtin_amdgcn_mfma_f64_16x16x4f64(ar + ai, bq.x + bq.y, p3, 0, 0, 0);
-        }
-        if (k0 + 16 < NP) {
-#pragma unroll
-            for (int s4 = 0; s4 < 4; ++s4) {
-                av[s4] = an[s4];
-                bv[s4] = bn[s4];
-            }
-        }
-    }
-}
-
-// Rotation for n in (64, 128]: U and X are read through the L2 as MFMA operands and U^+ X is staged
-// in a per-k scratch [NP][NP] (the LDS cannot hold U, X and U^+ X at these sizes).  One 256-thread
-// workgroup per k, channels 1 .. C-1 in turn; the (NP/16)^2 output tiles of each product are dealt to
-// the 4 waves.  Complex products as three real products (Gauss): for A B with A = (ar, ai), B = (br, bi),
-// P1 = ar br, P2 = ai bi, P3 = (ar + ai)(br + bi); real part P1 - P2, imaginary part P3 - P1 - P2.  Four K
-// steps of operands are loaded per iteration, the next iteration's before this one's products.
 template <int NP>
 __global__ __launch_bounds__(256) void k_rotate_big(double *__restrict__ x, const double2 *__restrict__ u, int n, int C,
                                                    int ldo, double2 *__restrict__ scratch) {
@@ -1732,7 +1676,7 @@ __global__ __launch_bounds__(256) void k_rotate_big(double *__restrict__ x, cons
         for (int tile = wave; tile < nt * nt; tile += 4) {
             const int r0 = (tile / nt) * 16, c0 = (tile % nt) * 16;
             f64x4 p1, p2, p3;
-            big_tile<NP>(uk + min(r0 + lr, n - 1), n, r0 + lr < n, n, -1.0, xc + min(c0 + lr, n - 1), n, c0 + lr < n,
+            wn_mfma_tile_c64<NP>(uk + min(r0 + lr, n - 1), n, r0 + lr < n, n, -1.0, xc + min(c0 + lr, n - 1), n, c0 + lr < n,
                          n, lq, p1, p2, p3);
 #pragma unroll
             for (int q = 0; q < 4; ++q)
@@ -1744,7 +1688,7 @@ __global__ __launch_bounds__(256) void k_rotate_big(double *__restrict__ x, cons
         for (int tile = wave; tile < nt * nt; tile += 4) {
             const int r0 = (tile / nt) * 16, c0 = (tile % nt) * 16;
             f64x4 p1, p2, p3;
-            big_tile<NP>(tk + (r0 + lr) * NP, 1, true, NP, 1.0, uk + min(c0 + lr, n - 1), n, c0 + lr < n, n, lq, p1, p2,
+            wn_mfma_tile_c64<NP>(tk + (r0 + lr) * NP, 1, true, NP, 1.0, uk + min(c0 + lr, n - 1), n, c0 + lr < n, n, lq, p1, p2,
                          p3);
 #pragma unroll
             for (int q = 0; q < 4; ++q) {
@@ -1822,11 +1766,12 @@ hipError_t wn_launch_eig_f64(const double *x, double *e, double2 *u, int *sweeps
     return wn_launch_eig_pass_f64(x, e, u, sweeps, nullptr, count, sh, tpb, -1, st, vscratch, nullptr, 0, status);
 }
 
-bool wn_warm_start(int n, int count) { return n >= 16 && n <= 64 && count >= WN_WARM_G; }
+bool wn_warm_start(int n, int count) { return n >= 16 && n <= 128 && count >= WN_WARM_G; }
 
 hipError_t wn_launch_warm_transform_f64(double *x, const double2 *u, int count, const WnShape &sh, hipStream_t st,
-                                        const double2 *uprev) {
+                                        const double2 *uprev, double2 *vscratch) {
     if (!wn_warm_start(sh.n, count)) return hipErrorInvalidValue;
+    if (sh.n > 64) return wn_launch_warm_big(x, u, count, sh, st, uprev, vscratch);
     launch_rotate_mfma<true>(x, u, count, sh, st, uprev);
     return hipGetLastError();
 }
