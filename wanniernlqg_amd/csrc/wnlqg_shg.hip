@@ -165,6 +165,64 @@ __global__ __launch_bounds__(WN_SHG_TPB) void k_shg_prep(const double *__restric
     }
 }
 
+// One block of a tile's product sum_p (f_L L[row][p]) (f_R R[p][col]) into the three Gauss accumulators of
+// the tile (t1 = sum Re Re, t2 = sum Im Im, t3 = sum (Re + Im)(Re + Im)).  FL: 0 -> 1, 1 -> -1, 2 -> i;
+// FR: 0 -> 1, 1 -> -1, 3 -> -i; ZL / ZR zero L's / R's diagonal (p == row / p == col).  Lane (lr, lk) takes
+// K index p0 + lk of each step; four steps per iteration, the next iteration's operands loaded first; full
+// iterations need no bounds (the tail iteration masks p >= n).
+template <int FL, int FR, bool ZL, bool ZR>
+__device__ __forceinline__ void vtx_block(const double2 *__restrict__ L, const double2 *__restrict__ R, int n, int row,
+                                          int col, int lk, f64x4 &t1, f64x4 &t2, f64x4 &t3) {
+    auto step = [&](double2 av, double2 cv, int p, bool valid) {
+        double ar = FL == 2 ? -av.y : (FL == 1 ? -av.x : av.x);
+        double ai = FL == 2 ? av.x : (FL == 1 ? -av.y : av.y);
+        double cr = FR == 3 ? cv.y : (FR == 1 ? -cv.x : cv.x);
+        double ci = FR == 3 ? -cv.x : (FR == 1 ? -cv.y : cv.y);
+        if (!valid || (ZL && p == row)) ar = ai = 0.0;
+        if (!valid || (ZR && p == col)) cr = ci = 0.0;
+        t1 = __builtin_amdgcn_mfma_f64_16x16x4f64(ar, cr, t1, 0, 0, 0);
+        t2 = __builtin_amdgcn_mfma_f64_16x16x4f64(ai, ci, t2, 0, 0, 0);
+        t3 = __builtin_amdgcn_mfma_f64_16x16x4f64(ar + ai, cr + ci, t3, 0, 0, 0);
+    };
+    const double2 *lp = L + row * n + lk;    // L[row][p0 + lk]
+    const double2 *rp = R + lk * n + col;    // R[p0 + lk][col]
+    const int nfull = n & ~15;
+    double2 lv[4], rv[4];
+    if (nfull > 0) {
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+            lv[u] = lp[4 * u];
+            rv[u] = rp[4 * u * n];
+        }
+    }
+    for (int p0 = 0; p0 < nfull; p0 += 16) {
+        double2 ln[4], rn[4];
+        if (p0 + 16 < nfull) {
+#pragma unroll
+            for (int u = 0; u < 4; ++u) {
+                ln[u] = lp[p0 + 16 + 4 * u];
+                rn[u] = rp[(p0 + 16 + 4 * u) * n];
+            }
+        }
+#pragma unroll
+        for (int u = 0; u < 4; ++u) step(lv[u], rv[u], p0 + 4 * u + lk, true);
+        if (p0 + 16 < nfull) {
+#pragma unroll
+            for (int u = 0; u < 4; ++u) {
+                lv[u] = ln[u];
+                rv[u] = rn[u];
+            }
+        }
+    }
+    if (nfull < n) {
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+            const int p = nfull + 4 * u + lk, pc = min(p, n - 1);
+            step(L[row * n + pc], R[pc * n + col], p, p < n);
+        }
+    }
+}
+
 // One 16 x 16 tile of F_ab and G_ab per wave on v_mfma_f64_16x16x4f64, complex products as three real
 // products (t1 = Lr Rr, t2 = Li Ri, t3 = (Lr + Li)(Rr + Ri); C = (t1 - t2, t3 - t1 - t2)), then the
 // derivative of every element of the tile.  Lane l holds the left operand (row l%16, band p0 + l/16) and
@@ -205,64 +263,22 @@ __global__ __launch_bounds__(64) void k_shg_vtx(const double *__restrict__ x, co
     const double2 *dk = der + (size_t)k * WN_SHG_DER_BLOCKS * n2;
     const bool pos = s.pos != 0, eta = pos && sp.eta_correction;
     f64x4 f1 = {0.0, 0.0, 0.0, 0.0}, f2 = f1, f3 = f1, g1 = f1, g2 = f1, g3 = f1;
-    // products L R: (left matrix, its factor, zero-diagonal), (right matrix, factor, zero-diagonal);
-    // factors 1, -1, i, -i as (re, im)
-    for (int blk = 0; blk < 8; ++blk) {
-        if (blk < 2 && !pos) continue;
-        if (blk >= 4 && !eta) continue;
-        const double2 *L, *R;
-        double2 fl = make_double2(1.0, 0.0), fr = make_double2(1.0, 0.0);
-        bool zl = false, zr = false;
-        const double2 *Aa = s.A(a), *Va = s.V(a), *Rb = dk + (size_t)(kShgR + bb) * n2;
-        switch (blk) {
-            case 0: L = Aa; R = Rb; break;                                          // F: A_a R_b
-            case 1: L = Rb; R = Aa; fr = make_double2(-1.0, 0.0); break;            //    - R_b A_a
-            case 2: L = Va; R = Rb; fl = make_double2(0.0, 1.0); break;             // G: i V_a R_b
-            case 3: L = Rb; R = Va; fr = make_double2(0.0, -1.0); break;            //    - i R_b V_a
-            case 4: L = Aa; zl = true; R = dk + (size_t)(kShgX + bb) * n2; break;   //    A'_a X_b
-            case 5: L = dk + (size_t)(kShgX + bb) * n2; R = Aa; zr = true; fr = make_double2(-1.0, 0.0); break;
-            case 6: L = dk + (size_t)(kShgQ + a) * n2; R = dk + (size_t)(kShgY + bb) * n2;
-                    fl = make_double2(-1.0, 0.0); break;                            //    - Q'_a Y_b
-            default: L = dk + (size_t)(kShgY + bb) * n2; R = dk + (size_t)(kShgQ + a) * n2; break;   // Y_b Q'_a
-        }
-        // four K steps per iteration; the next iteration's operands are loaded before this one's products
-        double2 lv[4], rv[4];
-        auto load = [&](int p0, double2 (&lo)[4], double2 (&ro)[4]) {
-#pragma unroll
-            for (int u = 0; u < 4; ++u) {
-                const int pc = min(p0 + 4 * u + lk, n - 1);
-                lo[u] = L[row * n + pc];
-                ro[u] = R[pc * n + col];
-            }
-        };
-        load(0, lv, rv);
-        for (int p0 = 0; p0 < n; p0 += 16) {
-            double2 ln[4], rn[4];
-            if (p0 + 16 < n) load(p0 + 16, ln, rn);
-#pragma unroll
-            for (int u = 0; u < 4; ++u) {
-                const int p = p0 + 4 * u + lk;
-                const bool lz = p >= n || (zl && p == row), rz = p >= n || (zr && p == col);
-                const double2 a = lz ? make_double2(0.0, 0.0)
-                                     : make_double2(lv[u].x * fl.x - lv[u].y * fl.y, lv[u].x * fl.y + lv[u].y * fl.x);
-                const double2 c = rz ? make_double2(0.0, 0.0)
-                                     : make_double2(rv[u].x * fr.x - rv[u].y * fr.y, rv[u].x * fr.y + rv[u].y * fr.x);
-                if (blk < 2) {
-                    f1 = __builtin_amdgcn_mfma_f64_16x16x4f64(a.x, c.x, f1, 0, 0, 0);
-                    f2 = __builtin_amdgcn_mfma_f64_16x16x4f64(a.y, c.y, f2, 0, 0, 0);
-                    f3 = __builtin_amdgcn_mfma_f64_16x16x4f64(a.x + a.y, c.x + c.y, f3, 0, 0, 0);
-                } else {
-                    g1 = __builtin_amdgcn_mfma_f64_16x16x4f64(a.x, c.x, g1, 0, 0, 0);
-                    g2 = __builtin_amdgcn_mfma_f64_16x16x4f64(a.y, c.y, g2, 0, 0, 0);
-                    g3 = __builtin_amdgcn_mfma_f64_16x16x4f64(a.x + a.y, c.x + c.y, g3, 0, 0, 0);
-                }
-            }
-#pragma unroll
-            for (int u = 0; u < 4; ++u) {
-                lv[u] = ln[u];
-                rv[u] = rn[u];
-            }
-        }
+    // products L R of the eight blocks; each block's factor on L or R (1, -1, i, -i) and zero diagonal are
+    // compile-time (vtx_block), so applying them costs at most a swap and sign changes per element
+    const double2 *Aa = s.A(a), *Va = s.V(a), *Rb = dk + (size_t)(kShgR + bb) * n2;
+    const double2 *Xb = dk + (size_t)(kShgX + bb) * n2, *Yb = dk + (size_t)(kShgY + bb) * n2;
+    const double2 *Qa = dk + (size_t)(kShgQ + a) * n2;
+    if (pos) {
+        vtx_block<0, 0, false, false>(Aa, Rb, n, row, col, lk, f1, f2, f3);    // F: A_a R_b
+        vtx_block<0, 1, false, false>(Rb, Aa, n, row, col, lk, f1, f2, f3);    //    - R_b A_a
+    }
+    vtx_block<2, 0, false, false>(Va, Rb, n, row, col, lk, g1, g2, g3);        // G: i V_a R_b
+    vtx_block<0, 3, false, false>(Rb, Va, n, row, col, lk, g1, g2, g3);        //    - i R_b V_a
+    if (eta) {
+        vtx_block<0, 0, true, false>(Aa, Xb, n, row, col, lk, g1, g2, g3);     //    A'_a X_b
+        vtx_block<0, 1, false, true>(Xb, Aa, n, row, col, lk, g1, g2, g3);     //    - X_b A'_a
+        vtx_block<1, 0, false, false>(Qa, Yb, n, row, col, lk, g1, g2, g3);    //    - Q'_a Y_b
+        vtx_block<0, 0, false, false>(Yb, Qa, n, row, col, lk, g1, g2, g3);    //    Y_b Q'_a
     }
     // element (nn, m) = (r0 + lk + 4q, c0 + lr) of the tile
     const int m = c0 + lr;
@@ -614,6 +630,18 @@ __device__ __forceinline__ double2 shg_resonance(double x, double w2, double iw,
     return make_double2(pv, d);
 }
 
+// pi delta(x) given 1 / (x^2 + w^2): Gaussian sqrt(pi) exp(-(x/w)^2) / w (0 beyond 27.3 w), Lorentzian w / (x^2 + w^2)
+__device__ __forceinline__ double shg_delta(double x, double inv, double iw, double gnorm, double cut, double w,
+                                            int gaussian) {
+    if (!gaussian) return w * inv;
+    double d = 0.0;
+    if (fabs(x) < cut) {
+        const double y = x * iw;
+        d = gnorm * exp(-y * y);
+    }
+    return d;
+}
+
 struct ShgTileSet {
     int t0, t1, t2;   // tiles of d1, d2, dn
     __device__ __host__ static ShgTileSet of(int nc) { return {nc, (6 * nc + 7) / 8, (2 * nc + 7) / 8}; }
@@ -671,29 +699,38 @@ __global__ __launch_bounds__(64, 2) void k_shg_acc(const int4 *__restrict__ chun
     // groups of four pairs in chunk order
     for (int k = ch.x; k < kend; ++k) {
         const int cnt = count[k];
+        const double2 *ck = coef + (size_t)k * pcap * ncol;     // this point's rows (uniform base)
+        const double *dk = pdiff + (size_t)k * pcap;
         for (int f0 = 0; f0 < cnt; f0 += 4) {
+            const bool full = f0 + 4 <= cnt;
             const bool pv = f0 + lg < cnt;
-            // a lane past the last pair reads the group's first pair (finite) and contributes R = 0
-            const size_t row = (size_t)k * pcap + f0 + (pv ? lg : 0);
-            const double dE = pdiff[row];
+            // a lane past the last pair reads the group's first pair (finite) and contributes R = 0; lanes of
+            // frequencies past the axis feed only output rows that are not stored
+            const int row = f0 + (pv ? lg : 0);
+            const double dE = dk[row];
             double2 bv[NT];
 #pragma unroll
-            for (int j = 0; j < NT; ++j) bv[j] = coef[row * ncol + col[j]];
+            for (int j = 0; j < NT; ++j) bv[j] = ck[row * ncol + col[j]];
             double2 R0[RTW], R1[RTW], R2[RTW];
 #pragma unroll
             for (int r = 0; r < RTW; ++r) {
-                const double on = (pv && wv[r]) ? 1.0 : 0.0;
-                if (N0) {
-                    const double2 v = shg_resonance(dE - om[r], w2, iw, gnorm, cut, sp.gaussian);
-                    R0[r] = make_double2(on * v.x, on * v.y);
+                if (N0 && N1 && N2) {
+                    // the three principal-value denominators x^2 + w^2 share one refined reciprocal
+                    const double x0 = dE - om[r], x1 = dE - 2.0 * om[r], x2 = -dE - om[r];
+                    const double a0 = x0 * x0 + w2, a1 = x1 * x1 + w2, a2 = x2 * x2 + w2;
+                    const double a12 = a1 * a2;
+                    const double q = wn_rcp(a0 * a12);
+                    const double i0 = a12 * q, i1 = a0 * a2 * q, i2 = a0 * a1 * q;
+                    R0[r] = make_double2(x0 * i0, shg_delta(x0, i0, iw, gnorm, cut, sp.width, sp.gaussian));
+                    R1[r] = make_double2(x1 * i1, shg_delta(x1, i1, iw, gnorm, cut, sp.width, sp.gaussian));
+                    R2[r] = make_double2(x2 * i2, shg_delta(x2, i2, iw, gnorm, cut, sp.width, sp.gaussian));
+                } else {
+                    if (N0) R0[r] = shg_resonance(dE - om[r], w2, iw, gnorm, cut, sp.gaussian);
+                    if (N1) R1[r] = shg_resonance(dE - 2.0 * om[r], w2, iw, gnorm, cut, sp.gaussian);
+                    if (N2) R2[r] = shg_resonance(-dE - om[r], w2, iw, gnorm, cut, sp.gaussian);
                 }
-                if (N1) {
-                    const double2 v = shg_resonance(dE - 2.0 * om[r], w2, iw, gnorm, cut, sp.gaussian);
-                    R1[r] = make_double2(on * v.x, on * v.y);
-                }
-                if (N2) {
-                    const double2 v = shg_resonance(-dE - om[r], w2, iw, gnorm, cut, sp.gaussian);
-                    R2[r] = make_double2(on * v.x, on * v.y);
+                if (!full && !pv) {
+                    R0[r] = R1[r] = R2[r] = make_double2(0.0, 0.0);
                 }
             }
             // first K step: Re R against (Cr, Ci); second: Im R against (-Ci, Cr)
