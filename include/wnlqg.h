@@ -28,7 +28,7 @@
 extern "C" {
 #endif
 
-#define WNLQG_ABI_VERSION 7
+#define WNLQG_ABI_VERSION 8
 
 /* Status codes. */
 #define WNLQG_OK 0
@@ -203,16 +203,20 @@ int32_t wnlqg_run_mesh(wnlqg_engine *engine, const int32_t mesh[3], int32_t n_la
                        double *lane_out, double *total_out, wnlqg_stats *stats);
 
 /*
- * Weighted point list for SC / IC engines: the k-loop over the evaluation items of a symmetry-reduced
- * mesh (src/Runtime/Execution/IntegralDriver.jl:822-831: kpoint_index = representatives[i],
- * kpoint_weight = multiplicities[i], scale = kpoint_weight / num_kpoints).  kpoints: column-major
- * [3,m] fractional; weights[i]: the scale of item i (multiplicity / N_k for a reduced mesh).  Lanes
- * partition the m items as wnlqg_run_mesh partitions the mesh: lane L covers items
- * floor(L*m/n_lanes) .. floor((L+1)*m/n_lanes)-1, 1 <= n_lanes <= m.  lane_out: complex
- * [n_lanes][n_states][n_w*27], each block sum_{i in lane} weights[i] * value_i with the state's
- * prefactor; total_out (optional): the lanes summed in lane order.  The tensor projection of
- * apply_response_symmetry! (ResponseSymmetryRuntime.jl:1121-1146) is the caller's.  Other engine
- * families return WNLQG_E_INVALID.
+ * Weighted point list for SC / IC, BCD and AHC engines: the k-loop over the evaluation items of a
+ * symmetry-reduced mesh (src/Runtime/Execution/IntegralDriver.jl:822-831: kpoint_index =
+ * representatives[i], kpoint_weight = multiplicities[i], scale = kpoint_weight / num_kpoints), or any
+ * weighted node set (sub-cell refinements of a mesh).  kpoints: column-major [3,m] fractional;
+ * weights[i]: the scale of item i (multiplicity / N_k for a reduced mesh).  Lanes partition the m items
+ * as wnlqg_run_mesh partitions the mesh: lane L covers items floor(L*m/n_lanes) ..
+ * floor((L+1)*m/n_lanes)-1, 1 <= n_lanes <= m.  SC / IC: lane_out complex [n_lanes][n_states][n_w*27],
+ * each block sum_{i in lane} weights[i] * value_i with the state's prefactor; the tensor projection of
+ * apply_response_symmetry! (ResponseSymmetryRuntime.jl:1121-1146) is the caller's.  BCD / AHC: lane
+ * blocks as in wnlqg_run_mesh, each sum_{i in lane} weights[i] * value_i of the per-point values that
+ * wnlqg_run_mesh's lanes sum (the weights take the place of the mesh's 1/N_k); total_out = the lanes
+ * summed in lane order,
+ * divided by the measure (|det L|, and 1e-30 for AHC), so weights 1/N_k over a whole mesh give
+ * wnlqg_run_mesh's total.  total_out is optional.  SHG engines return WNLQG_E_INVALID.
  */
 int32_t wnlqg_run_points(wnlqg_engine *engine, const double *kpoints, const double *weights, int64_t m,
                          int32_t n_lanes, double *lane_out, double *total_out);
@@ -294,19 +298,20 @@ int32_t wnlqg_batch_destroy(wnlqg_batch *batch);
 
 /*
  * Learned refinement proposal.  No counterpart in the reference (it has no learned component); for
- * callers that rank the cells of a k mesh for adaptive refinement from the integrand variance of
- * cheaper plain runs.  The integrand itself is always computed exactly by the engine above.
+ * callers that rank the cells of a k mesh for adaptive refinement from per-cell features of the base
+ * mesh, trained on the refinement gains of cheaper coarse runs.  The integrand itself is always
+ * computed exactly by the engine above.
  *
- * A 3 -> 64 -> 64 -> 1 multilayer perceptron with SiLU activations in FP32.  Parameter vector:
- * W1[64][3] | b1[64] | W2[64][64] | b2[64] | W3[64] | b3 (row-major, WNLQG_MLP_PARAMS floats).
- * Samples x are [n][3] floats, targets and outputs [n].  The loss is the mean squared error; the
+ * An 8 -> 64 -> 64 -> 1 multilayer perceptron with SiLU activations in FP32.  Parameter vector:
+ * W1[64][8] | b1[64] | W2[64][64] | b2[64] | W3[64] | b3 (row-major, WNLQG_MLP_PARAMS floats).
+ * Samples x are [n][8] floats, targets and outputs [n].  The loss is the mean squared error; the
  * gradient is deterministic (fixed per-wave sample ranges summed in wave order).  Training is Adam
  * (beta1 0.9, beta2 0.999, eps 1e-8) on minibatches of a per-epoch shuffle seeded by `seed`.
  * Errors: the same WNLQG_E_* codes, message from wnlqg_mlp_last_error.
  */
-#define WNLQG_MLP_INPUTS 3
+#define WNLQG_MLP_INPUTS 8
 #define WNLQG_MLP_HIDDEN 64
-#define WNLQG_MLP_PARAMS 4481
+#define WNLQG_MLP_PARAMS 4801
 
 typedef struct wnlqg_mlp wnlqg_mlp;
 
@@ -321,7 +326,7 @@ int32_t wnlqg_mlp_gradient(wnlqg_mlp *mlp, const float *x, const float *t, int64
 /*
  * wnlqg_mlp_gradient without any allocation.  No counterpart in the reference: it is for callers that cannot
  * allocate device memory (code under HIP graph capture, tight serving loops, harnesses that forbid
- * hipMalloc).  x [n][3], t [n] and grad [WNLQG_MLP_PARAMS + 1] are DEVICE pointers; grad receives the
+ * hipMalloc).  x [n][8], t [n] and grad [WNLQG_MLP_PARAMS + 1] are DEVICE pointers; grad receives the
  * gradient of the mean squared error followed by the SUM of squared errors (divide by n for the loss).
  * The per-wave partial gradients live in a library-owned static device array of
  * WNLQG_MLP_NOALLOC_SCRATCH_BYTES per GPU (held until the process ends) instead of buffers of the mlp object;
@@ -330,7 +335,7 @@ int32_t wnlqg_mlp_gradient(wnlqg_mlp *mlp, const float *x, const float *t, int64
  * before returning.  Every call on one GPU shares that GPU's array: calls that overlap in time on one GPU
  * (other streams, other threads, other mlp objects) race and must be serialized by the caller.
  */
-#define WNLQG_MLP_NOALLOC_SCRATCH_BYTES (512ULL * 4482ULL * 4ULL)
+#define WNLQG_MLP_NOALLOC_SCRATCH_BYTES (512ULL * 4802ULL * 4ULL)
 #define WNLQG_MLP_NOALLOC_MAX_SAMPLES (1LL << 40)
 int32_t wnlqg_mlp_gradient_noalloc(wnlqg_mlp *mlp, const float *x, const float *t, int64_t n, float *grad,
                                    void *stream);

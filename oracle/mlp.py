@@ -1,7 +1,7 @@
-"""NumPy FP64 forward/backward of the 3 -> 64 -> 64 -> 1 SiLU proposal network
+"""NumPy FP64 forward/backward of the 8 -> 64 -> 64 -> 1 SiLU proposal network
 (TEST INFRASTRUCTURE ONLY; the network has no counterpart in the reference).
 
-Parameter vector: W1[64][3] | b1[64] | W2[64][64] | b2[64] | W3[64] | b3.  Loss: mean squared error.
+Parameter vector: W1[64][8] | b1[64] | W2[64][64] | b2[64] | W3[64] | b3.  Loss: mean squared error.
 Pinned by central finite differences of the loss (tests/test_oracle_mlp.py)."""
 
 from __future__ import annotations
@@ -9,12 +9,13 @@ from __future__ import annotations
 import numpy as np
 
 H = 64
+IN = 8
 
 
 def unpack(p: np.ndarray):
     p = np.asarray(p, dtype=np.float64)
     o = 0
-    w1 = p[o:o + 3 * H].reshape(H, 3); o += 3 * H
+    w1 = p[o:o + IN * H].reshape(H, IN); o += IN * H
     b1 = p[o:o + H]; o += H
     w2 = p[o:o + H * H].reshape(H, H); o += H * H
     b2 = p[o:o + H]; o += H

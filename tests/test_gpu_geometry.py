@@ -171,3 +171,44 @@ def test_bcd_periodic_model_matches_reference_closed_form():
     ref = periodic_bcd_closed_form(32)
     got = total["bcd"][0]
     assert np.max(np.abs(got - ref)) <= 1e-10 * np.max(np.abs(ref))
+
+
+def test_bcd_ahc_run_points_against_oracle():
+    """Weighted point list on a BCD + AHC engine: each lane is the weighted sum of the oracle's per-point
+    values over its items, the total that sum divided by the measure."""
+    m = synthetic_model(8, 1, 5, position_scale=0.1)
+    mus = [-0.3, 0.4]
+    rng = np.random.default_rng(12)
+    k = rng.uniform(-0.5, 1.0, (301, 3))
+    w = rng.uniform(0.1, 2.0, 301)
+    nl = 7
+    say(f"[geometry run_points] n=8, m=301, lanes={nl}")
+    with E.Engine(m, E.EngineTask(berry_curvature_dipole=True, intrinsic_hall=True, fermi_energies=mus,
+                                  temperature=0.0, fermi_surface_broadening=0.2, denominator_regularization=0.0,
+                                  degeneracy_threshold=1e-9)) as eng:
+        lanes, total = eng.run_points(k, w, nl)
+    task = G.GeometryTask(fermi_energies=mus, temperature=0.0, fs_broadening=0.2, degeneracy_threshold=1e-9)
+    vals = {"bcd": G.bcd_point_moments(m, task, k), "ahc": G.ahc_point_values(m, task, k)}
+    vol = abs(np.linalg.det(m.lattice))
+    measure = {"bcd": vol, "ahc": vol * 1e-30}
+    for key in ("bcd", "ahc"):
+        for lane in range(nl):
+            a, b = lane * 301 // nl, (lane + 1) * 301 // nl
+            assert rel(lanes[key][lane], np.einsum("k,k...->...", w[a:b], vals[key][a:b])) < 1e-10, (key, lane)
+        assert rel(total[key], np.einsum("k,k...->...", w, vals[key]) / measure[key]) < 1e-10, key
+
+
+def test_bcd_run_points_over_a_shuffled_mesh_equals_run_mesh():
+    """Every point of a 5 x 4 x 6 mesh, in a random order, with weights 1/N_k: the mesh total."""
+    make, kw = BCD_CASES["n8_pos_gauss"]
+    m = make()
+    mesh = (5, 4, 6)
+    nk = int(np.prod(mesh))
+    idx = np.arange(nk)
+    pts = np.stack([idx % 5 / 5.0, (idx // 5) % 4 / 4.0, idx // 20 / 6.0], axis=1)
+    perm = np.random.default_rng(3).permutation(nk)
+    say("[bcd run_points] shuffled 5x4x6 mesh")
+    with bcd_engine(m, kw) as eng:
+        _, mesh_total, _ = eng.run_mesh(mesh)
+        _, pts_total = eng.run_points(pts[perm], np.full(nk, 1.0 / nk), 9)
+    assert rel(pts_total["bcd"], mesh_total["bcd"]) < 1e-12

@@ -83,10 +83,10 @@ def test_reduced_mesh_equals_full_mesh(group, n, rmax, mesh, batch):
         assert rel(lp[key], lq[key]) < 1e-10
 
 
-def test_run_points_rejects_other_families():
-    model = synthetic_model(6, 1, 8)
-    with E.Engine(model, E.EngineTask(berry_curvature_dipole=True, fermi_energies=[0.0], temperature=300.0,
-                                      denominator_regularization=0.0)) as eng:
+def test_run_points_rejects_shg_engines():
+    model = synthetic_model(6, 1, 8, position_scale=0.1)
+    with E.Engine(model, E.EngineTask(second_harmonic=True, shift_current=False, photon_energies=[0.5, 1.0],
+                                      broadening=0.05, shg_components=[(0, 1, 1)], fermi_energy=0.0)) as eng:
         with pytest.raises(E.WnlqgError):
             eng.run_points(np.zeros((4, 3)), np.ones(4))
 

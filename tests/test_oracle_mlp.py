@@ -6,13 +6,16 @@ from oracle import mlp
 
 def test_gradient_matches_finite_differences():
     rng = np.random.default_rng(0)
-    p = rng.normal(0, 0.3, 4481)
-    x = rng.uniform(-1, 1, (37, 3))
+    H, IN = mlp.H, mlp.IN
+    b1, w2, b2, w3, b3 = IN * H, IN * H + H, IN * H + H + H * H, IN * H + 2 * H + H * H, IN * H + 3 * H + H * H
+    p = rng.normal(0, 0.3, b3 + 1)
+    x = rng.uniform(-1, 1, (37, IN))
     t = rng.normal(0, 1, 37)
     loss, g = mlp.loss_and_gradient(p, x, t)
+    assert g.shape == p.shape == (4801,)
     assert np.isclose(loss, np.mean((mlp.forward(p, x) - t) ** 2))
-    idx = np.concatenate([np.arange(0, 192, 17), np.arange(192, 256, 9), rng.choice(np.arange(256, 4352), 60),
-                          np.arange(4352, 4416, 7), np.arange(4416, 4481, 6), [4480]])
+    idx = np.concatenate([np.arange(0, b1, 23), np.arange(b1, w2, 9), rng.choice(np.arange(w2, b2), 60),
+                          np.arange(b2, w3, 7), np.arange(w3, b3, 6), [b3]])
     h = 1e-6
     for i in idx:
         e = np.zeros_like(p)

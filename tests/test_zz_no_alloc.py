@@ -9,7 +9,7 @@ import numpy as np
 import pytest
 
 from wanniernlqg_amd.engine import load_library
-from wanniernlqg_amd.mlp import PARAMS, ProposalNetwork
+from wanniernlqg_amd.mlp import INPUTS, PARAMS, ProposalNetwork
 
 pytestmark = pytest.mark.gpu
 
@@ -41,8 +41,8 @@ def test_noalloc_gradient_equals_gradient_on_two_calls():
     net = ProposalNetwork(seed=11)
     rng = np.random.default_rng(4)
     for n in (50000, 3001):
-        x = rng.uniform(-1, 1, (n, 3)).astype(np.float32)
-        t = (np.sin(2 * x[:, 0]) + x[:, 1] * x[:, 2]).astype(np.float32)
+        x = rng.uniform(-1, 1, (n, INPUTS)).astype(np.float32)
+        t = (np.sin(2 * x[:, 0]) + x[:, 1] * x[:, 2] - 0.5 * x[:, 7]).astype(np.float32)
         g_ref, loss_ref = net.gradient(x, t)
         poisoned = np.full(PARAMS + 1, np.nan, dtype=np.float32)
         xd, td, gd = device_copy(hip, x), device_copy(hip, t), device_copy(hip, poisoned)

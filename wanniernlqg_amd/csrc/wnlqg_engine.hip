@@ -807,8 +807,8 @@ hipError_t launch_response(wnlqg_engine *g, const int4 *chunks, int nch, int cnt
     if (g->family == 0)
         return wn_launch_accum(chunks, nch, g->count.p, g->pmeta.p, g->pmd.p, g->q.p, g->omega.p, g->partial.p, g->sh,
                                g->sc, g->stream);
-    return wn_launch_geom_f64(g->x.p, g->e.p, g->omega.p, chunks, nch, g->partial.p, g->flags.p, kbase, g->sh, g->gp,
-                              g->stream);
+    return wn_launch_geom_f64(g->x.p, g->e.p, g->omega.p, chunks, nch, nullptr, g->partial.p, g->flags.p, kbase, g->sh,
+                              g->gp, g->stream);
 }
 
 }  // namespace
@@ -1105,7 +1105,8 @@ int enqueue_mesh(wnlqg_engine *g, std::vector<std::array<hipEvent_t, WN_MARKS>> 
             WN_HIP(wn_launch_shg(xb, eb, g->omega.p, chunks, nch, 0, g->der.p, g->count.p, g->pidx.p, g->pdiff.p,
                                  g->coef.p, g->one.p, g->partial.p, g->sh, g->sp, s1));
         } else {
-            WN_HIP(wn_launch_geom_f64(xb, eb, g->omega.p, chunks, nch, g->partial.p, g->flags.p, k0, g->sh, g->gp, s1));
+            WN_HIP(wn_launch_geom_f64(xb, eb, g->omega.p, chunks, nch, nullptr, g->partial.p, g->flags.p, k0, g->sh, g->gp,
+                                      s1));
             mark(bi, 5, s1);
         }
         WN_HIP(wn_launch_lane_reduce(p.dlanes.p + p.blane[bi], nln, g->partial.p, g->lanebuf.p, comp, nelem, s1));
@@ -1271,8 +1272,8 @@ int32_t wnlqg_run_points(wnlqg_engine *g, const double *kpoints, const double *w
                          double *lane_out, double *total_out) {
     if (!g || !kpoints || !weights || !lane_out || m < 1) return fail(WNLQG_E_INVALID, "invalid arguments");
     if (n_lanes < 1 || n_lanes > m) return fail(WNLQG_E_INVALID, "n_lanes must be in [1, m]");
-    if (g->family != 0)
-        return fail(WNLQG_E_INVALID, "weighted point lists integrate shift / injection current engines only");
+    if (g->family != 0 && g->family != 1)
+        return fail(WNLQG_E_INVALID, "weighted point lists integrate shift / injection current, BCD and AHC engines only");
     if (g->backend == WNLQG_FOURIER_MIXED)
         return fail(WNLQG_E_UNSUPPORTED, "weighted k-point lists (adaptive / region nodes) support direct evaluation; "
                     "use fourier_backend direct or auto");
@@ -1319,6 +1320,15 @@ int32_t wnlqg_run_points(wnlqg_engine *g, const double *kpoints, const double *w
         WN_HIP(hipMemcpy(g->lanes.p, ln.data(), ln.size() * sizeof(int4), hipMemcpyHostToDevice));
         rc = run_front(g, cnt, rows, true);
         if (rc) return rc;
+        if (g->family == 1) {
+            // BCD / AHC: every channel rotated, each point's moments scaled by its weight inside the chunk sums
+            WN_HIP(wn_launch_rotate_f64(g->x.p, g->u.p, cnt, g->sh, g->stream, g->rscratch.p));
+            WN_HIP(wn_launch_geom_f64(g->x.p, g->e.p, g->omega.p, g->chunks.p, (int)ch.size(), g->wts.p, g->partial.p,
+                                      g->flags.p, p0, g->sh, g->gp, g->stream));
+            WN_HIP(wn_launch_lane_reduce(g->lanes.p, (int)ln.size(), g->partial.p, g->lanebuf.p, comp, nelem,
+                                         g->stream));
+            continue;
+        }
         WN_HIP(wn_launch_screen(g->e.p, g->omega.p, g->count.p, g->pmeta.p, g->pmd.p, g->flags.p, cnt, g->sh, g->sc,
                                 g->stream));
         WN_HIP(wn_launch_weight_pairs(g->pmd.p, g->count.p, g->wts.p, cnt, g->sh, g->stream));

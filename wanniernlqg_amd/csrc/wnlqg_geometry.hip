@@ -65,8 +65,9 @@ __device__ inline void group_reduce9(double *red, const double v[9], int band, i
 
 __global__ __launch_bounds__(WN_GEOM_TPB) void k_geom_f64(const double *__restrict__ x, const double *__restrict__ eg,
                                                          const double *__restrict__ mu, const int4 *__restrict__ chunks,
-                                                         double2 *__restrict__ partial, int *__restrict__ flags,
-                                                         long long kbase, WnShape sh, WnGeom gp) {
+                                                         const double *__restrict__ wts, double2 *__restrict__ partial,
+                                                         int *__restrict__ flags, long long kbase, WnShape sh,
+                                                         WnGeom gp) {
     extern __shared__ double lds[];
     const int n = sh.n, n2 = n * n;
     const int t = threadIdx.x;
@@ -99,6 +100,7 @@ __global__ __launch_bounds__(WN_GEOM_TPB) void k_geom_f64(const double *__restri
         const double2 *X = (const double2 *)(x + (size_t)k * sh.ldo);
         const double2 *DH = X + (size_t)gp.c_dh * n2;
         const double2 *AA = has_pos ? X + (size_t)gp.c_a * n2 : nullptr;
+        const double wk = wts ? wts[k] : 1.0;
         if (t < n) E[t] = eg[(size_t)k * n + t];
         __syncthreads();
         if (do_bcd && t + 1 < n && !(fabs(E[t + 1] - E[t]) > gp.gap_tol)) {
@@ -180,7 +182,7 @@ __global__ __launch_bounds__(WN_GEOM_TPB) void k_geom_f64(const double *__restri
                     const int ab = 3 * a + b;
                     double s = 0.0;
                     for (int bb = 0; bb < n; ++bb) s += W[q * n + bb] * (Om[bb * 9 + ab] * Vd[bb * 3 + c]);
-                    acc[j] += s;
+                    acc[j] += wk * s;
                 }
             }
         }
@@ -232,7 +234,7 @@ __global__ __launch_bounds__(WN_GEOM_TPB) void k_geom_f64(const double *__restri
 #pragma unroll
             for (int j = 0; j < WN_GEOM_OUT; ++j) {
                 const int o = t + j * WN_GEOM_TPB;
-                if (o >= aoff && o < nelem) acc[j] += res[o - aoff];
+                if (o >= aoff && o < nelem) acc[j] += wk * res[o - aoff];
             }
         }
         __syncthreads();
@@ -250,8 +252,8 @@ __global__ __launch_bounds__(WN_GEOM_TPB) void k_geom_f64(const double *__restri
 }
 
 hipError_t wn_launch_geom_f64(const double *x, const double *e, const double *mu, const int4 *chunks, int nchunks,
-                              double2 *partial, int *flags, long long kbase, const WnShape &sh, const WnGeom &gp,
-                              hipStream_t st) {
+                              const double *wts, double2 *partial, int *flags, long long kbase, const WnShape &sh,
+                              const WnGeom &gp, hipStream_t st) {
     if (nchunks == 0) return hipSuccess;
     if (sh.n > 64 || gp.nmu < 1 || gp.nmu > WN_GEOM_MAXMU || gp.kind < 1 || gp.kind > 3)
         return hipErrorInvalidValue;
@@ -259,7 +261,7 @@ hipError_t wn_launch_geom_f64(const double *x, const double *e, const double *mu
     const size_t lds = sizeof(double) * (64 + (size_t)n * n + (size_t)n * 9 * 2 + (size_t)n * 3 +
                                          ((gp.kind & 1) ? (size_t)gp.nmu * n : 0) + (size_t)WN_GEOM_TPB * 9 +
                                          (size_t)9 * gp.nmu);
-    hipLaunchKernelGGL(k_geom_f64, dim3(nchunks), dim3(WN_GEOM_TPB), lds, st, x, e, mu, chunks, partial, flags, kbase,
-                       sh, gp);
+    hipLaunchKernelGGL(k_geom_f64, dim3(nchunks), dim3(WN_GEOM_TPB), lds, st, x, e, mu, chunks, wts, partial, flags,
+                       kbase, sh, gp);
     return hipGetLastError();
 }

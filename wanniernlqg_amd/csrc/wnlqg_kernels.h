@@ -208,12 +208,13 @@ hipError_t wn_launch_weight_pairs(double2 *pmd, const int *count, const double *
 hipError_t wn_launch_accum(const int4 *chunks, int nchunks, const int *count, const int4 *pmeta, const double2 *pmd,
                            const double2 *q, const double *omega, double2 *partial, const WnShape &sh,
                            const WnScreen &sc, hipStream_t st);
-// BCD moments or AHC curvature per chunk of k points: partial[chunk][comp][mu] (comp = a + 3b (+ 9c)).
+// BCD moments or AHC curvature per chunk of k points: partial[chunk][comp][mu] (comp = a + 3b (+ 9c)),
+// each point's value times wts[k] when wts is not null (weighted point lists).
 // flags[2] |= 1 and flags[3] = min global k index on a spectral isolation failure; flags[2] |= 2 on a
 // non-finite value.
 hipError_t wn_launch_geom_f64(const double *x, const double *e, const double *mu, const int4 *chunks, int nchunks,
-                              double2 *partial, int *flags, long long kbase, const WnShape &sh, const WnGeom &gp,
-                              hipStream_t st);
+                              const double *wts, double2 *partial, int *flags, long long kbase, const WnShape &sh,
+                              const WnGeom &gp, hipStream_t st);
 // SHG: optical position derivatives der[k][0..8][n][n] and the operands of their commutator products
 // (der holds WN_SHG_DER_BLOCKS n^2 complex per k point), pair lists and coefficient rows for `count_k` k
 // points, then the frequency contraction of `nchunks` chunks into partial[chunk] = [19 nc][nw] complex

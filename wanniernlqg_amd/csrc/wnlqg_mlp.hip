@@ -1,10 +1,11 @@
-// Learned refinement proposal: a 3 -> 64 -> 64 -> 1 SiLU multilayer perceptron in FP32 with
+// Learned refinement proposal: an 8 -> 64 -> 64 -> 1 SiLU multilayer perceptron in FP32 with
 // hand-written training and inference kernels.  It has no counterpart in the reference (which has no
-// learned component); it ranks integration cells of a k mesh for adaptive refinement from the
-// per-cell integrand variance of cheaper plain runs.  The integrand itself is always computed exactly.
+// learned component); it ranks integration cells of a k mesh for adaptive refinement from per-cell
+// features of the base mesh, trained on the refinement gains of cheaper coarse runs.  The integrand
+// itself is always computed exactly.
 //
-// Parameter vector (float, WNLQG_MLP_PARAMS = 4481):
-//   W1[64][3] | b1[64] | W2[64][64] | b2[64] | W3[64] | b3
+// Parameter vector (float, WNLQG_MLP_PARAMS = 4801):
+//   W1[64][8] | b1[64] | W2[64][64] | b2[64] | W3[64] | b3
 // Loss: mean squared error over the samples.  Gradients are deterministic: every wave owns a fixed
 // contiguous sample range and keeps its partial gradient in registers; partial gradients are summed
 // in wave order by a second kernel.  Training uses Adam.
@@ -23,7 +24,7 @@
 #include <vector>
 
 #define MLP_H 64
-#define MLP_IN 3
+#define MLP_IN WNLQG_MLP_INPUTS
 #define OFF_W1 0
 #define OFF_B1 (OFF_W1 + MLP_H * MLP_IN)
 #define OFF_W2 (OFF_B1 + MLP_H)
@@ -56,16 +57,26 @@ __global__ __launch_bounds__(64 * MLP_WAVES) void k_mlp_grad(const float *__rest
     __syncthreads();
     const int lane = threadIdx.x & 63;
     const long long wave = (long long)blockIdx.x * MLP_WAVES + (threadIdx.x >> 6);
-    const float w1[3] = {P[OFF_W1 + lane * 3], P[OFF_W1 + lane * 3 + 1], P[OFF_W1 + lane * 3 + 2]};
+    float w1[MLP_IN], gw1[MLP_IN];
+#pragma unroll
+    for (int c = 0; c < MLP_IN; ++c) {
+        w1[c] = P[OFF_W1 + lane * MLP_IN + c];
+        gw1[c] = 0.0f;
+    }
     const float b1 = P[OFF_B1 + lane], b2 = P[OFF_B2 + lane], w3 = P[OFF_W3 + lane], b3 = P[OFF_B3];
-    float gw1[3] = {0.0f, 0.0f, 0.0f}, gb1 = 0.0f, gb2 = 0.0f, gw3 = 0.0f, gb3 = 0.0f, lsum = 0.0f;
+    float gb1 = 0.0f, gb2 = 0.0f, gw3 = 0.0f, gb3 = 0.0f, lsum = 0.0f;
     float gw2[MLP_H];
 #pragma unroll
     for (int i = 0; i < MLP_H; ++i) gw2[i] = 0.0f;
     const long long s0 = wave * per_wave, s1 = s0 + per_wave < n ? s0 + per_wave : n;
     for (long long s = s0; s < s1; ++s) {
-        const float x0 = x[3 * s], x1 = x[3 * s + 1], x2 = x[3 * s + 2];
-        const float z1 = b1 + w1[0] * x0 + w1[1] * x1 + w1[2] * x2;
+        float xs[MLP_IN];
+        float z1 = b1;
+#pragma unroll
+        for (int c = 0; c < MLP_IN; ++c) {
+            xs[c] = x[MLP_IN * s + c];
+            z1 += w1[c] * xs[c];
+        }
         const float s1g = sigmoidf_(z1);
         const float h1 = z1 * s1g;
         float z2 = b2;
@@ -90,14 +101,12 @@ __global__ __launch_bounds__(64 * MLP_WAVES) void k_mlp_grad(const float *__rest
         for (int j = 0; j < MLP_H; ++j) dh1 += W2s[j * (MLP_H + 1) + lane] * __shfl(dz2, j);
         const float dz1 = dh1 * (s1g * (1.0f + z1 * (1.0f - s1g)));
         gb1 += dz1;
-        gw1[0] += dz1 * x0;
-        gw1[1] += dz1 * x1;
-        gw1[2] += dz1 * x2;
+#pragma unroll
+        for (int c = 0; c < MLP_IN; ++c) gw1[c] += dz1 * xs[c];
     }
     float *g = partial + wave * MLP_NP;
-    g[OFF_W1 + lane * 3] = gw1[0];
-    g[OFF_W1 + lane * 3 + 1] = gw1[1];
-    g[OFF_W1 + lane * 3 + 2] = gw1[2];
+#pragma unroll
+    for (int c = 0; c < MLP_IN; ++c) g[OFF_W1 + lane * MLP_IN + c] = gw1[c];
     g[OFF_B1 + lane] = gb1;
 #pragma unroll
     for (int i = 0; i < MLP_H; ++i) g[OFF_W2 + lane * MLP_H + i] = gw2[i];
@@ -152,11 +161,15 @@ __global__ __launch_bounds__(256) void k_mlp_predict(const float *__restrict__ P
     __syncthreads();
     const long long s = (long long)blockIdx.x * blockDim.x + threadIdx.x;
     if (s >= n) return;
-    const float x0 = x[3 * s], x1 = x[3 * s + 1], x2 = x[3 * s + 2];
+    float xs[MLP_IN];
+#pragma unroll
+    for (int c = 0; c < MLP_IN; ++c) xs[c] = x[MLP_IN * s + c];
     float h1[MLP_H];
 #pragma unroll
     for (int i = 0; i < MLP_H; ++i) {
-        const float z = Ps[OFF_B1 + i] + Ps[OFF_W1 + 3 * i] * x0 + Ps[OFF_W1 + 3 * i + 1] * x1 + Ps[OFF_W1 + 3 * i + 2] * x2;
+        float z = Ps[OFF_B1 + i];
+#pragma unroll
+        for (int c = 0; c < MLP_IN; ++c) z += Ps[OFF_W1 + MLP_IN * i + c] * xs[c];
         h1[i] = z * sigmoidf_(z);
     }
     float out = tail[OFF_B3 - OFF_B2];
@@ -255,8 +268,8 @@ int scratch_address(int device, float **out) {
 namespace {
 
 int upload_samples(wnlqg_mlp *g, const float *x, const float *t, long long n) {
-    MLP_HIP(g->x.ensure((size_t)3 * n));
-    MLP_HIP(hipMemcpy(g->x.p, x, (size_t)3 * n * sizeof(float), hipMemcpyHostToDevice));
+    MLP_HIP(g->x.ensure((size_t)MLP_IN * n));
+    MLP_HIP(hipMemcpy(g->x.p, x, (size_t)MLP_IN * n * sizeof(float), hipMemcpyHostToDevice));
     if (t) {
         MLP_HIP(g->t.ensure((size_t)n));
         MLP_HIP(hipMemcpy(g->t.p, t, (size_t)n * sizeof(float), hipMemcpyHostToDevice));
@@ -410,12 +423,14 @@ int32_t wnlqg_mlp_train(wnlqg_mlp *g, const float *x, const float *t, int64_t n,
                         float lr, uint64_t seed, float *loss_history) {
     if (!g || !x || !t || n < 1 || steps < 0 || batch < 1 || !(lr > 0.0f))
         return mlp_fail(WNLQG_E_INVALID, "invalid arguments");
-    for (int64_t i = 0; i < n; ++i)
-        if (!std::isfinite(t[i]) || !std::isfinite(x[3 * i]) || !std::isfinite(x[3 * i + 1]) || !std::isfinite(x[3 * i + 2]))
-            return mlp_fail(WNLQG_E_NONFINITE, "non-finite training sample %lld", (long long)i);
+    for (int64_t i = 0; i < n; ++i) {
+        bool ok = std::isfinite(t[i]);
+        for (int c = 0; c < MLP_IN; ++c) ok = ok && std::isfinite(x[MLP_IN * i + c]);
+        if (!ok) return mlp_fail(WNLQG_E_NONFINITE, "non-finite training sample %lld", (long long)i);
+    }
     const long long b = std::min<long long>(batch, n);
     const long long per_epoch = std::max<long long>(1, n / b);
-    std::vector<float> xs((size_t)3 * n), ts((size_t)n);
+    std::vector<float> xs((size_t)MLP_IN * n), ts((size_t)n);
     std::vector<long long> perm((size_t)n);
     for (long long i = 0; i < n; ++i) perm[i] = i;
     std::mt19937_64 rng(seed);
@@ -426,16 +441,14 @@ int32_t wnlqg_mlp_train(wnlqg_mlp *g, const float *x, const float *t, int64_t n,
             // new epoch: reshuffle on the host (deterministic in the seed) and upload
             std::shuffle(perm.begin(), perm.end(), rng);
             for (long long i = 0; i < n; ++i) {
-                xs[3 * i] = x[3 * perm[i]];
-                xs[3 * i + 1] = x[3 * perm[i] + 1];
-                xs[3 * i + 2] = x[3 * perm[i] + 2];
+                for (int c = 0; c < MLP_IN; ++c) xs[MLP_IN * i + c] = x[MLP_IN * perm[i] + c];
                 ts[i] = t[perm[i]];
             }
             MLP_HIP(hipStreamSynchronize(g->stream));
             int rc = upload_samples(g, xs.data(), ts.data(), n);
             if (rc) return rc;
         }
-        int rc = launch_gradient(g, g->x.p + 3 * e * b, g->t.p + e * b, b);
+        int rc = launch_gradient(g, g->x.p + (size_t)MLP_IN * e * b, g->t.p + e * b, b);
         if (rc) return rc;
         ++g->step;
         const float c1 = 1.0f - std::pow(0.9f, (float)g->step), c2 = 1.0f - std::pow(0.999f, (float)g->step);

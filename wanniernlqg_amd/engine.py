@@ -90,7 +90,7 @@ class Stats(C.Structure):
 _lib = None
 
 
-ABI_VERSION = 7   # WNLQG_ABI_VERSION of include/wnlqg.h
+ABI_VERSION = 8   # WNLQG_ABI_VERSION of include/wnlqg.h
 # fourier_backend names -> WNLQG_FOURIER_*: the reference's "auto", "direct", "mixed" and this build's "planes"
 FOURIER_BACKENDS = {"auto": 0, "direct": 1, "planes": 2, "mixed": 3}
 
@@ -305,8 +305,9 @@ class Engine:
         return lanes, total, st.as_dict()
 
     def run_points(self, kpoints: np.ndarray, weights: np.ndarray, n_lanes: Optional[int] = None):
-        """Weighted point list (SC / IC engines): returns (lanes, total) like run_mesh, each lane the sum
-        of weights[i] * value(k_i) over its items (the k-loop of a symmetry-reduced mesh)."""
+        """Weighted point list (SC / IC, BCD and AHC engines): returns (lanes, total) like run_mesh, each
+        lane the sum of weights[i] * value(k_i) over its items (the k-loop of a symmetry-reduced mesh, or
+        sub-cell nodes of a refinement: weights 1/N_k over a whole mesh give run_mesh's total)."""
         k = np.ascontiguousarray(kpoints, dtype=np.float64).reshape(-1, 3)
         w = np.ascontiguousarray(weights, dtype=np.float64).reshape(-1)
         if w.shape[0] != k.shape[0]:

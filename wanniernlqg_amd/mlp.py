@@ -1,5 +1,5 @@
 """ctypes binding of the learned refinement proposal network (``wnlqg_mlp_*`` in include/wnlqg.h):
-a 3 -> 64 -> 64 -> 1 SiLU perceptron trained and evaluated by the library's own HIP kernels.
+an 8 -> 64 -> 64 -> 1 SiLU perceptron trained and evaluated by the library's own HIP kernels.
 No CPU fallback: the library must be built."""
 
 from __future__ import annotations
@@ -11,9 +11,9 @@ import numpy as np
 
 from .engine import load_library
 
-PARAMS = 4481
+PARAMS = 4801
 HIDDEN = 64
-INPUTS = 3
+INPUTS = 8
 
 
 class MlpError(RuntimeError):
@@ -63,13 +63,13 @@ class ProposalNetwork:
         self._check(self.lib.wnlqg_mlp_set_params(self.handle, _fp(p)))
 
     def predict(self, x: np.ndarray) -> np.ndarray:
-        x = np.ascontiguousarray(x, dtype=np.float32).reshape(-1, 3)
+        x = np.ascontiguousarray(x, dtype=np.float32).reshape(-1, INPUTS)
         y = np.zeros(x.shape[0], dtype=np.float32)
         self._check(self.lib.wnlqg_mlp_predict(self.handle, _fp(x), x.shape[0], _fp(y)))
         return y
 
     def gradient(self, x: np.ndarray, t: np.ndarray):
-        x = np.ascontiguousarray(x, dtype=np.float32).reshape(-1, 3)
+        x = np.ascontiguousarray(x, dtype=np.float32).reshape(-1, INPUTS)
         t = np.ascontiguousarray(t, dtype=np.float32).reshape(-1)
         g = np.zeros(PARAMS, dtype=np.float32)
         loss = C.c_float()
@@ -77,7 +77,7 @@ class ProposalNetwork:
         return g, float(loss.value)
 
     def gradient_noalloc(self, x_ptr: int, t_ptr: int, n: int, grad_ptr: int, stream: int = 0):
-        """wnlqg_mlp_gradient_noalloc on device buffers (addresses): x [n][3], t [n], grad [PARAMS + 1] floats;
+        """wnlqg_mlp_gradient_noalloc on device buffers (addresses): x [n][INPUTS], t [n], grad [PARAMS + 1] floats;
         grad receives the gradient of the mean squared error and then the sum of squared errors.  stream 0:
         the network's own stream, synchronized before returning."""
         self._check(self.lib.wnlqg_mlp_gradient_noalloc(self.handle, C.c_void_p(x_ptr), C.c_void_p(t_ptr), int(n),
@@ -85,7 +85,7 @@ class ProposalNetwork:
 
     def train(self, x: np.ndarray, t: np.ndarray, steps: int, batch: int = 4096, lr: float = 3e-3,
               seed: int = 0, history: bool = True) -> Optional[np.ndarray]:
-        x = np.ascontiguousarray(x, dtype=np.float32).reshape(-1, 3)
+        x = np.ascontiguousarray(x, dtype=np.float32).reshape(-1, INPUTS)
         t = np.ascontiguousarray(t, dtype=np.float32).reshape(-1)
         h = np.zeros(steps, dtype=np.float32) if history else None
         self._check(self.lib.wnlqg_mlp_train(self.handle, _fp(x), _fp(t), x.shape[0], steps, batch, C.c_float(lr),

@@ -1,7 +1,7 @@
 """Learned adaptive refinement of the dense-mesh Berry-curvature dipole (the learned-proposal form).
 
 The reference has no learned component and no local refinement; this module is new and is judged
-only against the engine's own uniform dense mesh.  Scheme:
+against the engine's own uniform dense mesh.  Scheme:
 
 * Cells.  A Gamma-centred N^3 mesh (``integral_kpoint!``, reference
   ``src/Core/CalculationUtils.jl:280-292``) is read as N^3 cells centred on the mesh points; the
@@ -9,18 +9,22 @@ only against the engine's own uniform dense mesh.  Scheme:
 * Refinement.  A refined cell is replaced by the mean over its r^3 sub-cell centres, r odd, which
   are exactly the points of the uniform (N r)^3 mesh inside the cell (r = 3, 5, ..., 15).  Refining
   every cell therefore reproduces the uniform (N r)^3 result, which serves as both the fallback and
-  the judge of the error.
+  the judge of the error.  The sub-cell sums run on the GPU (``Engine.run_points`` with one lane per
+  cell), so refinements of 15 per axis keep the sub-cell moments on the device.
 * Features.  Every evaluation of the integrand also returns the band energies of the same point
-  (``Engine.eval_points(..., energies=True)``).  A cell is described by three spectral features of
-  its periodic 3 x 3 x 3 neighbourhood of mesh points: the smallest spacing of adjacent levels near
-  the chemical potential, the smallest distance of a level to the chemical potential in units of the
-  Fermi-surface energy scale (k_B T, or the Gaussian width at T = 0), and the largest integrand norm.
-  They are what makes a cell "hot" for a Fermi-surface integrand (near-degenerate pairs at the Fermi
-  level), stated in units that do not depend on the mesh spacing.
-* Proposal.  The 3 -> 64 -> 64 -> 1 SiLU network (``ProposalNetwork``, trained and evaluated by the
-  library's HIP kernels) maps the scaled features to log10 of the cell's integrand variance.  Training
-  targets come from plain runs on coarser meshes (variance of ||D(k)|| over each cell's periodic
-  3 x 3 x 3 neighbourhood).  Cells of the base mesh are refined in order of the predicted score.
+  (``Engine.eval_points(..., energies=True)``).  A cell is described by eight numbers from the base
+  mesh alone, in units that do not depend on the mesh spacing: where the nearest Fermi surface lies
+  (|E_n - mu| over the band's central-difference slope, in cell units), the thickness of its thermal
+  shell (the Fermi-surface energy scale over that slope), how deep the trilinear interpolation of
+  E_n - mu over the cell's centre and corners straddles mu, the smallest spacing of adjacent levels
+  near mu at the centre and over the 3 x 3 x 3 neighbourhood, the integrand norm at the centre and
+  its neighbourhood maximum, and the neighbourhood variance of the norm (the non-learned ranking's
+  own signal).
+* Proposal.  The 8 -> 64 -> 64 -> 1 SiLU network (``ProposalNetwork``, trained and evaluated by the
+  library's HIP kernels) maps the standardized features to log10 of the cell's refinement gain
+  |mean over its sub-cells - centre value| relative to the integral.  Training targets come from
+  coarse meshes refined everywhere (cheap: a few percent of the judged dense mesh).  Cells of the
+  base mesh are refined in order of the predicted gain.
 * The integrand is always the exact per-point BCD moment of the engine.
 """
 
@@ -35,8 +39,13 @@ from .mlp import ProposalNetwork
 
 KB_EV = 8.617333262145e-5          # Boltzmann constant, eV/K (Core/Constants.jl)
 GAP_WINDOW_EV = 0.25               # levels whose midpoint lies within this of mu count for the gap feature
-DISTANCE_CAP = 60.0                # |E - mu| / energy scale beyond which a level is "far"
-NORM_DECADES = 8.0                 # decades of ||D|| below the training maximum mapped to [-1, 1]
+SURFACE_CAP = 20.0                 # Fermi-surface distances (cell units) beyond this are "far"
+NORM_FLOOR = -12.0                 # log10 ||D|| / training maximum below this counts as zero
+VARIANCE_FLOOR = -30.0             # the same for the neighbourhood variance (relative to the maximum^2)
+TARGET_DECADES = 15.0              # training gains more than this many decades below the largest are floored
+FEATURES = ("log10 Fermi-surface distance", "log10 thermal shell", "log10 gap at the centre",
+            "log10 gap in the neighbourhood", "log10 norm, neighbourhood maximum", "log10 norm at the centre",
+            "log10 neighbourhood variance", "Fermi-surface crossing depth")
 
 
 def mesh_points(n: int) -> np.ndarray:
@@ -101,19 +110,105 @@ def point_spectral_features(energies: np.ndarray, mu: float, scale: float) -> np
     return np.stack([np.log10(np.maximum(gap, 1e-300)), dist], axis=1)
 
 
+def shifted(values: np.ndarray, n: int, d) -> np.ndarray:
+    """values at the mesh point i + d (periodic), d = (d1, d2, d3) in mesh steps; values [n^3, ...], axis 1
+    fastest."""
+    f = values.reshape((n, n, n) + values.shape[1:])      # [i3, i2, i1, ...]
+    return np.roll(f, (-d[2], -d[1], -d[0]), axis=(0, 1, 2)).reshape(values.shape)
+
+
+def band_slopes(energies: np.ndarray, n: int) -> np.ndarray:
+    """|grad E_n| times the mesh step (eV per cell) of every band at every point: central differences over
+    the periodic mesh, [n^3, n_bands] (levels in ascending order)."""
+    e = np.asarray(energies, dtype=np.float64)
+    sq = np.zeros_like(e)
+    for a in range(3):
+        d = [0, 0, 0]
+        d[a] = 1
+        g = 0.5 * (shifted(e, n, d) - shifted(e, n, [-x for x in d]))
+        sq += g * g
+    return np.sqrt(sq)
+
+
+def fermi_surface_features(energies: np.ndarray, n: int, mu: float, scale: float) -> np.ndarray:
+    """Per cell [n^3, 3]: the distance of the nearest Fermi surface from the centre in cell units
+    (min_n |E_n - mu| / slope_n, capped at SURFACE_CAP), the thermal shell thickness of that band in cell
+    units (scale / slope), and the crossing depth max_n min(max_c (E_n - mu), max_c (mu - E_n)) / slope_n
+    over the centre and the 8 corners c of the cell (a corner value is the mean of the 8 mesh points
+    around it: trilinear interpolation), positive when the interpolated level straddles mu inside the
+    cell, clipped to [-5, 2]."""
+    e = np.asarray(energies, dtype=np.float64) - mu
+    slope = band_slopes(energies, n) + 1e-12
+    ratio = np.abs(e) / slope
+    j = ratio.argmin(axis=1)
+    rows = np.arange(e.shape[0])
+    dist = np.minimum(ratio[rows, j], SURFACE_CAP)
+    shell = scale / slope[rows, j]
+    lo = e.copy()
+    hi = e.copy()
+    for s3 in (-1, 1):
+        for s2 in (-1, 1):
+            for s1 in (-1, 1):
+                corner = np.zeros_like(e)
+                for a in (0, 1):
+                    for b in (0, 1):
+                        for c in (0, 1):
+                            corner += shifted(e, n, (c * s1, b * s2, a * s3))
+                corner *= 0.125
+                lo = np.minimum(lo, corner)
+                hi = np.maximum(hi, corner)
+    depth = np.clip((np.minimum(hi, -lo) / slope).max(axis=1), -5.0, 2.0)
+    return np.stack([dist, shell, depth], axis=1)
+
+
 def cell_features(energies: np.ndarray, norms: np.ndarray, n: int, mu: float, scale: float,
                   norm_ref: float) -> np.ndarray:
-    """Scaled network inputs [n^3, 3] of the n^3 cells: neighbourhood minimum of the log gap and of
-    the distance to mu, neighbourhood maximum of log10 ||D|| (relative to log10 norm_ref)."""
+    """Raw network inputs [n^3, 8] of the n^3 cells, in the order of FEATURES (see the module notes);
+    the network sees them standardized with the training set's mean and spread."""
+    fs = fermi_surface_features(energies, n, mu, scale)
     pf = point_spectral_features(energies, mu, scale)
-    gap = neighbourhood(pf[:, 0], n, np.minimum)
-    dist = neighbourhood(pf[:, 1], n, np.minimum)
-    lnorm = neighbourhood(np.log10(np.maximum(norms, 1e-300)), n, np.maximum)
-    x = np.empty((n ** 3, 3), dtype=np.float32)
-    x[:, 0] = np.clip((gap + 2.0) / 2.0, -1.0, 1.0)                        # 1e-4 eV -> -1, 1 eV -> 1
-    x[:, 1] = np.clip(dist, 0.0, DISTANCE_CAP) / (0.5 * DISTANCE_CAP) - 1.0
-    x[:, 2] = np.clip((lnorm - np.log10(norm_ref)) / NORM_DECADES + 1.0, -1.0, 1.0)
+    norms = np.asarray(norms, dtype=np.float64)
+    ln = np.maximum(np.log10(np.maximum(norms, 1e-300)) - np.log10(norm_ref), NORM_FLOOR)
+    lv = np.maximum(local_log_variance(norms, n, relative_floor=1e-12) - 2.0 * np.log10(norm_ref), VARIANCE_FLOOR)
+    x = np.empty((n ** 3, len(FEATURES)))
+    x[:, 0] = np.log10(fs[:, 0] + 1e-3)
+    x[:, 1] = np.log10(fs[:, 1])
+    x[:, 2] = pf[:, 0]
+    x[:, 3] = neighbourhood(pf[:, 0], n, np.minimum)
+    x[:, 4] = neighbourhood(ln, n, np.maximum)
+    x[:, 5] = ln
+    x[:, 6] = lv
+    x[:, 7] = fs[:, 2]
     return x
+
+
+def refinement_gains(base: np.ndarray, refined: np.ndarray, mu_index: int = 0) -> np.ndarray:
+    """|refined mean - centre value| per cell (Frobenius norm over the 27 components at one chemical
+    potential)."""
+    d = (np.asarray(refined) - np.asarray(base))[:, mu_index].reshape(base.shape[0], -1)
+    return np.sqrt(np.sum(d * d, axis=1))
+
+
+def error_curve(order: np.ndarray, base: np.ndarray, refined: np.ndarray) -> np.ndarray:
+    """Relative error (Frobenius norm over all components) of the estimate that refines the first b cells
+    of ``order``, against the estimate that refines every cell (the uniform (N r)^3 mesh), for
+    b = 0 .. N: [N + 1]."""
+    n = base.shape[0]
+    delta = (np.asarray(refined) - np.asarray(base)).reshape(n, -1)[np.asarray(order)]
+    total = np.asarray(refined).reshape(n, -1).sum(axis=0)
+    rem = delta.sum(axis=0)[None, :] - np.vstack([np.zeros((1, delta.shape[1])), np.cumsum(delta, axis=0)])
+    return np.linalg.norm(rem, axis=1) / np.linalg.norm(total)
+
+
+def cells_needed(curve: np.ndarray, tol: float) -> int:
+    """Smallest refined-cell count b from which the error stays below tol for every larger count."""
+    bad = np.flatnonzero(~(curve < tol))
+    return 0 if bad.size == 0 else int(bad[-1] + 1)
+
+
+def variance_order(n: int, base: np.ndarray, mu_index: int = 0) -> np.ndarray:
+    """The non-learned ranking: cells in decreasing neighbourhood variance of the base-mesh integrand norm."""
+    return np.argsort(-local_log_variance(point_scalar(base, mu_index), n), kind="stable")
 
 
 @dataclass
@@ -143,6 +238,8 @@ class LearnedRefinement:
         else:
             raise ValueError("the Fermi-surface energy scale needs temperature > 0 or fermi_surface_broadening > 0")
         self.norm_ref = None
+        self.x_mean = np.zeros(len(FEATURES))
+        self.x_std = np.ones(len(FEATURES))
         self.target_shift = 0.0
 
     def point_data(self, k: np.ndarray):
@@ -155,30 +252,66 @@ class LearnedRefinement:
         self.evaluations += k.shape[0]
         return self.engine.eval_points(k)["bcd"]
 
+    def refined_cell_means(self, cells: np.ndarray, n: int, r: int, max_points: int = 1 << 24) -> np.ndarray:
+        """Mean BCD moments over the r^3 sub-cell centres of each listed cell of the n^3 mesh:
+        [len(cells), n_mu, 3, 3, 3].  The sums run on the device (``Engine.run_points``, one lane per cell,
+        weights 1/r^3), so the sub-cell moments never leave the GPU; at most ``max_points`` nodes per call.
+        Counted as len(cells) r^3 evaluations."""
+        cells = np.asarray(cells, dtype=np.int64).reshape(-1)
+        r3 = r ** 3
+        per = max(1, max_points // r3)
+        out = []
+        for c0 in range(0, cells.size, per):
+            cc = cells[c0:c0 + per]
+            k = subcell_points(cc, n, r)
+            lanes, _ = self.engine.run_points(k, np.full(k.shape[0], 1.0 / r3), cc.size)
+            out.append(lanes["bcd"])
+        self.evaluations += cells.size * r3
+        if not out:
+            nmu = np.asarray(self.engine.task.fermi_energies).reshape(-1).shape[0]
+            return np.zeros((0, nmu, 3, 3, 3))
+        return np.concatenate(out)
+
     def features(self, n: int, moments: np.ndarray, energies: np.ndarray) -> np.ndarray:
+        """Standardized network inputs [n^3, 8] (float32) of the n^3 cells from the base-mesh data."""
         if self.norm_ref is None:
             raise RuntimeError("train the proposal network first (it fixes the integrand scale)")
-        return cell_features(energies, point_scalar(moments, self.mu_index), n, self.mu, self.scale, self.norm_ref)
+        x = cell_features(energies, point_scalar(moments, self.mu_index), n, self.mu, self.scale, self.norm_ref)
+        return ((x - self.x_mean) / self.x_std).astype(np.float32)
 
-    def coarse_dataset(self, n: int, relative_floor: float = 1.0e-12):
-        """Plain n^3 run: raw data for the features and log10 local-variance targets."""
+    def coarse_dataset(self, n: int, r: int):
+        """Coarse n^3 run refined everywhere by r: base moments, energies and the log10 refinement gain of
+        every cell relative to the norm of the refined integral (n^3 r^3 evaluations, on the device)."""
         vals, energies = self.point_data(mesh_points(n))
-        target = local_log_variance(point_scalar(vals, self.mu_index), n, relative_floor=relative_floor)
-        return vals, energies, target.astype(np.float32)
+        means = self.refined_cell_means(np.arange(n ** 3), n, r)
+        self.evaluations -= n ** 3          # the refined means contain the centres
+        gains = refinement_gains(vals, means, self.mu_index)
+        total = np.linalg.norm(means[:, self.mu_index].mean(axis=0))
+        if not total > 0.0:
+            raise RuntimeError("the integrand vanishes on the training meshes")
+        return vals, energies, np.log10(gains / total + 1e-300)
 
-    def train(self, meshes: Sequence[int] = (32, 48), steps: int = 6000, batch: int = 4096, lr: float = 2e-3,
-              seed: int = 0):
-        runs = [(n,) + self.coarse_dataset(n) for n in meshes]
+    def train(self, meshes: Sequence[int] = (16, 20, 24), r: int = 5, steps: int = 6000, batch: int = 4096,
+              lr: float = 2e-3, seed: int = 0):
+        """Fit the network to the refinement gains of coarse meshes refined everywhere by r; targets more
+        than TARGET_DECADES below the largest are floored there (they never decide a ranking).  Returns the
+        per-step loss history."""
+        runs = [(n,) + self.coarse_dataset(n, r) for n in meshes]
         self.norm_ref = max(float(point_scalar(v, self.mu_index).max()) for _, v, _, _ in runs)
         if not self.norm_ref > 0.0:
             raise RuntimeError("the integrand vanishes on the training meshes")
-        x = np.concatenate([self.features(n, v, e) for n, v, e, _ in runs])
+        x = np.concatenate([cell_features(e, point_scalar(v, self.mu_index), n, self.mu, self.scale, self.norm_ref)
+                            for n, v, e, _ in runs])
+        self.x_mean = x.mean(axis=0)
+        self.x_std = x.std(axis=0) + 1e-9
         t = np.concatenate([t for _, _, _, t in runs])
+        t = np.maximum(t, t.max() - TARGET_DECADES)
         self.target_shift = float(np.median(t))
-        return self.network.train(x, t - self.target_shift, steps, batch, lr, seed)
+        xs = ((x - self.x_mean) / self.x_std).astype(np.float32)
+        return self.network.train(xs, (t - self.target_shift).astype(np.float32), steps, batch, lr, seed)
 
     def scores(self, n: int, moments: np.ndarray, energies: np.ndarray) -> np.ndarray:
-        """Predicted log10 variance (minus the training median) of the n^3 cells from the base data."""
+        """Predicted log10 refinement gain (minus the training median) of the n^3 cells from the base data."""
         return self.network.predict(self.features(n, moments, energies))
 
     def estimate(self, n: int, r: int, budget_cells: int, order: Optional[np.ndarray] = None,
@@ -199,9 +332,7 @@ class LearnedRefinement:
         cells = np.asarray(order[:budget_cells])
         vals = base.copy()
         if cells.size:
-            sub = self.point_moments(subcell_points(cells, n, r))
-            r3 = r ** 3
-            vals[cells] = sub.reshape((cells.size, r3) + sub.shape[1:]).mean(axis=1)
+            vals[cells] = self.refined_cell_means(cells, n, r)
             self.evaluations -= cells.size      # each refined cell's centre is one of its sub-cell centres
         est = vals.mean(axis=0) / self.volume
         return AdaptiveResult(est, self.evaluations - before, int(cells.size))
