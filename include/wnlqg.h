@@ -292,7 +292,8 @@ int32_t wnlqg_batch_create(int32_t count, const wnlqg_model *models, const wnlqg
                            const int32_t *n_lanes, int32_t autotune_tiles, wnlqg_batch **out);
 int32_t wnlqg_batch_run(wnlqg_batch *batch, double *const *lane_out, double *const *total_out, double *elapsed_ms);
 /* the eigensolver form chosen for entry `index`: 64, 128 or 256 (threads per k point of the LDS Jacobi),
- * 1 (register-resident eigenvectors, n_w <= 32) or 0 (size-class default) */
+ * 1 (register-resident eigenvectors, n_w <= 32), 2 (deferred eigenvectors: H-only sweeps with recorded
+ * rotations applied to V afterwards, n_w <= 32) or 0 (size-class default) */
 int32_t wnlqg_batch_tuning(const wnlqg_batch *batch, int32_t index, int32_t *eig_threads);
 int32_t wnlqg_batch_destroy(wnlqg_batch *batch);
 

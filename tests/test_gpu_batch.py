@@ -39,7 +39,7 @@ def test_batch_matches_individual_engines():
         tuning = b.tuning()
         r1, ms1 = b.run()
         r2, ms2 = b.run()
-    assert all(t in (1, 64, 128, 256) for t in tuning)
+    assert all(t in (1, 2, 64, 128, 256) for t in tuning)
     for (l1, t1), (l2, t2) in zip(r1, r2):
         for key in t1:
             assert np.array_equal(l1[key], l2[key]) and np.array_equal(t1[key], t2[key])
@@ -65,7 +65,7 @@ def test_batch_with_large_shg_entry_and_autotune():
     with E.Batch(entries, autotune=True) as b:
         tuning = b.tuning()
         res, _ = b.run()
-    assert tuning[0] == 0 and tuning[1] in (1, 64, 128, 256)
+    assert tuning[0] == 0 and tuning[1] in (1, 2, 64, 128, 256)
     for (model, task, mesh), (lanes, total) in zip(entries, res):
         with E.Engine(model, task) as eng:
             l0, t0, _ = eng.run_mesh(mesh)

@@ -271,6 +271,53 @@ def test_plane_warm_started_eigensolver(n, mesh):
     assert st["mean_sweeps"] <= st["max_sweeps"]
 
 
+@pytest.mark.parametrize("n", [8, 12, 16, 24, 32])
+def test_deferred_eigensolver_matches_register_v(monkeypatch, n):
+    """The deferred-eigenvector Jacobi (H-only sweeps recording each round's rotations, then the recorded
+    rounds applied to register rows of V) performs the register-V kernel's rotations in the same order.  Cold
+    solves (point spectra), group and previous-plane warm starts (mesh runs) and the continuation kernel
+    (record depth lowered to 1 and 2 sweeps, so that most points are finished from their saved state) give
+    the register-V kernel's eigenpairs and mesh integrals to within 1e-13 (bitwise in practice) and the same
+    sweep counts; the mesh integrals also match the oracle."""
+    model = synthetic_model(n, 2, 90 + n, position_scale=0.1)
+    kp = np.random.default_rng(n).uniform(-0.5, 0.5, (300, 3))
+    kw = dict(photon_energies=np.linspace(0, 5, 60), broadening=0.04, batch_size=128)
+    meshes = [(9, 7, 3), (10, 10, 5)]
+    runs = {}
+    for form, rec in (("regv", None), ("defer", None), ("defer", "2"), ("defer", "1")):
+        monkeypatch.setenv("WNLQG_EIG_KERNEL", form)
+        if rec:
+            monkeypatch.setenv("WNLQG_EIGD_RECORD_SWEEPS", rec)
+        else:
+            monkeypatch.delenv("WNLQG_EIGD_RECORD_SWEEPS", raising=False)
+        say(f"[eig-defer] n={n} kernel={form} record_sweeps={rec or 8}")
+        with engine(model, **kw) as eng:
+            e, u = eng.spectrum_points(kp)
+            mesh = [eng.run_mesh(m) for m in meshes]
+        runs[(form, rec)] = (e, u, mesh)
+    monkeypatch.delenv("WNLQG_EIG_KERNEL")
+    monkeypatch.delenv("WNLQG_EIGD_RECORD_SWEEPS", raising=False)
+    e0, u0, mesh0 = runs[("regv", None)]
+    for key, (e, u, mesh) in runs.items():
+        if key[0] == "regv":
+            continue
+        de = np.max(np.abs(e - e0)) / np.max(np.abs(e0))
+        du = np.max(np.abs(u - u0))
+        say(f"  {key}: eigenvalues {de:.2e}, eigenvectors {du:.2e}, bitwise {np.array_equal(e, e0) and np.array_equal(u, u0)}")
+        assert de <= 1e-13 and du <= 1e-12
+        for (l1, t1, s1), (l0, t0, s0) in zip(mesh, mesh0):
+            assert s1["mean_sweeps"] == s0["mean_sweeps"] and s1["max_sweeps"] == s0["max_sweeps"]
+            for q in ("sc", "ic"):
+                assert rel(l1[q], l0[q]) < 1e-12
+    okw = {k: v for k, v in kw.items() if k != "batch_size"}
+    for m, (lanes, total, st) in zip(meshes, runs[("defer", None)][2]):
+        ol, ot = kloop.integrate_mesh(model, oracle_task(**okw), m, batch=64)
+        for q in ("sc", "ic"):
+            assert kloop.spectrum_relative_error(total[q], ot[q]).max() < 1e-10
+            assert rel(lanes[q], ol[q]) < 1e-10
+        say(f"  mesh {m}: sweeps mean {st['mean_sweeps']:.2f} max {st['max_sweeps']}, oracle parity ok")
+
+
 @pytest.mark.parametrize("backend,batch,mesh,nbatch", [("direct", 128, (6, 5, 7), 2), ("planes", 128, (6, 5, 7), 2),
                                                       ("planes", 128, (40, 6, 3), 6)])
 def test_mesh_lane_counts_and_batches(backend, batch, mesh, nbatch):

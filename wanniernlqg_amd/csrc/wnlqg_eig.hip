@@ -22,6 +22,8 @@
 // V's LDS traffic (every rotated column pair read and written each round) is gone; what remains in
 // LDS is the packed H (read and written once per block per round) and the parameter broadcast.
 
+#include <algorithm>
+
 #include "wnlqg_common.h"
 #include "wnlqg_kernels.h"
 
@@ -82,18 +84,18 @@ __device__ __forceinline__ void wn_rotate_row(WnRow<N> &V, const double4 R) {
 // One unrolled round of the V wave: every slot's rotation applied to this lane's row.
 // The parameters are loaded four slots at a time (a compiler fence between groups), so that the row and
 // the parameters in flight stay within the register budget of two waves per SIMD.
-template <int N, int R, int T>
+template <int N, int R, int T, int G = 4>
 struct WnVSlots {
-    __device__ __forceinline__ static void run(WnRow<N> &V, const double4 *__restrict__ rot) {
-        // parameters loaded four slots at a time (a compiler fence between groups)
-        if constexpr (T % 4 == 0) wn_wave_order();
+    __device__ __forceinline__ static void run(WnRow<N> &V, const double4 *rot) {
+        // parameters loaded G slots at a time (a compiler fence between groups)
+        if constexpr (T % G == 0) wn_wave_order();
         wn_rotate_row<N, wn_rr_a(N, T, R), wn_rr_b(N, T, R)>(V, rot[T]);
-        WnVSlots<N, R, T + 1>::run(V, rot);
+        WnVSlots<N, R, T + 1, G>::run(V, rot);
     }
 };
-template <int N, int R>
-struct WnVSlots<N, R, N / 2> {
-    __device__ __forceinline__ static void run(WnRow<N> &, const double4 *__restrict__) {}
+template <int N, int R, int G>
+struct WnVSlots<N, R, N / 2, G> {
+    __device__ __forceinline__ static void run(WnRow<N> &, const double4 *) {}
 };
 
 // The V wave's rounds of one sweep; the round's parameters are in rot[(cb + R) & 1].  The barrier of round 0
@@ -123,6 +125,11 @@ struct WnRowIo {
         V.template set<C>((ua && l < n && C < n) ? ua[l * n + C] : make_double2(l == C ? 1.0 : 0.0, 0.0));
         WnRowIo<N, C + 1>::load(V, ua, n, l);
     }
+    // row l as it is (columns in band order)
+    __device__ __forceinline__ static void store_raw(const WnRow<N> &V, double2 *__restrict__ uk, int n, int l) {
+        if (C < n) uk[l * n + C] = V.template get<C>();
+        WnRowIo<N, C + 1>::store_raw(V, uk, n, l);
+    }
     // canonical phase of column C (first largest |V[i][C]| over the rows i < n made real and >= 0), written
     // to column rank[C] of the output
     __device__ __forceinline__ static void store(const WnRow<N> &V, double2 *__restrict__ uk, const int *rank, int n,
@@ -151,6 +158,7 @@ template <int N>
 struct WnRowIo<N, N> {
     __device__ __forceinline__ static void load(WnRow<N> &, const double2 *__restrict__, int, int) {}
     __device__ __forceinline__ static void store(const WnRow<N> &, double2 *__restrict__, const int *, int, int, int) {}
+    __device__ __forceinline__ static void store_raw(const WnRow<N> &, double2 *__restrict__, int, int) {}
 };
 
 }  // namespace
@@ -271,7 +279,9 @@ __device__ __forceinline__ void wn_eig_h_role(WnEigShared<N> &sm, const double *
                                               double *__restrict__ eout, int *__restrict__ sweeps_out,
                                               unsigned long long *__restrict__ sweep_total, int n, int ldo,
                                               double tol, int k, bool active, int j, int l, int lane, int bucket,
-                                              int *__restrict__ status, int max_sweeps) {
+                                              int *__restrict__ status, int max_sweeps,
+                                              const double2 *__restrict__ hsrc = nullptr, int sweep0 = 0,
+                                              double prev0 = 0.0) {
     using S = WnSlots<N>;
     constexpr int HALF = N / 2;
     constexpr int NB = S::NB;
@@ -280,9 +290,14 @@ __device__ __forceinline__ void wn_eig_h_role(WnEigShared<N> &sm, const double *
     // block slots of a lane: slot 0 is slot cs's critical block on lanes l < HALF and one of the first
     // HALF non-critical blocks on the others; the remaining non-critical blocks fill BPL more slots
     constexpr int BPL = (NREST - HALF + N - 1) / N;
-    // H(k) upper triangle in band order -> slot order of round 0
-    const double2 *xk = active ? (const double2 *)(x + (size_t)k * ldo) : nullptr;
-    for (int e = l; e < N * N; e += N) {
+    // H(k) upper triangle in band order -> slot order of round 0; or, continuing a solve, the slot-ordered H
+    // saved at the start of sweep sweep0 (every band at pos0) into the buffer that sweep reads
+    if (hsrc) {
+        for (int e = l; e < 4 * WnSlots<N>::NB; e += N)
+            sm.Hs[sweep0 & 1][j][e] = active ? hsrc[e] : make_double2(0.0, 0.0);
+    }
+    const double2 *xk = active && !hsrc ? (const double2 *)(x + (size_t)k * ldo) : nullptr;
+    for (int e = l; e < N * N && !hsrc; e += N) {
         const int i = e / N, jj = e - (e / N) * N;
         if (i <= jj) {
             double2 hv = make_double2(0.0, 0.0);
@@ -365,9 +380,9 @@ __device__ __forceinline__ void wn_eig_h_role(WnEigShared<N> &sm, const double *
         }
     };
 
-    int sweep = 0, done_sweep = 0;
+    int sweep = sweep0, done_sweep = 0;
     bool frozen = !active, converged = !active;
-    double prev_off = 0.0;
+    double prev_off = prev0;
     for (;;) {
         const int cb = sweep & 1;   // buffer holding H at the start of this sweep (N - 1 rounds per sweep, odd)
         const int g0 = sweep * (N - 1);
@@ -464,10 +479,10 @@ __device__ __forceinline__ void wn_eig_h_role(WnEigShared<N> &sm, const double *
 template <int N>
 __device__ __forceinline__ void wn_eig_v_role(WnEigShared<N> &sm, double2 *__restrict__ uout,
                                               const double2 *__restrict__ ua, int n, int k, bool active, int j,
-                                              int l) {
+                                              int l, int sweep0 = 0) {
     WnRow<N> V;
     WnRowIo<N, 0>::load(V, ua, n, l);
-    for (int sweep = 0;; ++sweep)
+    for (int sweep = sweep0;; ++sweep)
         if (!WnVRounds<N, 0>::run(V, sm.rot, j, sweep & 1, sm.go_flag)) break;
     wn_wg_barrier();
     if (!active) return;
@@ -518,6 +533,446 @@ __global__ __launch_bounds__(128, 2) void k_eig_rv(const double *__restrict__ x,
                                                  : (anchor >= 0 ? uout + (size_t)anchor * n * n : nullptr));
         wn_eig_v_role<N>(sm, uout, ua, n, k, active, j, l);
     }
+}
+
+// ------------------------------------------------------------------------------------------------
+// Deferred-eigenvector Jacobi for n_w <= 32 (the default for these sizes).
+//
+// The register-V kernel above keeps each workgroup's H wave and V wave in lock step (one barrier per round)
+// and needs the V rows' 128 VGPRs in every wave of the kernel and two H buffers in LDS, so a CU holds only 8
+// points at N = 32 and the round latency is exposed.  Here the same rounds run in two kernels:
+//
+//   * k_eigd_sweeps: H alone, 64 / N points per wave (N lanes each), no barrier.  H is
+//     kept slot-ordered in ONE LDS buffer: every lane reads all the elements its blocks and its diagonal
+//     block need before any lane writes (the LDS executes one wave's accesses in issue order), so the moved
+//     elements can be written over the old ones.  Each round's rotation parameters go to the record buffer.
+//     Same parameters, same block updates, same stopping rule as k_eig_rv, so H evolves identically.
+//   * k_eigd_vapply: the recorded rounds applied to the rows of V in registers (one lane per row, the round
+//     loop unrolled for compile-time column indices, exactly k_eig_rv's V wave), the records streamed through
+//     a four-deep register pipeline and an LDS stage; then the canonical sorted eigenvectors.
+//
+// At most WN_EIGD_SWEEPS sweeps (WnShape::rec_sweeps) are recorded per point.  A point that needs more (rare: warm starts take 4 to
+// 5 sweeps, cold starts 7 to 8 at n = 32) saves its slot-ordered H, and after the V kernel has applied the
+// recorded sweeps, k_eig_rv_cont finishes it with the register-V kernel's roles from that state, so every
+// point follows the same sequence of rotations whatever path finishes it.
+// ------------------------------------------------------------------------------------------------
+#define WN_EIGD_SWEEPS 8
+#ifndef WN_EIGD_VG
+#define WN_EIGD_VG 2   // rotation parameters of the V kernel read this many slots at a time
+#endif
+#ifndef WN_EIGD_VWAVES
+#define WN_EIGD_VWAVES 2   // waves per SIMD the V kernel's registers are sized for
+#endif
+
+template <int N>
+struct WnEigd {
+    static constexpr int HALF = N / 2;
+    static constexpr int NB = WnSlots<N>::NB;
+    static constexpr int NOFF = HALF * (HALF - 1) / 2;
+    static constexpr int LPK = N;                      // lanes per point in the sweep kernel
+    static constexpr int KPW = 64 / LPK;               // points per wave in the sweep kernel
+    static constexpr int BPL = (NOFF + LPK - 1) / LPK; // off-diagonal blocks per lane
+    static constexpr int KPV = 64 / N;                 // points per wave in the V kernel (one lane per row)
+    static constexpr int RPS = N;                      // record rounds per sweep (N - 1 used, padded to N)
+    // per point: records [sweep][RPS][N/2] double4 (2 double2 each), saved H [4 NB], state (4 double2).
+    // RPS is a multiple of 4, so the V kernel's four-deep record pipeline has the same ring slots every sweep.
+    static constexpr size_t REC2 = (size_t)WN_EIGD_SWEEPS * RPS * HALF * 2;
+    static constexpr size_t PER_K = REC2 + 4 * NB + 4;
+};
+
+struct WnEigdState {
+    int sweeps;        // sweeps done
+    int nrec;          // sweeps recorded for the V kernel
+    int cont;          // 1: finished by k_eig_rv_cont from the saved H
+    int pad;
+    double prev_off;   // off-diagonal mass at the stopping test before the last one (continuation)
+    unsigned char rank[32];
+};
+static_assert(sizeof(WnEigdState) <= 64, "state slot");
+
+struct WnEigdScratch {
+    double2 *rec;           // [count][REC2]
+    double2 *hsave;         // [count][4 NB]
+    WnEigdState *state;     // [count]
+};
+
+template <int N>
+static WnEigdScratch wn_eigd_layout(double2 *scr, int count) {
+    using D = WnEigd<N>;
+    WnEigdScratch s;
+    s.rec = scr;
+    s.hsave = scr + (size_t)count * D::REC2;
+    s.state = (WnEigdState *)(s.hsave + (size_t)count * 4 * D::NB);
+    return s;
+}
+
+// Point of item b of a pass: pass < 0 and pass 2 take b itself; passes 0 and 1 the warm-start groups.
+__device__ __forceinline__ int wn_eig_item_k(int b, int items, int pass, int count, int *anchor) {
+    *anchor = -1;
+    if (b >= items) return -1;
+    if (pass < 0 || pass == 2) return b;
+    return wn_warm_k(b, pass, count, anchor);
+}
+
+template <int N>
+__global__ __launch_bounds__(64, 4) void k_eigd_sweeps(const double *__restrict__ x, double *__restrict__ eout,
+                                                    int *__restrict__ sweeps_out,
+                                                    unsigned long long *__restrict__ sweep_total, int n, int ldo,
+                                                    double tol, int pass, int count, int items, WnEigdScratch scr,
+                                                    int ctr_stride, int *__restrict__ status, int max_sweeps,
+                                                    int rec_sweeps) {
+    using D = WnEigd<N>;
+    using S = WnSlots<N>;
+    constexpr int HALF = D::HALF, NB = D::NB, NOFF = D::NOFF, LPK = D::LPK, KPW = D::KPW, BPL = D::BPL;
+    __shared__ double2 Hs[KPW][4 * NB];
+    __shared__ double4 Rs[KPW][HALF];
+    const int lane = threadIdx.x;
+    const int j = lane / LPK, l = lane - (lane / LPK) * LPK;
+    const int item = blockIdx.x * KPW + j;
+    int anchor;
+    const int k = wn_eig_item_k(item, items, pass, count, &anchor);
+    const bool active = k >= 0 && k < count;
+    // H(k) upper triangle in band order -> slot order of round 0
+    const double2 *xk = active ? (const double2 *)(x + (size_t)k * ldo) : nullptr;
+    for (int e = l; e < N * N; e += LPK) {
+        const int i = e / N, jj = e - (e / N) * N;
+        if (i <= jj) {
+            double2 hv = make_double2(0.0, 0.0);
+            if (active && jj < n) {
+                hv = xk[i * n + jj];
+                if (i == jj) hv.y = 0.0;
+            }
+            const int pi = S::pos0(i), pj = S::pos0(jj);
+            if (pi <= pj)
+                Hs[j][S::at(pi, pj)] = hv;
+            else
+                Hs[j][S::at(pj, pi)] = make_double2(hv.x, -hv.y);
+        }
+    }
+    // this lane's off-diagonal blocks (row-major over t1 < t2), fixed for the whole solve
+    int bt1[BPL], bt2[BPL], bsrc[BPL], bdst[BPL][4];
+#pragma unroll
+    for (int i = 0; i < BPL; ++i) {
+        const int q = l + i * LPK;
+        int t1 = -1, t2 = -1;
+        if (q < NOFF) {
+            int rem = q;
+            t1 = 0;
+            while (rem >= HALF - 1 - t1) {
+                rem -= HALF - 1 - t1;
+                ++t1;
+            }
+            t2 = t1 + 1 + rem;
+        }
+        bt1[i] = t1;
+        bt2[i] = t2;
+        bsrc[i] = t1 >= 0 ? S::blk(t1, t2) : 0;
+#pragma unroll
+        for (int e = 0; e < 4; ++e) bdst[i][e] = t1 >= 0 ? S::dest(2 * t1 + (e >> 1), 2 * t2 + (e & 1)) : 0;
+    }
+    // slot cs = l (lanes l < HALF): its diagonal block and where its rotated entries go
+    const bool plane = l < HALF;
+    const int cs = plane ? l : 0;
+    const int dblk = S::blk(cs, cs);
+    const int dst_a = S::at(S::sigma(2 * cs), S::sigma(2 * cs));
+    const int dst_d = S::at(S::sigma(2 * cs + 1), S::sigma(2 * cs + 1));
+    const int dst_h = S::dest(2 * cs, 2 * cs + 1) & 0x7fffffff;
+    double4 *rec = active ? (double4 *)(scr.rec + (size_t)item * D::REC2) : nullptr;
+
+    int sweep = 0;
+    bool frozen = !active, converged = false, cont = false;
+    double prev_off = 0.0, saved_prev = 0.0;
+    for (;;) {
+        // stopping test (k_eig_rv's): ||offdiag||_F against tol ||H||_F, or the rounding floor
+        double off = 0.0, dia = 0.0;
+        if (!frozen) {
+            for (int b = l; b < NB; b += LPK) {
+                int t1 = 0, rem = b;
+                while (rem >= HALF - t1) {
+                    rem -= HALF - t1;
+                    ++t1;
+                }
+                const double2 e0 = Hs[j][b], e1 = Hs[j][NB + b], e3 = Hs[j][3 * NB + b];
+                const double m0 = e0.x * e0.x + e0.y * e0.y, m1 = e1.x * e1.x + e1.y * e1.y;
+                const double m3 = e3.x * e3.x + e3.y * e3.y;
+                if (rem == 0) {
+                    dia += m0 + m3;
+                    off += 2.0 * m1;
+                } else {
+                    const double2 e2 = Hs[j][2 * NB + b];
+                    off += 2.0 * (m0 + m1 + m3 + e2.x * e2.x + e2.y * e2.y);
+                }
+            }
+        }
+#pragma unroll
+        for (int o = LPK / 2; o > 0; o >>= 1) {
+            off += __shfl_xor(off, o);
+            dia += __shfl_xor(dia, o);
+        }
+        if (!frozen) {
+            if (off <= tol * tol * (dia + off) ||
+                (sweep > 0 && off > 0.25 * prev_off && off <= 1e-20 * (dia + off))) {
+                frozen = true;
+                converged = true;
+            } else if (sweep >= max_sweeps) {
+                frozen = true;   // sweep cap: reported through status word 1
+            } else if (sweep >= rec_sweeps) {
+                frozen = true;   // finished by k_eig_rv_cont from this state
+                cont = true;
+                saved_prev = prev_off;
+            }
+        }
+        prev_off = off;
+        if (__ballot(!frozen) == 0ull) break;
+        double4 *rs = frozen ? nullptr : rec + (size_t)sweep * D::RPS * HALF;
+        for (int r = 0; r < N - 1; ++r) {
+            // every read of the round first
+            double2 B[BPL][4];
+#pragma unroll
+            for (int i = 0; i < BPL; ++i)
+#pragma unroll
+                for (int e = 0; e < 4; ++e) B[i][e] = Hs[j][e * NB + bsrc[i]];
+            double a = 0.0, d = 0.0;
+            double2 h = make_double2(0.0, 0.0);
+            if (plane) {
+                a = Hs[j][dblk].x;
+                h = Hs[j][NB + dblk];
+                d = Hs[j][3 * NB + dblk].x;
+            }
+            const int2 pq = wn_rr_rt(N, cs, r);
+            const WnRotParams o = wn_rot_params(a, d, h, !frozen && plane && pq.x < n && pq.y < n);
+            if (plane && !frozen) {
+                Rs[j][cs] = o.R;
+                rs[r * HALF + cs] = o.R;
+            }
+            wn_wave_order();
+            if (!frozen) {
+#pragma unroll
+                for (int i = 0; i < BPL; ++i) {
+                    if (bt1[i] < 0) continue;
+                    const double4 R1 = Rs[j][bt1[i]], R2 = Rs[j][bt2[i]];
+                    double2 M[2][2] = {{B[i][0], B[i][1]}, {B[i][2], B[i][3]}};
+                    // B J2, then J1^+ (B J2): the block update of wn_block_update
+                    const double2 em2 = make_double2(R2.z, -R2.w);
+#pragma unroll
+                    for (int u = 0; u < 2; ++u) {
+                        const double2 y = cmul(em2, M[u][1]);
+                        const double2 x0 = M[u][0];
+                        M[u][0] = make_double2(R2.x * x0.x - R2.y * y.x, R2.x * x0.y - R2.y * y.y);
+                        M[u][1] = make_double2(R2.y * x0.x + R2.x * y.x, R2.y * x0.y + R2.x * y.y);
+                    }
+                    const double2 ep1 = make_double2(R1.z, R1.w);
+#pragma unroll
+                    for (int v = 0; v < 2; ++v) {
+                        const double2 y = cmul(ep1, M[1][v]);
+                        const double2 x0 = M[0][v];
+                        M[0][v] = make_double2(R1.x * x0.x - R1.y * y.x, R1.x * x0.y - R1.y * y.y);
+                        M[1][v] = make_double2(R1.y * x0.x + R1.x * y.x, R1.y * x0.y + R1.x * y.y);
+                    }
+#pragma unroll
+                    for (int e = 0; e < 4; ++e) {
+                        const int dd = bdst[i][e];
+                        const double2 val = M[e >> 1][e & 1];
+                        Hs[j][dd & 0x7fffffff] =
+                            make_double2(val.x, __longlong_as_double(__double_as_longlong(val.y) ^
+                                                                     ((unsigned long long)(dd < 0) << 63)));
+                    }
+                }
+                if (plane) {
+                    Hs[j][dst_a] = make_double2(o.an, 0.0);
+                    Hs[j][dst_d] = make_double2(o.dn, 0.0);
+                    Hs[j][dst_h] = o.h;
+                }
+            }
+            wn_wave_order();
+        }
+        if (!frozen) ++sweep;
+    }
+    if (!active) return;
+    WnEigdState *stp = scr.state + item;
+    if (cont) {
+        // H at a sweep boundary (every band at pos0), for k_eig_rv_cont
+        double2 *hs = scr.hsave + (size_t)item * 4 * NB;
+        for (int e = l; e < 4 * NB; e += LPK) hs[e] = Hs[j][e];
+        if (l == 0) {
+            stp->sweeps = sweep;
+            stp->nrec = sweep;
+            stp->cont = 1;
+            stp->prev_off = saved_prev;
+        }
+        return;
+    }
+    // eigenvalues ascending (stable by index) and their ranks
+    if (l < n) {
+        auto diag = [&](int b) {
+            const int p = S::pos0(b);
+            return Hs[j][S::at(p, p)].x;
+        };
+        const double et = diag(l);
+        int rk = 0;
+        for (int jj = 0; jj < n; ++jj) {
+            const double ej = diag(jj);
+            rk += (ej < et) || (ej == et && jj < l);
+        }
+        stp->rank[l] = (unsigned char)rk;
+        eout[(size_t)k * n + rk] = et;
+    }
+    if (l == 0) {
+        stp->sweeps = sweep;
+        stp->nrec = sweep;
+        stp->cont = 0;
+        const int bucket = ctr_stride * (int)(item % WN_CTR_BUCKETS);
+        if (sweeps_out) atomicMax(sweeps_out + bucket, sweep);
+        if (sweep_total) atomicAdd(sweep_total + bucket / 2, (unsigned long long)sweep);
+        if (!converged && status) atomicOr(status + 1, 1);
+    }
+}
+
+// Step T of a sweep in the V kernel (T = 0 .. RPS - 1; rounds T < N - 1): pre[T % 4] holds this lane's share
+// of round T's records (loaded four steps earlier); it is staged in LDS, the ring slot is refilled with the
+// records of step T + 4, and the round is applied to the row.  Step N - 1 .. RPS - 1 only keep the pipeline
+// moving.  g is the point's first record step of this sweep; later sweeps continue at g + RPS.
+template <int N, int T>
+struct WnVApply {
+    __device__ __forceinline__ static void run(WnRow<N> &V, double4 (*stage)[N / 2], double2 (&pre)[4],
+                                               const double2 *rp, int g, int glim, int l) {
+        // rp is not __restrict__: the wave-order fences must keep each record load at its step (a restrict
+        // pointer lets the compiler hoist every round's load to the start and spill the row)
+        // branch-free: the ring slot is refilled from a clamped step of a valid record region, and steps past
+        // the point's own records are replaced by the identity when they are staged
+        constexpr int GMAX = WN_EIGD_SWEEPS * WnEigd<N>::RPS;
+        if constexpr (T < N - 1)
+            ((double2 *)stage[T & 1])[l] = g + T < glim ? pre[T % 4] : make_double2(1.0, 0.0);
+        pre[T % 4] = rp[(size_t)min(g + T + 4, GMAX - 1) * N + l];
+        if constexpr (T < N - 1) {
+            wn_wave_order();
+            WnVSlots<N, T, 0, WN_EIGD_VG>::run(V, stage[T & 1]);
+            wn_wave_order();
+        }
+        WnVApply<N, T + 1>::run(V, stage, pre, rp, g, glim, l);
+    }
+};
+template <int N>
+struct WnVApply<N, N> {   // N == WnEigd<N>::RPS
+    __device__ __forceinline__ static void run(WnRow<N> &, double4 (*)[N / 2], double2 (&)[4], const double2 *, int,
+                                               int, int) {}
+};
+
+template <int N>
+__global__ __launch_bounds__(64, WN_EIGD_VWAVES) void k_eigd_vapply(double2 *__restrict__ uout, const double2 *__restrict__ uprev,
+                                                    int n, int pass, int count, int items, WnEigdScratch scr) {
+    using D = WnEigd<N>;
+    constexpr int KPV = D::KPV;
+    __shared__ double4 stage[KPV][2][N / 2];
+    __shared__ int rank[KPV][N];
+    const int lane = threadIdx.x;
+    const int j = lane / N, l = lane - (lane / N) * N;
+    const int item = blockIdx.x * KPV + j;
+    int anchor;
+    const int k = wn_eig_item_k(item, items, pass, count, &anchor);
+    const bool active = k >= 0 && k < count;
+    int nrec = 0, cont = 0;
+    if (active) {
+        nrec = scr.state[item].nrec;
+        cont = scr.state[item].cont;
+        rank[j][l] = scr.state[item].rank[l];
+    }
+    const double2 *ua = !active ? nullptr
+                                : (pass == 2 ? uprev + (size_t)k * n * n
+                                             : (anchor >= 0 ? uout + (size_t)anchor * n * n : nullptr));
+    WnRow<N> V;
+    WnRowIo<N, 0>::load(V, ua, n, l);
+    int nmax = nrec;
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) nmax = max(nmax, __shfl_xor(nmax, o));
+    // record steps of this point: sweep s, round r at step s * RPS + r; steps past its own sweeps (and the
+    // padding rounds) read as identity rotations, which leave the row exactly as it is
+    const double2 *rp = scr.rec + (active ? (size_t)item * D::REC2 : 0);
+    const int glim = nrec * D::RPS;
+    double2 pre[4];
+#pragma unroll
+    for (int q = 0; q < 4; ++q) pre[q] = rp[(size_t)q * N + l];
+    for (int s = 0; s < nmax; ++s) WnVApply<N, 0>::run(V, stage[j], pre, rp, s * D::RPS, glim, l);
+    wn_wave_order();
+    if (!active) return;
+    double2 *uk = uout + (size_t)k * n * n;
+    if (cont) {
+        // raw rows for k_eig_rv_cont
+        if (l < n) WnRowIo<N, 0>::store_raw(V, uk, n, l);
+        return;
+    }
+    WnRowIo<N, 0>::store(V, uk, rank[j], n, l, j * N);
+}
+
+// Points that k_eigd_sweeps left unfinished: the register-V kernel's roles from the saved H, sweep count and
+// stopping-test history, V from the rows the V kernel wrote.  One point per workgroup iteration.
+template <int N>
+__global__ __launch_bounds__(128, 2) void k_eig_rv_cont(WnEigdScratch scr, double *__restrict__ eout,
+                                                        double2 *__restrict__ uout, int *__restrict__ sweeps_out,
+                                                        unsigned long long *__restrict__ sweep_total, int n,
+                                                        double tol, int pass, int count, int items, int ctr_stride,
+                                                        int *__restrict__ status, int max_sweeps) {
+    __shared__ WnEigShared<N> sm;
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int j = lane / N, l = lane - (lane / N) * N;
+    for (int it = blockIdx.x; it < items; it += gridDim.x) {
+        int anchor;
+        const int k = wn_eig_item_k(it, items, pass, count, &anchor);
+        if (k < 0 || k >= count) continue;
+        if (!scr.state[it].cont) continue;   // uniform over the workgroup
+        const WnEigdState st = scr.state[it];
+        const bool act = j == 0;
+        if (wave == 0) {
+            const int bucket = ctr_stride * (int)(it % WN_CTR_BUCKETS);
+            wn_eig_h_role<N>(sm, nullptr, eout, sweeps_out, sweep_total, n, 0, tol, k, act, j, l, lane, bucket,
+                             status, max_sweeps, scr.hsave + (size_t)it * 4 * WnSlots<N>::NB, st.sweeps,
+                             st.prev_off);
+        } else {
+            wn_eig_v_role<N>(sm, uout, act ? uout + (size_t)k * n * n : nullptr, n, k, act, j, l, st.sweeps);
+        }
+    }
+}
+
+size_t wn_eigd_scratch_elems(int n, int count) {
+    if (n < 1 || n > 32) return 0;
+    const size_t per = n <= 8 ? WnEigd<8>::PER_K : n <= 16 ? WnEigd<16>::PER_K : WnEigd<32>::PER_K;
+    return per * (size_t)count;
+}
+
+template <int N>
+static hipError_t launch_eigd(const double *x, double *e, double2 *u, int *sweeps, unsigned long long *sweep_total,
+                              int count, const WnShape &sh, int pass, hipStream_t st, const double2 *uprev,
+                              int ctr_stride, int *status, double2 *scr, int items) {
+    using D = WnEigd<N>;
+    const double tol = 1.0e-15;
+    const WnEigdScratch s = wn_eigd_layout<N>(scr, items);
+    hipLaunchKernelGGL((k_eigd_sweeps<N>), dim3((items + D::KPW - 1) / D::KPW), dim3(64), 0, st, x, e, sweeps,
+                       sweep_total, sh.n, sh.ldo, tol, pass, count, items, s, ctr_stride, status, sh.max_sweeps,
+                       std::min(std::max(sh.rec_sweeps, 1), WN_EIGD_SWEEPS));
+    hipLaunchKernelGGL((k_eigd_vapply<N>), dim3((items + D::KPV - 1) / D::KPV), dim3(64), 0, st, u, uprev, sh.n,
+                       pass, count, items, s);
+    if (sh.max_sweeps > std::min(std::max(sh.rec_sweeps, 1), WN_EIGD_SWEEPS))
+        hipLaunchKernelGGL((k_eig_rv_cont<N>), dim3(std::min(items, 128)), dim3(128), 0, st, s, e, u, sweeps,
+                           sweep_total, sh.n, tol, pass, count, items, ctr_stride, status, sh.max_sweeps);
+    return hipGetLastError();
+}
+
+hipError_t wn_launch_eig_defer(const double *x, double *e, double2 *u, int *sweeps, unsigned long long *sweep_total,
+                               int count, const WnShape &sh, int pass, hipStream_t st, const double2 *uprev,
+                               int ctr_stride, int *status, double2 *scr) {
+    const int n = sh.n;
+    if (!wn_eig_rv_supported(n) || sh.max_sweeps < 1 || !scr) return hipErrorInvalidValue;
+    if (pass == 2 && !uprev) return hipErrorInvalidValue;
+    constexpr int G = 4;
+    const int groups = (count + G - 1) / G;
+    const int items = (pass < 0 || pass == 2) ? count : pass == 0 ? groups : groups * (G - 1);
+    if (items <= 0) return hipSuccess;
+    if (n <= 8) return launch_eigd<8>(x, e, u, sweeps, sweep_total, count, sh, pass, st, uprev, ctr_stride, status, scr,
+                                      items);
+    if (n <= 16)
+        return launch_eigd<16>(x, e, u, sweeps, sweep_total, count, sh, pass, st, uprev, ctr_stride, status, scr,
+                               items);
+    return launch_eigd<32>(x, e, u, sweeps, sweep_total, count, sh, pass, st, uprev, ctr_stride, status, scr, items);
 }
 
 bool wn_eig_rv_supported(int n) { return n >= 1 && n <= 32; }

@@ -107,7 +107,7 @@ struct wnlqg_engine {
     WnGeom gp{};        // BCD / AHC parameters
     WnShg sp{};         // SHG parameters
     DevBuf<double2> der, coef, one;  // SHG optical position derivatives, pair coefficients, one-band sums
-    DevBuf<double2> escratch, rscratch;  // n > 64: Jacobi V and rotation U^+ X in global memory
+    DevBuf<double2> escratch, rscratch;  // Jacobi records (n <= 32) or V (n > 64); rotation U^+ X (n > 64)
     DevBuf<int> pidx;                 // SHG pair index of every (n, m) (-1: inactive)
     DevBuf<double> pdiff;             // SHG pair energy differences
     int device = 0;
@@ -267,10 +267,8 @@ int ensure_rows(wnlqg_engine *g, int rows) {
     WN_HIP(g->e.alloc((size_t)2 * rows * g->sh.n));
     WN_HIP(g->u.alloc((size_t)2 * rows * n2));
     WN_HIP(g->sweeps.alloc((size_t)rows));
-    if (g->sh.n > 64) {
-        WN_HIP(g->escratch.alloc(wn_eig_scratch_elems(g->sh.n, rows)));
-        WN_HIP(g->rscratch.alloc(wn_rotate_scratch_elems(g->sh.n, rows)));
-    }
+    if (wn_eig_scratch_elems(g->sh.n, rows)) WN_HIP(g->escratch.alloc(wn_eig_scratch_elems(g->sh.n, rows)));
+    if (g->sh.n > 64) WN_HIP(g->rscratch.alloc(wn_rotate_scratch_elems(g->sh.n, rows)));
     if (g->family == 3) {
         WN_HIP(g->count.alloc((size_t)rows));
         WN_HIP(g->der.alloc((size_t)rows * WN_SHG_DER_BLOCKS * n2));   // 9 derivative blocks + 12 operand blocks
@@ -612,13 +610,22 @@ int32_t wnlqg_create(const wnlqg_model *model, const wnlqg_task *task, wnlqg_eng
         const int v = std::atoi(ms);
         if (v >= 1 && v <= 40) g->sh.max_sweeps = v;
     }
+    // deferred-eigenvector Jacobi: sweeps recorded per point (1..8) before a point is finished by the register-V
+    // kernel from its saved state; WNLQG_EIGD_RECORD_SWEEPS lowers it (tests drive the continuation with it)
+    g->sh.rec_sweeps = 8;
+    if (const char *rs = std::getenv("WNLQG_EIGD_RECORD_SWEEPS")) {
+        const int v = std::atoi(rs);
+        if (v >= 1 && v <= 8) g->sh.rec_sweeps = v;
+    }
     // eigensolver form: WNLQG_EIG_KERNEL = "lds64" / "lds128" / "lds256" (LDS kernel with that many threads per
-    // point) or "regv" (register-resident eigenvectors, n_w <= 32); default: the size class's default
+    // point), "regv" (register-resident eigenvectors, n_w <= 32) or "defer" (H-only sweeps with recorded rotations
+    // applied to V afterwards, n_w <= 32); default: the size class's default ("defer" for n_w <= 32)
     if (const char *ek = std::getenv("WNLQG_EIG_KERNEL")) {
         if (!strcmp(ek, "lds64")) g->eig_tpb = 64;
         else if (!strcmp(ek, "lds128")) g->eig_tpb = 128;
         else if (!strcmp(ek, "lds256")) g->eig_tpb = 256;
         else if (!strcmp(ek, "regv")) g->eig_tpb = WN_EIG_RV;
+        else if (!strcmp(ek, "defer")) g->eig_tpb = WN_EIG_DV;
     }
     g->sh.S = 1;
     for (int a = 0; a < 3; ++a) {
@@ -1573,9 +1580,13 @@ int autotune(wnlqg_batch *b) {
         if (rc) break;
         float best = 1e30f;
         int best_tpb = 0;
-        // candidates: the register-resident-V kernel (n_w <= 32), then the LDS kernel at 64, 128, 256 threads
+        // candidates: the deferred-eigenvector and register-resident-V kernels (n_w <= 32), then the LDS kernel at
+        // 64, 128, 256 threads
         std::vector<int> cands;
-        if (wn_eig_rv_supported(n)) cands.push_back(WN_EIG_RV);
+        if (wn_eig_rv_supported(n)) {
+            cands.push_back(WN_EIG_DV);
+            cands.push_back(WN_EIG_RV);
+        }
         for (int tpb : {64, 128, 256}) cands.push_back(tpb);
         for (int tpb : cands) {
             float ms = 0.0f;

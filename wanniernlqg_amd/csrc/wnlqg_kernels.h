@@ -16,6 +16,7 @@ struct WnShape {
     int ldo;      // per-k row stride (doubles) of the interpolation output = nNt * 128
     int S;        // span of one phase table axis
     int max_sweeps;  // Jacobi sweep cap; a solve that reaches it sets status word 1
+    int rec_sweeps;  // deferred-eigenvector Jacobi (n <= 32): sweeps recorded before the continuation kernel
     // stored channel s holds logical channel lch[s] of the full map (WN_CH_H .. WN_CH_DA + 8)
     signed char lch[WN_CH_FULL];
 };
@@ -127,11 +128,21 @@ hipError_t wn_launch_eig_pass_f64(const double *x, double *e, double2 *u, int *s
 // kernel, two waves per workgroup (H in LDS, V rows in registers), 64 / N points per workgroup.  It is the
 // default of wn_launch_eig_pass_f64 for n <= 32 (tpb 0 or WN_EIG_RV); tpb 64, 128, 256 select the LDS kernel.
 #define WN_EIG_RV 1
+// Deferred-eigenvector Jacobi (wnlqg_eig.hip) for n <= 32, the default there (tpb 0 or WN_EIG_DV): H-only sweeps
+// at one wave per point (single LDS buffer, rotations recorded), then the recorded rounds applied to register
+// rows of V; points needing more than the recorded sweeps are finished by the register-V kernel's roles.
+// scr: wn_eigd_scratch_elems(n, count) complex elements.
+#define WN_EIG_DV 2
+size_t wn_eigd_scratch_elems(int n, int count);
+hipError_t wn_launch_eig_defer(const double *x, double *e, double2 *u, int *sweeps, unsigned long long *sweep_total,
+                               int count, const WnShape &sh, int pass, hipStream_t st, const double2 *uprev,
+                               int ctr_stride, int *status, double2 *scr);
 bool wn_eig_rv_supported(int n);
 hipError_t wn_launch_eig_rv(const double *x, double *e, double2 *u, int *sweeps, unsigned long long *sweep_total,
                             int count, const WnShape &sh, int pass, hipStream_t st, const double2 *uprev,
                             int ctr_stride, int *status);
-// size in complex elements of the eigensolver / rotation scratch for count points (0 for n <= 64)
+// size in complex elements of the eigensolver / rotation scratch for count points (n <= 32: the deferred-
+// eigenvector records; 32 < n <= 64: 0)
 // 64 < n <= 128 (wnlqg_eigbig.hip): sweep chunks on H with recorded rotations, rotations applied to V by
 // row-owning waves, then sort and phase; scratch: wn_eig_big_scratch_elems(n, count) complex elements.
 size_t wn_eig_big_scratch_elems(int n, int count);

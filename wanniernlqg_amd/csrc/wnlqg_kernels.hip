@@ -1170,7 +1170,10 @@ hipError_t wn_launch_eig_pass_f64(const double *x, double *e, double2 *u, int *s
     if (sh.max_sweeps < 1) return hipErrorInvalidValue;
     if (pass == 2 && (!uprev || !wn_warm_start(sh.n, count))) return hipErrorInvalidValue;
     const int n = sh.n;
-    if (wn_eig_rv_supported(n) && (tpb == 0 || tpb == WN_EIG_RV))
+    if (wn_eig_rv_supported(n) && (tpb == 0 || tpb == WN_EIG_DV) && vscratch)
+        return wn_launch_eig_defer(x, e, u, sweeps, sweep_total, count, sh, pass, st, uprev, ctr_stride, status,
+                                   vscratch);
+    if (wn_eig_rv_supported(n) && (tpb == 0 || tpb == WN_EIG_RV || tpb == WN_EIG_DV))
         return wn_launch_eig_rv(x, e, u, sweeps, sweep_total, count, sh, pass, st, uprev, ctr_stride, status);
     const double tol = 1.0e-15;
     const int groups = (count + WN_WARM_G - 1) / WN_WARM_G;
@@ -1701,7 +1704,9 @@ __global__ __launch_bounds__(256) void k_rotate_big(double *__restrict__ x, cons
     }
 }
 
-size_t wn_eig_scratch_elems(int n, int count) { return wn_eig_big_scratch_elems(n, count); }
+size_t wn_eig_scratch_elems(int n, int count) {
+    return n <= 32 ? wn_eigd_scratch_elems(n, count) : wn_eig_big_scratch_elems(n, count);
+}
 
 size_t wn_rotate_scratch_elems(int n, int count) {
     if (n <= 64) return 0;
