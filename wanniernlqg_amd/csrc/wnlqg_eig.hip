@@ -571,7 +571,6 @@ struct WnEigd {
     static constexpr int NOFF = HALF * (HALF - 1) / 2;
     static constexpr int LPK = N;                      // lanes per point in the sweep kernel
     static constexpr int KPW = 64 / LPK;               // points per wave in the sweep kernel
-    static constexpr int BPL = (NOFF + LPK - 1) / LPK; // off-diagonal blocks per lane
     static constexpr int KPV = 64 / N;                 // points per wave in the V kernel (one lane per row)
     static constexpr int RPS = N;                      // record rounds per sweep (N - 1 used, padded to N)
     // per point: records [sweep][RPS][N/2] double4 (2 double2 each), saved H [4 NB], state (4 double2).
@@ -614,25 +613,66 @@ __device__ __forceinline__ int wn_eig_item_k(int b, int items, int pass, int cou
     return wn_warm_k(b, pass, count, anchor);
 }
 
+// LDS layout of one point's H in the sweep kernel (lane-major).  Each lane owns up to four off-diagonal 2x2
+// blocks: a 2x2 group of the slot grid (rows 2a, 2a+1 x columns 2b, 2b+1, a < b; lanes l < NGF, row-major over
+// the groups) or two of the blocks (2q, 2q + 1) next to the diagonal (lanes NGF + q / 2).  Element e of the
+// lane's block i sits at (4 i + e) LPK + lane, so the lanes of one access touch consecutive 16-byte words; the
+// diagonal blocks' entries (a, h, d) sit in three planes of HALF words after them.  A round moves every element
+// (x, y) to (sigma(x), sigma(y)), which is the neighbouring group's slot for every lane alike, so the writes are
+// consecutive words too (shifted by the group step).
 template <int N>
-__global__ __launch_bounds__(64, 4) void k_eigd_sweeps(const double *__restrict__ x, double *__restrict__ eout,
-                                                    int *__restrict__ sweeps_out,
-                                                    unsigned long long *__restrict__ sweep_total, int n, int ldo,
-                                                    double tol, int pass, int count, int items, WnEigdScratch scr,
-                                                    int ctr_stride, int *__restrict__ status, int max_sweeps,
-                                                    int rec_sweeps) {
+struct WnLane {
+    static constexpr int HALF = N / 2, G = HALF / 2, NGF = G * (G - 1) / 2, LPK = N;
+    static constexpr int DIAG = 16 * LPK;
+    static constexpr int SIZE = DIAG + 3 * HALF;   // double2 words per point
+    static_assert(NGF + (G + 1) / 2 <= LPK, "block groups per point");
+    // storage word of element (x, y), x <= y (slot positions)
+    __host__ __device__ static constexpr int at(int x, int y) {
+        const int t1 = x >> 1, t2 = y >> 1;
+        if (t1 == t2) return DIAG + (x == y ? ((x & 1) ? 2 : 0) : 1) * HALF + t1;
+        const int a = t1 >> 1, b = t2 >> 1;
+        int lane = 0, i = 0;
+        if (a < b) {
+            lane = a * (G - 1) - a * (a - 1) / 2 + (b - a - 1);
+            i = (t1 & 1) * 2 + (t2 & 1);
+        } else {   // a == b: the block (2a, 2a + 1) next to the diagonal, the second of lane NGF + a / 2 if a is odd
+            lane = NGF + (a >> 1);
+            i = (a & 1) ? 3 : 0;
+        }
+        return (i * 4 + (x & 1) * 2 + (y & 1)) * LPK + lane;
+    }
+    // destination of the element now at (x, y), x < y, after the round: word, bit 31 set when stored conjugated
+    __host__ __device__ static constexpr int dest(int x, int y) {
+        const int sx = WnSlots<N>::sigma(x), sy = WnSlots<N>::sigma(y);
+        return sx < sy ? at(sx, sy) : (at(sy, sx) | (int)0x80000000);
+    }
+};
+
+template <int N>
+// two waves per SIMD (at N = 32 the LDS holds no more: two points per wave, 19 KB per wave)
+__global__ __attribute__((amdgpu_flat_work_group_size(64, 64), amdgpu_waves_per_eu(2))) void
+k_eigd_sweeps(const double *__restrict__ x, double *__restrict__ eout, int *__restrict__ sweeps_out,
+              unsigned long long *__restrict__ sweep_total, int n, int ldo, double tol, int pass, int count,
+              int items, WnEigdScratch scr, int ctr_stride, int *__restrict__ status, int max_sweeps,
+              int rec_sweeps) {
     using D = WnEigd<N>;
     using S = WnSlots<N>;
-    constexpr int HALF = D::HALF, NB = D::NB, NOFF = D::NOFF, LPK = D::LPK, KPW = D::KPW, BPL = D::BPL;
-    __shared__ double2 Hs[KPW][4 * NB];
-    __shared__ double4 Rs[KPW][HALF];
+    using L = WnLane<N>;
+    constexpr int HALF = D::HALF, NB = D::NB, LPK = D::LPK, KPW = D::KPW;
+    constexpr int G = L::G, NGF = L::NGF, DG = L::DIAG;
+    __shared__ double2 Hs[KPW][L::SIZE];
+    // the round's parameters as two planes, (c, s) and (cos phi, sin phi): lanes reading consecutive slots
+    // then touch consecutive 16-byte words
+    __shared__ double2 Rcs[KPW][HALF], Rph[KPW][HALF];
     const int lane = threadIdx.x;
     const int j = lane / LPK, l = lane - (lane / LPK) * LPK;
     const int item = blockIdx.x * KPW + j;
     int anchor;
     const int k = wn_eig_item_k(item, items, pass, count, &anchor);
     const bool active = k >= 0 && k < count;
-    // H(k) upper triangle in band order -> slot order of round 0
+    // H(k) upper triangle in band order -> slot positions of round 0 (unused words zero)
+    for (int e = l; e < L::SIZE; e += LPK) Hs[j][e] = make_double2(0.0, 0.0);
+    wn_wave_order();
     const double2 *xk = active ? (const double2 *)(x + (size_t)k * ldo) : nullptr;
     for (int e = l; e < N * N; e += LPK) {
         const int i = e / N, jj = e - (e / N) * N;
@@ -644,68 +684,85 @@ __global__ __launch_bounds__(64, 4) void k_eigd_sweeps(const double *__restrict_
             }
             const int pi = S::pos0(i), pj = S::pos0(jj);
             if (pi <= pj)
-                Hs[j][S::at(pi, pj)] = hv;
+                Hs[j][L::at(pi, pj)] = hv;
             else
-                Hs[j][S::at(pj, pi)] = make_double2(hv.x, -hv.y);
+                Hs[j][L::at(pj, pi)] = make_double2(hv.x, -hv.y);
         }
     }
-    // this lane's off-diagonal blocks (row-major over t1 < t2), fixed for the whole solve
-    int bt1[BPL], bt2[BPL], bsrc[BPL], bdst[BPL][4];
-#pragma unroll
-    for (int i = 0; i < BPL; ++i) {
-        const int q = l + i * LPK;
-        int t1 = -1, t2 = -1;
-        if (q < NOFF) {
-            int rem = q;
-            t1 = 0;
-            while (rem >= HALF - 1 - t1) {
-                rem -= HALF - 1 - t1;
-                ++t1;
-            }
-            t2 = t1 + 1 + rem;
+    // this lane's blocks (WnLane): block i uses row slot rsl[i >> 1] and column slot csl[i & 1]
+    int rsl[2] = {0, 0}, csl[2] = {0, 0};
+    bool bon[4] = {false, false, false, false};
+    if (l < NGF) {
+        int a = 0, rem = l;
+        while (rem >= G - 1 - a) {
+            rem -= G - 1 - a;
+            ++a;
         }
-        bt1[i] = t1;
-        bt2[i] = t2;
-        bsrc[i] = t1 >= 0 ? S::blk(t1, t2) : 0;
-#pragma unroll
-        for (int e = 0; e < 4; ++e) bdst[i][e] = t1 >= 0 ? S::dest(2 * t1 + (e >> 1), 2 * t2 + (e & 1)) : 0;
+        const int b = a + 1 + rem;
+        rsl[0] = 2 * a;
+        rsl[1] = 2 * a + 1;
+        csl[0] = 2 * b;
+        csl[1] = 2 * b + 1;
+        bon[0] = bon[1] = bon[2] = bon[3] = true;
+    } else {
+        const int q0 = 2 * (l - NGF), q1 = q0 + 1;
+        if (q0 < G) {
+            rsl[0] = 2 * q0;
+            csl[0] = 2 * q0 + 1;
+            bon[0] = true;
+        }
+        if (q1 < G) {
+            rsl[1] = 2 * q1;
+            csl[1] = 2 * q1 + 1;
+            bon[3] = true;
+        }
     }
-    // slot cs = l (lanes l < HALF): its diagonal block and where its rotated entries go
+    int bdst[4][4];
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+        const int t1 = rsl[i >> 1], t2 = csl[i & 1];
+#pragma unroll
+        for (int e = 0; e < 4; ++e) bdst[i][e] = bon[i] ? L::dest(2 * t1 + (e >> 1), 2 * t2 + (e & 1)) : 0;
+    }
+    // slot cs = l (lanes l < HALF): its diagonal entries and where its rotated entries go
     const bool plane = l < HALF;
     const int cs = plane ? l : 0;
-    const int dblk = S::blk(cs, cs);
-    const int dst_a = S::at(S::sigma(2 * cs), S::sigma(2 * cs));
-    const int dst_d = S::at(S::sigma(2 * cs + 1), S::sigma(2 * cs + 1));
-    const int dst_h = S::dest(2 * cs, 2 * cs + 1) & 0x7fffffff;
+    const int dst_a = L::at(S::sigma(2 * cs), S::sigma(2 * cs));
+    const int dst_d = L::at(S::sigma(2 * cs + 1), S::sigma(2 * cs + 1));
+    const int dst_h = L::dest(2 * cs, 2 * cs + 1) & 0x7fffffff;   // never conjugated (sigma keeps the order)
     double4 *rec = active ? (double4 *)(scr.rec + (size_t)item * D::REC2) : nullptr;
+    wn_wave_order();
 
     int sweep = 0;
     bool frozen = !active, converged = false, cont = false;
     double prev_off = 0.0, saved_prev = 0.0;
     for (;;) {
-        // stopping test (k_eig_rv's): ||offdiag||_F against tol ||H||_F, or the rounding floor
+        // stopping test of k_eig_rv, with its summation order (blocks b = l, l + N, ... of the packed slot
+        // order, then the lane tree): ||offdiag||_F against tol ||H||_F, or the rounding floor
         double off = 0.0, dia = 0.0;
         if (!frozen) {
-            for (int b = l; b < NB; b += LPK) {
+            for (int b = l; b < NB; b += N) {
                 int t1 = 0, rem = b;
                 while (rem >= HALF - t1) {
                     rem -= HALF - t1;
                     ++t1;
                 }
-                const double2 e0 = Hs[j][b], e1 = Hs[j][NB + b], e3 = Hs[j][3 * NB + b];
+                const int t2 = t1 + rem;
+                const double2 e0 = Hs[j][L::at(2 * t1, 2 * t2)], e1 = Hs[j][L::at(2 * t1, 2 * t2 + 1)];
+                const double2 e3 = Hs[j][L::at(2 * t1 + 1, 2 * t2 + 1)];
                 const double m0 = e0.x * e0.x + e0.y * e0.y, m1 = e1.x * e1.x + e1.y * e1.y;
                 const double m3 = e3.x * e3.x + e3.y * e3.y;
                 if (rem == 0) {
                     dia += m0 + m3;
                     off += 2.0 * m1;
                 } else {
-                    const double2 e2 = Hs[j][2 * NB + b];
+                    const double2 e2 = Hs[j][L::at(2 * t1 + 1, 2 * t2)];
                     off += 2.0 * (m0 + m1 + m3 + e2.x * e2.x + e2.y * e2.y);
                 }
             }
         }
 #pragma unroll
-        for (int o = LPK / 2; o > 0; o >>= 1) {
+        for (int o = N / 2; o > 0; o >>= 1) {
             off += __shfl_xor(off, o);
             dia += __shfl_xor(dia, o);
         }
@@ -727,30 +784,39 @@ __global__ __launch_bounds__(64, 4) void k_eigd_sweeps(const double *__restrict_
         double4 *rs = frozen ? nullptr : rec + (size_t)sweep * D::RPS * HALF;
         for (int r = 0; r < N - 1; ++r) {
             // every read of the round first
-            double2 B[BPL][4];
+            double2 B[4][4];
 #pragma unroll
-            for (int i = 0; i < BPL; ++i)
+            for (int i = 0; i < 4; ++i)
 #pragma unroll
-                for (int e = 0; e < 4; ++e) B[i][e] = Hs[j][e * NB + bsrc[i]];
+                for (int e = 0; e < 4; ++e) B[i][e] = Hs[j][(4 * i + e) * LPK + l];
             double a = 0.0, d = 0.0;
             double2 h = make_double2(0.0, 0.0);
             if (plane) {
-                a = Hs[j][dblk].x;
-                h = Hs[j][NB + dblk];
-                d = Hs[j][3 * NB + dblk].x;
+                a = Hs[j][DG + cs].x;
+                h = Hs[j][DG + HALF + cs];
+                d = Hs[j][DG + 2 * HALF + cs].x;
             }
             const int2 pq = wn_rr_rt(N, cs, r);
             const WnRotParams o = wn_rot_params(a, d, h, !frozen && plane && pq.x < n && pq.y < n);
             if (plane && !frozen) {
-                Rs[j][cs] = o.R;
+                Rcs[j][cs] = make_double2(o.R.x, o.R.y);
+                Rph[j][cs] = make_double2(o.R.z, o.R.w);
                 rs[r * HALF + cs] = o.R;
             }
             wn_wave_order();
             if (!frozen) {
+                double4 Rr[2], Rc[2];
 #pragma unroll
-                for (int i = 0; i < BPL; ++i) {
-                    if (bt1[i] < 0) continue;
-                    const double4 R1 = Rs[j][bt1[i]], R2 = Rs[j][bt2[i]];
+                for (int hh = 0; hh < 2; ++hh) {
+                    const double2 a0 = Rcs[j][rsl[hh]], a1 = Rph[j][rsl[hh]];
+                    const double2 b0 = Rcs[j][csl[hh]], b1 = Rph[j][csl[hh]];
+                    Rr[hh] = make_double4(a0.x, a0.y, a1.x, a1.y);
+                    Rc[hh] = make_double4(b0.x, b0.y, b1.x, b1.y);
+                }
+#pragma unroll
+                for (int i = 0; i < 4; ++i) {
+                    if (!bon[i]) continue;
+                    const double4 R1 = Rr[i >> 1], R2 = Rc[i & 1];
                     double2 M[2][2] = {{B[i][0], B[i][1]}, {B[i][2], B[i][3]}};
                     // B J2, then J1^+ (B J2): the block update of wn_block_update
                     const double2 em2 = make_double2(R2.z, -R2.w);
@@ -791,9 +857,18 @@ __global__ __launch_bounds__(64, 4) void k_eigd_sweeps(const double *__restrict_
     if (!active) return;
     WnEigdState *stp = scr.state + item;
     if (cont) {
-        // H at a sweep boundary (every band at pos0), for k_eig_rv_cont
+        // H at a sweep boundary (every band at pos0) in k_eig_rv's packed slot order, for k_eig_rv_cont
         double2 *hs = scr.hsave + (size_t)item * 4 * NB;
-        for (int e = l; e < 4 * NB; e += LPK) hs[e] = Hs[j][e];
+        for (int e = l; e < 4 * NB; e += LPK) {
+            const int pl = e / NB, b = e - (e / NB) * NB;
+            int t1 = 0, rem = b;
+            while (rem >= HALF - t1) {
+                rem -= HALF - t1;
+                ++t1;
+            }
+            const int xx = 2 * t1 + (pl >> 1), yy = 2 * (t1 + rem) + (pl & 1);
+            hs[e] = xx <= yy ? Hs[j][L::at(xx, yy)] : make_double2(0.0, 0.0);   // (1, 0) of a diagonal block: unused
+        }
         if (l == 0) {
             stp->sweeps = sweep;
             stp->nrec = sweep;
@@ -806,7 +881,7 @@ __global__ __launch_bounds__(64, 4) void k_eigd_sweeps(const double *__restrict_
     if (l < n) {
         auto diag = [&](int b) {
             const int p = S::pos0(b);
-            return Hs[j][S::at(p, p)].x;
+            return Hs[j][L::at(p, p)].x;
         };
         const double et = diag(l);
         int rk = 0;
@@ -913,23 +988,39 @@ __global__ __launch_bounds__(128, 2) void k_eig_rv_cont(WnEigdScratch scr, doubl
                                                         double tol, int pass, int count, int items, int ctr_stride,
                                                         int *__restrict__ status, int max_sweeps) {
     __shared__ WnEigShared<N> sm;
+    __shared__ int list[128];
+    __shared__ int nlist, ncnt0;
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
     const int j = lane / N, l = lane - (lane / N) * N;
-    for (int it = blockIdx.x; it < items; it += gridDim.x) {
-        int anchor;
-        const int k = wn_eig_item_k(it, items, pass, count, &anchor);
-        if (k < 0 || k >= count) continue;
-        if (!scr.state[it].cont) continue;   // uniform over the workgroup
-        const WnEigdState st = scr.state[it];
-        const bool act = j == 0;
-        if (wave == 0) {
-            const int bucket = ctr_stride * (int)(it % WN_CTR_BUCKETS);
-            wn_eig_h_role<N>(sm, nullptr, eout, sweeps_out, sweep_total, n, 0, tol, k, act, j, l, lane, bucket,
-                             status, max_sweeps, scr.hsave + (size_t)it * 4 * WnSlots<N>::NB, st.sweeps,
-                             st.prev_off);
-        } else {
-            wn_eig_v_role<N>(sm, uout, act ? uout + (size_t)k * n * n : nullptr, n, k, act, j, l, st.sweeps);
+    // the continuation flags of 128 items at a time, read by one thread each and compacted in item order
+    for (int base = blockIdx.x * 128; base < items; base += gridDim.x * 128) {
+        const int it0 = base + (int)threadIdx.x;
+        int anchor0;
+        const int k0 = wn_eig_item_k(it0, items, pass, count, &anchor0);
+        const bool want = k0 >= 0 && k0 < count && scr.state[it0].cont;
+        const unsigned long long bal = __ballot(want);
+        if (threadIdx.x == 0) ncnt0 = __popcll(bal);   // thread 0 is in wave 0
+        __syncthreads();
+        if (want) list[(wave == 0 ? 0 : ncnt0) + __popcll(bal & ((1ull << lane) - 1))] = it0;
+        if (threadIdx.x == 64) nlist = ncnt0 + __popcll(bal);   // lane 0 of wave 1
+        __syncthreads();
+        const int nl = nlist;
+        for (int c = 0; c < nl; ++c) {
+            const int it = list[c];
+            int anchor;
+            const int k = wn_eig_item_k(it, items, pass, count, &anchor);
+            const WnEigdState st = scr.state[it];
+            const bool act = j == 0;
+            if (wave == 0) {
+                const int bucket = ctr_stride * (int)(it % WN_CTR_BUCKETS);
+                wn_eig_h_role<N>(sm, nullptr, eout, sweeps_out, sweep_total, n, 0, tol, k, act, j, l, lane, bucket,
+                                 status, max_sweeps, scr.hsave + (size_t)it * 4 * WnSlots<N>::NB, st.sweeps,
+                                 st.prev_off);
+            } else {
+                wn_eig_v_role<N>(sm, uout, act ? uout + (size_t)k * n * n : nullptr, n, k, act, j, l, st.sweeps);
+            }
         }
+        __syncthreads();
     }
 }
 
@@ -952,7 +1043,8 @@ static hipError_t launch_eigd(const double *x, double *e, double2 *u, int *sweep
     hipLaunchKernelGGL((k_eigd_vapply<N>), dim3((items + D::KPV - 1) / D::KPV), dim3(64), 0, st, u, uprev, sh.n,
                        pass, count, items, s);
     if (sh.max_sweeps > std::min(std::max(sh.rec_sweeps, 1), WN_EIGD_SWEEPS))
-        hipLaunchKernelGGL((k_eig_rv_cont<N>), dim3(std::min(items, 128)), dim3(128), 0, st, s, e, u, sweeps,
+        hipLaunchKernelGGL((k_eig_rv_cont<N>), dim3(std::min((items + 127) / 128, 256)), dim3(128), 0, st, s, e, u,
+                           sweeps,
                            sweep_total, sh.n, tol, pass, count, items, ctr_stride, status, sh.max_sweeps);
     return hipGetLastError();
 }

@@ -20,6 +20,7 @@
 #include <cstdlib>
 #include <cstring>
 #include <string>
+#include <map>
 #include <vector>
 
 #define WN_MARKS 10
@@ -449,6 +450,34 @@ int32_t wnlqg_destroy(wnlqg_engine *g) {
     return WNLQG_OK;
 }
 
+// Exact Hermitian symmetry of a model: every R has its -R with the same degeneracy, H(-R) = H(R)^+ and
+// A_a(-R) = A_a(R)^+ entry for entry (models built that way, such as the synthetic ones, satisfy it bitwise;
+// a model that is Hermitian only to rounding is treated as general).
+static bool model_hermitian(const wnlqg_model *m) {
+    const int n = m->num_orbitals, nr = m->num_r_vectors;
+    std::map<std::array<int, 3>, int> index;
+    for (int r = 0; r < nr; ++r)
+        index[{m->r_vectors[3 * r], m->r_vectors[3 * r + 1], m->r_vectors[3 * r + 2]}] = r;
+    const size_t n2 = (size_t)n * n;
+    for (int r = 0; r < nr; ++r) {
+        auto it = index.find({-m->r_vectors[3 * r], -m->r_vectors[3 * r + 1], -m->r_vectors[3 * r + 2]});
+        if (it == index.end()) return false;
+        const int q = it->second;
+        if (m->r_degeneracies[q] != m->r_degeneracies[r]) return false;
+        // column-major [n, n, ...]: entry (i, j) at i + j n
+        for (int blk = 0; blk < (m->position_r ? 4 : 1); ++blk) {
+            const double *a = blk == 0 ? m->hamiltonian_r + 2 * n2 * r : m->position_r + 2 * n2 * (3 * (size_t)r + blk - 1);
+            const double *b = blk == 0 ? m->hamiltonian_r + 2 * n2 * q : m->position_r + 2 * n2 * (3 * (size_t)q + blk - 1);
+            for (int j = 0; j < n; ++j)
+                for (int i = 0; i < n; ++i) {
+                    const size_t ij = 2 * (i + (size_t)j * n), ji = 2 * (j + (size_t)i * n);
+                    if (a[ij] != b[ji] || a[ij + 1] != -b[ji + 1]) return false;
+                }
+        }
+    }
+    return true;
+}
+
 int32_t wnlqg_create(const wnlqg_model *model, const wnlqg_task *task, wnlqg_engine **out) {
     if (!out) return fail(WNLQG_E_INVALID, "out is NULL");
     *out = nullptr;
@@ -617,6 +646,10 @@ int32_t wnlqg_create(const wnlqg_model *model, const wnlqg_task *task, wnlqg_eng
         const int v = std::atoi(rs);
         if (v >= 1 && v <= 8) g->sh.rec_sweeps = v;
     }
+    // rows-only element rotation for exactly Hermitian models (WNLQG_ROTATE_HERMITIAN=0 turns it off)
+    g->sh.herm = model_hermitian(model) ? 1 : 0;
+    if (const char *hv = std::getenv("WNLQG_ROTATE_HERMITIAN"))
+        if (!strcmp(hv, "0")) g->sh.herm = 0;
     // eigensolver form: WNLQG_EIG_KERNEL = "lds64" / "lds128" / "lds256" (LDS kernel with that many threads per
     // point), "regv" (register-resident eigenvectors, n_w <= 32) or "defer" (H-only sweeps with recorded rotations
     // applied to V afterwards, n_w <= 32); default: the size class's default ("defer" for n_w <= 32)

@@ -17,6 +17,8 @@ struct WnShape {
     int S;        // span of one phase table axis
     int max_sweeps;  // Jacobi sweep cap; a solve that reaches it sets status word 1
     int rec_sweeps;  // deferred-eigenvector Jacobi (n <= 32): sweeps recorded before the continuation kernel
+    int herm;        // the model is exactly Hermitian (H(-R) = H(R)^+, A(-R) = A(R)^+, equal degeneracies), so
+                     // every interpolated channel is Hermitian and the SC/IC element rotation may form rows only
     // stored channel s holds logical channel lch[s] of the full map (WN_CH_H .. WN_CH_DA + 8)
     signed char lch[WN_CH_FULL];
 };

@@ -1436,11 +1436,14 @@ __global__ __launch_bounds__(256) void k_rotate_mfma(double *__restrict__ x, con
 // C32: channels 1 .. C-1 arrive in FP32 (the first half of each channel's FP64 slot, n^2 float2 values; the
 // mixed form's mesh channels); the products are FP64 and every channel written back is FP64.
 template <int NP, int PF, bool C32>
-__global__ __launch_bounds__(256) void k_rotate_sc(double *__restrict__ x, const double2 *__restrict__ u,
+// four workgroups per CU at np = 32 (34 KB of LDS each): registers for four waves per SIMD
+__global__ __attribute__((amdgpu_flat_work_group_size(256, 256), amdgpu_waves_per_eu(NP == 32 ? 4 : 1))) void
+k_rotate_sc(double *__restrict__ x, const double2 *__restrict__ u,
                                                   const int *__restrict__ count, const int4 *__restrict__ pmeta,
                                                   double2 *__restrict__ pwd, int n, int C, int ldo,
-                                                  int *__restrict__ flags) {
+                                                  int *__restrict__ flags, int herm) {
     extern __shared__ double smd[];
+    __shared__ unsigned long long bmask;   // the pairs' bands as a bit set
     __shared__ int bl[16];            // bands of the pairs
     __shared__ int prow[WN_PCAP][2];  // row of n and of m in bl, per pair
     __shared__ int pband[WN_PCAP][2]; // n and m
@@ -1473,6 +1476,9 @@ __global__ __launch_bounds__(256) void k_rotate_sc(double *__restrict__ x, const
             }
         }
         nbl = nb_;
+        unsigned long long mk = 0ull;
+        for (int r = 0; r < nb_; ++r) mk |= 1ull << bl[r];
+        bmask = mk;
     }
     if (t == 0) {
         int *bk = flags + WN_CTR_BASE + WN_CTR_STRIDE * (k % WN_CTR_BUCKETS);
@@ -1522,9 +1528,13 @@ __global__ __launch_bounds__(256) void k_rotate_sc(double *__restrict__ x, const
         }
     };
     prefetch(1);
+    // K splits of the 16-row products (np = 32: two column tiles on four waves)
+    constexpr int KS = np == 32 ? 2 : 1;
     for (int c = 1; c < C; ++c) {
         double2 *xc = xk + (size_t)c * n * n;
         const bool full = !sparse || (c >= WN_CH_DH && c < WN_CH_DDH) || (c >= WN_CH_A && c < WN_CH_DA);
+        // Hermitian model, few pairs: the velocity and position channels as rows at the pairs' bands
+        const bool rows = full && sparse && herm;
         __syncthreads();
 #pragma unroll
         for (int q = 0; q < PF; ++q) {
@@ -1538,7 +1548,7 @@ __global__ __launch_bounds__(256) void k_rotate_sc(double *__restrict__ x, const
         // the next channel's loads are in flight during this channel's products
         if (c + 1 < C) prefetch(c + 1);
         __syncthreads();
-        if (full) {
+        if (full && !rows) {
             // T = U^+ X in registers, then to LDS
             f64x4 tr[4], ti[4];
 #pragma unroll
@@ -1597,16 +1607,20 @@ __global__ __launch_bounds__(256) void k_rotate_sc(double *__restrict__ x, const
                 }
             }
         } else {
-            // T_B = U_B^+ X: one 16-row tile (row r <-> band bl[r], zero rows beyond nbl), nt column tiles
+            // T_B = U_B^+ X: one 16-row tile (row r <-> band bl[r], zero rows beyond nbl) by nt column tiles; at
+            // np = 32 the two column tiles' K range is split over the wave pairs (every wave busy), the partial
+            // tiles landing in rows 16 kp .. 16 kp + 15 of the X area and summed where they are read
             const int nb_ = nbl;
+            const int kp = wave / nt;
+            const bool tw = wave < nt * KS;
             f64x4 tr = {0.0, 0.0, 0.0, 0.0}, ti = {0.0, 0.0, 0.0, 0.0};
-            if (wave < nt) {
-                const int c0 = wave * 16;
+            if (tw) {
+                const int c0 = (wave % nt) * 16;
                 const int br = lr < nb_ ? bl[lr] : 0;
                 const double rm = lr < nb_ ? 1.0 : 0.0;
                 f64x4 p1 = {0.0, 0.0, 0.0, 0.0}, p2 = {0.0, 0.0, 0.0, 0.0}, p3 = {0.0, 0.0, 0.0, 0.0};
 #pragma unroll
-                for (int k0 = 0; k0 < np; k0 += 4) {
+                for (int k0 = kp * (np / KS); k0 < (kp + 1) * (np / KS); k0 += 4) {
                     const int kr = k0 + lq;
                     const double ur = rm * Ur[kr * ld + br], ui = rm * Ui[kr * ld + br];
                     const double xr = Xr[kr * ld + c0 + lr], xi = Xi[kr * ld + c0 + lr];
@@ -1621,15 +1635,73 @@ __global__ __launch_bounds__(256) void k_rotate_sc(double *__restrict__ x, const
                 }
             }
             __syncthreads();
-            if (wave < nt) {
-                const int c0 = wave * 16;
+            if (tw) {
+                const int c0 = (wave % nt) * 16;
 #pragma unroll
                 for (int q = 0; q < 4; ++q) {
-                    Xr[(lq + 4 * q) * ld + c0 + lr] = tr[q];
-                    Xi[(lq + 4 * q) * ld + c0 + lr] = ti[q];
+                    Xr[(16 * kp + lq + 4 * q) * ld + c0 + lr] = tr[q];
+                    Xi[(16 * kp + lq + 4 * q) * ld + c0 + lr] = ti[q];
                 }
             }
             __syncthreads();
+            if (rows) {
+                // rows of the Hermitian U^+ X U at the bands B: R_B = T_B U (K split as above); row bl[r] is
+                // written, and its conjugate as column bl[r] outside the B x B block (one writer per element)
+                f64x4 yr = {0.0, 0.0, 0.0, 0.0}, yi = {0.0, 0.0, 0.0, 0.0};
+                if (tw) {
+                    const int c0 = (wave % nt) * 16;
+                    f64x4 p1 = {0.0, 0.0, 0.0, 0.0}, p2 = {0.0, 0.0, 0.0, 0.0}, p3 = {0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+                    for (int k0 = kp * (np / KS); k0 < (kp + 1) * (np / KS); k0 += 4) {
+                        const int kc = k0 + lq;
+                        double tr_ = Xr[lr * ld + kc], ti_ = Xi[lr * ld + kc];
+                        if constexpr (KS == 2) {
+                            tr_ += Xr[(16 + lr) * ld + kc];
+                            ti_ += Xi[(16 + lr) * ld + kc];
+                        }
+                        const double ur = Ur[kc * ld + c0 + lr], ui = Ui[kc * ld + c0 + lr];
+                        p1 = __builtin_amdgcn_mfma_f64_16x16x4f64(tr_, ur, p1, 0, 0, 0);
+                        p2 = __builtin_amdgcn_mfma_f64_16x16x4f64(ti_, ui, p2, 0, 0, 0);
+                        p3 = __builtin_amdgcn_mfma_f64_16x16x4f64(tr_ + ti_, ur + ui, p3, 0, 0, 0);
+                    }
+#pragma unroll
+                    for (int q = 0; q < 4; ++q) {
+                        yr[q] = p1[q] - p2[q];
+                        yi[q] = p3[q] - (p1[q] + p2[q]);
+                    }
+                }
+                if constexpr (KS == 2) {
+                    __syncthreads();
+                    if (tw && kp == 1) {
+                        const int c0 = (wave % nt) * 16;
+#pragma unroll
+                        for (int q = 0; q < 4; ++q) {
+                            Xr[(lq + 4 * q) * ld + c0 + lr] = yr[q];
+                            Xi[(lq + 4 * q) * ld + c0 + lr] = yi[q];
+                        }
+                    }
+                    __syncthreads();
+                }
+                if (tw && kp == 0) {
+                    const int c0 = (wave % nt) * 16;
+                    const unsigned long long mk = bmask;
+#pragma unroll
+                    for (int q = 0; q < 4; ++q) {
+                        const int r = lq + 4 * q, col = c0 + lr;
+                        double vr = yr[q], vi = yi[q];
+                        if constexpr (KS == 2) {
+                            vr += Xr[r * ld + col];
+                            vi += Xi[r * ld + col];
+                        }
+                        if (r < nb_ && col < n) {
+                            const int b = bl[r];
+                            xc[b * n + col] = make_double2(vr, vi);
+                            if (!((mk >> col) & 1ull)) xc[col * n + b] = make_double2(vr, -vi);
+                        }
+                    }
+                }
+                continue;
+            }
             // matrix elements: 16 lanes per (pair, direction), q strided over the lanes, butterfly sum
             const int item = t >> 4, sub = t & 15;
             const int p = item >> 1, d = item & 1;
@@ -1638,7 +1710,11 @@ __global__ __launch_bounds__(256) void k_rotate_sc(double *__restrict__ x, const
             double sr = 0.0, si = 0.0;
             if (ok) {
                 for (int q = sub; q < np; q += 16) {
-                    const double tr_ = Xr[ra * ld + q], ti_ = Xi[ra * ld + q];
+                    double tr_ = Xr[ra * ld + q], ti_ = Xi[ra * ld + q];
+                    if constexpr (KS == 2) {
+                        tr_ += Xr[(16 + ra) * ld + q];
+                        ti_ += Xi[(16 + ra) * ld + q];
+                    }
                     const double ur = Ur[q * ld + bb], ui = Ui[q * ld + bb];
                     sr += tr_ * ur - ti_ * ui;
                     si += tr_ * ui + ti_ * ur;
@@ -1749,10 +1825,10 @@ hipError_t wn_launch_rotate_sc(double *x, const double2 *u, const int *count, co
     do {                                                                                                            \
         if (ch32)                                                                                                   \
             hipLaunchKernelGGL((k_rotate_sc<NPV, PFV, true>), dim3(nk), dim3(256), lds, st, x, u, count, pmeta, pwd, \
-                               sh.n, sh.C, sh.ldo, flags);                                                          \
+                               sh.n, sh.C, sh.ldo, flags, sh.herm);                                                 \
         else                                                                                                        \
             hipLaunchKernelGGL((k_rotate_sc<NPV, PFV, false>), dim3(nk), dim3(256), lds, st, x, u, count, pmeta,     \
-                               pwd, sh.n, sh.C, sh.ldo, flags);                                                     \
+                               pwd, sh.n, sh.C, sh.ldo, flags, sh.herm);                                            \
     } while (0)
     if (np == 16)
         WN_ROTSC(16, 1);
