@@ -89,13 +89,33 @@ def jacobi_flops_per_k(n: int, sweeps: float) -> float:
     return sweeps * 16.0 * n ** 3
 
 
+def rotate_sc_bytes(st, n: int, precision: str) -> float:
+    """Algorithmic HBM bytes of the SC/IC rotation over a mesh.  Every point with pairs reads its C - 1 = 21
+    channels after H (n^2 complex each: 16 bytes per entry in FP64, 8 in the mixed form's FP32 storage) and
+    its eigenvectors U (16 n^2).  Writes: points rotated in full write 21 channels (16 n^2 each); element
+    points write the pairs' elements of dd H and dA (2 directions x 15 complex per pair) and dH, A (6
+    channels) either in full (16 n^2) or, for Hermitian models, the rows and columns at the pairs' bands
+    B (2 |B| n - |B|^2 entries each)."""
+    pts = st["rotate_full_points"] + st["rotate_element_points"]
+    ch = 8.0 if precision == "mixed" else 16.0
+    rd = pts * (21 * n * n * ch + 16.0 * n * n)
+    el, rows = st["rotate_element_points"], st["rotate_row_points"]
+    mb = st["rotate_element_bands"] / max(1, el)
+    wr = (st["rotate_full_points"] * 21 * 16.0 * n * n + st["active_pairs"] * 2 * 15 * 16.0
+          + (el - rows) * 6 * 16.0 * n * n + rows * 6 * 16.0 * (2 * mb * n - mb * mb))
+    return rd + wr
+
+
 def rotate_sc_flops(st, n: int) -> float:
     """Algorithmic flops of the SC/IC rotation over a mesh (8 flops per complex multiply-add): points rotated
-    in full transform 21 channels (16 n^3 each); element points transform dH and A (6 channels) in full and
-    form U_B^+ X (8 |B| n^2) for the 15 dd H and dA channels, |B| the bands of the point's pairs (the final
-    dot products are not counted); points without pairs are skipped."""
-    return (st["rotate_full_points"] * 21 * 16.0 * n ** 3 + st["rotate_element_points"] * 6 * 16.0 * n ** 3
-            + st["rotate_element_bands"] * 15 * 8.0 * n ** 2)
+    in full transform 21 channels (16 n^3 each); element points form U_B^+ X (8 |B| n^2) for the 15 dd H and
+    dA channels, |B| the bands of the point's pairs, and transform dH and A (6 channels) in full (16 n^3) or,
+    for Hermitian models, as the rows U_B^+ X U (16 |B| n^2); dot products for the pairs' elements are not
+    counted; points without pairs are skipped."""
+    el, rows = st["rotate_element_points"], st["rotate_row_points"]
+    mb = st["rotate_element_bands"] / max(1, el)
+    return (st["rotate_full_points"] * 21 * 16.0 * n ** 3 + el * 15 * 8.0 * mb * n ** 2
+            + (el - rows) * 6 * 16.0 * n ** 3 + rows * 6 * 16.0 * mb * n ** 2)
 
 
 def pairs_flops_per_pair(n: int) -> float:
@@ -125,41 +145,31 @@ def measured_traffic_per_k(precision="fp64"):
 
 
 def roofline(st, n, nk, precision):
-    """Roofline of the kernel with the largest device time: algorithmic work per launch over the
-    average launch duration (HIP events around the launches on their own stream)."""
+    """Roofline of the single kernel with the largest device time among those the run times alone with HIP
+    events on their stream (k_rotate_sc, k_contract, or the interpolation GEMM): algorithmic work per launch
+    over the average launch duration.  The eigensolver stage (two kernels, H sweeps and the V kernel, plus
+    the warm-start transform) is reported beside it in 'jacobi_stage'."""
     nb = max(1, st["batches"])
-    kern = {
-        "eig": st["eig_ms"],
-        "rotate": st["rotate_ms"],
-        "interp": st["interp_ms"],
-        "pairs": st["pairs_ms"],
-    }
+    kern = {"rotate": st["rotate_ms"], "interp": st["interp_ms"]}
     dominant = max(kern, key=kern.get)
     traffic = measured_traffic_per_k(precision)
-    if dominant == "eig":
-        launches = max(1, st["eig_launches"])
-        avg_s = 1e-3 * st["eig_ms"] / launches
-        flops = jacobi_flops_per_k(n, st["mean_sweeps"]) * nk / launches
-        # the default eigensolver: register-resident eigenvectors for n <= 32, the LDS kernel up to 64
-        kname = "k_eig_rv" if n <= 32 else "k_eig_f64"
-        roof = {"kernel": kname + " (Jacobi on the FP64 VALU, all warm-start passes; peak = FP64 vector peak)",
-                "bound": "valu", "achieved": flops / avg_s / 1e12, "peak": FP64_PEAK_TFLOPS,
-                "unit": "TFLOP/s", "mean_sweeps": st["mean_sweeps"]}
-        tkey = kname
-    elif dominant == "rotate":
+    if dominant == "rotate":
         launches = nb
         avg_s = 1e-3 * st["rotate_ms"] / launches
-        flops = rotate_sc_flops(st, n) / launches
-        roof = {"kernel": "k_rotate_sc (U^+ X U of dH and A; dd H and dA reduced to the pairs' elements; FP64 MFMA)",
-                "bound": "mfma", "achieved": flops / avg_s / 1e12, "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s"}
+        byts = rotate_sc_bytes(st, n, precision) / launches
+        roof = {"kernel": "k_rotate_sc (reads the 21 channels after H and U per point with pairs; dH and A as rows "
+                          "at the pairs' bands for Hermitian models, dd H and dA reduced to the pairs' elements; FP64 "
+                          "MFMA products)",
+                "bound": "hbm", "achieved": byts / avg_s / 1e9, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                "mfma_tflops": rotate_sc_flops(st, n) / launches / avg_s / 1e12}
         tkey = "k_rotate_sc"
-    elif dominant == "interp" and st["separable"]:
+    elif st["separable"]:
         launches = 3 * nb
         avg_s = 1e-3 * st["interp_ms"] / launches
         roof = {"kernel": "k_contract (separable interpolation, writes every interpolated matrix)", "bound": "hbm",
                 "achieved": st["interp_bytes"] / launches / avg_s / 1e9, "peak": HBM_PEAK_GBS, "unit": "GB/s"}
         tkey = "k_contract"
-    elif dominant == "interp":
+    else:
         launches = nb
         avg_s = 1e-3 * st["interp_ms"] / launches
         mult = 1.0 if precision == "fp64" else 3.0
@@ -167,20 +177,21 @@ def roofline(st, n, nk, precision):
                 "achieved": mult * st["interp_flops"] / launches / avg_s / 1e12,
                 "peak": FP64_PEAK_TFLOPS if precision == "fp64" else BF16_PEAK_TFLOPS, "unit": "TFLOP/s"}
         tkey = "k_interp"
-    else:
-        launches = nb
-        avg_s = 1e-3 * st["pairs_ms"] / launches
-        flops = pairs_flops_per_pair(n) * st["active_pairs"] / launches
-        roof = {"kernel": "k_pairs_f64 (generalized derivatives and SC/IC pair kernels on the FP64 VALU; peak = "
-                          "FP64 vector peak)", "bound": "valu", "achieved": flops / avg_s / 1e12,
-                "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s", "active_pairs_per_k": st["active_pairs"] / nk}
-        tkey = "k_pairs_f64"
     roof["frac"] = roof["achieved"] / roof["peak"] if roof["achieved"] is not None else None
     roof["traffic"] = traffic[tkey] * nk / launches if tkey in traffic else None
     if roof["traffic"] is not None:
         roof["traffic_source"] = os.path.relpath(TRAFFIC_PROFILE, ROOT)
     roof["avg_launch_ms"] = 1e3 * avg_s
     roof["launches"] = launches
+    # the eigensolver stage: sweeps * 16 n^3 flops per point over its event-timed span (both kernels)
+    jl = max(1, st["eig_launches"])
+    roof["jacobi_stage"] = {
+        "kernels": "k_eigd_sweeps + k_eigd_vapply (deferred-eigenvector Jacobi, FP64 VALU)" if n <= 32 else
+                   "k_eig_f64 (Jacobi, FP64 VALU)",
+        "ms": st["eig_ms"], "launches": jl, "mean_sweeps": st["mean_sweeps"],
+        "tflops": jacobi_flops_per_k(n, st["mean_sweeps"]) * nk / (1e-3 * st["eig_ms"]) / 1e12,
+        "frac_of_fp64_peak": jacobi_flops_per_k(n, st["mean_sweeps"]) * nk / (1e-3 * st["eig_ms"]) / 1e12
+                             / FP64_PEAK_TFLOPS}
     return roof
 
 
@@ -319,6 +330,7 @@ def main():
                                          "total_ms")},
         "stage_note": "stages run in order on one stream; each figure is the event-timed span of its kernels",
         "rotation_points": {"elements": st["rotate_element_points"], "full": st["rotate_full_points"],
+                            "rows": st["rotate_row_points"],
                             "mean_bands": st["rotate_element_bands"] / max(1, st["rotate_element_points"])},
         "active_pairs_per_k": st["active_pairs"] / nk,
     }

@@ -28,7 +28,7 @@
 extern "C" {
 #endif
 
-#define WNLQG_ABI_VERSION 8
+#define WNLQG_ABI_VERSION 9
 
 /* Status codes. */
 #define WNLQG_OK 0
@@ -132,7 +132,7 @@ typedef struct wnlqg_stats {
     double total_ms;        /* whole k-loop: phase, interpolation, eigen, screening, response, k-sum */
     double interp_ms;       /* interpolation GEMM kernel */
     double eig_ms;          /* Jacobi eigensolver kernel */
-    double rotate_ms;       /* U^+ X U kernel */
+    double rotate_ms;       /* U^+ X U kernel (SC / IC: k_rotate_sc alone) */
     double pairs_ms;        /* screening + generalized derivative + response kernels */
     double accum_ms;        /* frequency contraction + deterministic k-sum */
     int64_t kpoints;        /* k points evaluated */
@@ -146,7 +146,8 @@ typedef struct wnlqg_stats {
     double mean_sweeps;     /* Jacobi sweeps per k point, averaged over the mesh */
     double warm_ms;         /* warm-start transform U_a^+ H U_a of the mesh eigensolver */
     int32_t eig_launches;   /* eigensolver kernel launches (2 per batch when warm-started) */
-    int32_t reserved;
+    int32_t rotate_row_points;     /* SC / IC element points whose dH and A were formed as rows at the pairs'
+                                      bands (exactly Hermitian models); a subset of rotate_element_points */
     int64_t rotate_element_points; /* SC / IC points whose dd H and dA were reduced to the pairs' elements */
     int64_t rotate_full_points;    /* SC / IC points with every channel rotated (points without pairs: neither) */
     int64_t rotate_element_bands;  /* sum over the element points of the number of bands in their pairs */

@@ -23,7 +23,7 @@
 #include <map>
 #include <vector>
 
-#define WN_MARKS 10
+#define WN_MARKS 11
 
 namespace {
 
@@ -1043,8 +1043,8 @@ hipError_t launch_contract_batch(wnlqg_engine *g, const MeshBatch &bt, double *x
 // The kernel sequence of one planned mesh integral, ordered on g->stream (forked to g->stream2 and
 // joined back).  Capture-safe: no allocation, no host synchronization, no memset.  marks (optional)
 // holds WN_MARKS pre-created events per batch: 0 start, 1 k points, 2 interpolation, 8 eigensolver
-// pass 0, 9 warm-start transform, 3 eigensolver done (stream 0); 7 start, 4 rotation, 5 response,
-// 6 k sums (stream 1).
+// pass 0, 9 warm-start transform, 3 eigensolver done (stream 0); 7 start, 10 screening (SC / IC), 4 rotation,
+// 5 response, 6 k sums (stream 1).
 int enqueue_mesh(wnlqg_engine *g, std::vector<std::array<hipEvent_t, WN_MARKS>> *marks, double *interp_bytes,
                  bool overlap) {
     const MeshPlan &p = g->plan;
@@ -1122,6 +1122,7 @@ int enqueue_mesh(wnlqg_engine *g, std::vector<std::array<hipEvent_t, WN_MARKS>> 
             // dA to the pairs' elements
             WN_HIP(wn_launch_screen(eb, g->omega.p, g->count.p, g->pmeta.p, g->pmd.p, g->flags.p, cnt, g->sh, g->sc,
                                     s1));
+            mark(bi, 10, s1);
             WN_HIP(wn_launch_rotate_sc(xb, ub, g->count.p, g->pmeta.p, g->pwd.p, cnt, g->sh, s1, g->rscratch.p,
                                        g->flags.p, g->ch32 && p.sep));
         } else {
@@ -1265,7 +1266,13 @@ int32_t wnlqg_run_mesh(wnlqg_engine *g, const int32_t mesh[3], int32_t n_lanes, 
                 hipEventElapsedTime(&ms, mk[2], mk[3]); st.eig_ms += ms;
                 st.eig_launches += 1;
             }
-            hipEventElapsedTime(&ms, mk[7], mk[4]); st.rotate_ms += ms;
+            if (g->family == 0) {
+                // screening counts with the response kernels; the rotation timer holds k_rotate_sc alone
+                hipEventElapsedTime(&ms, mk[7], mk[10]); st.pairs_ms += ms;
+                hipEventElapsedTime(&ms, mk[10], mk[4]); st.rotate_ms += ms;
+            } else {
+                hipEventElapsedTime(&ms, mk[7], mk[4]); st.rotate_ms += ms;
+            }
             hipEventElapsedTime(&ms, mk[4], mk[5]); st.pairs_ms += ms;
             hipEventElapsedTime(&ms, mk[5], mk[6]); st.accum_ms += ms;
         }
@@ -1290,6 +1297,7 @@ int32_t wnlqg_run_mesh(wnlqg_engine *g, const int32_t mesh[3], int32_t n_lanes, 
         st.rotate_element_points += bk[6];
         st.rotate_full_points += bk[7];
         st.rotate_element_bands += bk[8];
+        st.rotate_row_points += bk[9];
     }
     st.mean_sweeps = nk > 0 ? (double)swt / (double)nk : 0.0;
     st.active_pairs = (int64_t)npairs;

@@ -184,7 +184,8 @@ hipError_t wn_launch_zero(double *p, long long n, hipStream_t st);
 // buckets of WN_CTR_STRIDE words (128 bytes apart, so that the per-workgroup atomics of a launch spread
 // over many addresses): bucket word [0] max sweeps, [2..3] total sweeps (64-bit), [4..5] total active
 // SC/IC pairs (64-bit), [6] SC/IC points rotated to pair elements, [7] SC/IC points rotated in full,
-// [8] bands of the element points; workgroup g adds to bucket g % WN_CTR_BUCKETS.
+// [8] bands of the element points, [9] element points whose dH and A were formed as rows (Hermitian model);
+// workgroup g adds to bucket g % WN_CTR_BUCKETS.
 #define WN_CTR_BASE 8
 #define WN_CTR_STRIDE 32
 #define WN_CTR_BUCKETS 64

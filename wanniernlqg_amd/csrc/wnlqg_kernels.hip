@@ -1485,6 +1485,7 @@ k_rotate_sc(double *__restrict__ x, const double2 *__restrict__ u,
         if (sparse) {
             atomicAdd(bk + 6, 1);
             atomicAdd(bk + 8, nbl);
+            if (herm) atomicAdd(bk + 9, 1);
         } else {
             atomicAdd(bk + 7, 1);
         }

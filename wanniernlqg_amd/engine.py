@@ -79,7 +79,7 @@ class Stats(C.Structure):
                 ("kpoints", C.c_int64), ("active_pairs", C.c_int64), ("batches", C.c_int32),
                 ("interp_launches", C.c_int32), ("interp_flops", C.c_double), ("max_sweeps", C.c_int32),
                 ("separable", C.c_int32), ("interp_bytes", C.c_double), ("mean_sweeps", C.c_double),
-                ("warm_ms", C.c_double), ("eig_launches", C.c_int32), ("reserved", C.c_int32),
+                ("warm_ms", C.c_double), ("eig_launches", C.c_int32), ("rotate_row_points", C.c_int32),
                 ("rotate_element_points", C.c_int64), ("rotate_full_points", C.c_int64),
                 ("rotate_element_bands", C.c_int64)]
 
@@ -90,7 +90,7 @@ class Stats(C.Structure):
 _lib = None
 
 
-ABI_VERSION = 8   # WNLQG_ABI_VERSION of include/wnlqg.h
+ABI_VERSION = 9   # WNLQG_ABI_VERSION of include/wnlqg.h
 # fourier_backend names -> WNLQG_FOURIER_*: the reference's "auto", "direct", "mixed" and this build's "planes"
 FOURIER_BACKENDS = {"auto": 0, "direct": 1, "planes": 2, "mixed": 3}
 
