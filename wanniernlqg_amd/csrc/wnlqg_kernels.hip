@@ -17,6 +17,8 @@
 //                 [accumulate_conventional_response!, ConventionalShiftCurrent.jl:774-884]
 //   lane_reduce   chunk partials -> deterministic lanes [consume_point!, IntegralDriver.jl:820-874]
 
+#include <type_traits>
+
 #include "wnlqg_common.h"
 #include "wnlqg_kernels.h"
 
@@ -1436,8 +1438,11 @@ __global__ __launch_bounds__(256) void k_rotate_mfma(double *__restrict__ x, con
 // C32: channels 1 .. C-1 arrive in FP32 (the first half of each channel's FP64 slot, n^2 float2 values; the
 // mixed form's mesh channels); the products are FP64 and every channel written back is FP64.
 template <int NP, int PF, bool C32>
-// four workgroups per CU at np = 32 (34 KB of LDS each): registers for four waves per SIMD
-__global__ __attribute__((amdgpu_flat_work_group_size(256, 256), amdgpu_waves_per_eu(NP == 32 ? 4 : 1))) void
+#ifndef WN_ROTSC_WAVES
+#define WN_ROTSC_WAVES 4
+#endif
+// up to four workgroups per CU at np = 32 (34 KB of LDS each): registers for WN_ROTSC_WAVES waves per SIMD
+__global__ __attribute__((amdgpu_flat_work_group_size(256, 256), amdgpu_waves_per_eu(NP == 32 ? WN_ROTSC_WAVES : 1))) void
 k_rotate_sc(double *__restrict__ x, const double2 *__restrict__ u,
                                                   const int *__restrict__ count, const int4 *__restrict__ pmeta,
                                                   double2 *__restrict__ pwd, int n, int C, int ldo,
@@ -1498,6 +1503,164 @@ k_rotate_sc(double *__restrict__ x, const double2 *__restrict__ u,
     }
     double2 *xk = (double2 *)(x + (size_t)k * ldo);
     const int lr = lane & 15, lq = lane >> 4;
+    // K splits of the 16-row products (np = 32: two column tiles on four waves)
+    constexpr int KS = np == 32 ? 2 : 1;
+    if (sparse && herm) {
+        // Hermitian model, at most WN_PCAP pairs: every channel's product is U_B^+ X (16 rows) and, for dH and A,
+        // (U_B^+ X) U.  Wave w owns column tile w % nt and K range w / nt of both products, so its MFMA operand
+        // slices of U_B^+ and U are the same for all 21 channels and stay in registers, and its slice of X
+        // (16 columns x np / KS rows) is loaded from global memory into registers one channel ahead; only the
+        // partial 16-row results go through the LDS.
+        constexpr int KST = np / KS / 4;   // MFMA K steps of a wave
+        const int tile = wave % nt, kp = wave / nt;
+        const bool tw = wave < nt * KS;
+        const int c0 = tile * 16, kb = kp * (np / KS);
+        __syncthreads();   // U staged
+        const int nb_ = nbl;
+        const int br = lr < nb_ ? bl[lr] : 0;
+        const double rm = lr < nb_ ? 1.0 : 0.0;
+        double aur[KST], aui[KST], bur[KST], bui[KST];
+#pragma unroll
+        for (int s = 0; s < KST; ++s) {
+            const int kr = kb + 4 * s + lq;
+            aur[s] = rm * Ur[kr * ld + br];
+            aui[s] = rm * Ui[kr * ld + br];
+            bur[s] = Ur[kr * ld + c0 + lr];
+            bui[s] = Ui[kr * ld + c0 + lr];
+        }
+        const bool colok = tw && c0 + lr < n;
+        // X[kr][c0 + lr] of channel c for this wave's K steps, kept in the channel's storage precision until used
+        using XT = typename std::conditional<C32, float2, double2>::type;
+        auto load_x = [&](int c, XT (&xv)[KST]) {
+            const XT *xc = (const XT *)(xk + (size_t)c * n * n);
+#pragma unroll
+            for (int s = 0; s < KST; ++s) {
+                const int kr = kb + 4 * s + lq;
+                xv[s] = (colok && kr < n) ? xc[kr * n + c0 + lr] : XT{};
+            }
+        };
+        XT xv[KST], nxv[KST];
+        load_x(1, xv);
+        const unsigned long long mk = bmask;
+        for (int c = 1; c < C; ++c) {
+            double2 *xc = xk + (size_t)c * n * n;
+            const bool full = (c >= WN_CH_DH && c < WN_CH_DDH) || (c >= WN_CH_A && c < WN_CH_DA);
+            if (c + 1 < C) load_x(c + 1, nxv);
+            double xr[KST], xi[KST];
+#pragma unroll
+            for (int s = 0; s < KST; ++s) {
+                xr[s] = (double)xv[s].x;
+                xi[s] = (double)xv[s].y;
+            }
+            // partial T_B = U_B^+ X over this wave's K range
+            f64x4 tr = {0.0, 0.0, 0.0, 0.0}, ti = {0.0, 0.0, 0.0, 0.0};
+            if (tw) {
+                f64x4 p1 = {0.0, 0.0, 0.0, 0.0}, p2 = {0.0, 0.0, 0.0, 0.0}, p3 = {0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+                for (int s = 0; s < KST; ++s) {
+                    p1 = __builtin_amdgcn_mfma_f64_16x16x4f64(aur[s], xr[s], p1, 0, 0, 0);
+                    p2 = __builtin_amdgcn_mfma_f64_16x16x4f64(aui[s], xi[s], p2, 0, 0, 0);
+                    p3 = __builtin_amdgcn_mfma_f64_16x16x4f64(aur[s] + aui[s], xi[s] - xr[s], p3, 0, 0, 0);
+                }
+#pragma unroll
+                for (int q = 0; q < 4; ++q) {
+                    tr[q] = p1[q] + p2[q];
+                    ti[q] = p3[q] + (p1[q] - p2[q]);
+                }
+            }
+            __syncthreads();   // the previous channel's readers of the partial rows are done
+            if (tw) {
+#pragma unroll
+                for (int q = 0; q < 4; ++q) {
+                    Xr[(16 * kp + lq + 4 * q) * ld + c0 + lr] = tr[q];
+                    Xi[(16 * kp + lq + 4 * q) * ld + c0 + lr] = ti[q];
+                }
+            }
+            __syncthreads();
+            if (full) {
+                // rows R_B = T_B U over this wave's K range; row bl[r] written, its conjugate as column bl[r]
+                // outside the B x B block (one writer per element)
+                f64x4 yr = {0.0, 0.0, 0.0, 0.0}, yi = {0.0, 0.0, 0.0, 0.0};
+                if (tw) {
+                    f64x4 p1 = {0.0, 0.0, 0.0, 0.0}, p2 = {0.0, 0.0, 0.0, 0.0}, p3 = {0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+                    for (int s = 0; s < KST; ++s) {
+                        const int kc = kb + 4 * s + lq;
+                        double tr_ = Xr[lr * ld + kc], ti_ = Xi[lr * ld + kc];
+                        if constexpr (KS == 2) {
+                            tr_ += Xr[(16 + lr) * ld + kc];
+                            ti_ += Xi[(16 + lr) * ld + kc];
+                        }
+                        p1 = __builtin_amdgcn_mfma_f64_16x16x4f64(tr_, bur[s], p1, 0, 0, 0);
+                        p2 = __builtin_amdgcn_mfma_f64_16x16x4f64(ti_, bui[s], p2, 0, 0, 0);
+                        p3 = __builtin_amdgcn_mfma_f64_16x16x4f64(tr_ + ti_, bur[s] + bui[s], p3, 0, 0, 0);
+                    }
+#pragma unroll
+                    for (int q = 0; q < 4; ++q) {
+                        yr[q] = p1[q] - p2[q];
+                        yi[q] = p3[q] - (p1[q] + p2[q]);
+                    }
+                }
+                if constexpr (KS == 2) {
+                    __syncthreads();
+                    if (tw && kp == 1) {
+#pragma unroll
+                        for (int q = 0; q < 4; ++q) {
+                            Xr[(lq + 4 * q) * ld + c0 + lr] = yr[q];
+                            Xi[(lq + 4 * q) * ld + c0 + lr] = yi[q];
+                        }
+                    }
+                    __syncthreads();
+                }
+                if (tw && kp == 0) {
+#pragma unroll
+                    for (int q = 0; q < 4; ++q) {
+                        const int r = lq + 4 * q, col = c0 + lr;
+                        double vr = yr[q], vi = yi[q];
+                        if constexpr (KS == 2) {
+                            vr += Xr[r * ld + col];
+                            vi += Xi[r * ld + col];
+                        }
+                        if (r < nb_ && col < n) {
+                            const int b = bl[r];
+                            xc[b * n + col] = make_double2(vr, vi);
+                            if (!((mk >> col) & 1ull)) xc[col * n + b] = make_double2(vr, -vi);
+                        }
+                    }
+                }
+            } else {
+                // the pairs' elements sum_q T_B[a][q] U[q][b]: 16 lanes per (pair, direction), butterfly sum
+                const int item = t >> 4, sub = t & 15;
+                const int p = item >> 1, d = item & 1;
+                const bool ok = p < cnt;
+                const int ra = ok ? prow[p][d] : 0, bb = ok ? pband[p][d ^ 1] : 0;
+                double sr = 0.0, si = 0.0;
+                if (ok) {
+                    for (int q = sub; q < np; q += 16) {
+                        double tr_ = Xr[ra * ld + q], ti_ = Xi[ra * ld + q];
+                        if constexpr (KS == 2) {
+                            tr_ += Xr[(16 + ra) * ld + q];
+                            ti_ += Xi[(16 + ra) * ld + q];
+                        }
+                        const double ur = Ur[q * ld + bb], ui = Ui[q * ld + bb];
+                        sr += tr_ * ur - ti_ * ui;
+                        si += tr_ * ui + ti_ * ur;
+                    }
+                }
+                for (int o = 8; o > 0; o >>= 1) {
+                    sr += __shfl_xor(sr, o);
+                    si += __shfl_xor(si, o);
+                }
+                if (ok && sub == 0) {
+                    const int slot = c < WN_CH_A ? c - WN_CH_DDH : 6 + (c - WN_CH_DA);
+                    pwd[(((size_t)k * WN_PCAP + p) * 2 + d) * 15 + slot] = make_double2(sr, si);
+                }
+            }
+#pragma unroll
+            for (int s = 0; s < KST; ++s) xv[s] = nxv[s];
+        }
+        return;
+    }
     // FP32 channels (n even): pairs of consecutive elements per 16-byte load when the tile splits into pairs
     // per thread
     constexpr bool C32P = PF % 2 == 0;
@@ -1529,8 +1692,6 @@ k_rotate_sc(double *__restrict__ x, const double2 *__restrict__ u,
         }
     };
     prefetch(1);
-    // K splits of the 16-row products (np = 32: two column tiles on four waves)
-    constexpr int KS = np == 32 ? 2 : 1;
     for (int c = 1; c < C; ++c) {
         double2 *xc = xk + (size_t)c * n * n;
         const bool full = !sparse || (c >= WN_CH_DH && c < WN_CH_DDH) || (c >= WN_CH_A && c < WN_CH_DA);
