@@ -1244,7 +1244,11 @@ int32_t wnlqg_run_mesh(wnlqg_engine *g, const int32_t mesh[3], int32_t n_lanes, 
                 if (e) hipEventDestroy(e);
     };
     double interp_bytes = 0.0;
-    rc = enqueue_mesh(g, events_ok ? &marks : nullptr, &interp_bytes, false);
+    // WNLQG_MESH_OVERLAP=1: the two-stream order of the batching graph (rotation and response of batch b - 1
+    // beside the interpolation and eigensystem of batch b); the stage timers then span shared GPU time
+    bool overlap = false;
+    if (const char *ov = std::getenv("WNLQG_MESH_OVERLAP")) overlap = !strcmp(ov, "1");
+    rc = enqueue_mesh(g, events_ok ? &marks : nullptr, &interp_bytes, overlap);
     if (!rc) rc = finish_lanes(g, g->plan.n_lanes, g->plan.nk, lane_out, total_out);
     if (rc) {
         release_events();

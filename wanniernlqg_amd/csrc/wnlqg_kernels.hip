@@ -752,43 +752,82 @@ hipError_t wn_launch_axis_phases(double2 *E, int N, int span, int rmin, hipStrea
 // out[b*ob + p*op + m] = sum_{q<Q} E[(p0 + p)*Q + q] * in[b*ib + q*iq + m]   (complex, m < M, p < P)
 // n2c32 > 0: entries m >= n2c32 (channels 1 .. of a k row) are stored in FP32, channel c = m / n2c32 as
 // n2c32 float2 values in the first half of its FP64 slot (float2 index m + c n2c32 of the row).
+// MT columns m per thread (m and m + 256, ...): every phase factor read from the LDS (a broadcast to the
+// wave) serves MT outputs, which the many-output stages (P = mesh points of a line) are bound by.
 #define WN_QMAX 16
 #define WN_PCH 64
+template <int MT, int QM>
 __global__ __launch_bounds__(256) void k_contract(const double2 *__restrict__ in, double2 *__restrict__ out,
                                                  const double2 *__restrict__ E, int P, int p0, int Q, int M,
                                                  long long ib, long long iq, long long ob, long long op, int n2c32) {
     extern __shared__ double2 es[];
     for (int e = threadIdx.x; e < P * Q; e += blockDim.x) es[e] = E[(size_t)p0 * Q + e];
     __syncthreads();
-    const int m = blockIdx.x * blockDim.x + threadIdx.x;
-    if (m >= M) return;
+    const int mb = blockIdx.x * (256 * MT) + threadIdx.x;
+    if (mb >= M) return;
     const long long b = blockIdx.y;
-    const double2 *src = in + b * ib + m;
-    double2 v[WN_QMAX];
+    double2 v[MT][QM];
+    bool on[MT], f32[MT];
+    long long dst[MT];
 #pragma unroll
-    for (int q = 0; q < WN_QMAX; ++q) v[q] = q < Q ? src[q * iq] : make_double2(0.0, 0.0);
-    double2 *dst = out + b * ob + m;
-    const bool f32 = n2c32 > 0 && m >= n2c32;
-    const long long m32 = f32 ? m + (long long)(m / n2c32) * n2c32 : 0;
+    for (int u = 0; u < MT; ++u) {
+        const int m = mb + 256 * u;
+        on[u] = m < M;
+        const double2 *src = in + b * ib + (on[u] ? m : 0);
+#pragma unroll
+        for (int q = 0; q < QM; ++q) v[u][q] = (on[u] && q < Q) ? src[q * iq] : make_double2(0.0, 0.0);
+        f32[u] = n2c32 > 0 && m >= n2c32;
+        // FP64: double2 index m of the row; FP32: float2 index m + (m / n2c32) n2c32
+        dst[u] = f32[u] ? m + (long long)(m / n2c32) * n2c32 : m;
+    }
     for (int p = 0; p < P; ++p) {
         const double2 *ep = es + p * Q;
-        double2 acc = make_double2(0.0, 0.0);
+        double2 acc[MT];
 #pragma unroll
-        for (int q = 0; q < WN_QMAX; ++q)
-            if (q < Q) acc = cfma(ep[q], v[q], acc);
-        if (f32)
-            ((float2 *)(out + b * ob + p * op))[m32] = make_float2((float)acc.x, (float)acc.y);
-        else
-            dst[p * op] = acc;
+        for (int u = 0; u < MT; ++u) acc[u] = make_double2(0.0, 0.0);
+#pragma unroll
+        for (int q = 0; q < QM; ++q)
+            if (q < Q) {
+                const double2 e = ep[q];
+#pragma unroll
+                for (int u = 0; u < MT; ++u) acc[u] = cfma(e, v[u][q], acc[u]);
+            }
+        double2 *row = out + b * ob + p * op;
+#pragma unroll
+        for (int u = 0; u < MT; ++u) {
+            if (!on[u]) continue;
+            if (f32[u])
+                ((float2 *)row)[dst[u]] = make_float2((float)acc[u].x, (float)acc[u].y);
+            else
+                row[dst[u]] = acc[u];
+        }
     }
 }
 
 hipError_t wn_launch_contract(const double2 *in, double2 *out, const double2 *E, int P, int p0, int Q, int M, int nb,
                               long long ib, long long iq, long long ob, long long op, hipStream_t st, int n2c32) {
     if (Q > WN_QMAX) return hipErrorInvalidValue;
-    dim3 grid((M + 255) / 256, nb);
-    hipLaunchKernelGGL(k_contract, grid, dim3(256), (size_t)P * Q * sizeof(double2), st, in, out, E, P, p0, Q, M, ib,
-                       iq, ob, op, n2c32);
+    const size_t lds = (size_t)P * Q * sizeof(double2);
+    // input terms held per column: the span Q rounded up to 8, 12 or 16
+#define WN_CONTRACT(MT, QM)                                                                                      \
+    hipLaunchKernelGGL((k_contract<MT, QM>), dim3((M + 256 * MT - 1) / (256 * MT), nb), dim3(256), lds, st, in, out, \
+                       E, P, p0, Q, M, ib, iq, ob, op, n2c32)
+    if (P >= 8) {
+        if (Q <= 8)
+            WN_CONTRACT(2, 8);
+        else if (Q <= 12)
+            WN_CONTRACT(2, 12);
+        else
+            WN_CONTRACT(2, 16);
+    } else {
+        if (Q <= 8)
+            WN_CONTRACT(1, 8);
+        else if (Q <= 12)
+            WN_CONTRACT(1, 12);
+        else
+            WN_CONTRACT(1, 16);
+    }
+#undef WN_CONTRACT
     return hipGetLastError();
 }
 
@@ -1439,10 +1478,19 @@ __global__ __launch_bounds__(256) void k_rotate_mfma(double *__restrict__ x, con
 // mixed form's mesh channels); the products are FP64 and every channel written back is FP64.
 template <int NP, int PF, bool C32>
 #ifndef WN_ROTSC_WAVES
-#define WN_ROTSC_WAVES 4
+#define WN_ROTSC_WAVES 4     // FP32 channels
 #endif
-// up to four workgroups per CU at np = 32 (34 KB of LDS each): registers for WN_ROTSC_WAVES waves per SIMD
-__global__ __attribute__((amdgpu_flat_work_group_size(256, 256), amdgpu_waves_per_eu(NP == 32 ? WN_ROTSC_WAVES : 1))) void
+#ifndef WN_ROTSC_WAVES64
+#define WN_ROTSC_WAVES64 4   // FP64 channels
+#endif
+#ifndef WN_ROTSC_REG64
+// register-operand loop for FP64 channels too: measured slower there (its double-width prefetch spills at
+// four waves per SIMD, 283 ms per 128^3 mesh, and runs at 215 ms with three) than the staged loop (193 ms)
+#define WN_ROTSC_REG64 0
+#endif
+// up to four workgroups per CU at np = 32 (34 KB of LDS each): registers for that many waves per SIMD
+__global__ __attribute__((amdgpu_flat_work_group_size(256, 256),
+                          amdgpu_waves_per_eu(NP == 32 ? (C32 ? WN_ROTSC_WAVES : WN_ROTSC_WAVES64) : 1))) void
 k_rotate_sc(double *__restrict__ x, const double2 *__restrict__ u,
                                                   const int *__restrict__ count, const int4 *__restrict__ pmeta,
                                                   double2 *__restrict__ pwd, int n, int C, int ldo,
@@ -1505,7 +1553,7 @@ k_rotate_sc(double *__restrict__ x, const double2 *__restrict__ u,
     const int lr = lane & 15, lq = lane >> 4;
     // K splits of the 16-row products (np = 32: two column tiles on four waves)
     constexpr int KS = np == 32 ? 2 : 1;
-    if (sparse && herm) {
+    if (sparse && herm && (C32 || WN_ROTSC_REG64)) {
         // Hermitian model, at most WN_PCAP pairs: every channel's product is U_B^+ X (16 rows) and, for dH and A,
         // (U_B^+ X) U.  Wave w owns column tile w % nt and K range w / nt of both products, so its MFMA operand
         // slices of U_B^+ and U are the same for all 21 channels and stay in registers, and its slice of X
