@@ -804,6 +804,77 @@ __global__ __launch_bounds__(256) void k_contract(const double2 *__restrict__ in
     }
 }
 
+// The same contraction for the stages with many outputs per input column (P >= 16: the points of a mesh line,
+// the lines of a plane), as FP64 MFMA products: per input row b, out(P x M) = E(P x Q) V(Q x M) complex, with
+// three real products (P1 = Er Vr, P2 = Ei Vi, P3 = (Er + Ei)(Vr + Vi); Re = P1 - P2, Im = P3 - P1 - P2) on
+// 16x16x4 tiles.  Wave w of a workgroup owns output rows z 128 + 32 w .. + 31 (two 16-row tiles), whose operand
+// slices of E stay in registers; the workgroup walks MC tiles of 16 columns m, each wave loading its 4 x 16
+// slices of V from global memory one tile ahead.  No LDS, no barrier.
+template <int QK, int MC>
+__global__ __launch_bounds__(256) void k_contract_mfma(const double2 *__restrict__ in, double2 *__restrict__ out,
+                                                      const double2 *__restrict__ E, int P, int p0, int Q, int M,
+                                                      long long ib, long long iq, long long ob, long long op,
+                                                      int n2c32) {
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int lr = lane & 15, lq = lane >> 4;
+    const long long b = blockIdx.y;
+    const int pbase = blockIdx.z * 128 + wave * 32;
+    double er[2][QK], ei[2][QK];
+#pragma unroll
+    for (int t = 0; t < 2; ++t)
+#pragma unroll
+        for (int s = 0; s < QK; ++s) {
+            const int p = pbase + 16 * t + lr, q = 4 * s + lq;
+            const double2 e = (p < P && q < Q) ? E[(size_t)(p0 + p) * Q + q] : make_double2(0.0, 0.0);
+            er[t][s] = e.x;
+            ei[t][s] = e.y;
+        }
+    const double2 *src = in + b * ib;
+    auto loadv = [&](int mt, double2 (&vv)[QK]) {
+        const int m = mt * 16 + lr;
+#pragma unroll
+        for (int s = 0; s < QK; ++s) {
+            const int q = 4 * s + lq;
+            vv[s] = (m < M && q < Q) ? src[q * iq + m] : make_double2(0.0, 0.0);
+        }
+    };
+    const int mt0 = blockIdx.x * MC;
+    double2 v[QK], nv[QK];
+    loadv(mt0, v);
+    for (int i = 0; i < MC; ++i) {
+        const int mt = mt0 + i;
+        if (mt * 16 >= M) break;
+        if (i + 1 < MC) loadv(mt + 1, nv);
+        const int m = mt * 16 + lr;
+        const bool f32 = n2c32 > 0 && m >= n2c32;
+        const long long dst = f32 ? m + (long long)(m / n2c32) * n2c32 : m;
+#pragma unroll
+        for (int t = 0; t < 2; ++t) {
+            if (pbase + 16 * t >= P) break;
+            f64x4 p1 = {0.0, 0.0, 0.0, 0.0}, p2 = {0.0, 0.0, 0.0, 0.0}, p3 = {0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+            for (int s = 0; s < QK; ++s) {
+                p1 = __builtin_amdgcn_mfma_f64_16x16x4f64(er[t][s], v[s].x, p1, 0, 0, 0);
+                p2 = __builtin_amdgcn_mfma_f64_16x16x4f64(ei[t][s], v[s].y, p2, 0, 0, 0);
+                p3 = __builtin_amdgcn_mfma_f64_16x16x4f64(er[t][s] + ei[t][s], v[s].x + v[s].y, p3, 0, 0, 0);
+            }
+#pragma unroll
+            for (int j = 0; j < 4; ++j) {
+                const int p = pbase + 16 * t + lq + 4 * j;
+                if (p >= P || m >= M) continue;
+                const double cr = p1[j] - p2[j], ci = p3[j] - (p1[j] + p2[j]);
+                double2 *row = out + b * ob + (long long)p * op;
+                if (f32)
+                    ((float2 *)row)[dst] = make_float2((float)cr, (float)ci);
+                else
+                    row[dst] = make_double2(cr, ci);
+            }
+        }
+#pragma unroll
+        for (int s = 0; s < QK; ++s) v[s] = nv[s];
+    }
+}
+
 hipError_t wn_launch_contract(const double2 *in, double2 *out, const double2 *E, int P, int p0, int Q, int M, int nb,
                               long long ib, long long iq, long long ob, long long op, hipStream_t st, int n2c32) {
     if (Q > WN_QMAX) return hipErrorInvalidValue;
@@ -812,6 +883,23 @@ hipError_t wn_launch_contract(const double2 *in, double2 *out, const double2 *E,
 #define WN_CONTRACT(MT, QM)                                                                                      \
     hipLaunchKernelGGL((k_contract<MT, QM>), dim3((M + 256 * MT - 1) / (256 * MT), nb), dim3(256), lds, st, in, out, \
                        E, P, p0, Q, M, ib, iq, ob, op, n2c32)
+#ifndef WN_CONTRACT_MFMA
+#define WN_CONTRACT_MFMA 1
+#endif
+    if (WN_CONTRACT_MFMA && P >= 16) {
+        constexpr int MC = 8;
+        const dim3 grid((unsigned)((M + 16 * MC - 1) / (16 * MC)), (unsigned)nb, (unsigned)((P + 127) / 128));
+        if (Q <= 8)
+            hipLaunchKernelGGL((k_contract_mfma<2, MC>), grid, dim3(256), 0, st, in, out, E, P, p0, Q, M, ib, iq, ob,
+                               op, n2c32);
+        else if (Q <= 12)
+            hipLaunchKernelGGL((k_contract_mfma<3, MC>), grid, dim3(256), 0, st, in, out, E, P, p0, Q, M, ib, iq, ob,
+                               op, n2c32);
+        else
+            hipLaunchKernelGGL((k_contract_mfma<4, MC>), grid, dim3(256), 0, st, in, out, E, P, p0, Q, M, ib, iq, ob,
+                               op, n2c32);
+        return hipGetLastError();
+    }
     if (P >= 8) {
         if (Q <= 8)
             WN_CONTRACT(2, 8);
