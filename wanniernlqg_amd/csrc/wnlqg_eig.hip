@@ -26,6 +26,7 @@
 
 #include "wnlqg_common.h"
 #include "wnlqg_kernels.h"
+#include "wnlqg_eigd_lds32.h"
 
 namespace {
 
@@ -563,6 +564,9 @@ __global__ __launch_bounds__(128, 2) void k_eig_rv(const double *__restrict__ x,
 #ifndef WN_EIGD_VWAVES
 #define WN_EIGD_VWAVES 2   // waves per SIMD the V kernel's registers are sized for
 #endif
+#ifndef WN_EIGD_VDEPTH
+#define WN_EIGD_VDEPTH 8   // record steps the V kernel loads ahead (divides the record rounds per sweep)
+#endif
 
 template <int N>
 struct WnEigd {
@@ -618,8 +622,8 @@ __device__ __forceinline__ int wn_eig_item_k(int b, int items, int pass, int cou
 // the groups) or two of the blocks (2q, 2q + 1) next to the diagonal (lanes NGF + q / 2).  Element e of the
 // lane's block i sits at (4 i + e) LPK + lane, so the lanes of one access touch consecutive 16-byte words; the
 // diagonal blocks' entries (a, h, d) sit in three planes of HALF words after them.  A round moves every element
-// (x, y) to (sigma(x), sigma(y)), which is the neighbouring group's slot for every lane alike, so the writes are
-// consecutive words too (shifted by the group step).
+// (x, y) to (sigma(x), sigma(y)), mostly the neighbouring group's slot, so the writes are runs of consecutive
+// words.  At N = 32 the kernel places the elements by WnLaneWord instead (same buffer size).
 template <int N>
 struct WnLane {
     static constexpr int HALF = N / 2, G = HALF / 2, NGF = G * (G - 1) / 2, LPK = N;
@@ -641,57 +645,47 @@ struct WnLane {
         }
         return (i * 4 + (x & 1) * 2 + (y & 1)) * LPK + lane;
     }
-    // destination of the element now at (x, y), x < y, after the round: word, bit 31 set when stored conjugated
-    __host__ __device__ static constexpr int dest(int x, int y) {
-        const int sx = WnSlots<N>::sigma(x), sy = WnSlots<N>::sigma(y);
-        return sx < sy ? at(sx, sy) : (at(sy, sx) | (int)0x80000000);
-    }
 };
 
+// Word of element (x, y), x <= y, in k_eigd_sweeps' buffer: WnLane's layout, or at N = 32 a placement that
+// makes every 16-byte LDS read and write of a round free of bank conflicts on gfx950 (reads are served 16
+// lanes per cycle from 64 banks, writes 8 lanes per cycle from 32 banks; WnLane's writes land on equal banks
+// within a cycle's lanes for about half their words).  The placement is a table (gen_eigd_lds32.py).
+// A lane's blocks 1 and 2 that it does not hold (lanes next to the diagonal) read and write spare words after
+// the elements (SPARE of them), so that every lane runs the same instructions: spare(l, i, e) gives the read
+// word (low 16 bits) and the write word (high 16 bits).
 template <int N>
-// two waves per SIMD (at N = 32 the LDS holds no more: two points per wave, 19 KB per wave)
-__global__ __attribute__((amdgpu_flat_work_group_size(64, 64), amdgpu_waves_per_eu(2))) void
-k_eigd_sweeps(const double *__restrict__ x, double *__restrict__ eout, int *__restrict__ sweeps_out,
-              unsigned long long *__restrict__ sweep_total, int n, int ldo, double tol, int pass, int count,
-              int items, WnEigdScratch scr, int ctr_stride, int *__restrict__ status, int max_sweeps,
-              int rec_sweeps) {
-    using D = WnEigd<N>;
-    using S = WnSlots<N>;
-    using L = WnLane<N>;
-    constexpr int HALF = D::HALF, NB = D::NB, LPK = D::LPK, KPW = D::KPW;
-    constexpr int G = L::G, NGF = L::NGF, DG = L::DIAG;
-    __shared__ double2 Hs[KPW][L::SIZE];
-    // the round's parameters as two planes, (c, s) and (cos phi, sin phi): lanes reading consecutive slots
-    // then touch consecutive 16-byte words
-    __shared__ double2 Rcs[KPW][HALF], Rph[KPW][HALF];
-    const int lane = threadIdx.x;
-    const int j = lane / LPK, l = lane - (lane / LPK) * LPK;
-    const int item = blockIdx.x * KPW + j;
-    int anchor;
-    const int k = wn_eig_item_k(item, items, pass, count, &anchor);
-    const bool active = k >= 0 && k < count;
-    // H(k) upper triangle in band order -> slot positions of round 0 (unused words zero)
-    for (int e = l; e < L::SIZE; e += LPK) Hs[j][e] = make_double2(0.0, 0.0);
-    wn_wave_order();
-    const double2 *xk = active ? (const double2 *)(x + (size_t)k * ldo) : nullptr;
-    for (int e = l; e < N * N; e += LPK) {
-        const int i = e / N, jj = e - (e / N) * N;
-        if (i <= jj) {
-            double2 hv = make_double2(0.0, 0.0);
-            if (active && jj < n) {
-                hv = xk[i * n + jj];
-                if (i == jj) hv.y = 0.0;
-            }
-            const int pi = S::pos0(i), pj = S::pos0(jj);
-            if (pi <= pj)
-                Hs[j][L::at(pi, pj)] = hv;
-            else
-                Hs[j][L::at(pj, pi)] = make_double2(hv.x, -hv.y);
-        }
+struct WnLaneWord {
+    static constexpr int SPARE = 16;
+    static constexpr int WORDS = WnLane<N>::SIZE + SPARE;
+    __device__ static int at(int x, int y) { return WnLane<N>::at(x, y); }
+    __device__ static unsigned spare(int, int i, int e) {
+        const unsigned w = WnLane<N>::SIZE + ((4 * i + e) & (SPARE - 1));
+        return w | (w << 16);
     }
-    // this lane's blocks (WnLane): block i uses row slot rsl[i >> 1] and column slot csl[i & 1]
-    int rsl[2] = {0, 0}, csl[2] = {0, 0};
-    bool bon[4] = {false, false, false, false};
+};
+template <>
+struct WnLaneWord<32> {
+    static constexpr int SPARE = 16;
+    static constexpr int WORDS = WnLane<32>::SIZE + SPARE;
+    __device__ static int at(int x, int y) { return (int)wn_eigd32_word[x * 32 - x * (x - 1) / 2 + (y - x)]; }
+    // spare words of free bank classes in the lane's read and write cycles (lanes 28..31, blocks 1 and 2)
+    __device__ static unsigned spare(int l, int i, int e) { return wn_eigd32_spare[(l - 28) & 3][(i - 1) & 1][e]; }
+};
+// destination of the element now at (x, y), x < y, after the round: word, bit 31 set when stored conjugated
+template <int N>
+__device__ __forceinline__ int wn_lane_dest(int x, int y) {
+    const int sx = WnSlots<N>::sigma(x), sy = WnSlots<N>::sigma(y);
+    return sx < sy ? WnLaneWord<N>::at(sx, sy) : (WnLaneWord<N>::at(sy, sx) | (int)0x80000000);
+}
+
+// 2x2 block groups of lane l of a point in k_eigd_sweeps: block i uses row slot rsl[i >> 1] and column slot
+// csl[i & 1]; lanes l < NGF hold a group of four blocks, the others up to two blocks next to the diagonal
+template <int N>
+__device__ __forceinline__ void wn_lane_blocks(int l, int (&rsl)[2], int (&csl)[2], bool (&bon)[4]) {
+    constexpr int G = WnLane<N>::G, NGF = WnLane<N>::NGF;
+    rsl[0] = rsl[1] = csl[0] = csl[1] = 0;
+    bon[0] = bon[1] = bon[2] = bon[3] = false;
     if (l < NGF) {
         int a = 0, rem = l;
         while (rem >= G - 1 - a) {
@@ -717,20 +711,97 @@ k_eigd_sweeps(const double *__restrict__ x, double *__restrict__ eout, int *__re
             bon[3] = true;
         }
     }
-    int bdst[4][4];
+}
+
+template <int N>
+// two waves per SIMD (at N = 32 the LDS holds no more: two points per wave, 19 KB per wave)
+__global__ __attribute__((amdgpu_flat_work_group_size(64, 64), amdgpu_waves_per_eu(2))) void
+k_eigd_sweeps(const double *__restrict__ x, double *__restrict__ eout, int *__restrict__ sweeps_out,
+              unsigned long long *__restrict__ sweep_total, int n, int ldo, double tol, int pass, int count,
+              int items, WnEigdScratch scr, int ctr_stride, int *__restrict__ status, int max_sweeps,
+              int rec_sweeps) {
+    using D = WnEigd<N>;
+    using S = WnSlots<N>;
+    using L = WnLane<N>;
+    using W = WnLaneWord<N>;
+    constexpr int HALF = D::HALF, NB = D::NB, LPK = D::LPK, KPW = D::KPW;
+    __shared__ double2 Hs[KPW][W::WORDS];
+    // the round's parameters as two planes, (c, s) at slots 0 .. HALF - 1 and (cos phi, sin phi) after them:
+    // lanes reading consecutive slots then touch consecutive 16-byte words
+    __shared__ double2 Rp[KPW][N];
+    const int lane = threadIdx.x;
+    const int j = lane / LPK, l = lane - (lane / LPK) * LPK;
+    const int item = blockIdx.x * KPW + j;
+    int anchor;
+    const int k = wn_eig_item_k(item, items, pass, count, &anchor);
+    const bool active = k >= 0 && k < count;
+    // H(k) upper triangle in band order -> slot positions of round 0 (unused words zero)
+    for (int e = l; e < W::WORDS; e += LPK) Hs[j][e] = make_double2(0.0, 0.0);
+    wn_wave_order();
+    const double2 *xk = active ? (const double2 *)(x + (size_t)k * ldo) : nullptr;
+    for (int e = l; e < N * N; e += LPK) {
+        const int i = e / N, jj = e - (e / N) * N;
+        if (i <= jj) {
+            double2 hv = make_double2(0.0, 0.0);
+            if (active && jj < n) {
+                hv = xk[i * n + jj];
+                if (i == jj) hv.y = 0.0;
+            }
+            const int pi = S::pos0(i), pj = S::pos0(jj);
+            if (pi <= pj)
+                Hs[j][W::at(pi, pj)] = hv;
+            else
+                Hs[j][W::at(pj, pi)] = make_double2(hv.x, -hv.y);
+        }
+    }
+    // this lane's blocks, the words it reads them from and the words their rotated elements go to (spare words
+    // for blocks it does not hold: their results are discarded)
+    int rsl[2], csl[2];
+    bool bon[4];
+    wn_lane_blocks<N>(l, rsl, csl, bon);
+    int brd[4][4], bdst[4][4];
 #pragma unroll
     for (int i = 0; i < 4; ++i) {
         const int t1 = rsl[i >> 1], t2 = csl[i & 1];
 #pragma unroll
-        for (int e = 0; e < 4; ++e) bdst[i][e] = bon[i] ? L::dest(2 * t1 + (e >> 1), 2 * t2 + (e & 1)) : 0;
+        for (int e = 0; e < 4; ++e) {
+            const int x = 2 * t1 + (e >> 1), y = 2 * t2 + (e & 1);
+            const unsigned sp = bon[i] ? 0u : W::spare(l, i, e);
+            brd[i][e] = bon[i] ? W::at(x, y) : (int)(sp & 0xffffu);
+            bdst[i][e] = bon[i] ? wn_lane_dest<N>(x, y) : (int)(sp >> 16);
+        }
     }
-    // slot cs = l (lanes l < HALF): its diagonal entries and where its rotated entries go
+    // slot cs = l mod HALF: lanes l and l + HALF read its diagonal block and compute its rotation alike; lanes
+    // l < HALF store (c, s) and the new a, the others (cos phi, sin phi) and the new d
     const bool plane = l < HALF;
-    const int cs = plane ? l : 0;
-    const int dst_a = L::at(S::sigma(2 * cs), S::sigma(2 * cs));
-    const int dst_d = L::at(S::sigma(2 * cs + 1), S::sigma(2 * cs + 1));
-    const int dst_h = L::dest(2 * cs, 2 * cs + 1) & 0x7fffffff;   // never conjugated (sigma keeps the order)
-    double4 *rec = active ? (double4 *)(scr.rec + (size_t)item * D::REC2) : nullptr;
+    const int cs = l & (HALF - 1);
+    const int rd_a = W::at(2 * cs, 2 * cs), rd_h = W::at(2 * cs, 2 * cs + 1), rd_d = W::at(2 * cs + 1, 2 * cs + 1);
+    const int dst_ad = plane ? W::at(S::sigma(2 * cs), S::sigma(2 * cs)) : W::at(S::sigma(2 * cs + 1), S::sigma(2 * cs + 1));
+    const int dst_h = wn_lane_dest<N>(2 * cs, 2 * cs + 1) & 0x7fffffff;   // never conjugated (sigma keeps the order)
+    const int rec_at = 2 * cs + (plane ? 0 : 1);                           // this lane's half of the slot's record
+    // words of the stopping test's blocks b = l, l + N, ... (k_eig_rv's packed slot order): (2 t1, 2 t2),
+    // (2 t1, 2 t2 + 1), (2 t1 + 1, 2 t2 + 1), (2 t1 + 1, 2 t2); the last is unused on the diagonal (rem == 0)
+    constexpr int NST = (NB + N - 1) / N;
+    int stw[NST][4];
+    bool stdiag[NST];
+#pragma unroll
+    for (int q = 0; q < NST; ++q) {
+        const int b = min(l + q * N, NB - 1);
+        int t1 = 0, rem = b;
+        while (rem >= HALF - t1) {
+            rem -= HALF - t1;
+            ++t1;
+        }
+        const int t2 = t1 + rem;
+        stdiag[q] = rem == 0;
+        stw[q][0] = W::at(2 * t1, 2 * t2);
+        stw[q][1] = W::at(2 * t1, 2 * t2 + 1);
+        stw[q][2] = W::at(2 * t1 + 1, 2 * t2 + 1);
+        stw[q][3] = rem == 0 ? stw[q][0] : W::at(2 * t1 + 1, 2 * t2);
+    }
+    // records of this point; a point past the last item writes (identity rotations, padding round only) into
+    // the records of the wave's first point, which is active whenever any point of the wave is
+    double2 *rec = scr.rec + (size_t)(active ? item : (int)blockIdx.x * KPW) * D::REC2;
     wn_wave_order();
 
     int sweep = 0;
@@ -741,22 +812,18 @@ k_eigd_sweeps(const double *__restrict__ x, double *__restrict__ eout, int *__re
         // order, then the lane tree): ||offdiag||_F against tol ||H||_F, or the rounding floor
         double off = 0.0, dia = 0.0;
         if (!frozen) {
-            for (int b = l; b < NB; b += N) {
-                int t1 = 0, rem = b;
-                while (rem >= HALF - t1) {
-                    rem -= HALF - t1;
-                    ++t1;
-                }
-                const int t2 = t1 + rem;
-                const double2 e0 = Hs[j][L::at(2 * t1, 2 * t2)], e1 = Hs[j][L::at(2 * t1, 2 * t2 + 1)];
-                const double2 e3 = Hs[j][L::at(2 * t1 + 1, 2 * t2 + 1)];
+#pragma unroll
+            for (int q = 0; q < NST; ++q) {
+                if (l + q * N >= NB) break;
+                const double2 e0 = Hs[j][stw[q][0]], e1 = Hs[j][stw[q][1]];
+                const double2 e3 = Hs[j][stw[q][2]];
                 const double m0 = e0.x * e0.x + e0.y * e0.y, m1 = e1.x * e1.x + e1.y * e1.y;
                 const double m3 = e3.x * e3.x + e3.y * e3.y;
-                if (rem == 0) {
+                if (stdiag[q]) {
                     dia += m0 + m3;
                     off += 2.0 * m1;
                 } else {
-                    const double2 e2 = Hs[j][L::at(2 * t1 + 1, 2 * t2)];
+                    const double2 e2 = Hs[j][stw[q][3]];
                     off += 2.0 * (m0 + m1 + m3 + e2.x * e2.x + e2.y * e2.y);
                 }
             }
@@ -781,41 +848,37 @@ k_eigd_sweeps(const double *__restrict__ x, double *__restrict__ eout, int *__re
         }
         prev_off = off;
         if (__ballot(!frozen) == 0ull) break;
-        double4 *rs = frozen ? nullptr : rec + (size_t)sweep * D::RPS * HALF;
+        // a frozen point runs the rounds with identity rotations (its H only moves through the slots and is
+        // back in place at the end of the sweep) and writes its records into the padding round, which the V
+        // kernel never applies
+        double2 *rs = rec + (size_t)min(sweep, WN_EIGD_SWEEPS - 1) * D::RPS * N;
         for (int r = 0; r < N - 1; ++r) {
             // every read of the round first
             double2 B[4][4];
 #pragma unroll
             for (int i = 0; i < 4; ++i)
 #pragma unroll
-                for (int e = 0; e < 4; ++e) B[i][e] = Hs[j][(4 * i + e) * LPK + l];
-            double a = 0.0, d = 0.0;
-            double2 h = make_double2(0.0, 0.0);
-            if (plane) {
-                a = Hs[j][DG + cs].x;
-                h = Hs[j][DG + HALF + cs];
-                d = Hs[j][DG + 2 * HALF + cs].x;
-            }
+                for (int e = 0; e < 4; ++e) B[i][e] = Hs[j][brd[i][e]];
+            const double a = Hs[j][rd_a].x;
+            const double2 h = Hs[j][rd_h];
+            const double d = Hs[j][rd_d].x;
             const int2 pq = wn_rr_rt(N, cs, r);
-            const WnRotParams o = wn_rot_params(a, d, h, !frozen && plane && pq.x < n && pq.y < n);
-            if (plane && !frozen) {
-                Rcs[j][cs] = make_double2(o.R.x, o.R.y);
-                Rph[j][cs] = make_double2(o.R.z, o.R.w);
-                rs[r * HALF + cs] = o.R;
-            }
+            const WnRotParams o = wn_rot_params(a, d, h, !frozen && pq.x < n && pq.y < n);
+            const double2 half = plane ? make_double2(o.R.x, o.R.y) : make_double2(o.R.z, o.R.w);
+            Rp[j][l] = half;
+            rs[(size_t)(frozen ? N - 1 : r) * N + rec_at] = half;
             wn_wave_order();
-            if (!frozen) {
-                double4 Rr[2], Rc[2];
+            {
+                double4 Rr[2], Rc[2];   // rotations of the row slots and the column slots
 #pragma unroll
                 for (int hh = 0; hh < 2; ++hh) {
-                    const double2 a0 = Rcs[j][rsl[hh]], a1 = Rph[j][rsl[hh]];
-                    const double2 b0 = Rcs[j][csl[hh]], b1 = Rph[j][csl[hh]];
+                    const double2 a0 = Rp[j][rsl[hh]], a1 = Rp[j][HALF + rsl[hh]];
+                    const double2 b0 = Rp[j][csl[hh]], b1 = Rp[j][HALF + csl[hh]];
                     Rr[hh] = make_double4(a0.x, a0.y, a1.x, a1.y);
                     Rc[hh] = make_double4(b0.x, b0.y, b1.x, b1.y);
                 }
 #pragma unroll
                 for (int i = 0; i < 4; ++i) {
-                    if (!bon[i]) continue;
                     const double4 R1 = Rr[i >> 1], R2 = Rc[i & 1];
                     double2 M[2][2] = {{B[i][0], B[i][1]}, {B[i][2], B[i][3]}};
                     // B J2, then J1^+ (B J2): the block update of wn_block_update
@@ -844,11 +907,10 @@ k_eigd_sweeps(const double *__restrict__ x, double *__restrict__ eout, int *__re
                                                                      ((unsigned long long)(dd < 0) << 63)));
                     }
                 }
-                if (plane) {
-                    Hs[j][dst_a] = make_double2(o.an, 0.0);
-                    Hs[j][dst_d] = make_double2(o.dn, 0.0);
-                    Hs[j][dst_h] = o.h;
-                }
+                // the slot's new diagonal entries (a from lanes l < HALF, d from the others) and its new (left,
+                // right) entry (lanes l and l + HALF write the same value)
+                Hs[j][dst_ad] = make_double2(plane ? o.an : o.dn, 0.0);
+                Hs[j][dst_h] = o.h;
             }
             wn_wave_order();
         }
@@ -867,7 +929,7 @@ k_eigd_sweeps(const double *__restrict__ x, double *__restrict__ eout, int *__re
                 ++t1;
             }
             const int xx = 2 * t1 + (pl >> 1), yy = 2 * (t1 + rem) + (pl & 1);
-            hs[e] = xx <= yy ? Hs[j][L::at(xx, yy)] : make_double2(0.0, 0.0);   // (1, 0) of a diagonal block: unused
+            hs[e] = xx <= yy ? Hs[j][W::at(xx, yy)] : make_double2(0.0, 0.0);   // (1, 0) of a diagonal block: unused
         }
         if (l == 0) {
             stp->sweeps = sweep;
@@ -881,7 +943,7 @@ k_eigd_sweeps(const double *__restrict__ x, double *__restrict__ eout, int *__re
     if (l < n) {
         auto diag = [&](int b) {
             const int p = S::pos0(b);
-            return Hs[j][L::at(p, p)].x;
+            return Hs[j][W::at(p, p)].x;
         };
         const double et = diag(l);
         int rk = 0;
@@ -909,16 +971,18 @@ k_eigd_sweeps(const double *__restrict__ x, double *__restrict__ eout, int *__re
 // moving.  g is the point's first record step of this sweep; later sweeps continue at g + RPS.
 template <int N, int T>
 struct WnVApply {
-    __device__ __forceinline__ static void run(WnRow<N> &V, double4 (*stage)[N / 2], double2 (&pre)[4],
+    __device__ __forceinline__ static void run(WnRow<N> &V, double4 (*stage)[N / 2], double2 (&pre)[WN_EIGD_VDEPTH],
                                                const double2 *rp, int g, int glim, int l) {
         // rp is not __restrict__: the wave-order fences must keep each record load at its step (a restrict
         // pointer lets the compiler hoist every round's load to the start and spill the row)
         // branch-free: the ring slot is refilled from a clamped step of a valid record region, and steps past
         // the point's own records are replaced by the identity when they are staged
         constexpr int GMAX = WN_EIGD_SWEEPS * WnEigd<N>::RPS;
+        constexpr int DP = WN_EIGD_VDEPTH;
+        static_assert(WnEigd<N>::RPS % DP == 0, "ring slots repeat every sweep");
         if constexpr (T < N - 1)
-            ((double2 *)stage[T & 1])[l] = g + T < glim ? pre[T % 4] : make_double2(1.0, 0.0);
-        pre[T % 4] = rp[(size_t)min(g + T + 4, GMAX - 1) * N + l];
+            ((double2 *)stage[T & 1])[l] = g + T < glim ? pre[T % DP] : make_double2(1.0, 0.0);
+        pre[T % DP] = rp[(size_t)min(g + T + DP, GMAX - 1) * N + l];
         if constexpr (T < N - 1) {
             wn_wave_order();
             WnVSlots<N, T, 0, WN_EIGD_VG>::run(V, stage[T & 1]);
@@ -929,7 +993,7 @@ struct WnVApply {
 };
 template <int N>
 struct WnVApply<N, N> {   // N == WnEigd<N>::RPS
-    __device__ __forceinline__ static void run(WnRow<N> &, double4 (*)[N / 2], double2 (&)[4], const double2 *, int,
+    __device__ __forceinline__ static void run(WnRow<N> &, double4 (*)[N / 2], double2 (&)[WN_EIGD_VDEPTH], const double2 *, int,
                                                int, int) {}
 };
 
@@ -964,9 +1028,9 @@ __global__ __launch_bounds__(64, WN_EIGD_VWAVES) void k_eigd_vapply(double2 *__r
     // padding rounds) read as identity rotations, which leave the row exactly as it is
     const double2 *rp = scr.rec + (active ? (size_t)item * D::REC2 : 0);
     const int glim = nrec * D::RPS;
-    double2 pre[4];
+    double2 pre[WN_EIGD_VDEPTH];
 #pragma unroll
-    for (int q = 0; q < 4; ++q) pre[q] = rp[(size_t)q * N + l];
+    for (int q = 0; q < WN_EIGD_VDEPTH; ++q) pre[q] = rp[(size_t)q * N + l];
     for (int s = 0; s < nmax; ++s) WnVApply<N, 0>::run(V, stage[j], pre, rp, s * D::RPS, glim, l);
     wn_wave_order();
     if (!active) return;
