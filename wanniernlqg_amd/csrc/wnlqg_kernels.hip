@@ -1642,25 +1642,25 @@ k_rotate_sc(double *__restrict__ x, const double2 *__restrict__ u,
     // K splits of the 16-row products (np = 32: two column tiles on four waves)
     constexpr int KS = np == 32 ? 2 : 1;
     if (sparse && herm && (C32 || WN_ROTSC_REG64)) {
-        // Hermitian model, at most WN_PCAP pairs: every channel's product is U_B^+ X (16 rows) and, for dH and A,
-        // (U_B^+ X) U.  Wave w owns column tile w % nt and K range w / nt of both products, so its MFMA operand
-        // slices of U_B^+ and U are the same for all 21 channels and stay in registers, and its slice of X
-        // (16 columns x np / KS rows) is loaded from global memory into registers one channel ahead; only the
-        // partial 16-row results go through the LDS.
+        // Hermitian model, at most WN_PCAP pairs: every channel's product is U_B^+ X (rows at the pairs' bands) and,
+        // for dH and A, (U_B^+ X) U.  Wave w owns column tile w % nt and K range w / nt of both products, so its
+        // MFMA operand slices of U_B^+ and U are the same for all 21 channels and stay in registers, and its slice
+        // of X (16 columns x np / KS rows) is loaded from global memory into registers one channel ahead; only
+        // the partial results go through the LDS.  Up to 8 bands the rows are one or two 4-row tiles on the
+        // four-block 4 x 4 x 4 MFMA (about 20 cycles against 64 for a 16-row tile): block b holds columns 4b ..
+        // 4b + 3 of the wave's 16, lane 16k + 4b + i carries A[i][k], lane 16k + 4b + j carries B[k][j] and
+        // element (i, j) comes back in lane 16i + 4b + j, so lane (lq, lr) holds row 4 t + lq of tile t at column
+        // c0 + lr, and the X slice is the 16 x 16 x 4 form's.  More bands: one 16-row tile (rows lq + 4 q).
         constexpr int KST = np / KS / 4;   // MFMA K steps of a wave
         const int tile = wave % nt, kp = wave / nt;
         const bool tw = wave < nt * KS;
         const int c0 = tile * 16, kb = kp * (np / KS);
         __syncthreads();   // U staged
         const int nb_ = nbl;
-        const int br = lr < nb_ ? bl[lr] : 0;
-        const double rm = lr < nb_ ? 1.0 : 0.0;
-        double aur[KST], aui[KST], bur[KST], bui[KST];
+        double bur[KST], bui[KST];
 #pragma unroll
         for (int s = 0; s < KST; ++s) {
             const int kr = kb + 4 * s + lq;
-            aur[s] = rm * Ur[kr * ld + br];
-            aui[s] = rm * Ui[kr * ld + br];
             bur[s] = Ur[kr * ld + c0 + lr];
             bui[s] = Ui[kr * ld + c0 + lr];
         }
@@ -1675,126 +1675,195 @@ k_rotate_sc(double *__restrict__ x, const double2 *__restrict__ u,
                 xv[s] = (colok && kr < n) ? xc[kr * n + c0 + lr] : XT{};
             }
         };
-        XT xv[KST], nxv[KST];
-        load_x(1, xv);
         const unsigned long long mk = bmask;
-        for (int c = 1; c < C; ++c) {
-            double2 *xc = xk + (size_t)c * n * n;
-            const bool full = (c >= WN_CH_DH && c < WN_CH_DDH) || (c >= WN_CH_A && c < WN_CH_DA);
-            if (c + 1 < C) load_x(c + 1, nxv);
-            double xr[KST], xi[KST];
+        auto channels = [&](auto small_tag) {
+            constexpr bool SM = decltype(small_tag)::value;
+            constexpr int NR = SM ? 2 : 1;   // row tiles
+            constexpr int NQ = SM ? 2 : 4;   // rows of the result per lane
+            // row of the result held in slot q of a lane (band-list index)
+            auto rrow = [&](int q) { return SM ? 4 * q + lq : lq + 4 * q; };
+            // A operand slices of U_B^+: row of the tile = lane & 3 (4-row tiles) or lr (16-row tile)
+            double aur[NR][KST], aui[NR][KST];
 #pragma unroll
-            for (int s = 0; s < KST; ++s) {
-                xr[s] = (double)xv[s].x;
-                xi[s] = (double)xv[s].y;
-            }
-            // partial T_B = U_B^+ X over this wave's K range
-            f64x4 tr = {0.0, 0.0, 0.0, 0.0}, ti = {0.0, 0.0, 0.0, 0.0};
-            if (tw) {
-                f64x4 p1 = {0.0, 0.0, 0.0, 0.0}, p2 = {0.0, 0.0, 0.0, 0.0}, p3 = {0.0, 0.0, 0.0, 0.0};
+            for (int rt = 0; rt < NR; ++rt)
 #pragma unroll
                 for (int s = 0; s < KST; ++s) {
-                    p1 = __builtin_amdgcn_mfma_f64_16x16x4f64(aur[s], xr[s], p1, 0, 0, 0);
-                    p2 = __builtin_amdgcn_mfma_f64_16x16x4f64(aui[s], xi[s], p2, 0, 0, 0);
-                    p3 = __builtin_amdgcn_mfma_f64_16x16x4f64(aur[s] + aui[s], xi[s] - xr[s], p3, 0, 0, 0);
+                    const int kr = kb + 4 * s + lq;
+                    const int row = SM ? 4 * rt + (lane & 3) : lr;
+                    const int br = row < nb_ ? bl[row] : 0;
+                    const double rm = row < nb_ ? 1.0 : 0.0;
+                    aur[rt][s] = rm * Ur[kr * ld + br];
+                    aui[rt][s] = rm * Ui[kr * ld + br];
                 }
+            XT xv[KST], nxv[KST];
+            load_x(1, xv);
+            for (int c = 1; c < C; ++c) {
+                double2 *xc = xk + (size_t)c * n * n;
+                const bool full = (c >= WN_CH_DH && c < WN_CH_DDH) || (c >= WN_CH_A && c < WN_CH_DA);
+                if (c + 1 < C) load_x(c + 1, nxv);
+                double xr[KST], xi[KST];
 #pragma unroll
-                for (int q = 0; q < 4; ++q) {
-                    tr[q] = p1[q] + p2[q];
-                    ti[q] = p3[q] + (p1[q] - p2[q]);
+                for (int s = 0; s < KST; ++s) {
+                    xr[s] = (double)xv[s].x;
+                    xi[s] = (double)xv[s].y;
                 }
-            }
-            __syncthreads();   // the previous channel's readers of the partial rows are done
-            if (tw) {
+                // partial T_B = U_B^+ X over this wave's K range
+                double tr[NQ], ti[NQ];
 #pragma unroll
-                for (int q = 0; q < 4; ++q) {
-                    Xr[(16 * kp + lq + 4 * q) * ld + c0 + lr] = tr[q];
-                    Xi[(16 * kp + lq + 4 * q) * ld + c0 + lr] = ti[q];
-                }
-            }
-            __syncthreads();
-            if (full) {
-                // rows R_B = T_B U over this wave's K range; row bl[r] written, its conjugate as column bl[r]
-                // outside the B x B block (one writer per element)
-                f64x4 yr = {0.0, 0.0, 0.0, 0.0}, yi = {0.0, 0.0, 0.0, 0.0};
+                for (int q = 0; q < NQ; ++q) tr[q] = ti[q] = 0.0;
                 if (tw) {
-                    f64x4 p1 = {0.0, 0.0, 0.0, 0.0}, p2 = {0.0, 0.0, 0.0, 0.0}, p3 = {0.0, 0.0, 0.0, 0.0};
+                    if constexpr (SM) {
 #pragma unroll
-                    for (int s = 0; s < KST; ++s) {
-                        const int kc = kb + 4 * s + lq;
-                        double tr_ = Xr[lr * ld + kc], ti_ = Xi[lr * ld + kc];
-                        if constexpr (KS == 2) {
-                            tr_ += Xr[(16 + lr) * ld + kc];
-                            ti_ += Xi[(16 + lr) * ld + kc];
-                        }
-                        p1 = __builtin_amdgcn_mfma_f64_16x16x4f64(tr_, bur[s], p1, 0, 0, 0);
-                        p2 = __builtin_amdgcn_mfma_f64_16x16x4f64(ti_, bui[s], p2, 0, 0, 0);
-                        p3 = __builtin_amdgcn_mfma_f64_16x16x4f64(tr_ + ti_, bur[s] + bui[s], p3, 0, 0, 0);
-                    }
+                        for (int rt = 0; rt < NR; ++rt) {
+                            if (rt * 4 >= nb_) break;   // a tile of padding rows only (zero)
+                            double p1 = 0.0, p2 = 0.0, p3 = 0.0;
 #pragma unroll
-                    for (int q = 0; q < 4; ++q) {
-                        yr[q] = p1[q] - p2[q];
-                        yi[q] = p3[q] - (p1[q] + p2[q]);
-                    }
-                }
-                if constexpr (KS == 2) {
-                    __syncthreads();
-                    if (tw && kp == 1) {
+                            for (int s = 0; s < KST; ++s) {
+                                p1 = __builtin_amdgcn_mfma_f64_4x4x4f64(aur[rt][s], xr[s], p1, 0, 0, 0);
+                                p2 = __builtin_amdgcn_mfma_f64_4x4x4f64(aui[rt][s], xi[s], p2, 0, 0, 0);
+                                p3 = __builtin_amdgcn_mfma_f64_4x4x4f64(aur[rt][s] + aui[rt][s], xi[s] - xr[s], p3,
+                                                                        0, 0, 0);
+                            }
+                            tr[rt] = p1 + p2;
+                            ti[rt] = p3 + (p1 - p2);
+                        }
+                    } else {
+                        f64x4 p1 = {0.0, 0.0, 0.0, 0.0}, p2 = {0.0, 0.0, 0.0, 0.0}, p3 = {0.0, 0.0, 0.0, 0.0};
 #pragma unroll
-                        for (int q = 0; q < 4; ++q) {
-                            Xr[(lq + 4 * q) * ld + c0 + lr] = yr[q];
-                            Xi[(lq + 4 * q) * ld + c0 + lr] = yi[q];
+                        for (int s = 0; s < KST; ++s) {
+                            p1 = __builtin_amdgcn_mfma_f64_16x16x4f64(aur[0][s], xr[s], p1, 0, 0, 0);
+                            p2 = __builtin_amdgcn_mfma_f64_16x16x4f64(aui[0][s], xi[s], p2, 0, 0, 0);
+                            p3 = __builtin_amdgcn_mfma_f64_16x16x4f64(aur[0][s] + aui[0][s], xi[s] - xr[s], p3, 0, 0,
+                                                                      0);
                         }
-                    }
-                    __syncthreads();
-                }
-                if (tw && kp == 0) {
 #pragma unroll
-                    for (int q = 0; q < 4; ++q) {
-                        const int r = lq + 4 * q, col = c0 + lr;
-                        double vr = yr[q], vi = yi[q];
-                        if constexpr (KS == 2) {
-                            vr += Xr[r * ld + col];
-                            vi += Xi[r * ld + col];
-                        }
-                        if (r < nb_ && col < n) {
-                            const int b = bl[r];
-                            xc[b * n + col] = make_double2(vr, vi);
-                            if (!((mk >> col) & 1ull)) xc[col * n + b] = make_double2(vr, -vi);
+                        for (int q = 0; q < NQ; ++q) {
+                            tr[q] = p1[q] + p2[q];
+                            ti[q] = p3[q] + (p1[q] - p2[q]);
                         }
                     }
                 }
-            } else {
-                // the pairs' elements sum_q T_B[a][q] U[q][b]: 16 lanes per (pair, direction), butterfly sum
-                const int item = t >> 4, sub = t & 15;
-                const int p = item >> 1, d = item & 1;
-                const bool ok = p < cnt;
-                const int ra = ok ? prow[p][d] : 0, bb = ok ? pband[p][d ^ 1] : 0;
-                double sr = 0.0, si = 0.0;
-                if (ok) {
-                    for (int q = sub; q < np; q += 16) {
-                        double tr_ = Xr[ra * ld + q], ti_ = Xi[ra * ld + q];
-                        if constexpr (KS == 2) {
-                            tr_ += Xr[(16 + ra) * ld + q];
-                            ti_ += Xi[(16 + ra) * ld + q];
-                        }
-                        const double ur = Ur[q * ld + bb], ui = Ui[q * ld + bb];
-                        sr += tr_ * ur - ti_ * ui;
-                        si += tr_ * ui + ti_ * ur;
+                __syncthreads();   // the previous channel's readers of the partial rows are done
+                if (tw) {
+#pragma unroll
+                    for (int q = 0; q < NQ; ++q) {
+                        Xr[(16 * kp + rrow(q)) * ld + c0 + lr] = tr[q];
+                        Xi[(16 * kp + rrow(q)) * ld + c0 + lr] = ti[q];
                     }
                 }
-                for (int o = 8; o > 0; o >>= 1) {
-                    sr += __shfl_xor(sr, o);
-                    si += __shfl_xor(si, o);
+                __syncthreads();
+                if (full) {
+                    // rows R_B = T_B U over this wave's K range; row bl[r] written, its conjugate as column bl[r]
+                    // outside the B x B block (one writer per element)
+                    double yr[NQ], yi[NQ];
+#pragma unroll
+                    for (int q = 0; q < NQ; ++q) yr[q] = yi[q] = 0.0;
+                    if (tw) {
+                        if constexpr (SM) {
+#pragma unroll
+                            for (int rt = 0; rt < NR; ++rt) {
+                                if (rt * 4 >= nb_) break;
+                                double p1 = 0.0, p2 = 0.0, p3 = 0.0;
+                                const int row = 4 * rt + (lane & 3);
+#pragma unroll
+                                for (int s = 0; s < KST; ++s) {
+                                    const int kc = kb + 4 * s + lq;
+                                    double tr_ = Xr[row * ld + kc], ti_ = Xi[row * ld + kc];
+                                    if constexpr (KS == 2) {
+                                        tr_ += Xr[(16 + row) * ld + kc];
+                                        ti_ += Xi[(16 + row) * ld + kc];
+                                    }
+                                    p1 = __builtin_amdgcn_mfma_f64_4x4x4f64(tr_, bur[s], p1, 0, 0, 0);
+                                    p2 = __builtin_amdgcn_mfma_f64_4x4x4f64(ti_, bui[s], p2, 0, 0, 0);
+                                    p3 = __builtin_amdgcn_mfma_f64_4x4x4f64(tr_ + ti_, bur[s] + bui[s], p3, 0, 0, 0);
+                                }
+                                yr[rt] = p1 - p2;
+                                yi[rt] = p3 - (p1 + p2);
+                            }
+                        } else {
+                            f64x4 p1 = {0.0, 0.0, 0.0, 0.0}, p2 = {0.0, 0.0, 0.0, 0.0}, p3 = {0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+                            for (int s = 0; s < KST; ++s) {
+                                const int kc = kb + 4 * s + lq;
+                                double tr_ = Xr[lr * ld + kc], ti_ = Xi[lr * ld + kc];
+                                if constexpr (KS == 2) {
+                                    tr_ += Xr[(16 + lr) * ld + kc];
+                                    ti_ += Xi[(16 + lr) * ld + kc];
+                                }
+                                p1 = __builtin_amdgcn_mfma_f64_16x16x4f64(tr_, bur[s], p1, 0, 0, 0);
+                                p2 = __builtin_amdgcn_mfma_f64_16x16x4f64(ti_, bui[s], p2, 0, 0, 0);
+                                p3 = __builtin_amdgcn_mfma_f64_16x16x4f64(tr_ + ti_, bur[s] + bui[s], p3, 0, 0, 0);
+                            }
+#pragma unroll
+                            for (int q = 0; q < NQ; ++q) {
+                                yr[q] = p1[q] - p2[q];
+                                yi[q] = p3[q] - (p1[q] + p2[q]);
+                            }
+                        }
+                    }
+                    if constexpr (KS == 2) {
+                        __syncthreads();
+                        if (tw && kp == 1) {
+#pragma unroll
+                            for (int q = 0; q < NQ; ++q) {
+                                Xr[rrow(q) * ld + c0 + lr] = yr[q];
+                                Xi[rrow(q) * ld + c0 + lr] = yi[q];
+                            }
+                        }
+                        __syncthreads();
+                    }
+                    if (tw && kp == 0) {
+#pragma unroll
+                        for (int q = 0; q < NQ; ++q) {
+                            const int r = rrow(q), col = c0 + lr;
+                            double vr = yr[q], vi = yi[q];
+                            if constexpr (KS == 2) {
+                                vr += Xr[r * ld + col];
+                                vi += Xi[r * ld + col];
+                            }
+                            if (r < nb_ && col < n) {
+                                const int b = bl[r];
+                                xc[b * n + col] = make_double2(vr, vi);
+                                if (!((mk >> col) & 1ull)) xc[col * n + b] = make_double2(vr, -vi);
+                            }
+                        }
+                    }
+                } else {
+                    // the pairs' elements sum_q T_B[a][q] U[q][b]: 16 lanes per (pair, direction), butterfly sum
+                    const int item = t >> 4, sub = t & 15;
+                    const int p = item >> 1, d = item & 1;
+                    const bool ok = p < cnt;
+                    const int ra = ok ? prow[p][d] : 0, bb = ok ? pband[p][d ^ 1] : 0;
+                    double sr = 0.0, si = 0.0;
+                    if (ok) {
+                        for (int q = sub; q < np; q += 16) {
+                            double tr_ = Xr[ra * ld + q], ti_ = Xi[ra * ld + q];
+                            if constexpr (KS == 2) {
+                                tr_ += Xr[(16 + ra) * ld + q];
+                                ti_ += Xi[(16 + ra) * ld + q];
+                            }
+                            const double ur = Ur[q * ld + bb], ui = Ui[q * ld + bb];
+                            sr += tr_ * ur - ti_ * ui;
+                            si += tr_ * ui + ti_ * ur;
+                        }
+                    }
+                    for (int o = 8; o > 0; o >>= 1) {
+                        sr += __shfl_xor(sr, o);
+                        si += __shfl_xor(si, o);
+                    }
+                    if (ok && sub == 0) {
+                        const int slot = c < WN_CH_A ? c - WN_CH_DDH : 6 + (c - WN_CH_DA);
+                        pwd[(((size_t)k * WN_PCAP + p) * 2 + d) * 15 + slot] = make_double2(sr, si);
+                    }
                 }
-                if (ok && sub == 0) {
-                    const int slot = c < WN_CH_A ? c - WN_CH_DDH : 6 + (c - WN_CH_DA);
-                    pwd[(((size_t)k * WN_PCAP + p) * 2 + d) * 15 + slot] = make_double2(sr, si);
-                }
+#pragma unroll
+                for (int s = 0; s < KST; ++s) xv[s] = nxv[s];
             }
-#pragma unroll
-            for (int s = 0; s < KST; ++s) xv[s] = nxv[s];
-        }
+        };
+        if (nb_ <= 8)
+            channels(std::true_type{});
+        else
+            channels(std::false_type{});
         return;
     }
     // FP32 channels (n even): pairs of consecutive elements per 16-byte load when the tile splits into pairs
