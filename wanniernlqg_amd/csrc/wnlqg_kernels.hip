@@ -1422,7 +1422,8 @@ __global__ __launch_bounds__(256) void k_rotate_f64(double *__restrict__ x, cons
 //   Y = T U:    P1 = Tr Ur,   P2 = Ti Ui,   P3 = (Tr + Ti) (Ur + Ui);    Yr = P1 - P2, Yi = P3 - P1 - P2
 // with U, X (then T) in LDS as planar real arrays padded to np = 16*ceil(n/16) (zeros beyond n).
 // The (np/16)^2 output tiles are dealt to the 4 waves.
-// WARM: the warm-start transform of the mesh eigensolver: channel 0 replaced by W^+ H W, with W the
+// WARM: the warm-start transform of the mesh eigensolver: channel 0's upper triangle replaced by that of W^+ H W
+// (H the Hermitian matrix of channel 0's upper triangle, read in row order), with W the
 // group anchor's eigenvectors for the pass-1 points (uw = nullptr), or W = uw[k], the eigenvectors of
 // the same point of the previous mesh plane, for every point (uw set).  Otherwise channels 1 .. C-1 of
 // every k point with its own U.
@@ -1461,19 +1462,9 @@ __global__ __launch_bounds__(256) void k_rotate_mfma(double *__restrict__ x, con
         for (int q = 0; q < PF; ++q) {
             const int e = t + 256 * q;
             const int i = e / np, j = e - (e / np) * np;
-            double2 v = make_double2(0.0, 0.0);
-            if (e < np * np && i < n && j < n) {
-                if (!WARM) {
-                    v = xc[i * n + j];
-                } else if (j >= i) {   // the Hermitian matrix of the upper triangle (the eigensolver's input)
-                    v = xc[i * n + j];
-                    if (i == j) v.y = 0.0;
-                } else {
-                    v = xc[j * n + i];
-                    v.y = -v.y;
-                }
-            }
-            pf[q] = v;
+            // (WARM: every element read in row order; the Hermitian matrix of the upper triangle, the eigensolver's
+            // input, is formed when the tile is stored)
+            pf[q] = (e < np * np && i < n && j < n) ? xc[i * n + j] : make_double2(0.0, 0.0);
         }
     };
     if (cbeg < cend) prefetch(cbeg);
@@ -1485,8 +1476,16 @@ __global__ __launch_bounds__(256) void k_rotate_mfma(double *__restrict__ x, con
             const int e = t + 256 * q;
             if (e < np * np) {
                 const int i = e / np, j = e - (e / np) * np;
-                Xr[i * ld + j] = pf[q].x;
-                Xi[i * ld + j] = pf[q].y;
+                if (!WARM) {
+                    Xr[i * ld + j] = pf[q].x;
+                    Xi[i * ld + j] = pf[q].y;
+                } else if (j >= i) {
+                    // upper-triangle element (i, j) and its conjugate at (j, i); the diagonal is real
+                    Xr[i * ld + j] = pf[q].x;
+                    Xi[i * ld + j] = i == j ? 0.0 : pf[q].y;
+                    Xr[j * ld + i] = pf[q].x;
+                    Xi[j * ld + i] = i == j ? 0.0 : -pf[q].y;
+                }
             }
         }
         __syncthreads();
@@ -1530,9 +1529,10 @@ __global__ __launch_bounds__(256) void k_rotate_mfma(double *__restrict__ x, con
         }
         if (c + 1 < cend) prefetch(c + 1);
         __syncthreads();
-        // Y = T U, written to global
+        // Y = T U, written to global (WARM: the upper triangle only, which is what the eigensolver reads)
         for (int tile = wave; tile < nt * nt; tile += 4) {
             const int r0 = (tile / nt) * 16, c0 = (tile % nt) * 16;
+            if (WARM && r0 > c0) continue;   // a tile below the diagonal
             f64x4 p1 = {0.0, 0.0, 0.0, 0.0}, p2 = {0.0, 0.0, 0.0, 0.0}, p3 = {0.0, 0.0, 0.0, 0.0};
 #pragma unroll
             for (int k0 = 0; k0 < np; k0 += 4) {
@@ -1547,7 +1547,8 @@ __global__ __launch_bounds__(256) void k_rotate_mfma(double *__restrict__ x, con
 #pragma unroll
             for (int q = 0; q < 4; ++q) {
                 const int row = r0 + lq + 4 * q, col = c0 + lr;
-                if (row < n && col < n) xc[row * n + col] = make_double2(p1[q] - p2[q], p3[q] - (p1[q] + p2[q]));
+                if (row < n && col < n && (!WARM || col >= row))
+                    xc[row * n + col] = make_double2(p1[q] - p2[q], p3[q] - (p1[q] + p2[q]));
             }
         }
     }
