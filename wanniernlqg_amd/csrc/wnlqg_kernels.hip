@@ -2263,6 +2263,7 @@ struct PairView {
     int n2;
     int has_pos;
     const double2 *pw;   // the pair's dd H and dA elements (k_rotate_sc), per direction; nullptr: read the channels
+    unsigned long long bands;   // pair bands of a point whose dH / A rows were stored at them (else 0)
     __device__ __forceinline__ const double2 *V(int y) const { return xk + (size_t)(WN_CH_DH + y) * n2; }
     __device__ __forceinline__ const double2 *W(int s) const { return xk + (size_t)(WN_CH_DDH + s) * n2; }
     __device__ __forceinline__ const double2 *A(int y) const { return xk + (size_t)(WN_CH_A + y) * n2; }
@@ -2291,14 +2292,19 @@ __device__ __forceinline__ void wn_gderiv_part(const PairView &pv, const double 
         const bool skip = (l == n) || (l == m);
         const double inl = skip ? 0.0 : inv[n * nb + l];
         const double ilm = skip ? 0.0 : inv[l * nb + m];
-        const int nl = n * nb + l, lm = l * nb + m;
-        const double2 vxnl = Vx[nl], vxlm = Vx[lm];
+        // column m at row l: for a point whose rotation stored rows at the pairs' bands (pv.bands != 0) and
+        // l not one of them, the stored element is the conjugate of row m's (l), read along the row instead
+        const bool via_row = pv.bands != 0ull && !((pv.bands >> l) & 1ull);
+        const int nl = n * nb + l, lm = via_row ? m * nb + l : l * nb + m;
+        const double sg = via_row ? -1.0 : 1.0;
+        auto cj = [&](double2 v) { return make_double2(v.x, sg * v.y); };
+        const double2 vxnl = Vx[nl], vxlm = cj(Vx[lm]);
         const double2 vnl[3] = {V0[nl], V1[nl], V2[nl]};
-        const double2 vlm[3] = {V0[lm], V1[lm], V2[lm]};
+        const double2 vlm[3] = {cj(V0[lm]), cj(V1[lm]), cj(V2[lm])};
         double2 axnl = make_double2(0.0, 0.0), axlm = make_double2(0.0, 0.0);
         if (POS) {
             axnl = Ax[nl];
-            axlm = Ax[lm];
+            axlm = cj(Ax[lm]);
         }
 #pragma unroll
         for (int y = 0; y < 3; ++y) {
@@ -2504,8 +2510,22 @@ __global__ __launch_bounds__(256, 3) void k_pairs_f64(const double *__restrict__
     pv.n2 = n2;
     pv.has_pos = sh.has_pos;
     pv.pw = nullptr;
+    pv.bands = 0ull;
     // dd H and dA elements precomputed by k_rotate_sc for points with at most WN_PCAP pairs
     const bool compact = pwd != nullptr && total <= WN_PCAP;
+    // Hermitian model: k_rotate_sc stored the velocity and position channels of these points as rows at the
+    // pairs' bands and their conjugates as columns (n <= 64)
+    __shared__ unsigned long long bmk;
+    if (compact && sh.herm) {
+        if (t == 0) bmk = 0ull;
+        __syncthreads();
+        if (t < total) {
+            const int4 mt = pmeta[pbase + t];
+            atomicOr(&bmk, (1ull << mt.x) | (1ull << mt.y));
+        }
+        __syncthreads();
+        pv.bands = bmk;
+    }
     const int ncomp = 27 * sc.nq;
     // Chunks of WN_PCH pairs.  Phase A: one item per (pair, direction d, component x) forms
     // r^y_{n'm';x} for y = 0..2 with (n', m') = (n, m) or (m, n); one item per pair also stages the
