@@ -94,15 +94,15 @@ def rotate_sc_bytes(st, n: int, precision: str) -> float:
     channels after H (n^2 complex each: 16 bytes per entry in FP64, 8 in the mixed form's FP32 storage) and
     its eigenvectors U (16 n^2).  Writes: points rotated in full write 21 channels (16 n^2 each); element
     points write the pairs' elements of dd H and dA (2 directions x 15 complex per pair) and dH, A (6
-    channels) either in full (16 n^2) or, for Hermitian models, the rows and columns at the pairs' bands
-    B (2 |B| n - |B|^2 entries each)."""
+    channels) either in full (16 n^2) or, for Hermitian models, the rows at the pairs' bands B (|B| n entries
+    each; the pair kernel reads the columns as their conjugates)."""
     pts = st["rotate_full_points"] + st["rotate_element_points"]
     ch = 8.0 if precision == "mixed" else 16.0
     rd = pts * (21 * n * n * ch + 16.0 * n * n)
     el, rows = st["rotate_element_points"], st["rotate_row_points"]
     mb = st["rotate_element_bands"] / max(1, el)
     wr = (st["rotate_full_points"] * 21 * 16.0 * n * n + st["active_pairs"] * 2 * 15 * 16.0
-          + (el - rows) * 6 * 16.0 * n * n + rows * 6 * 16.0 * (2 * mb * n - mb * mb))
+          + (el - rows) * 6 * 16.0 * n * n + rows * 6 * 16.0 * mb * n)
     return rd + wr
 
 

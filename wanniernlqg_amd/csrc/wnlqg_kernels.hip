@@ -1585,7 +1585,6 @@ k_rotate_sc(double *__restrict__ x, const double2 *__restrict__ u,
                                                   double2 *__restrict__ pwd, int n, int C, int ldo,
                                                   int *__restrict__ flags, int herm) {
     extern __shared__ double smd[];
-    __shared__ unsigned long long bmask;   // the pairs' bands as a bit set
     __shared__ int bl[16];            // bands of the pairs
     __shared__ int prow[WN_PCAP][2];  // row of n and of m in bl, per pair
     __shared__ int pband[WN_PCAP][2]; // n and m
@@ -1618,9 +1617,6 @@ k_rotate_sc(double *__restrict__ x, const double2 *__restrict__ u,
             }
         }
         nbl = nb_;
-        unsigned long long mk = 0ull;
-        for (int r = 0; r < nb_; ++r) mk |= 1ull << bl[r];
-        bmask = mk;
     }
     if (t == 0) {
         int *bk = flags + WN_CTR_BASE + WN_CTR_STRIDE * (k % WN_CTR_BUCKETS);
@@ -1676,7 +1672,6 @@ k_rotate_sc(double *__restrict__ x, const double2 *__restrict__ u,
                 xv[s] = (colok && kr < n) ? xc[kr * n + c0 + lr] : XT{};
             }
         };
-        const unsigned long long mk = bmask;
         auto channels = [&](auto small_tag) {
             constexpr bool SM = decltype(small_tag)::value;
             constexpr int NR = SM ? 2 : 1;   // row tiles
@@ -1754,8 +1749,8 @@ k_rotate_sc(double *__restrict__ x, const double2 *__restrict__ u,
                 }
                 __syncthreads();
                 if (full) {
-                    // rows R_B = T_B U over this wave's K range; row bl[r] written, its conjugate as column bl[r]
-                    // outside the B x B block (one writer per element)
+                    // rows R_B = T_B U over this wave's K range, written at rows bl[r] (the pair kernel reads the
+                    // other entries of the pairs' columns as conjugates of these rows)
                     double yr[NQ], yi[NQ];
 #pragma unroll
                     for (int q = 0; q < NQ; ++q) yr[q] = yi[q] = 0.0;
@@ -1822,11 +1817,7 @@ k_rotate_sc(double *__restrict__ x, const double2 *__restrict__ u,
                                 vr += Xr[r * ld + col];
                                 vi += Xi[r * ld + col];
                             }
-                            if (r < nb_ && col < n) {
-                                const int b = bl[r];
-                                xc[b * n + col] = make_double2(vr, vi);
-                                if (!((mk >> col) & 1ull)) xc[col * n + b] = make_double2(vr, -vi);
-                            }
+                            if (r < nb_ && col < n) xc[bl[r] * n + col] = make_double2(vr, vi);
                         }
                     }
                 } else {
@@ -2013,8 +2004,8 @@ k_rotate_sc(double *__restrict__ x, const double2 *__restrict__ u,
             }
             __syncthreads();
             if (rows) {
-                // rows of the Hermitian U^+ X U at the bands B: R_B = T_B U (K split as above); row bl[r] is
-                // written, and its conjugate as column bl[r] outside the B x B block (one writer per element)
+                // rows of the Hermitian U^+ X U at the bands B: R_B = T_B U (K split as above), written at rows
+                // bl[r] (the pair kernel reads the other entries of the pairs' columns as conjugates of these rows)
                 f64x4 yr = {0.0, 0.0, 0.0, 0.0}, yi = {0.0, 0.0, 0.0, 0.0};
                 if (tw) {
                     const int c0 = (wave % nt) * 16;
@@ -2052,7 +2043,6 @@ k_rotate_sc(double *__restrict__ x, const double2 *__restrict__ u,
                 }
                 if (tw && kp == 0) {
                     const int c0 = (wave % nt) * 16;
-                    const unsigned long long mk = bmask;
 #pragma unroll
                     for (int q = 0; q < 4; ++q) {
                         const int r = lq + 4 * q, col = c0 + lr;
@@ -2061,11 +2051,7 @@ k_rotate_sc(double *__restrict__ x, const double2 *__restrict__ u,
                             vr += Xr[r * ld + col];
                             vi += Xi[r * ld + col];
                         }
-                        if (r < nb_ && col < n) {
-                            const int b = bl[r];
-                            xc[b * n + col] = make_double2(vr, vi);
-                            if (!((mk >> col) & 1ull)) xc[col * n + b] = make_double2(vr, -vi);
-                        }
+                        if (r < nb_ && col < n) xc[bl[r] * n + col] = make_double2(vr, vi);
                     }
                 }
                 continue;
@@ -2292,8 +2278,8 @@ __device__ __forceinline__ void wn_gderiv_part(const PairView &pv, const double 
         const bool skip = (l == n) || (l == m);
         const double inl = skip ? 0.0 : inv[n * nb + l];
         const double ilm = skip ? 0.0 : inv[l * nb + m];
-        // column m at row l: for a point whose rotation stored rows at the pairs' bands (pv.bands != 0) and
-        // l not one of them, the stored element is the conjugate of row m's (l), read along the row instead
+        // column m at row l: for a point whose rotation stored only rows at the pairs' bands (pv.bands != 0) and
+        // l not one of them, the element is the conjugate of row m's element l (the channel is Hermitian)
         const bool via_row = pv.bands != 0ull && !((pv.bands >> l) & 1ull);
         const int nl = n * nb + l, lm = via_row ? m * nb + l : l * nb + m;
         const double sg = via_row ? -1.0 : 1.0;
@@ -2514,7 +2500,7 @@ __global__ __launch_bounds__(256, 3) void k_pairs_f64(const double *__restrict__
     // dd H and dA elements precomputed by k_rotate_sc for points with at most WN_PCAP pairs
     const bool compact = pwd != nullptr && total <= WN_PCAP;
     // Hermitian model: k_rotate_sc stored the velocity and position channels of these points as rows at the
-    // pairs' bands and their conjugates as columns (n <= 64)
+    // pairs' bands only (n <= 64); element (l, m) at another band l is the conjugate of row m's element l
     __shared__ unsigned long long bmk;
     if (compact && sh.herm) {
         if (t == 0) bmk = 0ull;
