@@ -1567,7 +1567,7 @@ __global__ __launch_bounds__(256) void k_rotate_mfma(double *__restrict__ x, con
 // mixed form's mesh channels); the products are FP64 and every channel written back is FP64.
 template <int NP, int PF, bool C32>
 #ifndef WN_ROTSC_WAVES
-#define WN_ROTSC_WAVES 4     // FP32 channels
+#define WN_ROTSC_WAVES 3     // FP32 channels
 #endif
 #ifndef WN_ROTSC_WAVES64
 #define WN_ROTSC_WAVES64 4   // FP64 channels
@@ -1577,7 +1577,8 @@ template <int NP, int PF, bool C32>
 // four waves per SIMD, 283 ms per 128^3 mesh, and runs at 215 ms with three) than the staged loop (193 ms)
 #define WN_ROTSC_REG64 0
 #endif
-// up to four workgroups per CU at np = 32 (34 KB of LDS each): registers for that many waves per SIMD
+// np = 32 (34 KB of LDS per workgroup): registers for three waves per SIMD with FP32 channels (the register
+// operand loop carries two channels' X slices and the next two), four with FP64 channels
 __global__ __attribute__((amdgpu_flat_work_group_size(256, 256),
                           amdgpu_waves_per_eu(NP == 32 ? (C32 ? WN_ROTSC_WAVES : WN_ROTSC_WAVES64) : 1))) void
 k_rotate_sc(double *__restrict__ x, const double2 *__restrict__ u,
@@ -1672,25 +1673,20 @@ k_rotate_sc(double *__restrict__ x, const double2 *__restrict__ u,
                 xv[s] = (colok && kr < n) ? xc[kr * n + c0 + lr] : XT{};
             }
         };
-        auto channels = [&](auto small_tag) {
-            constexpr bool SM = decltype(small_tag)::value;
-            constexpr int NR = SM ? 2 : 1;   // row tiles
-            constexpr int NQ = SM ? 2 : 4;   // rows of the result per lane
-            // row of the result held in slot q of a lane (band-list index)
-            auto rrow = [&](int q) { return SM ? 4 * q + lq : lq + 4 * q; };
-            // A operand slices of U_B^+: row of the tile = lane & 3 (4-row tiles) or lr (16-row tile)
-            double aur[NR][KST], aui[NR][KST];
+        // more than 8 bands: one 16-row tile per channel (rows lq + 4 q of the result in slot q of a lane)
+        auto channels16 = [&]() {
+            constexpr int NQ = 4;
+            auto rrow = [&](int q) { return lq + 4 * q; };
+            // A operand slices of U_B^+ (row lr of the tile)
+            double aur[KST], aui[KST];
 #pragma unroll
-            for (int rt = 0; rt < NR; ++rt)
-#pragma unroll
-                for (int s = 0; s < KST; ++s) {
-                    const int kr = kb + 4 * s + lq;
-                    const int row = SM ? 4 * rt + (lane & 3) : lr;
-                    const int br = row < nb_ ? bl[row] : 0;
-                    const double rm = row < nb_ ? 1.0 : 0.0;
-                    aur[rt][s] = rm * Ur[kr * ld + br];
-                    aui[rt][s] = rm * Ui[kr * ld + br];
-                }
+            for (int s = 0; s < KST; ++s) {
+                const int kr = kb + 4 * s + lq;
+                const int br = lr < nb_ ? bl[lr] : 0;
+                const double rm = lr < nb_ ? 1.0 : 0.0;
+                aur[s] = rm * Ur[kr * ld + br];
+                aui[s] = rm * Ui[kr * ld + br];
+            }
             XT xv[KST], nxv[KST];
             load_x(1, xv);
             for (int c = 1; c < C; ++c) {
@@ -1708,35 +1704,18 @@ k_rotate_sc(double *__restrict__ x, const double2 *__restrict__ u,
 #pragma unroll
                 for (int q = 0; q < NQ; ++q) tr[q] = ti[q] = 0.0;
                 if (tw) {
-                    if constexpr (SM) {
+                    f64x4 p1 = {0.0, 0.0, 0.0, 0.0}, p2 = {0.0, 0.0, 0.0, 0.0}, p3 = {0.0, 0.0, 0.0, 0.0};
 #pragma unroll
-                        for (int rt = 0; rt < NR; ++rt) {
-                            if (rt * 4 >= nb_) break;   // a tile of padding rows only (zero)
-                            double p1 = 0.0, p2 = 0.0, p3 = 0.0;
+                    for (int s = 0; s < KST; ++s) {
+                        p1 = __builtin_amdgcn_mfma_f64_16x16x4f64(aur[s], xr[s], p1, 0, 0, 0);
+                        p2 = __builtin_amdgcn_mfma_f64_16x16x4f64(aui[s], xi[s], p2, 0, 0, 0);
+                        p3 = __builtin_amdgcn_mfma_f64_16x16x4f64(aur[s] + aui[s], xi[s] - xr[s], p3, 0, 0,
+                                                                  0);
+                    }
 #pragma unroll
-                            for (int s = 0; s < KST; ++s) {
-                                p1 = __builtin_amdgcn_mfma_f64_4x4x4f64(aur[rt][s], xr[s], p1, 0, 0, 0);
-                                p2 = __builtin_amdgcn_mfma_f64_4x4x4f64(aui[rt][s], xi[s], p2, 0, 0, 0);
-                                p3 = __builtin_amdgcn_mfma_f64_4x4x4f64(aur[rt][s] + aui[rt][s], xi[s] - xr[s], p3,
-                                                                        0, 0, 0);
-                            }
-                            tr[rt] = p1 + p2;
-                            ti[rt] = p3 + (p1 - p2);
-                        }
-                    } else {
-                        f64x4 p1 = {0.0, 0.0, 0.0, 0.0}, p2 = {0.0, 0.0, 0.0, 0.0}, p3 = {0.0, 0.0, 0.0, 0.0};
-#pragma unroll
-                        for (int s = 0; s < KST; ++s) {
-                            p1 = __builtin_amdgcn_mfma_f64_16x16x4f64(aur[0][s], xr[s], p1, 0, 0, 0);
-                            p2 = __builtin_amdgcn_mfma_f64_16x16x4f64(aui[0][s], xi[s], p2, 0, 0, 0);
-                            p3 = __builtin_amdgcn_mfma_f64_16x16x4f64(aur[0][s] + aui[0][s], xi[s] - xr[s], p3, 0, 0,
-                                                                      0);
-                        }
-#pragma unroll
-                        for (int q = 0; q < NQ; ++q) {
-                            tr[q] = p1[q] + p2[q];
-                            ti[q] = p3[q] + (p1[q] - p2[q]);
-                        }
+                    for (int q = 0; q < NQ; ++q) {
+                        tr[q] = p1[q] + p2[q];
+                        ti[q] = p3[q] + (p1[q] - p2[q]);
                     }
                 }
                 __syncthreads();   // the previous channel's readers of the partial rows are done
@@ -1755,46 +1734,23 @@ k_rotate_sc(double *__restrict__ x, const double2 *__restrict__ u,
 #pragma unroll
                     for (int q = 0; q < NQ; ++q) yr[q] = yi[q] = 0.0;
                     if (tw) {
-                        if constexpr (SM) {
+                        f64x4 p1 = {0.0, 0.0, 0.0, 0.0}, p2 = {0.0, 0.0, 0.0, 0.0}, p3 = {0.0, 0.0, 0.0, 0.0};
 #pragma unroll
-                            for (int rt = 0; rt < NR; ++rt) {
-                                if (rt * 4 >= nb_) break;
-                                double p1 = 0.0, p2 = 0.0, p3 = 0.0;
-                                const int row = 4 * rt + (lane & 3);
-#pragma unroll
-                                for (int s = 0; s < KST; ++s) {
-                                    const int kc = kb + 4 * s + lq;
-                                    double tr_ = Xr[row * ld + kc], ti_ = Xi[row * ld + kc];
-                                    if constexpr (KS == 2) {
-                                        tr_ += Xr[(16 + row) * ld + kc];
-                                        ti_ += Xi[(16 + row) * ld + kc];
-                                    }
-                                    p1 = __builtin_amdgcn_mfma_f64_4x4x4f64(tr_, bur[s], p1, 0, 0, 0);
-                                    p2 = __builtin_amdgcn_mfma_f64_4x4x4f64(ti_, bui[s], p2, 0, 0, 0);
-                                    p3 = __builtin_amdgcn_mfma_f64_4x4x4f64(tr_ + ti_, bur[s] + bui[s], p3, 0, 0, 0);
-                                }
-                                yr[rt] = p1 - p2;
-                                yi[rt] = p3 - (p1 + p2);
+                        for (int s = 0; s < KST; ++s) {
+                            const int kc = kb + 4 * s + lq;
+                            double tr_ = Xr[lr * ld + kc], ti_ = Xi[lr * ld + kc];
+                            if constexpr (KS == 2) {
+                                tr_ += Xr[(16 + lr) * ld + kc];
+                                ti_ += Xi[(16 + lr) * ld + kc];
                             }
-                        } else {
-                            f64x4 p1 = {0.0, 0.0, 0.0, 0.0}, p2 = {0.0, 0.0, 0.0, 0.0}, p3 = {0.0, 0.0, 0.0, 0.0};
+                            p1 = __builtin_amdgcn_mfma_f64_16x16x4f64(tr_, bur[s], p1, 0, 0, 0);
+                            p2 = __builtin_amdgcn_mfma_f64_16x16x4f64(ti_, bui[s], p2, 0, 0, 0);
+                            p3 = __builtin_amdgcn_mfma_f64_16x16x4f64(tr_ + ti_, bur[s] + bui[s], p3, 0, 0, 0);
+                        }
 #pragma unroll
-                            for (int s = 0; s < KST; ++s) {
-                                const int kc = kb + 4 * s + lq;
-                                double tr_ = Xr[lr * ld + kc], ti_ = Xi[lr * ld + kc];
-                                if constexpr (KS == 2) {
-                                    tr_ += Xr[(16 + lr) * ld + kc];
-                                    ti_ += Xi[(16 + lr) * ld + kc];
-                                }
-                                p1 = __builtin_amdgcn_mfma_f64_16x16x4f64(tr_, bur[s], p1, 0, 0, 0);
-                                p2 = __builtin_amdgcn_mfma_f64_16x16x4f64(ti_, bui[s], p2, 0, 0, 0);
-                                p3 = __builtin_amdgcn_mfma_f64_16x16x4f64(tr_ + ti_, bur[s] + bui[s], p3, 0, 0, 0);
-                            }
-#pragma unroll
-                            for (int q = 0; q < NQ; ++q) {
-                                yr[q] = p1[q] - p2[q];
-                                yi[q] = p3[q] - (p1[q] + p2[q]);
-                            }
+                        for (int q = 0; q < NQ; ++q) {
+                            yr[q] = p1[q] - p2[q];
+                            yi[q] = p3[q] - (p1[q] + p2[q]);
                         }
                     }
                     if constexpr (KS == 2) {
@@ -1852,10 +1808,171 @@ k_rotate_sc(double *__restrict__ x, const double2 *__restrict__ u,
                 for (int s = 0; s < KST; ++s) xv[s] = nxv[s];
             }
         };
+        // Points with <= 8 bands, two channels per barrier phase: the 4-row tiles' partial rows of channel c sit
+        // at rows 16 kp + r and those of channel c + 1 at rows 8 + 16 kp + r (r < 8), so one pair of barriers
+        // serves both channels (and one more pair the K-split exchange of the full channels among them).
+        auto channel_pairs = [&]() {
+            double aur[2][KST], aui[2][KST];
+#pragma unroll
+            for (int rt = 0; rt < 2; ++rt)
+#pragma unroll
+                for (int s = 0; s < KST; ++s) {
+                    const int kr = kb + 4 * s + lq;
+                    const int row = 4 * rt + (lane & 3);
+                    const int br = row < nb_ ? bl[row] : 0;
+                    const double rm = row < nb_ ? 1.0 : 0.0;
+                    aur[rt][s] = rm * Ur[kr * ld + br];
+                    aui[rt][s] = rm * Ui[kr * ld + br];
+                }
+            auto is_full = [](int c) { return (c >= WN_CH_DH && c < WN_CH_DDH) || (c >= WN_CH_A && c < WN_CH_DA); };
+            XT xv[2][KST], nxv[2][KST];
+            load_x(1, xv[0]);
+            if (2 < C) load_x(2, xv[1]);
+            for (int c = 1; c < C; c += 2) {
+                const bool hasb = c + 1 < C;
+                if (c + 2 < C) load_x(c + 2, nxv[0]);
+                if (c + 3 < C) load_x(c + 3, nxv[1]);
+                // partial T_B = U_B^+ X over this wave's K range, both channels (rows 4 rt + lq, column c0 + lr)
+                double tr[2][2], ti[2][2];
+#pragma unroll
+                for (int ch = 0; ch < 2; ++ch) {
+                    tr[ch][0] = tr[ch][1] = ti[ch][0] = ti[ch][1] = 0.0;
+                    if (!tw || (ch == 1 && !hasb)) continue;
+                    double xr[KST], xi[KST];
+#pragma unroll
+                    for (int s = 0; s < KST; ++s) {
+                        xr[s] = (double)xv[ch][s].x;
+                        xi[s] = (double)xv[ch][s].y;
+                    }
+#pragma unroll
+                    for (int rt = 0; rt < 2; ++rt) {
+                        if (rt * 4 >= nb_) break;   // a tile of padding rows only (zero)
+                        double p1 = 0.0, p2 = 0.0, p3 = 0.0;
+#pragma unroll
+                        for (int s = 0; s < KST; ++s) {
+                            p1 = __builtin_amdgcn_mfma_f64_4x4x4f64(aur[rt][s], xr[s], p1, 0, 0, 0);
+                            p2 = __builtin_amdgcn_mfma_f64_4x4x4f64(aui[rt][s], xi[s], p2, 0, 0, 0);
+                            p3 = __builtin_amdgcn_mfma_f64_4x4x4f64(aur[rt][s] + aui[rt][s], xi[s] - xr[s], p3, 0,
+                                                                    0, 0);
+                        }
+                        tr[ch][rt] = p1 + p2;
+                        ti[ch][rt] = p3 + (p1 - p2);
+                    }
+                }
+                __syncthreads();   // the previous pair's readers of the partial rows are done
+                if (tw) {
+#pragma unroll
+                    for (int ch = 0; ch < 2; ++ch)
+#pragma unroll
+                        for (int q = 0; q < 2; ++q) {
+                            Xr[(8 * ch + 16 * kp + 4 * q + lq) * ld + c0 + lr] = tr[ch][q];
+                            Xi[(8 * ch + 16 * kp + 4 * q + lq) * ld + c0 + lr] = ti[ch][q];
+                        }
+                }
+                __syncthreads();
+                const bool fl[2] = {is_full(c), hasb && is_full(c + 1)};
+                double yr[2][2], yi[2][2];
+#pragma unroll
+                for (int ch = 0; ch < 2; ++ch) {
+                    yr[ch][0] = yr[ch][1] = yi[ch][0] = yi[ch][1] = 0.0;
+                    if (ch == 1 && !hasb) continue;
+                    if (fl[ch]) {
+                        // rows R_B = T_B U over this wave's K range
+                        if (!tw) continue;
+#pragma unroll
+                        for (int rt = 0; rt < 2; ++rt) {
+                            if (rt * 4 >= nb_) break;
+                            double p1 = 0.0, p2 = 0.0, p3 = 0.0;
+                            const int row = 8 * ch + 4 * rt + (lane & 3);
+#pragma unroll
+                            for (int s = 0; s < KST; ++s) {
+                                const int kc = kb + 4 * s + lq;
+                                double tr_ = Xr[row * ld + kc], ti_ = Xi[row * ld + kc];
+                                if constexpr (KS == 2) {
+                                    tr_ += Xr[(16 + row) * ld + kc];
+                                    ti_ += Xi[(16 + row) * ld + kc];
+                                }
+                                p1 = __builtin_amdgcn_mfma_f64_4x4x4f64(tr_, bur[s], p1, 0, 0, 0);
+                                p2 = __builtin_amdgcn_mfma_f64_4x4x4f64(ti_, bui[s], p2, 0, 0, 0);
+                                p3 = __builtin_amdgcn_mfma_f64_4x4x4f64(tr_ + ti_, bur[s] + bui[s], p3, 0, 0, 0);
+                            }
+                            yr[ch][rt] = p1 - p2;
+                            yi[ch][rt] = p3 - (p1 + p2);
+                        }
+                    } else {
+                        // the pairs' elements sum_q T_B[a][q] U[q][b]: 16 lanes per (pair, direction)
+                        const int item = t >> 4, sub = t & 15;
+                        const int p = item >> 1, d = item & 1;
+                        const bool ok = p < cnt;
+                        const int ra = ok ? 8 * ch + prow[p][d] : 0, bb = ok ? pband[p][d ^ 1] : 0;
+                        double sr = 0.0, si = 0.0;
+                        if (ok) {
+                            for (int q = sub; q < np; q += 16) {
+                                double tr_ = Xr[ra * ld + q], ti_ = Xi[ra * ld + q];
+                                if constexpr (KS == 2) {
+                                    tr_ += Xr[(16 + ra) * ld + q];
+                                    ti_ += Xi[(16 + ra) * ld + q];
+                                }
+                                const double ur = Ur[q * ld + bb], ui = Ui[q * ld + bb];
+                                sr += tr_ * ur - ti_ * ui;
+                                si += tr_ * ui + ti_ * ur;
+                            }
+                        }
+                        for (int o = 8; o > 0; o >>= 1) {
+                            sr += __shfl_xor(sr, o);
+                            si += __shfl_xor(si, o);
+                        }
+                        if (ok && sub == 0) {
+                            const int cc = c + ch;
+                            const int slot = cc < WN_CH_A ? cc - WN_CH_DDH : 6 + (cc - WN_CH_DA);
+                            pwd[(((size_t)k * WN_PCAP + p) * 2 + d) * 15 + slot] = make_double2(sr, si);
+                        }
+                    }
+                }
+                if (fl[0] || fl[1]) {
+                    if constexpr (KS == 2) {
+                        __syncthreads();   // every read of the partial rows is done
+                        if (tw && kp == 1) {
+#pragma unroll
+                            for (int ch = 0; ch < 2; ++ch) {
+                                if (!fl[ch]) continue;
+#pragma unroll
+                                for (int q = 0; q < 2; ++q) {
+                                    Xr[(8 * ch + 4 * q + lq) * ld + c0 + lr] = yr[ch][q];
+                                    Xi[(8 * ch + 4 * q + lq) * ld + c0 + lr] = yi[ch][q];
+                                }
+                            }
+                        }
+                        __syncthreads();
+                    }
+                    if (tw && kp == 0) {
+#pragma unroll
+                        for (int ch = 0; ch < 2; ++ch) {
+                            if (!fl[ch]) continue;
+                            double2 *xc = xk + (size_t)(c + ch) * n * n;
+#pragma unroll
+                            for (int q = 0; q < 2; ++q) {
+                                const int r = 4 * q + lq, col = c0 + lr;
+                                double vr = yr[ch][q], vi = yi[ch][q];
+                                if constexpr (KS == 2) {
+                                    vr += Xr[(8 * ch + r) * ld + col];
+                                    vi += Xi[(8 * ch + r) * ld + col];
+                                }
+                                if (r < nb_ && col < n) xc[bl[r] * n + col] = make_double2(vr, vi);
+                            }
+                        }
+                    }
+                }
+#pragma unroll
+                for (int ch = 0; ch < 2; ++ch)
+#pragma unroll
+                    for (int s = 0; s < KST; ++s) xv[ch][s] = nxv[ch][s];
+            }
+        };
         if (nb_ <= 8)
-            channels(std::true_type{});
+            channel_pairs();
         else
-            channels(std::false_type{});
+            channels16();
         return;
     }
     // FP32 channels (n even): pairs of consecutive elements per 16-byte load when the tile splits into pairs
