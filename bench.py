@@ -166,7 +166,9 @@ def roofline(st, n, nk, precision):
     elif st["separable"]:
         launches = 3 * nb
         avg_s = 1e-3 * st["interp_ms"] / launches
-        roof = {"kernel": "k_contract (separable interpolation, writes every interpolated matrix)", "bound": "hbm",
+        roof = {"kernel": "k_contract (separable interpolation in three stages: the coefficient box read once per "
+                          "plane, every interpolated matrix written, the channels after H in FP32 in the mixed form; "
+                          "the partial sums between stages not counted)", "bound": "hbm",
                 "achieved": st["interp_bytes"] / launches / avg_s / 1e9, "peak": HBM_PEAK_GBS, "unit": "GB/s"}
         tkey = "k_contract"
     else:

@@ -142,7 +142,7 @@ typedef struct wnlqg_stats {
     double interp_flops;    /* algorithmic interpolation flops: 8 * N_R * n^2 * C per k point */
     int32_t max_sweeps;     /* largest Jacobi sweep count over the k points */
     int32_t separable;      /* 1 if the mesh used the separable plane contraction */
-    double interp_bytes;    /* bytes of interpolated channels written (separable contraction) */
+    double interp_bytes;    /* separable contraction: interpolated channels written + coefficient box read per plane */
     double mean_sweeps;     /* Jacobi sweeps per k point, averaged over the mesh */
     double warm_ms;         /* warm-start transform U_a^+ H U_a of the mesh eigensolver */
     int32_t eig_launches;   /* eigensolver kernel launches (2 per batch when warm-started) */

@@ -1089,7 +1089,14 @@ int enqueue_mesh(wnlqg_engine *g, std::vector<std::array<hipEvent_t, WN_MARKS>> 
         mark(bi, 1, s0);
         if (p.sep) {
             WN_HIP(launch_contract_batch(g, bt, xb, s0));
-            if (interp_bytes) *interp_bytes += (double)cnt * g->sh.ldo * 8.0;
+            if (interp_bytes) {
+                // algorithmic bytes: the interpolated channels written (those after H in FP32 when ch32) and
+                // the coefficient box read once per plane; the partial sums between stages are not counted
+                const double nn = (double)g->sh.n * g->sh.n;
+                *interp_bytes += (double)cnt * (g->ch32 ? 16.0 * nn + 8.0 * (m2 - nn) : 16.0 * m2);
+                if (bt.k2b == 0)
+                    *interp_bytes += (double)bt.nk3 * g->span[0] * g->span[1] * g->span[2] * m2 * 16.0;
+            }
         } else {
             WN_HIP(launch_interp(g, rows, xb));
         }
