@@ -864,10 +864,17 @@ __global__ __launch_bounds__(256) void k_contract_mfma(const double2 *__restrict
                 if (p >= P || m >= M) continue;
                 const double cr = p1[j] - p2[j], ci = p3[j] - (p1[j] + p2[j]);
                 double2 *row = out + b * ob + (long long)p * op;
-                if (f32)
-                    ((float2 *)row)[dst] = make_float2((float)cr, (float)ci);
-                else
-                    row[dst] = make_double2(cr, ci);
+                // streaming stores: what one launch writes (a plane's line sums, 0.5 GB at n = 32, or its
+                // channels, 3 GB) is read back only after more data than the caches hold
+                if (f32) {
+                    float *w = (float *)row + 2 * dst;
+                    __builtin_nontemporal_store((float)cr, w);
+                    __builtin_nontemporal_store((float)ci, w + 1);
+                } else {
+                    double *w = (double *)row + 2 * dst;
+                    __builtin_nontemporal_store(cr, w);
+                    __builtin_nontemporal_store(ci, w + 1);
+                }
             }
         }
 #pragma unroll
