@@ -145,10 +145,11 @@ def measured_traffic_per_k(precision="fp64"):
 
 
 def roofline(st, n, nk, precision):
-    """Roofline of the single kernel with the largest device time among those the run times alone with HIP
-    events on their stream (k_rotate_sc, k_contract, or the interpolation GEMM): algorithmic work per launch
-    over the average launch duration.  The eigensolver stage (two kernels, H sweeps and the V kernel, plus
-    the warm-start transform) is reported beside it in 'jacobi_stage'."""
+    """Roofline of the single kernel with the largest device time among those the run times with HIP events on
+    their stream (k_rotate_sc, k_contract, or the interpolation GEMM): algorithmic work per launch over the
+    average launch duration (in a two-stream step the launches share the GPU with the other stream's).  The
+    eigensolver stage (two kernels, H sweeps and the V kernel, plus the warm-start transform) is reported
+    beside it in 'jacobi_stage'."""
     nb = max(1, st["batches"])
     kern = {"rotate": st["rotate_ms"], "interp": st["interp_ms"]}
     dominant = max(kern, key=kern.get)
@@ -257,11 +258,23 @@ def main():
     ms_per_step = 1e3 * wall / args.steps
     value = nk * args.steps / wall
     head_total = tot
+    # after the timed region: one step in one stream (WNLQG_MESH_OVERLAP=0), each kernel timed alone
+    os.environ["WNLQG_MESH_OVERLAP"] = "0"
+    try:
+        _, _, st_alone = eng.run_mesh(mesh)
+    finally:
+        os.environ.pop("WNLQG_MESH_OVERLAP", None)
     eng.close()
 
-    # roofline of the dominant kernel, from its device event timings inside run_mesh
+    # roofline of the dominant kernel, from its device event timings inside run_mesh: in the timed steps
+    # (two streams, the kernels of batch b - 1's response sharing the GPU with batch b's contraction and
+    # eigensystem) and in the one-stream step after them
     st = stats[-1]
     roof = roofline(st, n, nk, args.precision)
+    roof_alone = roofline(st_alone, n, nk, args.precision)
+    roof["one_stream"] = {k: roof_alone[k] for k in ("kernel", "achieved", "frac", "avg_launch_ms", "launches",
+                                                     "traffic")}
+    roof["one_stream"]["jacobi_stage"] = roof_alone["jacobi_stage"]
 
     # the FP64 GPU path on the same mesh: the reference of the metric's "mixed vs FP64 at rel err <= 1e-6"
     if args.precision != "fp64":
@@ -330,7 +343,13 @@ def main():
         "cpu_baseline": cpu,
         "stages_ms": {k: st[k] for k in ("interp_ms", "eig_ms", "warm_ms", "rotate_ms", "pairs_ms", "accum_ms",
                                          "total_ms")},
-        "stage_note": "stages run in order on one stream; each figure is the event-timed span of its kernels",
+        "stage_note": "timed steps: two streams (batch b - 1's screening, rotation, pairs, frequency contraction "
+                      "and lane sums beside batch b's contraction, warm transform and Jacobi); each figure is the "
+                      "event-timed span of its kernels on their stream, so spans share GPU time and their sum "
+                      "exceeds total_ms; stages_one_stream_ms: the step after the timed region on one stream, "
+                      "each kernel alone",
+        "stages_one_stream_ms": {k: st_alone[k] for k in ("interp_ms", "eig_ms", "warm_ms", "rotate_ms", "pairs_ms",
+                                                          "accum_ms", "total_ms")},
         "rotation_points": {"elements": st["rotate_element_points"], "full": st["rotate_full_points"],
                             "rows": st["rotate_row_points"],
                             "mean_bands": st["rotate_element_bands"] / max(1, st["rotate_element_points"])},

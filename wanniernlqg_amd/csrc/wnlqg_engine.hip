@@ -1060,9 +1060,8 @@ int enqueue_mesh(wnlqg_engine *g, std::vector<std::array<hipEvent_t, WN_MARKS>> 
     double2 *comp = g->nterms ? g->lanecomp.p : nullptr;
     const long long m2 = g->sh.ldo / 2;
     // overlap: rotation and response of batch b - 1 on the second stream while batch b is interpolated and
-    // diagonalized (the captured batching graph); otherwise one stream, in program order (every stage fills
-    // the GPU by itself: on the 128^3 mesh both orders take the same time, and one stream times each
-    // kernel alone)
+    // diagonalized (the captured batching graph and run_mesh's default: 0.91 of the one-stream time on the
+    // 128^3 mesh); otherwise one stream, in program order, which times each kernel alone
     hipStream_t s0 = g->stream, s1 = overlap ? g->stream2 : g->stream;
     // fork the second stream from the first (this also brings it into a stream capture)
     WN_HIP(hipEventRecord(g->ev_fork, s0));
@@ -1255,10 +1254,12 @@ int32_t wnlqg_run_mesh(wnlqg_engine *g, const int32_t mesh[3], int32_t n_lanes, 
                 if (e) hipEventDestroy(e);
     };
     double interp_bytes = 0.0;
-    // WNLQG_MESH_OVERLAP=1: the two-stream order of the batching graph (rotation and response of batch b - 1
-    // beside the interpolation and eigensystem of batch b); the stage timers then span shared GPU time
-    bool overlap = false;
-    if (const char *ov = std::getenv("WNLQG_MESH_OVERLAP")) overlap = !strcmp(ov, "1");
+    // the two-stream order of the batching graph (rotation and response of batch b - 1 beside the
+    // interpolation and eigensystem of batch b: the stages are bound by different units, HBM writes, FP64
+    // VALU and barrier latency), whose stage timers span shared GPU time; WNLQG_MESH_OVERLAP=0 selects one
+    // stream in program order, each kernel then timed alone
+    bool overlap = true;
+    if (const char *ov = std::getenv("WNLQG_MESH_OVERLAP")) overlap = strcmp(ov, "0") != 0;
     rc = enqueue_mesh(g, events_ok ? &marks : nullptr, &interp_bytes, overlap);
     if (!rc) rc = finish_lanes(g, g->plan.n_lanes, g->plan.nk, lane_out, total_out);
     if (rc) {
