@@ -127,7 +127,11 @@ typedef struct wnlqg_task {
     int32_t shg_eta_correction;       /* 1: eta-corrected optical position derivative (reference default) */
 } wnlqg_task;
 
-/* Counters of the last run (device event timers, milliseconds). */
+/* Counters of the last run (device event timers, milliseconds).  wnlqg_run_mesh runs batch b - 1's
+ * screening, rotation, response and k-sums on a second stream beside batch b's interpolation and
+ * eigensystem, so the stage timers span GPU time the two streams share and sum to more than total_ms;
+ * with the environment variable WNLQG_MESH_OVERLAP=0 it runs one stream in program order and each
+ * stage timer holds its kernels alone. */
 typedef struct wnlqg_stats {
     double total_ms;        /* whole k-loop: phase, interpolation, eigen, screening, response, k-sum */
     double interp_ms;       /* interpolation GEMM kernel */
